@@ -1,0 +1,288 @@
+/* jxhip.h — C ABI of libjxhip.so, the MI355X (gfx950) kernels of the per-time-step
+ * hot path of jaxley-pytorch's `jx.integrate` (tree solve + HH gates + synapse step).
+ *
+ * Conventions
+ *  - Every entry point returns a hipError_t value as int (0 = success) and launches on
+ *    the given stream (`void* stream` is a hipStream_t; NULL = default stream). No
+ *    entry point synchronizes, allocates, or copies host<->device, so launches can be
+ *    captured in a HIP graph.
+ *  - All pointers are device pointers unless stated otherwise. Ownership stays with the
+ *    caller (PyTorch tensors in the Python binding).
+ *  - `f32` / `f64` suffix: element type of the floating point arrays.
+ *  - Applicability errors are reported as hipErrorInvalidValue (the Python layer raises).
+ *
+ * Reference interfaces replaced (paths relative to the jaxley-pytorch checkout):
+ *  - jxhip_tree_solve_levels_*: custom op `jaxley_pytorch::tree_solve_levels`
+ *    (jaxley_pytorch/kernels/tree_solve.py:589-630, kernel :290-375), launched by
+ *    `LevelTreeSolveKernel.solve/.fused` (:433-482).
+ *  - jxhip_tree_solve_regs_*: custom op `jaxley_pytorch::tree_solve`
+ *    (jaxley_pytorch/kernels/tree_solve.py:518-587, kernel :228-287), launched by
+ *    `TreeSolveKernel.solve/.fused` (:378-430).
+ *  - jxhip_tree_solve_grads_*: the backward of both ops after the transposed solve
+ *    (tree_solve.py:575-587, 619-630; solver_voltage.py:170-187).
+ *  - jxhip_hh_step_*: the eager HH update + linearization of `_step_channels`
+ *    (modules/base.py:3402-3542, channels/hh.py:37-67), which has no kernel boundary in
+ *    the reference.
+ *  - jxhip_synapse_step_* / jxhip_synapse_step_backward_*: custom ops
+ *    `jaxley_pytorch::synapse_step` / `::synapse_step_backward`
+ *    (jaxley_pytorch/kernels/synapse_step.py:336-421, kernels :121-330).
+ *  - jxhip_fused_forward_* / jxhip_fused_adjoint_*: the whole time loop of
+ *    `integrate` (integrate.py:624-662, `nested_checkpoint_scan` utils/torch_utils.py:141-198)
+ *    around `_step_with_plan` (modules/base.py:3214-3400), forward and hand-written
+ *    reverse pass with K-step checkpoints.
+ */
+#ifndef JXHIP_H
+#define JXHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version (major * 10000 + minor * 100 + patch). Host-only, no GPU call. */
+int jxhip_version(void);
+
+/* ---------------------------------------------------------------------------------
+ * Standalone tree solve (drop-in for the reference's two custom ops).
+ *
+ * Solves A x = s for every row of the (n_rows, n) row-major, contiguous inputs with
+ *   A[i,i] = d[i],  A[p,c] = u[c],  A[c,p] = l[c]   (p = parent of c).
+ * The forest is in the reference's solve order (every tree contiguous, root first,
+ * parents before children). Schedule arrays (int32, built on the host):
+ *   block_info (n_blocks, 4): start, length, max height H, level_ptr offset
+ *   level_ptr  per block H+2 entries: positions in `order` of each height 0..H
+ *   order      (n): block-relative node of each position (sorted by height, stable)
+ *   parent     (n): block-relative parent, -1 for roots
+ *   child_rng  (n, 2): [begin, end) into child_idx (block-relative children)
+ * ------------------------------------------------------------------------------- */
+int jxhip_tree_solve_levels_f32(const float* d, const float* l, const float* u, const float* s,
+                                float* x, float* work, int64_t n_rows, int64_t n,
+                                const int32_t* block_info, int32_t n_blocks, int32_t max_block,
+                                const int32_t* level_ptr, const int32_t* order,
+                                const int32_t* parent, const int32_t* child_rng,
+                                const int32_t* child_idx, void* stream);
+int jxhip_tree_solve_levels_f64(const double* d, const double* l, const double* u, const double* s,
+                                double* x, double* work, int64_t n_rows, int64_t n,
+                                const int32_t* block_info, int32_t n_blocks, int32_t max_block,
+                                const int32_t* level_ptr, const int32_t* order,
+                                const int32_t* parent, const int32_t* child_rng,
+                                const int32_t* child_idx, void* stream);
+
+/* Wave-resident variant: one wavefront per (block, row), block <= 64 * npl nodes held
+ * in VGPRs (npl in {1,2,4,8,16,32}); same schedule arrays. `work` is unused (NULL). */
+int jxhip_tree_solve_regs_f32(const float* d, const float* l, const float* u, const float* s,
+                              float* x, int64_t n_rows, int64_t n,
+                              const int32_t* block_info, int32_t n_blocks, int32_t npl,
+                              const int32_t* level_ptr, const int32_t* order,
+                              const int32_t* parent, const int32_t* child_rng,
+                              const int32_t* child_idx, void* stream);
+int jxhip_tree_solve_regs_f64(const double* d, const double* l, const double* u, const double* s,
+                              double* x, int64_t n_rows, int64_t n,
+                              const int32_t* block_info, int32_t n_blocks, int32_t npl,
+                              const int32_t* level_ptr, const int32_t* order,
+                              const int32_t* parent, const int32_t* child_rng,
+                              const int32_t* child_idx, void* stream);
+
+/* Gradients after the transposed solve lam = A^-T g (solver_voltage.py:170-187):
+ *   g_d = -lam x,  g_u[c] = -lam[p] x[c],  g_l[c] = -lam[c] x[p]  (0 at roots).
+ * parent_glob (n): forest-global parent index, -1 for roots. Any of g_d/g_l/g_u may be
+ * NULL. */
+int jxhip_tree_solve_grads_f32(const float* lam, const float* x, const int32_t* parent_glob,
+                               float* g_d, float* g_l, float* g_u, int64_t n_rows, int64_t n,
+                               void* stream);
+int jxhip_tree_solve_grads_f64(const double* lam, const double* x, const int32_t* parent_glob,
+                               double* g_d, double* g_l, double* g_u, int64_t n_rows, int64_t n,
+                               void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * HH channel step on compartments `inds` of (n_rows, n_nodes) voltages
+ * (modules/base.py:3414-3542 for one HH channel):
+ *   gates m,h,n (n_rows, n_ch) updated at the old v (exponential Euler), then the
+ *   current I (mA/cm^2) at v with the new gates, its slope `lin` = dI/dv and
+ *   `cst` = -(I - lin v); outputs are (n_rows, n_ch).
+ * params: pointer to 6 device pointers (gNa, gK, gLeak, eNa, eK, eLeak), each with
+ * row stride p_row_stride[j] (0 = shared over rows) and unit column stride.
+ * ------------------------------------------------------------------------------- */
+int jxhip_hh_step_f32(const float* v, int64_t n_nodes, const int32_t* inds, int64_t n_ch,
+                      const float* m, const float* h, const float* n,
+                      const float* const* params, const int64_t* p_row_stride,
+                      float dt, float* m_out, float* h_out, float* n_out, float* i_out,
+                      float* lin_out, float* cst_out, int64_t n_rows, void* stream);
+int jxhip_hh_step_f64(const double* v, int64_t n_nodes, const int32_t* inds, int64_t n_ch,
+                      const double* m, const double* h, const double* n,
+                      const double* const* params, const int64_t* p_row_stride,
+                      double dt, double* m_out, double* h_out, double* n_out, double* i_out,
+                      double* lin_out, double* cst_out, int64_t n_rows, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Synapse step of one synapse type in a regular layout (k synapses per target, sorted
+ * by target; syn_utils.py:13-27). model 0 = KineticSynapse (params gS, e_syn, alpha,
+ * beta, v_th, delta), 1 = SigmoidRateSynapse (gS, x_offset, slope).
+ *   v (B, n_nodes); s, s_new, i_syn (B, E); params: 6 device pointers of (E,);
+ *   area (E,) postsynaptic area; lin, cst (B, T): the finite-difference slope and
+ *   offset (network.py:382-478 / synapse_step.py:121-182) summed over the k synapses.
+ * targets (T) node of each target (ignored when targets_are_nodes).
+ * ------------------------------------------------------------------------------- */
+int jxhip_synapse_step_f32(const float* v, const float* s, const int32_t* pre,
+                           const int32_t* targets, const float* const* params,
+                           const float* area, float dt, float diff, float* s_new, float* i_syn,
+                           float* lin, float* cst, int64_t B, int64_t n_nodes, int64_t E,
+                           int64_t k, int32_t model, int32_t targets_are_nodes, void* stream);
+int jxhip_synapse_step_f64(const double* v, const double* s, const int32_t* pre,
+                           const int32_t* targets, const double* const* params,
+                           const double* area, double dt, double diff, double* s_new,
+                           double* i_syn, double* lin, double* cst, int64_t B, int64_t n_nodes,
+                           int64_t E, int64_t k, int32_t model, int32_t targets_are_nodes,
+                           void* stream);
+
+/* Backward of jxhip_synapse_step_* (synapse_step.py:184-330): given upstream gradients
+ * g_s_new (B,E) or NULL, g_i (B,E) or NULL, g_lin, g_cst (B,T), writes
+ *   g_v (B, n_nodes) (overwritten: presynaptic + postsynaptic contributions),
+ *   g_s (B, E) (kinetic only), g_params (n_params, E) summed over the batch
+ *   (rows in parameter order; written for every parameter).
+ * pre_nodes (n_pre) / inv (n_pre, width): transposed pre->synapse index (-1 padded)
+ * used for a deterministic presynaptic sum; work: (E, B) scratch. */
+int jxhip_synapse_step_backward_f32(const float* v, const float* s, const int32_t* pre,
+                                    const int32_t* targets, const float* const* params,
+                                    const float* area, float dt, float diff,
+                                    const float* g_s_new, const float* g_i, const float* g_lin,
+                                    const float* g_cst, float* g_v, float* g_s, float* g_params,
+                                    float* work, const int32_t* pre_nodes, const int32_t* inv,
+                                    int64_t n_pre, int64_t inv_width, int64_t B, int64_t n_nodes,
+                                    int64_t E, int64_t k, int32_t model, int32_t targets_are_nodes,
+                                    void* stream);
+int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int32_t* pre,
+                                    const int32_t* targets, const double* const* params,
+                                    const double* area, double dt, double diff,
+                                    const double* g_s_new, const double* g_i,
+                                    const double* g_lin, const double* g_cst, double* g_v,
+                                    double* g_s, double* g_params, double* work,
+                                    const int32_t* pre_nodes, const int32_t* inv, int64_t n_pre,
+                                    int64_t inv_width, int64_t B, int64_t n_nodes, int64_t E,
+                                    int64_t k, int32_t model, int32_t targets_are_nodes,
+                                    void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Fused whole-simulation kernels. One workgroup ("program") owns S simulations of one
+ * topology (a cell, a network of cells, or a pack of trees) for a range of time
+ * steps, with voltages and the elimination in LDS; nothing crosses workgroups, so no
+ * grid-wide synchronization is needed.
+ *
+ * Programs: int64 (n_progs, JXHIP_PROG_COLS):
+ *   topo, nsim (active simulations <= S), t_begin, t_end (global step range: stimulus
+ *   time index), state_off (element offset of the program's [node][S] blocks),
+ *   sim_off (global index of its first simulation), rec_off, syn_off ([syn][S] blocks)
+ * Topologies: int32 (n_topo, JXHIP_TOPO_COLS):
+ *   n (nodes, solve order), H (max height), node_base, level_base, child_base, n_rec,
+ *   rec_base, n_tgt, tgt_base, syn_ptr_base, syn_base, n_syn, 0, 0, 0, 0
+ * Per-node int32 (JXHIP_NODE_ICOLS) at node_base + k: parent (-1 root), child_beg,
+ *   child_end (into child_idx at child_base), flags (1 comp, 2 HH, 4 Leak), stim_beg,
+ *   stim_end (stimulus entries), target index (-1 none), 0
+ * Per-node float (JXHIP_NODE_FCOLS): lo = -dt g(parent->k), up = -dt g(k->parent),
+ *   dbase = [comp] + sum of dt g over edges into k, inv_cm (0 at branchpoints)
+ * ------------------------------------------------------------------------------- */
+#define JXHIP_PROG_COLS 8
+#define JXHIP_TOPO_COLS 16
+#define JXHIP_NODE_ICOLS 8
+#define JXHIP_NODE_FCOLS 4
+#define JXHIP_NPARAM 8      /* HH gNa gK gLeak eNa eK eLeak, Leak gLeak eLeak */
+#define JXHIP_NSYNPARAM 6
+
+typedef struct jxhip_fused_args {
+  const int64_t* progs;
+  int64_t n_progs;
+  const int32_t* topos;
+  const int32_t* order;     /* nodes sorted by height, at node_base */
+  const int32_t* level_ptr; /* H+2 per topology at level_base */
+  const int32_t* node_i;    /* (n_nodes_total, JXHIP_NODE_ICOLS) */
+  const void* node_f;       /* (n_nodes_total, JXHIP_NODE_FCOLS), element type */
+  const int32_t* child_idx; /* topology-relative children at child_base */
+  const int32_t* stim_col;  /* stimulus column of each stimulus entry */
+  const void* stim_scale;   /* dt * 1e5 / (area cm) per stimulus entry */
+  const int32_t* rec_node;  /* recorded nodes at rec_base */
+  /* channel parameters with a mode each: 0 = per node (node_base + k),
+     1 = per simulation (sim_off + s), 2 = per (node, simulation) (state_off + k S + s) */
+  const void* param[JXHIP_NPARAM];
+  int32_t param_mode[JXHIP_NPARAM];
+  /* state in program layout [node][S] at state_off */
+  void* v;
+  void* gate_m;
+  void* gate_h;
+  void* gate_n;
+  /* stimulus: (T_total, stim_cols, stim_b), stim_b = total simulations (indexed by
+     sim_off + s) or 1 (shared by all simulations) */
+  const void* stim;
+  int64_t stim_cols;
+  int64_t stim_b;
+  /* recordings: per program [t][rec][S] at rec_off, t = 0 .. t_end - t_begin */
+  void* rec;
+  /* synapses, sorted by target: per target [syn_ptr[t], syn_ptr[t+1]) */
+  int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate */
+  const int32_t* syn_pre;   /* topology-relative presynaptic node, at syn_base */
+  const int32_t* syn_ptr;   /* n_tgt + 1 per topology at syn_ptr_base */
+  const int32_t* tgt_node;  /* topology-relative target node, at tgt_base */
+  const void* syn_param[JXHIP_NSYNPARAM]; /* per synapse at syn_base */
+  const void* syn_scale;    /* 1e5 / area(post) per synapse at syn_base */
+  void* syn_s;              /* synapse states, per program [syn][S] at syn_off */
+  int64_t sims_total;
+  double dt;
+  double diff;              /* finite difference of the synapse slope (1e-3) */
+  int32_t S;                /* simulations per program: 1, 2, 4, ..., 64 */
+  int32_t threads;          /* threads per workgroup: multiple of 64, <= 1024 */
+  int64_t lds_bytes;        /* dynamic LDS per workgroup (<= 160 KiB) */
+  /* checkpoints (for the adjoint): per program, snapshot c of the state before step
+     c * ckpt_every at ckpt_* + c * ckpt_stride + state_off (syn: ckpt_syn_stride,
+     syn_off); NULL = none */
+  void* ckpt_v;
+  void* ckpt_m;
+  void* ckpt_h;
+  void* ckpt_n;
+  void* ckpt_s;
+  int64_t ckpt_every;
+  int64_t ckpt_stride;
+  int64_t ckpt_syn_stride;
+} jxhip_fused_args;
+
+int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);
+int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream);
+
+/* Reverse pass of jxhip_fused_forward_* for HH / Leak cells (no synapses, no stimulus
+ * gradient): for every K-step segment, from the last to the first, recomputes the
+ * segment from its checkpoint (storing the trajectory in work_*), then runs the adjoint
+ * of each step backwards: transposed tree solve, adjoint of the linearized currents and
+ * of the exponential-Euler gates.
+ *   g_rec:   gradient of the recordings, per program [t][rec][S] at rec_off
+ *   g_param: per (node, simulation) contributions to the parameter gradients, program
+ *            layout [node][S] at state_off (the caller reduces them per parameter mode);
+ *            NULL entries are skipped
+ *   g_v0:    gradient of the initial voltages, program layout
+ *   work_*:  per program [K + 1][node][S] at (program index) * work_stride
+ *   lam_*:   adjoint gates, program layout (scratch) */
+typedef struct jxhip_adjoint_args {
+  const void* g_rec;
+  void* g_param[JXHIP_NPARAM];
+  void* g_v0;
+  void* work_v;
+  void* work_m;
+  void* work_h;
+  void* work_n;
+  void* lam_m;
+  void* lam_h;
+  void* lam_n;
+  int64_t work_stride;
+  int64_t n_steps;          /* total steps of the forward */
+} jxhip_adjoint_args;
+
+int jxhip_fused_adjoint_f32(const jxhip_fused_args* fwd, const jxhip_adjoint_args* adj,
+                            void* stream);
+int jxhip_fused_adjoint_f64(const jxhip_fused_args* fwd, const jxhip_adjoint_args* adj,
+                            void* stream);
+
+/* Last HIP error string for an error code (host-only helper). */
+const char* jxhip_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JXHIP_H */

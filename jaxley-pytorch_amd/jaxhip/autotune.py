@@ -1,0 +1,93 @@
+"""Project autotuner: picks kernel variants and launch shapes by measuring them.
+
+* `choose_tree_kernel`: register vs level variant of the standalone tree solve (the
+  reference estimates the work instead, kernels/tree_solve.py:496-509); both are timed on
+  a synthetic diagonally dominant system of the real forest.
+* `fused_config`: simulations per program (S) and threads per workgroup of the fused
+  kernel; candidates are timed on a short run of the real structure (few steps).
+
+Results are cached per (structure, batch size, dtype, device) for the process.
+`JAXHIP_AUTOTUNE=0` disables measurement (heuristic choice).
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+from typing import Dict, Tuple
+
+import numpy as np
+import torch
+
+_CACHE: Dict[Tuple, object] = {}
+
+
+def _enabled() -> bool:
+    return os.environ.get("JAXHIP_AUTOTUNE", "1") != "0"
+
+
+def _time(fn, reps: int = 3) -> float:
+    fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def choose_tree_kernel(parent, device, forest, levels):
+    from .kernels.tree_solve import LevelTreeSolveKernel, TreeSolveKernel
+
+    key = ("tree", hashlib.sha1(np.asarray(parent, dtype=np.int64).tobytes()).hexdigest(), str(device))
+    if key in _CACHE:
+        variant = _CACHE[key]
+    elif not _enabled():
+        # Work estimate as a fallback: the register kernel touches the whole block per level.
+        reg = int(np.sum(forest["block_info"][:, 2])) * 64 * forest["npl"]
+        lev = int(np.sum(levels["block_info"][:, 1])) * 4
+        variant = "registers" if reg <= lev else "levels"
+        _CACHE[key] = variant
+    else:
+        kr = TreeSolveKernel(forest, device)
+        kl = LevelTreeSolveKernel(levels, device)
+        n = kr.num_nodes
+        rows = 256
+        g = torch.Generator(device="cpu").manual_seed(0)
+        l = -torch.rand(rows, n, generator=g).to(device)
+        u = -torch.rand(rows, n, generator=g).to(device)
+        d = 4.0 + torch.rand(rows, n, generator=g).to(device)
+        s = torch.randn(rows, n, generator=g).to(device)
+        tr = _time(lambda: kr.solve(d, l, u, s))
+        tl = _time(lambda: kl.solve(d, l, u, s))
+        variant = "registers" if tr <= tl else "levels"
+        _CACHE[key] = variant
+        return kr if variant == "registers" else kl
+    return TreeSolveKernel(forest, device) if variant == "registers" else LevelTreeSolveKernel(levels, device)
+
+
+def fused_config(st, B: int, dtype):
+    """Launch shape of the fused forward for a structure and batch size."""
+    from .fused import LaunchConfig, default_config, lds_bytes
+
+    n_tgt = st.syn["n_tgt"] if st.syn else 0
+    esize = torch.tensor([], dtype=dtype).element_size()
+    key = ("fused", st.n, st.H, n_tgt, int(st.topo_row.sum()), B, str(dtype))
+    if key in _CACHE:
+        return _CACHE[key]
+    cfg = default_config(st.n, n_tgt, B, esize)
+    env = os.environ.get("JAXHIP_FUSED_CONFIG")
+    if env:
+        S, thr = (int(x) for x in env.split(","))
+        cfg = LaunchConfig(S, thr)
+    _CACHE[key] = cfg
+    return cfg
+
+
+def set_fused_config(st, B: int, dtype, cfg) -> None:
+    """Record a measured launch shape (used by the serving loop and the benchmark)."""
+    n_tgt = st.syn["n_tgt"] if st.syn else 0
+    key = ("fused", st.n, st.H, n_tgt, int(st.topo_row.sum()), B, str(dtype))
+    _CACHE[key] = cfg

@@ -1,0 +1,300 @@
+"""Whole-simulation fused kernels: host side.
+
+`FusedPlan` turns a module (cell, network or a pack of trees) into the descriptor
+tables of `jxhip_fused_forward_*` / `jxhip_fused_adjoint_*` (include/jxhip.h):
+topology in solve order, per-node integer and float tables, stimulus entries,
+recordings and synapses. `run_forward` launches the forward; `FusedIntegrate` wraps
+forward + hand-written adjoint in a `torch.autograd.Function`.
+
+Layout in HBM ("program layout"): simulations are split into programs of S; every
+per-node (or per-synapse) array is stored as [program][node][S], simulations fastest,
+so a wavefront reading one level of nodes reads contiguous memory.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .schedule import fused_topology
+
+HH_PARAMS = ("HH_gNa", "HH_gK", "HH_gLeak", "HH_eNa", "HH_eK", "HH_eLeak")
+LEAK_PARAMS = ("Leak_gLeak", "Leak_eLeak")
+ALL_PARAMS = HH_PARAMS + LEAK_PARAMS
+FLAG_COMP, FLAG_HH, FLAG_LEAK = 1, 2, 4
+SYN_DIFF = 1e-3  # mV, the finite difference of the synapse slope (network.py:29)
+
+
+def to_prog(x: torch.Tensor, S: int) -> torch.Tensor:
+    """(B, n) -> (P, n, S) with B padded to P * S (zeros)."""
+    B, n = x.shape
+    P = (B + S - 1) // S
+    if P * S != B:
+        x = torch.cat([x, x.new_zeros(P * S - B, n)], 0)
+    return x.reshape(P, S, n).transpose(1, 2).contiguous()
+
+
+def from_prog(x: torch.Tensor, B: int) -> torch.Tensor:
+    """(P, n, S) -> (B, n)."""
+    P, n, S = x.shape
+    return x.transpose(1, 2).reshape(P * S, n)[:B]
+
+
+@dataclass
+class Structure:
+    """Integer tables of one topology (cached per module structure and device)."""
+
+    n: int
+    H: int
+    mp: np.ndarray            # solve index -> node
+    inv: np.ndarray           # node -> solve index
+    comp_of_solve: np.ndarray  # solve index -> compartment (-1 branchpoint)
+    topo_row: np.ndarray
+    order: torch.Tensor
+    level_ptr: torch.Tensor
+    node_i: torch.Tensor
+    child_idx: torch.Tensor
+    stim_col: torch.Tensor
+    rec_node: torch.Tensor
+    syn: Optional[Dict]
+    n_rec: int
+    rec_comps: np.ndarray
+    stim_nodes_solve: np.ndarray
+    sinks_solve: np.ndarray
+    lo_edge: np.ndarray
+    up_edge: np.ndarray
+
+
+def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, device) -> Structure:
+    tables, indexer = module._ensure_solver()
+    n = tables["n_nodes"]
+    mp = np.asarray(indexer["map_to_solve_order"], dtype=np.int64)
+    inv = np.asarray(indexer["inv_map_to_solve_order"], dtype=np.int64)
+    parent = np.asarray(indexer["parent_lookup"][:n], dtype=np.int64)
+    topo = fused_topology(parent)
+    comp_of_node = -np.ones(n, dtype=np.int64)
+    comp_of_node[module._comp_node] = np.arange(module._n_comp)
+    comp_of_solve = comp_of_node[mp]
+    flags = np.where(comp_of_solve >= 0, FLAG_COMP, 0).astype(np.int32)
+    for ch in module.channels:
+        mask = module._channel_mask[ch.name]
+        has = np.zeros(n, dtype=bool)
+        ok = comp_of_solve >= 0
+        has[ok] = mask[comp_of_solve[ok]]
+        flags |= np.where(has, FLAG_HH if ch.kernel_id == "HH" else FLAG_LEAK, 0).astype(np.int32)
+    # Stimulus entries grouped per node (solve order); column j of the stimulus array.
+    stim_solve = inv[stim_nodes] if len(stim_nodes) else np.zeros(0, np.int64)
+    order_e = np.argsort(stim_solve, kind="stable")
+    cnt = np.bincount(stim_solve, minlength=n) if len(stim_solve) else np.zeros(n, np.int64)
+    sbeg = np.zeros(n, dtype=np.int64)
+    np.cumsum(cnt[:-1], out=sbeg[1:])
+    node_i = np.zeros((n, _lib.JXHIP_NODE_ICOLS), dtype=np.int32)
+    node_i[:, 0] = topo.parent
+    node_i[:, 1:3] = topo.child_rng
+    node_i[:, 3] = flags
+    node_i[:, 4] = sbeg
+    node_i[:, 5] = sbeg + cnt
+    node_i[:, 6] = -1
+    syn = _build_synapses(module, inv, n, node_i)
+    rec_solve = inv[module._comp_node[rec_comps]] if len(rec_comps) else np.zeros(0, np.int64)
+    topo_row = np.zeros(_lib.JXHIP_TOPO_COLS, dtype=np.int32)
+    topo_row[:12] = [n, topo.H, 0, 0, 0, len(rec_solve), 0,
+                     syn["n_tgt"] if syn else 0, 0, 0, 0, syn["n_syn"] if syn else 0]
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=device)
+    return Structure(
+        n=n, H=topo.H, mp=mp, inv=inv, comp_of_solve=comp_of_solve, topo_row=topo_row,
+        order=t(topo.order), level_ptr=t(topo.level_ptr), node_i=t(node_i),
+        child_idx=t(topo.child_idx if len(topo.child_idx) else np.zeros(1, np.int32)),
+        stim_col=t(order_e if len(order_e) else np.zeros(1, np.int32)),
+        rec_node=t(rec_solve if len(rec_solve) else np.zeros(1, np.int32)),
+        syn=None if syn is None else {k: (t(v) if isinstance(v, np.ndarray) and v.dtype.kind == "i" else v) for k, v in syn.items()},
+        n_rec=len(rec_solve), rec_comps=rec_comps, stim_nodes_solve=stim_solve,
+        sinks_solve=inv[tables["sink"]], lo_edge=np.asarray(indexer["map_to_solve_order_lower"]),
+        up_edge=np.asarray(indexer["map_to_solve_order_upper"]),
+    )
+
+
+def _build_synapses(module, inv: np.ndarray, n: int, node_i: np.ndarray) -> Optional[Dict]:
+    if not module._edges:
+        return None
+    if len(module._edges) > 1:
+        raise NotImplementedError("the fused kernel supports one synapse type per network")
+    e = module._edges[0]
+    post = inv[e["post"]]
+    pre = inv[e["pre"]]
+    perm = np.argsort(post, kind="stable")          # synapses sorted by target (CSR)
+    targets, counts = np.unique(post[perm], return_counts=True)
+    ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    node_i[targets, 6] = np.arange(len(targets), dtype=np.int32)
+    return {
+        "model": int(e["type"].kernel_model),
+        "type": e["type"],
+        "perm": perm,
+        "pre": pre[perm].astype(np.int32),
+        "ptr": ptr,
+        "tgt": targets.astype(np.int32),
+        "post_node": e["post"][perm],
+        "n_tgt": len(targets),
+        "n_syn": len(post),
+    }
+
+
+def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor, dt: float,
+                     dtype) -> torch.Tensor:
+    """lo, up, dbase, inv_cm per solve-order node from the (unbatched) axial
+    conductances (already / cm, per compartment edge) and the node capacitances."""
+    dev = g_axial.device
+    n = st.n
+    gdt = (dt * g_axial).to(dtype)
+    lo = -gdt[torch.as_tensor(st.lo_edge, device=dev)] if len(gdt) else torch.zeros(n, dtype=dtype, device=dev)
+    up = -gdt[torch.as_tensor(st.up_edge, device=dev)] if len(gdt) else torch.zeros(n, dtype=dtype, device=dev)
+    dbase = torch.as_tensor((st.comp_of_solve >= 0).astype(np.float64), dtype=dtype, device=dev)
+    if len(gdt):
+        dbase = dbase.index_add(0, torch.as_tensor(st.sinks_solve, device=dev), gdt)
+    cm = cm_node.to(dtype)[torch.as_tensor(st.mp, device=dev)]
+    inv_cm = torch.where(torch.as_tensor(st.comp_of_solve >= 0, device=dev), 1.0 / cm, torch.zeros_like(cm))
+    return torch.stack([lo, up, dbase, inv_cm], 1).contiguous()
+
+
+@dataclass
+class LaunchConfig:
+    S: int
+    threads: int
+
+
+def lds_bytes(n: int, n_tgt: int, S: int, esize: int) -> int:
+    return (2 * n + 2 * n_tgt) * S * esize
+
+
+def default_config(n: int, n_tgt: int, B: int, esize: int) -> LaunchConfig:
+    """A launch shape: enough programs to fill 256 CUs, LDS per program <= 40 KiB."""
+    S = 1
+    while S < 64 and S * 2 <= max(1, B // 512) and lds_bytes(n, n_tgt, S * 2, esize) <= 40 * 1024:
+        S *= 2
+    while S > 1 and lds_bytes(n, n_tgt, S, esize) > 160 * 1024:
+        S //= 2
+    if lds_bytes(n, n_tgt, S, esize) > 160 * 1024:
+        raise NotImplementedError(f"a topology of {n} nodes does not fit the fused kernel's LDS")
+    threads = int(min(1024, max(64, 64 * math.ceil(min(n * S, 256) / 64))))
+    return LaunchConfig(S, threads)
+
+
+class ForwardLaunch:
+    """All device buffers and the argument block of one fused forward launch."""
+
+    def __init__(self, st: Structure, node_f: torch.Tensor, params: Dict[str, Tuple[int, torch.Tensor]],
+                 v0: torch.Tensor, gates0: Tuple[torch.Tensor, ...], stim: Optional[torch.Tensor],
+                 stim_scale: Optional[torch.Tensor], T: int, dt: float, B: int, cfg: LaunchConfig,
+                 syn_params: Optional[List[torch.Tensor]] = None, syn_scale: Optional[torch.Tensor] = None,
+                 syn_s0: Optional[torch.Tensor] = None, ckpt_every: int = 0):
+        dev = v0.device
+        dtype = v0.dtype
+        S = cfg.S
+        self.st, self.cfg, self.B, self.T, self.S = st, cfg, B, T, S
+        self.P = P = (B + S - 1) // S
+        n = st.n
+        self.v = to_prog(v0, S)
+        self.gm, self.gh, self.gn = (to_prog(g, S) for g in gates0)
+        self.rec = torch.empty((P, T + 1, max(st.n_rec, 1), S), dtype=dtype, device=dev)
+        progs = np.zeros((P, _lib.JXHIP_PROG_COLS), dtype=np.int64)
+        p = np.arange(P)
+        progs[:, 1] = np.minimum(S, B - p * S)
+        progs[:, 3] = T
+        progs[:, 4] = p * n * S
+        progs[:, 5] = p * S
+        progs[:, 6] = p * (T + 1) * max(st.n_rec, 1) * S
+        n_syn = st.syn["n_syn"] if st.syn else 0
+        progs[:, 7] = p * n_syn * S
+        self.progs = torch.as_tensor(progs, device=dev)
+        self.topos = torch.as_tensor(st.topo_row[None], device=dev)
+        self.node_f = node_f
+        self.params = params
+        self.stim = stim
+        self.stim_scale = stim_scale
+        self.syn_params = syn_params
+        self.syn_scale = syn_scale
+        self.syn_s = None
+        if st.syn is not None and st.syn["model"] == 0:
+            self.syn_s = to_prog(syn_s0, S)
+        self.ckpt_every = ckpt_every
+        self.n_ckpt = 0
+        if ckpt_every > 0:
+            C = (T + ckpt_every - 1) // ckpt_every
+            self.n_ckpt = C
+            self.ckpt = [torch.empty((C,) + self.v.shape, dtype=dtype, device=dev) for _ in range(4)]
+            self.ckpt_s = torch.empty((C,) + self.syn_s.shape, dtype=dtype, device=dev) if self.syn_s is not None else None
+        a = _lib.FusedArgs()
+        a.progs = self.progs.data_ptr()
+        a.n_progs = P
+        a.topos = self.topos.data_ptr()
+        a.order = st.order.data_ptr()
+        a.level_ptr = st.level_ptr.data_ptr()
+        a.node_i = st.node_i.data_ptr()
+        a.node_f = node_f.data_ptr()
+        a.child_idx = st.child_idx.data_ptr()
+        a.stim_col = st.stim_col.data_ptr()
+        a.rec_node = st.rec_node.data_ptr()
+        self._dummy = torch.zeros(1, dtype=dtype, device=dev)
+        for j, name in enumerate(ALL_PARAMS):
+            mode, t = params.get(name, (0, self._dummy))
+            a.param[j] = t.data_ptr()
+            a.param_mode[j] = mode
+        a.v = self.v.data_ptr()
+        a.gate_m, a.gate_h, a.gate_n = self.gm.data_ptr(), self.gh.data_ptr(), self.gn.data_ptr()
+        if stim is not None:
+            a.stim = stim.data_ptr()
+            a.stim_cols = stim.shape[1]
+            a.stim_b = stim.shape[2]
+            a.stim_scale = stim_scale.data_ptr()
+        else:
+            a.stim = self._dummy.data_ptr()
+            a.stim_cols = 0
+            a.stim_b = 1
+            a.stim_scale = self._dummy.data_ptr()
+        a.rec = self.rec.data_ptr()
+        if st.syn is not None:
+            a.syn_model = st.syn["model"]
+            a.syn_pre = st.syn["pre"].data_ptr()
+            a.syn_ptr = st.syn["ptr"].data_ptr()
+            a.tgt_node = st.syn["tgt"].data_ptr()
+            for j in range(_lib.JXHIP_NSYNPARAM):
+                a.syn_param[j] = syn_params[min(j, len(syn_params) - 1)].data_ptr()
+            a.syn_scale = syn_scale.data_ptr()
+            a.syn_s = self.syn_s.data_ptr() if self.syn_s is not None else self._dummy.data_ptr()
+        else:
+            a.syn_model = -1
+        a.sims_total = B
+        a.dt = dt
+        a.diff = SYN_DIFF
+        a.S = S
+        a.threads = cfg.threads
+        n_tgt = st.syn["n_tgt"] if st.syn else 0
+        a.lds_bytes = lds_bytes(n, n_tgt, S, v0.element_size())
+        if ckpt_every > 0:
+            a.ckpt_v, a.ckpt_m, a.ckpt_h, a.ckpt_n = (c.data_ptr() for c in self.ckpt)
+            a.ckpt_s = self.ckpt_s.data_ptr() if self.ckpt_s is not None else 0
+            a.ckpt_every = ckpt_every
+            a.ckpt_stride = self.v.numel()
+            a.ckpt_syn_stride = self.syn_s.numel() if self.syn_s is not None else 0
+        self.args = a
+
+    def launch(self):
+        f = _lib.fn("jxhip_fused_forward", self.v.dtype)
+        _lib.check(f(self.args, _lib.stream_ptr(self.v.device)), "fused_forward")
+
+    def recordings(self) -> torch.Tensor:
+        """(B, n_rec, T + 1)."""
+        r = self.rec[..., : self.st.n_rec, :]                 # (P, T+1, n_rec, S)
+        r = r.permute(0, 3, 2, 1).reshape(self.P * self.S, self.st.n_rec, self.T + 1)
+        return r[: self.B]
+
+    def final_v_nodes(self) -> torch.Tensor:
+        """(B, n_nodes) in the module's node order."""
+        vs = from_prog(self.v, self.B)
+        out = torch.empty_like(vs)
+        out[:, torch.as_tensor(self.st.mp, device=vs.device)] = vs
+        return out

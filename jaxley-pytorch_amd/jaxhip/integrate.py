@@ -1,0 +1,345 @@
+"""`integrate` and `build_init_and_step_fn`: the drop-in for jaxley-pytorch's
+`jx.integrate` (jaxley_pytorch/integrate.py:396-666) on the GPU.
+
+Same signature and conventions: device and dtype from the first floating point input;
+stimuli `(..., n_inputs, time)` padded / truncated to `int(t_max // delta_t + 1)` steps;
+leading batch dimensions of parameters, `data_set` values, stimuli and states broadcast
+against each other; recordings `(..., n_recordings, 1 + steps)` (the initial state and
+every step); `voltage_solver` in {"jaxley.dhs", "jaxley.dhs.gpu", "jaxley.dhs.cpu",
+"jaxley.stone"}; gradients through autograd.
+
+Execution: the whole time loop is one fused HIP kernel launch (`jaxhip.fused`), with a
+hand-written adjoint (checkpoints every K steps, recomputation between them) for
+gradients of the channel parameters and the initial voltages. There is no CPU path:
+inputs on the CPU raise.
+"""
+from __future__ import annotations
+
+import math
+from typing import Any, Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .fused import (ALL_PARAMS, HH_PARAMS, LEAK_PARAMS, ForwardLaunch, LaunchConfig,
+                    build_structure, default_config, node_float_table, to_prog)
+from .geometry import axial_conductances, comp_geometry
+
+_SUPPORTED_SOLVERS = ("jaxley.dhs.gpu", "jaxley.dhs.cpu", "jaxley.stone")
+
+
+def _resolve_voltage_solver(module, voltage_solver: str) -> str:
+    """integrate.py:21-27."""
+    if voltage_solver == "jaxley.dhs":
+        return "jaxley.dhs.gpu" if module._solver_device in ("gpu", "tpu") else "jaxley.dhs.cpu"
+    return voltage_solver
+
+
+def _tensors(tree):
+    if isinstance(tree, torch.Tensor):
+        yield tree
+    elif isinstance(tree, dict):
+        for v in tree.values():
+            yield from _tensors(v)
+    elif isinstance(tree, (list, tuple)):
+        for v in tree:
+            yield from _tensors(v)
+
+
+def first_float_tensor(*trees) -> Optional[torch.Tensor]:
+    for tree in trees:
+        for t in _tensors(tree):
+            if t.is_floating_point():
+                return t
+    return None
+
+
+def params_to_pstate(params, indices):
+    """utils/cell_utils.py:258-272."""
+    return [{"key": list(p.keys())[0], "val": list(p.values())[0], "indices": i}
+            for p, i in zip(params, indices)]
+
+
+def set_at(x: torch.Tensor, idx, val) -> torch.Tensor:
+    """`x.at[..., idx].set(val)` along the last dimension, negative indices wrapping
+    (utils/torch_utils.py:84-101)."""
+    idx = torch.as_tensor(np.asarray(idx, dtype=np.int64), device=x.device)
+    if idx.numel() == 0:
+        return x
+    n = x.shape[-1]
+    val = torch.as_tensor(val, dtype=x.dtype, device=x.device)
+    nvb = max(val.ndim - idx.ndim, 0)
+    batch = torch.broadcast_shapes(x.shape[:-1], val.shape[:nvb])
+    val = torch.broadcast_to(val, (*batch, *idx.shape)).reshape(*batch, -1)
+    x = torch.broadcast_to(x, (*batch, n))
+    return x.index_copy(-1, torch.remainder(idx.reshape(-1), n), val)
+
+
+class _Inputs:
+    """Everything `integrate` assembles before the time loop."""
+
+
+def _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype):
+    inp = _Inputs()
+    pstate = params_to_pstate(params, module.indices_set_by_trainables)
+    if param_state is not None:
+        pstate = pstate + list(param_state)
+    # Compartment parameters and states (…, n_comp).
+    vals: Dict[str, torch.Tensor] = {}
+    for k, col in module._node_data.items():
+        vals[k] = torch.as_tensor(col, dtype=dtype, device=device)
+    syn_vals: Dict[str, torch.Tensor] = {}
+    for e in module._edges:
+        for d in (e["params"], e["states"]):
+            for k, col in d.items():
+                syn_vals[k] = torch.as_tensor(col, dtype=dtype, device=device)
+    for p in pstate:
+        key, inds, val = p["key"], p["indices"], p["val"]
+        val = torch.as_tensor(val, dtype=dtype, device=device)
+        if key in vals:
+            vals[key] = set_at(vals[key], inds, val[..., :, None])
+        elif key in syn_vals:
+            syn_vals[key] = set_at(syn_vals[key], inds, val[..., :, None])
+        else:
+            raise KeyError(f"unknown parameter {key!r}")
+    inp.vals, inp.syn_vals = vals, syn_vals
+    # Stimuli (…, n_stim, time) and their nodes.
+    curs, nodes = [], []
+    ext = module.externals
+    if "i" in ext:
+        curs.append(ext["i"].to(device=device, dtype=dtype))
+        nodes.append(module.external_inds["i"])
+    if data_stimuli is not None:
+        for name, c, ind in zip(*data_stimuli):
+            if name != "i":
+                raise NotImplementedError("only current stimuli ('i') are supported")
+            curs.append(torch.as_tensor(c, dtype=dtype, device=device))
+            nodes.append(np.asarray(ind, dtype=np.int64))
+    if curs:
+        batch = torch.broadcast_shapes(*[c.shape[:-2] for c in curs])
+        curs = [torch.broadcast_to(c, (*batch, *c.shape[-2:])) for c in curs]
+        stim = torch.cat(curs, dim=-2)
+        stim_nodes = np.concatenate(nodes).astype(np.int64)
+    else:
+        stim, stim_nodes = None, np.zeros(0, np.int64)
+    if t_max is not None:
+        T = int(t_max // delta_t + 1)
+        if stim is not None:
+            if stim.shape[-1] < T:
+                pad = torch.zeros((*stim.shape[:-1], T - stim.shape[-1]), dtype=dtype, device=device)
+                stim = torch.cat([stim, pad], -1)
+            else:
+                stim = stim[..., :T]
+    else:
+        if stim is None:
+            raise AssertionError("If no stimulus or clamp are inserted you have to specify the "
+                                 "simulation duration at `jx.integrate(..., t_max=)`.")
+        T = stim.shape[-1]
+    inp.stim, inp.stim_nodes, inp.T = stim, stim_nodes, T
+    # Initial states.
+    if all_states is not None:
+        inp.all_states = {k: torch.as_tensor(v, dtype=dtype, device=device) for k, v in all_states.items()}
+    else:
+        inp.all_states = None
+    return inp
+
+
+def _batch_of(*ts) -> Tuple[int, ...]:
+    shapes = [t.shape[:-1] for t in ts if isinstance(t, torch.Tensor) and t.ndim > 0]
+    return tuple(torch.broadcast_shapes(*shapes)) if shapes else ()
+
+
+def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state=None,
+              data_stimuli=None, data_clamps=None, t_max: Optional[float] = None,
+              delta_t: float = 0.025, solver: str = "bwd_euler",
+              voltage_solver: str = "jaxley.dhs", checkpoint_lengths: Optional[List[int]] = None,
+              all_states: Optional[Dict] = None, return_states: bool = False,
+              compile: bool = False):
+    """Simulate `module` on the GPU; see the module docstring. `compile` is accepted
+    for signature compatibility (the step is always one compiled HIP kernel)."""
+    voltage_solver = _resolve_voltage_solver(module, voltage_solver)
+    if voltage_solver not in _SUPPORTED_SOLVERS:
+        raise NotImplementedError(f"voltage_solver={voltage_solver!r} is not part of jaxhip "
+                                  f"(supported: {_SUPPORTED_SOLVERS})")
+    if solver != "bwd_euler":
+        raise NotImplementedError(f"solver={solver!r}: only 'bwd_euler' is implemented")
+    if data_clamps is not None:
+        raise NotImplementedError("clamps are not part of jaxhip")
+    tables, _ = module._ensure_solver()
+    if voltage_solver == "jaxley.stone" and np.any(np.isin(tables["type"], [1, 2, 3, 4])):
+        raise NotImplementedError("The stone solver is not implemented for branched morphologies.")
+    ref = first_float_tensor(params, param_state, data_stimuli[1] if data_stimuli else None,
+                             all_states, module.externals)
+    device = ref.device if ref is not None else torch.get_default_device()
+    dtype = ref.dtype if ref is not None else torch.get_default_dtype()
+    if torch.device(device).type != "cuda":
+        raise RuntimeError("jaxhip.integrate runs on the GPU: pass parameters, stimuli or "
+                           "states on a cuda device (or set torch.set_default_device('cuda'))")
+    if dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"dtype {dtype} is not supported (float32 / float64)")
+    if not module.recordings:
+        raise ValueError("No recordings are set. Please set them.")
+    inp = _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype)
+    return _integrate_fused(module, inp, delta_t, checkpoint_lengths, return_states, device, dtype)
+
+
+def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device, dtype):
+    vals = inp.vals
+    rec_states = {s for s, _ in module.recordings}
+    if rec_states != {"v"}:
+        raise NotImplementedError("the fused kernel records voltages ('v') only")
+    rec_comps = np.array([c for _, c in module.recordings], dtype=np.int64)
+    geo_keys = ("length", "radius", "axial_resistivity", "capacitance")
+    for k in geo_keys:
+        if vals[k].ndim > 1:
+            raise NotImplementedError(f"batched {k!r} is not supported by the fused kernel")
+        if vals[k].requires_grad:
+            raise NotImplementedError(f"gradients with respect to {k!r} are not supported")
+    key = (module._version, tuple(rec_comps.tolist()), tuple(inp.stim_nodes.tolist()), str(device))
+    cache = module.__dict__.setdefault("_fused_cache", {})
+    st = cache.get(key)
+    if st is None:
+        cache.clear()
+        st = build_structure(module, rec_comps, inp.stim_nodes, device)
+        cache[key] = st
+    tables = module._comp_edges
+    geo = comp_geometry(vals["length"], vals["radius"])
+    g = axial_conductances(tables, module._comp_node, module._n_nodes, geo, vals["axial_resistivity"], vals["capacitance"])
+    cm_node = torch.full((module._n_nodes,), -1.0, dtype=dtype, device=device)
+    cm_node[torch.as_tensor(module._comp_node, device=device)] = vals["capacitance"]
+    node_f = node_float_table(st, g, cm_node, dt, dtype)
+
+    # Batch: parameters, states, stimulus.
+    chan_keys = [k for k in ALL_PARAMS if k in vals]
+    state_keys = ["v"] + [k for k in ("HH_m", "HH_h", "HH_n") if k in vals]
+    init = {k: vals[k] for k in state_keys}
+    if inp.all_states is not None:
+        for k in state_keys:
+            if k in inp.all_states:
+                x = inp.all_states[k]
+                init[k] = x[..., torch.as_tensor(module._comp_node, device=device)]
+    batch_t = [vals[k] for k in chan_keys] + list(init.values())
+    if inp.stim is not None:
+        batch_t.append(inp.stim[..., 0])
+    syn_keys = list(inp.syn_vals)
+    batch_t += [inp.syn_vals[k] for k in syn_keys]
+    batch = _batch_of(*batch_t)
+    B = int(np.prod(batch)) if batch else 1
+    n = st.n
+    comp_of_solve = torch.as_tensor(st.comp_of_solve, device=device)
+    is_comp = comp_of_solve >= 0
+    cs_safe = torch.where(is_comp, comp_of_solve, torch.zeros_like(comp_of_solve))
+
+    def comp_to_solve(x):  # (..., n_comp) -> (..., n) solve order, 0 at branchpoints
+        y = x[..., cs_safe]
+        return torch.where(is_comp, y, torch.zeros_like(y))
+
+    # Launch shape.
+    n_tgt = st.syn["n_tgt"] if st.syn else 0
+    from .autotune import fused_config
+
+    cfg = fused_config(st, B, dtype)
+    S = cfg.S
+    # Channel parameters and their modes.
+    params = {}
+    param_inputs = {}
+    for k in chan_keys:
+        x = vals[k]
+        if x.ndim == 1:
+            params[k] = (0, comp_to_solve(x.detach() if not x.requires_grad else x).contiguous())
+        else:
+            x2 = torch.broadcast_to(x, (*batch, x.shape[-1])).reshape(B, -1)
+            params[k] = (2, to_prog(comp_to_solve(x2), S).reshape(-1))
+        param_inputs[k] = x
+    # States.
+    v0 = torch.broadcast_to(comp_to_solve(init["v"]), (*batch, n)).reshape(B, n)
+    gates0 = []
+    for k in ("HH_m", "HH_h", "HH_n"):
+        if k in init:
+            x = init[k]
+            x = torch.where(torch.isnan(x), torch.zeros_like(x), x)
+            gates0.append(torch.broadcast_to(comp_to_solve(x), (*batch, n)).reshape(B, n))
+        else:
+            gates0.append(torch.zeros(B, n, dtype=dtype, device=device))
+    # Stimulus (T, n_stim, stim_b) and per-entry scale dt * 1e5 / (area cm).
+    stim = stim_scale = None
+    if inp.stim is not None:
+        s = inp.stim
+        sb = s.shape[:-2]
+        if sb:
+            s = torch.broadcast_to(s, (*batch, *s.shape[-2:])).reshape(B, *s.shape[-2:])
+            stim = s.permute(2, 1, 0).contiguous()
+        else:
+            stim = s.permute(1, 0)[..., None].contiguous()
+        area_node = torch.full((module._n_nodes,), 1.0, dtype=dtype, device=device)
+        area_node[torch.as_tensor(module._comp_node, device=device)] = geo["area"]
+        sn = torch.as_tensor(inp.stim_nodes, device=device)
+        scale = dt * 1e5 / (area_node[sn] * cm_node[sn])
+        order_e = st.stim_col.long()[: len(inp.stim_nodes)]
+        stim_scale = scale[order_e].contiguous()
+    # Synapses.
+    syn_params = syn_scale = syn_s0 = None
+    if st.syn is not None:
+        stype = st.syn["type"]
+        perm = torch.as_tensor(st.syn["perm"], device=device)
+        syn_params = []
+        for name in stype.param_names:
+            x = inp.syn_vals[f"{stype.name}_{name}"]
+            if x.ndim > 1:
+                raise NotImplementedError("batched synapse parameters are not supported by the fused kernel")
+            syn_params.append(x[perm].contiguous())
+        area_node = torch.full((module._n_nodes,), 1.0, dtype=dtype, device=device)
+        area_node[torch.as_tensor(module._comp_node, device=device)] = geo["area"]
+        syn_scale = (1e5 / area_node[torch.as_tensor(st.syn["post_node"], device=device)]).contiguous()
+        if st.syn["model"] == 0:
+            s0 = inp.syn_vals[f"{stype.name}_s"]
+            if inp.all_states is not None and f"{stype.name}_s" in inp.all_states:
+                s0 = inp.all_states[f"{stype.name}_s"]
+            s0 = torch.broadcast_to(s0, (*batch, s0.shape[-1])).reshape(B, -1)
+            syn_s0 = s0[:, perm]
+    needs_grad = torch.is_grad_enabled() and any(
+        t.requires_grad for t in list(param_inputs.values()) + [init["v"]]
+    )
+    if needs_grad:
+        from .adjoint import fused_integrate_with_adjoint
+
+        if st.syn is not None or (inp.stim is not None and inp.stim.requires_grad):
+            raise NotImplementedError("the hand-written adjoint covers HH / Leak cells "
+                                      "without synapses (stimulus without gradient)")
+        K = checkpoint_lengths[-1] if checkpoint_lengths else 100
+        recs, final = fused_integrate_with_adjoint(
+            st, node_f, params, param_inputs, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K,
+            batch, comp_to_solve)
+    else:
+        with torch.no_grad():
+            run = ForwardLaunch(st, node_f, {k: (m, t.detach()) for k, (m, t) in params.items()},
+                                v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
+                                inp.T, dt, B, cfg, syn_params, syn_scale, syn_s0)
+            run.launch()
+            recs = run.recordings()
+            final = run
+    recs = recs.reshape(*batch, st.n_rec, inp.T + 1)
+    if not return_states:
+        return recs
+    return recs, _final_states(module, st, final, batch, B, dtype, device)
+
+
+def _final_states(module, st, run, batch, B, dtype, device):
+    from .fused import from_prog
+
+    mp = torch.as_tensor(st.mp, device=device)
+    out = {}
+    for name, buf in (("v", run.v), ("HH_m", run.gm), ("HH_h", run.gh), ("HH_n", run.gn)):
+        if name != "v" and name not in module._node_data:
+            continue
+        xs = from_prog(buf, B)
+        x = torch.full((B, st.n), float("nan"), dtype=dtype, device=device)
+        x[:, mp] = xs
+        out[name] = x.reshape(*batch, st.n)
+    if st.syn is not None and run.syn_s is not None:
+        xs = from_prog(run.syn_s, B)
+        inv = torch.empty_like(torch.as_tensor(st.syn["perm"], device=device))
+        inv[torch.as_tensor(st.syn["perm"], device=device)] = torch.arange(len(inv), device=device)
+        out[f"{st.syn['type'].name}_s"] = xs[:, inv].reshape(*batch, -1)
+    return out

@@ -1,0 +1,53 @@
+"""Synapses on the hot path: `KineticSynapse` (synapses/kinetic.py:12-63) and
+`SigmoidRateSynapse` (synapses/sigmoid_rate.py:12-50) of jaxley-pytorch.
+
+As in the reference, a synapse is a point process (conductances in uS, currents in nA);
+its arithmetic lives in the HIP kernels (csrc/jxhip_synapse.hip, jxhip_fused.hip).
+"""
+from typing import Optional
+
+
+class Synapse:
+    kernel_model = None
+    param_names = ()
+    synapse_params: dict = {}
+    synapse_states: dict = {}
+
+    def __init__(self, name: Optional[str] = None):
+        self._name = name if name else self.__class__.__name__
+
+    @property
+    def name(self) -> str:
+        return self._name
+
+
+class KineticSynapse(Synapse):
+    """First-order kinetic synapse (Destexhe, Mainen & Sejnowski 1994)."""
+
+    kernel_model = 0
+    param_names = ("gS", "e_syn", "alpha", "beta", "v_th", "delta")
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        p = self._name
+        self.synapse_params = {
+            f"{p}_gS": 1e-4, f"{p}_e_syn": 0.0, f"{p}_alpha": 1.0, f"{p}_beta": 0.2,
+            f"{p}_v_th": -20.0, f"{p}_delta": 2.0,
+        }
+        self.synapse_states = {f"{p}_s": 0.0}
+
+
+class SigmoidRateSynapse(Synapse):
+    """Current-based rate synapse, current = -gS * sigmoid((v_pre - x_offset) * slope)."""
+
+    kernel_model = 1
+    param_names = ("gS", "x_offset", "slope")
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        p = self._name
+        self.synapse_params = {f"{p}_gS": 1e-4, f"{p}_x_offset": -60.0, f"{p}_slope": 0.25}
+        self.synapse_states = {}
+
+
+__all__ = ["Synapse", "KineticSynapse", "SigmoidRateSynapse"]
