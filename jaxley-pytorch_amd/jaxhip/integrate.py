@@ -145,6 +145,12 @@ def _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_sta
     return inp
 
 
+def _require_gpu(device) -> None:
+    if torch.device(device).type != "cuda":
+        raise RuntimeError("jaxhip.integrate runs on the GPU: pass parameters, stimuli or "
+                           "states on a cuda device (or set torch.set_default_device('cuda'))")
+
+
 def _batch_of(*ts) -> Tuple[int, ...]:
     shapes = [t.shape[:-1] for t in ts if isinstance(t, torch.Tensor) and t.ndim > 0]
     return tuple(torch.broadcast_shapes(*shapes)) if shapes else ()
@@ -170,12 +176,14 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     if voltage_solver == "jaxley.stone" and np.any(np.isin(tables["type"], [1, 2, 3, 4])):
         raise NotImplementedError("The stone solver is not implemented for branched morphologies.")
     ref = first_float_tensor(params, param_state, data_stimuli[1] if data_stimuli else None,
-                             all_states, module.externals)
+                             all_states)
     device = ref.device if ref is not None else torch.get_default_device()
-    dtype = ref.dtype if ref is not None else torch.get_default_dtype()
-    if torch.device(device).type != "cuda":
-        raise RuntimeError("jaxhip.integrate runs on the GPU: pass parameters, stimuli or "
-                           "states on a cuda device (or set torch.set_default_device('cuda'))")
+    if ref is None:
+        ext = first_float_tensor(module.externals)
+        dtype = ext.dtype if ext is not None else torch.get_default_dtype()
+    else:
+        dtype = ref.dtype
+    _require_gpu(device)
     if dtype not in (torch.float32, torch.float64):
         raise TypeError(f"dtype {dtype} is not supported (float32 / float64)")
     if not module.recordings:

@@ -1,0 +1,107 @@
+"""Seeded workloads shared by the tests, the benchmark and the smoke check.
+
+Each builder returns the jaxhip module and the matching oracle model (same morphology,
+geometry, mechanisms, synapses; compartment indices agree).
+
+* c0: one cell, root + full binary tree of depth 3 (15 branches), 4 compartments per
+  branch, 10 um per branch (2.5 um per compartment), radius 1 um, HH everywhere;
+  0.1 nA step into compartment 0 from 1 to 2 ms; dt 0.025 ms, 400 steps.
+* c1: a seeded random tree of `nbranch` branches (each new branch on a uniformly chosen
+  earlier branch), 4 compartments per branch, branch length lognormal (median 30 um,
+  sigma 0.5), radius 2 um * 0.85^generation; HH with gNa, gK, gLeak drawn in +-50 % of
+  the defaults per simulation; 0.2 nA step into the root from 1 to 21 ms.
+* c4: ball-and-stick network (1-compartment HH soma, 8-compartment Leak dendrite),
+  KineticSynapses with fixed in-degree onto one dendritic compartment per cell.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import jaxhip as jx
+from oracle import jx_oracle as O
+
+DT = 0.025
+
+
+def binary_tree_parents(nbranches: int):
+    """`_binary_tree_parents` of the reference's tests/conftest.py:25-31."""
+    parents = [-1]
+    depth = 0
+    while nbranches > len(parents):
+        parents = [-1] + [b // 2 for b in range(0, 2**depth - 2)]
+        depth += 1
+    return parents[:nbranches]
+
+
+def c0_stimulus(t_steps: int = 400) -> np.ndarray:
+    cur = np.zeros(t_steps)
+    cur[int(1.0 / DT):int(2.0 / DT)] = 0.1
+    return cur
+
+
+def build_c0(device=None, dtype=torch.float64):
+    parents = binary_tree_parents(15)
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 4), parents)
+    cell.set("length", 2.5)
+    cell.insert(jx.HH())
+    cell.record()
+    om = O.OracleModel([(parents, [4] * 15)])
+    om.length = np.full(60, 2.5)
+    om.hh_comps = np.arange(60)
+    return cell, om
+
+
+def random_tree(nbranch: int, seed: int):
+    rng = np.random.default_rng(seed)
+    parents = [-1] + [int(rng.integers(0, k)) for k in range(1, nbranch)]
+    gen = np.zeros(nbranch, dtype=np.int64)
+    for k in range(1, nbranch):
+        gen[k] = gen[parents[k]] + 1
+    blen = rng.lognormal(np.log(30.0), 0.5, size=nbranch)
+    rad = 2.0 * 0.85 ** gen
+    return parents, blen, rad
+
+
+def build_tree(nbranch: int, seed: int, ncomp: int = 4):
+    parents, blen, rad = random_tree(nbranch, seed)
+    comp_len = np.repeat(blen / ncomp, ncomp)
+    comp_rad = np.repeat(rad, ncomp)
+    cell = jx.Cell(jx.Branch(jx.Compartment(), ncomp), parents)
+    cell.set("length", comp_len)
+    cell.set("radius", comp_rad)
+    cell.insert(jx.HH())
+    om = O.OracleModel([(parents, [ncomp] * nbranch)])
+    om.length = comp_len
+    om.radius = comp_rad
+    om.hh_comps = np.arange(nbranch * ncomp)
+    return cell, om
+
+
+def c1_stimulus(t_steps: int) -> np.ndarray:
+    cur = np.zeros(t_steps)
+    cur[int(1.0 / DT):int(21.0 / DT)] = 0.2
+    return cur
+
+
+def c1_params(B: int, seed: int):
+    rng = np.random.default_rng(seed + 1)
+    scale = rng.uniform(0.5, 1.5, size=(B, 3))
+    return {
+        "HH_gNa": 0.12 * scale[:, 0:1],
+        "HH_gK": 0.036 * scale[:, 1:2],
+        "HH_gLeak": 0.0003 * scale[:, 2:3],
+    }
+
+
+def data_set_params(cell, params: dict, device, dtype):
+    """`data_set` of per-simulation values `(B, 1)` on the whole cell."""
+    ps = None
+    for k, v in params.items():
+        ps = cell.data_set(k, torch.as_tensor(v, dtype=dtype, device=device), ps)
+    return ps
+
+
+def oracle_overrides(om, params: dict):
+    nc = len(om.internal)
+    return {k: np.broadcast_to(v, (v.shape[0], nc)).copy() for k, v in params.items()}

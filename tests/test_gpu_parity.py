@@ -1,0 +1,141 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Tolerances (voltages, mV): 1e-6 in float64, 1e-2 in float32 (max abs error over every
+recorded sample). Tree solves: 1e-10 relative to the dense solve in float64.
+"""
+import numpy as np
+import pytest
+import torch
+
+import jaxhip as jx
+from jaxhip.kernels.tree_solve import make_tree_solve_kernel
+from oracle import jx_oracle as O
+
+from models import (DT, build_c0, build_tree, c0_stimulus, c1_params, c1_stimulus,
+                    data_set_params, oracle_overrides)
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_c0(device, dtype, T=400):
+    cell, om = build_c0()
+    cur = c0_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=dtype, device=device))
+    v = jx.integrate(cell, data_stimuli=ds, delta_t=DT)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0],
+                     rec_comps=list(range(60)))
+    return v.double().cpu().numpy(), ref.numpy()
+
+
+def test_c0_float64(device):
+    v, ref = _run_c0(device, torch.float64)
+    assert v.shape == (60, 401)
+    err = np.abs(v - ref).max()
+    print("c0 fp64 max err", err)
+    assert err < 1e-6
+
+
+def test_c0_float32(device):
+    v, ref = _run_c0(device, torch.float32)
+    err = np.abs(v - ref).max()
+    print("c0 fp32 max err", err)
+    assert err < 1e-2
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-6), (torch.float32, 1e-2)])
+def test_batched_random_tree(device, dtype, tol):
+    B, T, nb = 6, 600, 40
+    cell, om = build_tree(nb, seed=3)
+    cell.branch(0).comp(0).record()
+    prm = c1_params(B, seed=3)
+    ps = data_set_params(cell, prm, device, dtype)
+    cur = c1_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=dtype, device=device))
+    v = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0], rec_comps=[0],
+                     overrides=oracle_overrides(om, prm))
+    err = np.abs(v.double().cpu().numpy() - ref.numpy()).max()
+    print(f"tree {dtype} max err", err)
+    assert v.shape == (B, 1, T + 1)
+    assert err < tol
+
+
+def _dense_solve(d, l, u, s, parent):
+    n = len(d)
+    A = np.diag(d.astype(np.float64))
+    for c in range(n):
+        p = parent[c]
+        if p >= 0:
+            A[c, p] += l[c]
+            A[p, c] += u[c]
+    return np.linalg.solve(A, s)
+
+
+@pytest.mark.parametrize("mode", ["registers", "levels"])
+@pytest.mark.parametrize("which", ["c0", "tree"])
+def test_tree_solve_vs_dense(device, mode, which):
+    cell, _ = build_c0() if which == "c0" else build_tree(60, seed=5)
+    idx = cell._dhs_solve_indexer
+    parent = np.asarray(idx["parent_lookup"])  # includes the spurious node
+    kern = make_tree_solve_kernel(parent, device, mode)
+    assert kern is not None
+    n = len(parent)
+    rng = np.random.default_rng(0)
+    rows = 5
+    l = -rng.uniform(0.1, 1.0, (rows, n))
+    u = -rng.uniform(0.1, 1.0, (rows, n))
+    d = 4.0 + rng.uniform(0, 1, (rows, n))
+    s = rng.normal(size=(rows, n))
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    x = kern.solve(t(d), t(l), t(u), t(s)).cpu().numpy()
+    for r in range(rows):
+        ref = _dense_solve(d[r], l[r], u[r], s[r], parent)
+        np.testing.assert_allclose(x[r], ref, rtol=1e-10, atol=1e-12)
+    # Gradient: lambda = A^-T g through the custom backward.
+    D, L, U, Sv = (t(a).requires_grad_(True) for a in (d, l, u, s))
+    w = t(rng.normal(size=(rows, n)))
+    (kern.fused(D, L, U, Sv) * w).sum().backward()
+    for r in range(rows):
+        A = np.diag(d[r])
+        for c in range(n):
+            p = parent[c]
+            if p >= 0:
+                A[c, p] += l[r, c]
+                A[p, c] += u[r, c]
+        lam = np.linalg.solve(A.T, w[r].cpu().numpy())
+        np.testing.assert_allclose(Sv.grad[r].cpu().numpy(), lam, rtol=1e-9, atol=1e-12)
+        xr = np.linalg.solve(A, s[r])
+        np.testing.assert_allclose(D.grad[r].cpu().numpy(), -lam * xr, rtol=1e-9, atol=1e-12)
+
+
+def test_adjoint_matches_oracle_autograd(device):
+    """Gradients of a loss on the recorded voltage w.r.t. per-branch gNa / gK / gLeak
+    (fp64) against autograd through the oracle, checkpoints every 50 steps."""
+    nb, B, T = 12, 2, 200
+    cell, om = build_tree(nb, seed=7)
+    cell.branch(0).comp(0).record()
+    cell.branch("all").make_trainable("HH_gNa", verbose=False)
+    cell.branch("all").make_trainable("HH_gK", verbose=False)
+    cell.branch("all").make_trainable("HH_gLeak", verbose=False)
+    rng = np.random.default_rng(1)
+    base = np.array([0.12, 0.036, 0.0003])
+    vals = base[None, :, None] * rng.uniform(0.7, 1.3, size=(B, 3, nb))
+    params = [{k: torch.tensor(vals[:, j], dtype=torch.float64, device=device, requires_grad=True)}
+              for j, k in enumerate(("HH_gNa", "HH_gK", "HH_gLeak"))]
+    cur = c1_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=torch.float64, device=device))
+    v = jx.integrate(cell, params, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[4, 50])
+    target = torch.linspace(-70, -60, T + 1, dtype=torch.float64, device=device)
+    loss = ((v[:, 0] - target) ** 2).mean()
+    loss.backward()
+    # Oracle autograd.
+    ovals = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    comp = torch.repeat_interleave(ovals, 4, dim=-1)  # (B, 3, nb*4)
+    ov = {"HH_gNa": comp[:, 0], "HH_gK": comp[:, 1], "HH_gLeak": comp[:, 2]}
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0], rec_comps=[0],
+                     overrides=ov)
+    oloss = ((ref[:, 0] - target.cpu()) ** 2).mean()
+    oloss.backward()
+    np.testing.assert_allclose(loss.item(), oloss.item(), rtol=1e-9)
+    for j, p in enumerate(params):
+        np.testing.assert_allclose(p.grad.cpu().numpy(), ovals.grad[:, j].numpy(), rtol=1e-6, atol=1e-9)

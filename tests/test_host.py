@@ -1,0 +1,133 @@
+"""Host-side logic on the CPU: product schedules vs the oracle, the C ABI exports, and
+the fused kernel's descriptor tables executed by a CPU model of the kernel."""
+import ctypes
+import re
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import jaxhip as jx
+from jaxhip import _lib
+from jaxhip import schedule as S
+from oracle import jx_oracle as O
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_schedule_matches_oracle(seed):
+    rng = np.random.default_rng(100 + seed)
+    nb = int(rng.integers(1, 80))
+    parents = [-1] + [int(rng.integers(0, k)) for k in range(1, nb)]
+    ncomp = rng.integers(1, 6, size=nb)
+    for allowed in (1, 4, 32):
+        a = S.dhs_indexer(S.cell_tables(parents, ncomp), allowed)
+        b = O.dhs_indexer_cell(O.cell_tables(parents, ncomp), allowed)
+        for k in ("map_to_solve_order", "inv_map_to_solve_order", "map_to_solve_order_lower",
+                  "map_to_solve_order_upper", "parent_lookup", "node_order"):
+            assert np.array_equal(a[k], b[k]), k
+        assert all(np.array_equal(x, y) for x, y in zip(a["node_order_grouped"], b["node_order_grouped"]))
+
+
+def test_network_schedule_matches_oracle():
+    specs = [([-1, 0, 0], [2, 2, 2]), ([-1], [1]), ([-1, 0, 0, 1, 1], [4] * 5)]
+    cell_tabs = [S.cell_tables(p, n) for p, n in specs]
+    inds = [S.dhs_indexer(t) for t in cell_tabs]
+    ta, ia = S.network_indexer(cell_tabs, inds, True)
+    otabs = [O.cell_tables(p, n) for p, n in specs]
+    tb, ib = O.network_tables_and_indexer(otabs, [O.dhs_indexer_cell(t) for t in otabs], "gpu")
+    for k in ("source", "sink", "type", "ordered"):
+        assert np.array_equal(ta[k], tb[k])
+    for k in ("map_to_solve_order", "inv_map_to_solve_order", "map_to_solve_order_lower",
+              "map_to_solve_order_upper", "parent_lookup"):
+        assert np.array_equal(ia[k], ib[k]), k
+
+
+def test_forest_schedule_invariants():
+    rng = np.random.default_rng(3)
+    parents = [-1] + [int(rng.integers(0, k)) for k in range(1, 50)]
+    ind = S.dhs_indexer(S.cell_tables(parents, [4] * 50))
+    pl = ind["parent_lookup"]
+    sch = S.forest_schedule(pl, 1 << 20)
+    h = sch["height"]
+    for c, p in enumerate(pl):
+        if p >= 0:
+            assert h[p] >= h[c] + 1 and p < c
+    # Every node appears exactly once in the height order; levels are contiguous.
+    assert sorted(sch["order"].tolist()) == list(range(len(pl)))
+    lp = sch["level_ptr"]
+    for lev in range(len(lp) - 1):
+        assert all(h[k] == lev for k in sch["order"][lp[lev]:lp[lev + 1]])
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "jxhip.h").read_text()
+    declared = set(re.findall(r"^(?:int|const char\*)\s+(jxhip_\w+)\(", header, re.M))
+    assert declared == set(_lib.exported_symbols())
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert _lib.fn("jxhip_version")() == 100
+
+
+def test_struct_layouts_match_header():
+    """ctypes mirrors of the argument blocks have the C field order of jxhip.h."""
+    header = (ROOT / "include" / "jxhip.h").read_text()
+    body = header[header.index("typedef struct jxhip_fused_args"):header.index("} jxhip_fused_args;")]
+    names = re.findall(r"\b(\w+)(?:\[\w+\])?;", body)
+    assert names == [f[0] for f in _lib.FusedArgs._fields_]
+    body = header[header.index("typedef struct jxhip_adjoint_args"):header.index("} jxhip_adjoint_args;")]
+    names = re.findall(r"\b(\w+)(?:\[\w+\])?;", body)
+    assert names == [f[0] for f in _lib.AdjointArgs._fields_]
+
+
+def test_integrate_refuses_cpu_tensors():
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 2), [-1, 0, 0])
+    cell.insert(jx.HH())
+    cell.record()
+    ps = cell.data_set("HH_gNa", torch.tensor([[0.1]], dtype=torch.float64), None)
+    with pytest.raises(RuntimeError, match="GPU"):
+        jx.integrate(cell, param_state=ps, t_max=1.0)
+
+
+@pytest.fixture
+def cpu_model(monkeypatch):
+    """Run `integrate`'s host path on the CPU, the kernel replaced by its CPU model."""
+    from kernel_model import run_forward_model
+    import jaxhip.fused as F
+
+    monkeypatch.setattr(sys.modules["jaxhip.integrate"], "_require_gpu", lambda d: None)
+    monkeypatch.setattr(F.ForwardLaunch, "launch", lambda self: run_forward_model(self))
+
+
+def test_host_tables_c0(cpu_model):
+    from models import DT, build_c0, c0_stimulus
+
+    cell, om = build_c0()
+    cur = c0_stimulus(80)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=torch.float64))
+    v = jx.integrate(cell, data_stimuli=ds, delta_t=DT)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0], rec_comps=list(range(60)))
+    assert np.abs(v.numpy() - ref.numpy()).max() < 1e-9
+
+
+def test_host_tables_batched_tree(cpu_model):
+    from models import DT, build_tree, c1_params, c1_stimulus, data_set_params, oracle_overrides
+
+    B, T = 3, 60
+    cell, om = build_tree(12, seed=2)
+    cell.branch(0).comp(0).record()
+    cell.branch(3).comp(2).record()
+    prm = c1_params(B, seed=2)
+    ps = data_set_params(cell, prm, torch.device("cpu"), torch.float64)
+    cur = c1_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=torch.float64))
+    v = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT)
+    rc = int(cell._cumsum_ncomp[3] + 2)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0], rec_comps=[0, rc],
+                     overrides=oracle_overrides(om, prm))
+    assert v.shape == (B, 2, T + 1)
+    assert np.abs(v.numpy() - ref.numpy()).max() < 1e-9

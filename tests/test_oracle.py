@@ -1,0 +1,153 @@
+"""The oracle pinned against the reference's golden traces, closed forms and networkx."""
+import json
+from pathlib import Path
+
+import networkx as nx
+import numpy as np
+import pytest
+import torch
+
+from oracle import jx_oracle as O
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_traces.json").read_text())
+CUR = O.step_current(0.5, 1.0, 0.02, 0.025, 5.0)
+
+
+def _check(v, key, tol=1e-8):
+    gold = np.asarray(GOLD[key]["values"])
+    err = np.abs(v[..., ::20].numpy() - gold).max()
+    assert err <= tol, (key, err)
+
+
+def test_step_current_window():
+    # stimulus.py:8-36: int(t_max // dt) + 2 samples (5.0 // 0.025 == 199.0 in binary
+    # floating point), on for [int(0.5 / dt), int(1.5 / dt)) = [20, 60).
+    assert CUR.shape == (201,)
+    assert np.nonzero(CUR.numpy())[0].tolist() == list(range(20, 60))
+
+
+def test_golden_compartment():
+    m = O.OracleModel([([-1], [1])])
+    m.hh_comps = np.array([0])
+    v = O.simulate(m, None, 0.025, stim=CUR[None], stim_comps=[0], rec_comps=[0])
+    _check(v[0], "compartment")
+
+
+def test_golden_branch():
+    m = O.OracleModel([([-1], [2])])
+    m.hh_comps = np.arange(2)
+    v = O.simulate(m, None, 0.025, stim=CUR[None], stim_comps=[0], rec_comps=[0])
+    _check(v[0], "branch2")
+
+
+@pytest.mark.parametrize("solver", ["jaxley.dhs.gpu", "jaxley.dhs.cpu", "jax.sparse"])
+def test_golden_cell(solver):
+    m = O.OracleModel([([-1, 0, 0], [2, 2, 2])])
+    m.hh_comps = np.arange(6)
+    v = O.simulate(m, None, 0.025, stim=CUR[None], stim_comps=[2], rec_comps=[2], voltage_solver=solver)
+    _check(v[0], "cell_3_2")
+
+
+def test_golden_radius_and_length():
+    np.random.seed(1)
+    m = O.OracleModel([([-1], [1])])
+    m.hh_comps = np.array([0])
+    m.length = 5 * np.random.rand(1)
+    m.radius = np.random.rand(1)
+    v = O.simulate(m, None, 0.025, stim=CUR[None], stim_comps=[0], rec_comps=[0])
+    _check(v[0], "radius_and_length_compartment")
+
+
+def test_golden_network_ionotropic():
+    m = O.OracleModel([([-1, 0, 0], [2, 2, 2])] * 2)
+    m.hh_comps = np.arange(12)
+    gs = 0.5 / (100000.0 / (2 * np.pi * 10.0 * 1.0))
+    p = dict(O.SYN_DEFAULTS["IonotropicSynapse"], gS=gs)
+    # cell 1's first compartment is node 7 (6 compartments + 1 branchpoint in cell 0).
+    m.synapses = [{"type": "IonotropicSynapse", "pre": np.array([0]), "post": np.array([7]),
+                   "params": {k: np.array([v]) for k, v in p.items()}, "s0": np.array([0.2])}]
+    v = O.simulate(m, None, 0.025, stim=torch.stack([CUR, CUR]), stim_comps=[2, 8], rec_comps=[2, 8])
+    _check(v, "net_2x_cell_3_2")
+
+
+def test_ragged_indexer_solve_vs_dense():
+    """The hand-built ragged indexer of the reference's tests/test_solver.py:78-104."""
+    ind = {"node_order_grouped": [np.array([[1, 0], [2, 0]]), np.array([[3, 1]])],
+           "parent_lookup": np.array([-1, 0, 0, 1, -1])}
+    d = torch.tensor([4.0, 5.0, 6.0, 7.0, 1.0], dtype=torch.float64)
+    l = torch.tensor([0.0, -0.4, -0.3, -0.2, 0.0], dtype=torch.float64)
+    u = torch.tensor([0.0, -0.5, -0.1, -0.6, 0.0], dtype=torch.float64)
+    b = torch.tensor([1.0, 2.0, 3.0, 4.0, 0.0], dtype=torch.float64)
+    A = O.dense_matrix(d, l, u, ind["parent_lookup"])
+    ref = np.linalg.solve(A, b.numpy())
+    for gpu in (False, True):
+        x = O.DHSSolverOracle(ind, optimize_for_gpu=gpu).solve(d, l, u, b)
+        np.testing.assert_allclose(x.numpy(), ref, rtol=1e-12)
+
+
+def _nx_order(tables, allowed):
+    """The reference's own construction with networkx (to_graph edges -> to_undirected
+    -> bfs_edges) for comparison with the oracle's restatement."""
+    g = nx.DiGraph()
+    g.add_nodes_from(range(tables["n_nodes"]))
+    typ, s, t = tables["type"], tables["source"], tables["sink"]
+    keep = np.isin(typ, [2, 3]) | ((typ == 0) & (s < t))
+    g.add_edges_from(np.stack([s[keep], t[keep]], 1).tolist())
+    ug = g.to_undirected()
+    dist = nx.single_source_shortest_path_length(ug, 0)
+    out, depth, since, last = [], 0, 0, 0
+    for u_, v_ in nx.bfs_edges(ug, 0):
+        if dist[v_] > last:
+            depth, since, last = depth + 1, 0, dist[v_]
+        elif since >= allowed:
+            depth, since = depth + 1, 0
+        since += 1
+        out.append((u_, v_, depth))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_bfs_order_matches_networkx(seed):
+    rng = np.random.default_rng(seed)
+    nb = int(rng.integers(1, 60))
+    parents = [-1] + [int(rng.integers(0, k)) for k in range(1, nb)]
+    ncomp = rng.integers(1, 6, size=nb)
+    tab = O.cell_tables(parents, ncomp)
+    for allowed in (1, 4, 32):
+        ours = O.bfs_edge_hops(O._undirected_adjacency(tab["n_nodes"], tab), 0, allowed)
+        assert ours == _nx_order(tab, allowed)
+
+
+def test_random_tree_solve_vs_dense():
+    rng = np.random.default_rng(0)
+    parents = [-1] + [int(rng.integers(0, k)) for k in range(1, 30)]
+    tab = O.cell_tables(parents, [3] * 30)
+    ind = O.dhs_indexer_cell(tab, 32)
+    n = tab["n_nodes"] + 1
+    pl = ind["parent_lookup"]
+    d = torch.tensor(4 + rng.uniform(size=n))
+    l = torch.tensor(-rng.uniform(size=n))
+    u = torch.tensor(-rng.uniform(size=n))
+    b = torch.tensor(rng.normal(size=n))
+    for gpu in (False, True):
+        x = O.DHSSolverOracle(ind, optimize_for_gpu=gpu).solve(d, l, u, b)
+        np.testing.assert_allclose(x.numpy(), np.linalg.solve(O.dense_matrix(d, l, u, pl), b.numpy()), rtol=1e-10)
+
+
+def test_kinetic_closed_form():
+    """Exponential Euler is exact for ds/dt = a (1 - s) - beta s at a fixed v_pre."""
+    p = {k: torch.tensor(v, dtype=torch.float64) for k, v in O.SYN_DEFAULTS["KineticSynapse"].items()}
+    vpre = torch.tensor(-10.0, dtype=torch.float64)
+    s = torch.tensor(0.1, dtype=torch.float64)
+    dt = 0.025
+    a = p["alpha"] / (1 + np.exp(-(vpre - p["v_th"]) / p["delta"]))
+    r = a + p["beta"]
+    exact = a / r + (s - a / r) * np.exp(-r * dt)
+    np.testing.assert_allclose(float(O.kinetic_update(s, vpre, p, dt)), float(exact), rtol=1e-14)
+
+
+def test_x_over_expm1_removable_singularity():
+    """tests/test_float32_safety.py:25-73: alpha_m(-40) = 1, alpha_n(-55) = 0.1."""
+    (am, _), _, (an, _) = O.hh_rates(torch.tensor([-40.0, -55.0], dtype=torch.float64))
+    np.testing.assert_allclose(float(am[0]), 1.0, rtol=1e-12)
+    np.testing.assert_allclose(float(an[1]), 0.1, rtol=1e-12)
