@@ -138,4 +138,5 @@ def test_adjoint_matches_oracle_autograd(device):
     oloss.backward()
     np.testing.assert_allclose(loss.item(), oloss.item(), rtol=1e-9)
     for j, p in enumerate(params):
-        np.testing.assert_allclose(p.grad.cpu().numpy(), ovals.grad[:, j].numpy(), rtol=1e-6, atol=1e-9)
+        g = list(p.values())[0].grad
+        np.testing.assert_allclose(g.cpu().numpy(), ovals.grad[:, j].numpy(), rtol=1e-6, atol=1e-9)
