@@ -174,13 +174,22 @@ int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int3
  *   time index), state_off (element offset of the program's [node][S] blocks),
  *   sim_off (global index of its first simulation), rec_off, syn_off ([syn][S] blocks)
  * Topologies: int32 (n_topo, JXHIP_TOPO_COLS):
- *   n (nodes, solve order), H (max height), node_base, level_base, child_base, n_rec,
- *   rec_base, n_tgt, tgt_base, syn_ptr_base, syn_base, n_syn, 0, 0, 0, 0
+ *   n (nodes), L (max segment level), node_base, seg_level_base, child_base, n_rec,
+ *   rec_base, n_tgt, tgt_base, syn_ptr_base, syn_base, n_syn, seg_base, seg_child_base,
+ *   0, 0
+ * Nodes are in "kernel order": segments (maximal single-child chains, top first) grouped
+ * by segment level, leaf segments first. A segment's level is 0 if its bottom node has
+ * no children, else 1 + the largest level of its child segments.
  * Per-node int32 (JXHIP_NODE_ICOLS) at node_base + k: parent (-1 root), child_beg,
  *   child_end (into child_idx at child_base), flags (1 comp, 2 HH, 4 Leak), stim_beg,
  *   stim_end (stimulus entries), target index (-1 none), 0
- * Per-node float (JXHIP_NODE_FCOLS): lo = -dt g(parent->k), up = -dt g(k->parent),
+ * Per-node solve type (JXHIP_NODE_FCOLS): lo = -dt g(parent->k), up = -dt g(k->parent),
  *   dbase = [comp] + sum of dt g over edges into k, inv_cm (0 at branchpoints)
+ * Segments int32 (n_seg, 4) at seg_base: first node, end node, child_beg, child_end (into
+ *   seg_child at seg_child_base: the top nodes of the child segments); seg_level_ptr:
+ *   L + 2 offsets (segments of level l are [ptr[l], ptr[l+1])).
+ * Voltages are float64 in both modes (the solve runs on increments, see jxhip_fused.hip);
+ * every other floating array has the entry point's element type.
  * ------------------------------------------------------------------------------- */
 #define JXHIP_PROG_COLS 8
 #define JXHIP_TOPO_COLS 16
@@ -193,8 +202,9 @@ typedef struct jxhip_fused_args {
   const int64_t* progs;
   int64_t n_progs;
   const int32_t* topos;
-  const int32_t* order;     /* nodes sorted by height, at node_base */
-  const int32_t* level_ptr; /* H+2 per topology at level_base */
+  const int32_t* seg;       /* (n_seg_total, 4) */
+  const int32_t* seg_level_ptr;
+  const int32_t* seg_child;
   const int32_t* node_i;    /* (n_nodes_total, JXHIP_NODE_ICOLS) */
   const void* node_f;       /* (n_nodes_total, JXHIP_NODE_FCOLS), element type */
   const int32_t* child_idx; /* topology-relative children at child_base */
@@ -205,7 +215,7 @@ typedef struct jxhip_fused_args {
      1 = per simulation (sim_off + s), 2 = per (node, simulation) (state_off + k S + s) */
   const void* param[JXHIP_NPARAM];
   int32_t param_mode[JXHIP_NPARAM];
-  /* state in program layout [node][S] at state_off */
+  /* state in program layout [node][S] at state_off; v is float64 */
   void* v;
   void* gate_m;
   void* gate_h;
@@ -230,11 +240,14 @@ typedef struct jxhip_fused_args {
   double diff;              /* finite difference of the synapse slope (1e-3) */
   int32_t S;                /* simulations per program: 1, 2, 4, ..., 64 */
   int32_t threads;          /* threads per workgroup: multiple of 64, <= 1024 */
+  int32_t items_per_thread; /* forward: (node, simulation) items per thread, held in
+                               registers: 1, 2, 4, 8, 12, 16, 20, 24 or 32, with
+                               n S <= items_per_thread threads */
   int64_t lds_bytes;        /* dynamic LDS per workgroup (<= 160 KiB) */
   /* checkpoints (for the adjoint): per program, snapshot c of the state before step
      c * ckpt_every at ckpt_* + c * ckpt_stride + state_off (syn: ckpt_syn_stride,
      syn_off); NULL = none */
-  void* ckpt_v;
+  void* ckpt_v;             /* float64 */
   void* ckpt_m;
   void* ckpt_h;
   void* ckpt_n;
@@ -263,7 +276,7 @@ typedef struct jxhip_adjoint_args {
   const void* g_rec;
   void* g_param[JXHIP_NPARAM];
   void* g_v0;
-  void* work_v;
+  void* work_v;             /* float64 */
   void* work_m;
   void* work_h;
   void* work_n;

@@ -1,111 +1,139 @@
-// Fused whole-simulation kernel for gfx950: every workgroup ("program") owns S
-// simulations of one topology for a range of time steps and runs, per step,
-//   synapses (regular layout)  -> channels (HH, Leak) + stimulus -> implicit Euler
-//   system -> leaves-first elimination by height -> back-substitution -> record,
+// Fused whole-simulation kernels for gfx950.
+//
+// Every workgroup ("program") owns S simulations of one topology (a cell, a network of
+// cells, or a pack of trees) for a range of time steps and runs, per step,
+//   synapses -> channels (HH, Leak) + stimulus -> implicit Euler system -> tree solve
+//   -> voltage update -> record,
 // i.e. `_step_with_plan` (modules/base.py:3214-3400) with the DHS solve
-// (solver_voltage.py:294-359), inside the time loop of `integrate`
-// (integrate.py:624-662). Voltages, the diagonal and the right-hand side live in LDS
-// for the whole launch ([node][S], simulations fastest); gates and synapse states are
-// streamed from HBM in the same [node][S] program layout (coalesced). Simulations are
-// independent, so no workgroup ever waits for another: there is no grid-wide sync.
+// (solver_voltage.py:294-359), inside the time loop of `integrate` (integrate.py:624-662).
+// Simulations are independent: no workgroup waits for another (no grid-wide sync).
+//
+// Numerics. The backward-Euler system A v(t+1) = b is solved for the increment
+//   A dv = r,  r = b - A v(t),  v(t+1) = v(t) + dv,
+// which is the same solution; r is formed from the ionic currents and the axial
+// currents (differences of neighbouring voltages), so the solve and the gates can run
+// in float32 while the voltage state is accumulated in float64: the increment is small,
+// its float32 rounding is far below a float32 ulp of the voltage. In float64 mode every
+// stage is float64.
+//
+// Layout. Nodes are renumbered by the host into "kernel order": segments (maximal
+// single-child chains, top to bottom) grouped by segment level (leaf segments first).
+// The triangularization runs segment level by segment level, one thread per (segment,
+// simulation) walking its chain bottom-up in registers; the back-substitution walks
+// top-down. LDS holds the diagonal and rhs/increment, [node][S] (simulations fastest).
+// In the forward kernel every thread owns the same (node, simulation) items for the
+// whole run, so gates and voltages of its items stay in VGPRs; a float64 copy of the
+// voltages in HBM gives neighbours' voltages to the axial currents.
 #include "jxhip_common.h"
 #include "../../include/jxhip.h"
 
 namespace jxhip {
 
-template <typename T>
+// Column layout of the tables (see jxhip.h).
+enum : int { kNiParent = 0, kNiChildBeg, kNiChildEnd, kNiFlags, kNiStimBeg, kNiStimEnd, kNiTgt };
+enum : int { kNfLo = 0, kNfUp, kNfDbase, kNfInvCm };
+enum : int32_t { kComp = 1, kHH = 2, kLeak = 4 };
+
+template <typename G, typename L>
 struct Fused {
   const int64_t* progs;
   const int32_t* topos;
-  const int32_t* order;
-  const int32_t* level_ptr;
+  const int32_t* seg;        // (n_seg_total, 4): beg, end, child_beg, child_end
+  const int32_t* seg_lptr;   // per topology: L+2 segment offsets per segment level
+  const int32_t* seg_child;  // child segment tops (kernel node index)
   const int32_t* node_i;
-  const T* node_f;
+  const L* node_f;
   const int32_t* child_idx;
   const int32_t* stim_col;
-  const T* stim_scale;
+  const G* stim_scale;
   const int32_t* rec_node;
-  const T* param[JXHIP_NPARAM];
+  const G* param[JXHIP_NPARAM];
   int32_t param_mode[JXHIP_NPARAM];
-  T* v;
-  T* gm;
-  T* gh;
-  T* gn;
-  const T* stim;
+  double* v;
+  G* gm;
+  G* gh;
+  G* gn;
+  const G* stim;
   int64_t stim_cols;
   int64_t stim_b;
-  T* rec;
+  G* rec;
   int32_t syn_model;
   const int32_t* syn_pre;
   const int32_t* syn_ptr;
   const int32_t* tgt_node;
-  const T* syn_param[JXHIP_NSYNPARAM];
-  const T* syn_scale;
-  T* syn_s;
-  T dt;
-  T diff;
-  T* ckpt_v;
-  T* ckpt_m;
-  T* ckpt_h;
-  T* ckpt_n;
-  T* ckpt_s;
+  const G* syn_param[JXHIP_NSYNPARAM];
+  const G* syn_scale;
+  G* syn_s;
+  G dt;
+  G diff;
+  int32_t S;
+  int32_t log2S;
+  double* ckpt_v;
+  G* ckpt_m;
+  G* ckpt_h;
+  G* ckpt_n;
+  G* ckpt_s;
   int64_t ckpt_every;
   int64_t ckpt_stride;
   int64_t ckpt_syn_stride;
 };
 
-template <typename T>
-Fused<T> unpack(const jxhip_fused_args& a) {
-  Fused<T> f;
+template <typename G, typename L>
+Fused<G, L> unpack(const jxhip_fused_args& a) {
+  Fused<G, L> f;
   f.progs = a.progs;
   f.topos = a.topos;
-  f.order = a.order;
-  f.level_ptr = a.level_ptr;
+  f.seg = a.seg;
+  f.seg_lptr = a.seg_level_ptr;
+  f.seg_child = a.seg_child;
   f.node_i = a.node_i;
-  f.node_f = static_cast<const T*>(a.node_f);
+  f.node_f = static_cast<const L*>(a.node_f);
   f.child_idx = a.child_idx;
   f.stim_col = a.stim_col;
-  f.stim_scale = static_cast<const T*>(a.stim_scale);
+  f.stim_scale = static_cast<const G*>(a.stim_scale);
   f.rec_node = a.rec_node;
   for (int j = 0; j < JXHIP_NPARAM; ++j) {
-    f.param[j] = static_cast<const T*>(a.param[j]);
+    f.param[j] = static_cast<const G*>(a.param[j]);
     f.param_mode[j] = a.param_mode[j];
   }
-  f.v = static_cast<T*>(a.v);
-  f.gm = static_cast<T*>(a.gate_m);
-  f.gh = static_cast<T*>(a.gate_h);
-  f.gn = static_cast<T*>(a.gate_n);
-  f.stim = static_cast<const T*>(a.stim);
+  f.v = static_cast<double*>(a.v);
+  f.gm = static_cast<G*>(a.gate_m);
+  f.gh = static_cast<G*>(a.gate_h);
+  f.gn = static_cast<G*>(a.gate_n);
+  f.stim = static_cast<const G*>(a.stim);
   f.stim_cols = a.stim_cols;
   f.stim_b = a.stim_b;
-  f.rec = static_cast<T*>(a.rec);
+  f.rec = static_cast<G*>(a.rec);
   f.syn_model = a.syn_model;
   f.syn_pre = a.syn_pre;
   f.syn_ptr = a.syn_ptr;
   f.tgt_node = a.tgt_node;
-  for (int j = 0; j < JXHIP_NSYNPARAM; ++j) f.syn_param[j] = static_cast<const T*>(a.syn_param[j]);
-  f.syn_scale = static_cast<const T*>(a.syn_scale);
-  f.syn_s = static_cast<T*>(a.syn_s);
-  f.dt = static_cast<T>(a.dt);
-  f.diff = static_cast<T>(a.diff);
-  f.ckpt_v = static_cast<T*>(a.ckpt_v);
-  f.ckpt_m = static_cast<T*>(a.ckpt_m);
-  f.ckpt_h = static_cast<T*>(a.ckpt_h);
-  f.ckpt_n = static_cast<T*>(a.ckpt_n);
-  f.ckpt_s = static_cast<T*>(a.ckpt_s);
+  for (int j = 0; j < JXHIP_NSYNPARAM; ++j) f.syn_param[j] = static_cast<const G*>(a.syn_param[j]);
+  f.syn_scale = static_cast<const G*>(a.syn_scale);
+  f.syn_s = static_cast<G*>(a.syn_s);
+  f.dt = static_cast<G>(a.dt);
+  f.diff = static_cast<G>(a.diff);
+  f.S = a.S;
+  int l = 0;
+  while ((1 << l) < a.S) ++l;
+  f.log2S = l;
+  f.ckpt_v = static_cast<double*>(a.ckpt_v);
+  f.ckpt_m = static_cast<G*>(a.ckpt_m);
+  f.ckpt_h = static_cast<G*>(a.ckpt_h);
+  f.ckpt_n = static_cast<G*>(a.ckpt_n);
+  f.ckpt_s = static_cast<G*>(a.ckpt_s);
   f.ckpt_every = a.ckpt_every;
   f.ckpt_stride = a.ckpt_stride;
   f.ckpt_syn_stride = a.ckpt_syn_stride;
   return f;
 }
 
-// Program / topology descriptors (column meaning in jxhip.h).
 struct Prog {
   int32_t topo, nsim;
   int64_t t0, t1, soff, simoff, roff, synoff;
 };
 struct Topo {
-  int32_t n, H, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn;
+  int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb;
 };
 
 __device__ __forceinline__ Prog load_prog(const int64_t* progs, int64_t p) {
@@ -126,7 +154,7 @@ __device__ __forceinline__ Topo load_topo(const int32_t* topos, int32_t t) {
   const int32_t* r = topos + static_cast<int64_t>(t) * JXHIP_TOPO_COLS;
   Topo o;
   o.n = r[0];
-  o.H = r[1];
+  o.L = r[1];
   o.nb = r[2];
   o.lb = r[3];
   o.cb = r[4];
@@ -137,293 +165,350 @@ __device__ __forceinline__ Topo load_topo(const int32_t* topos, int32_t t) {
   o.spb = r[9];
   o.synb = r[10];
   o.n_syn = r[11];
+  o.sb = r[12];
+  o.scb = r[13];
   return o;
 }
 
-enum : int32_t { kComp = 1, kHH = 2, kLeak = 4 };
+// Reciprocal: hardware rcp in float32 (the operands are increments and diagonals of a
+// diagonally dominant system; 1 ulp is far below the tolerance), IEEE in float64.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 
-template <typename T, int S>
-__device__ __forceinline__ T param_at(const Fused<T>& a, int j, const Prog& pg, const Topo& tp,
+template <typename G, typename L>
+__device__ __forceinline__ G param_at(const Fused<G, L>& a, int j, const Prog& pg, const Topo& tp,
                                       int k, int s) {
   const int32_t mode = a.param_mode[j];
   if (mode == 0) return a.param[j][tp.nb + k];
   if (mode == 1) return a.param[j][pg.simoff + s];
-  return a.param[j][pg.soff + static_cast<int64_t>(k) * S + s];
+  return a.param[j][pg.soff + (static_cast<int64_t>(k) << a.log2S) + s];
 }
 
-// Synapse step for one (target, simulation): the regular-layout synapse step
-// (modules/network.py:382-478, kernels/synapse_step.py:121-182). Returns the summed
-// slope (lin) and minus the summed offset (cst) in uA/cm^2 (per mV).
-template <typename T, int S>
-__device__ __forceinline__ void synapse_target(const Fused<T>& a, const Prog& pg, const Topo& tp,
-                                               const T* xs, int tgt, int s, T vp, T& lin,
-                                               T& cst) {
-  T sl = T(0), sc = T(0);
-  const T dt = a.dt;
-  // Synapses of a target: [syn_ptr[t], syn_ptr[t+1]) (sorted by target; regular layouts
-  // have syn_ptr[t] = t k).
+// Synapses of one target and simulation (modules/network.py:382-478,
+// kernels/synapse_step.py:121-182): `lin` = summed slope of i / area * 1e5 (the
+// reference's finite difference with both voltages perturbed by `diff`), `cur` = summed
+// i / area * 1e5 at the unperturbed voltages.
+template <typename G, typename L>
+__device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog& pg,
+                                               const Topo& tp, int tgt, int s, G& lin, G& cur) {
+  G sl = G(0), sc = G(0);
+  const G dt = a.dt;
   const int32_t* sptr = a.syn_ptr + tp.spb;
+  const int S = a.S;
+  const double* v = a.v + pg.soff;
   for (int syn = sptr[tgt]; syn < sptr[tgt + 1]; ++syn) {
     const int64_t gsyn = tp.synb + syn;
     const int pre = a.syn_pre[gsyn];
-    const T vpre = xs[pre * S + s];
-    const T scale = a.syn_scale[gsyn];
-    const T gs = a.syn_param[0][gsyn];
-    T vt, ct;
+    const G vpre = static_cast<G>(v[pre * S + s]);
+    const G scale = a.syn_scale[gsyn];
+    const G gs = a.syn_param[0][gsyn];
     if (a.syn_model == 0) {
-      const T e_syn = a.syn_param[1][gsyn];
-      const T alpha = a.syn_param[2][gsyn];
-      const T beta = a.syn_param[3][gsyn];
-      const T v_th = a.syn_param[4][gsyn];
-      const T delta = a.syn_param[5][gsyn];
-      T* sp = a.syn_s + pg.synoff + static_cast<int64_t>(syn) * S + s;
-      const T st = *sp;
-      const T tr = sigmoid((vpre - v_th) / delta);
-      const T aa = alpha * tr;
-      const T rate = aa + beta;
-      const T sinf = aa / rate;
-      const T snew = sinf + (st - sinf) * dexp(-dt * rate);
+      const int node = a.tgt_node[tp.tgb + tgt];
+      const G e_syn = a.syn_param[1][gsyn];
+      const G alpha = a.syn_param[2][gsyn];
+      const G beta = a.syn_param[3][gsyn];
+      const G v_th = a.syn_param[4][gsyn];
+      const G delta = a.syn_param[5][gsyn];
+      G* sp = a.syn_s + pg.synoff + static_cast<int64_t>(syn) * S + s;
+      const G st = *sp;
+      const G tr = sigmoid((vpre - v_th) / delta);
+      const G aa = alpha * tr;
+      const G rate = aa + beta;
+      const G sinf = aa / rate;
+      const G snew = sinf + (st - sinf) * dexp(-dt * rate);
       *sp = snew;
-      // i = gS s (v_post - e_syn) is linear in v_post: slope gS s, offset -gS s e_syn.
-      vt = gs * snew * scale;
-      ct = -vt * e_syn;
+      // i = gS s (v_post - e_syn) is linear in v_post: the slope is gS s.
+      const G g = gs * snew * scale;
+      sl += g;
+      sc += g * static_cast<G>(v[node * S + s] - static_cast<double>(e_syn));
     } else {
-      const T xo = a.syn_param[1][gsyn];
-      const T slope = a.syn_param[2][gsyn];
-      // The reference perturbs v_pre and v_post by `diff` (network.py:452-457); the
-      // slope is the difference of sigmoid((v_pre - x_off) slope) at v_pre and
-      // v_pre + diff, computed without cancellation:
-      //   sig(b) - sig(a) = (1 - sig(a)) sig(b) (-expm1(-(b - a))).
-      const T za = (vpre - xo) * slope;
-      const T zb = ((vpre + a.diff) - xo) * slope;
-      const T sa = sigmoid(za);
-      const T sb = sigmoid(zb);
-      const T dsig = (T(1) - sa) * sb * (-dexpm1(-(zb - za)));
-      const T d0 = -gs * sa * scale;
-      vt = -gs * dsig * scale / a.diff;
-      ct = d0 - vt * vp;
+      const G xo = a.syn_param[1][gsyn];
+      const G slope = a.syn_param[2][gsyn];
+      // sig(b) - sig(a) = (1 - sig(a)) sig(b) (-expm1(-(b - a))) without cancellation.
+      const G za = (vpre - xo) * slope;
+      const G zb = ((vpre + a.diff) - xo) * slope;
+      const G sa = sigmoid(za);
+      const G sb = sigmoid(zb);
+      const G dsig = (G(1) - sa) * sb * (-dexpm1(-(zb - za)));
+      sl += -gs * dsig * scale / a.diff;
+      sc += -gs * sa * scale;
     }
-    sl += vt;
-    sc += ct;
   }
   lin = sl;
-  cst = -sc;
+  cur = sc;
 }
 
-// LDS of a program: xs (voltages / rhs / solution) and ds (diagonal), [node][S]; then
-// the synapse slope and offset per (target, simulation).
-template <typename T, int S>
+// LDS of a program.
+template <typename L>
 struct Lds {
-  T* xs;
-  T* ds;
-  T* tl;
-  T* tc;
+  L* xs;   // rhs, then the increment (solution)
+  L* ds;   // diagonal, then its reciprocal after elimination
+  L* tl;   // synapse slope per (target, simulation)
+  L* tc;   // synapse current per (target, simulation)
 };
 
-// Gate pointers of one step: read at `*_in`, written at `*_out` (program blocks).
-template <typename T>
-struct Gates {
-  const T* m_in;
-  const T* h_in;
-  const T* n_in;
-  T* m_out;
-  T* h_out;
-  T* n_out;
-};
-
-// Triangularization, leaves first (a node gathers its already eliminated children), and
-// back-substitution, roots first. `kTransposed` swaps lower and upper (A^T).
-template <typename T, int S, bool kTransposed>
-__device__ __forceinline__ void tree_solve(const Fused<T>& a, const Topo& tp, T* xs, T* ds) {
+// Triangularization leaves first, one thread per (segment, simulation) walking its
+// chain bottom-up; back-substitution top-down. kT: solve with A^T (lower <-> upper).
+template <typename G, typename L, bool kT>
+__device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp, L* xs, L* ds) {
   const int tid = threadIdx.x, nthr = blockDim.x;
+  const int S = a.S, lS = a.log2S;
+  const int32_t* sg = a.seg + static_cast<int64_t>(tp.sb) * 4;
+  const int32_t* lptr = a.seg_lptr + tp.lb;
+  const int32_t* sch = a.seg_child + tp.scb;
+  const L* nf = a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS;
   const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
-  const T* nf = a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS;
-  const int32_t* order = a.order + tp.nb;
-  const int32_t* lptr = a.level_ptr + tp.lb;
-  const int32_t* cidx = a.child_idx + tp.cb;
-  constexpr int kLo = kTransposed ? 1 : 0;
-  constexpr int kUp = kTransposed ? 0 : 1;
-  for (int h = 1; h <= tp.H; ++h) {
-    const int lo = lptr[h], w = (lptr[h + 1] - lo) * S;
+  constexpr int kLo = kT ? kNfUp : kNfLo;
+  constexpr int kUp = kT ? kNfLo : kNfUp;
+  for (int lv = 0; lv <= tp.L; ++lv) {
+    const int lo = lptr[lv], w = (lptr[lv + 1] - lo) << lS;
     for (int i = tid; i < w; i += nthr) {
-      const int k = order[lo + i / S], s = i % S;
-      const int32_t* nik = ni + k * JXHIP_NODE_ICOLS;
-      T dk = ds[k * S + s], xk = xs[k * S + s];
-      for (int j = nik[1]; j < nik[2]; ++j) {
-        const int c = cidx[j];
-        const T f = nf[c * JXHIP_NODE_FCOLS + kUp] / ds[c * S + s];
+      const int q = lo + (i >> lS), s = i & (S - 1);
+      const int beg = sg[4 * q], end = sg[4 * q + 1];
+      int k = end - 1;
+      L dk = ds[(k << lS) + s], xk = xs[(k << lS) + s];
+      for (int j = sg[4 * q + 2]; j < sg[4 * q + 3]; ++j) {
+        const int c = sch[j];
+        // ds[c] holds 1 / d[c] of the already eliminated child segment top.
+        const L f = nf[c * JXHIP_NODE_FCOLS + kUp] * ds[(c << lS) + s];
         dk -= nf[c * JXHIP_NODE_FCOLS + kLo] * f;
-        xk -= xs[c * S + s] * f;
+        xk -= xs[(c << lS) + s] * f;
       }
-      ds[k * S + s] = dk;
-      xs[k * S + s] = xk;
+      L inv = recip(dk);
+      ds[(k << lS) + s] = inv;
+      xs[(k << lS) + s] = xk;
+      for (k = end - 2; k >= beg; --k) {
+        const L f = nf[(k + 1) * JXHIP_NODE_FCOLS + kUp] * inv;
+        const L d2 = ds[(k << lS) + s] - nf[(k + 1) * JXHIP_NODE_FCOLS + kLo] * f;
+        xk = xs[(k << lS) + s] - xk * f;
+        inv = recip(d2);
+        ds[(k << lS) + s] = inv;
+        xs[(k << lS) + s] = xk;
+      }
     }
     __syncthreads();
   }
-  for (int h = tp.H; h >= 0; --h) {
-    const int lo = lptr[h], w = (lptr[h + 1] - lo) * S;
+  for (int lv = tp.L; lv >= 0; --lv) {
+    const int lo = lptr[lv], w = (lptr[lv + 1] - lo) << lS;
     for (int i = tid; i < w; i += nthr) {
-      const int k = order[lo + i / S], s = i % S;
-      const int p = ni[k * JXHIP_NODE_ICOLS + 0];
-      T xk = xs[k * S + s];
-      if (p >= 0) xk -= nf[k * JXHIP_NODE_FCOLS + kLo] * xs[p * S + s];
-      xs[k * S + s] = xk / ds[k * S + s];
+      const int q = lo + (i >> lS), s = i & (S - 1);
+      const int beg = sg[4 * q], end = sg[4 * q + 1];
+      const int p = ni[beg * JXHIP_NODE_ICOLS + kNiParent];
+      L xp = p >= 0 ? xs[(p << lS) + s] : L(0);
+      for (int k = beg; k < end; ++k) {
+        L xk = xs[(k << lS) + s];
+        if (k > beg || p >= 0) xk -= nf[k * JXHIP_NODE_FCOLS + kLo] * xp;
+        xk *= ds[(k << lS) + s];
+        xs[(k << lS) + s] = xk;
+        xp = xk;
+      }
     }
     __syncthreads();
   }
 }
 
-// One time step t of `_step_with_plan`: synapses, channels + stimulus, the implicit
-// Euler system and its solve. On entry xs holds v(t); on exit v(t+1).
-template <typename T, int S>
-__device__ __forceinline__ void step_body(const Fused<T>& a, const Prog& pg, const Topo& tp,
-                                          const Lds<T, S>& L, int64_t t, const Gates<T>& G) {
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int nS = tp.n * S;
-  const T dt = a.dt;
-  const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
-  const T* nf = a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS;
-  T* xs = L.xs;
-  T* ds = L.ds;
-  // Synapses (they read the voltages of the previous step).
-  if (a.syn_model >= 0) {
-    for (int i = tid; i < tp.n_tgt * S; i += nthr) {
-      const int tgt = i / S, s = i % S;
-      const int node = a.tgt_node[tp.tgb + tgt];
-      T lin = T(0), cst = T(0);
-      if (s < pg.nsim) synapse_target<T, S>(a, pg, tp, xs, tgt, s, xs[node * S + s], lin, cst);
-      L.tl[i] = lin;
-      L.tc[i] = cst;
-    }
-    __syncthreads();
+// System of one (node, simulation) item: diagonal d and residual r = b - A v.
+// Gates are read from / written to the references (registers or memory).
+template <typename G, typename L>
+__device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
+                                           const Lds<L>& Ld, const double* vg, int k, int s,
+                                           double v, int64_t t, G& m, G& h, G& n) {
+  const int lS = a.log2S;
+  const int i = (k << lS) + s;
+  const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
+  const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
+  const int32_t flags = nik[kNiFlags];
+  // Axial part of the residual: lo_k (v_k - v_parent) + sum_c up_c (v_k - v_c).
+  L ax = L(0);
+  const int p = nik[kNiParent];
+  if (p >= 0) ax += nfk[kNfLo] * static_cast<L>(v - vg[(p << lS) + s]);
+  const int32_t* cidx = a.child_idx + tp.cb;
+  for (int j = nik[kNiChildBeg]; j < nik[kNiChildEnd]; ++j) {
+    const int c = cidx[j];
+    ax += a.node_f[static_cast<int64_t>(tp.nb + c) * JXHIP_NODE_FCOLS + kNfUp] *
+          static_cast<L>(v - vg[(c << lS) + s]);
   }
-  // Channels, stimulus and the implicit Euler system (solver_voltage.py:250-291).
-  for (int i = tid; i < nS; i += nthr) {
-    const int k = i / S, s = i % S;
-    const int32_t* nik = ni + k * JXHIP_NODE_ICOLS;
-    const int32_t flags = nik[3];
-    const T dbase = nf[k * JXHIP_NODE_FCOLS + 2];
-    if (!(flags & kComp) || s >= pg.nsim) {
-      ds[i] = dbase;
-      xs[i] = T(0);
-      continue;
-    }
-    const T inv_cm = nf[k * JXHIP_NODE_FCOLS + 3];
-    const T v = xs[i];
-    T lin = T(0), cst = T(0);
-    if (flags & kHH) {
-      T am, bm, ah, bh, an, bn;
-      hh_rates(v, am, bm, ah, bh, an, bn);
-      const T mm = gate_step(G.m_in[i], dt, am, bm);
-      const T hh = gate_step(G.h_in[i], dt, ah, bh);
-      const T nn = gate_step(G.n_in[i], dt, an, bn);
-      G.m_out[i] = mm;
-      G.h_out[i] = hh;
-      G.n_out[i] = nn;
-      const T gNa = param_at<T, S>(a, 0, pg, tp, k, s) * (mm * mm * mm) * hh;
-      const T nn2 = nn * nn;
-      const T gK = param_at<T, S>(a, 1, pg, tp, k, s) * (nn2 * nn2);
-      const T gL = param_at<T, S>(a, 2, pg, tp, k, s);
-      // I = gNa (v - eNa) + gK (v - eK) + gL (v - eL) is linear in v: its slope is the
-      // total conductance and the constant term is gNa eNa + gK eK + gL eL (x1000:
-      // mA/cm^2 -> uA/cm^2, modules/base.py:3500-3502).
-      lin += T(1000) * (gNa + gK + gL);
-      cst += T(1000) * (gNa * param_at<T, S>(a, 3, pg, tp, k, s) +
-                        gK * param_at<T, S>(a, 4, pg, tp, k, s) +
-                        gL * param_at<T, S>(a, 5, pg, tp, k, s));
-    } else if (G.m_out != G.m_in) {
-      G.m_out[i] = G.m_in[i];
-      G.h_out[i] = G.h_in[i];
-      G.n_out[i] = G.n_in[i];
-    }
-    if (flags & kLeak) {
-      const T gl = param_at<T, S>(a, 6, pg, tp, k, s);
-      lin += T(1000) * gl;
-      cst += T(1000) * gl * param_at<T, S>(a, 7, pg, tp, k, s);
-    }
-    const int tg = nik[6];
-    if (tg >= 0) {
-      lin += L.tl[tg * S + s];
-      cst += L.tc[tg * S + s];
-    }
-    T rhs = v + dt * cst * inv_cm;
-    for (int e = nik[4]; e < nik[5]; ++e) {
-      const int64_t col = a.stim_col[e];
-      const int64_t bidx = a.stim_b > 1 ? (pg.simoff + s) : 0;
-      rhs += a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e];
-    }
-    ds[i] = dbase + dt * lin * inv_cm;
-    xs[i] = rhs;
+  if (!(flags & kComp)) {
+    Ld.ds[i] = nfk[kNfDbase];
+    Ld.xs[i] = ax;
+    return;
+  }
+  const G dt = a.dt;
+  const G inv_cm = static_cast<G>(nfk[kNfInvCm]);
+  const G vv = static_cast<G>(v);
+  G g = G(0);    // total conductance (mS/cm^2 -> x1000 below)
+  G cur = G(0);  // ionic current sum g_i (v - E_i), uA/cm^2 / 1000
+  if (flags & kHH) {
+    G am, bm, ah, bh, an, bn;
+    hh_rates(vv, am, bm, ah, bh, an, bn);
+    m = gate_step(m, dt, am, bm);
+    h = gate_step(h, dt, ah, bh);
+    n = gate_step(n, dt, an, bn);
+    const G gNa = param_at(a, 0, pg, tp, k, s) * (m * m * m) * h;
+    const G n2 = n * n;
+    const G gK = param_at(a, 1, pg, tp, k, s) * (n2 * n2);
+    const G gL = param_at(a, 2, pg, tp, k, s);
+    g += gNa + gK + gL;
+    cur += gNa * static_cast<G>(v - static_cast<double>(param_at(a, 3, pg, tp, k, s))) +
+           gK * static_cast<G>(v - static_cast<double>(param_at(a, 4, pg, tp, k, s))) +
+           gL * static_cast<G>(v - static_cast<double>(param_at(a, 5, pg, tp, k, s)));
+  }
+  if (flags & kLeak) {
+    const G gl = param_at(a, 6, pg, tp, k, s);
+    g += gl;
+    cur += gl * static_cast<G>(v - static_cast<double>(param_at(a, 7, pg, tp, k, s)));
+  }
+  G lin = G(1000) * g;
+  G net = -G(1000) * cur;  // (cst - lin v) of the channels
+  const int tg = nik[kNiTgt];
+  if (tg >= 0) {
+    lin += static_cast<G>(Ld.tl[(tg << lS) + s]);
+    net -= static_cast<G>(Ld.tc[(tg << lS) + s]);
+  }
+  L r = ax + static_cast<L>(dt * inv_cm * net);
+  for (int e = nik[kNiStimBeg]; e < nik[kNiStimEnd]; ++e) {
+    const int64_t col = a.stim_col[e];
+    const int64_t bidx = a.stim_b > 1 ? (pg.simoff + s) : 0;
+    r += static_cast<L>(a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e]);
+  }
+  Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * inv_cm * lin);
+  Ld.xs[i] = r;
+}
+
+// Items of simulations beyond `nsim` (padding): a well-conditioned row, zero rhs.
+template <typename G, typename L>
+__device__ __forceinline__ void idle_item(const Fused<G, L>& a, const Topo& tp, const Lds<L>& Ld,
+                                          int k, int i) {
+  const int32_t flags = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiFlags];
+  Ld.ds[i] = a.node_f[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS + kNfDbase] +
+             ((flags & kComp) ? L(1) : L(0));
+  Ld.xs[i] = L(0);
+}
+
+template <typename G, typename L>
+__device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
+                                              const Lds<L>& Ld) {
+  if (a.syn_model < 0) return;
+  const int lS = a.log2S, S = a.S;
+  for (int i = threadIdx.x; i < (tp.n_tgt << lS); i += blockDim.x) {
+    const int tgt = i >> lS, s = i & (S - 1);
+    G lin = G(0), cur = G(0);
+    if (s < pg.nsim) synapse_target(a, pg, tp, tgt, s, lin, cur);
+    Ld.tl[i] = static_cast<L>(lin);
+    Ld.tc[i] = static_cast<L>(cur);
   }
   __syncthreads();
-  tree_solve<T, S, false>(a, tp, xs, ds);
 }
 
-template <typename T, int S>
-__global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<T> a) {
+// Forward: items of a thread are i = tid + j * blockDim (j < IPT); their voltages and
+// gates stay in registers for the whole launch.
+template <typename G, typename L, int IPT>
+__global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
-  const int nS = tp.n * S;
-  Lds<T, S> L;
-  L.xs = reinterpret_cast<T*>(smem_raw);
-  L.ds = L.xs + nS;
-  L.tl = L.ds + nS;
-  L.tc = L.tl + tp.n_tgt * S;
+  const int lS = a.log2S, S = a.S;
+  const int nS = tp.n << lS;
+  Lds<L> Ld;
+  Ld.xs = reinterpret_cast<L*>(smem_raw);
+  Ld.ds = Ld.xs + nS;
+  Ld.tl = Ld.ds + nS;
+  Ld.tc = Ld.tl + (tp.n_tgt << lS);
   const int tid = threadIdx.x, nthr = blockDim.x;
-  T* xs = L.xs;
-  for (int i = tid; i < nS; i += nthr) xs[i] = a.v[pg.soff + i];
-  __syncthreads();
-  for (int i = tid; i < tp.n_rec * S; i += nthr) {
-    const int r = i / S, s = i % S;
-    a.rec[pg.roff + i] = xs[a.rec_node[tp.rb + r] * S + s];
+  double* vg = a.v + pg.soff;
+  double vr[IPT];
+  G mr[IPT], hr[IPT], nr[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * nthr;
+    if (i < nS) {
+      vr[j] = vg[i];
+      mr[j] = a.gm[pg.soff + i];
+      hr[j] = a.gh[pg.soff + i];
+      nr[j] = a.gn[pg.soff + i];
+    }
   }
-  Gates<T> G;
-  G.m_in = G.m_out = a.gm + pg.soff;
-  G.h_in = G.h_out = a.gh + pg.soff;
-  G.n_in = G.n_out = a.gn + pg.soff;
+  for (int i = tid; i < (tp.n_rec << lS); i += nthr) {
+    const int r = i >> lS, s = i & (S - 1);
+    a.rec[pg.roff + i] = static_cast<G>(vg[(a.rec_node[tp.rb + r] << lS) + s]);
+  }
   const int64_t nsteps = pg.t1 - pg.t0;
   for (int64_t step = 0; step < nsteps; ++step) {
     if (a.ckpt_v != nullptr && step % a.ckpt_every == 0) {
       const int64_t off = pg.soff + (step / a.ckpt_every) * a.ckpt_stride;
-      for (int i = tid; i < nS; i += nthr) {
-        a.ckpt_v[off + i] = xs[i];
-        a.ckpt_m[off + i] = G.m_in[i];
-        a.ckpt_h[off + i] = G.h_in[i];
-        a.ckpt_n[off + i] = G.n_in[i];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const int i = tid + j * nthr;
+        if (i < nS) {
+          a.ckpt_v[off + i] = vr[j];
+          a.ckpt_m[off + i] = mr[j];
+          a.ckpt_h[off + i] = hr[j];
+          a.ckpt_n[off + i] = nr[j];
+        }
       }
       if (a.syn_model == 0 && a.ckpt_s != nullptr) {
-        const int64_t nsyn = static_cast<int64_t>(tp.n_syn) * S;
+        const int64_t nsyn = static_cast<int64_t>(tp.n_syn) << lS;
         const int64_t so = pg.synoff + (step / a.ckpt_every) * a.ckpt_syn_stride;
         for (int64_t i = tid; i < nsyn; i += nthr) a.ckpt_s[so + i] = a.syn_s[pg.synoff + i];
       }
     }
-    step_body<T, S>(a, pg, tp, L, pg.t0 + step, G);
-    for (int i = tid; i < tp.n_rec * S; i += nthr) {
-      const int r = i / S, s = i % S;
-      a.rec[pg.roff + ((step + 1) * tp.n_rec + r) * S + s] = xs[a.rec_node[tp.rb + r] * S + s];
+    synapse_phase(a, pg, tp, Ld);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * nthr;
+      if (i < nS) {
+        const int k = i >> lS, s = i & (S - 1);
+        if (s < pg.nsim) {
+          build_item(a, pg, tp, Ld, vg, k, s, vr[j], pg.t0 + step, mr[j], hr[j], nr[j]);
+        } else {
+          idle_item(a, tp, Ld, k, i);
+        }
+      }
     }
     __syncthreads();
+    tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * nthr;
+      if (i < nS) {
+        vr[j] += static_cast<double>(Ld.xs[i]);
+        vg[i] = vr[j];
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < (tp.n_rec << lS); i += nthr) {
+      const int r = i >> lS, s = i & (S - 1);
+      a.rec[pg.roff + ((step + 1) * tp.n_rec + r) * S + s] =
+          static_cast<G>(vg[(a.rec_node[tp.rb + r] << lS) + s]);
+    }
   }
-  for (int i = tid; i < nS; i += nthr) a.v[pg.soff + i] = xs[i];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * nthr;
+    if (i < nS) {
+      a.gm[pg.soff + i] = mr[j];
+      a.gh[pg.soff + i] = hr[j];
+      a.gn[pg.soff + i] = nr[j];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------
 // Adjoint
 // ---------------------------------------------------------------------------------
 
-template <typename T>
+template <typename G>
 struct Adj {
-  const T* g_rec;
-  T* g_param[JXHIP_NPARAM];
-  T* g_v0;
-  T* wv;
-  T* wm;
-  T* wh;
-  T* wn;
-  T* lm;
-  T* lh;
-  T* ln;
+  const G* g_rec;
+  G* g_param[JXHIP_NPARAM];
+  G* g_v0;
+  double* wv;
+  G* wm;
+  G* wh;
+  G* wn;
+  G* lm;
+  G* lh;
+  G* ln;
   int64_t work_stride;
 };
 
@@ -438,242 +523,249 @@ __device__ __forceinline__ T d_x_over_expm1(T x, T y) {
   return T(1) / em1 - x * (em1 + T(1)) / (em1 * em1) * pass / y;
 }
 
-// Derivative of save_exp(u) = exp(min(u, 20)) with respect to u.
 template <typename T>
 __device__ __forceinline__ T d_save_exp(T u) {
   return u <= T(20) ? dexp(u) : T(0);
 }
 
-// dx'/dv of the exponential-Euler gate step through alpha(v), beta(v).
+// dx'/dv of the exponential-Euler gate step through alpha(v), beta(v); e = dx'/dx.
 template <typename T>
 __device__ __forceinline__ T d_gate_dv(T x, T dt, T al, T be, T dal, T dbe, T& e) {
   const T tau = T(1) / (al + be);
   const T xinf = al * tau;
   e = save_exp(-dt / tau);
-  const T de = -dt * e;  // d e / d(alpha + beta)
+  const T de = -dt * e;
   const T dxa = be * tau * tau * (T(1) - e) + (x - xinf) * de;
   const T dxb = -al * tau * tau * (T(1) - e) + (x - xinf) * de;
   return dxa * dal + dxb * dbe;
 }
 
-template <typename T, int S>
-__global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<T> a, Adj<T> g) {
+template <typename G, typename L>
+__global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
-  const int nS = tp.n * S;
-  Lds<T, S> L;
-  L.xs = reinterpret_cast<T*>(smem_raw);
-  L.ds = L.xs + nS;
-  L.tl = L.ds + nS;
-  L.tc = L.tl;
-  T* lv = L.ds + nS;  // adjoint of the voltages (no synapses in the adjoint)
-  T* xs = L.xs;
-  T* ds = L.ds;
+  const int lS = a.log2S, S = a.S;
+  const int nS = tp.n << lS;
+  Lds<L> Ld;
+  Ld.xs = reinterpret_cast<L*>(smem_raw);
+  Ld.ds = Ld.xs + nS;
+  L* lv = Ld.ds + nS;  // adjoint of the voltages
+  Ld.tl = lv;          // unused (no synapses in the adjoint)
+  Ld.tc = lv;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const T dt = a.dt;
-  const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
-  const T* nf = a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS;
+  const G dt = a.dt;
   const int64_t nsteps = pg.t1 - pg.t0;
   const int64_t K = a.ckpt_every;
   const int64_t C = (nsteps + K - 1) / K;
   const int64_t wbase = static_cast<int64_t>(blockIdx.x) * g.work_stride;
-  T* lm = g.lm + pg.soff;
-  T* lh = g.lh + pg.soff;
-  T* ln = g.ln + pg.soff;
+  G* lm = g.lm + pg.soff;
+  G* lh = g.lh + pg.soff;
+  G* ln = g.ln + pg.soff;
   for (int i = tid; i < nS; i += nthr) {
-    lv[i] = T(0);
-    lm[i] = T(0);
-    lh[i] = T(0);
-    ln[i] = T(0);
+    lv[i] = L(0);
+    lm[i] = G(0);
+    lh[i] = G(0);
+    ln[i] = G(0);
   }
   __syncthreads();
+  double* wv = g.wv + wbase;
+  G* wm = g.wm + wbase;
+  G* wh = g.wh + wbase;
+  G* wn = g.wn + wbase;
   for (int64_t c = C - 1; c >= 0; --c) {
     const int64_t ts = c * K;
     const int64_t te = ts + K < nsteps ? ts + K : nsteps;
-    // Recompute the segment from its checkpoint; trajectory into work[0 .. te - ts].
+    // Recompute the segment from its checkpoint into work[0 .. te - ts]; the voltage
+    // copy of step j is work_v[j] (program layout, as the forward's HBM copy).
     const int64_t coff = pg.soff + c * a.ckpt_stride;
-    T* wv = g.wv + wbase;
-    T* wm = g.wm + wbase;
-    T* wh = g.wh + wbase;
-    T* wn = g.wn + wbase;
     for (int i = tid; i < nS; i += nthr) {
-      xs[i] = a.ckpt_v[coff + i];
-      wv[i] = xs[i];
+      wv[i] = a.ckpt_v[coff + i];
       wm[i] = a.ckpt_m[coff + i];
       wh[i] = a.ckpt_h[coff + i];
       wn[i] = a.ckpt_n[coff + i];
     }
     __syncthreads();
     for (int64_t j = 0; j < te - ts; ++j) {
-      Gates<T> G;
-      G.m_in = wm + j * nS;
-      G.h_in = wh + j * nS;
-      G.n_in = wn + j * nS;
-      G.m_out = wm + (j + 1) * nS;
-      G.h_out = wh + (j + 1) * nS;
-      G.n_out = wn + (j + 1) * nS;
-      step_body<T, S>(a, pg, tp, L, pg.t0 + ts + j, G);
-      for (int i = tid; i < nS; i += nthr) wv[(j + 1) * nS + i] = xs[i];
+      for (int i = tid; i < nS; i += nthr) {
+        const int k = i >> lS, s = i & (S - 1);
+        G m = wm[j * nS + i], h = wh[j * nS + i], n = wn[j * nS + i];
+        if (s < pg.nsim) {
+          build_item(a, pg, tp, Ld, wv + j * nS, k, s, wv[j * nS + i], pg.t0 + ts + j, m, h, n);
+        } else {
+          idle_item(a, tp, Ld, k, i);
+        }
+        wm[(j + 1) * nS + i] = m;
+        wh[(j + 1) * nS + i] = h;
+        wn[(j + 1) * nS + i] = n;
+      }
+      __syncthreads();
+      tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds);
+      for (int i = tid; i < nS; i += nthr) wv[(j + 1) * nS + i] = wv[j * nS + i] + static_cast<double>(Ld.xs[i]);
       __syncthreads();
     }
-    // Adjoint of the steps of the segment, last first.
     for (int64_t j = te - ts - 1; j >= 0; --j) {
-      const int64_t t1 = ts + j + 1;  // local time of v(t+1) (recording index)
-      // Gradient of the recording of v(t+1); one thread per simulation adds all
-      // recordings (a node may be recorded more than once).
+      const int64_t t1 = ts + j + 1;
       for (int s = tid; s < S; s += nthr) {
         for (int r = 0; r < tp.n_rec; ++r) {
           const int node = a.rec_node[tp.rb + r];
-          lv[node * S + s] += g.g_rec[pg.roff + (t1 * tp.n_rec + r) * S + s];
+          lv[(node << lS) + s] += static_cast<L>(g.g_rec[pg.roff + (t1 * tp.n_rec + r) * S + s]);
         }
       }
       __syncthreads();
-      // Diagonal of step t (gates at t+1) and the right-hand side lambda_v(t+1).
-      const T* m1 = wm + (j + 1) * nS;
-      const T* h1 = wh + (j + 1) * nS;
-      const T* n1 = wn + (j + 1) * nS;
+      const G* m1 = wm + (j + 1) * nS;
+      const G* h1 = wh + (j + 1) * nS;
+      const G* n1 = wn + (j + 1) * nS;
       for (int i = tid; i < nS; i += nthr) {
-        const int k = i / S, s = i % S;
-        const int32_t flags = ni[k * JXHIP_NODE_ICOLS + 3];
-        const T dbase = nf[k * JXHIP_NODE_FCOLS + 2];
-        T lin = T(0);
+        const int k = i >> lS, s = i & (S - 1);
+        const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
+        const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
+        const int32_t flags = nik[kNiFlags];
+        G lin = G(0);
         if ((flags & kComp) && s < pg.nsim) {
           if (flags & kHH) {
-            const T mm = m1[i], hh = h1[i], nn = n1[i];
-            const T nn2 = nn * nn;
-            lin += T(1000) * (param_at<T, S>(a, 0, pg, tp, k, s) * (mm * mm * mm) * hh +
-                              param_at<T, S>(a, 1, pg, tp, k, s) * (nn2 * nn2) +
-                              param_at<T, S>(a, 2, pg, tp, k, s));
+            const G mm = m1[i], hh = h1[i], nn = n1[i];
+            const G n2 = nn * nn;
+            lin += G(1000) * (param_at(a, 0, pg, tp, k, s) * (mm * mm * mm) * hh +
+                              param_at(a, 1, pg, tp, k, s) * (n2 * n2) + param_at(a, 2, pg, tp, k, s));
           }
-          if (flags & kLeak) lin += T(1000) * param_at<T, S>(a, 6, pg, tp, k, s);
-          ds[i] = dbase + dt * lin * nf[k * JXHIP_NODE_FCOLS + 3];
+          if (flags & kLeak) lin += G(1000) * param_at(a, 6, pg, tp, k, s);
+          Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * static_cast<G>(nfk[kNfInvCm]) * lin);
         } else {
-          ds[i] = dbase;
+          Ld.ds[i] = nfk[kNfDbase] + ((flags & kComp) ? L(1) : L(0));
         }
-        xs[i] = lv[i];
+        Ld.xs[i] = lv[i];
       }
       __syncthreads();
-      tree_solve<T, S, true>(a, tp, xs, ds);  // xs = mu = A^-T lambda_v(t+1)
-      // Adjoint of the system build, the currents and the gates.
-      const T* v0 = wv + j * nS;
-      const T* v1 = wv + (j + 1) * nS;
-      const T* m0 = wm + j * nS;
-      const T* h0 = wh + j * nS;
-      const T* n0 = wn + j * nS;
+      tree_solve<G, L, true>(a, tp, Ld.xs, Ld.ds);  // xs = mu = A^-T lambda_v(t+1)
+      const double* v0 = wv + j * nS;
+      const double* v1 = wv + (j + 1) * nS;
+      const G* m0 = wm + j * nS;
+      const G* h0 = wh + j * nS;
+      const G* n0 = wn + j * nS;
       for (int i = tid; i < nS; i += nthr) {
-        const int k = i / S, s = i % S;
-        const int32_t flags = ni[k * JXHIP_NODE_ICOLS + 3];
+        const int k = i >> lS, s = i & (S - 1);
+        const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
+        const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
+        const int32_t flags = nik[kNiFlags];
         if (!(flags & kComp) || s >= pg.nsim) {
-          lv[i] = T(0);
+          lv[i] = L(0);
           continue;
         }
-        const T inv_cm = nf[k * JXHIP_NODE_FCOLS + 3];
-        const T mu = xs[i];
-        const T gl = -mu * v1[i] * dt * inv_cm;  // dL/dlin
-        const T gc = mu * dt * inv_cm;           // dL/dcst
-        T lvn = mu;                              // rhs = v(t) + ...
+        const G inv_cm = static_cast<G>(nfk[kNfInvCm]);
+        const G mu = static_cast<G>(Ld.xs[i]);
+        const G gl = -mu * static_cast<G>(v1[i]) * dt * inv_cm;  // dL/dlin
+        const G gc = mu * dt * inv_cm;                           // dL/dcst
+        G lvn = mu;
         if (flags & kHH) {
-          const T gNa = param_at<T, S>(a, 0, pg, tp, k, s);
-          const T gK = param_at<T, S>(a, 1, pg, tp, k, s);
-          const T eNa = param_at<T, S>(a, 3, pg, tp, k, s);
-          const T eK = param_at<T, S>(a, 4, pg, tp, k, s);
-          const T eL = param_at<T, S>(a, 5, pg, tp, k, s);
-          const T mm = m1[i], hh = h1[i], nn = n1[i];
-          const T m3 = mm * mm * mm, n3 = nn * nn * nn, n4 = n3 * nn;
-          const T wNa = T(1000) * (gl + gc * eNa);
-          const T wK = T(1000) * (gl + gc * eK);
-          const T wL = T(1000) * (gl + gc * eL);
+          const G gNa = param_at(a, 0, pg, tp, k, s);
+          const G gK = param_at(a, 1, pg, tp, k, s);
+          const G eNa = param_at(a, 3, pg, tp, k, s);
+          const G eK = param_at(a, 4, pg, tp, k, s);
+          const G eL = param_at(a, 5, pg, tp, k, s);
+          const G mm = m1[i], hh = h1[i], nn = n1[i];
+          const G m3 = mm * mm * mm, n3 = nn * nn * nn, n4 = n3 * nn;
+          const G wNa = G(1000) * (gl + gc * eNa);
+          const G wK = G(1000) * (gl + gc * eK);
+          const G wL = G(1000) * (gl + gc * eL);
           const int64_t pi = pg.soff + i;
           if (g.g_param[0]) g.g_param[0][pi] += wNa * m3 * hh;
           if (g.g_param[1]) g.g_param[1][pi] += wK * n4;
           if (g.g_param[2]) g.g_param[2][pi] += wL;
-          if (g.g_param[3]) g.g_param[3][pi] += T(1000) * gc * gNa * m3 * hh;
-          if (g.g_param[4]) g.g_param[4][pi] += T(1000) * gc * gK * n4;
-          if (g.g_param[5]) g.g_param[5][pi] += T(1000) * gc * param_at<T, S>(a, 2, pg, tp, k, s);
-          const T lam_m1 = lm[i] + wNa * gNa * T(3) * mm * mm * hh;
-          const T lam_h1 = lh[i] + wNa * gNa * m3;
-          const T lam_n1 = ln[i] + wK * gK * T(4) * n3;
-          // Gate step at v(t) (channels/hh.py:87-108 and their derivatives).
-          const T v = v0[i];
-          const T xm = -(v + T(40)), xn = -(v + T(55));
-          const T um = -(v + T(65)) / T(18), uh = -(v + T(65)) / T(20);
-          const T ub = -(v + T(35)) / T(10), un = -(v + T(65)) / T(80);
-          const T am = T(0.1) * x_over_expm1(xm, T(10));
-          const T dam = -T(0.1) * d_x_over_expm1(xm, T(10));
-          const T bm = T(4) * save_exp(um);
-          const T dbm = -T(4) / T(18) * d_save_exp(um);
-          const T ah = T(0.07) * save_exp(uh);
-          const T dah = -T(0.07) / T(20) * d_save_exp(uh);
-          const T Eb = save_exp(ub);
-          const T bh = T(1) / (Eb + T(1));
-          const T dbh = d_save_exp(ub) / (T(10) * (Eb + T(1)) * (Eb + T(1)));
-          const T an = T(0.01) * x_over_expm1(xn, T(10));
-          const T dan = -T(0.01) * d_x_over_expm1(xn, T(10));
-          const T bn = T(0.125) * save_exp(un);
-          const T dbn = -T(0.125) / T(80) * d_save_exp(un);
-          T em, eh, en;
-          const T dmv = d_gate_dv(m0[i], dt, am, bm, dam, dbm, em);
-          const T dhv = d_gate_dv(h0[i], dt, ah, bh, dah, dbh, eh);
-          const T dnv = d_gate_dv(n0[i], dt, an, bn, dan, dbn, en);
+          if (g.g_param[3]) g.g_param[3][pi] += G(1000) * gc * gNa * m3 * hh;
+          if (g.g_param[4]) g.g_param[4][pi] += G(1000) * gc * gK * n4;
+          if (g.g_param[5]) g.g_param[5][pi] += G(1000) * gc * param_at(a, 2, pg, tp, k, s);
+          const G lam_m1 = lm[i] + wNa * gNa * G(3) * mm * mm * hh;
+          const G lam_h1 = lh[i] + wNa * gNa * m3;
+          const G lam_n1 = ln[i] + wK * gK * G(4) * n3;
+          const G v = static_cast<G>(v0[i]);
+          const G xm = -(v + G(40)), xn = -(v + G(55));
+          const G um = -(v + G(65)) / G(18), uh = -(v + G(65)) / G(20);
+          const G ub = -(v + G(35)) / G(10), un = -(v + G(65)) / G(80);
+          const G am = G(0.1) * x_over_expm1(xm, G(10));
+          const G dam = -G(0.1) * d_x_over_expm1(xm, G(10));
+          const G bm = G(4) * save_exp(um);
+          const G dbm = -G(4) / G(18) * d_save_exp(um);
+          const G ah = G(0.07) * save_exp(uh);
+          const G dah = -G(0.07) / G(20) * d_save_exp(uh);
+          const G Eb = save_exp(ub);
+          const G bh = G(1) / (Eb + G(1));
+          const G dbh = d_save_exp(ub) / (G(10) * (Eb + G(1)) * (Eb + G(1)));
+          const G an = G(0.01) * x_over_expm1(xn, G(10));
+          const G dan = -G(0.01) * d_x_over_expm1(xn, G(10));
+          const G bn = G(0.125) * save_exp(un);
+          const G dbn = -G(0.125) / G(80) * d_save_exp(un);
+          G em, eh, en;
+          const G dmv = d_gate_dv(m0[i], dt, am, bm, dam, dbm, em);
+          const G dhv = d_gate_dv(h0[i], dt, ah, bh, dah, dbh, eh);
+          const G dnv = d_gate_dv(n0[i], dt, an, bn, dan, dbn, en);
           lvn += lam_m1 * dmv + lam_h1 * dhv + lam_n1 * dnv;
           lm[i] = lam_m1 * em;
           lh[i] = lam_h1 * eh;
           ln[i] = lam_n1 * en;
         }
         if (flags & kLeak) {
-          const T eL = param_at<T, S>(a, 7, pg, tp, k, s);
+          const G eLk = param_at(a, 7, pg, tp, k, s);
           const int64_t pi = pg.soff + i;
-          if (g.g_param[6]) g.g_param[6][pi] += T(1000) * (gl + gc * eL);
-          if (g.g_param[7]) g.g_param[7][pi] += T(1000) * gc * param_at<T, S>(a, 6, pg, tp, k, s);
+          if (g.g_param[6]) g.g_param[6][pi] += G(1000) * (gl + gc * eLk);
+          if (g.g_param[7]) g.g_param[7][pi] += G(1000) * gc * param_at(a, 6, pg, tp, k, s);
         }
-        lv[i] = lvn;
+        lv[i] = static_cast<L>(lvn);
       }
       __syncthreads();
     }
   }
-  // The initial recording (t = 0) and the gradient of the initial voltages.
   for (int s = tid; s < S; s += nthr) {
     for (int r = 0; r < tp.n_rec; ++r) {
       const int node = a.rec_node[tp.rb + r];
-      lv[node * S + s] += g.g_rec[pg.roff + r * S + s];
+      lv[(node << lS) + s] += static_cast<L>(g.g_rec[pg.roff + r * S + s]);
     }
   }
   __syncthreads();
-  for (int i = tid; i < nS; i += nthr) g.g_v0[pg.soff + i] = lv[i];
+  for (int i = tid; i < nS; i += nthr) g.g_v0[pg.soff + i] = static_cast<G>(lv[i]);
 }
 
-template <typename T>
+template <typename K>
+hipError_t set_lds(K kern, size_t lds) {
+  if (lds <= 64 * 1024) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+}
+
+inline int check_args(const jxhip_fused_args& a) {
+  if (a.n_progs > 0x7fffffffLL || a.threads < 64 || a.threads > 1024 || a.threads % 64)
+    return hipErrorInvalidValue;
+  if (a.S < 1 || a.S > 64 || (a.S & (a.S - 1))) return hipErrorInvalidValue;
+  if (a.lds_bytes < 0 || a.lds_bytes > 160 * 1024) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+template <typename G, typename L>
 int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
   if (args == nullptr) return hipErrorInvalidValue;
   const jxhip_fused_args& a = *args;
   if (a.n_progs <= 0) return hipSuccess;
-  if (a.n_progs > 0x7fffffffLL || a.threads < 64 || a.threads > 1024 || a.threads % 64)
-    return hipErrorInvalidValue;
-  if (a.lds_bytes < 0 || a.lds_bytes > 160 * 1024) return hipErrorInvalidValue;
-  const Fused<T> f = unpack<T>(a);
+  if (int e = check_args(a)) return e;
+  const Fused<G, L> f = unpack<G, L>(a);
   const dim3 grid(static_cast<unsigned>(a.n_progs)), block(a.threads);
   const size_t lds = static_cast<size_t>(a.lds_bytes);
-#define JXHIP_FWD_CASE(SV)                                                             \
-  case SV: {                                                                           \
-    auto kern = fused_forward_kernel<T, SV>;                                           \
-    if (lds > 64 * 1024) {                                                             \
-      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),          \
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                         static_cast<int>(lds));                       \
-      if (e != hipSuccess) return e;                                                   \
-    }                                                                                  \
-    hipLaunchKernelGGL(kern, grid, block, lds, stream, f);                             \
-    break;                                                                             \
+#define JXHIP_FWD_CASE(IPT)                                         \
+  case IPT: {                                                       \
+    auto kern = fused_forward_kernel<G, L, IPT>;                    \
+    if (hipError_t e = set_lds(kern, lds)) return e;                \
+    hipLaunchKernelGGL(kern, grid, block, lds, stream, f);          \
+    break;                                                          \
   }
-  switch (a.S) {
+  switch (a.items_per_thread) {
     JXHIP_FWD_CASE(1)
     JXHIP_FWD_CASE(2)
     JXHIP_FWD_CASE(4)
     JXHIP_FWD_CASE(8)
+    JXHIP_FWD_CASE(12)
     JXHIP_FWD_CASE(16)
+    JXHIP_FWD_CASE(20)
+    JXHIP_FWD_CASE(24)
     JXHIP_FWD_CASE(32)
-    JXHIP_FWD_CASE(64)
     default:
       return hipErrorInvalidValue;
   }
@@ -681,55 +773,32 @@ int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename G, typename L>
 int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
                    hipStream_t stream) {
   if (args == nullptr || adj == nullptr) return hipErrorInvalidValue;
   const jxhip_fused_args& a = *args;
   if (a.n_progs <= 0) return hipSuccess;
-  if (a.n_progs > 0x7fffffffLL || a.threads < 64 || a.threads > 1024 || a.threads % 64)
-    return hipErrorInvalidValue;
+  if (int e = check_args(a)) return e;
   if (a.syn_model >= 0 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
-  if (a.lds_bytes < 0 || a.lds_bytes > 160 * 1024) return hipErrorInvalidValue;
-  const Fused<T> f = unpack<T>(a);
-  Adj<T> g;
-  g.g_rec = static_cast<const T*>(adj->g_rec);
-  for (int j = 0; j < JXHIP_NPARAM; ++j) g.g_param[j] = static_cast<T*>(adj->g_param[j]);
-  g.g_v0 = static_cast<T*>(adj->g_v0);
-  g.wv = static_cast<T*>(adj->work_v);
-  g.wm = static_cast<T*>(adj->work_m);
-  g.wh = static_cast<T*>(adj->work_h);
-  g.wn = static_cast<T*>(adj->work_n);
-  g.lm = static_cast<T*>(adj->lam_m);
-  g.lh = static_cast<T*>(adj->lam_h);
-  g.ln = static_cast<T*>(adj->lam_n);
+  const Fused<G, L> f = unpack<G, L>(a);
+  Adj<G> g;
+  g.g_rec = static_cast<const G*>(adj->g_rec);
+  for (int j = 0; j < JXHIP_NPARAM; ++j) g.g_param[j] = static_cast<G*>(adj->g_param[j]);
+  g.g_v0 = static_cast<G*>(adj->g_v0);
+  g.wv = static_cast<double*>(adj->work_v);
+  g.wm = static_cast<G*>(adj->work_m);
+  g.wh = static_cast<G*>(adj->work_h);
+  g.wn = static_cast<G*>(adj->work_n);
+  g.lm = static_cast<G*>(adj->lam_m);
+  g.lh = static_cast<G*>(adj->lam_h);
+  g.ln = static_cast<G*>(adj->lam_n);
   g.work_stride = adj->work_stride;
   const dim3 grid(static_cast<unsigned>(a.n_progs)), block(a.threads);
   const size_t lds = static_cast<size_t>(a.lds_bytes);
-#define JXHIP_ADJ_CASE(SV)                                                             \
-  case SV: {                                                                           \
-    auto kern = fused_adjoint_kernel<T, SV>;                                           \
-    if (lds > 64 * 1024) {                                                             \
-      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),          \
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                         static_cast<int>(lds));                       \
-      if (e != hipSuccess) return e;                                                   \
-    }                                                                                  \
-    hipLaunchKernelGGL(kern, grid, block, lds, stream, f, g);                          \
-    break;                                                                             \
-  }
-  switch (a.S) {
-    JXHIP_ADJ_CASE(1)
-    JXHIP_ADJ_CASE(2)
-    JXHIP_ADJ_CASE(4)
-    JXHIP_ADJ_CASE(8)
-    JXHIP_ADJ_CASE(16)
-    JXHIP_ADJ_CASE(32)
-    JXHIP_ADJ_CASE(64)
-    default:
-      return hipErrorInvalidValue;
-  }
-#undef JXHIP_ADJ_CASE
+  auto kern = fused_adjoint_kernel<G, L>;
+  if (hipError_t e = set_lds(kern, lds)) return e;
+  hipLaunchKernelGGL(kern, grid, block, lds, stream, f, g);
   return hipGetLastError();
 }
 
@@ -737,22 +806,22 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
 
 extern "C" {
 
+int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream) {
+  return jxhip::launch_forward<float, float>(args, static_cast<hipStream_t>(stream));
+}
+
+int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream) {
+  return jxhip::launch_forward<double, double>(args, static_cast<hipStream_t>(stream));
+}
+
 int jxhip_fused_adjoint_f32(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
                             void* stream) {
-  return jxhip::launch_adjoint<float>(args, adj, static_cast<hipStream_t>(stream));
+  return jxhip::launch_adjoint<float, float>(args, adj, static_cast<hipStream_t>(stream));
 }
 
 int jxhip_fused_adjoint_f64(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
                             void* stream) {
-  return jxhip::launch_adjoint<double>(args, adj, static_cast<hipStream_t>(stream));
-}
-
-int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream) {
-  return jxhip::launch_forward<float>(args, static_cast<hipStream_t>(stream));
-}
-
-int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream) {
-  return jxhip::launch_forward<double>(args, static_cast<hipStream_t>(stream));
+  return jxhip::launch_adjoint<double, double>(args, adj, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

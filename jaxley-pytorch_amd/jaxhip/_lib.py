@@ -30,7 +30,8 @@ class FusedArgs(C.Structure):
     """`jxhip_fused_args` (include/jxhip.h)."""
 
     _fields_ = [
-        ("progs", _p), ("n_progs", _i64), ("topos", _p), ("order", _p), ("level_ptr", _p),
+        ("progs", _p), ("n_progs", _i64), ("topos", _p), ("seg", _p), ("seg_level_ptr", _p),
+        ("seg_child", _p),
         ("node_i", _p), ("node_f", _p), ("child_idx", _p), ("stim_col", _p),
         ("stim_scale", _p), ("rec_node", _p),
         ("param", _p * JXHIP_NPARAM), ("param_mode", _i32 * JXHIP_NPARAM),
@@ -41,7 +42,7 @@ class FusedArgs(C.Structure):
         ("syn_param", _p * JXHIP_NSYNPARAM),
         ("syn_scale", _p), ("syn_s", _p),
         ("sims_total", _i64), ("dt", C.c_double), ("diff", C.c_double),
-        ("S", _i32), ("threads", _i32), ("lds_bytes", _i64),
+        ("S", _i32), ("threads", _i32), ("items_per_thread", _i32), ("lds_bytes", _i64),
         ("ckpt_v", _p), ("ckpt_m", _p), ("ckpt_h", _p), ("ckpt_n", _p), ("ckpt_s", _p),
         ("ckpt_every", _i64), ("ckpt_stride", _i64), ("ckpt_syn_stride", _i64),
     ]

@@ -40,13 +40,14 @@ class _FusedIntegrate(torch.autograd.Function):
         run: ForwardLaunch = holder["run"]
         st, K = run.st, run.ckpt_every
         P, S, n, T = run.P, run.S, st.n, run.T
-        dev, dtype = run.v.device, run.v.dtype
+        dev, dtype = run.v.device, run.dtype
         # (B, n_rec, T+1) -> program layout (P, T+1, n_rec, S)
         gr = g_rec.to(dtype)
         if P * S != run.B:
             gr = torch.cat([gr, gr.new_zeros(P * S - run.B, *gr.shape[1:])], 0)
         gr = gr.reshape(P, S, st.n_rec, T + 1).permute(0, 3, 2, 1).contiguous()
-        work = [torch.empty((P, K + 1, n, S), dtype=dtype, device=dev) for _ in range(4)]
+        work = [torch.empty((P, K + 1, n, S), dtype=torch.float64, device=dev)]
+        work += [torch.empty((P, K + 1, n, S), dtype=dtype, device=dev) for _ in range(3)]
         lam = [torch.zeros((P, n, S), dtype=dtype, device=dev) for _ in range(3)]
         g_v0 = torch.zeros((P, n, S), dtype=dtype, device=dev)
         gpar = {}
@@ -65,7 +66,7 @@ class _FusedIntegrate(torch.autograd.Function):
         adj.n_steps = T
         args = run.args
         saved_lds = args.lds_bytes
-        args.lds_bytes = 3 * n * S * run.v.element_size()
+        args.lds_bytes = 3 * n * S * run.node_f.element_size()
         f = _lib.fn("jxhip_fused_adjoint", dtype)
         try:
             _lib.check(f(args, adj, _lib.stream_ptr(dev)), "fused_adjoint")

@@ -74,7 +74,7 @@ def fused_config(st, B: int, dtype):
 
     n_tgt = st.syn["n_tgt"] if st.syn else 0
     esize = torch.tensor([], dtype=dtype).element_size()
-    key = ("fused", st.n, st.H, n_tgt, int(st.topo_row.sum()), B, str(dtype))
+    key = ("fused", st.n, st.L, n_tgt, int(st.topo_row.sum()), B, str(dtype))
     if key in _CACHE:
         return _CACHE[key]
     cfg = default_config(st.n, n_tgt, B, esize)
@@ -89,5 +89,5 @@ def fused_config(st, B: int, dtype):
 def set_fused_config(st, B: int, dtype, cfg) -> None:
     """Record a measured launch shape (used by the serving loop and the benchmark)."""
     n_tgt = st.syn["n_tgt"] if st.syn else 0
-    key = ("fused", st.n, st.H, n_tgt, int(st.topo_row.sum()), B, str(dtype))
+    key = ("fused", st.n, st.L, n_tgt, int(st.topo_row.sum()), B, str(dtype))
     _CACHE[key] = cfg

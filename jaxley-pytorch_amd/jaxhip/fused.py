@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .schedule import fused_topology
+from .schedule import check_forest, segments
 
 HH_PARAMS = ("HH_gNa", "HH_gK", "HH_gLeak", "HH_eNa", "HH_eK", "HH_eLeak")
 LEAK_PARAMS = ("Leak_gLeak", "Leak_eLeak")
@@ -46,16 +46,17 @@ def from_prog(x: torch.Tensor, B: int) -> torch.Tensor:
 
 @dataclass
 class Structure:
-    """Integer tables of one topology (cached per module structure and device)."""
+    """Integer tables of one topology in kernel order (cached per module structure)."""
 
     n: int
-    H: int
-    mp: np.ndarray            # solve index -> node
-    inv: np.ndarray           # node -> solve index
-    comp_of_solve: np.ndarray  # solve index -> compartment (-1 branchpoint)
+    L: int
+    mp: np.ndarray             # kernel index -> node
+    kidx_of_node: np.ndarray   # node -> kernel index
+    comp_of_k: np.ndarray      # kernel index -> compartment (-1 branchpoint)
     topo_row: np.ndarray
-    order: torch.Tensor
-    level_ptr: torch.Tensor
+    seg: torch.Tensor
+    seg_level_ptr: torch.Tensor
+    seg_child: torch.Tensor
     node_i: torch.Tensor
     child_idx: torch.Tensor
     stim_col: torch.Tensor
@@ -63,69 +64,89 @@ class Structure:
     syn: Optional[Dict]
     n_rec: int
     rec_comps: np.ndarray
-    stim_nodes_solve: np.ndarray
-    sinks_solve: np.ndarray
-    lo_edge: np.ndarray
+    stim_order: np.ndarray     # stimulus entry e -> stimulus column
+    sinks_k: np.ndarray        # sink of every compartment edge, kernel index
+    lo_edge: np.ndarray        # edge of the lower coefficient of every kernel node
     up_edge: np.ndarray
+    parent_k: np.ndarray
 
 
 def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, device) -> Structure:
     tables, indexer = module._ensure_solver()
     n = tables["n_nodes"]
-    mp = np.asarray(indexer["map_to_solve_order"], dtype=np.int64)
-    inv = np.asarray(indexer["inv_map_to_solve_order"], dtype=np.int64)
-    parent = np.asarray(indexer["parent_lookup"][:n], dtype=np.int64)
-    topo = fused_topology(parent)
+    mp_solve = np.asarray(indexer["map_to_solve_order"], dtype=np.int64)
+    inv_solve = np.asarray(indexer["inv_map_to_solve_order"], dtype=np.int64)
+    parent_solve = np.asarray(indexer["parent_lookup"][:n], dtype=np.int64)
+    if check_forest(parent_solve) is None:
+        raise ValueError("the solve order must be a forest with parents before children")
+    sg = segments(parent_solve)
+    korder = sg["korder"]                           # kernel -> solve
+    kidx_solve = sg["kidx"]                         # solve -> kernel
+    mp = mp_solve[korder]                           # kernel -> node
+    kidx_of_node = kidx_solve[inv_solve]            # node -> kernel
+    parent_k = np.where(parent_solve[korder] >= 0, kidx_solve[np.maximum(parent_solve[korder], 0)], -1)
     comp_of_node = -np.ones(n, dtype=np.int64)
     comp_of_node[module._comp_node] = np.arange(module._n_comp)
-    comp_of_solve = comp_of_node[mp]
-    flags = np.where(comp_of_solve >= 0, FLAG_COMP, 0).astype(np.int32)
+    comp_of_k = comp_of_node[mp]
+    flags = np.where(comp_of_k >= 0, FLAG_COMP, 0).astype(np.int32)
     for ch in module.channels:
         mask = module._channel_mask[ch.name]
         has = np.zeros(n, dtype=bool)
-        ok = comp_of_solve >= 0
-        has[ok] = mask[comp_of_solve[ok]]
+        ok = comp_of_k >= 0
+        has[ok] = mask[comp_of_k[ok]]
         flags |= np.where(has, FLAG_HH if ch.kernel_id == "HH" else FLAG_LEAK, 0).astype(np.int32)
-    # Stimulus entries grouped per node (solve order); column j of the stimulus array.
-    stim_solve = inv[stim_nodes] if len(stim_nodes) else np.zeros(0, np.int64)
-    order_e = np.argsort(stim_solve, kind="stable")
-    cnt = np.bincount(stim_solve, minlength=n) if len(stim_solve) else np.zeros(n, np.int64)
+    # Children of every kernel node (CSR, in increasing kernel index).
+    ch_nodes = np.nonzero(parent_k >= 0)[0]
+    srt = np.lexsort((ch_nodes, parent_k[ch_nodes]))
+    ch_nodes = ch_nodes[srt]
+    cnt = np.bincount(parent_k[ch_nodes], minlength=n)
+    cbeg = np.zeros(n, dtype=np.int64)
+    np.cumsum(cnt[:-1], out=cbeg[1:])
+    # Stimulus entries grouped per kernel node; entry e reads stimulus column stim_order[e].
+    stim_k = kidx_of_node[stim_nodes] if len(stim_nodes) else np.zeros(0, np.int64)
+    stim_order = np.argsort(stim_k, kind="stable")
+    scnt = np.bincount(stim_k, minlength=n) if len(stim_k) else np.zeros(n, np.int64)
     sbeg = np.zeros(n, dtype=np.int64)
-    np.cumsum(cnt[:-1], out=sbeg[1:])
+    np.cumsum(scnt[:-1], out=sbeg[1:])
     node_i = np.zeros((n, _lib.JXHIP_NODE_ICOLS), dtype=np.int32)
-    node_i[:, 0] = topo.parent
-    node_i[:, 1:3] = topo.child_rng
+    node_i[:, 0] = parent_k
+    node_i[:, 1] = cbeg
+    node_i[:, 2] = cbeg + cnt
     node_i[:, 3] = flags
     node_i[:, 4] = sbeg
-    node_i[:, 5] = sbeg + cnt
+    node_i[:, 5] = sbeg + scnt
     node_i[:, 6] = -1
-    syn = _build_synapses(module, inv, n, node_i)
-    rec_solve = inv[module._comp_node[rec_comps]] if len(rec_comps) else np.zeros(0, np.int64)
+    syn = _build_synapses(module, kidx_of_node, n, node_i)
+    rec_k = kidx_of_node[module._comp_node[rec_comps]] if len(rec_comps) else np.zeros(0, np.int64)
     topo_row = np.zeros(_lib.JXHIP_TOPO_COLS, dtype=np.int32)
-    topo_row[:12] = [n, topo.H, 0, 0, 0, len(rec_solve), 0,
-                     syn["n_tgt"] if syn else 0, 0, 0, 0, syn["n_syn"] if syn else 0]
+    topo_row[:14] = [n, sg["L"], 0, 0, 0, len(rec_k), 0, syn["n_tgt"] if syn else 0, 0, 0, 0,
+                     syn["n_syn"] if syn else 0, 0, 0]
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=device)
+    if syn is not None:
+        for key in ("pre", "ptr", "tgt"):
+            syn[key] = t(syn[key])
     return Structure(
-        n=n, H=topo.H, mp=mp, inv=inv, comp_of_solve=comp_of_solve, topo_row=topo_row,
-        order=t(topo.order), level_ptr=t(topo.level_ptr), node_i=t(node_i),
-        child_idx=t(topo.child_idx if len(topo.child_idx) else np.zeros(1, np.int32)),
-        stim_col=t(order_e if len(order_e) else np.zeros(1, np.int32)),
-        rec_node=t(rec_solve if len(rec_solve) else np.zeros(1, np.int32)),
-        syn=None if syn is None else {k: (t(v) if isinstance(v, np.ndarray) and v.dtype.kind == "i" else v) for k, v in syn.items()},
-        n_rec=len(rec_solve), rec_comps=rec_comps, stim_nodes_solve=stim_solve,
-        sinks_solve=inv[tables["sink"]], lo_edge=np.asarray(indexer["map_to_solve_order_lower"]),
-        up_edge=np.asarray(indexer["map_to_solve_order_upper"]),
+        n=n, L=sg["L"], mp=mp, kidx_of_node=kidx_of_node, comp_of_k=comp_of_k, topo_row=topo_row,
+        seg=t(sg["seg"]), seg_level_ptr=t(sg["seg_level_ptr"]), seg_child=t(sg["seg_child"]),
+        node_i=t(node_i), child_idx=t(ch_nodes if len(ch_nodes) else np.zeros(1, np.int64)),
+        stim_col=t(stim_order if len(stim_order) else np.zeros(1, np.int64)),
+        rec_node=t(rec_k if len(rec_k) else np.zeros(1, np.int64)),
+        syn=syn, n_rec=len(rec_k), rec_comps=rec_comps, stim_order=stim_order,
+        sinks_k=kidx_of_node[tables["sink"]],
+        lo_edge=np.asarray(indexer["map_to_solve_order_lower"])[korder],
+        up_edge=np.asarray(indexer["map_to_solve_order_upper"])[korder],
+        parent_k=parent_k,
     )
 
 
-def _build_synapses(module, inv: np.ndarray, n: int, node_i: np.ndarray) -> Optional[Dict]:
+def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray) -> Optional[Dict]:
     if not module._edges:
         return None
     if len(module._edges) > 1:
         raise NotImplementedError("the fused kernel supports one synapse type per network")
     e = module._edges[0]
-    post = inv[e["post"]]
-    pre = inv[e["pre"]]
+    post = kidx_of_node[e["post"]]
+    pre = kidx_of_node[e["pre"]]
     perm = np.argsort(post, kind="stable")          # synapses sorted by target (CSR)
     targets, counts = np.unique(post[perm], return_counts=True)
     ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
@@ -145,19 +166,25 @@ def _build_synapses(module, inv: np.ndarray, n: int, node_i: np.ndarray) -> Opti
 
 def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor, dt: float,
                      dtype) -> torch.Tensor:
-    """lo, up, dbase, inv_cm per solve-order node from the (unbatched) axial
-    conductances (already / cm, per compartment edge) and the node capacitances."""
+    """lo, up, dbase, inv_cm per kernel node from the (unbatched) axial conductances
+    (already / cm, per compartment edge) and the node capacitances."""
     dev = g_axial.device
     n = st.n
-    gdt = (dt * g_axial).to(dtype)
-    lo = -gdt[torch.as_tensor(st.lo_edge, device=dev)] if len(gdt) else torch.zeros(n, dtype=dtype, device=dev)
-    up = -gdt[torch.as_tensor(st.up_edge, device=dev)] if len(gdt) else torch.zeros(n, dtype=dtype, device=dev)
-    dbase = torch.as_tensor((st.comp_of_solve >= 0).astype(np.float64), dtype=dtype, device=dev)
+    gdt = (dt * g_axial.double())
     if len(gdt):
-        dbase = dbase.index_add(0, torch.as_tensor(st.sinks_solve, device=dev), gdt)
-    cm = cm_node.to(dtype)[torch.as_tensor(st.mp, device=dev)]
-    inv_cm = torch.where(torch.as_tensor(st.comp_of_solve >= 0, device=dev), 1.0 / cm, torch.zeros_like(cm))
-    return torch.stack([lo, up, dbase, inv_cm], 1).contiguous()
+        lo = -gdt[torch.as_tensor(st.lo_edge, device=dev)]
+        up = -gdt[torch.as_tensor(st.up_edge, device=dev)]
+    else:
+        lo = up = torch.zeros(n, dtype=torch.float64, device=dev)
+    has_par = torch.as_tensor(st.parent_k >= 0, device=dev)
+    lo = torch.where(has_par, lo, torch.zeros_like(lo))
+    up = torch.where(has_par, up, torch.zeros_like(up))
+    dbase = torch.as_tensor((st.comp_of_k >= 0).astype(np.float64), device=dev)
+    if len(gdt):
+        dbase = dbase.index_add(0, torch.as_tensor(st.sinks_k, device=dev), gdt)
+    cm = cm_node.double()[torch.as_tensor(st.mp, device=dev)]
+    inv_cm = torch.where(torch.as_tensor(st.comp_of_k >= 0, device=dev), 1.0 / cm, torch.zeros_like(cm))
+    return torch.stack([lo, up, dbase, inv_cm], 1).to(dtype).contiguous()
 
 
 @dataclass
@@ -166,21 +193,36 @@ class LaunchConfig:
     threads: int
 
 
+IPT_CHOICES = (1, 2, 4, 8, 12, 16, 20, 24, 32)
+
+
 def lds_bytes(n: int, n_tgt: int, S: int, esize: int) -> int:
     return (2 * n + 2 * n_tgt) * S * esize
 
 
+def items_per_thread(n: int, S: int, threads: int) -> int:
+    need = -(-n * S // threads)
+    for c in IPT_CHOICES:
+        if c >= need:
+            return c
+    raise NotImplementedError(f"{n} nodes x {S} simulations need more than 32 items per thread")
+
+
 def default_config(n: int, n_tgt: int, B: int, esize: int) -> LaunchConfig:
-    """A launch shape: enough programs to fill 256 CUs, LDS per program <= 40 KiB."""
+    """A launch shape: enough programs to fill the chip, LDS per program <= 40 KiB,
+    at most 24 register-resident items per thread."""
     S = 1
-    while S < 64 and S * 2 <= max(1, B // 512) and lds_bytes(n, n_tgt, S * 2, esize) <= 40 * 1024:
+    while S < 64 and S * 2 <= max(1, B // 1024) and lds_bytes(n, n_tgt, S * 2, esize) <= 40 * 1024:
         S *= 2
     while S > 1 and lds_bytes(n, n_tgt, S, esize) > 160 * 1024:
         S //= 2
     if lds_bytes(n, n_tgt, S, esize) > 160 * 1024:
         raise NotImplementedError(f"a topology of {n} nodes does not fit the fused kernel's LDS")
-    threads = int(min(1024, max(64, 64 * math.ceil(min(n * S, 256) / 64))))
-    return LaunchConfig(S, threads)
+    threads = 64
+    while threads < 1024 and -(-n * S // threads) > 20:
+        threads *= 2
+    threads = max(threads, min(256, 64 * -(-n * S // 64)))
+    return LaunchConfig(S, int(min(threads, 1024)))
 
 
 class ForwardLaunch:
@@ -192,12 +234,12 @@ class ForwardLaunch:
                  syn_params: Optional[List[torch.Tensor]] = None, syn_scale: Optional[torch.Tensor] = None,
                  syn_s0: Optional[torch.Tensor] = None, ckpt_every: int = 0):
         dev = v0.device
-        dtype = v0.dtype
+        dtype = gates0[0].dtype
         S = cfg.S
-        self.st, self.cfg, self.B, self.T, self.S = st, cfg, B, T, S
+        self.st, self.cfg, self.B, self.T, self.S, self.dtype = st, cfg, B, T, S, dtype
         self.P = P = (B + S - 1) // S
         n = st.n
-        self.v = to_prog(v0, S)
+        self.v = to_prog(v0.double(), S)
         self.gm, self.gh, self.gn = (to_prog(g, S) for g in gates0)
         self.rec = torch.empty((P, T + 1, max(st.n_rec, 1), S), dtype=dtype, device=dev)
         progs = np.zeros((P, _lib.JXHIP_PROG_COLS), dtype=np.int64)
@@ -225,14 +267,16 @@ class ForwardLaunch:
         if ckpt_every > 0:
             C = (T + ckpt_every - 1) // ckpt_every
             self.n_ckpt = C
-            self.ckpt = [torch.empty((C,) + self.v.shape, dtype=dtype, device=dev) for _ in range(4)]
+            self.ckpt = [torch.empty((C,) + self.v.shape, dtype=torch.float64, device=dev)]
+            self.ckpt += [torch.empty((C,) + self.gm.shape, dtype=dtype, device=dev) for _ in range(3)]
             self.ckpt_s = torch.empty((C,) + self.syn_s.shape, dtype=dtype, device=dev) if self.syn_s is not None else None
         a = _lib.FusedArgs()
         a.progs = self.progs.data_ptr()
         a.n_progs = P
         a.topos = self.topos.data_ptr()
-        a.order = st.order.data_ptr()
-        a.level_ptr = st.level_ptr.data_ptr()
+        a.seg = st.seg.data_ptr()
+        a.seg_level_ptr = st.seg_level_ptr.data_ptr()
+        a.seg_child = st.seg_child.data_ptr()
         a.node_i = st.node_i.data_ptr()
         a.node_f = node_f.data_ptr()
         a.child_idx = st.child_idx.data_ptr()
@@ -272,8 +316,9 @@ class ForwardLaunch:
         a.diff = SYN_DIFF
         a.S = S
         a.threads = cfg.threads
+        a.items_per_thread = items_per_thread(n, S, cfg.threads)
         n_tgt = st.syn["n_tgt"] if st.syn else 0
-        a.lds_bytes = lds_bytes(n, n_tgt, S, v0.element_size())
+        a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size())
         if ckpt_every > 0:
             a.ckpt_v, a.ckpt_m, a.ckpt_h, a.ckpt_n = (c.data_ptr() for c in self.ckpt)
             a.ckpt_s = self.ckpt_s.data_ptr() if self.ckpt_s is not None else 0
@@ -283,7 +328,7 @@ class ForwardLaunch:
         self.args = a
 
     def launch(self):
-        f = _lib.fn("jxhip_fused_forward", self.v.dtype)
+        f = _lib.fn("jxhip_fused_forward", self.dtype)
         _lib.check(f(self.args, _lib.stream_ptr(self.v.device)), "fused_forward")
 
     def recordings(self) -> torch.Tensor:
@@ -291,10 +336,3 @@ class ForwardLaunch:
         r = self.rec[..., : self.st.n_rec, :]                 # (P, T+1, n_rec, S)
         r = r.permute(0, 3, 2, 1).reshape(self.P * self.S, self.st.n_rec, self.T + 1)
         return r[: self.B]
-
-    def final_v_nodes(self) -> torch.Tensor:
-        """(B, n_nodes) in the module's node order."""
-        vs = from_prog(self.v, self.B)
-        out = torch.empty_like(vs)
-        out[:, torch.as_tensor(self.st.mp, device=vs.device)] = vs
-        return out

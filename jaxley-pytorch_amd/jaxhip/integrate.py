@@ -235,11 +235,16 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
     batch = _batch_of(*batch_t)
     B = int(np.prod(batch)) if batch else 1
     n = st.n
-    comp_of_solve = torch.as_tensor(st.comp_of_solve, device=device)
-    is_comp = comp_of_solve >= 0
-    cs_safe = torch.where(is_comp, comp_of_solve, torch.zeros_like(comp_of_solve))
+    comp_of_k = torch.as_tensor(st.comp_of_k, device=device)
+    is_comp = comp_of_k >= 0
+    cs_safe = torch.where(is_comp, comp_of_k, torch.zeros_like(comp_of_k))
+    # Branchpoints take the value of their parent compartment (their initial voltage is
+    # never read by the reference: their right-hand side is 0; here it seeds the
+    # increment form with a consistent value).
+    par_k = torch.as_tensor(st.parent_k, device=device)
+    bp_src = torch.where(is_comp, cs_safe, cs_safe[torch.clamp(par_k, min=0)])
 
-    def comp_to_solve(x):  # (..., n_comp) -> (..., n) solve order, 0 at branchpoints
+    def comp_to_kernel(x):  # (..., n_comp) -> (..., n) kernel order, 0 at branchpoints
         y = x[..., cs_safe]
         return torch.where(is_comp, y, torch.zeros_like(y))
 
@@ -255,19 +260,19 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
     for k in chan_keys:
         x = vals[k]
         if x.ndim == 1:
-            params[k] = (0, comp_to_solve(x.detach() if not x.requires_grad else x).contiguous())
+            params[k] = (0, comp_to_kernel(x.detach() if not x.requires_grad else x).contiguous())
         else:
             x2 = torch.broadcast_to(x, (*batch, x.shape[-1])).reshape(B, -1)
-            params[k] = (2, to_prog(comp_to_solve(x2), S).reshape(-1))
+            params[k] = (2, to_prog(comp_to_kernel(x2), S).reshape(-1))
         param_inputs[k] = x
     # States.
-    v0 = torch.broadcast_to(comp_to_solve(init["v"]), (*batch, n)).reshape(B, n)
+    v0 = torch.broadcast_to(init["v"][..., bp_src], (*batch, n)).reshape(B, n)
     gates0 = []
     for k in ("HH_m", "HH_h", "HH_n"):
         if k in init:
             x = init[k]
             x = torch.where(torch.isnan(x), torch.zeros_like(x), x)
-            gates0.append(torch.broadcast_to(comp_to_solve(x), (*batch, n)).reshape(B, n))
+            gates0.append(torch.broadcast_to(comp_to_kernel(x), (*batch, n)).reshape(B, n))
         else:
             gates0.append(torch.zeros(B, n, dtype=dtype, device=device))
     # Stimulus (T, n_stim, stim_b) and per-entry scale dt * 1e5 / (area cm).
@@ -284,7 +289,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         area_node[torch.as_tensor(module._comp_node, device=device)] = geo["area"]
         sn = torch.as_tensor(inp.stim_nodes, device=device)
         scale = dt * 1e5 / (area_node[sn] * cm_node[sn])
-        order_e = st.stim_col.long()[: len(inp.stim_nodes)]
+        order_e = torch.as_tensor(st.stim_order, device=device)
         stim_scale = scale[order_e].contiguous()
     # Synapses.
     syn_params = syn_scale = syn_s0 = None
@@ -318,7 +323,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         K = checkpoint_lengths[-1] if checkpoint_lengths else 100
         recs, final = fused_integrate_with_adjoint(
             st, node_f, params, param_inputs, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K,
-            batch, comp_to_solve)
+            batch, comp_to_kernel)
     else:
         with torch.no_grad():
             run = ForwardLaunch(st, node_f, {k: (m, t.detach()) for k, (m, t) in params.items()},

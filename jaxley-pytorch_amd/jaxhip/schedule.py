@@ -339,6 +339,66 @@ def forest_schedule(parent: np.ndarray, max_block: int) -> Optional[Dict[str, np
     }
 
 
+def segments(parent: np.ndarray):
+    """Kernel order of a forest (parents before children): segments are maximal chains
+    of single-child nodes (top first); a segment's level is 0 if its bottom node has no
+    children, else 1 + the largest level of its child segments. Returns the kernel order
+    (kernel index -> input index), the segments `(beg, end)` in kernel indices grouped by
+    level (level pointer), and the child-segment tops of every segment."""
+    parent = np.asarray(parent, dtype=np.int64)
+    n = len(parent)
+    nch = np.bincount(parent[parent >= 0], minlength=n)
+    kids: List[List[int]] = [[] for _ in range(n)]
+    for c in np.nonzero(parent >= 0)[0]:
+        kids[parent[c]].append(int(c))
+    is_top = (parent < 0) | (nch[np.maximum(parent, 0)] != 1)
+    tops = np.nonzero(is_top)[0]
+    seg_nodes = []
+    seg_of = -np.ones(n, dtype=np.int64)
+    for t in tops:
+        chain = [int(t)]
+        while nch[chain[-1]] == 1:
+            chain.append(kids[chain[-1]][0])
+        seg_of[chain] = len(seg_nodes)
+        seg_nodes.append(chain)
+    nseg = len(seg_nodes)
+    child_segs = [[int(seg_of[c]) for c in kids[ch[-1]]] if nch[ch[-1]] >= 2 else [] for ch in seg_nodes]
+    level = np.zeros(nseg, dtype=np.int64)
+    # Children have larger input indices than their parents: process segments by
+    # decreasing top index so child levels are final before their parent's.
+    for q in np.argsort([ch[0] for ch in seg_nodes])[::-1]:
+        if child_segs[q]:
+            level[q] = 1 + max(level[c] for c in child_segs[q])
+    seg_order = np.lexsort((np.array([ch[0] for ch in seg_nodes]), level))
+    korder = np.concatenate([seg_nodes[q] for q in seg_order]).astype(np.int64)
+    kidx = np.empty(n, dtype=np.int64)
+    kidx[korder] = np.arange(n)
+    seg = np.zeros((nseg, 4), dtype=np.int32)
+    seg_child = []
+    pos = 0
+    for r, q in enumerate(seg_order):
+        ch = seg_nodes[q]
+        seg[r, 0] = kidx[ch[0]]
+        seg[r, 1] = kidx[ch[0]] + len(ch)
+        tops_k = [int(kidx[seg_nodes[c][0]]) for c in child_segs[q]]
+        seg[r, 2] = pos
+        seg[r, 3] = pos + len(tops_k)
+        seg_child += tops_k
+        pos += len(tops_k)
+    L = int(level.max()) if nseg else 0
+    counts = np.bincount(level, minlength=L + 1)
+    lptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    return {
+        "korder": korder,
+        "kidx": kidx,
+        "seg": seg,
+        "seg_level_ptr": lptr,
+        "seg_child": np.asarray(seg_child if seg_child else [0], dtype=np.int32),
+        "L": L,
+        "nseg": nseg,
+    }
+
+
 @dataclass
 class FusedTopology:
     """One topology of the fused kernels, in solve order (node k = solve index k).

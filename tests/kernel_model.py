@@ -1,10 +1,10 @@
-"""CPU model of the fused forward kernel's indexing (tests only).
+"""CPU model of the fused forward kernel's algorithm and indexing (tests only).
 
-Executes, in numpy float64, the algorithm of `fused_forward_kernel`
-(jaxley-pytorch_amd/csrc/jxhip_fused.hip) on the exact descriptor arrays that
-`jaxhip.fused.ForwardLaunch` builds (programs, topology, per-node tables, program
-layout). It lets the CPU test suite check the host-side tables against the oracle
-without a GPU; GPU tests then check the kernel itself.
+Executes, in numpy float64, what `fused_forward_kernel`
+(jaxley-pytorch_amd/csrc/jxhip_fused.hip) computes, on the exact descriptor arrays that
+`jaxhip.fused.ForwardLaunch` builds: programs, kernel-order node tables, segments,
+program layout, the increment form A dv = b - A v. It lets the CPU test suite check the
+host-side tables against the oracle without a GPU; GPU tests then check the kernel.
 """
 import numpy as np
 import torch
@@ -15,43 +15,40 @@ def _np(t):
 
 
 def _xoe(x, y):
-    z = min(x / y, 20.0)
-    if abs(z) < 1e-4:
-        return y * (1 - z / 2 + z * z / 12)
-    return x / np.expm1(z)
+    z = np.minimum(x / y, 20.0)
+    small = np.abs(z) < 1e-4
+    zs = np.where(small, 1.0, z)
+    return np.where(small, y * (1 - z / 2 + z * z / 12), x / np.expm1(zs))
 
 
 def _se(x):
-    return np.exp(min(x, 20.0))
+    return np.exp(np.minimum(x, 20.0))
 
 
 def _gate(x, dt, a, b):
     tau = 1 / (a + b)
-    xinf = a * tau
     e = _se(-dt / tau)
-    return x * e + xinf * (1 - e)
+    return x * e + a * tau * (1 - e)
 
 
 def run_forward_model(run):
-    """Fill `run.rec`, `run.v` and the gates as the kernel would (in place, CPU copy)."""
-    from jaxhip import _lib
-
-    progs = _np(run.progs)
-    topo = _np(run.topos)[0]
-    n, H = int(topo[0]), int(topo[1])
-    order = _np(run.st.order)
-    lptr = _np(run.st.level_ptr)
-    ni = _np(run.st.node_i)
-    nf = _np(run.node_f).astype(np.float64)
-    cidx = _np(run.st.child_idx)
-    rec_node = _np(run.st.rec_node)
-    n_rec = run.st.n_rec
-    a = run.args
-    S = run.S
-    dt = a.dt
-    params = {}
+    """Fill `run.rec`, `run.v` and the gates as the kernel would (in place)."""
     from jaxhip.fused import ALL_PARAMS
 
+    st = run.st
+    progs = _np(run.progs)
+    n, Lmax = int(st.topo_row[0]), int(st.topo_row[1])
+    seg = _np(st.seg)
+    lptr = _np(st.seg_level_ptr)
+    sch = _np(st.seg_child)
+    ni = _np(st.node_i)
+    nf = _np(run.node_f).astype(np.float64)
+    cidx = _np(st.child_idx)
+    rec_node = _np(st.rec_node)
+    n_rec = st.n_rec
+    S = run.S
+    dt = run.args.dt
+    params = {}
     for j, name in enumerate(ALL_PARAMS):
         mode, t = run.params.get(name, (0, None))
         params[j] = (mode, None if t is None else _np(t).astype(np.float64))
@@ -61,35 +58,46 @@ def run_forward_model(run):
     gn = _np(run.gn).astype(np.float64).reshape(-1)
     rec = np.zeros(run.rec.numel())
     stim = None if run.stim is None else _np(run.stim).astype(np.float64)
-    stim_scale = None if run.stim_scale is None else _np(run.stim_scale).astype(np.float64)
-    stim_col = _np(run.st.stim_col)
+    scale = None if run.stim_scale is None else _np(run.stim_scale).astype(np.float64)
+    scol = _np(st.stim_col)
+    ss = np.arange(S)
 
-    def P(j, pg, k, s):
+    def P(j, pg, k):
         mode, arr = params[j]
         if mode == 0:
-            return arr[k]
+            return np.full(S, arr[k])
         if mode == 1:
-            return arr[pg[5] + s]
-        return arr[pg[4] + k * S + s]
+            return arr[pg[5] + ss]
+        return arr[pg[4] + k * S + ss]
 
     for pg in progs:
         nsim, t0, t1, soff, simoff, roff = pg[1], pg[2], pg[3], pg[4], pg[5], pg[6]
-        xs = v[soff:soff + n * S].copy()
-        ds = np.zeros(n * S)
-        for i in range(n_rec * S):
-            r, s = divmod(i, S)
-            rec[roff + i] = xs[rec_node[r] * S + s]
+        act = ss < nsim
+        V = v[soff:soff + n * S].reshape(n, S).copy()
+        Gm = gm[soff:soff + n * S].reshape(n, S).copy()
+        Gh = gh[soff:soff + n * S].reshape(n, S).copy()
+        Gn = gn[soff:soff + n * S].reshape(n, S).copy()
+        for r in range(n_rec):
+            rec[roff + r * S:roff + (r + 1) * S] = V[rec_node[r]]
         for step in range(t1 - t0):
             t = t0 + step
-            for i in range(n * S):
-                k, s = divmod(i, S)
+            D = np.zeros((n, S))
+            X = np.zeros((n, S))
+            for k in range(n):
                 flags = ni[k, 3]
-                dbase = nf[k, 2]
-                if not (flags & 1) or s >= nsim:
-                    ds[i], xs[i] = dbase, 0.0
+                p = ni[k, 0]
+                ax = np.zeros(S)
+                if p >= 0:
+                    ax += nf[k, 0] * (V[k] - V[p])
+                for j in range(ni[k, 1], ni[k, 2]):
+                    c = cidx[j]
+                    ax += nf[c, 1] * (V[k] - V[c])
+                if not (flags & 1):
+                    D[k], X[k] = nf[k, 2], ax
                     continue
-                vv = xs[i]
-                lin = cst = 0.0
+                g = np.zeros(S)
+                cur = np.zeros(S)
+                vv = V[k]
                 if flags & 2:
                     am = 0.1 * _xoe(-(vv + 40), 10)
                     bm = 4.0 * _se(-(vv + 65) / 18)
@@ -97,51 +105,64 @@ def run_forward_model(run):
                     bh = 1.0 / (_se(-(vv + 35) / 10) + 1)
                     an = 0.01 * _xoe(-(vv + 55), 10)
                     bn = 0.125 * _se(-(vv + 65) / 80)
-                    gi = soff + i
-                    gm[gi] = _gate(gm[gi], dt, am, bm)
-                    gh[gi] = _gate(gh[gi], dt, ah, bh)
-                    gn[gi] = _gate(gn[gi], dt, an, bn)
-                    gNa = P(0, pg, k, s) * gm[gi] ** 3 * gh[gi]
-                    gK = P(1, pg, k, s) * gn[gi] ** 4
-                    gL = P(2, pg, k, s)
-                    lin += 1000 * (gNa + gK + gL)
-                    cst += 1000 * (gNa * P(3, pg, k, s) + gK * P(4, pg, k, s) + gL * P(5, pg, k, s))
+                    Gm[k] = _gate(Gm[k], dt, am, bm)
+                    Gh[k] = _gate(Gh[k], dt, ah, bh)
+                    Gn[k] = _gate(Gn[k], dt, an, bn)
+                    gNa = P(0, pg, k) * Gm[k] ** 3 * Gh[k]
+                    gK = P(1, pg, k) * Gn[k] ** 4
+                    gL = P(2, pg, k)
+                    g += gNa + gK + gL
+                    cur += gNa * (vv - P(3, pg, k)) + gK * (vv - P(4, pg, k)) + gL * (vv - P(5, pg, k))
                 if flags & 4:
-                    gl = P(6, pg, k, s)
-                    lin += 1000 * gl
-                    cst += 1000 * gl * P(7, pg, k, s)
-                rhs = vv + dt * cst * nf[k, 3]
+                    gl = P(6, pg, k)
+                    g += gl
+                    cur += gl * (vv - P(7, pg, k))
+                r = ax + dt * nf[k, 3] * (-1000 * cur)
                 for e in range(ni[k, 4], ni[k, 5]):
-                    col = stim_col[e]
                     sb = stim.shape[2]
-                    bidx = simoff + s if sb > 1 else 0
-                    rhs += stim[t, col, bidx] * stim_scale[e]
-                ds[i] = dbase + dt * lin * nf[k, 3]
-                xs[i] = rhs
-            for h in range(1, H + 1):
-                for pos in range(lptr[h], lptr[h + 1]):
-                    k = order[pos]
-                    for s in range(S):
-                        dk, xk = ds[k * S + s], xs[k * S + s]
-                        for j in range(ni[k, 1], ni[k, 2]):
-                            c = cidx[j]
-                            f = nf[c, 1] / ds[c * S + s]
-                            dk -= nf[c, 0] * f
-                            xk -= xs[c * S + s] * f
-                        ds[k * S + s], xs[k * S + s] = dk, xk
-            for h in range(H, -1, -1):
-                for pos in range(lptr[h], lptr[h + 1]):
-                    k = order[pos]
-                    p = ni[k, 0]
-                    for s in range(S):
-                        xk = xs[k * S + s]
-                        if p >= 0:
-                            xk -= nf[k, 0] * xs[p * S + s]
-                        xs[k * S + s] = xk / ds[k * S + s]
-            for i in range(n_rec * S):
-                r, s = divmod(i, S)
-                rec[roff + ((step + 1) * n_rec + r) * S + s] = xs[rec_node[r] * S + s]
-        v[soff:soff + n * S] = xs
-    shape = run.rec.shape
-    run.rec.copy_(torch.as_tensor(rec.reshape(shape), dtype=run.rec.dtype))
+                    b = simoff + ss if sb > 1 else np.zeros(S, np.int64)
+                    r = r + stim[t, scol[e], b] * scale[e]
+                D[k] = np.where(act, nf[k, 2] + dt * nf[k, 3] * 1000 * g, nf[k, 2] + 1)
+                X[k] = np.where(act, r, 0.0)
+            # Elimination by segment level, chains bottom-up; 1/d kept in D.
+            for lv in range(Lmax + 1):
+                for q in range(lptr[lv], lptr[lv + 1]):
+                    beg, end, cb, ce = seg[q]
+                    k = end - 1
+                    dk, xk = D[k].copy(), X[k].copy()
+                    for c in sch[cb:ce]:
+                        f = nf[c, 1] * D[c]
+                        dk -= nf[c, 0] * f
+                        xk -= X[c] * f
+                    inv = 1 / dk
+                    D[k], X[k] = inv, xk
+                    for k in range(end - 2, beg - 1, -1):
+                        f = nf[k + 1, 1] * inv
+                        d2 = D[k] - nf[k + 1, 0] * f
+                        xk = X[k] - xk * f
+                        inv = 1 / d2
+                        D[k], X[k] = inv, xk
+            for lv in range(Lmax, -1, -1):
+                for q in range(lptr[lv], lptr[lv + 1]):
+                    beg, end = seg[q, 0], seg[q, 1]
+                    p = ni[beg, 0]
+                    xp = X[p] if p >= 0 else np.zeros(S)
+                    for k in range(beg, end):
+                        xk = X[k].copy()
+                        if k > beg or p >= 0:
+                            xk -= nf[k, 0] * xp
+                        xk *= D[k]
+                        X[k] = xk
+                        xp = xk
+            V = V + X
+            for r in range(n_rec):
+                o = roff + ((step + 1) * n_rec + r) * S
+                rec[o:o + S] = V[rec_node[r]]
+        v[soff:soff + n * S] = V.reshape(-1)
+        gm[soff:soff + n * S] = Gm.reshape(-1)
+        gh[soff:soff + n * S] = Gh.reshape(-1)
+        gn[soff:soff + n * S] = Gn.reshape(-1)
+    run.rec.copy_(torch.as_tensor(rec.reshape(run.rec.shape), dtype=run.rec.dtype))
     run.v.copy_(torch.as_tensor(v.reshape(run.v.shape), dtype=run.v.dtype))
+    for buf, arr in ((run.gm, gm), (run.gh, gh), (run.gn, gn)):
+        buf.copy_(torch.as_tensor(arr.reshape(buf.shape), dtype=buf.dtype))
