@@ -164,12 +164,27 @@ def main():
                      "bytes_per_comp_step": BYTES_PER_COMP_STEP},
     }
     rec32 = run()
+    from jaxhip.integrate import _F32_MODE, set_float32_mode
+
+    out["f32_mode"] = _F32_MODE
     if not args.no_extras:
         # Float64 on the GPU, all simulations.
         cell64, _, ps64, ds64, _ = setup(B, T, torch.float64, device)
         with torch.no_grad():
             rec64 = jx.integrate(cell64, param_state=ps64, data_stimuli=ds64, delta_t=DT)
-        out["max_err_f32_vs_f64_gpu_mV"] = float((rec32.double() - rec64).abs().max())
+        err = (rec32.double() - rec64).abs().amax(dim=(1, 2))
+        out["max_err_f32_vs_f64_gpu_mV"] = float(err.max())
+        out["frac_sims_within_1e-2_mV"] = float((err <= 1e-2).double().mean())
+        # The other float32 mode, for comparison.
+        other = "fast" if _F32_MODE == "mixed" else "mixed"
+        set_float32_mode(other)
+        so = timed(run, 1, 1, device)
+        ro = run()
+        set_float32_mode(_F32_MODE)
+        erro = (ro.double() - rec64).abs().amax(dim=(1, 2))
+        out[f"{other}_f32"] = {"value": comp_steps / so, "max_err_vs_f64_gpu_mV": float(erro.max()),
+                               "frac_sims_within_1e-2_mV": float((erro <= 1e-2).double().mean())}
+        log(f"[bench] {other} f32: {comp_steps / so:.3e} comp-steps/s, max err {float(erro.max()):.3e} mV")
         # Plain: one simulation per launch.
         n_plain = 4
         cp, _, psp, dsp, _ = setup(1, T, dtype, device)

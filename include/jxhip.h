@@ -211,10 +211,13 @@ typedef struct jxhip_fused_args {
   const int32_t* stim_col;  /* stimulus column of each stimulus entry */
   const void* stim_scale;   /* dt * 1e5 / (area cm) per stimulus entry */
   const int32_t* rec_node;  /* recorded nodes at rec_base */
-  /* channel parameters with a mode each: 0 = per node (node_base + k),
-     1 = per simulation (sim_off + s), 2 = per (node, simulation) (state_off + k S + s) */
-  const void* param[JXHIP_NPARAM];
-  int32_t param_mode[JXHIP_NPARAM];
+  /* channel parameters: node_p (n_nodes_total, JXHIP_NPARAM) per node; the
+     conductances gNa, gK, gLeak (HH) and gLeak (Leak) may instead be batched in
+     bparam, 4 values per entry: bparam_mode 1 = per simulation ([sim_off + s][4]),
+     2 = per (node, simulation) ([state_off + k S + s][4]); 0 = not batched */
+  const void* node_p;
+  const void* bparam;
+  int32_t bparam_mode;
   /* state in program layout [node][S] at state_off; v is float64 */
   void* v;
   void* gate_m;
@@ -240,9 +243,9 @@ typedef struct jxhip_fused_args {
   double diff;              /* finite difference of the synapse slope (1e-3) */
   int32_t S;                /* simulations per program: 1, 2, 4, ..., 64 */
   int32_t threads;          /* threads per workgroup: multiple of 64, <= 1024 */
-  int32_t items_per_thread; /* forward: (node, simulation) items per thread, held in
-                               registers: 1, 2, 4, 8, 12, 16, 20, 24 or 32, with
-                               n S <= items_per_thread threads */
+  int32_t items_per_thread; /* forward: (node, simulation) items per thread whose gates
+                               stay in registers: 0, 1, 2, 4, 8, 12 or 16 (further items
+                               stream their gates from HBM) */
   int64_t lds_bytes;        /* dynamic LDS per workgroup (<= 160 KiB) */
   /* checkpoints (for the adjoint): per program, snapshot c of the state before step
      c * ckpt_every at ckpt_* + c * ckpt_stride + state_off (syn: ckpt_syn_stride,
@@ -259,6 +262,10 @@ typedef struct jxhip_fused_args {
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);
 int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream);
+/* Mixed precision: float64 gates, rates, parameters, stimulus and recordings (every
+ * "element type" array is float64) with the tree solve on float32 increments (node_f
+ * and LDS in float32). Meets float64-level trace accuracy at float32 solve cost. */
+int jxhip_fused_forward_mixed(const jxhip_fused_args* args, void* stream);
 
 /* Reverse pass of jxhip_fused_forward_* for HH / Leak cells (no synapses, no stimulus
  * gradient): for every K-step segment, from the last to the first, recomputes the
@@ -271,7 +278,10 @@ int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream);
  *            NULL entries are skipped
  *   g_v0:    gradient of the initial voltages, program layout
  *   work_*:  per program [K + 1][node][S] at (program index) * work_stride
- *   lam_*:   adjoint gates, program layout (scratch) */
+ *   lam_*:   adjoint gates, program layout (scratch)
+ * Launch limits of the reverse pass: fwd->threads <= 512 and
+ * fwd->lds_bytes = (3 * n * S + 2 * n) * sizeof(solve type) <= 160 KiB; the forward that
+ * wrote the checkpoints must have used the same S. */
 typedef struct jxhip_adjoint_args {
   const void* g_rec;
   void* g_param[JXHIP_NPARAM];
@@ -291,6 +301,8 @@ int jxhip_fused_adjoint_f32(const jxhip_fused_args* fwd, const jxhip_adjoint_arg
                             void* stream);
 int jxhip_fused_adjoint_f64(const jxhip_fused_args* fwd, const jxhip_adjoint_args* adj,
                             void* stream);
+int jxhip_fused_adjoint_mixed(const jxhip_fused_args* fwd, const jxhip_adjoint_args* adj,
+                              void* stream);
 
 /* Last HIP error string for an error code (host-only helper). */
 const char* jxhip_error_string(int code);

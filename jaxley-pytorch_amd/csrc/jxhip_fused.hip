@@ -47,8 +47,9 @@ struct Fused {
   const int32_t* stim_col;
   const G* stim_scale;
   const int32_t* rec_node;
-  const G* param[JXHIP_NPARAM];
-  int32_t param_mode[JXHIP_NPARAM];
+  const G* node_p;     // (n_nodes_total, JXHIP_NPARAM) shared channel parameters
+  const G* bparam;     // batched conductances (gNa, gK, gLeak of HH, gLeak of Leak)
+  int32_t bmode;       // 0 none, 1 per simulation, 2 per (node, simulation)
   double* v;
   G* gm;
   G* gh;
@@ -92,10 +93,9 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.stim_col = a.stim_col;
   f.stim_scale = static_cast<const G*>(a.stim_scale);
   f.rec_node = a.rec_node;
-  for (int j = 0; j < JXHIP_NPARAM; ++j) {
-    f.param[j] = static_cast<const G*>(a.param[j]);
-    f.param_mode[j] = a.param_mode[j];
-  }
+  f.node_p = static_cast<const G*>(a.node_p);
+  f.bparam = static_cast<const G*>(a.bparam);
+  f.bmode = a.bparam_mode;
   f.v = static_cast<double*>(a.v);
   f.gm = static_cast<G*>(a.gate_m);
   f.gh = static_cast<G*>(a.gate_h);
@@ -175,13 +175,18 @@ __device__ __forceinline__ Topo load_topo(const int32_t* topos, int32_t t) {
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 
-template <typename G, typename L>
-__device__ __forceinline__ G param_at(const Fused<G, L>& a, int j, const Prog& pg, const Topo& tp,
-                                      int k, int s) {
-  const int32_t mode = a.param_mode[j];
-  if (mode == 0) return a.param[j][tp.nb + k];
-  if (mode == 1) return a.param[j][pg.simoff + s];
-  return a.param[j][pg.soff + (static_cast<int64_t>(k) << a.log2S) + s];
+// Channel parameter J of node k, simulation s: the batched conductances (J = 0, 1, 2:
+// gNa, gK, gLeak of HH; J = 6: gLeak of Leak) come from `bparam` when batched, every
+// other value from the per-node table.
+template <int J, typename G, typename L>
+__device__ __forceinline__ G prm(const Fused<G, L>& a, const Prog& pg, const Topo& tp, int k, int s) {
+  constexpr int col = J == 0 ? 0 : J == 1 ? 1 : J == 2 ? 2 : J == 6 ? 3 : -1;
+  if constexpr (col >= 0) {
+    if (a.bmode == 1) return a.bparam[((pg.simoff + s) << 2) + col];
+    if (a.bmode == 2)
+      return a.bparam[((pg.soff + (static_cast<int64_t>(k) << a.log2S) + s) << 2) + col];
+  }
+  return a.node_p[static_cast<int64_t>(tp.nb + k) * JXHIP_NPARAM + J];
 }
 
 // Synapses of one target and simulation (modules/network.py:382-478,
@@ -245,21 +250,23 @@ struct Lds {
   L* ds;   // diagonal, then its reciprocal after elimination
   L* tl;   // synapse slope per (target, simulation)
   L* tc;   // synapse current per (target, simulation)
+  L* cl;   // lower / upper coefficient per node
 };
 
 // Triangularization leaves first, one thread per (segment, simulation) walking its
 // chain bottom-up; back-substitution top-down. kT: solve with A^T (lower <-> upper).
 template <typename G, typename L, bool kT>
-__device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp, L* xs, L* ds) {
+__device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp, L* xs, L* ds,
+                                           const L* cl) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int S = a.S, lS = a.log2S;
   const int32_t* sg = a.seg + static_cast<int64_t>(tp.sb) * 4;
   const int32_t* lptr = a.seg_lptr + tp.lb;
   const int32_t* sch = a.seg_child + tp.scb;
-  const L* nf = a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS;
   const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
-  constexpr int kLo = kT ? kNfUp : kNfLo;
-  constexpr int kUp = kT ? kNfLo : kNfUp;
+  // cl: [lo, up] per node in LDS.
+  constexpr int kLo = kT ? 1 : 0;
+  constexpr int kUp = kT ? 0 : 1;
   for (int lv = 0; lv <= tp.L; ++lv) {
     const int lo = lptr[lv], w = (lptr[lv + 1] - lo) << lS;
     for (int i = tid; i < w; i += nthr) {
@@ -270,16 +277,16 @@ __device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp,
       for (int j = sg[4 * q + 2]; j < sg[4 * q + 3]; ++j) {
         const int c = sch[j];
         // ds[c] holds 1 / d[c] of the already eliminated child segment top.
-        const L f = nf[c * JXHIP_NODE_FCOLS + kUp] * ds[(c << lS) + s];
-        dk -= nf[c * JXHIP_NODE_FCOLS + kLo] * f;
+        const L f = cl[2 * c + kUp] * ds[(c << lS) + s];
+        dk -= cl[2 * c + kLo] * f;
         xk -= xs[(c << lS) + s] * f;
       }
       L inv = recip(dk);
       ds[(k << lS) + s] = inv;
       xs[(k << lS) + s] = xk;
       for (k = end - 2; k >= beg; --k) {
-        const L f = nf[(k + 1) * JXHIP_NODE_FCOLS + kUp] * inv;
-        const L d2 = ds[(k << lS) + s] - nf[(k + 1) * JXHIP_NODE_FCOLS + kLo] * f;
+        const L f = cl[2 * (k + 1) + kUp] * inv;
+        const L d2 = ds[(k << lS) + s] - cl[2 * (k + 1) + kLo] * f;
         xk = xs[(k << lS) + s] - xk * f;
         inv = recip(d2);
         ds[(k << lS) + s] = inv;
@@ -297,13 +304,23 @@ __device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp,
       L xp = p >= 0 ? xs[(p << lS) + s] : L(0);
       for (int k = beg; k < end; ++k) {
         L xk = xs[(k << lS) + s];
-        if (k > beg || p >= 0) xk -= nf[k * JXHIP_NODE_FCOLS + kLo] * xp;
+        if (k > beg || p >= 0) xk -= cl[2 * k + kLo] * xp;
         xk *= ds[(k << lS) + s];
         xs[(k << lS) + s] = xk;
         xp = xk;
       }
     }
     __syncthreads();
+  }
+}
+
+// Copies the lower / upper coefficients of the topology into LDS (once per launch).
+template <typename G, typename L>
+__device__ __forceinline__ void load_coefficients(const Fused<G, L>& a, const Topo& tp, L* cl) {
+  const L* nf = a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS;
+  for (int k = threadIdx.x; k < tp.n; k += blockDim.x) {
+    cl[2 * k] = nf[k * JXHIP_NODE_FCOLS + kNfLo];
+    cl[2 * k + 1] = nf[k * JXHIP_NODE_FCOLS + kNfUp];
   }
 }
 
@@ -344,19 +361,19 @@ __device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg,
     m = gate_step(m, dt, am, bm);
     h = gate_step(h, dt, ah, bh);
     n = gate_step(n, dt, an, bn);
-    const G gNa = param_at(a, 0, pg, tp, k, s) * (m * m * m) * h;
+    const G gNa = prm<0>(a, pg, tp, k, s) * (m * m * m) * h;
     const G n2 = n * n;
-    const G gK = param_at(a, 1, pg, tp, k, s) * (n2 * n2);
-    const G gL = param_at(a, 2, pg, tp, k, s);
+    const G gK = prm<1>(a, pg, tp, k, s) * (n2 * n2);
+    const G gL = prm<2>(a, pg, tp, k, s);
     g += gNa + gK + gL;
-    cur += gNa * static_cast<G>(v - static_cast<double>(param_at(a, 3, pg, tp, k, s))) +
-           gK * static_cast<G>(v - static_cast<double>(param_at(a, 4, pg, tp, k, s))) +
-           gL * static_cast<G>(v - static_cast<double>(param_at(a, 5, pg, tp, k, s)));
+    cur += gNa * static_cast<G>(v - static_cast<double>(prm<3>(a, pg, tp, k, s))) +
+           gK * static_cast<G>(v - static_cast<double>(prm<4>(a, pg, tp, k, s))) +
+           gL * static_cast<G>(v - static_cast<double>(prm<5>(a, pg, tp, k, s)));
   }
   if (flags & kLeak) {
-    const G gl = param_at(a, 6, pg, tp, k, s);
+    const G gl = prm<6>(a, pg, tp, k, s);
     g += gl;
-    cur += gl * static_cast<G>(v - static_cast<double>(param_at(a, 7, pg, tp, k, s)));
+    cur += gl * static_cast<G>(v - static_cast<double>(prm<7>(a, pg, tp, k, s)));
   }
   G lin = G(1000) * g;
   G net = -G(1000) * cur;  // (cst - lin v) of the channels
@@ -400,8 +417,10 @@ __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& 
   __syncthreads();
 }
 
-// Forward: items of a thread are i = tid + j * blockDim (j < IPT); their voltages and
-// gates stay in registers for the whole launch.
+// Forward. The first IPT items of a thread, i = tid + j * blockDim (j < IPT), keep their
+// gates in VGPRs for the whole launch; further items stream their gates from HBM.
+// Voltages live in HBM (float64, program layout); LDS holds the system and the
+// coefficients of the tree.
 template <typename G, typename L, int IPT>
 __global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
   extern __shared__ unsigned char smem_raw[];
@@ -414,18 +433,24 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
   Ld.ds = Ld.xs + nS;
   Ld.tl = Ld.ds + nS;
   Ld.tc = Ld.tl + (tp.n_tgt << lS);
+  Ld.cl = Ld.tc + (tp.n_tgt << lS);
+  load_coefficients(a, tp, Ld.cl);
   const int tid = threadIdx.x, nthr = blockDim.x;
   double* vg = a.v + pg.soff;
-  double vr[IPT];
-  G mr[IPT], hr[IPT], nr[IPT];
+  G* gm = a.gm + pg.soff;
+  G* gh = a.gh + pg.soff;
+  G* gn = a.gn + pg.soff;
+  constexpr int kR = IPT > 0 ? IPT : 1;
+  G mr[kR], hr[kR], nr[kR];
+  if constexpr (IPT > 0) {
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const int i = tid + j * nthr;
-    if (i < nS) {
-      vr[j] = vg[i];
-      mr[j] = a.gm[pg.soff + i];
-      hr[j] = a.gh[pg.soff + i];
-      nr[j] = a.gn[pg.soff + i];
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * nthr;
+      if (i < nS) {
+        mr[j] = gm[i];
+        hr[j] = gh[i];
+        nr[j] = gn[i];
+      }
     }
   }
   for (int i = tid; i < (tp.n_rec << lS); i += nthr) {
@@ -436,15 +461,22 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
   for (int64_t step = 0; step < nsteps; ++step) {
     if (a.ckpt_v != nullptr && step % a.ckpt_every == 0) {
       const int64_t off = pg.soff + (step / a.ckpt_every) * a.ckpt_stride;
+      for (int i = tid; i < nS; i += nthr) a.ckpt_v[off + i] = vg[i];
+      if constexpr (IPT > 0) {
 #pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const int i = tid + j * nthr;
-        if (i < nS) {
-          a.ckpt_v[off + i] = vr[j];
-          a.ckpt_m[off + i] = mr[j];
-          a.ckpt_h[off + i] = hr[j];
-          a.ckpt_n[off + i] = nr[j];
+        for (int j = 0; j < IPT; ++j) {
+          const int i = tid + j * nthr;
+          if (i < nS) {
+            a.ckpt_m[off + i] = mr[j];
+            a.ckpt_h[off + i] = hr[j];
+            a.ckpt_n[off + i] = nr[j];
+          }
         }
+      }
+      for (int i = tid + IPT * nthr; i < nS; i += nthr) {
+        a.ckpt_m[off + i] = gm[i];
+        a.ckpt_h[off + i] = gh[i];
+        a.ckpt_n[off + i] = gn[i];
       }
       if (a.syn_model == 0 && a.ckpt_s != nullptr) {
         const int64_t nsyn = static_cast<int64_t>(tp.n_syn) << lS;
@@ -453,28 +485,40 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
       }
     }
     synapse_phase(a, pg, tp, Ld);
+    const int64_t t = pg.t0 + step;
+    if constexpr (IPT > 0) {
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int i = tid + j * nthr;
-      if (i < nS) {
-        const int k = i >> lS, s = i & (S - 1);
-        if (s < pg.nsim) {
-          build_item(a, pg, tp, Ld, vg, k, s, vr[j], pg.t0 + step, mr[j], hr[j], nr[j]);
-        } else {
-          idle_item(a, tp, Ld, k, i);
+      for (int j = 0; j < IPT; ++j) {
+        int i = tid + j * nthr;
+        // Opaque to the optimizer: the item's addresses are recomputed every step instead
+        // of being hoisted out of the time loop for all IPT items (register pressure).
+        asm volatile("" : "+v"(i));
+        if (i < nS) {
+          const int k = i >> lS, s = i & (S - 1);
+          if (s < pg.nsim) {
+            build_item(a, pg, tp, Ld, vg, k, s, vg[i], t, mr[j], hr[j], nr[j]);
+          } else {
+            idle_item(a, tp, Ld, k, i);
+          }
         }
       }
     }
-    __syncthreads();
-    tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds);
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int i = tid + j * nthr;
-      if (i < nS) {
-        vr[j] += static_cast<double>(Ld.xs[i]);
-        vg[i] = vr[j];
+    // Items beyond the register-resident ones: gates streamed from HBM.
+    for (int i = tid + IPT * nthr; i < nS; i += nthr) {
+      const int k = i >> lS, s = i & (S - 1);
+      if (s < pg.nsim) {
+        G m = gm[i], h = gh[i], n = gn[i];
+        build_item(a, pg, tp, Ld, vg, k, s, vg[i], t, m, h, n);
+        gm[i] = m;
+        gh[i] = h;
+        gn[i] = n;
+      } else {
+        idle_item(a, tp, Ld, k, i);
       }
     }
+    __syncthreads();
+    tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds, Ld.cl);
+    for (int i = tid; i < nS; i += nthr) vg[i] += static_cast<double>(Ld.xs[i]);
     __syncthreads();
     for (int i = tid; i < (tp.n_rec << lS); i += nthr) {
       const int r = i >> lS, s = i & (S - 1);
@@ -482,13 +526,15 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
           static_cast<G>(vg[(a.rec_node[tp.rb + r] << lS) + s]);
     }
   }
+  if constexpr (IPT > 0) {
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const int i = tid + j * nthr;
-    if (i < nS) {
-      a.gm[pg.soff + i] = mr[j];
-      a.gh[pg.soff + i] = hr[j];
-      a.gn[pg.soff + i] = nr[j];
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * nthr;
+      if (i < nS) {
+        gm[i] = mr[j];
+        gh[i] = hr[j];
+        gn[i] = nr[j];
+      }
     }
   }
 }
@@ -531,9 +577,9 @@ __device__ __forceinline__ T d_save_exp(T u) {
 // dx'/dv of the exponential-Euler gate step through alpha(v), beta(v); e = dx'/dx.
 template <typename T>
 __device__ __forceinline__ T d_gate_dv(T x, T dt, T al, T be, T dal, T dbe, T& e) {
-  const T tau = T(1) / (al + be);
+  const T tau = ddiv(T(1), al + be);
   const T xinf = al * tau;
-  e = save_exp(-dt / tau);
+  e = save_exp(-dt * (al + be));
   const T de = -dt * e;
   const T dxa = be * tau * tau * (T(1) - e) + (x - xinf) * de;
   const T dxb = -al * tau * tau * (T(1) - e) + (x - xinf) * de;
@@ -541,7 +587,7 @@ __device__ __forceinline__ T d_gate_dv(T x, T dt, T al, T be, T dal, T dbe, T& e
 }
 
 template <typename G, typename L>
-__global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
+__global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
@@ -553,6 +599,8 @@ __global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<
   L* lv = Ld.ds + nS;  // adjoint of the voltages
   Ld.tl = lv;          // unused (no synapses in the adjoint)
   Ld.tc = lv;
+  Ld.cl = lv + nS;
+  load_coefficients(a, tp, Ld.cl);
   const int tid = threadIdx.x, nthr = blockDim.x;
   const G dt = a.dt;
   const int64_t nsteps = pg.t1 - pg.t0;
@@ -600,7 +648,7 @@ __global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<
         wn[(j + 1) * nS + i] = n;
       }
       __syncthreads();
-      tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds);
+      tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds, Ld.cl);
       for (int i = tid; i < nS; i += nthr) wv[(j + 1) * nS + i] = wv[j * nS + i] + static_cast<double>(Ld.xs[i]);
       __syncthreads();
     }
@@ -626,10 +674,10 @@ __global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<
           if (flags & kHH) {
             const G mm = m1[i], hh = h1[i], nn = n1[i];
             const G n2 = nn * nn;
-            lin += G(1000) * (param_at(a, 0, pg, tp, k, s) * (mm * mm * mm) * hh +
-                              param_at(a, 1, pg, tp, k, s) * (n2 * n2) + param_at(a, 2, pg, tp, k, s));
+            lin += G(1000) * (prm<0>(a, pg, tp, k, s) * (mm * mm * mm) * hh +
+                              prm<1>(a, pg, tp, k, s) * (n2 * n2) + prm<2>(a, pg, tp, k, s));
           }
-          if (flags & kLeak) lin += G(1000) * param_at(a, 6, pg, tp, k, s);
+          if (flags & kLeak) lin += G(1000) * prm<6>(a, pg, tp, k, s);
           Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * static_cast<G>(nfk[kNfInvCm]) * lin);
         } else {
           Ld.ds[i] = nfk[kNfDbase] + ((flags & kComp) ? L(1) : L(0));
@@ -637,7 +685,7 @@ __global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<
         Ld.xs[i] = lv[i];
       }
       __syncthreads();
-      tree_solve<G, L, true>(a, tp, Ld.xs, Ld.ds);  // xs = mu = A^-T lambda_v(t+1)
+      tree_solve<G, L, true>(a, tp, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
       const double* v0 = wv + j * nS;
       const double* v1 = wv + (j + 1) * nS;
       const G* m0 = wm + j * nS;
@@ -658,11 +706,11 @@ __global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<
         const G gc = mu * dt * inv_cm;                           // dL/dcst
         G lvn = mu;
         if (flags & kHH) {
-          const G gNa = param_at(a, 0, pg, tp, k, s);
-          const G gK = param_at(a, 1, pg, tp, k, s);
-          const G eNa = param_at(a, 3, pg, tp, k, s);
-          const G eK = param_at(a, 4, pg, tp, k, s);
-          const G eL = param_at(a, 5, pg, tp, k, s);
+          const G gNa = prm<0>(a, pg, tp, k, s);
+          const G gK = prm<1>(a, pg, tp, k, s);
+          const G eNa = prm<3>(a, pg, tp, k, s);
+          const G eK = prm<4>(a, pg, tp, k, s);
+          const G eL = prm<5>(a, pg, tp, k, s);
           const G mm = m1[i], hh = h1[i], nn = n1[i];
           const G m3 = mm * mm * mm, n3 = nn * nn * nn, n4 = n3 * nn;
           const G wNa = G(1000) * (gl + gc * eNa);
@@ -674,7 +722,7 @@ __global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<
           if (g.g_param[2]) g.g_param[2][pi] += wL;
           if (g.g_param[3]) g.g_param[3][pi] += G(1000) * gc * gNa * m3 * hh;
           if (g.g_param[4]) g.g_param[4][pi] += G(1000) * gc * gK * n4;
-          if (g.g_param[5]) g.g_param[5][pi] += G(1000) * gc * param_at(a, 2, pg, tp, k, s);
+          if (g.g_param[5]) g.g_param[5][pi] += G(1000) * gc * prm<2>(a, pg, tp, k, s);
           const G lam_m1 = lm[i] + wNa * gNa * G(3) * mm * mm * hh;
           const G lam_h1 = lh[i] + wNa * gNa * m3;
           const G lam_n1 = ln[i] + wK * gK * G(4) * n3;
@@ -705,10 +753,10 @@ __global__ void __launch_bounds__(1024) fused_adjoint_kernel(Fused<G, L> a, Adj<
           ln[i] = lam_n1 * en;
         }
         if (flags & kLeak) {
-          const G eLk = param_at(a, 7, pg, tp, k, s);
+          const G eLk = prm<7>(a, pg, tp, k, s);
           const int64_t pi = pg.soff + i;
           if (g.g_param[6]) g.g_param[6][pi] += G(1000) * (gl + gc * eLk);
-          if (g.g_param[7]) g.g_param[7][pi] += G(1000) * gc * param_at(a, 6, pg, tp, k, s);
+          if (g.g_param[7]) g.g_param[7][pi] += G(1000) * gc * prm<6>(a, pg, tp, k, s);
         }
         lv[i] = static_cast<L>(lvn);
       }
@@ -757,15 +805,13 @@ int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
     break;                                                          \
   }
   switch (a.items_per_thread) {
+    JXHIP_FWD_CASE(0)
     JXHIP_FWD_CASE(1)
     JXHIP_FWD_CASE(2)
     JXHIP_FWD_CASE(4)
     JXHIP_FWD_CASE(8)
     JXHIP_FWD_CASE(12)
     JXHIP_FWD_CASE(16)
-    JXHIP_FWD_CASE(20)
-    JXHIP_FWD_CASE(24)
-    JXHIP_FWD_CASE(32)
     default:
       return hipErrorInvalidValue;
   }
@@ -781,6 +827,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (a.n_progs <= 0) return hipSuccess;
   if (int e = check_args(a)) return e;
   if (a.syn_model >= 0 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
+  if (a.threads > 512) return hipErrorInvalidValue;  // fused_adjoint_kernel's launch bound
   const Fused<G, L> f = unpack<G, L>(a);
   Adj<G> g;
   g.g_rec = static_cast<const G*>(adj->g_rec);
@@ -812,6 +859,15 @@ int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream) {
 
 int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream) {
   return jxhip::launch_forward<double, double>(args, static_cast<hipStream_t>(stream));
+}
+
+int jxhip_fused_forward_mixed(const jxhip_fused_args* args, void* stream) {
+  return jxhip::launch_forward<double, float>(args, static_cast<hipStream_t>(stream));
+}
+
+int jxhip_fused_adjoint_mixed(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
+                              void* stream) {
+  return jxhip::launch_adjoint<double, float>(args, adj, static_cast<hipStream_t>(stream));
 }
 
 int jxhip_fused_adjoint_f32(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,

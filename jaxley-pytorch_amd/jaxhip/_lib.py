@@ -34,7 +34,7 @@ class FusedArgs(C.Structure):
         ("seg_child", _p),
         ("node_i", _p), ("node_f", _p), ("child_idx", _p), ("stim_col", _p),
         ("stim_scale", _p), ("rec_node", _p),
-        ("param", _p * JXHIP_NPARAM), ("param_mode", _i32 * JXHIP_NPARAM),
+        ("node_p", _p), ("bparam", _p), ("bparam_mode", _i32),
         ("v", _p), ("gate_m", _p), ("gate_h", _p), ("gate_n", _p),
         ("stim", _p), ("stim_cols", _i64), ("stim_b", _i64),
         ("rec", _p),
@@ -71,8 +71,11 @@ _SIGS = {
     "jxhip_synapse_step_backward": ([_p, _p, _p, _p, _p, _p, "dt", "dt", _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _i32, _p], C.c_int),
     "jxhip_fused_forward": ([C.POINTER(FusedArgs), _p], C.c_int),
     "jxhip_fused_adjoint": ([C.POINTER(FusedArgs), C.POINTER(AdjointArgs), _p], C.c_int),
+    "jxhip_fused_forward_mixed": ([C.POINTER(FusedArgs), _p], C.c_int),
+    "jxhip_fused_adjoint_mixed": ([C.POINTER(FusedArgs), C.POINTER(AdjointArgs), _p], C.c_int),
 }
-TYPED = {k for k in _SIGS if k not in ("jxhip_version", "jxhip_error_string")}
+UNTYPED = ("jxhip_version", "jxhip_error_string", "jxhip_fused_forward_mixed", "jxhip_fused_adjoint_mixed")
+TYPED = {k for k in _SIGS if k not in UNTYPED}
 
 
 def exported_symbols():

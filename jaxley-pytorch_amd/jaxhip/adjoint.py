@@ -17,21 +17,20 @@ from typing import Dict, List, Tuple
 import torch
 
 from . import _lib
-from .fused import ALL_PARAMS, ForwardLaunch, from_prog, to_prog
+from .fused import ALL_PARAMS, BATCHABLE, ForwardLaunch, from_prog, to_prog
 
 
 class _FusedIntegrate(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, holder, v0, *param_tensors):
-        st, node_f, modes, names, gates0, stim, stim_scale, T, dt, B, cfg, K = holder["args"]
-        params = {k: (m, t.detach()) for k, m, t in zip(names, modes, param_tensors)}
-        run = ForwardLaunch(st, node_f, params, v0.detach(), tuple(g.detach() for g in gates0),
-                            stim, stim_scale, T, dt, B, cfg, ckpt_every=K)
+    def forward(ctx, holder, v0, node_p, bparam):
+        st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K = holder["args"]
+        run = ForwardLaunch(st, node_f, node_p.detach(), None if bparam is None else bparam.detach(),
+                            bmode, v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
+                            T, dt, B, cfg, ckpt_every=K)
         run.launch()
         holder["run"] = run
         ctx.holder = holder
-        ctx.modes = modes
-        ctx.names = names
+        ctx.bmode = bmode
         return run.recordings()
 
     @staticmethod
@@ -50,15 +49,13 @@ class _FusedIntegrate(torch.autograd.Function):
         work += [torch.empty((P, K + 1, n, S), dtype=dtype, device=dev) for _ in range(3)]
         lam = [torch.zeros((P, n, S), dtype=dtype, device=dev) for _ in range(3)]
         g_v0 = torch.zeros((P, n, S), dtype=dtype, device=dev)
-        gpar = {}
+        need_p = ctx.needs_input_grad[2] or ctx.needs_input_grad[3]
+        gpar = [torch.zeros((P, n, S), dtype=dtype, device=dev) if need_p else None
+                for _ in ALL_PARAMS]
         adj = _lib.AdjointArgs()
         adj.g_rec = gr.data_ptr()
-        for j, name in enumerate(ALL_PARAMS):
-            if name in ctx.names and ctx.needs_input_grad[2 + ctx.names.index(name)]:
-                gpar[name] = torch.zeros((P, n, S), dtype=dtype, device=dev)
-                adj.g_param[j] = gpar[name].data_ptr()
-            else:
-                adj.g_param[j] = 0
+        for j in range(len(ALL_PARAMS)):
+            adj.g_param[j] = gpar[j].data_ptr() if gpar[j] is not None else 0
         adj.g_v0 = g_v0.data_ptr()
         adj.work_v, adj.work_m, adj.work_h, adj.work_n = (w.data_ptr() for w in work)
         adj.lam_m, adj.lam_h, adj.lam_n = (l.data_ptr() for l in lam)
@@ -66,35 +63,32 @@ class _FusedIntegrate(torch.autograd.Function):
         adj.n_steps = T
         args = run.args
         saved_lds = args.lds_bytes
-        args.lds_bytes = 3 * n * S * run.node_f.element_size()
-        f = _lib.fn("jxhip_fused_adjoint", dtype)
+        args.lds_bytes = (3 * n * S + 2 * n) * run.node_f.element_size()
+        f = run.kernel_name("jxhip_fused_adjoint")
         try:
             _lib.check(f(args, adj, _lib.stream_ptr(dev)), "fused_adjoint")
         finally:
             args.lds_bytes = saved_lds
-        grads: List = []
-        for name, mode in zip(ctx.names, ctx.modes):
-            gp = gpar.get(name)
-            if gp is None:
-                grads.append(None)
-            elif mode == 0:
-                grads.append(gp.sum(dim=(0, 2)))          # shared per node: sum over sims
+        g_node_p = g_bparam = None
+        if ctx.needs_input_grad[2]:
+            g_node_p = torch.stack([g.sum(dim=(0, 2)) for g in gpar], 1)
+        if ctx.needs_input_grad[3]:
+            cols = [gpar[ALL_PARAMS.index(k)] for k in BATCHABLE]
+            if ctx.bmode == 1:
+                g_bparam = torch.stack([c.sum(1).reshape(-1) for c in cols], -1)
             else:
-                grads.append(gp.reshape(-1))              # per (node, sim): program layout
+                g_bparam = torch.stack(cols, -1)
         gv = from_prog(g_v0, run.B) if ctx.needs_input_grad[1] else None
         holder.pop("run", None)
-        return (None, gv, *grads)
+        return None, gv, g_node_p, g_bparam
 
 
-def fused_integrate_with_adjoint(st, node_f, params: Dict[str, Tuple[int, torch.Tensor]], param_inputs,
-                                 v0, gates0, stim, stim_scale, T, dt, B, cfg, K, batch, comp_to_solve):
+def fused_integrate_with_adjoint(st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale,
+                                 T, dt, B, cfg, K):
     """Differentiable fused simulation: returns `(recordings (B, n_rec, T+1), run)`."""
     if any(g.requires_grad for g in gates0):
         raise NotImplementedError("gradients with respect to initial gate states are not supported")
     K = int(max(1, min(K, T)))
-    names = [k for k in ALL_PARAMS if k in params]
-    modes = [params[k][0] for k in names]
-    tensors = [params[k][1] for k in names]
-    holder = {"args": (st, node_f, modes, names, gates0, stim, stim_scale, T, dt, B, cfg, K)}
-    recs = _FusedIntegrate.apply(holder, v0, *tensors)
+    holder = {"args": (st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K)}
+    recs = _FusedIntegrate.apply(holder, v0, node_p, bparam)
     return recs, holder.get("run")

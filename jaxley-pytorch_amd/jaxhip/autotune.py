@@ -68,26 +68,31 @@ def choose_tree_kernel(parent, device, forest, levels):
     return TreeSolveKernel(forest, device) if variant == "registers" else LevelTreeSolveKernel(levels, device)
 
 
-def fused_config(st, B: int, dtype):
-    """Launch shape of the fused forward for a structure and batch size."""
-    from .fused import LaunchConfig, default_config, lds_bytes
+def _key(st, B, dtype, lsize, grad):
+    n_tgt = st.syn["n_tgt"] if st.syn else 0
+    return ("fused", st.n, st.L, n_tgt, int(st.topo_row.sum()), B, str(dtype), lsize, bool(grad))
+
+
+def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False):
+    """Launch shape of the fused forward for a structure and batch size. With `grad` the
+    shape is shared with the reverse pass and obeys its limits."""
+    from .fused import ADJOINT_THREADS_MAX, LaunchConfig, default_config, grad_config
 
     n_tgt = st.syn["n_tgt"] if st.syn else 0
     esize = torch.tensor([], dtype=dtype).element_size()
-    key = ("fused", st.n, st.L, n_tgt, int(st.topo_row.sum()), B, str(dtype))
+    key = _key(st, B, dtype, lsize, grad)
     if key in _CACHE:
         return _CACHE[key]
-    cfg = default_config(st.n, n_tgt, B, esize)
+    lsz = lsize if lsize else esize
+    cfg = grad_config(st.n, B, lsz) if grad else default_config(st.n, n_tgt, B, lsz)
     env = os.environ.get("JAXHIP_FUSED_CONFIG")
-    if env:
+    if env and not grad:
         S, thr = (int(x) for x in env.split(","))
         cfg = LaunchConfig(S, thr)
     _CACHE[key] = cfg
     return cfg
 
 
-def set_fused_config(st, B: int, dtype, cfg) -> None:
+def set_fused_config(st, B: int, dtype, cfg, lsize: int = 0, grad: bool = False) -> None:
     """Record a measured launch shape (used by the serving loop and the benchmark)."""
-    n_tgt = st.syn["n_tgt"] if st.syn else 0
-    key = ("fused", st.n, st.L, n_tgt, int(st.topo_row.sum()), B, str(dtype))
-    _CACHE[key] = cfg
+    _CACHE[_key(st, B, dtype, lsize, grad)] = cfg

@@ -25,6 +25,9 @@ from .schedule import check_forest, segments
 HH_PARAMS = ("HH_gNa", "HH_gK", "HH_gLeak", "HH_eNa", "HH_eK", "HH_eLeak")
 LEAK_PARAMS = ("Leak_gLeak", "Leak_eLeak")
 ALL_PARAMS = HH_PARAMS + LEAK_PARAMS
+# Parameters that may carry batch dimensions in the fused kernel, and their column in
+# the batched table.
+BATCHABLE = {"HH_gNa": 0, "HH_gK": 1, "HH_gLeak": 2, "Leak_gLeak": 3}
 FLAG_COMP, FLAG_HH, FLAG_LEAK = 1, 2, 4
 SYN_DIFF = 1e-3  # mV, the finite difference of the synapse slope (network.py:29)
 
@@ -193,42 +196,68 @@ class LaunchConfig:
     threads: int
 
 
-IPT_CHOICES = (1, 2, 4, 8, 12, 16, 20, 24, 32)
+IPT_CHOICES = (0, 1, 2, 4, 8, 12, 16)
+# Register-resident items per thread that fit the VGPR budget (no spills), by the
+# element sizes of (gates, solve).
+IPT_MAX = {(4, 4): 16, (8, 8): 2, (8, 4): 2}
 
 
 def lds_bytes(n: int, n_tgt: int, S: int, esize: int) -> int:
-    return (2 * n + 2 * n_tgt) * S * esize
+    """Forward: diagonal and rhs per (node, simulation), synapse terms per (target,
+    simulation), lower / upper coefficient per node."""
+    return (2 * n + 2 * n_tgt) * S * esize + 2 * n * esize
 
 
-def items_per_thread(n: int, S: int, threads: int) -> int:
+def items_per_thread(n: int, S: int, threads: int, gsize: int, lsize: int) -> int:
+    """Largest register-resident item count <= the items of a thread and the budget."""
     need = -(-n * S // threads)
+    best = 0
     for c in IPT_CHOICES:
-        if c >= need:
-            return c
-    raise NotImplementedError(f"{n} nodes x {S} simulations need more than 32 items per thread")
+        if c <= min(need, IPT_MAX[(gsize, lsize)]):
+            best = c
+    return best
 
 
 def default_config(n: int, n_tgt: int, B: int, esize: int) -> LaunchConfig:
-    """A launch shape: enough programs to fill the chip, LDS per program <= 40 KiB,
-    at most 24 register-resident items per thread."""
+    """A launch shape. Simulations per program S as large as the LDS allows (one program
+    per CU holds the tree's coefficients once for S simulations) while the batch still
+    fills the 256 CUs; then enough threads (<= 1024) for the items."""
     S = 1
-    while S < 64 and S * 2 <= max(1, B // 1024) and lds_bytes(n, n_tgt, S * 2, esize) <= 40 * 1024:
+    while (S < 64 and (S * 2) * 256 <= max(B, 256) and lds_bytes(n, n_tgt, S * 2, esize) <= 160 * 1024):
         S *= 2
-    while S > 1 and lds_bytes(n, n_tgt, S, esize) > 160 * 1024:
-        S //= 2
     if lds_bytes(n, n_tgt, S, esize) > 160 * 1024:
         raise NotImplementedError(f"a topology of {n} nodes does not fit the fused kernel's LDS")
-    threads = 64
-    while threads < 1024 and -(-n * S // threads) > 20:
-        threads *= 2
-    threads = max(threads, min(256, 64 * -(-n * S // 64)))
-    return LaunchConfig(S, int(min(threads, 1024)))
+    threads = int(min(1024, max(64, 64 * -(-n * S // (64 * 16)))))
+    return LaunchConfig(S, threads)
+
+
+ADJOINT_THREADS_MAX = 512
+
+
+def adjoint_lds_bytes(n: int, S: int, esize: int) -> int:
+    """LDS of one reverse-pass program: right-hand side, pivots and voltages per
+    (node, simulation) plus the tree's two coupling coefficients per node."""
+    return (3 * n * S + 2 * n) * esize
+
+
+def grad_config(n: int, B: int, esize: int) -> LaunchConfig:
+    """Launch shape shared by the checkpointed forward and the reverse pass: S bounded by
+    the reverse pass's LDS, threads by its launch bound."""
+    S = 1
+    while (S < 64 and (S * 2) * 256 <= max(B, 256)
+           and adjoint_lds_bytes(n, S * 2, esize) <= 160 * 1024):
+        S *= 2
+    if adjoint_lds_bytes(n, S, esize) > 160 * 1024:
+        raise NotImplementedError(f"a topology of {n} nodes does not fit the reverse pass's LDS")
+    threads = int(min(ADJOINT_THREADS_MAX, max(64, 64 * -(-n * S // (64 * 16)))))
+    return LaunchConfig(S, threads)
 
 
 class ForwardLaunch:
     """All device buffers and the argument block of one fused forward launch."""
 
-    def __init__(self, st: Structure, node_f: torch.Tensor, params: Dict[str, Tuple[int, torch.Tensor]],
+    def __init__(self, st: Structure, node_f: torch.Tensor, node_p: torch.Tensor,
+                 bparam: Optional[torch.Tensor], bmode: int,
                  v0: torch.Tensor, gates0: Tuple[torch.Tensor, ...], stim: Optional[torch.Tensor],
                  stim_scale: Optional[torch.Tensor], T: int, dt: float, B: int, cfg: LaunchConfig,
                  syn_params: Optional[List[torch.Tensor]] = None, syn_scale: Optional[torch.Tensor] = None,
@@ -254,7 +283,9 @@ class ForwardLaunch:
         self.progs = torch.as_tensor(progs, device=dev)
         self.topos = torch.as_tensor(st.topo_row[None], device=dev)
         self.node_f = node_f
-        self.params = params
+        self.node_p = node_p
+        self.bparam = bparam
+        self.bmode = bmode
         self.stim = stim
         self.stim_scale = stim_scale
         self.syn_params = syn_params
@@ -283,10 +314,9 @@ class ForwardLaunch:
         a.stim_col = st.stim_col.data_ptr()
         a.rec_node = st.rec_node.data_ptr()
         self._dummy = torch.zeros(1, dtype=dtype, device=dev)
-        for j, name in enumerate(ALL_PARAMS):
-            mode, t = params.get(name, (0, self._dummy))
-            a.param[j] = t.data_ptr()
-            a.param_mode[j] = mode
+        a.node_p = node_p.data_ptr()
+        a.bparam = bparam.data_ptr() if bparam is not None else self._dummy.data_ptr()
+        a.bparam_mode = bmode if bparam is not None else 0
         a.v = self.v.data_ptr()
         a.gate_m, a.gate_h, a.gate_n = self.gm.data_ptr(), self.gh.data_ptr(), self.gn.data_ptr()
         if stim is not None:
@@ -316,7 +346,8 @@ class ForwardLaunch:
         a.diff = SYN_DIFF
         a.S = S
         a.threads = cfg.threads
-        a.items_per_thread = items_per_thread(n, S, cfg.threads)
+        a.items_per_thread = items_per_thread(n, S, cfg.threads, self.gm.element_size(),
+                                              node_f.element_size())
         n_tgt = st.syn["n_tgt"] if st.syn else 0
         a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size())
         if ckpt_every > 0:
@@ -327,8 +358,13 @@ class ForwardLaunch:
             a.ckpt_syn_stride = self.syn_s.numel() if self.syn_s is not None else 0
         self.args = a
 
+    def kernel_name(self, base: str):
+        if self.dtype == torch.float64 and self.node_f.dtype == torch.float32:
+            return _lib.fn(base + "_mixed")
+        return _lib.fn(base, self.dtype)
+
     def launch(self):
-        f = _lib.fn("jxhip_fused_forward", self.dtype)
+        f = self.kernel_name("jxhip_fused_forward")
         _lib.check(f(self.args, _lib.stream_ptr(self.v.device)), "fused_forward")
 
     def recordings(self) -> torch.Tensor:

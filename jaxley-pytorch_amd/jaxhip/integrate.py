@@ -16,13 +16,14 @@ inputs on the CPU raise.
 from __future__ import annotations
 
 import math
+import os
 from typing import Any, Dict, List, Optional, Tuple
 
 import numpy as np
 import torch
 
 from . import _lib
-from .fused import (ALL_PARAMS, HH_PARAMS, LEAK_PARAMS, ForwardLaunch, LaunchConfig,
+from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, ForwardLaunch, LaunchConfig,
                     build_structure, default_config, node_float_table, to_prog)
 from .geometry import axial_conductances, comp_geometry
 
@@ -192,8 +193,24 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     return _integrate_fused(module, inp, delta_t, checkpoint_lengths, return_states, device, dtype)
 
 
-def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device, dtype):
-    vals = inp.vals
+_F32_MODE = os.environ.get("JAXHIP_F32_MODE", "mixed")
+
+
+def set_float32_mode(mode: str) -> None:
+    """How float32 simulations run: "mixed" (default) keeps gates, rates and voltages
+    in float64 and solves the tree on float32 increments, which meets float64-level
+    accuracy (<= 1e-2 mV); "fast" runs gates and rates in float32 (faster, errors up to
+    tenths of a mV on spiking cells over thousands of steps)."""
+    global _F32_MODE
+    if mode not in ("mixed", "fast"):
+        raise ValueError(f"unknown float32 mode {mode!r}")
+    _F32_MODE = mode
+
+
+def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device, user_dtype):
+    # Element type of gates, rates, parameters and recordings in the kernel.
+    dtype = torch.float64 if (user_dtype == torch.float32 and _F32_MODE == "mixed") else user_dtype
+    vals = {k: v.to(dtype) for k, v in inp.vals.items()}
     rec_states = {s for s, _ in module.recordings}
     if rec_states != {"v"}:
         raise NotImplementedError("the fused kernel records voltages ('v') only")
@@ -216,7 +233,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
     g = axial_conductances(tables, module._comp_node, module._n_nodes, geo, vals["axial_resistivity"], vals["capacitance"])
     cm_node = torch.full((module._n_nodes,), -1.0, dtype=dtype, device=device)
     cm_node[torch.as_tensor(module._comp_node, device=device)] = vals["capacitance"]
-    node_f = node_float_table(st, g, cm_node, dt, dtype)
+    node_f = node_float_table(st, g, cm_node, dt, user_dtype)
 
     # Batch: parameters, states, stimulus.
     chan_keys = [k for k in ALL_PARAMS if k in vals]
@@ -252,19 +269,54 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
     n_tgt = st.syn["n_tgt"] if st.syn else 0
     from .autotune import fused_config
 
-    cfg = fused_config(st, B, dtype)
+    needs_grad = torch.is_grad_enabled() and any(
+        t.requires_grad for t in [vals[k] for k in chan_keys] + [init["v"]]
+    )
+    cfg = fused_config(st, B, dtype, torch.tensor([], dtype=user_dtype).element_size(), needs_grad)
     S = cfg.S
-    # Channel parameters and their modes.
-    params = {}
-    param_inputs = {}
-    for k in chan_keys:
-        x = vals[k]
-        if x.ndim == 1:
-            params[k] = (0, comp_to_kernel(x.detach() if not x.requires_grad else x).contiguous())
+    # Channel parameters: a per-node table (kernel order) and, when some conductance
+    # carries batch dimensions, a batched table of the four conductances.
+    def kernel_cols(x):
+        return torch.nan_to_num(comp_to_kernel(x), nan=0.0)
+
+    zero_n = torch.zeros(n, dtype=dtype, device=device)
+    batched = [k for k in ALL_PARAMS if k in vals and vals[k].ndim > 1]
+    for k in batched:
+        if k not in BATCHABLE:
+            raise NotImplementedError(f"batched {k!r} is not supported by the fused kernel")
+    bmode, bparam = 0, None
+    if batched:
+        tabs = []
+        for k in BATCHABLE:
+            x = vals.get(k)
+            if x is None:
+                tabs.append(torch.zeros(B, n, dtype=dtype, device=device))
+            else:
+                tabs.append(kernel_cols(torch.broadcast_to(x, (*batch, x.shape[-1])).reshape(B, -1)))
+        stack = torch.stack(tabs, -1)                                     # (B, n, 4)
+        chan = torch.as_tensor(np.asarray(st.node_i.cpu())[:, 3] & 6 != 0, device=device)
+        first = int(torch.nonzero(chan)[0]) if bool(chan.any()) else 0
+        with torch.no_grad():
+            per_sim = bool(((stack - stack[:, first:first + 1]).abs()[:, chan] == 0).all())
+        if per_sim:
+            bmode = 1
+            bsim = stack[:, first]                                        # (B, 4)
+            P = -(-B // S)
+            if P * S != B:
+                bsim = torch.cat([bsim, bsim.new_zeros(P * S - B, 4)], 0)
+            bparam = bsim.contiguous()
         else:
-            x2 = torch.broadcast_to(x, (*batch, x.shape[-1])).reshape(B, -1)
-            params[k] = (2, to_prog(comp_to_kernel(x2), S).reshape(-1))
-        param_inputs[k] = x
+            bmode = 2
+            bparam = torch.stack([to_prog(stack[..., c], S) for c in range(4)], -1).contiguous()
+    cols = []
+    for j, k in enumerate(ALL_PARAMS):
+        x = vals.get(k)
+        if x is None or (bmode and k in BATCHABLE):
+            cols.append(zero_n)
+        else:
+            cols.append(kernel_cols(x))
+    node_p = torch.stack(cols, 1).contiguous()
+    param_inputs = {k: vals[k] for k in chan_keys}
     # States.
     v0 = torch.broadcast_to(init["v"][..., bp_src], (*batch, n)).reshape(B, n)
     gates0 = []
@@ -278,7 +330,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
     # Stimulus (T, n_stim, stim_b) and per-entry scale dt * 1e5 / (area cm).
     stim = stim_scale = None
     if inp.stim is not None:
-        s = inp.stim
+        s = inp.stim.to(dtype)
         sb = s.shape[:-2]
         if sb:
             s = torch.broadcast_to(s, (*batch, *s.shape[-2:])).reshape(B, *s.shape[-2:])
@@ -298,7 +350,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         perm = torch.as_tensor(st.syn["perm"], device=device)
         syn_params = []
         for name in stype.param_names:
-            x = inp.syn_vals[f"{stype.name}_{name}"]
+            x = inp.syn_vals[f"{stype.name}_{name}"].to(dtype)
             if x.ndim > 1:
                 raise NotImplementedError("batched synapse parameters are not supported by the fused kernel")
             syn_params.append(x[perm].contiguous())
@@ -306,14 +358,11 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         area_node[torch.as_tensor(module._comp_node, device=device)] = geo["area"]
         syn_scale = (1e5 / area_node[torch.as_tensor(st.syn["post_node"], device=device)]).contiguous()
         if st.syn["model"] == 0:
-            s0 = inp.syn_vals[f"{stype.name}_s"]
+            s0 = inp.syn_vals[f"{stype.name}_s"].to(dtype)
             if inp.all_states is not None and f"{stype.name}_s" in inp.all_states:
-                s0 = inp.all_states[f"{stype.name}_s"]
+                s0 = inp.all_states[f"{stype.name}_s"].to(dtype)
             s0 = torch.broadcast_to(s0, (*batch, s0.shape[-1])).reshape(B, -1)
             syn_s0 = s0[:, perm]
-    needs_grad = torch.is_grad_enabled() and any(
-        t.requires_grad for t in list(param_inputs.values()) + [init["v"]]
-    )
     if needs_grad:
         from .adjoint import fused_integrate_with_adjoint
 
@@ -322,20 +371,19 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
                                       "without synapses (stimulus without gradient)")
         K = checkpoint_lengths[-1] if checkpoint_lengths else 100
         recs, final = fused_integrate_with_adjoint(
-            st, node_f, params, param_inputs, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K,
-            batch, comp_to_kernel)
+            st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K)
     else:
         with torch.no_grad():
-            run = ForwardLaunch(st, node_f, {k: (m, t.detach()) for k, (m, t) in params.items()},
-                                v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
+            run = ForwardLaunch(st, node_f, node_p.detach(), None if bparam is None else bparam.detach(),
+                                bmode, v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
                                 inp.T, dt, B, cfg, syn_params, syn_scale, syn_s0)
             run.launch()
             recs = run.recordings()
             final = run
-    recs = recs.reshape(*batch, st.n_rec, inp.T + 1)
+    recs = recs.to(user_dtype).reshape(*batch, st.n_rec, inp.T + 1)
     if not return_states:
         return recs
-    return recs, _final_states(module, st, final, batch, B, dtype, device)
+    return recs, _final_states(module, st, final, batch, B, user_dtype, device)
 
 
 def _final_states(module, st, run, batch, B, dtype, device):

@@ -48,10 +48,10 @@ def run_forward_model(run):
     n_rec = st.n_rec
     S = run.S
     dt = run.args.dt
-    params = {}
-    for j, name in enumerate(ALL_PARAMS):
-        mode, t = run.params.get(name, (0, None))
-        params[j] = (mode, None if t is None else _np(t).astype(np.float64))
+    node_p = _np(run.node_p).astype(np.float64)
+    bmode = run.bmode if run.bparam is not None else 0
+    bparam = None if run.bparam is None else _np(run.bparam).astype(np.float64).reshape(-1, 4)
+    bcol = {0: 0, 1: 1, 2: 2, 6: 3}
     v = _np(run.v).astype(np.float64).reshape(-1)
     gm = _np(run.gm).astype(np.float64).reshape(-1)
     gh = _np(run.gh).astype(np.float64).reshape(-1)
@@ -63,12 +63,11 @@ def run_forward_model(run):
     ss = np.arange(S)
 
     def P(j, pg, k):
-        mode, arr = params[j]
-        if mode == 0:
-            return np.full(S, arr[k])
-        if mode == 1:
-            return arr[pg[5] + ss]
-        return arr[pg[4] + k * S + ss]
+        if bmode and j in bcol:
+            if bmode == 1:
+                return bparam[pg[5] + ss, bcol[j]]
+            return bparam[pg[4] + k * S + ss, bcol[j]]
+        return np.full(S, node_p[k, j])
 
     for pg in progs:
         nsim, t0, t1, soff, simoff, roff = pg[1], pg[2], pg[3], pg[4], pg[5], pg[6]
