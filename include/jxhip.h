@@ -181,7 +181,9 @@ int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int3
  * by segment level, leaf segments first. A segment's level is 0 if its bottom node has
  * no children, else 1 + the largest level of its child segments.
  * Per-node int32 (JXHIP_NODE_ICOLS) at node_base + k: parent (-1 root), child_beg,
- *   child_end (into child_idx at child_base), flags (1 comp, 2 HH, 4 Leak), stim_beg,
+ *   child_end (into child_idx at child_base), flags (1 comp, 2 HH, 4 Leak, 8 export:
+ *   voltage copied to HBM every step; set for recorded nodes and synapse pre- and
+ *   postsynaptic nodes), stim_beg,
  *   stim_end (stimulus entries), target index (-1 none), 0
  * Per-node solve type (JXHIP_NODE_FCOLS): lo = -dt g(parent->k), up = -dt g(k->parent),
  *   dbase = [comp] + sum of dt g over edges into k, inv_cm (0 at branchpoints)
@@ -243,9 +245,12 @@ typedef struct jxhip_fused_args {
   double diff;              /* finite difference of the synapse slope (1e-3) */
   int32_t S;                /* simulations per program: 1, 2, 4, ..., 64 */
   int32_t threads;          /* threads per workgroup: multiple of 64, <= 1024 */
-  int32_t items_per_thread; /* forward: (node, simulation) items per thread whose gates
-                               stay in registers: 0, 1, 2, 4, 8, 12 or 16 (further items
-                               stream their gates from HBM) */
+  int32_t items_per_thread; /* forward: 0 = streaming (state read from and written to
+                               HBM every step); > 0 = every thread keeps that many
+                               (node, simulation) items in VGPRs for the whole launch:
+                               float32 gates 1, 2, 3, 4, 6, 8, 10, 12 (threads <=
+                               1024) or 16 (threads <= 512); float64 gates 1, 2, 3, 4
+                               (<= 1024) or 6, 8, 12 (<= 512) */
   int64_t lds_bytes;        /* dynamic LDS per workgroup (<= 160 KiB) */
   /* checkpoints (for the adjoint): per program, snapshot c of the state before step
      c * ckpt_every at ckpt_* + c * ckpt_stride + state_off (syn: ckpt_syn_stride,
@@ -258,6 +263,12 @@ typedef struct jxhip_fused_args {
   int64_t ckpt_every;
   int64_t ckpt_stride;
   int64_t ckpt_syn_stride;
+  int64_t max_items;        /* largest n * S over the programs of the launch; with
+                               items_per_thread > 0 it must be <= items_per_thread *
+                               threads (else the call fails with an invalid argument) */
+  int32_t sched_global;     /* forward, streaming only: 1 = the solve schedule is read
+                               from HBM / L2 (lds_bytes then excludes it), 0 = it is
+                               staged in LDS */
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

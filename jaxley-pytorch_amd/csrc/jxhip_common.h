@@ -30,9 +30,18 @@ __device__ __forceinline__ float dexpm1(float x) {
 }
 __device__ __forceinline__ float ddiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 #endif
+// float64 reciprocal: the hardware estimate refined by two Newton steps (within 2 ulp;
+// the operands here are never 0, infinite or subnormal).
+__device__ __forceinline__ double drcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 __device__ __forceinline__ double dexpm1(double x) { return expm1(x); }
-__device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
+__device__ __forceinline__ double ddiv(double a, double b) { return a * drcp(b); }
 
 template <typename T>
 __device__ __forceinline__ T save_exp(T x) {
@@ -71,6 +80,47 @@ __device__ __forceinline__ void hh_rates(T v, T& am, T& bm, T& ah, T& bh, T& an,
   bh = ddiv(T(1.0), save_exp(-(v + T(35)) * T(0.1)) + T(1));
   an = T(0.01) * x_over_expm1(-(v + T(55)), T(10));
   bn = T(0.125) * save_exp(-(v + T(65)) * T(1.0 / 80));
+}
+
+// float64 HH rates from one exponential. With u = -(v + 65) every exponent of the six
+// rates is a multiple of u / 720 plus a constant:
+//   exp(u/80) = E^9, exp(u/20) = E^36, exp(u/18) = E^40, exp(u/10) = E^72, E = exp(u/720),
+//   and -(v+35)/10 = u/10 + 3, -(v+40)/10 = u/10 + 5/2, -(v+55)/10 = u/10 + 1,
+// so the powers replace five exponentials (relative error below 72 ulp of E, < 1e-14).
+// Where x / expm1(z) would cancel (|z| < 1/4) expm1 is evaluated directly. The clamps of
+// save_exp / x_over_expm1 at 20 are inactive for u <= 170 (v >= -235 mV); other voltages
+// take the direct formulas.
+__device__ __forceinline__ void hh_rates_f64(double v, double& am, double& bm, double& ah,
+                                             double& bh, double& an, double& bn) {
+  const double u = -(v + 65.0);
+  if (!(u <= 170.0)) {
+    hh_rates(v, am, bm, ah, bh, an, bn);
+    return;
+  }
+  const double e1 = exp(u * (1.0 / 720));
+  const double e2 = e1 * e1, e4 = e2 * e2, e8 = e4 * e4, e9 = e8 * e1;
+  const double e18 = e9 * e9, e36 = e18 * e18, e40 = e36 * e4, e72 = e36 * e36;
+  constexpr double kExp1 = 2.718281828459045235;    // e
+  constexpr double kExp25 = 12.18249396070347343;   // e^(5/2)
+  constexpr double kExp3 = 20.08553692318766774;    // e^3
+  bm = 4.0 * e40;
+  ah = 0.07 * e36;
+  bn = 0.125 * e9;
+  bh = drcp(e72 * kExp3 + 1.0);
+  const double xm = -(v + 40.0), xn = -(v + 55.0);
+  am = fabs(xm) < 2.5 ? 0.1 * x_over_expm1(xm, 10.0) : 0.1 * xm * drcp(e72 * kExp25 - 1.0);
+  an = fabs(xn) < 2.5 ? 0.01 * x_over_expm1(xn, 10.0) : 0.01 * xn * drcp(e72 * kExp1 - 1.0);
+}
+
+// The rates as the fused kernels evaluate them: the shared-exponential form in float64,
+// the direct form in float32 (where powers of E would lose accuracy).
+__device__ __forceinline__ void hh_rates_step(float v, float& am, float& bm, float& ah, float& bh,
+                                              float& an, float& bn) {
+  hh_rates(v, am, bm, ah, bh, an, bn);
+}
+__device__ __forceinline__ void hh_rates_step(double v, double& am, double& bm, double& ah,
+                                              double& bh, double& an, double& bn) {
+  hh_rates_f64(v, am, bm, ah, bh, an, bn);
 }
 
 }  // namespace jxhip

@@ -1,4 +1,5 @@
-// Fused whole-simulation kernels for gfx950.
+// Fused whole-simulation kernels for gfx950 (templates; the entry points for each
+// element type are in jxhip_fused_{f32,f64,mixed}.hip).
 //
 // Every workgroup ("program") owns S simulations of one topology (a cell, a network of
 // cells, or a pack of trees) for a range of time steps and runs, per step,
@@ -24,6 +25,7 @@
 // In the forward kernel every thread owns the same (node, simulation) items for the
 // whole run, so gates and voltages of its items stay in VGPRs; a float64 copy of the
 // voltages in HBM gives neighbours' voltages to the axial currents.
+#pragma once
 #include "jxhip_common.h"
 #include "../../include/jxhip.h"
 
@@ -32,7 +34,7 @@ namespace jxhip {
 // Column layout of the tables (see jxhip.h).
 enum : int { kNiParent = 0, kNiChildBeg, kNiChildEnd, kNiFlags, kNiStimBeg, kNiStimEnd, kNiTgt };
 enum : int { kNfLo = 0, kNfUp, kNfDbase, kNfInvCm };
-enum : int32_t { kComp = 1, kHH = 2, kLeak = 4 };
+enum : int32_t { kComp = 1, kHH = 2, kLeak = 4, kExport = 8 };
 
 template <typename G, typename L>
 struct Fused {
@@ -170,10 +172,14 @@ __device__ __forceinline__ Topo load_topo(const int32_t* topos, int32_t t) {
   return o;
 }
 
-// Reciprocal: hardware rcp in float32 (the operands are increments and diagonals of a
-// diagonally dominant system; 1 ulp is far below the tolerance), IEEE in float64.
+// Reciprocal of a pivot (diagonals of a diagonally dominant system, > 0): hardware rcp
+// in float32, hardware estimate + two Newton steps in float64.
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ double recip(double x) { return drcp(x); }
+
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
 
 // Channel parameter J of node k, simulation s: the batched conductances (J = 0, 1, 2:
 // gNa, gK, gLeak of HH; J = 6: gLeak of Leak) come from `bparam` when batched, every
@@ -243,6 +249,44 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
   cur = sc;
 }
 
+// Solve schedule of a topology staged in LDS: segment level pointers (L + 2), segments
+// (beg, end, child_beg, child_end), the parent of every segment top, the child segment
+// tops.
+struct Sched {
+  const int* lptr;
+  const int4* sq;
+  const int* sp;        // NULL: the parent comes from the node table ni
+  const int* sch;
+  const int32_t* ni;
+  int lw;  // levels lw .. L hold at most 64 (segment, simulation) pairs each
+};
+
+// First level from which every level holds at most 64 (segment, simulation) pairs.
+template <typename G, typename L>
+__device__ __forceinline__ int wave_levels(const Fused<G, L>& a, const Topo& tp) {
+  int lw = tp.L + 1;
+  while (lw > 0 && ((a.seg_lptr[tp.lb + lw] - a.seg_lptr[tp.lb + lw - 1]) << a.log2S) <= 64) --lw;
+  return lw;
+}
+
+// The schedule read in place from HBM / L2 (when the LDS has no room for it).
+template <typename G, typename L>
+__device__ __forceinline__ Sched global_schedule(const Fused<G, L>& a, const Topo& tp) {
+  Sched sc;
+  sc.lptr = a.seg_lptr + tp.lb;
+  sc.sq = reinterpret_cast<const int4*>(a.seg + static_cast<int64_t>(tp.sb) * 4);
+  sc.sp = nullptr;
+  sc.sch = a.seg_child + tp.scb;
+  sc.ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
+  sc.lw = wave_levels(a, tp);
+  return sc;
+}
+
+// Bytes of a staged schedule (a multiple of 16).
+__host__ __device__ __forceinline__ int sched_bytes(int nseg, int L, int ncs) {
+  return (16 * nseg + 4 * (L + 2 + nseg + ncs) + 15) & ~15;
+}
+
 // LDS of a program.
 template <typename L>
 struct Lds {
@@ -251,65 +295,197 @@ struct Lds {
   L* tl;   // synapse slope per (target, simulation)
   L* tc;   // synapse current per (target, simulation)
   L* cl;   // lower / upper coefficient per node
+  Sched sc;
 };
 
-// Triangularization leaves first, one thread per (segment, simulation) walking its
-// chain bottom-up; back-substitution top-down. kT: solve with A^T (lower <-> upper).
+struct SchedSize {
+  int nseg, ncs, bytes;
+};
+
+template <typename G, typename L>
+__device__ __forceinline__ SchedSize schedule_size(const Fused<G, L>& a, const Topo& tp) {
+  SchedSize z;
+  z.nseg = a.seg_lptr[tp.lb + tp.L + 1];
+  z.ncs = z.nseg > 0 ? a.seg[(static_cast<int64_t>(tp.sb) + z.nseg - 1) * 4 + 3] : 0;
+  z.bytes = sched_bytes(z.nseg, tp.L, z.ncs);
+  return z;
+}
+
+// Copies the schedule of the topology into LDS at `base` (16-byte aligned).
+template <typename G, typename L>
+__device__ __forceinline__ Sched stage_schedule(const Fused<G, L>& a, const Topo& tp,
+                                                const SchedSize& z, unsigned char* base) {
+  int4* sq = reinterpret_cast<int4*>(base);
+  int* lp = reinterpret_cast<int*>(sq + z.nseg);
+  int* sp = lp + tp.L + 2;
+  int* sch = sp + z.nseg;
+  const int4* gsq = reinterpret_cast<const int4*>(a.seg + static_cast<int64_t>(tp.sb) * 4);
+  const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
+  for (int q = threadIdx.x; q < z.nseg; q += blockDim.x) {
+    const int4 r = gsq[q];
+    sq[q] = r;
+    sp[q] = ni[r.x * JXHIP_NODE_ICOLS + kNiParent];
+  }
+  for (int l = threadIdx.x; l < tp.L + 2; l += blockDim.x) lp[l] = a.seg_lptr[tp.lb + l];
+  for (int j = threadIdx.x; j < z.ncs; j += blockDim.x) sch[j] = a.seg_child[tp.scb + j];
+  Sched sc;
+  sc.lptr = lp;
+  sc.sq = sq;
+  sc.sp = sp;
+  sc.sch = sch;
+  sc.ni = ni;
+  sc.lw = wave_levels(a, tp);
+  return sc;
+}
+
+// Lower / upper coefficient pair of node k from LDS in one load; with kT (solve with
+// A^T) the two swap roles.
+template <typename L, bool kT>
+__device__ __forceinline__ void coef(const L* cl, int k, L& lo, L& up) {
+  using V = typename Vec2<L>::type;
+  const V c = reinterpret_cast<const V*>(cl)[k];
+  lo = kT ? c.y : c.x;
+  up = kT ? c.x : c.y;
+}
+
+// Eliminates segment q of simulation s, its child segments being eliminated: merges the
+// child segment tops into the bottom node, then walks the chain bottom-up. ds keeps
+// 1 / (eliminated diagonal), xs the eliminated rhs. The walk is latency-bound (each node
+// depends on the previous one): the LDS operands of kB nodes (kC children) are loaded
+// together before their dependent updates.
+template <typename L, bool kT>
+__device__ __forceinline__ void elim_segment(const Sched& sc, int q, int s, int lS, L* xs, L* ds,
+                                             const L* cl) {
+  constexpr int kB = 4;
+  const int4 sg = sc.sq[q];
+  const int beg = sg.x, end = sg.y, cb = sg.z, nch = sg.w - sg.z;
+  const int id = ((end - 1) << lS) + s;
+  L dk = ds[id], xk = xs[id];
+  constexpr int kC = 2;
+  for (int j0 = 0; j0 < nch; j0 += kC) {
+    int c4[kC];
+    L lo4[kC], up4[kC], dc4[kC], xc4[kC];
+#pragma unroll
+    for (int u = 0; u < kC; ++u) c4[u] = sc.sch[cb + (j0 + u < nch ? j0 + u : nch - 1)];
+#pragma unroll
+    for (int u = 0; u < kC; ++u) {
+      coef<L, kT>(cl, c4[u], lo4[u], up4[u]);
+      dc4[u] = ds[(c4[u] << lS) + s];
+      xc4[u] = xs[(c4[u] << lS) + s];
+    }
+#pragma unroll
+    for (int u = 0; u < kC; ++u) {
+      if (j0 + u < nch) {
+        // ds[c] holds 1 / d[c] of the already eliminated child segment top.
+        const L f = up4[u] * dc4[u];
+        dk -= lo4[u] * f;
+        xk -= xc4[u] * f;
+      }
+    }
+  }
+  L inv = recip(dk);
+  ds[id] = inv;
+  xs[id] = xk;
+  // Node k uses the coefficients of k + 1 and its own diagonal and rhs. Indices below beg
+  // are clamped to beg (loaded, not used).
+  for (int k = end - 2; k >= beg; k -= kB) {
+    L lo8[kB], up8[kB], d8[kB], x8[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int kk = k - u < beg ? beg : k - u;
+      coef<L, kT>(cl, kk + 1, lo8[u], up8[u]);
+      d8[u] = ds[(kk << lS) + s];
+      x8[u] = xs[(kk << lS) + s];
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      if (k - u >= beg) {
+        const L f = up8[u] * inv;
+        const L d2 = d8[u] - lo8[u] * f;
+        xk = x8[u] - xk * f;
+        inv = recip(d2);
+        const int idu = ((k - u) << lS) + s;
+        ds[idu] = inv;
+        xs[idu] = xk;
+      }
+    }
+  }
+}
+
+// Back-substitution of segment q of simulation s, its parent segment being solved.
+template <typename L, bool kT>
+__device__ __forceinline__ void back_segment(const Sched& sc, int q, int s, int lS, L* xs,
+                                             const L* ds, const L* cl) {
+  constexpr int kB = 4;
+  const int4 sg = sc.sq[q];
+  const int beg = sg.x, end = sg.y;
+  const int p = sc.sp != nullptr ? sc.sp[q] : sc.ni[beg * JXHIP_NODE_ICOLS + kNiParent];
+  // Roots have zero coupling coefficients, so xp = 0 there adds nothing.
+  L xp = p >= 0 ? xs[(p << lS) + s] : L(0);
+  const int last = end - 1;
+  for (int k = beg; k < end; k += kB) {
+    L lo8[kB], d8[kB], x8[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int kk = k + u > last ? last : k + u;
+      L up_unused;
+      coef<L, kT>(cl, kk, lo8[u], up_unused);
+      d8[u] = ds[(kk << lS) + s];
+      x8[u] = xs[(kk << lS) + s];
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      if (k + u < end) {
+        xp = (x8[u] - lo8[u] * xp) * d8[u];
+        xs[((k + u) << lS) + s] = xp;
+      }
+    }
+  }
+}
+
+// Orders the LDS accesses of one wavefront between two levels it processes alone.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tree solve of all S simulations of a program, segment level by segment level: leaves
+// first for the triangularization, root first for the back-substitution, one thread per
+// (segment, simulation). The upper levels with at most 64 pairs each (sc.lw .. L) are
+// processed by the first wavefront alone, without workgroup barriers. kT: solve with A^T
+// (lower <-> upper).
 template <typename G, typename L, bool kT>
-__device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp, L* xs, L* ds,
-                                           const L* cl) {
+__device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp, const Sched& sc,
+                                           L* xs, L* ds, const L* cl) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int S = a.S, lS = a.log2S;
-  const int32_t* sg = a.seg + static_cast<int64_t>(tp.sb) * 4;
-  const int32_t* lptr = a.seg_lptr + tp.lb;
-  const int32_t* sch = a.seg_child + tp.scb;
-  const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
-  // cl: [lo, up] per node in LDS.
-  constexpr int kLo = kT ? 1 : 0;
-  constexpr int kUp = kT ? 0 : 1;
-  for (int lv = 0; lv <= tp.L; ++lv) {
-    const int lo = lptr[lv], w = (lptr[lv + 1] - lo) << lS;
-    for (int i = tid; i < w; i += nthr) {
-      const int q = lo + (i >> lS), s = i & (S - 1);
-      const int beg = sg[4 * q], end = sg[4 * q + 1];
-      int k = end - 1;
-      L dk = ds[(k << lS) + s], xk = xs[(k << lS) + s];
-      for (int j = sg[4 * q + 2]; j < sg[4 * q + 3]; ++j) {
-        const int c = sch[j];
-        // ds[c] holds 1 / d[c] of the already eliminated child segment top.
-        const L f = cl[2 * c + kUp] * ds[(c << lS) + s];
-        dk -= cl[2 * c + kLo] * f;
-        xk -= xs[(c << lS) + s] * f;
+  int lo = sc.lptr[0];
+  for (int lv = 0; lv < sc.lw; ++lv) {
+    const int hi = sc.lptr[lv + 1], w = (hi - lo) << lS;
+    for (int i = tid; i < w; i += nthr) elim_segment<L, kT>(sc, lo + (i >> lS), i & (S - 1), lS, xs, ds, cl);
+    lo = hi;
+    __syncthreads();
+  }
+  if (sc.lw <= tp.L) {
+    if (tid < 64) {
+      for (int lv = sc.lw; lv <= tp.L; ++lv) {
+        const int hi = sc.lptr[lv + 1], w = (hi - lo) << lS;
+        if (tid < w) elim_segment<L, kT>(sc, lo + (tid >> lS), tid & (S - 1), lS, xs, ds, cl);
+        lo = hi;
+        wave_sync();
       }
-      L inv = recip(dk);
-      ds[(k << lS) + s] = inv;
-      xs[(k << lS) + s] = xk;
-      for (k = end - 2; k >= beg; --k) {
-        const L f = cl[2 * (k + 1) + kUp] * inv;
-        const L d2 = ds[(k << lS) + s] - cl[2 * (k + 1) + kLo] * f;
-        xk = xs[(k << lS) + s] - xk * f;
-        inv = recip(d2);
-        ds[(k << lS) + s] = inv;
-        xs[(k << lS) + s] = xk;
+      for (int lv = tp.L; lv >= sc.lw; --lv) {
+        const int l0 = sc.lptr[lv], w = (sc.lptr[lv + 1] - l0) << lS;
+        if (tid < w) back_segment<L, kT>(sc, l0 + (tid >> lS), tid & (S - 1), lS, xs, ds, cl);
+        wave_sync();
       }
     }
     __syncthreads();
   }
-  for (int lv = tp.L; lv >= 0; --lv) {
-    const int lo = lptr[lv], w = (lptr[lv + 1] - lo) << lS;
-    for (int i = tid; i < w; i += nthr) {
-      const int q = lo + (i >> lS), s = i & (S - 1);
-      const int beg = sg[4 * q], end = sg[4 * q + 1];
-      const int p = ni[beg * JXHIP_NODE_ICOLS + kNiParent];
-      L xp = p >= 0 ? xs[(p << lS) + s] : L(0);
-      for (int k = beg; k < end; ++k) {
-        L xk = xs[(k << lS) + s];
-        if (k > beg || p >= 0) xk -= cl[2 * k + kLo] * xp;
-        xk *= ds[(k << lS) + s];
-        xs[(k << lS) + s] = xk;
-        xp = xk;
-      }
-    }
+  for (int lv = sc.lw - 1; lv >= 0; --lv) {
+    const int l0 = sc.lptr[lv], w = (sc.lptr[lv + 1] - l0) << lS;
+    for (int i = tid; i < w; i += nthr) back_segment<L, kT>(sc, l0 + (i >> lS), i & (S - 1), lS, xs, ds, cl);
     __syncthreads();
   }
 }
@@ -324,30 +500,37 @@ __device__ __forceinline__ void load_coefficients(const Fused<G, L>& a, const To
   }
 }
 
-// System of one (node, simulation) item: diagonal d and residual r = b - A v.
-// Gates are read from / written to the references (registers or memory).
-template <typename G, typename L>
-__device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
-                                           const Lds<L>& Ld, const double* vg, int k, int s,
-                                           double v, int64_t t, G& m, G& h, G& n) {
+// Axial part of the residual of item (k, s), lo_k (x_k - x_parent) + sum_c up_c (x_k - x_c),
+// in float64 from values x in program layout (voltages, or increments of voltages).
+template <typename G, typename L, typename X>
+__device__ __forceinline__ double axial(const Fused<G, L>& a, const Topo& tp, const int32_t* nik,
+                                        int k, const L* cl, const X* x, int s, double xk) {
   const int lS = a.log2S;
-  const int i = (k << lS) + s;
-  const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
-  const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
-  const int32_t flags = nik[kNiFlags];
-  // Axial part of the residual: lo_k (v_k - v_parent) + sum_c up_c (v_k - v_c).
-  L ax = L(0);
+  double ax = 0.0;
   const int p = nik[kNiParent];
-  if (p >= 0) ax += nfk[kNfLo] * static_cast<L>(v - vg[(p << lS) + s]);
+  if (p >= 0) ax += static_cast<double>(cl[2 * k]) * (xk - static_cast<double>(x[(p << lS) + s]));
   const int32_t* cidx = a.child_idx + tp.cb;
   for (int j = nik[kNiChildBeg]; j < nik[kNiChildEnd]; ++j) {
     const int c = cidx[j];
-    ax += a.node_f[static_cast<int64_t>(tp.nb + c) * JXHIP_NODE_FCOLS + kNfUp] *
-          static_cast<L>(v - vg[(c << lS) + s]);
+    ax += static_cast<double>(cl[2 * c + 1]) * (xk - static_cast<double>(x[(c << lS) + s]));
   }
+  return ax;
+}
+
+// System of one (node, simulation) item from its voltage v and the axial part ax of its
+// residual: diagonal d and residual r = b - A v into LDS. The gates are read from /
+// written to the references (registers or memory).
+template <typename G, typename L>
+__device__ __forceinline__ void build_core(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
+                                           const Lds<L>& Ld, const int32_t* nik, int k, int s,
+                                           int i, double v, double ax, int64_t t, G& m, G& h,
+                                           G& n) {
+  const int lS = a.log2S;
+  const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
+  const int32_t flags = nik[kNiFlags];
   if (!(flags & kComp)) {
     Ld.ds[i] = nfk[kNfDbase];
-    Ld.xs[i] = ax;
+    Ld.xs[i] = static_cast<L>(ax);
     return;
   }
   const G dt = a.dt;
@@ -357,7 +540,7 @@ __device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg,
   G cur = G(0);  // ionic current sum g_i (v - E_i), uA/cm^2 / 1000
   if (flags & kHH) {
     G am, bm, ah, bh, an, bn;
-    hh_rates(vv, am, bm, ah, bh, an, bn);
+    hh_rates_step(vv, am, bm, ah, bh, an, bn);
     m = gate_step(m, dt, am, bm);
     h = gate_step(h, dt, ah, bh);
     n = gate_step(n, dt, an, bn);
@@ -382,14 +565,24 @@ __device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg,
     lin += static_cast<G>(Ld.tl[(tg << lS) + s]);
     net -= static_cast<G>(Ld.tc[(tg << lS) + s]);
   }
-  L r = ax + static_cast<L>(dt * inv_cm * net);
+  G r = static_cast<G>(ax) + dt * inv_cm * net;
   for (int e = nik[kNiStimBeg]; e < nik[kNiStimEnd]; ++e) {
     const int64_t col = a.stim_col[e];
     const int64_t bidx = a.stim_b > 1 ? (pg.simoff + s) : 0;
-    r += static_cast<L>(a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e]);
+    r += a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e];
   }
   Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * inv_cm * lin);
-  Ld.xs[i] = r;
+  Ld.xs[i] = static_cast<L>(r);
+}
+
+// build_core with the axial residual from the float64 voltages vg (program layout).
+template <typename G, typename L>
+__device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
+                                           const Lds<L>& Ld, const double* vg, int k, int s,
+                                           double v, int64_t t, G& m, G& h, G& n) {
+  const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
+  const double ax = axial(a, tp, nik, k, Ld.cl, vg, s, v);
+  build_core(a, pg, tp, Ld, nik, k, s, (k << a.log2S) + s, v, ax, t, m, h, n);
 }
 
 // Items of simulations beyond `nsim` (padding): a well-conditioned row, zero rhs.
@@ -417,95 +610,96 @@ __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& 
   __syncthreads();
 }
 
-// Forward. The first IPT items of a thread, i = tid + j * blockDim (j < IPT), keep their
-// gates in VGPRs for the whole launch; further items stream their gates from HBM.
-// Voltages live in HBM (float64, program layout); LDS holds the system and the
-// coefficients of the tree.
-template <typename G, typename L, int IPT>
-__global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
-  extern __shared__ unsigned char smem_raw[];
-  const Prog pg = load_prog(a.progs, blockIdx.x);
-  const Topo tp = load_topo(a.topos, pg.topo);
-  const int lS = a.log2S, S = a.S;
-  const int nS = tp.n << lS;
+// LDS of a forward program: the staged schedule (unless kGS: read from HBM / L2), the
+// lower / upper coefficient per node, rhs / increment and diagonal per (node,
+// simulation), synapse terms per (target, simulation). Stages the schedule and the
+// coefficients (the caller synchronizes).
+template <typename G, typename L, bool kGS = false>
+__device__ __forceinline__ Lds<L> forward_lds(const Fused<G, L>& a, unsigned char* smem, int nS,
+                                              const Topo& tp, int lS) {
   Lds<L> Ld;
-  Ld.xs = reinterpret_cast<L*>(smem_raw);
+  int off = 0;
+  if constexpr (kGS) {
+    Ld.sc = global_schedule(a, tp);
+  } else {
+    const SchedSize z = schedule_size(a, tp);
+    Ld.sc = stage_schedule(a, tp, z, smem);
+    off = z.bytes;
+  }
+  Ld.cl = reinterpret_cast<L*>(smem + off);
+  Ld.xs = Ld.cl + 2 * tp.n;
   Ld.ds = Ld.xs + nS;
   Ld.tl = Ld.ds + nS;
   Ld.tc = Ld.tl + (tp.n_tgt << lS);
-  Ld.cl = Ld.tc + (tp.n_tgt << lS);
   load_coefficients(a, tp, Ld.cl);
+  return Ld;
+}
+
+template <typename G, typename L>
+__device__ __forceinline__ void record_step(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
+                                            const double* vg, int64_t slot) {
+  const int lS = a.log2S, S = a.S;
+  for (int i = threadIdx.x; i < (tp.n_rec << lS); i += blockDim.x) {
+    const int r = i >> lS, s = i & (S - 1);
+    a.rec[pg.roff + (slot * tp.n_rec + r) * S + s] =
+        static_cast<G>(vg[(a.rec_node[tp.rb + r] << lS) + s]);
+  }
+}
+
+template <typename G, typename L>
+__device__ __forceinline__ void checkpoint_syn(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
+                                               int64_t c) {
+  if (a.syn_model == 0 && a.ckpt_s != nullptr) {
+    const int64_t nsyn = static_cast<int64_t>(tp.n_syn) << a.log2S;
+    const int64_t so = pg.synoff + c * a.ckpt_syn_stride;
+    for (int64_t i = threadIdx.x; i < nsyn; i += blockDim.x) a.ckpt_s[so + i] = a.syn_s[pg.synoff + i];
+  }
+}
+
+// Checkpoint c (state before step c * ckpt_every) of the items held in memory.
+template <typename G, typename L>
+__device__ __forceinline__ void checkpoint_mem(const Fused<G, L>& a, const Prog& pg, int nS,
+                                               int64_t c) {
+  const int64_t off = pg.soff + c * a.ckpt_stride;
+  for (int i = threadIdx.x; i < nS; i += blockDim.x) {
+    a.ckpt_v[off + i] = a.v[pg.soff + i];
+    a.ckpt_m[off + i] = a.gm[pg.soff + i];
+    a.ckpt_h[off + i] = a.gh[pg.soff + i];
+    a.ckpt_n[off + i] = a.gn[pg.soff + i];
+  }
+}
+
+// Streaming forward: every step reads the voltages and gates of the program from HBM and
+// writes them back (for programs whose items do not fit the registers of their threads).
+// kSyn: the topologies carry synapses; kCkpt: checkpoints are written for the adjoint;
+// kGS: the solve schedule is read from HBM / L2 instead of being staged in LDS.
+template <typename G, typename L, bool kSyn, bool kCkpt, bool kGS>
+__global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
+  extern __shared__ unsigned char smem_raw[];
+  const Prog pg = load_prog(a.progs, blockIdx.x);
+  const Topo tp = load_topo(a.topos, pg.topo);
+  const int lS = a.log2S;
+  const int nS = tp.n << lS;
+  const Lds<L> Ld = forward_lds<G, L, kGS>(a, smem_raw, nS, tp, lS);
+  __syncthreads();
   const int tid = threadIdx.x, nthr = blockDim.x;
   double* vg = a.v + pg.soff;
   G* gm = a.gm + pg.soff;
   G* gh = a.gh + pg.soff;
   G* gn = a.gn + pg.soff;
-  constexpr int kR = IPT > 0 ? IPT : 1;
-  G mr[kR], hr[kR], nr[kR];
-  if constexpr (IPT > 0) {
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int i = tid + j * nthr;
-      if (i < nS) {
-        mr[j] = gm[i];
-        hr[j] = gh[i];
-        nr[j] = gn[i];
-      }
-    }
-  }
-  for (int i = tid; i < (tp.n_rec << lS); i += nthr) {
-    const int r = i >> lS, s = i & (S - 1);
-    a.rec[pg.roff + i] = static_cast<G>(vg[(a.rec_node[tp.rb + r] << lS) + s]);
-  }
+  record_step(a, pg, tp, vg, 0);
   const int64_t nsteps = pg.t1 - pg.t0;
   for (int64_t step = 0; step < nsteps; ++step) {
-    if (a.ckpt_v != nullptr && step % a.ckpt_every == 0) {
-      const int64_t off = pg.soff + (step / a.ckpt_every) * a.ckpt_stride;
-      for (int i = tid; i < nS; i += nthr) a.ckpt_v[off + i] = vg[i];
-      if constexpr (IPT > 0) {
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-          const int i = tid + j * nthr;
-          if (i < nS) {
-            a.ckpt_m[off + i] = mr[j];
-            a.ckpt_h[off + i] = hr[j];
-            a.ckpt_n[off + i] = nr[j];
-          }
-        }
-      }
-      for (int i = tid + IPT * nthr; i < nS; i += nthr) {
-        a.ckpt_m[off + i] = gm[i];
-        a.ckpt_h[off + i] = gh[i];
-        a.ckpt_n[off + i] = gn[i];
-      }
-      if (a.syn_model == 0 && a.ckpt_s != nullptr) {
-        const int64_t nsyn = static_cast<int64_t>(tp.n_syn) << lS;
-        const int64_t so = pg.synoff + (step / a.ckpt_every) * a.ckpt_syn_stride;
-        for (int64_t i = tid; i < nsyn; i += nthr) a.ckpt_s[so + i] = a.syn_s[pg.synoff + i];
+    if constexpr (kCkpt) {
+      if (step % a.ckpt_every == 0) {
+        checkpoint_mem(a, pg, nS, step / a.ckpt_every);
+        if constexpr (kSyn) checkpoint_syn(a, pg, tp, step / a.ckpt_every);
       }
     }
-    synapse_phase(a, pg, tp, Ld);
+    if constexpr (kSyn) synapse_phase(a, pg, tp, Ld);
     const int64_t t = pg.t0 + step;
-    if constexpr (IPT > 0) {
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        int i = tid + j * nthr;
-        // Opaque to the optimizer: the item's addresses are recomputed every step instead
-        // of being hoisted out of the time loop for all IPT items (register pressure).
-        asm volatile("" : "+v"(i));
-        if (i < nS) {
-          const int k = i >> lS, s = i & (S - 1);
-          if (s < pg.nsim) {
-            build_item(a, pg, tp, Ld, vg, k, s, vg[i], t, mr[j], hr[j], nr[j]);
-          } else {
-            idle_item(a, tp, Ld, k, i);
-          }
-        }
-      }
-    }
-    // Items beyond the register-resident ones: gates streamed from HBM.
-    for (int i = tid + IPT * nthr; i < nS; i += nthr) {
-      const int k = i >> lS, s = i & (S - 1);
+    for (int i = tid; i < nS; i += nthr) {
+      const int k = i >> lS, s = i & (a.S - 1);
       if (s < pg.nsim) {
         G m = gm[i], h = gh[i], n = gn[i];
         build_item(a, pg, tp, Ld, vg, k, s, vg[i], t, m, h, n);
@@ -517,24 +711,117 @@ __global__ void __launch_bounds__(1024) fused_forward_kernel(Fused<G, L> a) {
       }
     }
     __syncthreads();
-    tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds, Ld.cl);
+    tree_solve<G, L, false>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
     for (int i = tid; i < nS; i += nthr) vg[i] += static_cast<double>(Ld.xs[i]);
     __syncthreads();
-    for (int i = tid; i < (tp.n_rec << lS); i += nthr) {
-      const int r = i >> lS, s = i & (S - 1);
-      a.rec[pg.roff + ((step + 1) * tp.n_rec + r) * S + s] =
-          static_cast<G>(vg[(a.rec_node[tp.rb + r] << lS) + s]);
+    record_step(a, pg, tp, vg, step + 1);
+  }
+}
+
+// Register-resident forward. Every thread owns the items i = tid + j * blockDim (j < IPT)
+// for the whole launch and keeps their voltage, gates and the axial part of their
+// residual in VGPRs: after the solve the axial part is updated from the increments of the
+// item and of its neighbours (in LDS), so no voltage is read from HBM inside the time
+// loop. Only nodes flagged kExport (recorded nodes, synapse pre- and postsynaptic nodes)
+// copy their voltage to HBM every step. The host guarantees n * S <= IPT * blockDim for
+// every program of the launch (a program that does not fit is left untouched).
+// The item loops are not unrolled (one copy of the item code keeps the kernel inside the
+// instruction cache): the per-item arrays are indexed with the uniform loop counter,
+// which the compiler lowers to VGPR indexing (the build promotes them to vectors).
+template <typename G, typename L, int IPT, int MAXT, bool kSyn, bool kCkpt>
+__global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
+  extern __shared__ unsigned char smem_raw[];
+  const Prog pg = load_prog(a.progs, blockIdx.x);
+  const Topo tp = load_topo(a.topos, pg.topo);
+  const int lS = a.log2S, S = a.S;
+  const int nS = tp.n << lS;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (nS > IPT * nthr) return;
+  const Lds<L> Ld = forward_lds(a, smem_raw, nS, tp, lS);
+  __syncthreads();
+  double* vg = a.v + pg.soff;
+  G* gm = a.gm + pg.soff;
+  G* gh = a.gh + pg.soff;
+  G* gn = a.gn + pg.soff;
+  const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
+  // The axial part of the residual accumulates in the gate type (float64 except in the
+  // all-float32 mode).
+  double vr[IPT];
+  G ar[IPT], mr[IPT], hr[IPT], nr[IPT];
+#pragma unroll 1
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * nthr;
+    vr[j] = 0.0;
+    ar[j] = mr[j] = hr[j] = nr[j] = G(0);
+    if (i < nS) {
+      const int k = i >> lS, s = i & (S - 1);
+      vr[j] = vg[i];
+      mr[j] = gm[i];
+      hr[j] = gh[i];
+      nr[j] = gn[i];
+      ar[j] = static_cast<G>(axial(a, tp, ni + k * JXHIP_NODE_ICOLS, k, Ld.cl, vg, s, vr[j]));
     }
   }
-  if constexpr (IPT > 0) {
-#pragma unroll
+  record_step(a, pg, tp, vg, 0);
+  const int64_t nsteps = pg.t1 - pg.t0;
+  for (int64_t step = 0; step < nsteps; ++step) {
+    if constexpr (kCkpt) {
+      if (step % a.ckpt_every == 0) {
+        const int64_t c = step / a.ckpt_every;
+        const int64_t off = pg.soff + c * a.ckpt_stride;
+#pragma unroll 1
+        for (int j = 0; j < IPT; ++j) {
+          const int i = tid + j * nthr;
+          if (i < nS) {
+            a.ckpt_v[off + i] = vr[j];
+            a.ckpt_m[off + i] = mr[j];
+            a.ckpt_h[off + i] = hr[j];
+            a.ckpt_n[off + i] = nr[j];
+          }
+        }
+        if constexpr (kSyn) checkpoint_syn(a, pg, tp, c);
+      }
+    }
+    if constexpr (kSyn) synapse_phase(a, pg, tp, Ld);
+    const int64_t t = pg.t0 + step;
+#pragma unroll 1
     for (int j = 0; j < IPT; ++j) {
       const int i = tid + j * nthr;
       if (i < nS) {
-        gm[i] = mr[j];
-        gh[i] = hr[j];
-        gn[i] = nr[j];
+        const int k = i >> lS, s = i & (S - 1);
+        if (s < pg.nsim) {
+          build_core(a, pg, tp, Ld, ni + k * JXHIP_NODE_ICOLS, k, s, i, vr[j], ar[j], t, mr[j],
+                     hr[j], nr[j]);
+        } else {
+          idle_item(a, tp, Ld, k, i);
+        }
       }
+    }
+    __syncthreads();
+    tree_solve<G, L, false>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+#pragma unroll 1
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * nthr;
+      if (i < nS) {
+        const int k = i >> lS, s = i & (S - 1);
+        const int32_t* nik = ni + k * JXHIP_NODE_ICOLS;
+        const double dv = static_cast<double>(Ld.xs[i]);
+        vr[j] += dv;
+        ar[j] += static_cast<G>(axial(a, tp, nik, k, Ld.cl, Ld.xs, s, dv));
+        if (nik[kNiFlags] & kExport) vg[i] = vr[j];
+      }
+    }
+    __syncthreads();
+    record_step(a, pg, tp, vg, step + 1);
+  }
+#pragma unroll 1
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * nthr;
+    if (i < nS) {
+      vg[i] = vr[j];
+      gm[i] = mr[j];
+      gh[i] = hr[j];
+      gn[i] = nr[j];
     }
   }
 }
@@ -593,13 +880,17 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   const Topo tp = load_topo(a.topos, pg.topo);
   const int lS = a.log2S, S = a.S;
   const int nS = tp.n << lS;
+  // The schedule and the coefficient pairs first (aligned for their vector loads), then
+  // the system and the adjoint of the voltages.
+  const SchedSize z = schedule_size(a, tp);
   Lds<L> Ld;
-  Ld.xs = reinterpret_cast<L*>(smem_raw);
+  Ld.sc = stage_schedule(a, tp, z, smem_raw);
+  Ld.cl = reinterpret_cast<L*>(smem_raw + z.bytes);
+  Ld.xs = Ld.cl + 2 * tp.n;
   Ld.ds = Ld.xs + nS;
   L* lv = Ld.ds + nS;  // adjoint of the voltages
   Ld.tl = lv;          // unused (no synapses in the adjoint)
   Ld.tc = lv;
-  Ld.cl = lv + nS;
   load_coefficients(a, tp, Ld.cl);
   const int tid = threadIdx.x, nthr = blockDim.x;
   const G dt = a.dt;
@@ -648,7 +939,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         wn[(j + 1) * nS + i] = n;
       }
       __syncthreads();
-      tree_solve<G, L, false>(a, tp, Ld.xs, Ld.ds, Ld.cl);
+      tree_solve<G, L, false>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
       for (int i = tid; i < nS; i += nthr) wv[(j + 1) * nS + i] = wv[j * nS + i] + static_cast<double>(Ld.xs[i]);
       __syncthreads();
     }
@@ -685,7 +976,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         Ld.xs[i] = lv[i];
       }
       __syncthreads();
-      tree_solve<G, L, true>(a, tp, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
+      tree_solve<G, L, true>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
       const double* v0 = wv + j * nS;
       const double* v1 = wv + (j + 1) * nS;
       const G* m0 = wm + j * nS;
@@ -788,6 +1079,60 @@ inline int check_args(const jxhip_fused_args& a) {
   return hipSuccess;
 }
 
+template <typename K, typename... Args>
+int launch_kernel(K kern, const jxhip_fused_args& a, hipStream_t stream, Args... args) {
+  const dim3 grid(static_cast<unsigned>(a.n_progs)), block(a.threads);
+  const size_t lds = static_cast<size_t>(a.lds_bytes);
+  if (hipError_t e = set_lds(kern, lds)) return e;
+  hipLaunchKernelGGL(kern, grid, block, lds, stream, args...);
+  return hipGetLastError();
+}
+
+// Register-resident item counts with a kernel, by state element size: float gates
+// (<= 12 items at 1024 threads, <= 16 at 512), float64 gates (<= 4 at 1024, <= 12 at 512).
+template <typename G, typename L, bool kSyn, bool kCkpt>
+int launch_forward_feat(const jxhip_fused_args& a, const Fused<G, L>& f, hipStream_t st) {
+  if (a.items_per_thread == 0) {
+    if (a.sched_global) return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, true>, a, st, f);
+    return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, false>, a, st, f);
+  }
+  if (a.sched_global) return hipErrorInvalidValue;
+  if (a.max_items > static_cast<int64_t>(a.items_per_thread) * a.threads) return hipErrorInvalidValue;
+  constexpr bool kF = sizeof(G) == 4;
+#define JXHIP_REGS(IPT, MAXT)                                                            \
+  case IPT:                                                                              \
+    if (a.threads > MAXT) return hipErrorInvalidValue;                                   \
+    return launch_kernel(fused_regs_kernel<G, L, IPT, MAXT, kSyn, kCkpt>, a, st, f);
+  if constexpr (kF) {
+    switch (a.items_per_thread) {
+      JXHIP_REGS(1, 1024)
+      JXHIP_REGS(2, 1024)
+      JXHIP_REGS(3, 1024)
+      JXHIP_REGS(4, 1024)
+      JXHIP_REGS(6, 1024)
+      JXHIP_REGS(8, 1024)
+      JXHIP_REGS(10, 1024)
+      JXHIP_REGS(12, 1024)
+      JXHIP_REGS(16, 512)
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else {
+    switch (a.items_per_thread) {
+      JXHIP_REGS(1, 1024)
+      JXHIP_REGS(2, 1024)
+      JXHIP_REGS(3, 1024)
+      JXHIP_REGS(4, 1024)
+      JXHIP_REGS(6, 512)
+      JXHIP_REGS(8, 512)
+      JXHIP_REGS(12, 512)
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
+#undef JXHIP_REGS
+}
+
 template <typename G, typename L>
 int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
   if (args == nullptr) return hipErrorInvalidValue;
@@ -795,28 +1140,12 @@ int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
   if (a.n_progs <= 0) return hipSuccess;
   if (int e = check_args(a)) return e;
   const Fused<G, L> f = unpack<G, L>(a);
-  const dim3 grid(static_cast<unsigned>(a.n_progs)), block(a.threads);
-  const size_t lds = static_cast<size_t>(a.lds_bytes);
-#define JXHIP_FWD_CASE(IPT)                                         \
-  case IPT: {                                                       \
-    auto kern = fused_forward_kernel<G, L, IPT>;                    \
-    if (hipError_t e = set_lds(kern, lds)) return e;                \
-    hipLaunchKernelGGL(kern, grid, block, lds, stream, f);          \
-    break;                                                          \
-  }
-  switch (a.items_per_thread) {
-    JXHIP_FWD_CASE(0)
-    JXHIP_FWD_CASE(1)
-    JXHIP_FWD_CASE(2)
-    JXHIP_FWD_CASE(4)
-    JXHIP_FWD_CASE(8)
-    JXHIP_FWD_CASE(12)
-    JXHIP_FWD_CASE(16)
-    default:
-      return hipErrorInvalidValue;
-  }
-#undef JXHIP_FWD_CASE
-  return hipGetLastError();
+  const bool syn = a.syn_model >= 0, ckpt = a.ckpt_v != nullptr;
+  if (ckpt && a.ckpt_every <= 0) return hipErrorInvalidValue;
+  if (syn && ckpt) return launch_forward_feat<G, L, true, true>(a, f, stream);
+  if (syn) return launch_forward_feat<G, L, true, false>(a, f, stream);
+  if (ckpt) return launch_forward_feat<G, L, false, true>(a, f, stream);
+  return launch_forward_feat<G, L, false, false>(a, f, stream);
 }
 
 template <typename G, typename L>
@@ -850,34 +1179,3 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
 }
 
 }  // namespace jxhip
-
-extern "C" {
-
-int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream) {
-  return jxhip::launch_forward<float, float>(args, static_cast<hipStream_t>(stream));
-}
-
-int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream) {
-  return jxhip::launch_forward<double, double>(args, static_cast<hipStream_t>(stream));
-}
-
-int jxhip_fused_forward_mixed(const jxhip_fused_args* args, void* stream) {
-  return jxhip::launch_forward<double, float>(args, static_cast<hipStream_t>(stream));
-}
-
-int jxhip_fused_adjoint_mixed(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
-                              void* stream) {
-  return jxhip::launch_adjoint<double, float>(args, adj, static_cast<hipStream_t>(stream));
-}
-
-int jxhip_fused_adjoint_f32(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
-                            void* stream) {
-  return jxhip::launch_adjoint<float, float>(args, adj, static_cast<hipStream_t>(stream));
-}
-
-int jxhip_fused_adjoint_f64(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
-                            void* stream) {
-  return jxhip::launch_adjoint<double, double>(args, adj, static_cast<hipStream_t>(stream));
-}
-
-}  // extern "C"

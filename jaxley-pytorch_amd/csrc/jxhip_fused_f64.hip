@@ -1,0 +1,16 @@
+// C entry points of the fused forward and adjoint for one element type pair
+// (gates / parameters: double, tree solve: double); the kernels are in jxhip_fused.h.
+#include "jxhip_fused.h"
+
+extern "C" {
+
+int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream) {
+  return jxhip::launch_forward<double, double>(args, static_cast<hipStream_t>(stream));
+}
+
+int jxhip_fused_adjoint_f64(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
+                              void* stream) {
+  return jxhip::launch_adjoint<double, double>(args, adj, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

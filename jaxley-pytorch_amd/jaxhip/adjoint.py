@@ -17,7 +17,7 @@ from typing import Dict, List, Tuple
 import torch
 
 from . import _lib
-from .fused import ALL_PARAMS, BATCHABLE, ForwardLaunch, from_prog, to_prog
+from .fused import ALL_PARAMS, BATCHABLE, ForwardLaunch, adjoint_lds_bytes, from_prog, to_prog
 
 
 class _FusedIntegrate(torch.autograd.Function):
@@ -63,7 +63,7 @@ class _FusedIntegrate(torch.autograd.Function):
         adj.n_steps = T
         args = run.args
         saved_lds = args.lds_bytes
-        args.lds_bytes = (3 * n * S + 2 * n) * run.node_f.element_size()
+        args.lds_bytes = adjoint_lds_bytes(n, S, run.node_f.element_size(), st.sched)
         f = run.kernel_name("jxhip_fused_adjoint")
         try:
             _lib.check(f(args, adj, _lib.stream_ptr(dev)), "fused_adjoint")

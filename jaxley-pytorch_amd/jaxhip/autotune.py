@@ -3,8 +3,9 @@
 * `choose_tree_kernel`: register vs level variant of the standalone tree solve (the
   reference estimates the work instead, kernels/tree_solve.py:496-509); both are timed on
   a synthetic diagonally dominant system of the real forest.
-* `fused_config`: simulations per program (S) and threads per workgroup of the fused
-  kernel; candidates are timed on a short run of the real structure (few steps).
+* `fused_config`: simulations per program (S), threads per workgroup and kernel
+  (register-resident or streaming) of the fused forward; candidates are timed on a short
+  run of the real structure and batch (a few steps).
 
 Results are cached per (structure, batch size, dtype, device) for the process.
 `JAXHIP_AUTOTUNE=0` disables measurement (heuristic choice).
@@ -73,10 +74,22 @@ def _key(st, B, dtype, lsize, grad):
     return ("fused", st.n, st.L, n_tgt, int(st.topo_row.sum()), B, str(dtype), lsize, bool(grad))
 
 
-def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False):
+def parse_config(text: str):
+    """"S,threads[,stream|gsched]" (the JAXHIP_FUSED_CONFIG format)."""
+    from .fused import LaunchConfig
+
+    parts = [x.strip() for x in text.split(",")]
+    S, thr = int(parts[0]), int(parts[1])
+    flag = parts[2] if len(parts) > 2 else ""
+    return LaunchConfig(S, thr, stream=flag in ("stream", "gsched"), gsched=flag == "gsched")
+
+
+def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False, measure=None):
     """Launch shape of the fused forward for a structure and batch size. With `grad` the
-    shape is shared with the reverse pass and obeys its limits."""
-    from .fused import ADJOINT_THREADS_MAX, LaunchConfig, default_config, grad_config
+    shape is shared with the reverse pass and obeys its limits. `measure(cfg)` returns
+    the time of a short run with that shape; when given (and JAXHIP_AUTOTUNE != 0) the
+    fastest candidate is chosen, else a heuristic one."""
+    from .fused import config_candidates, default_config, grad_config
 
     n_tgt = st.syn["n_tgt"] if st.syn else 0
     esize = torch.tensor([], dtype=dtype).element_size()
@@ -84,11 +97,17 @@ def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False):
     if key in _CACHE:
         return _CACHE[key]
     lsz = lsize if lsize else esize
-    cfg = grad_config(st.n, B, lsz) if grad else default_config(st.n, n_tgt, B, lsz)
     env = os.environ.get("JAXHIP_FUSED_CONFIG")
-    if env and not grad:
-        S, thr = (int(x) for x in env.split(","))
-        cfg = LaunchConfig(S, thr)
+    if grad:
+        cfg = grad_config(st.n, B, lsz, st.sched)
+    elif env:
+        cfg = parse_config(env)
+    elif measure is not None and _enabled():
+        cands = config_candidates(st.n, n_tgt, B, esize, lsz, st.sched)
+        times = [(measure(c), i) for i, c in enumerate(cands)]
+        cfg = cands[min(times)[1]]
+    else:
+        cfg = default_config(st.n, n_tgt, B, lsz, esize, st.sched)
     _CACHE[key] = cfg
     return cfg
 

@@ -31,6 +31,10 @@ HIPCC_FLAGS = [
     # Packed FP32 instructions stay off (they are generated from ordinary float code).
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast",
+    # Per-thread item arrays indexed by a loop counter stay in VGPRs (indexed moves)
+    # instead of scratch memory.
+    "-mllvm", "-amdgpu-promote-alloca-to-vector-max-regs=128",
+    "-mllvm", "-amdgpu-promote-alloca-to-vector-vgpr-ratio=1",
     f"-I{INCLUDE}",
 ]
 

@@ -28,17 +28,20 @@ ALL_PARAMS = HH_PARAMS + LEAK_PARAMS
 # Parameters that may carry batch dimensions in the fused kernel, and their column in
 # the batched table.
 BATCHABLE = {"HH_gNa": 0, "HH_gK": 1, "HH_gLeak": 2, "Leak_gLeak": 3}
-FLAG_COMP, FLAG_HH, FLAG_LEAK = 1, 2, 4
+FLAG_COMP, FLAG_HH, FLAG_LEAK, FLAG_EXPORT = 1, 2, 4, 8
 SYN_DIFF = 1e-3  # mV, the finite difference of the synapse slope (network.py:29)
 
 
 def to_prog(x: torch.Tensor, S: int) -> torch.Tensor:
-    """(B, n) -> (P, n, S) with B padded to P * S (zeros)."""
+    """(B, n) -> (P, n, S) with B padded to P * S (zeros). Always a new tensor (the
+    kernels update states in place)."""
     B, n = x.shape
     P = (B + S - 1) // S
     if P * S != B:
         x = torch.cat([x, x.new_zeros(P * S - B, n)], 0)
-    return x.reshape(P, S, n).transpose(1, 2).contiguous()
+    out = torch.empty((P, n, S), dtype=x.dtype, device=x.device)
+    out.copy_(x.reshape(P, S, n).transpose(1, 2))
+    return out
 
 
 def from_prog(x: torch.Tensor, B: int) -> torch.Tensor:
@@ -72,6 +75,7 @@ class Structure:
     lo_edge: np.ndarray        # edge of the lower coefficient of every kernel node
     up_edge: np.ndarray
     parent_k: np.ndarray
+    sched: int = 0             # bytes of the solve schedule staged in LDS
 
 
 def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, device) -> Structure:
@@ -121,6 +125,11 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
     node_i[:, 6] = -1
     syn = _build_synapses(module, kidx_of_node, n, node_i)
     rec_k = kidx_of_node[module._comp_node[rec_comps]] if len(rec_comps) else np.zeros(0, np.int64)
+    # Nodes whose voltage other threads read from HBM during the time loop.
+    node_i[rec_k, 3] |= FLAG_EXPORT
+    if syn is not None:
+        node_i[syn["pre"], 3] |= FLAG_EXPORT
+        node_i[syn["tgt"], 3] |= FLAG_EXPORT
     topo_row = np.zeros(_lib.JXHIP_TOPO_COLS, dtype=np.int32)
     topo_row[:14] = [n, sg["L"], 0, 0, 0, len(rec_k), 0, syn["n_tgt"] if syn else 0, 0, 0, 0,
                      syn["n_syn"] if syn else 0, 0, 0]
@@ -139,6 +148,7 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         lo_edge=np.asarray(indexer["map_to_solve_order_lower"])[korder],
         up_edge=np.asarray(indexer["map_to_solve_order_upper"])[korder],
         parent_k=parent_k,
+        sched=sched_bytes(len(sg["seg"]), sg["L"], int(sg["seg"][-1, 3]) if len(sg["seg"]) else 0),
     )
 
 
@@ -194,60 +204,101 @@ def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor
 class LaunchConfig:
     S: int
     threads: int
+    stream: bool = False       # streaming kernel even if the items fit the registers
+    gsched: bool = False       # streaming kernel reading the schedule from HBM / L2
 
 
-IPT_CHOICES = (0, 1, 2, 4, 8, 12, 16)
-# Register-resident items per thread that fit the VGPR budget (no spills), by the
-# element sizes of (gates, solve).
-IPT_MAX = {(4, 4): 16, (8, 8): 2, (8, 4): 2}
+# Register-resident items per thread with a forward kernel, by the element sizes of
+# (gates, solve) and the launch bound: threads <= 1024 (128 VGPRs) or <= 512 (256 VGPRs).
+REGS_IPT = {
+    (4, 4): {1024: (1, 2, 3, 4, 6, 8, 10, 12), 512: (16,)},
+    (8, 8): {1024: (1, 2, 3, 4), 512: (6, 8, 12)},
+    (8, 4): {1024: (1, 2, 3, 4), 512: (6, 8, 12)},
+}
 
 
-def lds_bytes(n: int, n_tgt: int, S: int, esize: int) -> int:
-    """Forward: diagonal and rhs per (node, simulation), synapse terms per (target,
-    simulation), lower / upper coefficient per node."""
-    return (2 * n + 2 * n_tgt) * S * esize + 2 * n * esize
+def sched_bytes(nseg: int, L: int, ncs: int) -> int:
+    """LDS bytes of a staged solve schedule (segments, level pointers, segment parents,
+    child segment tops), rounded to 16."""
+    return (16 * nseg + 4 * (L + 2 + nseg + ncs) + 15) & ~15
+
+
+def lds_bytes(n: int, n_tgt: int, S: int, esize: int, sched: int) -> int:
+    """Forward: the staged schedule, lower / upper coefficient per node, diagonal and
+    rhs per (node, simulation), synapse terms per (target, simulation)."""
+    return sched + 2 * n * esize + (2 * n + 2 * n_tgt) * S * esize
 
 
 def items_per_thread(n: int, S: int, threads: int, gsize: int, lsize: int) -> int:
-    """Largest register-resident item count <= the items of a thread and the budget."""
+    """Register-resident items per thread of the forward: the smallest count with a
+    kernel that holds all n * S items of a program, or 0 (streaming) if none does."""
     need = -(-n * S // threads)
-    best = 0
-    for c in IPT_CHOICES:
-        if c <= min(need, IPT_MAX[(gsize, lsize)]):
-            best = c
-    return best
+    sets = REGS_IPT[(gsize, lsize)]
+    avail = sorted(set(sets[1024]) | (set(sets[512]) if threads <= 512 else set()))
+    for c in avail:
+        if c >= need:
+            return c
+    return 0
 
 
-def default_config(n: int, n_tgt: int, B: int, esize: int) -> LaunchConfig:
-    """A launch shape. Simulations per program S as large as the LDS allows (one program
-    per CU holds the tree's coefficients once for S simulations) while the batch still
-    fills the 256 CUs; then enough threads (<= 1024) for the items."""
+def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
+                      sched: int = 0) -> List[LaunchConfig]:
+    """Launch shapes worth measuring: S simulations per program (a power of two, LDS
+    permitting, at least 256 programs when the batch allows) with every thread count
+    whose registers hold the items of a program, and the streaming kernel at each S
+    (with the schedule in HBM / L2 when only that fits the LDS)."""
+    out = []
     S = 1
-    while (S < 64 and (S * 2) * 256 <= max(B, 256) and lds_bytes(n, n_tgt, S * 2, esize) <= 160 * 1024):
+    while S <= 64:
+        if S > 1 and S * 256 > B:
+            break
+        fits = lds_bytes(n, n_tgt, S, lsize, sched) <= 160 * 1024
+        fits_g = lds_bytes(n, n_tgt, S, lsize, 0) <= 160 * 1024
+        if not fits_g:
+            break
+        if fits:
+            for thr in (256, 512, 1024):
+                if items_per_thread(n, S, thr, gsize, lsize):
+                    out.append(LaunchConfig(S, thr))
+        thr = int(min(1024, max(64, 64 * -(-n * S // (64 * 16)))))
+        for t in sorted({thr, 1024}):
+            out.append(LaunchConfig(S, t, stream=True, gsched=not fits))
         S *= 2
-    if lds_bytes(n, n_tgt, S, esize) > 160 * 1024:
+    if not out:
         raise NotImplementedError(f"a topology of {n} nodes does not fit the fused kernel's LDS")
-    threads = int(min(1024, max(64, 64 * -(-n * S // (64 * 16)))))
-    return LaunchConfig(S, threads)
+    return out
+
+
+def default_config(n: int, n_tgt: int, B: int, esize: int, gsize: int = 0,
+                   sched: int = 0) -> LaunchConfig:
+    """A launch shape without measurement: the largest S with a register-resident shape
+    (fewest threads), else the largest streaming S."""
+    gsize = gsize or esize
+    cands = config_candidates(n, n_tgt, B, gsize, esize, sched)
+    regs = [c for c in cands if not c.stream]
+    if regs:
+        return max(regs, key=lambda c: (c.S, -c.threads))
+    return max(cands, key=lambda c: c.S)
 
 
 ADJOINT_THREADS_MAX = 512
 
 
-def adjoint_lds_bytes(n: int, S: int, esize: int) -> int:
-    """LDS of one reverse-pass program: right-hand side, pivots and voltages per
-    (node, simulation) plus the tree's two coupling coefficients per node."""
-    return (3 * n * S + 2 * n) * esize
+def adjoint_lds_bytes(n: int, S: int, esize: int, sched: int) -> int:
+    """LDS of one reverse-pass program: the staged schedule, the tree's two coupling
+    coefficients per node, right-hand side, pivots and voltage adjoints per (node,
+    simulation)."""
+    return sched + (3 * n * S + 2 * n) * esize
 
 
-def grad_config(n: int, B: int, esize: int) -> LaunchConfig:
+def grad_config(n: int, B: int, esize: int, sched: int = 0) -> LaunchConfig:
     """Launch shape shared by the checkpointed forward and the reverse pass: S bounded by
     the reverse pass's LDS, threads by its launch bound."""
     S = 1
     while (S < 64 and (S * 2) * 256 <= max(B, 256)
-           and adjoint_lds_bytes(n, S * 2, esize) <= 160 * 1024):
+           and adjoint_lds_bytes(n, S * 2, esize, sched) <= 160 * 1024):
         S *= 2
-    if adjoint_lds_bytes(n, S, esize) > 160 * 1024:
+    if adjoint_lds_bytes(n, S, esize, sched) > 160 * 1024:
         raise NotImplementedError(f"a topology of {n} nodes does not fit the reverse pass's LDS")
     threads = int(min(ADJOINT_THREADS_MAX, max(64, 64 * -(-n * S // (64 * 16)))))
     return LaunchConfig(S, threads)
@@ -346,10 +397,12 @@ class ForwardLaunch:
         a.diff = SYN_DIFF
         a.S = S
         a.threads = cfg.threads
-        a.items_per_thread = items_per_thread(n, S, cfg.threads, self.gm.element_size(),
-                                              node_f.element_size())
+        a.items_per_thread = 0 if cfg.stream else items_per_thread(
+            n, S, cfg.threads, self.gm.element_size(), node_f.element_size())
         n_tgt = st.syn["n_tgt"] if st.syn else 0
-        a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size())
+        a.sched_global = int(cfg.stream and cfg.gsched)
+        a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size(), 0 if a.sched_global else st.sched)
+        a.max_items = n * S
         if ckpt_every > 0:
             a.ckpt_v, a.ckpt_m, a.ckpt_h, a.ckpt_n = (c.data_ptr() for c in self.ckpt)
             a.ckpt_s = self.ckpt_s.data_ptr() if self.ckpt_s is not None else 0

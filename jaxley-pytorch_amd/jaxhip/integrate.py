@@ -265,15 +265,9 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         y = x[..., cs_safe]
         return torch.where(is_comp, y, torch.zeros_like(y))
 
-    # Launch shape.
-    n_tgt = st.syn["n_tgt"] if st.syn else 0
-    from .autotune import fused_config
-
     needs_grad = torch.is_grad_enabled() and any(
         t.requires_grad for t in [vals[k] for k in chan_keys] + [init["v"]]
     )
-    cfg = fused_config(st, B, dtype, torch.tensor([], dtype=user_dtype).element_size(), needs_grad)
-    S = cfg.S
     # Channel parameters: a per-node table (kernel order) and, when some conductance
     # carries batch dimensions, a batched table of the four conductances.
     def kernel_cols(x):
@@ -298,16 +292,18 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         first = int(torch.nonzero(chan)[0]) if bool(chan.any()) else 0
         with torch.no_grad():
             per_sim = bool(((stack - stack[:, first:first + 1]).abs()[:, chan] == 0).all())
-        if per_sim:
-            bmode = 1
-            bsim = stack[:, first]                                        # (B, 4)
+        bmode = 1 if per_sim else 2
+        bsim = stack[:, first] if per_sim else None                       # (B, 4)
+
+    def bparam_for(S):
+        """The batched conductances in the layout of programs of S simulations."""
+        if bmode == 1:
             P = -(-B // S)
-            if P * S != B:
-                bsim = torch.cat([bsim, bsim.new_zeros(P * S - B, 4)], 0)
-            bparam = bsim.contiguous()
-        else:
-            bmode = 2
-            bparam = torch.stack([to_prog(stack[..., c], S) for c in range(4)], -1).contiguous()
+            b = bsim if P * S == B else torch.cat([bsim, bsim.new_zeros(P * S - B, 4)], 0)
+            return b.contiguous()
+        if bmode == 2:
+            return torch.stack([to_prog(stack[..., c], S) for c in range(4)], -1).contiguous()
+        return None
     cols = []
     for j, k in enumerate(ALL_PARAMS):
         x = vals.get(k)
@@ -363,6 +359,24 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
                 s0 = inp.all_states[f"{stype.name}_s"].to(dtype)
             s0 = torch.broadcast_to(s0, (*batch, s0.shape[-1])).reshape(B, -1)
             syn_s0 = s0[:, perm]
+    # Launch shape (measured on a short run of the real problem when not cached).
+    from .autotune import _time, fused_config
+
+    def measure(c):
+        Tm = min(inp.T, 32)
+        try:
+            with torch.no_grad():
+                run = ForwardLaunch(st, node_f, node_p.detach(),
+                                    None if bmode == 0 else bparam_for(c.S).detach(), bmode,
+                                    v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
+                                    Tm, dt, B, c, syn_params, syn_scale, syn_s0)
+                return _time(run.launch, reps=2)
+        except RuntimeError:
+            return float("inf")
+
+    cfg = fused_config(st, B, dtype, torch.tensor([], dtype=user_dtype).element_size(), needs_grad,
+                       measure=measure)
+    bparam = bparam_for(cfg.S)
     if needs_grad:
         from .adjoint import fused_integrate_with_adjoint
 

@@ -344,7 +344,8 @@ def segments(parent: np.ndarray):
     of single-child nodes (top first); a segment's level is 0 if its bottom node has no
     children, else 1 + the largest level of its child segments. Returns the kernel order
     (kernel index -> input index), the segments `(beg, end)` in kernel indices grouped by
-    level (level pointer), and the child-segment tops of every segment."""
+    level (level pointer; longest first within a level), and the child-segment tops of
+    every segment."""
     parent = np.asarray(parent, dtype=np.int64)
     n = len(parent)
     nch = np.bincount(parent[parent >= 0], minlength=n)
@@ -369,7 +370,10 @@ def segments(parent: np.ndarray):
     for q in np.argsort([ch[0] for ch in seg_nodes])[::-1]:
         if child_segs[q]:
             level[q] = 1 + max(level[c] for c in child_segs[q])
-    seg_order = np.lexsort((np.array([ch[0] for ch in seg_nodes]), level))
+    # Within a level, longer segments first: the threads of a wavefront then walk chains
+    # of similar length.
+    seg_len = np.array([len(ch) for ch in seg_nodes], dtype=np.int64)
+    seg_order = np.lexsort((np.array([ch[0] for ch in seg_nodes]), -seg_len, level))
     korder = np.concatenate([seg_nodes[q] for q in seg_order]).astype(np.int64)
     kidx = np.empty(n, dtype=np.int64)
     kidx[korder] = np.arange(n)

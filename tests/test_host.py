@@ -101,6 +101,8 @@ def cpu_model(monkeypatch):
 
     monkeypatch.setattr(sys.modules["jaxhip.integrate"], "_require_gpu", lambda d: None)
     monkeypatch.setattr(F.ForwardLaunch, "launch", lambda self: run_forward_model(self))
+    # No launch-shape measurement on the CPU (the heuristic shape is used).
+    monkeypatch.setenv("JAXHIP_AUTOTUNE", "0")
 
 
 def test_host_tables_c0(cpu_model):
