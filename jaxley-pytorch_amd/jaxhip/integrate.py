@@ -207,7 +207,13 @@ def set_float32_mode(mode: str) -> None:
     _F32_MODE = mode
 
 
-def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device, user_dtype):
+class Problem:
+    """A simulation prepared for the fused kernel: structure, node tables, batched
+    conductances, initial states, stimulus and synapses (see `_prepare`)."""
+
+
+def _prepare(module, inp, dt, device, user_dtype) -> Problem:
+    """Everything the fused kernel needs, independent of the launch shape."""
     # Element type of gates, rates, parameters and recordings in the kernel.
     dtype = torch.float64 if (user_dtype == torch.float32 and _F32_MODE == "mixed") else user_dtype
     vals = {k: v.to(dtype) for k, v in inp.vals.items()}
@@ -359,6 +365,22 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
                 s0 = inp.all_states[f"{stype.name}_s"].to(dtype)
             s0 = torch.broadcast_to(s0, (*batch, s0.shape[-1])).reshape(B, -1)
             syn_s0 = s0[:, perm]
+    P = Problem()
+    P.module, P.st, P.node_f, P.node_p, P.bmode = module, st, node_f, node_p, bmode
+    P.bparam_for, P.v0, P.gates0, P.stim, P.stim_scale = bparam_for, v0, gates0, stim, stim_scale
+    P.syn_params, P.syn_scale, P.syn_s0 = syn_params, syn_scale, syn_s0
+    P.B, P.batch, P.T, P.dt, P.dtype, P.user_dtype = B, batch, inp.T, dt, dtype, user_dtype
+    P.needs_grad, P.device, P.stim_requires_grad = needs_grad, device, (
+        inp.stim is not None and inp.stim.requires_grad)
+    return P
+
+
+def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device, user_dtype):
+    P = _prepare(module, inp, dt, device, user_dtype)
+    st, node_f, node_p, bmode, bparam_for = P.st, P.node_f, P.node_p, P.bmode, P.bparam_for
+    v0, gates0, stim, stim_scale = P.v0, P.gates0, P.stim, P.stim_scale
+    syn_params, syn_scale, syn_s0 = P.syn_params, P.syn_scale, P.syn_s0
+    B, batch, dtype, needs_grad = P.B, P.batch, P.dtype, P.needs_grad
     # Launch shape (measured on a short run of the real problem when not cached).
     from .autotune import _time, fused_config
 
@@ -380,7 +402,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
     if needs_grad:
         from .adjoint import fused_integrate_with_adjoint
 
-        if st.syn is not None or (inp.stim is not None and inp.stim.requires_grad):
+        if st.syn is not None or P.stim_requires_grad:
             raise NotImplementedError("the hand-written adjoint covers HH / Leak cells "
                                       "without synapses (stimulus without gradient)")
         K = checkpoint_lengths[-1] if checkpoint_lengths else 100

@@ -105,3 +105,37 @@ def data_set_params(cell, params: dict, device, dtype):
 def oracle_overrides(om, params: dict):
     nc = len(om.internal)
     return {k: np.broadcast_to(v, (v.shape[0], nc)).copy() for k, v in params.items()}
+
+
+def c2_specs(n_req: int, seed: int = 7):
+    """Uneven-batch workload: `n_req` random trees with compartment counts log-uniform in
+    [20, 2000] (4 per branch), per-request HH conductances in +-50 % of the defaults and
+    t_max uniform in [10, 100] ms."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n_req):
+        ncomp = int(4 * round(np.exp(rng.uniform(np.log(20), np.log(2000))) / 4))
+        scale = rng.uniform(0.5, 1.5, size=3)
+        specs.append({
+            "nbranch": max(ncomp // 4, 1),
+            "seed": int(rng.integers(0, 2**31 - 1)),
+            "t_max": float(rng.uniform(10.0, 100.0)),
+            "params": {"HH_gNa": 0.12 * scale[0:1][None], "HH_gK": 0.036 * scale[1:2][None],
+                       "HH_gLeak": 0.0003 * scale[2:3][None]},
+        })
+    return specs
+
+
+def c2_request(spec, device, dtype, cell=None):
+    """A served request for one spec (the cell built here unless given): 0.2 nA step into
+    the root from 1 to 21 ms, voltage recorded at the root."""
+    from jaxhip.serve import SimRequest
+
+    if cell is None:
+        cell, _ = build_tree(spec["nbranch"], spec["seed"])
+        cell.branch(0).comp(0).record()
+    T = int(spec["t_max"] // DT + 1)
+    cur = torch.tensor(c1_stimulus(T), dtype=dtype, device=device)
+    ds = cell.branch(0).comp(0).data_stimulate(cur)
+    ps = data_set_params(cell, spec["params"], device, dtype)
+    return SimRequest(cell, t_max=spec["t_max"], param_state=ps, data_stimuli=ds)

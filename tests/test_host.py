@@ -133,3 +133,35 @@ def test_host_tables_batched_tree(cpu_model):
                      overrides=oracle_overrides(om, prm))
     assert v.shape == (B, 2, T + 1)
     assert np.abs(v.numpy() - ref.numpy()).max() < 1e-9
+
+
+def test_serve_arena_allocator():
+    from jaxhip.serve import _Arena
+
+    a = _Arena(100)
+    x = a.alloc(10)
+    y = a.alloc(20, align=8)
+    z = a.alloc(70)
+    assert (x, y) == (0, 16) and z is None
+    assert a.alloc(6) == 10          # the alignment gap is reused
+    a.free(y, 20)
+    a.free(x, 10)
+    assert a.alloc(30) == 0 or a.free_list[0][0] == 0
+    rng = np.random.default_rng(0)
+    b, held = _Arena(1000), []
+    for _ in range(2000):
+        if held and rng.random() < 0.5:
+            off, sz = held.pop(int(rng.integers(len(held))))
+            b.free(off, sz)
+        else:
+            sz = int(rng.integers(1, 60))
+            off = b.alloc(sz, align=int(rng.choice([1, 4])))
+            if off is not None:
+                held.append((off, sz))
+        spans = sorted(held)
+        for (o1, s1), (o2, _) in zip(spans, spans[1:]):
+            assert o1 + s1 <= o2                    # no overlap
+        assert all(0 <= o and o + s <= 1000 for o, s in held)
+    for off, sz in held:
+        b.free(off, sz)
+    assert b.free_list == [(0, 1000)] and b.used == 0
