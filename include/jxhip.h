@@ -142,7 +142,7 @@ int jxhip_synapse_step_f64(const double* v, const double* s, const int32_t* pre,
  *   g_s (B, E) (kinetic only), g_params (n_params, E) summed over the batch
  *   (rows in parameter order; written for every parameter).
  * pre_nodes (n_pre) / inv (n_pre, width): transposed pre->synapse index (-1 padded)
- * used for a deterministic presynaptic sum; work: (E, B) scratch. */
+ * used for a deterministic presynaptic sum; work: 2 * E * B elements of scratch. */
 int jxhip_synapse_step_backward_f32(const float* v, const float* s, const int32_t* pre,
                                     const int32_t* targets, const float* const* params,
                                     const float* area, float dt, float diff,

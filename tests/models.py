@@ -139,3 +139,52 @@ def c2_request(spec, device, dtype, cell=None):
     ds = cell.branch(0).comp(0).data_stimulate(cur)
     ps = data_set_params(cell, spec["params"], device, dtype)
     return SimRequest(cell, t_max=spec["t_max"], param_state=ps, data_stimuli=ds)
+
+
+def build_c4(n_cells: int, k: int, seed: int, synapse: str = "KineticSynapse"):
+    """Ball-and-stick network: per cell a 1-compartment soma (radius 10 um, length 20 um,
+    the area of a 10 um sphere) with HH and an 8-compartment dendrite (200 um, radius
+    1 um) with Leak. Every cell receives k synapses from uniformly drawn presynaptic
+    somata, all onto one uniformly drawn dendritic compartment; synapses are sorted by
+    target. KineticSynapse: gS ~ U(0, 1e-3) uS, e_syn 0 mV (80 % of presynaptic cells)
+    or -80 mV; SigmoidRateSynapse: gS ~ U(0, 1e-3), default offset and slope.
+    Returns (network, oracle model, info)."""
+    rng = np.random.default_rng(seed)
+    cell = jx.Cell([jx.Branch(jx.Compartment(), 1), jx.Branch(jx.Compartment(), 8)], [-1, 0])
+    length = np.array([20.0] + [25.0] * 8)
+    radius = np.array([10.0] + [1.0] * 8)
+    cell.set("length", length)
+    cell.set("radius", radius)
+    cell.branch(0).insert(jx.HH())
+    cell.branch(1).insert(jx.Leak())
+    net = jx.Network([cell] * n_cells)
+    excit = rng.uniform(size=n_cells) < 0.8
+    pre_cells = rng.integers(0, n_cells, size=(n_cells, k))
+    tgt_comp = rng.integers(1, 9, size=n_cells)
+    pre_comp = (pre_cells * 9).reshape(-1)
+    post_comp = np.repeat(np.arange(n_cells) * 9 + tgt_comp, k)
+    syn = jx.KineticSynapse() if synapse == "KineticSynapse" else jx.SigmoidRateSynapse()
+    net.add_synapses(net._comp_node[pre_comp], net._comp_node[post_comp], syn)
+    E = len(pre_comp)
+    getattr(net, syn.name).set(f"{syn.name}_gS", rng.uniform(0.0, 1e-3, size=E))
+    if synapse == "KineticSynapse":
+        e_syn = np.where(excit[pre_cells.reshape(-1)], 0.0, -80.0)
+        getattr(net, syn.name).set(f"{syn.name}_e_syn", e_syn)
+    edge = net._edges[0]
+    params = {name: edge["params"][f"{syn.name}_{name}"].copy() for name in syn.param_names}
+    om = O.OracleModel([([-1, 0], [1, 8])] * n_cells)
+    om.length = np.tile(length, n_cells)
+    om.radius = np.tile(radius, n_cells)
+    om.hh_comps = np.arange(n_cells) * 9
+    om.leak_comps = np.setdiff1d(np.arange(9 * n_cells), om.hh_comps)
+    s0 = edge["states"][f"{syn.name}_s"].copy() if synapse == "KineticSynapse" else None
+    om.synapses = [{"type": synapse, "pre": om.internal[pre_comp], "post": om.internal[post_comp],
+                    "params": params, "s0": s0}]
+    info = {"soma_comps": np.arange(n_cells) * 9, "pre_comp": pre_comp, "post_comp": post_comp}
+    return net, om, info
+
+
+def c4_stimulus(B: int, n_cells: int, T: int, seed: int, amp: float = 0.05, p: float = 0.01):
+    """Bernoulli(p) pulses of `amp` nA per soma and step, (B, n_cells, T)."""
+    rng = np.random.default_rng(seed)
+    return amp * (rng.uniform(size=(B, n_cells, T)) < p)
