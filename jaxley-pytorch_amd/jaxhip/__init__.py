@@ -1,14 +1,14 @@
 """jaxhip: the per-time-step hot path of jaxley-pytorch's `jx.integrate` on MI355X.
 
 Drop-in surface (`import jaxhip as jx`): `Compartment`, `Branch`, `Cell`, `Network`,
-`connect`, `integrate`, `step_current`, `datapoint_to_step_currents`; channels `HH`,
+`connect`, `integrate`, `build_init_and_step_fn`, `step_current`, `datapoint_to_step_currents`; channels `HH`,
 `Leak`; synapses `KineticSynapse`, `SigmoidRateSynapse`; the GPU tree-solve operator
 `jaxhip.kernels.tree_solve.make_tree_solve_kernel`; the synapse operator
 `jaxhip.kernels.synapse_step`. Kernels are hand-written HIP for gfx950 in
 `libjxhip.so` (C ABI: include/jxhip.h).
 """
 from .channels import HH, Leak
-from .integrate import integrate
+from .integrate import build_init_and_step_fn, integrate
 from .modules import Branch, Cell, Compartment, Network, connect
 from .stimulus import datapoint_to_step_currents, step_current
 from .synapses import KineticSynapse, SigmoidRateSynapse
@@ -16,6 +16,7 @@ from .synapses import KineticSynapse, SigmoidRateSynapse
 __version__ = "0.1.0"
 
 __all__ = [
-    "Branch", "Cell", "Compartment", "Network", "connect", "integrate", "step_current",
+    "Branch", "Cell", "Compartment", "Network", "connect", "integrate", "build_init_and_step_fn",
+    "step_current",
     "datapoint_to_step_currents", "HH", "Leak", "KineticSynapse", "SigmoidRateSynapse",
 ]

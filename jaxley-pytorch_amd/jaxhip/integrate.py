@@ -81,7 +81,8 @@ class _Inputs:
     """Everything `integrate` assembles before the time loop."""
 
 
-def _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype):
+def _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype,
+              module_stimuli: bool = True):
     inp = _Inputs()
     pstate = params_to_pstate(params, module.indices_set_by_trainables)
     if param_state is not None:
@@ -107,7 +108,7 @@ def _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_sta
     inp.vals, inp.syn_vals = vals, syn_vals
     # Stimuli (…, n_stim, time) and their nodes.
     curs, nodes = [], []
-    ext = module.externals
+    ext = module.externals if module_stimuli else {}
     if "i" in ext:
         curs.append(ext["i"].to(device=device, dtype=dtype))
         nodes.append(module.external_inds["i"])
@@ -440,3 +441,165 @@ def _final_states(module, st, run, batch, B, dtype, device):
         inv[torch.as_tensor(st.syn["perm"], device=device)] = torch.arange(len(inv), device=device)
         out[f"{st.syn['type'].name}_s"] = xs[:, inv].reshape(*batch, -1)
     return out
+
+
+class _StepRunner:
+    """One fused step (T = 1) of a module with fixed structure, stimulus nodes and batch:
+    the kernel's buffers persist between steps; every call copies the states in, runs the
+    step on the current stream and copies them out (no host synchronization, so a step
+    can be captured in a CUDA / HIP graph)."""
+
+    def __init__(self, module, params, all_states, param_state, delta_t, ext_nodes, ext_batch):
+        ref = first_float_tensor(params, param_state, all_states)
+        device = ref.device if ref is not None else torch.get_default_device()
+        dtype = ref.dtype if ref is not None else torch.get_default_dtype()
+        _require_gpu(device)
+        ext_nodes = np.asarray(ext_nodes, dtype=np.int64)
+        ds = None
+        if len(ext_nodes):
+            dummy = torch.zeros((*ext_batch, len(ext_nodes), 1), dtype=dtype, device=device)
+            ds = (["i"], [dummy], [ext_nodes])
+        inp = _assemble(module, params, param_state, ds, delta_t, delta_t, all_states, device, dtype,
+                        module_stimuli=False)
+        inp.T = 1
+        if inp.stim is not None:
+            inp.stim = inp.stim[..., :1]
+        with torch.no_grad():
+            P = _prepare(module, inp, delta_t, device, dtype)
+        from .autotune import fused_config
+
+        cfg = fused_config(P.st, P.B, P.dtype, torch.tensor([], dtype=dtype).element_size(), False)
+        self.P, self.cfg, self.device, self.dtype = P, cfg, device, dtype
+        self.module, self.n_nodes = module, module._n_nodes
+        self.run = ForwardLaunch(P.st, P.node_f, P.node_p, None if P.bmode == 0 else P.bparam_for(cfg.S),
+                                 P.bmode, P.v0, tuple(P.gates0), P.stim, P.stim_scale, 1, delta_t, P.B, cfg,
+                                 P.syn_params, P.syn_scale, P.syn_s0)
+        self.mp = torch.as_tensor(P.st.mp, device=device)
+        hh = [k for k in ("HH_m", "HH_h", "HH_n") if k in module._node_data]
+        self.gate_names = hh
+        comp_of_k = torch.as_tensor(P.st.comp_of_k, device=device)
+        self.is_comp = comp_of_k >= 0
+        if P.st.syn is not None:
+            self.syn_perm = torch.as_tensor(P.st.syn["perm"], device=device)
+            inv = torch.empty_like(self.syn_perm)
+            inv[self.syn_perm] = torch.arange(len(inv), device=device)
+            self.syn_inv = inv
+
+    def step(self, all_states, externals):
+        from .fused import from_prog
+
+        P, run, B = self.P, self.run, self.P.B
+        v = torch.as_tensor(all_states["v"], dtype=torch.float64, device=self.device)
+        v = torch.broadcast_to(v, (*P.batch, self.n_nodes)).reshape(B, self.n_nodes)
+        run.v.copy_(to_prog(v[:, self.mp], run.S))
+        for buf, name in zip((run.gm, run.gh, run.gn), ("HH_m", "HH_h", "HH_n")):
+            if name in self.gate_names and name in all_states:
+                x = torch.as_tensor(all_states[name], device=self.device).to(run.dtype)
+                x = torch.broadcast_to(x, (*P.batch, self.n_nodes)).reshape(B, self.n_nodes)[:, self.mp]
+                x = torch.nan_to_num(x, nan=0.0)
+                buf.copy_(to_prog(x, run.S))
+        if run.syn_s is not None:
+            key = f"{P.st.syn['type'].name}_s"
+            if key in all_states:
+                s = torch.as_tensor(all_states[key], device=self.device).to(run.dtype)
+                s = torch.broadcast_to(s, (*P.batch, s.shape[-1])).reshape(B, -1)[:, self.syn_perm]
+                run.syn_s.copy_(to_prog(s, run.S))
+        if P.stim is not None:
+            cur = torch.as_tensor(externals["i"], device=self.device).to(run.dtype)
+            if run.stim.shape[2] == 1:
+                cur = cur.reshape(-1, cur.shape[-1])[:1]
+            else:
+                cur = torch.broadcast_to(cur, (*P.batch, cur.shape[-1])).reshape(B, -1)
+            run.stim.copy_(cur.t()[None])
+        run.launch()
+        out = dict(all_states)
+        vk = from_prog(run.v, B)
+        vn = torch.empty((B, self.n_nodes), dtype=self.dtype, device=self.device)
+        vn[:, self.mp] = vk.to(self.dtype)
+        out["v"] = vn.reshape(*P.batch, self.n_nodes)
+        for buf, name in zip((run.gm, run.gh, run.gn), ("HH_m", "HH_h", "HH_n")):
+            if name in self.gate_names:
+                gk = from_prog(buf, B).to(self.dtype)
+                gk = torch.where(self.is_comp[None], gk, torch.full_like(gk, float("nan")))
+                gn = torch.full((B, self.n_nodes), float("nan"), dtype=self.dtype, device=self.device)
+                gn[:, self.mp] = gk
+                if name in all_states:
+                    prev = torch.as_tensor(all_states[name], dtype=self.dtype, device=self.device)
+                    prev = torch.broadcast_to(prev, (*P.batch, self.n_nodes)).reshape(B, self.n_nodes)
+                    gn = torch.where(torch.isnan(prev), prev, gn)
+                out[name] = gn.reshape(*P.batch, self.n_nodes)
+        if run.syn_s is not None:
+            s = from_prog(run.syn_s, B)[:, self.syn_inv].to(self.dtype)
+            out[f"{P.st.syn['type'].name}_s"] = s.reshape(*P.batch, -1)
+        return out
+
+
+def build_init_and_step_fn(module, voltage_solver: str = "jaxley.dhs", solver: str = "bwd_euler"):
+    """`init_fn, step_fn` for step-by-step simulation (integrate.py:87-274).
+
+    * `init_fn(params, all_states=None, param_state=None, delta_t=0.025)` returns
+      `(all_states, all_params)`: the states over nodes (`v`, HH gates with NaN where
+      the channel is absent, synapse states) and the parameters to pass to `step_fn`.
+    * `step_fn(all_states, all_params, externals, external_inds, delta_t=0.025)` runs one
+      backward-Euler step with the currents `externals["i"]` (`(..., n_inputs)`, nA) into
+      the nodes `external_inds["i"]` and returns the new states.
+
+    The step is one launch of the fused kernel with T = 1; its buffers are kept between
+    calls (keyed by the stimulus nodes and batch shape), so a step can be captured in a
+    CUDA / HIP graph. Gradients are not propagated through `step_fn` (the differentiable
+    path is `integrate`)."""
+    voltage_solver = _resolve_voltage_solver(module, voltage_solver)
+    if voltage_solver not in _SUPPORTED_SOLVERS:
+        raise NotImplementedError(f"voltage_solver={voltage_solver!r} is not part of jaxhip")
+    if solver != "bwd_euler":
+        raise NotImplementedError(f"solver={solver!r}: only 'bwd_euler' is implemented")
+    external_inds0 = module.external_inds.copy()
+    cache: Dict[Any, _StepRunner] = {}
+
+    def init_fn(params, all_states=None, param_state=None, delta_t: float = 0.025):
+        ref = first_float_tensor(params, param_state, all_states, module.externals)
+        device = ref.device if ref is not None else torch.get_default_device()
+        dtype = ref.dtype if ref is not None else torch.get_default_dtype()
+        _require_gpu(device)
+        n = module._n_nodes
+        comp_node = torch.as_tensor(module._comp_node, device=device)
+        states = {}
+        inp = _assemble(module, params, param_state, None, delta_t, delta_t, None, device, dtype,
+                        module_stimuli=False)
+        for k in ["v"] + [k for k in ("HH_m", "HH_h", "HH_n") if k in inp.vals]:
+            x = inp.vals[k]
+            out = torch.full((*x.shape[:-1], n), float("nan"), dtype=dtype, device=device)
+            out[..., comp_node] = x
+            if k == "v":   # branchpoints start at their parent compartment's voltage
+                out = torch.nan_to_num(out, nan=float(x.reshape(-1)[0]))
+            states[k] = out
+        for k, x in inp.syn_vals.items():
+            if k.endswith("_s"):
+                states[k] = x
+        if all_states is not None:
+            states.update({k: torch.as_tensor(v, device=device) for k, v in all_states.items()})
+        all_params = {"params": params, "param_state": param_state, "delta_t": delta_t}
+        return states, all_params
+
+    def step_fn(all_states, all_params, externals, external_inds=external_inds0, delta_t: float = 0.025):
+        inds = np.zeros(0, np.int64)
+        if "i" in externals:
+            inds = np.asarray(external_inds["i"], dtype=np.int64)
+        v = all_states["v"]
+        batch = tuple(v.shape[:-1])
+        if "i" in externals:
+            batch = tuple(torch.broadcast_shapes(batch, tuple(torch.as_tensor(externals["i"]).shape[:-1])))
+        key = (tuple(inds.tolist()), batch, float(delta_t), str(v.device), v.dtype, module._version,
+               id(all_params))
+        runner = cache.get(key)
+        if runner is None:
+            base = {k: torch.broadcast_to(torch.as_tensor(x), (*batch, x.shape[-1])).contiguous()
+                    for k, x in all_states.items()}
+            runner = _StepRunner(module, all_params["params"], base, all_params["param_state"],
+                                 delta_t, inds, batch)
+            cache.clear()
+            cache[key] = runner
+        with torch.no_grad():
+            return runner.step(all_states, externals)
+
+    return init_fn, step_fn
