@@ -4,7 +4,8 @@ Simulations of different morphologies, durations and parameters arrive as reques
 (`SimRequest`: a cell with its recordings, `t_max`, parameters and stimuli exactly as
 `jaxhip.integrate` takes them, batch size 1). The server
 
-* prepares each request on arrival (schedule and node tables, `integrate._prepare`),
+* prepares each request on arrival, on the host (schedule and node tables,
+  `integrate._prepare`; request tensors are best given on the CPU),
 * admits it when a slot and room in its HBM arenas are free (`hbm_budget` bytes for
   node tables, states, schedules, stimuli and recordings; first fit),
 * keeps the state of every admitted simulation resident in HBM between launches,
@@ -131,7 +132,8 @@ class SimServer:
 
     def __init__(self, device=None, dtype=torch.float32, delta_t: float = 0.025,
                  hbm_budget: int = 2 << 30, max_active: int = 1024, chunk_steps: int = 200,
-                 max_steps: int = 4001, max_stim_entries: int = 4096, autotune: bool = False):
+                 max_steps: int = 4001, max_stim_entries: int = 4096, autotune: bool = False,
+                 thread_order=(256, 512, 1024)):
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         _require_gpu(device)
         self.device, self.user_dtype, self.dt = device, dtype, float(delta_t)
@@ -139,6 +141,7 @@ class SimServer:
         self.gdtype = torch.float64 if (dtype == torch.float64 or mode == "mixed") else torch.float32
         self.ldtype = dtype
         self.chunk_steps, self.max_steps, self.autotune = int(chunk_steps), int(max_steps), autotune
+        self.thread_order = tuple(thread_order)
         gs, ls = torch.tensor([], dtype=self.gdtype).element_size(), torch.tensor([], dtype=self.ldtype).element_size()
         self.gsize, self.lsize = gs, ls
         # Arena capacities from the budget: per node 8 int32 + 4 solve + 8 parameters +
@@ -171,6 +174,7 @@ class SimServer:
         self._next_id = 0
         self._cfg_cache: Dict[int, LaunchConfig] = {}
         self.stream = torch.cuda.current_stream(dev)
+        self.up_stream = torch.cuda.Stream(dev)
         self.launches = 0
         self.steps_run = 0
 
@@ -179,10 +183,13 @@ class SimServer:
         h = SimHandle(self._next_id, request)
         self._next_id += 1
         m = request.module
+        # Prepared on the host (no GPU work, so a submission never waits for a running
+        # chunk); the tables go to the arenas at admission by asynchronous copies.
+        cpu = torch.device("cpu")
         inp = _assemble(m, request.params, request.param_state, request.data_stimuli,
-                        request.t_max, self.dt, request.all_states, self.device, self.user_dtype)
+                        request.t_max, self.dt, request.all_states, cpu, self.user_dtype)
         with torch.no_grad():
-            P = _prepare(m, inp, self.dt, self.device, self.user_dtype)
+            P = _prepare(m, inp, self.dt, cpu, self.user_dtype)
         if P.B != 1:
             raise ValueError("a served request is one simulation (batch size 1)")
         if P.st.syn is not None:
@@ -225,56 +232,60 @@ class SimServer:
         return True
 
     def _upload(self, L: _Live) -> None:
+        """Copies the request's tables and initial state into its arena places: pinned
+        host buffers, asynchronous copies on the upload stream (the compute stream waits
+        for it before the next chunk)."""
         P, st = L.P, L.P.st
         n, no = L.n, L.node_off
+        pins = []
+
+        def put(dst, src):
+            src = src.contiguous()
+            if src.device.type == "cpu":
+                src = src.pin_memory()
+                pins.append(src)
+            dst.copy_(src, non_blocking=True)
+
         ni = st.node_i.clone()
         ni[:, 4:6] += L.stim_off
-        self.node_i[no:no + n] = ni
-        self.node_f[no:no + n] = P.node_f.to(self.ldtype)
         node_p = P.node_p.clone()
         if P.bmode:
             bp = P.bparam_for(1)                       # (1, 4) or (1, n, 1, 4)
-            cols = (0, 1, 2, 6)
             flags = st.node_i[:, 3]
-            for c, j in enumerate(cols):
+            for c, j in enumerate((0, 1, 2, 6)):
                 val = bp[0, c] if P.bmode == 1 else bp.reshape(n, 4)[:, c]
                 mask = (flags & (2 if j < 6 else 4)) != 0
                 node_p[:, j] = torch.where(mask, val.to(node_p.dtype) * torch.ones_like(node_p[:, j]), node_p[:, j])
-        self.node_p[no:no + n] = node_p.to(self.gdtype)
-        self.v[no:no + n] = P.v0[0].double()
-        for buf, g in zip((self.gm, self.gh, self.gn), P.gates0):
-            buf[no:no + n] = g[0].to(self.gdtype)
         a = L.aux_off
         sb = a // 4
-        seg = st.seg.reshape(-1)
-        self.aux[a:a + seg.numel()] = seg
-        p = a + seg.numel()
-        lb = p
-        self.aux[p:p + st.L + 2] = st.seg_level_ptr[:st.L + 2]
-        p += st.L + 2
-        scb = p
-        if L.ncs:
-            self.aux[p:p + L.ncs] = st.seg_child[:L.ncs]
-        p += L.ncs
-        cb = p
-        self.aux[p:p + L.nchild] = st.child_idx
-        p += L.nchild
-        rb = p
-        self.aux[p:p + max(L.n_rec, 1)] = st.rec_node[:max(L.n_rec, 1)]
+        parts = [st.seg.reshape(-1), st.seg_level_ptr[:st.L + 2], st.seg_child[:L.ncs], st.child_idx,
+                 st.rec_node[:max(L.n_rec, 1)]]
+        offs = np.cumsum([a] + [int(x.numel()) for x in parts])
+        lb, scb, cb, rb = (int(x) for x in offs[1:5])
         row = torch.tensor([n, st.L, no, lb, cb, L.n_rec, rb, 0, 0, 0, 0, 0, sb, scb, 0, 0],
                            dtype=torch.int32)
-        self.topos[L.slot] = row.to(self.device)
-        if L.n_stim:
-            e0 = L.stim_off
-            self.stim_col[e0:e0 + L.n_stim] = st.stim_col[:L.n_stim] + e0
-            self.stim_scale[e0:e0 + L.n_stim] = P.stim_scale.to(self.gdtype)
-            self.stim[:P.T, e0:e0 + P.stim.shape[1], 0] = P.stim[:, :, 0].to(self.gdtype)
+        with torch.cuda.stream(self.up_stream):
+            put(self.node_i[no:no + n], ni)
+            put(self.node_f[no:no + n], P.node_f.to(self.ldtype))
+            put(self.node_p[no:no + n], node_p.to(self.gdtype))
+            put(self.v[no:no + n], P.v0[0].double())
+            for buf, g in zip((self.gm, self.gh, self.gn), P.gates0):
+                put(buf[no:no + n], g[0].to(self.gdtype))
+            put(self.aux[a:int(offs[-1])], torch.cat([x.to(torch.int32) for x in parts]))
+            put(self.topos[L.slot], row)
+            if L.n_stim:
+                e0 = L.stim_off
+                put(self.stim_col[e0:e0 + L.n_stim], st.stim_col[:L.n_stim] + e0)
+                put(self.stim_scale[e0:e0 + L.n_stim], P.stim_scale.to(self.gdtype))
+                put(self.stim[:P.T, e0:e0 + P.stim.shape[1], 0], P.stim[:, :, 0].to(self.gdtype))
+        L.pins = pins
 
     def _retire(self, L: _Live) -> None:
         h = L.handle
         r = self.rec[L.rec_off:L.rec_off + (L.T + 1) * max(L.n_rec, 1)]
         h.result = r.reshape(L.T + 1, max(L.n_rec, 1))[:, :L.n_rec].t().to(self.user_dtype).clone()
         h.t_done = time.perf_counter()
+        L.pins = None
         for arena, off, size in L.allocs:
             arena.free(off, size)
         self.slots.append(L.slot)
@@ -283,7 +294,10 @@ class SimServer:
 
     # ---------------------------------------------------------------- launches
     def _config(self, n_max: int, sched_max: int) -> LaunchConfig:
-        for thr in (256, 512, 1024):
+        # A program is one simulation: the first thread count of `thread_order` whose
+        # registers hold the largest tree (small workgroups let several programs share a
+        # CU, which hides the latency of each program's tree solve).
+        for thr in self.thread_order:
             if items_per_thread(n_max, 1, thr, self.gsize, self.lsize):
                 cfg = LaunchConfig(1, thr)
                 break
@@ -335,6 +349,7 @@ class SimServer:
         if cfg is None:
             cfg = self._tune(n_max, sched_max, progs) if self.autotune else self._config(n_max, sched_max)
             self._cfg_cache[n_max.bit_length()] = cfg
+        self.stream.wait_stream(self.up_stream)
         host = torch.from_numpy(progs).pin_memory()
         dev = host.to(self.device, non_blocking=True)
         a = self._args(dev, len(lives), cfg, n_max, sched_max)
@@ -350,7 +365,7 @@ class SimServer:
     def _tune(self, n_max, sched_max, progs) -> LaunchConfig:
         """Measure the candidate shapes on copies of the live state (a few steps)."""
         cands = []
-        for thr in (256, 512, 1024):
+        for thr in self.thread_order:
             if items_per_thread(n_max, 1, thr, self.gsize, self.lsize):
                 cands.append(LaunchConfig(1, thr))
         g = lds_bytes(n_max, 0, 1, self.lsize, sched_max) > 160 * 1024

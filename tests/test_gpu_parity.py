@@ -140,3 +140,49 @@ def test_adjoint_matches_oracle_autograd(device):
     for j, p in enumerate(params):
         g = list(p.values())[0].grad
         np.testing.assert_allclose(g.cpu().numpy(), ovals.grad[:, j].numpy(), rtol=1e-6, atol=1e-9)
+
+
+def _c3_loss(cell, params_vals, ds, target, dtype, device, K=50):
+    params = [{k: torch.tensor(params_vals[:, j], dtype=dtype, device=device, requires_grad=True)}
+              for j, k in enumerate(("HH_gNa", "HH_gK", "HH_gLeak"))]
+    v = jx.integrate(cell, params, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[4, K])
+    loss = ((v[:, 0].double() - target) ** 2).mean()
+    return loss, params
+
+
+def test_adjoint_vs_central_differences(device):
+    """Per-branch conductance gradients (fp64 adjoint, checkpoints every 50 steps) against
+    central finite differences with h = 1e-6 on 12 random parameters; the float32 adjoint
+    against the float64 one."""
+    nb, B, T = 12, 2, 200
+    cell, _ = build_tree(nb, seed=9)
+    cell.branch(0).comp(0).record()
+    for k in ("HH_gNa", "HH_gK", "HH_gLeak"):
+        cell.branch("all").make_trainable(k, verbose=False)
+    rng = np.random.default_rng(2)
+    vals = np.array([0.12, 0.036, 0.0003])[None, :, None] * rng.uniform(0.7, 1.3, size=(B, 3, nb))
+    cur = torch.tensor(c1_stimulus(T), dtype=torch.float64, device=device)
+    ds = cell.branch(0).comp(0).data_stimulate(cur)
+    target = torch.linspace(-70, -60, T + 1, dtype=torch.float64, device=device)
+    loss, params = _c3_loss(cell, vals, ds, target, torch.float64, device)
+    loss.backward()
+    grads = np.stack([list(p.values())[0].grad.cpu().numpy() for p in params], 1)   # (B, 3, nb)
+    h = 1e-6
+    with torch.no_grad():
+        for _ in range(12):
+            b, j, i = int(rng.integers(B)), int(rng.integers(3)), int(rng.integers(nb))
+            scale = vals[b, j, i]
+            up, dn = vals.copy(), vals.copy()
+            up[b, j, i] += h * scale
+            dn[b, j, i] -= h * scale
+            fd = (_c3_loss(cell, up, ds, target, torch.float64, device)[0].item()
+                  - _c3_loss(cell, dn, ds, target, torch.float64, device)[0].item()) / (2 * h * scale)
+            print("fd", b, j, i, fd, grads[b, j, i])
+            assert abs(fd - grads[b, j, i]) <= 1e-4 * abs(grads[b, j, i]) + 1e-7
+    ds32 = cell.branch(0).comp(0).data_stimulate(cur.float())
+    loss32, p32 = _c3_loss(cell, vals, ds32, target, torch.float32, device)
+    loss32.backward()
+    g32 = np.stack([list(p.values())[0].grad.double().cpu().numpy() for p in p32], 1)
+    rel = np.abs(g32 - grads).max() / np.abs(grads).max()
+    print("fp32 vs fp64 gradient, max error relative to the largest gradient", rel)
+    assert rel < 1e-2
