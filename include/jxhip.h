@@ -44,7 +44,11 @@ extern "C" {
 int jxhip_version(void);
 
 /* ---------------------------------------------------------------------------------
- * Standalone tree solve (drop-in for the reference's two custom ops).
+ * Standalone tree solve (drop-in for the reference's two custom ops:
+ * jxhip_tree_solve_levels_* replaces `jaxley_pytorch::tree_solve_levels`
+ * (jaxley_pytorch/kernels/tree_solve.py:589-630, kernel :290-375), jxhip_tree_solve_regs_*
+ * replaces `jaxley_pytorch::tree_solve` (tree_solve.py:518-587, kernel :228-287); both
+ * are reached from `dhs_solve` (solver_voltage.py:237-240)).
  *
  * Solves A x = s for every row of the (n_rows, n) row-major, contiguous inputs with
  *   A[i,i] = d[i],  A[p,c] = u[c],  A[c,p] = l[c]   (p = parent of c).

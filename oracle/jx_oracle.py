@@ -10,7 +10,7 @@ schedules, synapse ordering) and follows the reference's op order for the floati
 point work (finite-difference linearization with `diff = 1e-3`, `save_exp` clamps,
 the `x_over_expm1` series switch, level-by-level `index_add` elimination).
 
-Pinning (what checks each part; see tests/test_oracle_*.py):
+Pinning (what checks each part; tests/test_oracle.py, tests/test_host.py):
   * morphology tables, axial conductances, HH, stimulus, implicit Euler step and the
     DHS solve: the reference's golden traces (tests/jaxley_identical/*), restated as
     fixtures under tests/golden/, at 1e-8 mV;
