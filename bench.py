@@ -52,7 +52,7 @@ BYTES_PER_COMP_STEP = {"f64": 64, "f32": 32}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
 # 256 MiB device copy: fetch x 2.0, write x 1.0) for the headline launch shape; reported
 # as roofline.traffic when the autotuner picks that shape.
-MEASURED_TRAFFIC = {("float64", 4096, 2000, "S=8, threads=1024, stream=True, gsched=True"): 627.2e9}
+MEASURED_TRAFFIC = {}
 
 
 def log(msg):

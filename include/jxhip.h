@@ -270,9 +270,9 @@ typedef struct jxhip_fused_args {
   int64_t max_items;        /* largest n * S over the programs of the launch; with
                                items_per_thread > 0 it must be <= items_per_thread *
                                threads (else the call fails with an invalid argument) */
-  int32_t sched_global;     /* forward, streaming only: 1 = the solve schedule is read
-                               from HBM / L2 (lds_bytes then excludes it), 0 = it is
-                               staged in LDS */
+  int32_t coef_global;      /* forward, streaming only: 1 = the tree's coupling
+                               coefficients are read from node_f (HBM / L2) and lds_bytes
+                               excludes their LDS copy; 0 = copied to LDS */
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

@@ -255,9 +255,8 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
 struct Sched {
   const int* lptr;
   const int4* sq;
-  const int* sp;        // NULL: the parent comes from the node table ni
+  const int* sp;
   const int* sch;
-  const int32_t* ni;
   int lw;  // levels lw .. L hold at most 64 (segment, simulation) pairs each
 };
 
@@ -269,18 +268,6 @@ __device__ __forceinline__ int wave_levels(const Fused<G, L>& a, const Topo& tp)
   return lw;
 }
 
-// The schedule read in place from HBM / L2 (when the LDS has no room for it).
-template <typename G, typename L>
-__device__ __forceinline__ Sched global_schedule(const Fused<G, L>& a, const Topo& tp) {
-  Sched sc;
-  sc.lptr = a.seg_lptr + tp.lb;
-  sc.sq = reinterpret_cast<const int4*>(a.seg + static_cast<int64_t>(tp.sb) * 4);
-  sc.sp = nullptr;
-  sc.sch = a.seg_child + tp.scb;
-  sc.ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
-  sc.lw = wave_levels(a, tp);
-  return sc;
-}
 
 // Bytes of a staged schedule (a multiple of 16).
 __host__ __device__ __forceinline__ int sched_bytes(int nseg, int L, int ncs) {
@@ -333,17 +320,17 @@ __device__ __forceinline__ Sched stage_schedule(const Fused<G, L>& a, const Topo
   sc.sq = sq;
   sc.sp = sp;
   sc.sch = sch;
-  sc.ni = ni;
   sc.lw = wave_levels(a, tp);
   return sc;
 }
 
-// Lower / upper coefficient pair of node k from LDS in one load; with kT (solve with
-// A^T) the two swap roles.
-template <typename L, bool kT>
+// Lower / upper coefficient pair of node k in one load; with kT (solve with A^T) the two
+// swap roles. kCS = 0: packed pairs (LDS copy); kCS = 1: the node table rows (lo, up,
+// dbase, inv_cm) in HBM / L2.
+template <typename L, bool kT, int kCS = 0>
 __device__ __forceinline__ void coef(const L* cl, int k, L& lo, L& up) {
   using V = typename Vec2<L>::type;
-  const V c = reinterpret_cast<const V*>(cl)[k];
+  const V c = reinterpret_cast<const V*>(cl)[k << kCS];
   lo = kT ? c.y : c.x;
   up = kT ? c.x : c.y;
 }
@@ -353,7 +340,7 @@ __device__ __forceinline__ void coef(const L* cl, int k, L& lo, L& up) {
 // 1 / (eliminated diagonal), xs the eliminated rhs. The walk is latency-bound (each node
 // depends on the previous one): the LDS operands of kB nodes (kC children) are loaded
 // together before their dependent updates.
-template <typename L, bool kT>
+template <typename L, bool kT, int kCS>
 __device__ __forceinline__ void elim_segment(const Sched& sc, int q, int s, int lS, L* xs, L* ds,
                                              const L* cl) {
   constexpr int kB = 4;
@@ -369,7 +356,7 @@ __device__ __forceinline__ void elim_segment(const Sched& sc, int q, int s, int 
     for (int u = 0; u < kC; ++u) c4[u] = sc.sch[cb + (j0 + u < nch ? j0 + u : nch - 1)];
 #pragma unroll
     for (int u = 0; u < kC; ++u) {
-      coef<L, kT>(cl, c4[u], lo4[u], up4[u]);
+      coef<L, kT, kCS>(cl, c4[u], lo4[u], up4[u]);
       dc4[u] = ds[(c4[u] << lS) + s];
       xc4[u] = xs[(c4[u] << lS) + s];
     }
@@ -386,60 +373,73 @@ __device__ __forceinline__ void elim_segment(const Sched& sc, int q, int s, int 
   L inv = recip(dk);
   ds[id] = inv;
   xs[id] = xk;
-  // Node k uses the coefficients of k + 1 and its own diagonal and rhs. Indices below beg
-  // are clamped to beg (loaded, not used).
-  for (int k = end - 2; k >= beg; k -= kB) {
+  // Node k uses the coefficients of k + 1 and its own diagonal and rhs: full batches of
+  // kB nodes without guards, then the remaining nodes one by one.
+  int k = end - 2;
+  for (; k - (kB - 1) >= beg; k -= kB) {
     L lo8[kB], up8[kB], d8[kB], x8[kB];
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      const int kk = k - u < beg ? beg : k - u;
-      coef<L, kT>(cl, kk + 1, lo8[u], up8[u]);
-      d8[u] = ds[(kk << lS) + s];
-      x8[u] = xs[(kk << lS) + s];
+      coef<L, kT, kCS>(cl, k - u + 1, lo8[u], up8[u]);
+      d8[u] = ds[((k - u) << lS) + s];
+      x8[u] = xs[((k - u) << lS) + s];
     }
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      if (k - u >= beg) {
-        const L f = up8[u] * inv;
-        const L d2 = d8[u] - lo8[u] * f;
-        xk = x8[u] - xk * f;
-        inv = recip(d2);
-        const int idu = ((k - u) << lS) + s;
-        ds[idu] = inv;
-        xs[idu] = xk;
-      }
+      const L f = up8[u] * inv;
+      const L d2 = d8[u] - lo8[u] * f;
+      xk = x8[u] - xk * f;
+      inv = recip(d2);
+      const int idu = ((k - u) << lS) + s;
+      ds[idu] = inv;
+      xs[idu] = xk;
     }
+  }
+  for (; k >= beg; --k) {
+    L lo1, up1;
+    coef<L, kT, kCS>(cl, k + 1, lo1, up1);
+    const int id1 = (k << lS) + s;
+    const L f = up1 * inv;
+    const L d2 = ds[id1] - lo1 * f;
+    xk = xs[id1] - xk * f;
+    inv = recip(d2);
+    ds[id1] = inv;
+    xs[id1] = xk;
   }
 }
 
 // Back-substitution of segment q of simulation s, its parent segment being solved.
-template <typename L, bool kT>
+template <typename L, bool kT, int kCS>
 __device__ __forceinline__ void back_segment(const Sched& sc, int q, int s, int lS, L* xs,
                                              const L* ds, const L* cl) {
   constexpr int kB = 4;
   const int4 sg = sc.sq[q];
   const int beg = sg.x, end = sg.y;
-  const int p = sc.sp != nullptr ? sc.sp[q] : sc.ni[beg * JXHIP_NODE_ICOLS + kNiParent];
+  const int p = sc.sp[q];
   // Roots have zero coupling coefficients, so xp = 0 there adds nothing.
   L xp = p >= 0 ? xs[(p << lS) + s] : L(0);
-  const int last = end - 1;
-  for (int k = beg; k < end; k += kB) {
+  int k = beg;
+  for (; k + (kB - 1) < end; k += kB) {
     L lo8[kB], d8[kB], x8[kB];
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      const int kk = k + u > last ? last : k + u;
       L up_unused;
-      coef<L, kT>(cl, kk, lo8[u], up_unused);
-      d8[u] = ds[(kk << lS) + s];
-      x8[u] = xs[(kk << lS) + s];
+      coef<L, kT, kCS>(cl, k + u, lo8[u], up_unused);
+      d8[u] = ds[((k + u) << lS) + s];
+      x8[u] = xs[((k + u) << lS) + s];
     }
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      if (k + u < end) {
-        xp = (x8[u] - lo8[u] * xp) * d8[u];
-        xs[((k + u) << lS) + s] = xp;
-      }
+      xp = (x8[u] - lo8[u] * xp) * d8[u];
+      xs[((k + u) << lS) + s] = xp;
     }
+  }
+  for (; k < end; ++k) {
+    L lo1, up_unused;
+    coef<L, kT, kCS>(cl, k, lo1, up_unused);
+    const int id1 = (k << lS) + s;
+    xp = (xs[id1] - lo1 * xp) * ds[id1];
+    xs[id1] = xp;
   }
 }
 
@@ -455,7 +455,7 @@ __device__ __forceinline__ void wave_sync() {
 // (segment, simulation). The upper levels with at most 64 pairs each (sc.lw .. L) are
 // processed by the first wavefront alone, without workgroup barriers. kT: solve with A^T
 // (lower <-> upper).
-template <typename G, typename L, bool kT>
+template <typename G, typename L, bool kT, int kCS = 0>
 __device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp, const Sched& sc,
                                            L* xs, L* ds, const L* cl) {
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -463,7 +463,7 @@ __device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp,
   int lo = sc.lptr[0];
   for (int lv = 0; lv < sc.lw; ++lv) {
     const int hi = sc.lptr[lv + 1], w = (hi - lo) << lS;
-    for (int i = tid; i < w; i += nthr) elim_segment<L, kT>(sc, lo + (i >> lS), i & (S - 1), lS, xs, ds, cl);
+    for (int i = tid; i < w; i += nthr) elim_segment<L, kT, kCS>(sc, lo + (i >> lS), i & (S - 1), lS, xs, ds, cl);
     lo = hi;
     __syncthreads();
   }
@@ -471,13 +471,13 @@ __device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp,
     if (tid < 64) {
       for (int lv = sc.lw; lv <= tp.L; ++lv) {
         const int hi = sc.lptr[lv + 1], w = (hi - lo) << lS;
-        if (tid < w) elim_segment<L, kT>(sc, lo + (tid >> lS), tid & (S - 1), lS, xs, ds, cl);
+        if (tid < w) elim_segment<L, kT, kCS>(sc, lo + (tid >> lS), tid & (S - 1), lS, xs, ds, cl);
         lo = hi;
         wave_sync();
       }
       for (int lv = tp.L; lv >= sc.lw; --lv) {
         const int l0 = sc.lptr[lv], w = (sc.lptr[lv + 1] - l0) << lS;
-        if (tid < w) back_segment<L, kT>(sc, l0 + (tid >> lS), tid & (S - 1), lS, xs, ds, cl);
+        if (tid < w) back_segment<L, kT, kCS>(sc, l0 + (tid >> lS), tid & (S - 1), lS, xs, ds, cl);
         wave_sync();
       }
     }
@@ -485,7 +485,7 @@ __device__ __forceinline__ void tree_solve(const Fused<G, L>& a, const Topo& tp,
   }
   for (int lv = sc.lw - 1; lv >= 0; --lv) {
     const int l0 = sc.lptr[lv], w = (sc.lptr[lv + 1] - l0) << lS;
-    for (int i = tid; i < w; i += nthr) back_segment<L, kT>(sc, l0 + (i >> lS), i & (S - 1), lS, xs, ds, cl);
+    for (int i = tid; i < w; i += nthr) back_segment<L, kT, kCS>(sc, l0 + (i >> lS), i & (S - 1), lS, xs, ds, cl);
     __syncthreads();
   }
 }
@@ -501,18 +501,21 @@ __device__ __forceinline__ void load_coefficients(const Fused<G, L>& a, const To
 }
 
 // Axial part of the residual of item (k, s), lo_k (x_k - x_parent) + sum_c up_c (x_k - x_c),
-// in float64 from values x in program layout (voltages, or increments of voltages).
-template <typename G, typename L, typename X>
+// in float64 from values x in program layout (voltages, or increments of voltages); the
+// coefficients as in `coef` (kCS).
+template <int kCS = 0, typename G, typename L, typename X>
 __device__ __forceinline__ double axial(const Fused<G, L>& a, const Topo& tp, const int32_t* nik,
                                         int k, const L* cl, const X* x, int s, double xk) {
   const int lS = a.log2S;
   double ax = 0.0;
   const int p = nik[kNiParent];
-  if (p >= 0) ax += static_cast<double>(cl[2 * k]) * (xk - static_cast<double>(x[(p << lS) + s]));
+  if (p >= 0) {
+    ax += static_cast<double>(cl[k << (kCS + 1)]) * (xk - static_cast<double>(x[(p << lS) + s]));
+  }
   const int32_t* cidx = a.child_idx + tp.cb;
   for (int j = nik[kNiChildBeg]; j < nik[kNiChildEnd]; ++j) {
     const int c = cidx[j];
-    ax += static_cast<double>(cl[2 * c + 1]) * (xk - static_cast<double>(x[(c << lS) + s]));
+    ax += static_cast<double>(cl[(c << (kCS + 1)) + 1]) * (xk - static_cast<double>(x[(c << lS) + s]));
   }
   return ax;
 }
@@ -576,12 +579,12 @@ __device__ __forceinline__ void build_core(const Fused<G, L>& a, const Prog& pg,
 }
 
 // build_core with the axial residual from the float64 voltages vg (program layout).
-template <typename G, typename L>
+template <int kCS = 0, typename G, typename L>
 __device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
                                            const Lds<L>& Ld, const double* vg, int k, int s,
                                            double v, int64_t t, G& m, G& h, G& n) {
   const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
-  const double ax = axial(a, tp, nik, k, Ld.cl, vg, s, v);
+  const double ax = axial<kCS>(a, tp, nik, k, Ld.cl, vg, s, v);
   build_core(a, pg, tp, Ld, nik, k, s, (k << a.log2S) + s, v, ax, t, m, h, n);
 }
 
@@ -610,28 +613,27 @@ __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& 
   __syncthreads();
 }
 
-// LDS of a forward program: the staged schedule (unless kGS: read from HBM / L2), the
-// lower / upper coefficient per node, rhs / increment and diagonal per (node,
-// simulation), synapse terms per (target, simulation). Stages the schedule and the
-// coefficients (the caller synchronizes).
-template <typename G, typename L, bool kGS = false>
+// LDS of a forward program: the staged schedule, the lower / upper coefficient per node
+// (unless kGC: read from the node table in HBM / L2), rhs / increment and diagonal per
+// (node, simulation), synapse terms per (target, simulation). Stages the schedule and
+// the coefficients (the caller synchronizes).
+template <typename G, typename L, bool kGC = false>
 __device__ __forceinline__ Lds<L> forward_lds(const Fused<G, L>& a, unsigned char* smem, int nS,
                                               const Topo& tp, int lS) {
   Lds<L> Ld;
-  int off = 0;
-  if constexpr (kGS) {
-    Ld.sc = global_schedule(a, tp);
+  const SchedSize z = schedule_size(a, tp);
+  Ld.sc = stage_schedule(a, tp, z, smem);
+  if constexpr (kGC) {
+    Ld.cl = const_cast<L*>(a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS);
+    Ld.xs = reinterpret_cast<L*>(smem + z.bytes);
   } else {
-    const SchedSize z = schedule_size(a, tp);
-    Ld.sc = stage_schedule(a, tp, z, smem);
-    off = z.bytes;
+    Ld.cl = reinterpret_cast<L*>(smem + z.bytes);
+    Ld.xs = Ld.cl + 2 * tp.n;
+    load_coefficients(a, tp, Ld.cl);
   }
-  Ld.cl = reinterpret_cast<L*>(smem + off);
-  Ld.xs = Ld.cl + 2 * tp.n;
   Ld.ds = Ld.xs + nS;
   Ld.tl = Ld.ds + nS;
   Ld.tc = Ld.tl + (tp.n_tgt << lS);
-  load_coefficients(a, tp, Ld.cl);
   return Ld;
 }
 
@@ -672,15 +674,17 @@ __device__ __forceinline__ void checkpoint_mem(const Fused<G, L>& a, const Prog&
 // Streaming forward: every step reads the voltages and gates of the program from HBM and
 // writes them back (for programs whose items do not fit the registers of their threads).
 // kSyn: the topologies carry synapses; kCkpt: checkpoints are written for the adjoint;
-// kGS: the solve schedule is read from HBM / L2 instead of being staged in LDS.
-template <typename G, typename L, bool kSyn, bool kCkpt, bool kGS>
+// kGC: the tree's coupling coefficients are read from the node table (HBM / L2) instead
+// of an LDS copy, which leaves the LDS to the system of more simulations.
+template <typename G, typename L, bool kSyn, bool kCkpt, bool kGC>
 __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
   const int lS = a.log2S;
   const int nS = tp.n << lS;
-  const Lds<L> Ld = forward_lds<G, L, kGS>(a, smem_raw, nS, tp, lS);
+  const Lds<L> Ld = forward_lds<G, L, kGC>(a, smem_raw, nS, tp, lS);
+  constexpr int kCS = kGC ? 1 : 0;
   __syncthreads();
   const int tid = threadIdx.x, nthr = blockDim.x;
   double* vg = a.v + pg.soff;
@@ -702,7 +706,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
       const int k = i >> lS, s = i & (a.S - 1);
       if (s < pg.nsim) {
         G m = gm[i], h = gh[i], n = gn[i];
-        build_item(a, pg, tp, Ld, vg, k, s, vg[i], t, m, h, n);
+        build_item<kCS>(a, pg, tp, Ld, vg, k, s, vg[i], t, m, h, n);
         gm[i] = m;
         gh[i] = h;
         gn[i] = n;
@@ -711,7 +715,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
       }
     }
     __syncthreads();
-    tree_solve<G, L, false>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+    tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
     for (int i = tid; i < nS; i += nthr) vg[i] += static_cast<double>(Ld.xs[i]);
     __syncthreads();
     record_step(a, pg, tp, vg, step + 1);
@@ -1093,10 +1097,10 @@ int launch_kernel(K kern, const jxhip_fused_args& a, hipStream_t stream, Args...
 template <typename G, typename L, bool kSyn, bool kCkpt>
 int launch_forward_feat(const jxhip_fused_args& a, const Fused<G, L>& f, hipStream_t st) {
   if (a.items_per_thread == 0) {
-    if (a.sched_global) return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, true>, a, st, f);
+    if (a.coef_global) return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, true>, a, st, f);
     return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, false>, a, st, f);
   }
-  if (a.sched_global) return hipErrorInvalidValue;
+  if (a.coef_global) return hipErrorInvalidValue;
   if (a.max_items > static_cast<int64_t>(a.items_per_thread) * a.threads) return hipErrorInvalidValue;
   constexpr bool kF = sizeof(G) == 4;
 #define JXHIP_REGS(IPT, MAXT)                                                            \

@@ -205,7 +205,7 @@ class LaunchConfig:
     S: int
     threads: int
     stream: bool = False       # streaming kernel even if the items fit the registers
-    gsched: bool = False       # streaming kernel reading the schedule from HBM / L2
+    gcoef: bool = False        # streaming kernel reading the coupling coefficients from HBM / L2
 
 
 # Register-resident items per thread with a forward kernel, by the element sizes of
@@ -223,10 +223,11 @@ def sched_bytes(nseg: int, L: int, ncs: int) -> int:
     return (16 * nseg + 4 * (L + 2 + nseg + ncs) + 15) & ~15
 
 
-def lds_bytes(n: int, n_tgt: int, S: int, esize: int, sched: int) -> int:
-    """Forward: the staged schedule, lower / upper coefficient per node, diagonal and
-    rhs per (node, simulation), synapse terms per (target, simulation)."""
-    return sched + 2 * n * esize + (2 * n + 2 * n_tgt) * S * esize
+def lds_bytes(n: int, n_tgt: int, S: int, esize: int, sched: int, gcoef: bool = False) -> int:
+    """Forward: the staged schedule, lower / upper coefficient per node (unless read from
+    HBM / L2), diagonal and rhs per (node, simulation), synapse terms per (target,
+    simulation)."""
+    return sched + (0 if gcoef else 2 * n * esize) + (2 * n + 2 * n_tgt) * S * esize
 
 
 def items_per_thread(n: int, S: int, threads: int, gsize: int, lsize: int) -> int:
@@ -246,14 +247,14 @@ def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
     """Launch shapes worth measuring: S simulations per program (a power of two, LDS
     permitting, at least 256 programs when the batch allows) with every thread count
     whose registers hold the items of a program, and the streaming kernel at each S
-    (with the schedule in HBM / L2 when only that fits the LDS)."""
+    (with the coupling coefficients in HBM / L2 when only that fits the LDS)."""
     out = []
     S = 1
     while S <= 64:
         if S > 1 and S * 256 > B:
             break
         fits = lds_bytes(n, n_tgt, S, lsize, sched) <= 160 * 1024
-        fits_g = lds_bytes(n, n_tgt, S, lsize, 0) <= 160 * 1024
+        fits_g = lds_bytes(n, n_tgt, S, lsize, sched, gcoef=True) <= 160 * 1024
         if not fits_g:
             break
         if fits:
@@ -262,7 +263,7 @@ def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
                     out.append(LaunchConfig(S, thr))
         thr = int(min(1024, max(64, 64 * -(-n * S // (64 * 16)))))
         for t in sorted({thr, 1024}):
-            out.append(LaunchConfig(S, t, stream=True, gsched=not fits))
+            out.append(LaunchConfig(S, t, stream=True, gcoef=not fits))
         S *= 2
     if not out:
         raise NotImplementedError(f"a topology of {n} nodes does not fit the fused kernel's LDS")
@@ -400,8 +401,8 @@ class ForwardLaunch:
         a.items_per_thread = 0 if cfg.stream else items_per_thread(
             n, S, cfg.threads, self.gm.element_size(), node_f.element_size())
         n_tgt = st.syn["n_tgt"] if st.syn else 0
-        a.sched_global = int(cfg.stream and cfg.gsched)
-        a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size(), 0 if a.sched_global else st.sched)
+        a.coef_global = int(cfg.stream and cfg.gcoef)
+        a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size(), st.sched, bool(a.coef_global))
         a.max_items = n * S
         if ckpt_every > 0:
             a.ckpt_v, a.ckpt_m, a.ckpt_h, a.ckpt_n = (c.data_ptr() for c in self.ckpt)

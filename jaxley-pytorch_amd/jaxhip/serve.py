@@ -303,7 +303,7 @@ class SimServer:
                 break
         else:
             g = lds_bytes(n_max, 0, 1, self.lsize, sched_max) > 160 * 1024
-            cfg = LaunchConfig(1, 1024, stream=True, gsched=g)
+            cfg = LaunchConfig(1, 1024, stream=True, gcoef=g)
         return cfg
 
     def _args(self, progs_dev, n_progs, cfg, n_max, sched_max):
@@ -322,8 +322,8 @@ class SimServer:
         a.dt, a.diff, a.S = self.dt, 1e-3, 1
         a.threads = cfg.threads
         a.items_per_thread = 0 if cfg.stream else items_per_thread(n_max, 1, cfg.threads, self.gsize, self.lsize)
-        a.sched_global = int(cfg.stream and cfg.gsched)
-        a.lds_bytes = lds_bytes(n_max, 0, 1, self.lsize, 0 if a.sched_global else sched_max)
+        a.coef_global = int(cfg.stream and cfg.gcoef)
+        a.lds_bytes = lds_bytes(n_max, 0, 1, self.lsize, sched_max, bool(a.coef_global))
         a.max_items = n_max
         return a
 
@@ -369,7 +369,7 @@ class SimServer:
             if items_per_thread(n_max, 1, thr, self.gsize, self.lsize):
                 cands.append(LaunchConfig(1, thr))
         g = lds_bytes(n_max, 0, 1, self.lsize, sched_max) > 160 * 1024
-        cands.append(LaunchConfig(1, 1024, stream=True, gsched=g))
+        cands.append(LaunchConfig(1, 1024, stream=True, gcoef=g))
         saved = [t.clone() for t in (self.v, self.gm, self.gh, self.gn, self.rec)]
         p = progs.copy()
         p[:, 3] = np.minimum(p[:, 2] + 16, p[:, 3])
