@@ -108,6 +108,11 @@ def kernel_time_ms(fn, device, reps=2):
     return float(np.mean([a.elapsed_time(b) for a, b in times]))
 
 
+def spikes(v: torch.Tensor) -> torch.Tensor:
+    """Upward crossings of 0 mV per simulation and recording."""
+    return ((v[..., 1:] >= 0.0) & (v[..., :-1] < 0.0)).sum(-1)
+
+
 def cpu_baseline(prm, T_sample, B_sample, threads):
     """The CPU oracle (a from-scratch restatement of the reference's level-parallel CPU
     step, float64, PyTorch on the host) on a bounded sample of the same workload."""
@@ -258,9 +263,11 @@ def main():
             k32 = kernel_time_ms(run32, device)
             r32 = run32()
             err = (r32.double() - rec64).abs().amax(dim=(1, 2))
+            spikes_equal = bool((spikes(r32.double()) == spikes(rec64)).all())
             out[f"f32_{mode}"] = {"value": comp_steps / s32, "kernel_ms": k32,
                                   "max_err_vs_f64_gpu_mV": float(err.max()),
-                                  "frac_sims_within_1e-2_mV": float((err <= 1e-2).double().mean())}
+                                  "frac_sims_within_1e-2_mV": float((err <= 1e-2).double().mean()),
+                                  "spike_counts_identical": spikes_equal}
             log(f"[bench] f32 {mode}: {comp_steps / s32:.3e} comp-steps/s, max err {float(err.max()):.3e} mV, "
                 f"{float((err <= 1e-2).double().mean()):.4f} within 1e-2")
         set_float32_mode(saved_mode)
@@ -328,6 +335,7 @@ def main():
                                      f"workload, float64, {el:.1f} s; CPU {platform.processor() or platform.machine()}"}
     err = float((rec64[:Bs].cpu() - ref).abs().max())
     out["max_err_f64_vs_oracle_mV"] = err
+    out["spike_counts_f64_vs_oracle_identical"] = bool((spikes(rec64[:Bs].cpu()) == spikes(ref)).all())
     log(f"[bench] cpu {cps:.3e} comp-steps/s; float64 GPU vs oracle {err:.3e} mV")
     print(json.dumps(out), flush=True)
 

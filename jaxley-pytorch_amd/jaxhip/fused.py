@@ -301,7 +301,9 @@ def grad_config(n: int, B: int, esize: int, sched: int = 0) -> LaunchConfig:
         S *= 2
     if adjoint_lds_bytes(n, S, esize, sched) > 160 * 1024:
         raise NotImplementedError(f"a topology of {n} nodes does not fit the reverse pass's LDS")
-    threads = int(min(ADJOINT_THREADS_MAX, max(64, 64 * -(-n * S // (64 * 16)))))
+    # As many threads as the reverse pass allows: its per-step work is latency-bound, so
+    # fewer items per thread shorten every step.
+    threads = int(min(ADJOINT_THREADS_MAX, max(64, 64 * -(-n * S // 64))))
     return LaunchConfig(S, threads)
 
 

@@ -27,12 +27,18 @@ def _run_c0(device, dtype, T=400):
     return v.double().cpu().numpy(), ref.numpy()
 
 
+def spike_counts(v: np.ndarray) -> np.ndarray:
+    """Upward crossings of 0 mV along the last axis (training/spiking_network.py:509-513)."""
+    return ((v[..., 1:] >= 0.0) & (v[..., :-1] < 0.0)).sum(-1)
+
+
 def test_c0_float64(device):
     v, ref = _run_c0(device, torch.float64)
     assert v.shape == (60, 401)
     err = np.abs(v - ref).max()
     print("c0 fp64 max err", err)
     assert err < 1e-6
+    assert np.array_equal(spike_counts(v), spike_counts(ref))
 
 
 def test_c0_float32(device):
@@ -40,6 +46,7 @@ def test_c0_float32(device):
     err = np.abs(v - ref).max()
     print("c0 fp32 max err", err)
     assert err < 1e-2
+    assert np.array_equal(spike_counts(v), spike_counts(ref))
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-6), (torch.float32, 1e-2)])
@@ -58,6 +65,8 @@ def test_batched_random_tree(device, dtype, tol):
     print(f"tree {dtype} max err", err)
     assert v.shape == (B, 1, T + 1)
     assert err < tol
+    assert np.array_equal(spike_counts(v.double().cpu().numpy()), spike_counts(ref.numpy()))
+    assert spike_counts(ref.numpy()).sum() > 0
 
 
 def _dense_solve(d, l, u, s, parent):
