@@ -273,6 +273,11 @@ typedef struct jxhip_fused_args {
   int32_t coef_global;      /* forward, streaming only: 1 = the tree's coupling
                                coefficients are read from node_f (HBM / L2) and lds_bytes
                                excludes their LDS copy; 0 = copied to LDS */
+  void* work_ax;            /* forward, streaming only (required there): scratch of the
+                               axial part of every item's residual, program layout
+                               [node][S] at state_off, gate type; computed from v at the
+                               start of the launch and updated from the increments after
+                               every solve. Unused by the register-resident kernel. */
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

@@ -27,7 +27,7 @@ __global__ void hh_step_kernel(const T* __restrict__ v, int64_t n_nodes,
   const int64_t j = i - row * n_ch;
   const T vv = v[row * n_nodes + inds[j]];
   T am, bm, ah, bh, an, bn;
-  hh_rates(vv, am, bm, ah, bh, an, bn);
+  hh_rates_step(vv, am, bm, ah, bh, an, bn);
   const T mm = gate_step(m[i], dt, am, bm);
   const T hh = gate_step(h[i], dt, ah, bh);
   const T nn = gate_step(n[i], dt, an, bn);

@@ -15,7 +15,8 @@ namespace jxhip {
 // float32 transcendentals: the hardware exp2 (v_exp_f32) after an exact-enough scaling,
 // and expm1 from its Taylor series near 0 (|x| < 1/4: degree 7, relative error < 2e-8)
 // where exp(x) - 1 would cancel. Define JXHIP_ACCURATE_F32 for the libm versions.
-// float64 uses the correctly rounded libm functions.
+// float64: exp and x / expm1 below are branch-free and accurate to a few ulp (a wave's
+// lanes never diverge on them); expm1 itself (adjoint and synapse slopes only) is libm's.
 #ifdef JXHIP_ACCURATE_F32
 __device__ __forceinline__ float dexp(float x) { return expf(x); }
 __device__ __forceinline__ float dexpm1(float x) { return expm1f(x); }
@@ -39,7 +40,46 @@ __device__ __forceinline__ double drcp(double x) {
   e = fma(-x, r, 1.0);
   return fma(r, e, r);
 }
-__device__ __forceinline__ double dexp(double x) { return exp(x); }
+// float64 exp without branches: Cody-Waite reduction x = k ln2 + r (ln2 split in a
+// 32-bit head and a tail, |r| <= ln2 / 2), the Taylor polynomial of degree 12 for e^r
+// (truncation below 2e-16 relative) and a scaling by 2^k. The argument is clamped to
+// [-746, 710], where the scaling gives 0 and infinity. Accurate to a few ulp.
+__device__ __forceinline__ double exp_f64(double x) {
+  x = fmin(fmax(x, -746.0), 710.0);
+  const double k = rint(x * 1.4426950408889634074);          // log2(e)
+  double r = fma(-k, 6.93147180369123816490e-01, x);           // ln2, head
+  r = fma(-k, 1.90821492927058770002e-10, r);                  // ln2, tail
+  double p = 2.08767569878680989792e-09;                       // 1/12!
+  p = fma(p, r, 2.50521083854417187751e-08);                   // 1/11!
+  p = fma(p, r, 2.75573192239858906526e-07);                   // 1/10!
+  p = fma(p, r, 2.75573192239858906526e-06);                   // 1/9!
+  p = fma(p, r, 2.48015873015873015873e-05);                   // 1/8!
+  p = fma(p, r, 1.98412698412698412698e-04);                   // 1/7!
+  p = fma(p, r, 1.38888888888888888889e-03);                   // 1/6!
+  p = fma(p, r, 8.33333333333333333333e-03);                   // 1/5!
+  p = fma(p, r, 4.16666666666666666667e-02);                   // 1/4!
+  p = fma(p, r, 1.66666666666666666667e-01);                   // 1/3!
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, static_cast<int>(k));
+}
+// z / expm1(z) for |z| < 1/4 from its Bernoulli series
+//   z / (e^z - 1) = 1 - z/2 + z^2/12 - z^4/720 + z^6/30240 - z^8/1209600
+//                   + z^10/47900160 - 691 z^12/1307674368000 + ...
+// (the next term is below 6e-20 at |z| = 1/4).
+__device__ __forceinline__ double z_over_expm1_series(double z) {
+  const double y = z * z;
+  double p = -5.28419013868749318484e-10;                      // -691/1307674368000
+  p = fma(p, y, 2.08767569878680989792e-08);                   // 1/47900160
+  p = fma(p, y, -8.26719576719576719577e-07);                  // -1/1209600
+  p = fma(p, y, 3.30687830687830687831e-05);                   // 1/30240
+  p = fma(p, y, -1.38888888888888888889e-03);                  // -1/720
+  p = fma(p, y, 8.33333333333333333333e-02);                   // 1/12
+  p = fma(p, y, 1.0);
+  return fma(-0.5, z, p);
+}
+__device__ __forceinline__ double dexp(double x) { return exp_f64(x); }
 __device__ __forceinline__ double dexpm1(double x) { return expm1(x); }
 __device__ __forceinline__ double ddiv(double a, double b) { return a * drcp(b); }
 
@@ -56,6 +96,17 @@ __device__ __forceinline__ T x_over_expm1(T x, T y) {
   if (fabs(z) < T(1e-4)) return y * (T(1) - z / T(2) + z * z / T(12));
   return ddiv(x, dexpm1(z));
 }
+// float64: y z / expm1(z) from the series for |z| < 1/4 (which also covers the
+// reference's three-term switch at 1e-4 to within 2e-19 relative), x / (e^z - 1)
+// elsewhere (cancellation at most a factor 4.5 there); both are evaluated and one is
+// selected, so no lane diverges.
+__device__ __forceinline__ double x_over_expm1(double x, double y) {
+  double z = x * (1.0 / y);
+  z = z < 20.0 ? z : 20.0;
+  const double ser = y * z_over_expm1_series(z);
+  const double dir = x * drcp(exp_f64(z) - 1.0);
+  return fabs(z) < 0.25 ? ser : dir;
+}
 
 template <typename T>
 __device__ __forceinline__ T sigmoid(T x) {
@@ -68,7 +119,7 @@ __device__ __forceinline__ T gate_step(T x, T dt, T alpha, T beta) {
   T tau = ddiv(T(1), alpha + beta);
   T xinf = alpha * tau;
   T e = save_exp(-dt * (alpha + beta));
-  return x * e + xinf * (T(1) - e);
+  return fma(x - xinf, e, xinf);  // x e + x_inf (1 - e)
 }
 
 // HH rates (channels/hh.py:87-108).
@@ -87,9 +138,10 @@ __device__ __forceinline__ void hh_rates(T v, T& am, T& bm, T& ah, T& bh, T& an,
 //   exp(u/80) = E^9, exp(u/20) = E^36, exp(u/18) = E^40, exp(u/10) = E^72, E = exp(u/720),
 //   and -(v+35)/10 = u/10 + 3, -(v+40)/10 = u/10 + 5/2, -(v+55)/10 = u/10 + 1,
 // so the powers replace five exponentials (relative error below 72 ulp of E, < 1e-14).
-// Where x / expm1(z) would cancel (|z| < 1/4) expm1 is evaluated directly. The clamps of
-// save_exp / x_over_expm1 at 20 are inactive for u <= 170 (v >= -235 mV); other voltages
-// take the direct formulas.
+// x / expm1(z) is the series near z = 0 and x / (e^z - 1) elsewhere, selected without a
+// branch. The clamps of save_exp / x_over_expm1 at 20 are inactive for u <= 170
+// (v >= -235 mV); other voltages take the direct formulas (a wave-uniform branch in
+// practice).
 __device__ __forceinline__ void hh_rates_f64(double v, double& am, double& bm, double& ah,
                                              double& bh, double& an, double& bn) {
   const double u = -(v + 65.0);
@@ -97,7 +149,7 @@ __device__ __forceinline__ void hh_rates_f64(double v, double& am, double& bm, d
     hh_rates(v, am, bm, ah, bh, an, bn);
     return;
   }
-  const double e1 = exp(u * (1.0 / 720));
+  const double e1 = exp_f64(u * (1.0 / 720));
   const double e2 = e1 * e1, e4 = e2 * e2, e8 = e4 * e4, e9 = e8 * e1;
   const double e18 = e9 * e9, e36 = e18 * e18, e40 = e36 * e4, e72 = e36 * e36;
   constexpr double kExp1 = 2.718281828459045235;    // e
@@ -107,9 +159,15 @@ __device__ __forceinline__ void hh_rates_f64(double v, double& am, double& bm, d
   ah = 0.07 * e36;
   bn = 0.125 * e9;
   bh = drcp(e72 * kExp3 + 1.0);
-  const double xm = -(v + 40.0), xn = -(v + 55.0);
-  am = fabs(xm) < 2.5 ? 0.1 * x_over_expm1(xm, 10.0) : 0.1 * xm * drcp(e72 * kExp25 - 1.0);
-  an = fabs(xn) < 2.5 ? 0.01 * x_over_expm1(xn, 10.0) : 0.01 * xn * drcp(e72 * kExp1 - 1.0);
+  // -(v + 40) = u + 25, -(v + 55) = u + 10; z = x / 10.
+  const double xm = u + 25.0, xn = u + 10.0;
+  const double zm = xm * 0.1, zn = xn * 0.1;
+  const double am_s = z_over_expm1_series(zm);           // 0.1 * 10 * z / expm1(z)
+  const double an_s = 0.1 * z_over_expm1_series(zn);     // 0.01 * 10 * z / expm1(z)
+  const double am_d = 0.1 * xm * drcp(e72 * kExp25 - 1.0);
+  const double an_d = 0.01 * xn * drcp(e72 * kExp1 - 1.0);
+  am = fabs(zm) < 0.25 ? am_s : am_d;
+  an = fabs(zn) < 0.25 ? an_s : an_d;
 }
 
 // The rates as the fused kernels evaluate them: the shared-exponential form in float64,

@@ -56,6 +56,7 @@ struct Fused {
   G* gm;
   G* gh;
   G* gn;
+  G* ax;               // streaming kernel: axial part of every item's residual
   const G* stim;
   int64_t stim_cols;
   int64_t stim_b;
@@ -102,6 +103,7 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.gm = static_cast<G*>(a.gate_m);
   f.gh = static_cast<G*>(a.gate_h);
   f.gn = static_cast<G*>(a.gate_n);
+  f.ax = static_cast<G*>(a.work_ax);
   f.stim = static_cast<const G*>(a.stim);
   f.stim_cols = a.stim_cols;
   f.stim_b = a.stim_b;
@@ -588,6 +590,134 @@ __device__ __forceinline__ void build_item(const Fused<G, L>& a, const Prog& pg,
   build_core(a, pg, tp, Ld, nik, k, s, (k << a.log2S) + s, v, ax, t, m, h, n);
 }
 
+// Inputs of one (node, simulation) item of the streaming build, in two parts: its state
+// in HBM (voltage, gates, axial part of the residual), loaded one item ahead of its use
+// so that the HBM latency overlaps the arithmetic of the previous item, and its node's
+// tables (flags, stimulus range, synapse target, diagonal base, 1 / cm, channel
+// parameters; small, cache-resident), loaded at the start of the item's own turn and
+// before the next item's state, so that waiting for them does not wait for that state
+// (vector memory counters complete in order). The arithmetic itself then loads nothing
+// but LDS (and the stimulus of stimulated nodes).
+template <typename G>
+struct ItemState {
+  double v;
+  G m, h, n, ax;
+};
+
+template <typename G, typename L>
+struct ItemConst {
+  int32_t flags, sbeg, send, tgt;
+  L dbase, inv_cm;
+  G p[JXHIP_NPARAM];
+};
+
+template <typename G, typename L>
+__device__ __forceinline__ ItemState<G> load_state(const Fused<G, L>& a, const Prog& pg, int i) {
+  const int64_t gi = pg.soff + i;
+  ItemState<G> st;
+  st.v = a.v[gi];
+  st.m = a.gm[gi];
+  st.h = a.gh[gi];
+  st.n = a.gn[gi];
+  st.ax = a.ax[gi];
+  return st;
+}
+
+template <typename G, typename L>
+__device__ __forceinline__ ItemConst<G, L> load_const(const Fused<G, L>& a, const Prog& pg,
+                                                      const Topo& tp, int i) {
+  using VG = typename Vec2<G>::type;
+  using VL = typename Vec2<L>::type;
+  const int k = i >> a.log2S, s = i & (a.S - 1);
+  const int64_t row = static_cast<int64_t>(tp.nb) + k;
+  ItemConst<G, L> c;
+  const int4* ni4 = reinterpret_cast<const int4*>(a.node_i + row * JXHIP_NODE_ICOLS);
+  const int4 r0 = ni4[0], r1 = ni4[1];
+  c.flags = r0.w;
+  c.sbeg = r1.x;
+  c.send = r1.y;
+  c.tgt = r1.z;
+  const VL f = reinterpret_cast<const VL*>(a.node_f + row * JXHIP_NODE_FCOLS)[1];
+  c.dbase = f.x;
+  c.inv_cm = f.y;
+  const G* prow = a.node_p + row * JXHIP_NPARAM;
+  const VG* np = reinterpret_cast<const VG*>(prow);
+#pragma unroll
+  for (int q = 0; q < JXHIP_NPARAM / 2; ++q) {
+    const VG w = np[q];
+    c.p[2 * q] = w.x;
+    c.p[2 * q + 1] = w.y;
+  }
+  // Batched conductances (one load from a uniformly selected row, no branch): per
+  // simulation (mode 1) or per item (mode 2); mode 0 reads the node row again and keeps
+  // the node's values.
+  const G* brow = a.bmode == 1   ? a.bparam + ((pg.simoff + s) << 2)
+                  : a.bmode == 2 ? a.bparam + ((pg.soff + i) << 2)
+                                 : prow;
+  const VG b0 = reinterpret_cast<const VG*>(brow)[0], b1 = reinterpret_cast<const VG*>(brow)[1];
+  const bool bat = a.bmode != 0;
+  c.p[0] = bat ? b0.x : c.p[0];
+  c.p[1] = bat ? b0.y : c.p[1];
+  c.p[2] = bat ? b1.x : c.p[2];
+  c.p[6] = bat ? b1.y : c.p[6];
+  return c;
+}
+
+// build_core from loaded inputs: diagonal and residual r = b - A v of item i into LDS,
+// the updated gates into `it`.
+template <typename G, typename L>
+__device__ __forceinline__ void build_loaded(const Fused<G, L>& a, const Prog& pg,
+                                             const Lds<L>& Ld, ItemState<G>& it,
+                                             const ItemConst<G, L>& c, int i, int s,
+                                             int64_t t) {
+  if (!(c.flags & kComp)) {
+    Ld.ds[i] = c.dbase;
+    Ld.xs[i] = static_cast<L>(it.ax);
+    return;
+  }
+  const int lS = a.log2S;
+  const G dt = a.dt;
+  const G inv_cm = static_cast<G>(c.inv_cm);
+  const double v = it.v;
+  const G vv = static_cast<G>(v);
+  G g = G(0);
+  G cur = G(0);
+  if (c.flags & kHH) {
+    G am, bm, ah, bh, an, bn;
+    hh_rates_step(vv, am, bm, ah, bh, an, bn);
+    it.m = gate_step(it.m, dt, am, bm);
+    it.h = gate_step(it.h, dt, ah, bh);
+    it.n = gate_step(it.n, dt, an, bn);
+    const G gNa = c.p[0] * (it.m * it.m * it.m) * it.h;
+    const G n2 = it.n * it.n;
+    const G gK = c.p[1] * (n2 * n2);
+    const G gL = c.p[2];
+    g += gNa + gK + gL;
+    cur += gNa * static_cast<G>(v - static_cast<double>(c.p[3])) +
+           gK * static_cast<G>(v - static_cast<double>(c.p[4])) +
+           gL * static_cast<G>(v - static_cast<double>(c.p[5]));
+  }
+  if (c.flags & kLeak) {
+    const G gl = c.p[6];
+    g += gl;
+    cur += gl * static_cast<G>(v - static_cast<double>(c.p[7]));
+  }
+  G lin = G(1000) * g;
+  G net = -G(1000) * cur;
+  if (c.tgt >= 0) {
+    lin += static_cast<G>(Ld.tl[(c.tgt << lS) + s]);
+    net -= static_cast<G>(Ld.tc[(c.tgt << lS) + s]);
+  }
+  G r = it.ax + dt * inv_cm * net;
+  for (int e = c.sbeg; e < c.send; ++e) {
+    const int64_t col = a.stim_col[e];
+    const int64_t bidx = a.stim_b > 1 ? (pg.simoff + s) : 0;
+    r += a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e];
+  }
+  Ld.ds[i] = c.dbase + static_cast<L>(dt * inv_cm * lin);
+  Ld.xs[i] = static_cast<L>(r);
+}
+
 // Items of simulations beyond `nsim` (padding): a well-conditioned row, zero rhs.
 template <typename G, typename L>
 __device__ __forceinline__ void idle_item(const Fused<G, L>& a, const Topo& tp, const Lds<L>& Ld,
@@ -671,8 +801,59 @@ __device__ __forceinline__ void checkpoint_mem(const Fused<G, L>& a, const Prog&
   }
 }
 
-// Streaming forward: every step reads the voltages and gates of the program from HBM and
-// writes them back (for programs whose items do not fit the registers of their threads).
+// Update of one item after the solve: its voltage and axial part of the residual (HBM,
+// prefetched one item ahead) and its node row (parent, child range), own lower
+// coefficient and the upper coefficient of node k + 1 (its child when it has exactly one),
+// loaded at the start of the item's turn (as ItemState / ItemConst).
+template <typename G>
+struct UpdState {
+  double v;
+  G ax;
+};
+
+struct UpdConst {
+  int4 r0;
+  double lo, upc;
+};
+
+template <int kCS, typename L>
+__device__ __forceinline__ UpdConst load_update_const(const Topo& tp, const int32_t* ni, const L* cl,
+                                                      int k) {
+  const int kc = k + 1 < tp.n ? k + 1 : k;
+  UpdConst u;
+  u.r0 = reinterpret_cast<const int4*>(ni + k * JXHIP_NODE_ICOLS)[0];
+  u.lo = static_cast<double>(cl[k << (kCS + 1)]);
+  u.upc = static_cast<double>(cl[(kc << (kCS + 1)) + 1]);
+  return u;
+}
+
+// Change of the axial part of the residual of item (k, s) for the increments x (LDS), as
+// `axial`: the parent from the node row, the single child of a chain node at k + 1 (kernel
+// order places a one-child node's child right after it), the child list otherwise.
+template <int kCS, typename G, typename L>
+__device__ __forceinline__ double apply_update(const Fused<G, L>& a, const Topo& tp,
+                                               const UpdConst& u, int k, const L* cl, const L* x,
+                                               int s, double xk) {
+  const int lS = a.log2S;
+  double ax = 0.0;
+  if (u.r0.x >= 0) ax += u.lo * (xk - static_cast<double>(x[(u.r0.x << lS) + s]));
+  if (u.r0.z - u.r0.y == 1) {
+    ax += u.upc * (xk - static_cast<double>(x[((k + 1) << lS) + s]));
+  } else {
+    const int32_t* cidx = a.child_idx + tp.cb;
+    for (int j = u.r0.y; j < u.r0.z; ++j) {
+      const int c = cidx[j];
+      ax += static_cast<double>(cl[(c << (kCS + 1)) + 1]) * (xk - static_cast<double>(x[(c << lS) + s]));
+    }
+  }
+  return ax;
+}
+
+// Streaming forward: every step reads the state of the program (voltages, gates and the
+// axial part of each item's residual) from HBM and writes it back, for programs whose
+// items do not fit the registers of their threads. The axial part is computed from the
+// voltages once per launch and then updated from the increments after every solve (the
+// neighbours' increments are in LDS), so the build of an item reads only its own state.
 // kSyn: the topologies carry synapses; kCkpt: checkpoints are written for the adjoint;
 // kGC: the tree's coupling coefficients are read from the node table (HBM / L2) instead
 // of an LDS copy, which leaves the LDS to the system of more simulations.
@@ -681,7 +862,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
-  const int lS = a.log2S;
+  const int lS = a.log2S, S = a.S;
   const int nS = tp.n << lS;
   const Lds<L> Ld = forward_lds<G, L, kGC>(a, smem_raw, nS, tp, lS);
   constexpr int kCS = kGC ? 1 : 0;
@@ -691,6 +872,16 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   G* gm = a.gm + pg.soff;
   G* gh = a.gh + pg.soff;
   G* gn = a.gn + pg.soff;
+  G* axg = a.ax + pg.soff;
+  const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
+  // Every thread owns the items tid + j * nthr throughout, all of simulation s_own (the
+  // thread count is a multiple of S); no barrier is needed between the writes and reads
+  // of an item's own state.
+  const int s_own = tid & (S - 1);
+  for (int i = tid; i < nS; i += nthr) {
+    const int k = i >> lS, s = i & (S - 1);
+    axg[i] = static_cast<G>(axial<kCS>(a, tp, ni + k * JXHIP_NODE_ICOLS, k, Ld.cl, vg, s, vg[i]));
+  }
   record_step(a, pg, tp, vg, 0);
   const int64_t nsteps = pg.t1 - pg.t0;
   for (int64_t step = 0; step < nsteps; ++step) {
@@ -702,21 +893,43 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     }
     if constexpr (kSyn) synapse_phase(a, pg, tp, Ld);
     const int64_t t = pg.t0 + step;
-    for (int i = tid; i < nS; i += nthr) {
-      const int k = i >> lS, s = i & (a.S - 1);
-      if (s < pg.nsim) {
-        G m = gm[i], h = gh[i], n = gn[i];
-        build_item<kCS>(a, pg, tp, Ld, vg, k, s, vg[i], t, m, h, n);
-        gm[i] = m;
-        gh[i] = h;
-        gn[i] = n;
-      } else {
-        idle_item(a, tp, Ld, k, i);
+    if (tid < nS) {
+      ItemState<G> cur = load_state(a, pg, tid);
+      for (int i = tid; i < nS; i += nthr) {
+        const int inx = i + nthr < nS ? i + nthr : i;
+        const ItemConst<G, L> c = load_const(a, pg, tp, i);
+        __builtin_amdgcn_sched_barrier(0);
+        const ItemState<G> nxt = load_state(a, pg, inx);
+        __builtin_amdgcn_sched_barrier(0);
+        if (s_own < pg.nsim) {
+          build_loaded(a, pg, Ld, cur, c, i, s_own, t);
+          gm[i] = cur.m;
+          gh[i] = cur.h;
+          gn[i] = cur.n;
+        } else {
+          Ld.ds[i] = c.dbase + ((c.flags & kComp) ? L(1) : L(0));
+          Ld.xs[i] = L(0);
+        }
+        cur = nxt;
       }
     }
     __syncthreads();
     tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
-    for (int i = tid; i < nS; i += nthr) vg[i] += static_cast<double>(Ld.xs[i]);
+    if (tid < nS) {
+      UpdState<G> cur{vg[tid], axg[tid]};
+      for (int i = tid; i < nS; i += nthr) {
+        const int inx = i + nthr < nS ? i + nthr : i;
+        const int k = i >> lS;
+        const UpdConst c = load_update_const<kCS>(tp, ni, Ld.cl, k);
+        __builtin_amdgcn_sched_barrier(0);
+        const UpdState<G> nxt{vg[inx], axg[inx]};
+        __builtin_amdgcn_sched_barrier(0);
+        const double dv = static_cast<double>(Ld.xs[i]);
+        vg[i] = cur.v + dv;
+        axg[i] = cur.ax + static_cast<G>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
+        cur = nxt;
+      }
+    }
     __syncthreads();
     record_step(a, pg, tp, vg, step + 1);
   }
@@ -1097,6 +1310,7 @@ int launch_kernel(K kern, const jxhip_fused_args& a, hipStream_t stream, Args...
 template <typename G, typename L, bool kSyn, bool kCkpt>
 int launch_forward_feat(const jxhip_fused_args& a, const Fused<G, L>& f, hipStream_t st) {
   if (a.items_per_thread == 0) {
+    if (a.work_ax == nullptr) return hipErrorInvalidValue;
     if (a.coef_global) return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, true>, a, st, f);
     return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, false>, a, st, f);
   }

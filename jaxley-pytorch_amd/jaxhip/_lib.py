@@ -45,7 +45,7 @@ class FusedArgs(C.Structure):
         ("S", _i32), ("threads", _i32), ("items_per_thread", _i32), ("lds_bytes", _i64),
         ("ckpt_v", _p), ("ckpt_m", _p), ("ckpt_h", _p), ("ckpt_n", _p), ("ckpt_s", _p),
         ("ckpt_every", _i64), ("ckpt_stride", _i64), ("ckpt_syn_stride", _i64),
-        ("max_items", _i64), ("coef_global", _i32),
+        ("max_items", _i64), ("coef_global", _i32), ("work_ax", _p),
     ]
 
 

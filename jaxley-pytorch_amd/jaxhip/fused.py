@@ -405,6 +405,9 @@ class ForwardLaunch:
         n_tgt = st.syn["n_tgt"] if st.syn else 0
         a.coef_global = int(cfg.stream and cfg.gcoef)
         a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size(), st.sched, bool(a.coef_global))
+        # Scratch of the streaming kernel: the axial part of every item's residual.
+        self.work_ax = torch.empty_like(self.gm) if a.items_per_thread == 0 else None
+        a.work_ax = self.work_ax.data_ptr() if self.work_ax is not None else 0
         a.max_items = n * S
         if ckpt_every > 0:
             a.ckpt_v, a.ckpt_m, a.ckpt_h, a.ckpt_n = (c.data_ptr() for c in self.ckpt)

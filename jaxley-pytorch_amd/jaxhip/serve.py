@@ -145,8 +145,9 @@ class SimServer:
         gs, ls = torch.tensor([], dtype=self.gdtype).element_size(), torch.tensor([], dtype=self.ldtype).element_size()
         self.gsize, self.lsize = gs, ls
         # Arena capacities from the budget: per node 8 int32 + 4 solve + 8 parameters +
-        # float64 voltage + 3 gates; schedules ~8 int32 per node; recordings get a third.
-        per_node = 32 + 4 * ls + 8 * gs + 8 + 3 * gs + 32
+        # float64 voltage + 3 gates + the axial scratch; schedules ~8 int32 per node;
+        # recordings get a third.
+        per_node = 32 + 4 * ls + 8 * gs + 8 + 4 * gs + 32
         stim_bytes = max_steps * max_stim_entries * gs + max_stim_entries * (4 + gs)
         rest = max(int(hbm_budget) - stim_bytes, 1 << 20)
         self.node_cap = int(rest * 2 // 3 // per_node)
@@ -157,6 +158,8 @@ class SimServer:
         self.node_p = torch.zeros((self.node_cap, _lib.JXHIP_NPARAM), dtype=self.gdtype, device=dev)
         self.v = torch.zeros(self.node_cap, dtype=torch.float64, device=dev)
         self.gm, self.gh, self.gn = (torch.zeros(self.node_cap, dtype=self.gdtype, device=dev) for _ in range(3))
+        # Scratch of the streaming kernel (axial part of every item's residual), same places.
+        self.work_ax = torch.zeros(self.node_cap, dtype=self.gdtype, device=dev)
         self.aux = torch.zeros(8 * self.node_cap, dtype=torch.int32, device=dev)
         self.rec = torch.zeros(self.rec_cap, dtype=self.gdtype, device=dev)
         self.stim = torch.zeros((max_steps, max_stim_entries, 1), dtype=self.gdtype, device=dev)
@@ -324,6 +327,7 @@ class SimServer:
         a.items_per_thread = 0 if cfg.stream else items_per_thread(n_max, 1, cfg.threads, self.gsize, self.lsize)
         a.coef_global = int(cfg.stream and cfg.gcoef)
         a.lds_bytes = lds_bytes(n_max, 0, 1, self.lsize, sched_max, bool(a.coef_global))
+        a.work_ax = self.work_ax.data_ptr()
         a.max_items = n_max
         return a
 
