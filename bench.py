@@ -18,9 +18,12 @@ Also reported (not the headline value):
   fwd_adj   - forward + hand-written adjoint of the gradient workload: the same tree,
               B = 256, per-branch trainable gNa / gK / gLeak (750 per simulation), MSE
               to target traces over 2000 steps, checkpoints every 100 steps, float32;
+  fwd_adj_sweep - forward + adjoint of the headline sweep itself (B = 4096, per-simulation
+              trainable conductances, float64);
+  network   - 1024 ball-and-stick cells with 32768 KineticSynapses, B = 64, 4000 steps;
   served    - 512 requests of random trees (20-2000 compartments, t_max 10-100 ms)
               arriving as a Poisson stream, continuous batching: p50 / p99 latency and
-              throughput;
+              throughput; served_burst: the same requests all at time 0 (capacity);
   roofline  - of the fused forward kernel: 64 algorithmic bytes per compartment-step in
               float64 (read + write v, m, h, n) / measured kernel time, vs 8 TB/s;
   cpu_baseline - the CPU oracle (float64 PyTorch on the host) on a bounded sample.
@@ -129,6 +132,73 @@ def cpu_baseline(prm, T_sample, B_sample, threads):
     return ref, el, nc * T_sample * B_sample / el
 
 
+def bench_fwd_adj_sweep(device, B, T, K=100):
+    """Forward + hand-written adjoint of the batched sweep (B simulations, per-simulation
+    trainable gNa, gK, gLeak), float64: MSE of the root voltage against target traces
+    simulated from the seeded sweep parameters, checkpoints every K steps."""
+    cell, _, _, ds, prm = setup(B, T, torch.float64, device)
+    rng = np.random.default_rng(12)
+    guess = {k: v * rng.uniform(0.9, 1.1, size=v.shape) for k, v in prm.items()}
+    with torch.no_grad():
+        ps_t = None
+        for k, v in prm.items():
+            ps_t = cell.data_set(k, torch.as_tensor(v, dtype=torch.float64, device=device), ps_t)
+        target = jx.integrate(cell, param_state=ps_t, data_stimuli=ds, delta_t=DT)[:, 0]
+    leaves = {k: torch.tensor(v, dtype=torch.float64, device=device, requires_grad=True)
+              for k, v in guess.items()}
+
+    def run():
+        ps = None
+        for k, v in leaves.items():
+            ps = cell.data_set(k, v, ps)
+        v = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT,
+                         checkpoint_lengths=[T // K, K])
+        loss = ((v[:, 0] - target) ** 2).mean()
+        loss.backward()
+
+    sec = timed(run, 1, 1, device)
+    n_comp = cell._n_comp
+    cs = n_comp * T * B / sec
+    res = {"comp_steps_per_s": cs, "ms": sec * 1e3, "batch": B, "checkpoint_every": K, "dtype": "f64",
+           "trainable_per_sim": 3, "bytes_per_comp_step": 224,
+           "frac_of_hbm_roofline": 224 * cs / HBM_PEAK}
+    log(f"[bench] forward+adjoint sweep B={B} f64: {sec*1e3:.1f} ms, {cs:.3e} comp-steps/s")
+    return res
+
+
+def bench_network(device, n_cells=1024, k=32, B=64, T=4000, dtype=torch.float32):
+    """The ball-and-stick network (1-compartment HH soma + 8-compartment Leak dendrite per
+    cell, KineticSynapses with in-degree k onto one dendritic compartment, Bernoulli(0.01)
+    0.05 nA pulses into every soma), B simulations, T steps, somata recorded; float32
+    (the float32 mode of integrate). One program holds the whole network (10240 solve
+    nodes): in float64 its system exceeds the LDS of a workgroup."""
+    from models import build_c4, c4_stimulus
+
+    net, _, info = build_c4(n_cells, k, seed=9)
+    soma = info["soma_comps"]
+    for c in soma:
+        net.cell(int(c) // 9).branch(0).comp(0).record()
+    cur = torch.tensor(c4_stimulus(B, n_cells, T, seed=10), dtype=dtype, device=device)
+    ds = None
+    for i, c in enumerate(soma):
+        ds = net.cell(int(c) // 9).branch(0).comp(0).data_stimulate(cur[:, i, :], ds)
+
+    def run():
+        with torch.no_grad():
+            return jx.integrate(net, data_stimuli=ds, delta_t=DT)
+
+    sec = timed(run, 1, 1, device)
+    n_comp = net._n_comp
+    v = run()
+    res = {"comp_steps_per_s": n_comp * T * B / sec, "ms": sec * 1e3, "batch": B, "cells": n_cells,
+           "synapses": n_cells * k, "compartments": n_comp, "time_steps": T,
+           "dtype": str(dtype).replace("torch.", ""),
+           "mean_spikes_per_soma": float(spikes(v.double()).float().mean())}
+    log(f"[bench] network {n_cells} cells x {k} synapses, B={B}, T={T}: {sec*1e3:.1f} ms, "
+        f"{res['comp_steps_per_s']:.3e} comp-steps/s")
+    return res
+
+
 def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
     """Poisson arrivals of `n_req` requests at `rate` per second into a SimServer."""
     from jaxhip.serve import SimServer
@@ -140,7 +210,9 @@ def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
         cell.branch(0).comp(0).record()
         cells.append(cell)
     rng = np.random.default_rng(8)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    # rate None: every request arrives at time 0 (the server's capacity).
+    arrivals = (np.zeros(n_req) if rate is None
+                else np.cumsum(rng.exponential(1.0 / rate, size=n_req)))
     srv = SimServer(device, dtype=dtype, delta_t=DT, hbm_budget=4 << 30, chunk_steps=chunk,
                     max_steps=4001, max_stim_entries=1024, **server_kw)
     # Warm the launch path (kernel load, first shapes) on one request.
@@ -151,22 +223,25 @@ def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
     handles = []
     t0 = time.perf_counter()
     i = 0
-    while i < n_req or srv.queue or srv.live:
+    while i < n_req or srv.outstanding():
         now = time.perf_counter() - t0
         while i < n_req and arrivals[i] <= now:
             h = srv.submit(c2_request(specs[i], cpu, dtype, cells[i]))
             h.t_submit = t0 + arrivals[i]     # latency counts from the arrival time
             handles.append(h)
             i += 1
-        busy = srv.poll(block=False)
-        if not busy and i < n_req:
-            time.sleep(max(0.0, min(1e-3, arrivals[i] - (time.perf_counter() - t0))))
-        elif busy and len(srv.inflight) >= 2:
-            srv.poll(block=True)
+        srv.poll(block=False)
+        if len(srv.inflight) >= 2:
+            srv.poll(block=True)              # waits on the GPU (releases the interpreter)
+        else:
+            # Yield to the preparation thread until the next arrival or event.
+            nxt = arrivals[i] - (time.perf_counter() - t0) if i < n_req else 5e-4
+            time.sleep(max(5e-5, min(5e-4, nxt)))
     makespan = time.perf_counter() - t0
     lat = np.array([h.latency for h in handles])
     comp_steps = sum(len(c._comp_node) * h.T for c, h in zip(cells, handles))
-    return {"requests": n_req, "arrival_rate_per_s": rate, "chunk_steps": chunk,
+    return {"requests": n_req, "arrival_rate_per_s": rate if rate is not None else "all at t=0",
+            "chunk_steps": chunk,
             "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
             "makespan_s": makespan, "comp_steps_per_s": comp_steps / makespan,
@@ -183,7 +258,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-sims", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=512)
-    ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -317,13 +392,18 @@ def main():
                           "bytes_per_comp_step": 112,
                           "frac_of_hbm_roofline": 112 * n_comp * T * Bg / sg / HBM_PEAK}
         log(f"[bench] forward+adjoint B={Bg}: {sg*1e3:.1f} ms, {n_comp * T * Bg / sg:.3e} comp-steps/s")
+        # ---- forward + adjoint of the batched sweep itself: B = 4096, float64 ----
+        out["fwd_adj_sweep"] = bench_fwd_adj_sweep(device, B, T)
+        # ---- network: 1024 ball-and-stick cells, 32768 KineticSynapses, B = 64 ----
+        out["network"] = bench_network(device)
         # ---- served: continuous batching of uneven requests ----
         if args.serve_requests > 0:
-            sv = bench_served(device, args.serve_requests, args.serve_rate, torch.float64, 200)
-            out["served"] = sv
-            log(f"[bench] served {sv['requests']} requests at {sv['arrival_rate_per_s']}/s: p50 "
-                f"{sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms, "
-                f"{sv['comp_steps_per_s']:.3e} comp-steps/s")
+            for key, rate in (("served", args.serve_rate), ("served_burst", None)):
+                sv = bench_served(device, args.serve_requests, rate, torch.float64, 200)
+                out[key] = sv
+                log(f"[bench] {key} {sv['requests']} requests at {sv['arrival_rate_per_s']}/s: p50 "
+                    f"{sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms, "
+                    f"{sv['comp_steps_per_s']:.3e} comp-steps/s")
     # ---- CPU baseline on a bounded sample (rank 0, N = 1) ----
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(16, ncpu))

@@ -5,7 +5,8 @@ Simulations of different morphologies, durations and parameters arrive as reques
 `jaxhip.integrate` takes them, batch size 1). The server
 
 * prepares each request on arrival, on the host (schedule and node tables,
-  `integrate._prepare`; request tensors are best given on the CPU),
+  `integrate._prepare`; request tensors are best given on the CPU), in a background
+  thread (`prep_threads`), so that admissions and launches go on meanwhile,
 * admits it when a slot and room in its HBM arenas are free (`hbm_budget` bytes for
   node tables, states, schedules, stimuli and recordings; first fit),
 * keeps the state of every admitted simulation resident in HBM between launches,
@@ -26,6 +27,7 @@ tests check alone vs in a mix, and against `integrate`).
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
 import importlib
 import time
 from dataclasses import dataclass, field
@@ -128,12 +130,14 @@ class _Live:
 class SimServer:
     """See the module docstring. `dtype` float32 (mode of `integrate.set_float32_mode`)
     or float64; `delta_t` is shared by all requests; `max_steps` bounds a request's
-    duration (the stimulus table holds `max_steps` rows)."""
+    duration (the stimulus table holds `max_steps` rows). `prep_threads` > 0 prepares
+    requests in that many background threads (0: inside `submit`); with more than one,
+    requests must not share a module object (its schedule cache is built on first use)."""
 
     def __init__(self, device=None, dtype=torch.float32, delta_t: float = 0.025,
                  hbm_budget: int = 2 << 30, max_active: int = 1024, chunk_steps: int = 200,
                  max_steps: int = 4001, max_stim_entries: int = 4096, autotune: bool = False,
-                 thread_order=(256, 512, 1024)):
+                 thread_order=(256, 512, 1024), prep_threads: int = 1):
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         _require_gpu(device)
         self.device, self.user_dtype, self.dt = device, dtype, float(delta_t)
@@ -171,6 +175,8 @@ class SimServer:
         self.rec_a, self.stim_a = _Arena(self.rec_cap), _Arena(max_stim_entries)
         self.slots = list(range(max_active))[::-1]
         self.queue: List[SimHandle] = []
+        self.preparing: List = []     # (handle, future) in submission order
+        self._pool = cf.ThreadPoolExecutor(max_workers=prep_threads) if prep_threads > 0 else None
         self.live: Dict[int, _Live] = {}
         self.inflight: List = []      # (event, [live finishing], pinned progs)
         self.handles: List[SimHandle] = []
@@ -183,11 +189,22 @@ class SimServer:
 
     # ---------------------------------------------------------------- requests
     def submit(self, request: SimRequest) -> SimHandle:
+        """Registers a request and starts its preparation; returns its handle at once."""
         h = SimHandle(self._next_id, request)
         self._next_id += 1
-        m = request.module
+        self.handles.append(h)
+        if self._pool is None:
+            self._prepare_request(h)
+            self.queue.append(h)
+        else:
+            self.preparing.append((h, self._pool.submit(self._prepare_request, h)))
+        return h
+
+    def _prepare_request(self, h: SimHandle) -> None:
         # Prepared on the host (no GPU work, so a submission never waits for a running
         # chunk); the tables go to the arenas at admission by asynchronous copies.
+        request = h.request
+        m = request.module
         cpu = torch.device("cpu")
         inp = _assemble(m, request.params, request.param_state, request.data_stimuli,
                         request.t_max, self.dt, request.all_states, cpu, self.user_dtype)
@@ -207,9 +224,24 @@ class SimServer:
             raise ValueError("the request does not fit the server's HBM budget")
         h.T = P.T
         h._P = P
-        self.queue.append(h)
-        self.handles.append(h)
-        return h
+
+    def _collect_prepared(self, wait: bool = False) -> None:
+        """Moves prepared requests (in submission order) to the admission queue; a failed
+        preparation raises here. `wait`: block for the oldest one if none is ready."""
+        while self.preparing:
+            h, fut = self.preparing[0]
+            if not fut.done():
+                if not wait:
+                    return
+                fut.result()
+            fut.result()              # re-raises a preparation error
+            self.preparing.pop(0)
+            self.queue.append(h)
+            wait = False
+
+    def outstanding(self) -> int:
+        """Requests not finished: being prepared, queued for admission or running."""
+        return len(self.preparing) + len(self.queue) + len(self.live)
 
     def _try_admit(self, h: SimHandle) -> bool:
         P = h._P
@@ -403,6 +435,8 @@ class SimServer:
             for L in finishing:
                 self._retire(L)
             block = False
+        # With nothing running or queued, a blocking poll waits for a preparation.
+        self._collect_prepared(wait=block and not self.queue and not self.live)
         pending = []
         for h in self.queue:
             if not self._try_admit(h):
@@ -416,10 +450,10 @@ class SimServer:
                 self._launch_chunk()
             finally:
                 self.live = saved
-        return len(self.queue) + len(self.live)
+        return self.outstanding()
 
     def run_until_idle(self) -> None:
-        while self.poll(block=bool(self.inflight)):
+        while self.poll(block=bool(self.inflight) or bool(self.preparing)):
             pass
         while self.inflight:
             self.poll(block=True)

@@ -69,6 +69,39 @@ def test_batched_random_tree(device, dtype, tol):
     assert spike_counts(ref.numpy()).sum() > 0
 
 
+SHAPES = ["8,1024,gcoef", "4,512,gcoef", "4,512,stream", "2,256", "1,1024", "8,512"]
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-6), (torch.float32, 1e-2)])
+def test_launch_shapes_match_oracle(device, dtype, tol, monkeypatch):
+    """Kernels and launch shapes of the fused forward (streaming with the coupling
+    coefficients in LDS or read from L2, register-resident; B = 5 leaves padding
+    simulations in the last program) against the oracle."""
+    from jaxhip import autotune
+
+    B, T, nb = 5, 300, 30
+    cell, om = build_tree(nb, seed=4)
+    cell.branch(0).comp(0).record()
+    cell.branch(nb - 1).comp(3).record()
+    prm = c1_params(B, seed=4)
+    ps = data_set_params(cell, prm, device, dtype)
+    cur = c1_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=dtype, device=device))
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0],
+                     rec_comps=[0, 4 * nb - 1], overrides=oracle_overrides(om, prm)).numpy()
+    for shape in SHAPES:
+        monkeypatch.setenv("JAXHIP_FUSED_CONFIG", shape)
+        autotune._CACHE.clear()
+        print("shape", shape, dtype, flush=True)
+        v = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT)
+        err = np.abs(v.double().cpu().numpy() - ref).max()
+        print(f"shape {shape} {dtype} max err {err:.3e}", flush=True)
+        assert v.shape == (B, 2, T + 1)
+        assert err < tol, shape
+        assert np.array_equal(spike_counts(v.double().cpu().numpy()), spike_counts(ref)), shape
+    autotune._CACHE.clear()
+
+
 def _dense_solve(d, l, u, s, parent):
     n = len(d)
     A = np.diag(d.astype(np.float64))
