@@ -21,6 +21,7 @@ Also reported (not the headline value):
   fwd_adj_sweep - forward + adjoint of the headline sweep itself (B = 4096, per-simulation
               trainable conductances, float64);
   network   - 1024 ball-and-stick cells with 32768 KineticSynapses, B = 64, 4000 steps;
+              network_fwd_adj: forward + adjoint with every gS trainable;
   served    - 512 requests of random trees (20-2000 compartments, t_max 10-100 ms)
               arriving as a Poisson stream, continuous batching: p50 / p99 latency and
               throughput; served_burst: the same requests all at time 0 (capacity);
@@ -196,6 +197,41 @@ def bench_network(device, n_cells=1024, k=32, B=64, T=4000, dtype=torch.float32)
            "mean_spikes_per_soma": float(spikes(v.double()).float().mean())}
     log(f"[bench] network {n_cells} cells x {k} synapses, B={B}, T={T}: {sec*1e3:.1f} ms, "
         f"{res['comp_steps_per_s']:.3e} comp-steps/s")
+    return res
+
+
+def bench_network_fwd_adj(device, n_cells=1024, k=32, B=64, T=1000, K=100, dtype=torch.float32):
+    """Forward + adjoint of the network workload with every synapse's gS trainable (shared
+    by the B simulations): a loss on the somatic voltages, checkpoints every K steps,
+    through the differentiable per-step path (synapse operator with its hand-written
+    backward, tree solve with its transposed solve). T = 1000 of the workload's 4000
+    steps keeps the benchmark short; the rate per step does not depend on T."""
+    from models import build_c4, c4_stimulus
+
+    net, _, info = build_c4(n_cells, k, seed=9)
+    soma = info["soma_comps"]
+    for c in soma:
+        net.cell(int(c) // 9).branch(0).comp(0).record()
+    cur = torch.tensor(c4_stimulus(B, n_cells, T, seed=10), dtype=dtype, device=device)
+    ds = None
+    for i, c in enumerate(soma):
+        ds = net.cell(int(c) // 9).branch(0).comp(0).data_stimulate(cur[:, i, :], ds)
+    gs = torch.tensor(net._edges[0]["params"]["KineticSynapse_gS"], dtype=dtype, device=device,
+                      requires_grad=True)
+
+    def run():
+        gs.grad = None
+        ps = net.data_set("KineticSynapse_gS", gs, None)
+        v = jx.integrate(net, param_state=ps, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[T // K, K])
+        (v[..., -1] + 65.0).square().mean().backward()
+
+    sec = timed(run, 1, 1, device)
+    n_comp = net._n_comp
+    res = {"comp_steps_per_s": n_comp * T * B / sec, "ms": sec * 1e3, "batch": B, "cells": n_cells,
+           "time_steps": T, "trainable": n_cells * k, "checkpoint_every": K,
+           "dtype": str(dtype).replace("torch.", ""),
+           "path": "per-step (synapse operator + tree-solve operator with their backward kernels)"}
+    log(f"[bench] network forward+adjoint B={B}, T={T}: {sec*1e3:.1f} ms, {res['comp_steps_per_s']:.3e} comp-steps/s")
     return res
 
 
@@ -396,6 +432,7 @@ def main():
         out["fwd_adj_sweep"] = bench_fwd_adj_sweep(device, B, T)
         # ---- network: 1024 ball-and-stick cells, 32768 KineticSynapses, B = 64 ----
         out["network"] = bench_network(device)
+        out["network_fwd_adj"] = bench_network_fwd_adj(device)
         # ---- served: continuous batching of uneven requests ----
         if args.serve_requests > 0:
             for key, rate in (("served", args.serve_rate), ("served_burst", None)):

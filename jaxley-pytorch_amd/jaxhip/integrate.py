@@ -191,7 +191,24 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     if not module.recordings:
         raise ValueError("No recordings are set. Please set them.")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype)
+    if module._edges and _needs_grad(inp):
+        # Gradients through synapses: the differentiable per-step path (the fused
+        # adjoint covers cells without synapses).
+        from .stepwise import integrate_stepwise
+
+        return integrate_stepwise(module, inp, delta_t, checkpoint_lengths, device, dtype, return_states)
     return _integrate_fused(module, inp, delta_t, checkpoint_lengths, return_states, device, dtype)
+
+
+def _needs_grad(inp) -> bool:
+    if not torch.is_grad_enabled():
+        return False
+    ts = list(inp.vals.values()) + list(inp.syn_vals.values())
+    if inp.stim is not None:
+        ts.append(inp.stim)
+    if inp.all_states is not None:
+        ts += list(inp.all_states.values())
+    return any(t.requires_grad for t in ts)
 
 
 _F32_MODE = os.environ.get("JAXHIP_F32_MODE", "mixed")

@@ -1,0 +1,273 @@
+"""Differentiable per-step simulation on the GPU, for gradients through networks.
+
+The fused kernels run a whole simulation in one launch, and their hand-written adjoint
+covers cells (HH / Leak, no synapses) whose system fits a workgroup's LDS. Gradients
+through synapses take this path instead: the reference's eager step (`_step_with_plan`,
+modules/base.py:3214-3400, inside the time loop of integrate.py:624-662), one step at a
+time with autograd, on the device:
+
+* synapses: the fused synapse-step operator (`kernels.synapse_step`, the HIP kernel
+  pair K3 / K4-K5 with its hand-written backward; network.py:350-478);
+* HH / Leak: elementwise tensor operations, as the reference computes them
+  (channels/hh.py:37-108, channels/pospischil.py:39-71, solver_gate.py:8-64; the reference
+  has no kernel there);
+* the implicit system (solver_voltage.py:250-359) and its solve by the HIP tree-solve
+  kernel with its transposed-solve backward (`kernels.tree_solve`, K1 / K2).
+
+The time loop is checkpointed every K steps (`checkpoint_lengths[-1]`, default 100):
+the backward recomputes each K-step chunk from its start, as the reference's
+`nested_checkpoint_scan` does (utils/torch_utils.py:141-198).
+"""
+from __future__ import annotations
+
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+from torch.utils.checkpoint import checkpoint
+
+from .geometry import axial_conductances, comp_geometry
+from .kernels.synapse_step import MODELS, PARAM_NAMES, kernel_plan, synapse_step
+from .kernels.tree_solve import fused_tree_solve, make_tree_solve_kernel
+
+DIFF = 1e-3          # mV, finite difference of the synapse linearization (network.py:29)
+DEFAULT_CHUNK = 100  # steps per checkpoint when integrate() gets no checkpoint_lengths
+
+
+def _save_exp(x):
+    """exp(min(x, 20)) (solver_gate.py:8-11)."""
+    return torch.exp(torch.clamp(x, max=20.0))
+
+
+def _x_over_expm1(x, y):
+    """x / expm1(min(x / y, 20)), y (1 - z/2 + z^2/12) for |z| < 1e-4 (solver_gate.py:14-30);
+    the unused branch is evaluated at a safe argument so that its gradient stays finite."""
+    z = torch.clamp(x / y, max=20.0)
+    small = torch.abs(z) < 1e-4
+    z_safe = torch.where(small, torch.ones_like(z), z)
+    return torch.where(small, y * (1.0 - z / 2.0 + z * z / 12.0), x / torch.expm1(z_safe))
+
+
+def _gate(x, dt, alpha, beta):
+    """Exponential Euler (solver_gate.py:45-64)."""
+    tau = 1.0 / (alpha + beta)
+    xinf = alpha * tau
+    e = _save_exp(-dt / tau)
+    return x * e + xinf * (1.0 - e)
+
+
+def _hh_gates(m, h, n, v, dt):
+    """HH gate update at the old voltage (channels/hh.py:37-50, rates :87-108)."""
+    am = 0.1 * _x_over_expm1(-(v + 40.0), 10.0)
+    bm = 4.0 * _save_exp(-(v + 65.0) / 18.0)
+    ah = 0.07 * _save_exp(-(v + 65.0) / 20.0)
+    bh = 1.0 / (_save_exp(-(v + 35.0) / 10.0) + 1.0)
+    an = 0.01 * _x_over_expm1(-(v + 55.0), 10.0)
+    bn = 0.125 * _save_exp(-(v + 65.0) / 80.0)
+    return _gate(m, dt, am, bm), _gate(h, dt, ah, bh), _gate(n, dt, an, bn)
+
+
+class _SynGroup:
+    """One synapse type of the network in the regular layout of the synapse operator."""
+
+    def __init__(self, edge: Dict, syn_vals: Dict[str, torch.Tensor], area_node: torch.Tensor,
+                 n_nodes: int, B: int, device, dtype):
+        stype = edge["type"]
+        self.name = stype.name
+        if self.name not in MODELS:
+            raise NotImplementedError(f"{self.name}: the synapse operator runs "
+                                      f"{tuple(MODELS)} (gradients through networks)")
+        post = np.asarray(edge["post"], dtype=np.int64)
+        pre = np.asarray(edge["pre"], dtype=np.int64)
+        perm = np.argsort(post, kind="stable")     # the operator takes synapses by target
+        self.plan = kernel_plan(self.name, pre[perm], post[perm], n_nodes, device)
+        if self.plan is None:
+            raise NotImplementedError("gradients through networks need every target to hold "
+                                      "the same number of synapses (regular layout)")
+        self.model = self.plan["model"]
+        self.perm = torch.as_tensor(perm, device=device)
+        self.inv = torch.empty_like(self.perm)
+        self.inv[self.perm] = torch.arange(len(perm), device=device)
+        self.params = []
+        for p in PARAM_NAMES[self.model]:
+            x = syn_vals[f"{self.name}_{p}"].to(dtype)
+            if x.ndim > 1:
+                raise NotImplementedError("batched synapse parameters are not supported")
+            self.params.append(x[self.perm])
+        self.area = area_node[torch.as_tensor(post[perm], device=device)]
+        self.targets = self.plan["targets"].long()
+        E = len(perm)
+        key = f"{self.name}_s"
+        if self.model == 0:
+            s0 = syn_vals[key].to(dtype)
+            self.s0 = torch.broadcast_to(s0, (*s0.shape[:-1], E)).reshape(-1, E)[:, self.perm]
+            self.s0 = self.s0.expand(B, E) if self.s0.shape[0] == 1 else self.s0
+        else:
+            self.s0 = torch.zeros(B, E, dtype=dtype, device=device)
+        self.state_key = key if self.model == 0 else None
+
+
+def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List[int]], device,
+                       dtype, return_states: bool = False):
+    """`integrate` through the differentiable per-step path (see the module docstring).
+    Returns the recordings `(*batch, n_rec, 1 + T)`."""
+    tables, ind = module._ensure_solver()
+    n = module._n_nodes
+    vals = {k: x.to(dtype) for k, x in inp.vals.items()}
+    rec_states = {s for s, _ in module.recordings}
+    if rec_states != {"v"}:
+        raise NotImplementedError("recordings of voltages ('v') only")
+    comp_node = torch.as_tensor(module._comp_node, device=device)
+    rec_nodes = comp_node[torch.as_tensor([c for _, c in module.recordings], device=device)]
+    geo = comp_geometry(vals["length"], vals["radius"])
+    g_ax = axial_conductances(tables, module._comp_node, n, geo, vals["axial_resistivity"],
+                              vals["capacitance"])
+    batch_src = [x for x in vals.values() if x.ndim > 1]
+    if inp.stim is not None and inp.stim.ndim > 2:
+        batch_src.append(inp.stim[..., 0])
+    if inp.all_states is not None:
+        batch_src += [x for x in inp.all_states.values() if x.ndim > 1]
+    batch = tuple(torch.broadcast_shapes(*[x.shape[:-1] for x in batch_src])) if batch_src else ()
+    B = int(np.prod(batch)) if batch else 1
+
+    def flat(x):  # (..., m) -> (B, m) or (1, m)
+        if x.ndim <= 1:
+            return x.reshape(1, -1)
+        return torch.broadcast_to(x, (*batch, x.shape[-1])).reshape(B, -1)
+
+    def node_array(comp_vals, fill):
+        cv = flat(comp_vals)
+        out = torch.full((cv.shape[0], n), fill, dtype=dtype, device=device)
+        return out.index_copy(1, comp_node, cv)
+
+    cm = node_array(vals["capacitance"], 1.0)
+    area = node_array(geo["area"], 1.0)
+    gdt = dt * flat(g_ax)
+    sinks = torch.as_tensor(np.asarray(tables["sink"], dtype=np.int64), device=device)
+    # Channels (HH, Leak) and their compartments.
+    chans = {}
+    for ch in module.channels:
+        comps = np.nonzero(module._channel_mask[ch.name])[0]
+        if not len(comps):
+            continue
+        cidx = torch.as_tensor(comps, device=device)
+        names = ("HH_gNa", "HH_gK", "HH_gLeak", "HH_eNa", "HH_eK", "HH_eLeak") if ch.kernel_id == "HH" \
+            else ("Leak_gLeak", "Leak_eLeak")
+        prm = {k: flat(vals[k])[:, cidx] for k in names}
+        chans[ch.kernel_id] = (comp_node[cidx], prm, cidx)
+    # Initial states.
+    init = {k: vals[k] for k in vals if k in ("v", "HH_m", "HH_h", "HH_n")}
+    if inp.all_states is not None:
+        for k, x in inp.all_states.items():
+            if k in init:
+                init[k] = x[..., comp_node]
+    v = torch.broadcast_to(node_array(init["v"], 0.0), (B, n))
+    gates = []
+    if "HH" in chans:
+        cidx = chans["HH"][2]
+        gates = [torch.broadcast_to(flat(init[k])[:, cidx], (B, len(cidx))) for k in ("HH_m", "HH_h", "HH_n")]
+    groups = [_SynGroup(e, inp.syn_vals, area[0] if area.shape[0] == 1 else area[0], n, B, device, dtype)
+              for e in module._edges]
+    if area.shape[0] > 1 and groups:
+        raise NotImplementedError("batched geometry with synapses is not supported")
+    syn_s = [g.s0 for g in groups]
+    # Stimulus (B or 1, n_stim, T), scaled to uA/cm^2 (cell_utils.py:275-288).
+    stim = stim_nodes = None
+    if inp.stim is not None:
+        s = inp.stim.to(dtype)
+        stim = s.reshape(-1, *s.shape[-2:]) if s.ndim > 2 else s[None]
+        if stim.shape[0] > 1:
+            stim = torch.broadcast_to(s, (*batch, *s.shape[-2:])).reshape(B, *s.shape[-2:])
+        stim_nodes = torch.as_tensor(np.asarray(inp.stim_nodes, dtype=np.int64), device=device)
+    # Solve order (solver_voltage.py:294-359) and the tree-solve kernel.
+    mp = torch.as_tensor(np.asarray(ind["map_to_solve_order"], dtype=np.int64), device=device)
+    inv = torch.as_tensor(np.asarray(ind["inv_map_to_solve_order"], dtype=np.int64), device=device)
+    mlo = torch.as_tensor(np.asarray(ind["map_to_solve_order_lower"], dtype=np.int64), device=device)
+    mup = torch.as_tensor(np.asarray(ind["map_to_solve_order_upper"], dtype=np.int64), device=device)
+    kern = make_tree_solve_kernel(np.asarray(ind["parent_lookup"]), device)
+    if kern is None:
+        raise NotImplementedError("the solve order is not supported by the tree-solve kernel")
+    internal_mask = torch.zeros(n, dtype=torch.bool, device=device)
+    internal_mask[comp_node] = True
+    lu = torch.broadcast_to(-gdt, (B, gdt.shape[-1]))
+    lo = torch.cat([lu[:, mlo], lu.new_zeros(B, 1)], 1) if len(mlo) else lu.new_zeros(B, n + 1)
+    up = torch.cat([lu[:, mup], lu.new_zeros(B, 1)], 1) if len(mup) else lu.new_zeros(B, n + 1)
+    diag0 = torch.zeros(B, n, dtype=dtype, device=device).index_add(
+        1, comp_node, torch.ones(B, len(comp_node), dtype=dtype, device=device))
+    if len(sinks):
+        diag0 = diag0.index_add(1, sinks, torch.broadcast_to(gdt, (B, gdt.shape[-1])))
+
+    def step(v, gates, syn_s, t):
+        lin = torch.zeros(B, n, dtype=dtype, device=device)
+        cst = torch.zeros(B, n, dtype=dtype, device=device)
+        if "HH" in chans:
+            nodes, p, _ = chans["HH"]
+            vh = v[:, nodes]
+            m, h, nn = _hh_gates(*gates, vh, dt)
+            gates = [m, h, nn]
+            gNa = p["HH_gNa"] * m**3 * h
+            gK = p["HH_gK"] * nn**4
+            g = gNa + gK + p["HH_gLeak"]
+            # i = g v - (gNa eNa + gK eK + gL eL): the slope of a current linear in v is g
+            # (the reference's finite difference up to round-off).
+            lin = lin.index_add(1, nodes, g)
+            cst = cst.index_add(1, nodes, gNa * p["HH_eNa"] + gK * p["HH_eK"] + p["HH_gLeak"] * p["HH_eLeak"])
+        if "Leak" in chans:
+            nodes, p, _ = chans["Leak"]
+            lin = lin.index_add(1, nodes, torch.broadcast_to(p["Leak_gLeak"], (B, len(nodes))))
+            cst = cst.index_add(1, nodes, torch.broadcast_to(p["Leak_gLeak"] * p["Leak_eLeak"], (B, len(nodes))))
+        lin = lin * 1000.0
+        cst = cst * 1000.0
+        new_s = []
+        for grp, s in zip(groups, syn_s):
+            s2, _, sl, sc = synapse_step(grp.plan, v, s, grp.params, grp.area, dt, DIFF)
+            new_s.append(s2)
+            lin = lin.index_add(1, grp.targets, sl)
+            cst = cst.index_add(1, grp.targets, -sc)
+        if stim is not None:
+            cur = stim[..., t]
+            cst = cst.index_add(1, stim_nodes, torch.broadcast_to(cur / area[:, stim_nodes] * 1e5, (B, len(stim_nodes))))
+        diag = diag0 + dt * lin / cm
+        rhs = torch.where(internal_mask, v + dt * cst / cm, torch.zeros_like(v))
+        dd = torch.cat([diag[:, mp], diag.new_ones(B, 1)], 1)
+        ss = torch.cat([rhs[:, mp], rhs.new_zeros(B, 1)], 1)
+        x = fused_tree_solve(kern, dd, lo, up, ss)[:, :-1]
+        return x[:, inv], gates, new_s
+
+    T = inp.T
+    K = int(checkpoint_lengths[-1]) if checkpoint_lengths else DEFAULT_CHUNK
+    K = max(1, min(K, T))
+    n_g, n_s = len(gates), len(syn_s)
+
+    def chunk(t0, t1, v, *rest):
+        gates, syn_s = list(rest[:n_g]), list(rest[n_g:])
+        recs = []
+        for t in range(t0, t1):
+            v, gates, syn_s = step(v, gates, syn_s, t)
+            recs.append(v[:, rec_nodes])
+        return (v, *gates, *syn_s, torch.stack(recs, -1))
+
+    recs = [v[:, rec_nodes][..., None]]
+    state = (v, *gates, *syn_s)
+    for t0 in range(0, T, K):
+        t1 = min(t0 + K, T)
+        if torch.is_grad_enabled():
+            out = checkpoint(chunk, t0, t1, *state, use_reentrant=False)
+        else:
+            out = chunk(t0, t1, *state)
+        state, r = out[:-1], out[-1]
+        recs.append(r)
+    rec = torch.cat(recs, -1).reshape(*batch, len(rec_nodes), T + 1)
+    if not return_states:
+        return rec
+    v, gates, syn_s = state[0], list(state[1:1 + n_g]), list(state[1 + n_g:])
+    out = {"v": v.reshape(*batch, n)}
+    if gates:
+        hh_nodes = chans["HH"][0]
+        for k, g in zip(("HH_m", "HH_h", "HH_n"), gates):
+            full = torch.full((B, n), float("nan"), dtype=dtype, device=device)
+            out[k] = full.index_copy(1, hh_nodes, g).reshape(*batch, n)
+    for grp, s in zip(groups, syn_s):
+        if grp.state_key is not None:
+            out[grp.state_key] = s[:, grp.inv].reshape(*batch, -1)
+    return rec, out

@@ -45,3 +45,50 @@ def test_network_float32(device):
     err = float((v.double().cpu() - ref).abs().max())
     print("fp32 max err", err)
     assert err < 1e-2
+
+
+@pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse"])
+def test_network_gradients_vs_oracle_autograd(device, synapse):
+    """Gradients through a network (the differentiable per-step path: synapse operator
+    with its hand-written backward, tree solve with its transposed solve, checkpoints
+    every 40 steps) with respect to every synapse's gS and the somata's gNa, against
+    autograd through the float64 CPU oracle; the recordings against the oracle too."""
+    n_cells, k, T, seed = 8, 3, 240, 6
+    net, om, info = build_c4(n_cells, k, seed, synapse)
+    soma = info["soma_comps"]
+    for c in soma:
+        net.cell(int(c) // 9).branch(0).comp(0).record()
+    cur = c4_stimulus(1, n_cells, T, seed + 1) * 4.0
+    ds = None
+    for i, c in enumerate(soma):
+        ds = net.cell(int(c) // 9).branch(0).comp(0).data_stimulate(
+            torch.tensor(cur[0, i, :], dtype=torch.float64, device=device), ds)
+    name = synapse
+    E = n_cells * k
+    gs0 = np.asarray(om.synapses[0]["params"]["gS"], dtype=np.float64) * 20.0
+    gs = torch.tensor(gs0, dtype=torch.float64, device=device, requires_grad=True)
+    gna = torch.full((n_cells,), 0.12, dtype=torch.float64, device=device, requires_grad=True)
+    ps = net.data_set(f"{name}_gS", gs, None)
+    ps = net.data_set("HH_gNa", gna, ps)
+    w = torch.tensor(np.random.default_rng(seed).normal(size=(n_cells, T + 1)), dtype=torch.float64)
+    v = jx.integrate(net, param_state=ps, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[6, 40])
+    v = v.reshape(n_cells, T + 1)
+    loss = (v.cpu() * w).sum()
+    loss.backward()
+    # oracle
+    gs_o = torch.tensor(gs0, dtype=torch.float64, requires_grad=True)
+    om.synapses[0]["params"]["gS"] = gs_o
+    gna_o = torch.full((len(om.internal),), 0.12, dtype=torch.float64, requires_grad=True)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur), stim_comps=list(soma), rec_comps=list(soma),
+                     overrides={"HH_gNa": gna_o})
+    (ref[0] * w).sum().backward()
+    err_v = float((v.detach().cpu() - ref[0].detach()).abs().max())
+    g1, g1o = gs.grad.cpu(), gs_o.grad
+    g2, g2o = gna.grad.cpu(), gna_o.grad[torch.as_tensor(soma)]
+    e1 = float((g1 - g1o).abs().max() / g1o.abs().max())
+    e2 = float((g2 - g2o).abs().max() / g2o.abs().max())
+    print(f"{synapse}: v err {err_v:.2e}, rel grad err gS {e1:.2e}, gNa {e2:.2e}, "
+          f"|dL/dgS| max {float(g1o.abs().max()):.3e}")
+    assert err_v < 1e-6
+    assert float(g1o.abs().max()) > 0 and float(g2o.abs().max()) > 0
+    assert e1 < 1e-6 and e2 < 1e-6
