@@ -54,9 +54,11 @@ NBRANCH, SEED = 250, 11
 HBM_PEAK = 8.0e12
 BYTES_PER_COMP_STEP = {"f64": 64, "f32": 32}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
-# 256 MiB device copy: fetch x 2.0, write x 1.0) for the headline launch shape; reported
-# as roofline.traffic when the autotuner picks that shape.
-MEASURED_TRAFFIC = {}
+# 256 MiB device copy: fetch x 2.0, write x 1.0) for the headline launch shape and
+# workload; reported as roofline.traffic when the autotuner picks that shape.
+MEASURED_TRAFFIC = {
+    ("float64", 4096, 2000, "S=8, threads=1024, stream=True, gcoef=True"): 520.2e9 + 371.2e9,
+}
 
 
 def log(msg):
@@ -333,6 +335,7 @@ def main():
 
     cfg = [v for k, v in _CACHE.items() if k[0] == "fused" and k[5] == B and "float64" in k[6]]
     bpc = BYTES_PER_COMP_STEP["f64"]
+    traffic = MEASURED_TRAFFIC.get(("float64", B, T, str(cfg[0])[13:-1] if cfg else ""))
     achieved = bpc * comp_steps / (ktime * 1e-3)
     log(f"[bench] {sec*1e3:.1f} ms per batch, kernel {ktime:.1f} ms, {value:.3e} comp-steps/s, cfg {cfg}")
     out = {
@@ -354,7 +357,9 @@ def main():
                    "launch": str(cfg[0]) if cfg else None},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
-                     "traffic": MEASURED_TRAFFIC.get(("float64", B, T, str(cfg[0])[13:-1] if cfg else "")),
+                     "traffic": traffic,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "traffic_GBps": traffic / (ktime * 1e-3) / 1e9 if traffic else None,
                      "kernel": "fused_stream_kernel / fused_regs_kernel (float64)", "kernel_ms": ktime,
                      "bytes_per_comp_step": bpc},
     }
