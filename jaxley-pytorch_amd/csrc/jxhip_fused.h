@@ -788,29 +788,9 @@ __device__ __forceinline__ void checkpoint_syn(const Fused<G, L>& a, const Prog&
   }
 }
 
-// Checkpoint c (state before step c * ckpt_every) of the items held in memory.
-template <typename G, typename L>
-__device__ __forceinline__ void checkpoint_mem(const Fused<G, L>& a, const Prog& pg, int nS,
-                                               int64_t c) {
-  const int64_t off = pg.soff + c * a.ckpt_stride;
-  for (int i = threadIdx.x; i < nS; i += blockDim.x) {
-    a.ckpt_v[off + i] = a.v[pg.soff + i];
-    a.ckpt_m[off + i] = a.gm[pg.soff + i];
-    a.ckpt_h[off + i] = a.gh[pg.soff + i];
-    a.ckpt_n[off + i] = a.gn[pg.soff + i];
-  }
-}
-
-// Update of one item after the solve: its voltage and axial part of the residual (HBM,
-// prefetched one item ahead) and its node row (parent, child range), own lower
-// coefficient and the upper coefficient of node k + 1 (its child when it has exactly one),
-// loaded at the start of the item's turn (as ItemState / ItemConst).
-template <typename G>
-struct UpdState {
-  double v;
-  G ax;
-};
-
+// Node data of the axial change of one item after the solve: its node row (parent, child
+// range, flags), own lower coefficient and the upper coefficient of node k + 1 (its child
+// when it has exactly one).
 struct UpdConst {
   int4 r0;
   double lo, upc;
@@ -852,8 +832,13 @@ __device__ __forceinline__ double apply_update(const Fused<G, L>& a, const Topo&
 // Streaming forward: every step reads the state of the program (voltages, gates and the
 // axial part of each item's residual) from HBM and writes it back, for programs whose
 // items do not fit the registers of their threads. The axial part is computed from the
-// voltages once per launch and then updated from the increments after every solve (the
+// voltages once per launch and then updated from the increments of every solve (the
 // neighbours' increments are in LDS), so the build of an item reads only its own state.
+// After a solve, one pass over the items leaves each item's axial change in its pivot
+// slot (free after the solve) and brings exported voltages (recordings, synapse ends) up
+// to date in HBM; the next step's build applies the increment and the axial change from
+// the item's own LDS slots when it loads the item's state (and takes the checkpoints).
+// The last step applies every increment in that pass.
 // kSyn: the topologies carry synapses; kCkpt: checkpoints are written for the adjoint;
 // kGC: the tree's coupling coefficients are read from the node table (HBM / L2) instead
 // of an LDS copy, which leaves the LDS to the system of more simulations.
@@ -876,7 +861,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   const int32_t* ni = a.node_i + static_cast<int64_t>(tp.nb) * JXHIP_NODE_ICOLS;
   // Every thread owns the items tid + j * nthr throughout, all of simulation s_own (the
   // thread count is a multiple of S); no barrier is needed between the writes and reads
-  // of an item's own state.
+  // of an item's own state or LDS slots.
   const int s_own = tid & (S - 1);
   for (int i = tid; i < nS; i += nthr) {
     const int k = i >> lS, s = i & (S - 1);
@@ -885,11 +870,11 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   record_step(a, pg, tp, vg, 0);
   const int64_t nsteps = pg.t1 - pg.t0;
   for (int64_t step = 0; step < nsteps; ++step) {
-    if constexpr (kCkpt) {
-      if (step % a.ckpt_every == 0) {
-        checkpoint_mem(a, pg, nS, step / a.ckpt_every);
-        if constexpr (kSyn) checkpoint_syn(a, pg, tp, step / a.ckpt_every);
-      }
+    const bool pend = step > 0;  // the last solve's increments wait in LDS
+    const bool ck = kCkpt && (step % a.ckpt_every == 0);
+    const int64_t coff = ck ? pg.soff + (step / a.ckpt_every) * a.ckpt_stride : 0;
+    if constexpr (kSyn && kCkpt) {
+      if (ck) checkpoint_syn(a, pg, tp, step / a.ckpt_every);
     }
     if constexpr (kSyn) synapse_phase(a, pg, tp, Ld);
     const int64_t t = pg.t0 + step;
@@ -902,6 +887,20 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
         const ItemState<G> nxt = load_state(a, pg, inx);
         __builtin_amdgcn_sched_barrier(0);
         if (s_own < pg.nsim) {
+          if (pend) {
+            if (!(c.flags & kExport)) cur.v += static_cast<double>(Ld.xs[i]);
+            cur.ax += static_cast<G>(Ld.ds[i]);
+          }
+          if constexpr (kCkpt) {
+            if (ck) {
+              a.ckpt_v[coff + i] = cur.v;
+              a.ckpt_m[coff + i] = cur.m;
+              a.ckpt_h[coff + i] = cur.h;
+              a.ckpt_n[coff + i] = cur.n;
+            }
+          }
+          vg[i] = cur.v;
+          axg[i] = cur.ax;
           build_loaded(a, pg, Ld, cur, c, i, s_own, t);
           gm[i] = cur.m;
           gh[i] = cur.h;
@@ -915,19 +914,17 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     }
     __syncthreads();
     tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
-    if (tid < nS) {
-      UpdState<G> cur{vg[tid], axg[tid]};
+    const bool last = step + 1 == nsteps;
+    if (s_own < pg.nsim && tid < nS) {
+      UpdConst c = load_update_const<kCS>(tp, ni, Ld.cl, tid >> lS);
       for (int i = tid; i < nS; i += nthr) {
-        const int inx = i + nthr < nS ? i + nthr : i;
         const int k = i >> lS;
-        const UpdConst c = load_update_const<kCS>(tp, ni, Ld.cl, k);
-        __builtin_amdgcn_sched_barrier(0);
-        const UpdState<G> nxt{vg[inx], axg[inx]};
-        __builtin_amdgcn_sched_barrier(0);
+        const int inx = i + nthr < nS ? i + nthr : i;
+        const UpdConst cn = load_update_const<kCS>(tp, ni, Ld.cl, inx >> lS);  // next item's
         const double dv = static_cast<double>(Ld.xs[i]);
-        vg[i] = cur.v + dv;
-        axg[i] = cur.ax + static_cast<G>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
-        cur = nxt;
+        if (last || (c.r0.w & kExport)) vg[i] += dv;
+        if (!last) Ld.ds[i] = static_cast<L>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
+        c = cn;
       }
     }
     __syncthreads();
