@@ -237,7 +237,7 @@ typedef struct jxhip_fused_args {
   /* recordings: per program [t][rec][S] at rec_off, t = 0 .. t_end - t_begin */
   void* rec;
   /* synapses, sorted by target: per target [syn_ptr[t], syn_ptr[t+1]) */
-  int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate */
+  int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate, 2 ionotropic */
   const int32_t* syn_pre;   /* topology-relative presynaptic node, at syn_base */
   const int32_t* syn_ptr;   /* n_tgt + 1 per topology at syn_ptr_base */
   const int32_t* tgt_node;  /* topology-relative target node, at tgt_base */

@@ -234,6 +234,28 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
       const G g = gs * snew * scale;
       sl += g;
       sc += g * static_cast<G>(v[node * S + s] - static_cast<double>(e_syn));
+    } else if (a.syn_model == 2) {
+      // IonotropicSynapse (synapses/ionotropic.py:44-80): u = (v_pre - v_th) / delta,
+      // s_inf = 1 / (1 + save_exp(-u)), rate = k_minus (1 + exp(clamp(u, -20, 50))),
+      // s' = s e + s_inf (1 - e) with e = save_exp(-rate dt); i = gS s' (v_post - e_syn),
+      // linear in v_post (slope gS s').
+      const int node = a.tgt_node[tp.tgb + tgt];
+      const G e_syn = a.syn_param[1][gsyn];
+      const G k_minus = a.syn_param[2][gsyn];
+      const G v_th = a.syn_param[3][gsyn];
+      const G delta = a.syn_param[4][gsyn];
+      G* sp = a.syn_s + pg.synoff + static_cast<int64_t>(syn) * S + s;
+      const G st = *sp;
+      const G u = (vpre - v_th) / delta;
+      const G sinf = G(1) / (G(1) + save_exp(-u));
+      const G uc = u < G(-20) ? G(-20) : (u > G(50) ? G(50) : u);
+      const G rate = k_minus * (G(1) + dexp(uc));
+      const G e = save_exp(-rate * dt);
+      const G snew = st * e + sinf * (G(1) - e);
+      *sp = snew;
+      const G g = gs * snew * scale;
+      sl += g;
+      sc += g * static_cast<G>(v[node * S + s] - static_cast<double>(e_syn));
     } else {
       const G xo = a.syn_param[1][gsyn];
       const G slope = a.syn_param[2][gsyn];
@@ -781,7 +803,7 @@ __device__ __forceinline__ void record_step(const Fused<G, L>& a, const Prog& pg
 template <typename G, typename L>
 __device__ __forceinline__ void checkpoint_syn(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
                                                int64_t c) {
-  if (a.syn_model == 0 && a.ckpt_s != nullptr) {
+  if ((a.syn_model == 0 || a.syn_model == 2) && a.ckpt_s != nullptr) {
     const int64_t nsyn = static_cast<int64_t>(tp.n_syn) << a.log2S;
     const int64_t so = pg.synoff + c * a.ckpt_syn_stride;
     for (int64_t i = threadIdx.x; i < nsyn; i += blockDim.x) a.ckpt_s[so + i] = a.syn_s[pg.synoff + i];
@@ -1290,6 +1312,7 @@ inline int check_args(const jxhip_fused_args& a) {
     return hipErrorInvalidValue;
   if (a.S < 1 || a.S > 64 || (a.S & (a.S - 1))) return hipErrorInvalidValue;
   if (a.lds_bytes < 0 || a.lds_bytes > 160 * 1024) return hipErrorInvalidValue;
+  if (a.syn_model < -1 || a.syn_model > 2) return hipErrorInvalidValue;
   return hipSuccess;
 }
 

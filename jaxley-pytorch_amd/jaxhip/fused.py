@@ -29,6 +29,7 @@ ALL_PARAMS = HH_PARAMS + LEAK_PARAMS
 # the batched table.
 BATCHABLE = {"HH_gNa": 0, "HH_gK": 1, "HH_gLeak": 2, "Leak_gLeak": 3}
 FLAG_COMP, FLAG_HH, FLAG_LEAK, FLAG_EXPORT = 1, 2, 4, 8
+STATEFUL_SYN = (0, 2)  # synapse models with a state s (Kinetic, Ionotropic)
 SYN_DIFF = 1e-3  # mV, the finite difference of the synapse slope (network.py:29)
 
 
@@ -345,7 +346,7 @@ class ForwardLaunch:
         self.syn_params = syn_params
         self.syn_scale = syn_scale
         self.syn_s = None
-        if st.syn is not None and st.syn["model"] == 0:
+        if st.syn is not None and st.syn["model"] in STATEFUL_SYN:
             self.syn_s = to_prog(syn_s0, S)
         self.ckpt_every = ckpt_every
         self.n_ckpt = 0

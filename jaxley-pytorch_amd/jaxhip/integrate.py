@@ -23,8 +23,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, ForwardLaunch, LaunchConfig,
-                    build_structure, default_config, node_float_table, to_prog)
+from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, STATEFUL_SYN, ForwardLaunch,
+                    LaunchConfig, build_structure, default_config, node_float_table, to_prog)
 from .geometry import axial_conductances, comp_geometry
 
 _SUPPORTED_SOLVERS = ("jaxley.dhs.gpu", "jaxley.dhs.cpu", "jaxley.stone")
@@ -377,7 +377,7 @@ def _prepare(module, inp, dt, device, user_dtype) -> Problem:
         area_node = torch.full((module._n_nodes,), 1.0, dtype=dtype, device=device)
         area_node[torch.as_tensor(module._comp_node, device=device)] = geo["area"]
         syn_scale = (1e5 / area_node[torch.as_tensor(st.syn["post_node"], device=device)]).contiguous()
-        if st.syn["model"] == 0:
+        if st.syn["model"] in STATEFUL_SYN:
             s0 = inp.syn_vals[f"{stype.name}_s"].to(dtype)
             if inp.all_states is not None and f"{stype.name}_s" in inp.all_states:
                 s0 = inp.all_states[f"{stype.name}_s"].to(dtype)

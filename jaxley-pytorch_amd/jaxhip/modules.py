@@ -18,7 +18,7 @@ import torch
 
 from . import schedule as sch
 from .channels import HH, Channel, Leak
-from .synapses import KineticSynapse, SigmoidRateSynapse, Synapse
+from .synapses import IonotropicSynapse, KineticSynapse, SigmoidRateSynapse, Synapse
 
 COMP_PARAMS = {"length": 10.0, "radius": 1.0, "axial_resistivity": 5000.0, "capacitance": 1.0}
 COMP_STATES = {"v": -70.0}
@@ -314,7 +314,7 @@ class Module(View):
         """Append synapses between node indices (`Network._append_multiple_synapses`,
         modules/network.py:736-790): one row per (pre, post) pair, parameters and states
         at the synapse type's defaults."""
-        if not isinstance(synapse_type, (KineticSynapse, SigmoidRateSynapse)):
+        if not isinstance(synapse_type, (KineticSynapse, SigmoidRateSynapse, IonotropicSynapse)):
             raise NotImplementedError(f"synapse {type(synapse_type).__name__} is not on the jaxhip path")
         pre = np.asarray(pre_nodes, dtype=np.int64).ravel()
         post = np.asarray(post_nodes, dtype=np.int64).ravel()

@@ -1,5 +1,6 @@
-"""Synapses on the hot path: `KineticSynapse` (synapses/kinetic.py:12-63) and
-`SigmoidRateSynapse` (synapses/sigmoid_rate.py:12-50) of jaxley-pytorch.
+"""Synapses on the hot path: `KineticSynapse` (synapses/kinetic.py:12-63),
+`SigmoidRateSynapse` (synapses/sigmoid_rate.py:12-50) and `IonotropicSynapse`
+(synapses/ionotropic.py:13-80; fused forward only) of jaxley-pytorch.
 
 As in the reference, a synapse is a point process (conductances in uS, currents in nA);
 its arithmetic lives in the HIP kernels (csrc/jxhip_synapse.hip, jxhip_fused.hip).
@@ -50,4 +51,22 @@ class SigmoidRateSynapse(Synapse):
         self.synapse_states = {}
 
 
-__all__ = ["Synapse", "KineticSynapse", "SigmoidRateSynapse"]
+class IonotropicSynapse(Synapse):
+    """Generic ionotropic synapse (Abbott & Marder 1998): s relaxes to
+    1 / (1 + exp(-(v_pre - v_th) / delta)) at rate k_minus (1 + exp(u)); current
+    gS s (v_post - e_syn)."""
+
+    kernel_model = 2
+    param_names = ("gS", "e_syn", "k_minus", "v_th", "delta")
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        p = self._name
+        self.synapse_params = {
+            f"{p}_gS": 1e-4, f"{p}_e_syn": 0.0, f"{p}_k_minus": 0.025, f"{p}_v_th": -35.0,
+            f"{p}_delta": 10.0,
+        }
+        self.synapse_states = {f"{p}_s": 0.2}
+
+
+__all__ = ["Synapse", "KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse"]

@@ -146,8 +146,9 @@ def build_c4(n_cells: int, k: int, seed: int, synapse: str = "KineticSynapse"):
     the area of a 10 um sphere) with HH and an 8-compartment dendrite (200 um, radius
     1 um) with Leak. Every cell receives k synapses from uniformly drawn presynaptic
     somata, all onto one uniformly drawn dendritic compartment; synapses are sorted by
-    target. KineticSynapse: gS ~ U(0, 1e-3) uS, e_syn 0 mV (80 % of presynaptic cells)
-    or -80 mV; SigmoidRateSynapse: gS ~ U(0, 1e-3), default offset and slope.
+    target. KineticSynapse / IonotropicSynapse: gS ~ U(0, 1e-3) uS, e_syn 0 mV (80 % of
+    presynaptic cells) or -80 mV; SigmoidRateSynapse: gS ~ U(0, 1e-3), default offset and
+    slope.
     Returns (network, oracle model, info)."""
     rng = np.random.default_rng(seed)
     cell = jx.Cell([jx.Branch(jx.Compartment(), 1), jx.Branch(jx.Compartment(), 8)], [-1, 0])
@@ -163,11 +164,12 @@ def build_c4(n_cells: int, k: int, seed: int, synapse: str = "KineticSynapse"):
     tgt_comp = rng.integers(1, 9, size=n_cells)
     pre_comp = (pre_cells * 9).reshape(-1)
     post_comp = np.repeat(np.arange(n_cells) * 9 + tgt_comp, k)
-    syn = jx.KineticSynapse() if synapse == "KineticSynapse" else jx.SigmoidRateSynapse()
+    syn = {"KineticSynapse": jx.KineticSynapse, "SigmoidRateSynapse": jx.SigmoidRateSynapse,
+           "IonotropicSynapse": jx.IonotropicSynapse}[synapse]()
     net.add_synapses(net._comp_node[pre_comp], net._comp_node[post_comp], syn)
     E = len(pre_comp)
     getattr(net, syn.name).set(f"{syn.name}_gS", rng.uniform(0.0, 1e-3, size=E))
-    if synapse == "KineticSynapse":
+    if synapse in ("KineticSynapse", "IonotropicSynapse"):
         e_syn = np.where(excit[pre_cells.reshape(-1)], 0.0, -80.0)
         getattr(net, syn.name).set(f"{syn.name}_e_syn", e_syn)
     edge = net._edges[0]
@@ -177,7 +179,7 @@ def build_c4(n_cells: int, k: int, seed: int, synapse: str = "KineticSynapse"):
     om.radius = np.tile(radius, n_cells)
     om.hh_comps = np.arange(n_cells) * 9
     om.leak_comps = np.setdiff1d(np.arange(9 * n_cells), om.hh_comps)
-    s0 = edge["states"][f"{syn.name}_s"].copy() if synapse == "KineticSynapse" else None
+    s0 = edge["states"][f"{syn.name}_s"].copy() if synapse != "SigmoidRateSynapse" else None
     om.synapses = [{"type": synapse, "pre": om.internal[pre_comp], "post": om.internal[post_comp],
                     "params": params, "s0": s0}]
     info = {"soma_comps": np.arange(n_cells) * 9, "pre_comp": pre_comp, "post_comp": post_comp}

@@ -1,0 +1,89 @@
+"""The product path (`jaxhip.integrate` through the fused HIP kernel, float64) against the
+reference's own golden voltage traces (tests/jaxley_identical/test_basic_modules.py and
+test_radius_and_length.py of jaxley-pytorch, frozen as tests/golden/reference_traces.json:
+every 20th step of a 0.02 nA step current from 0.5 to 1.5 ms, dt = 0.025 ms, 201 steps).
+The reference checks these traces at 1e-8 mV; so does this test."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import jaxhip as jx
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_traces.json").read_text())
+
+
+@pytest.fixture(autouse=True)
+def float64_default():
+    """The reference's tests run with float64 as the default dtype (tests/conftest.py:17),
+    which `step_current` follows (stimulus.py:8-36)."""
+    old = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(old)
+
+
+def _run(module, stim_view, rec_view, device):
+    cur = jx.step_current(0.5, 1.0, 0.02, 0.025, 5.0).to(device)
+    assert cur.dtype == torch.float64
+    rec_view.record()
+    v = jx.integrate(module, data_stimuli=stim_view.data_stimulate(cur), delta_t=0.025)
+    return v.double().cpu().numpy()
+
+
+def _check(v, key):
+    gold = np.asarray(GOLD[key]["values"])
+    err = np.abs(v[..., ::20] - gold).max()
+    print(key, "max err vs reference golden", err)
+    assert v.shape[-1] == 202          # the initial state and 201 steps
+    assert err <= 1e-8, (key, err)
+
+
+def test_golden_compartment(device):
+    comp = jx.Compartment()
+    comp.insert(jx.HH())
+    _check(_run(comp, comp, comp, device)[0], "compartment")
+
+
+def test_golden_branch(device):
+    branch = jx.Branch(jx.Compartment(), 2)
+    branch.insert(jx.HH())
+    _check(_run(branch, branch.loc(0.0), branch.loc(0.0), device)[0], "branch2")
+
+
+def test_golden_cell(device):
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 2), parents=[-1, 0, 0])
+    cell.insert(jx.HH())
+    _check(_run(cell, cell.branch(1).loc(0.0), cell.branch(1).loc(0.0), device)[0], "cell_3_2")
+
+
+def test_golden_radius_and_length(device):
+    np.random.seed(1)
+    comp = jx.Compartment()
+    comp.insert(jx.HH())
+    comp.set("length", 5 * np.random.rand(1))
+    comp.set("radius", np.random.rand(1))
+    _check(_run(comp, comp, comp, device)[0], "radius_and_length_compartment")
+
+
+def test_golden_network_ionotropic(device):
+    """Two SimpleCell(3, 2) cells, an IonotropicSynapse from cell 0 (branch 0, loc 0) to
+    cell 1 (branch 0, loc 0), gS = 0.5 / (1e5 / (2 pi 10)), both cells stimulated and
+    recorded at branch 1, loc 0 (test_basic_modules.py:336-409)."""
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 2), parents=[-1, 0, 0])
+    net = jx.Network([cell, cell])
+    net.insert(jx.HH())
+    jx.connect(net.cell(0).branch(0).loc(0.0), net.cell(1).branch(0).loc(0.0), jx.IonotropicSynapse())
+    net.IonotropicSynapse.set("IonotropicSynapse_gS", 0.5 / (100000.0 / (2 * np.pi * 10.0 * 1.0)))
+    cur = jx.step_current(0.5, 1.0, 0.02, 0.025, 5.0).to(device)
+    ds = None
+    for c in (0, 1):
+        net.cell(c).branch(1).loc(0.0).record()
+        ds = net.cell(c).branch(1).loc(0.0).data_stimulate(cur, ds)
+    v = jx.integrate(net, data_stimuli=ds, delta_t=0.025).double().cpu().numpy()
+    assert v.shape == (2, 202)
+    _check(v, "net_2x_cell_3_2")

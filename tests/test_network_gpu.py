@@ -1,7 +1,7 @@
 """Networks through the fused kernel (synapse step fused into the voltage and gate
-steps) against the oracle, float64 and float32: ball-and-stick cells, KineticSynapse
-and SigmoidRateSynapse with fixed in-degree onto one dendritic compartment per cell,
-Bernoulli current pulses into every soma."""
+steps) against the oracle, float64 and float32: ball-and-stick cells, KineticSynapse,
+SigmoidRateSynapse and IonotropicSynapse with fixed in-degree onto one dendritic
+compartment per cell, Bernoulli current pulses into every soma."""
 import numpy as np
 import pytest
 import torch
@@ -31,7 +31,7 @@ def _run(device, synapse, dtype, n_cells=12, k=4, B=2, T=400, seed=4):
     return v, ref
 
 
-@pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse"])
+@pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse"])
 def test_network_float64(device, synapse):
     v, ref = _run(device, synapse, torch.float64)
     assert v.shape == ref.shape
