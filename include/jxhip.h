@@ -278,6 +278,14 @@ typedef struct jxhip_fused_args {
                                [node][S] at state_off, gate type; computed from v at the
                                start of the launch and updated from the increments after
                                every solve. Unused by the register-resident kernel. */
+  /* The voltage step (base.py:3327-3366 of jaxley-pytorch): backward Euler solves with
+     dt; Crank-Nicolson solves with dt / 2 and takes v' = 2 x - v, which in the increment
+     form is a right-hand side scaled by 2. Gates and synapse states always advance by
+     dt. node_f's coupling coefficients and stim_scale are built with dt_v. 0 = the
+     backward-Euler values (dt_v = dt, rhs_scale = 1); the reverse pass accepts only
+     those. */
+  double dt_v;
+  double rhs_scale;
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

@@ -69,6 +69,8 @@ struct Fused {
   const G* syn_scale;
   G* syn_s;
   G dt;
+  G dtv;     // step of the voltage solve (dt, or dt / 2 for Crank-Nicolson)
+  G rscale;  // right-hand-side factor (1, or 2 for Crank-Nicolson)
   G diff;
   int32_t S;
   int32_t log2S;
@@ -116,6 +118,8 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.syn_scale = static_cast<const G*>(a.syn_scale);
   f.syn_s = static_cast<G*>(a.syn_s);
   f.dt = static_cast<G>(a.dt);
+  f.dtv = static_cast<G>(a.dt_v > 0 ? a.dt_v : a.dt);
+  f.rscale = static_cast<G>(a.rhs_scale > 0 ? a.rhs_scale : 1.0);
   f.diff = static_cast<G>(a.diff);
   f.S = a.S;
   int l = 0;
@@ -557,7 +561,7 @@ __device__ __forceinline__ void build_core(const Fused<G, L>& a, const Prog& pg,
   const int32_t flags = nik[kNiFlags];
   if (!(flags & kComp)) {
     Ld.ds[i] = nfk[kNfDbase];
-    Ld.xs[i] = static_cast<L>(ax);
+    Ld.xs[i] = static_cast<L>(a.rscale * static_cast<G>(ax));
     return;
   }
   const G dt = a.dt;
@@ -592,14 +596,14 @@ __device__ __forceinline__ void build_core(const Fused<G, L>& a, const Prog& pg,
     lin += static_cast<G>(Ld.tl[(tg << lS) + s]);
     net -= static_cast<G>(Ld.tc[(tg << lS) + s]);
   }
-  G r = static_cast<G>(ax) + dt * inv_cm * net;
+  G r = static_cast<G>(ax) + a.dtv * inv_cm * net;
   for (int e = nik[kNiStimBeg]; e < nik[kNiStimEnd]; ++e) {
     const int64_t col = a.stim_col[e];
     const int64_t bidx = a.stim_b > 1 ? (pg.simoff + s) : 0;
     r += a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e];
   }
-  Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * inv_cm * lin);
-  Ld.xs[i] = static_cast<L>(r);
+  Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(a.dtv * inv_cm * lin);
+  Ld.xs[i] = static_cast<L>(a.rscale * r);
 }
 
 // build_core with the axial residual from the float64 voltages vg (program layout).
@@ -694,7 +698,7 @@ __device__ __forceinline__ void build_loaded(const Fused<G, L>& a, const Prog& p
                                              int64_t t) {
   if (!(c.flags & kComp)) {
     Ld.ds[i] = c.dbase;
-    Ld.xs[i] = static_cast<L>(it.ax);
+    Ld.xs[i] = static_cast<L>(a.rscale * it.ax);
     return;
   }
   const int lS = a.log2S;
@@ -730,14 +734,14 @@ __device__ __forceinline__ void build_loaded(const Fused<G, L>& a, const Prog& p
     lin += static_cast<G>(Ld.tl[(c.tgt << lS) + s]);
     net -= static_cast<G>(Ld.tc[(c.tgt << lS) + s]);
   }
-  G r = it.ax + dt * inv_cm * net;
+  G r = it.ax + a.dtv * inv_cm * net;
   for (int e = c.sbeg; e < c.send; ++e) {
     const int64_t col = a.stim_col[e];
     const int64_t bidx = a.stim_b > 1 ? (pg.simoff + s) : 0;
     r += a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e];
   }
-  Ld.ds[i] = c.dbase + static_cast<L>(dt * inv_cm * lin);
-  Ld.xs[i] = static_cast<L>(r);
+  Ld.ds[i] = c.dbase + static_cast<L>(a.dtv * inv_cm * lin);
+  Ld.xs[i] = static_cast<L>(a.rscale * r);
 }
 
 // Items of simulations beyond `nsim` (padding): a well-conditioned row, zero rhs.
@@ -1313,6 +1317,7 @@ inline int check_args(const jxhip_fused_args& a) {
   if (a.S < 1 || a.S > 64 || (a.S & (a.S - 1))) return hipErrorInvalidValue;
   if (a.lds_bytes < 0 || a.lds_bytes > 160 * 1024) return hipErrorInvalidValue;
   if (a.syn_model < -1 || a.syn_model > 2) return hipErrorInvalidValue;
+  if (a.dt_v < 0 || a.rhs_scale < 0) return hipErrorInvalidValue;
   return hipSuccess;
 }
 
@@ -1395,6 +1400,8 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (int e = check_args(a)) return e;
   if (a.syn_model >= 0 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
   if (a.threads > 512) return hipErrorInvalidValue;  // fused_adjoint_kernel's launch bound
+  // The reverse pass differentiates the backward-Euler step only.
+  if ((a.dt_v > 0 && a.dt_v != a.dt) || (a.rhs_scale > 0 && a.rhs_scale != 1.0)) return hipErrorInvalidValue;
   const Fused<G, L> f = unpack<G, L>(a);
   Adj<G> g;
   g.g_rec = static_cast<const G*>(adj->g_rec);

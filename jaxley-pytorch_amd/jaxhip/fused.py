@@ -316,7 +316,8 @@ class ForwardLaunch:
                  v0: torch.Tensor, gates0: Tuple[torch.Tensor, ...], stim: Optional[torch.Tensor],
                  stim_scale: Optional[torch.Tensor], T: int, dt: float, B: int, cfg: LaunchConfig,
                  syn_params: Optional[List[torch.Tensor]] = None, syn_scale: Optional[torch.Tensor] = None,
-                 syn_s0: Optional[torch.Tensor] = None, ckpt_every: int = 0):
+                 syn_s0: Optional[torch.Tensor] = None, ckpt_every: int = 0,
+                 dt_v: float = 0.0, rhs_scale: float = 0.0):
         dev = v0.device
         dtype = gates0[0].dtype
         S = cfg.S
@@ -398,6 +399,10 @@ class ForwardLaunch:
             a.syn_model = -1
         a.sims_total = B
         a.dt = dt
+        # Crank-Nicolson: solve with dt / 2 (node_f and stim_scale built with it) and a
+        # doubled right-hand side; 0 = backward Euler.
+        a.dt_v = dt_v
+        a.rhs_scale = rhs_scale
         a.diff = SYN_DIFF
         a.S = S
         a.threads = cfg.threads

@@ -6,7 +6,7 @@ stimuli `(..., n_inputs, time)` padded / truncated to `int(t_max // delta_t + 1)
 leading batch dimensions of parameters, `data_set` values, stimuli and states broadcast
 against each other; recordings `(..., n_recordings, 1 + steps)` (the initial state and
 every step); `voltage_solver` in {"jaxley.dhs", "jaxley.dhs.gpu", "jaxley.dhs.cpu",
-"jaxley.stone"}; gradients through autograd.
+"jaxley.stone"}; `solver` "bwd_euler" or "crank_nicolson"; gradients through autograd.
 
 Execution: the whole time loop is one fused HIP kernel launch (`jaxhip.fused`), with a
 hand-written adjoint (checkpoints every K steps, recomputation between them) for
@@ -28,6 +28,9 @@ from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, STATEFUL_SYN,
 from .geometry import axial_conductances, comp_geometry
 
 _SUPPORTED_SOLVERS = ("jaxley.dhs.gpu", "jaxley.dhs.cpu", "jaxley.stone")
+# Time steppers on the implicit solve (base.py:3327-3366): Crank-Nicolson solves with
+# dt / 2 and takes v' = 2 x - v.
+_SUPPORTED_STEPS = ("bwd_euler", "crank_nicolson")
 
 
 def _resolve_voltage_solver(module, voltage_solver: str) -> str:
@@ -170,8 +173,8 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     if voltage_solver not in _SUPPORTED_SOLVERS:
         raise NotImplementedError(f"voltage_solver={voltage_solver!r} is not part of jaxhip "
                                   f"(supported: {_SUPPORTED_SOLVERS})")
-    if solver != "bwd_euler":
-        raise NotImplementedError(f"solver={solver!r}: only 'bwd_euler' is implemented")
+    if solver not in _SUPPORTED_STEPS:
+        raise NotImplementedError(f"solver={solver!r}: supported are {_SUPPORTED_STEPS}")
     if data_clamps is not None:
         raise NotImplementedError("clamps are not part of jaxhip")
     tables, _ = module._ensure_solver()
@@ -191,13 +194,15 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     if not module.recordings:
         raise ValueError("No recordings are set. Please set them.")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype)
-    if module._edges and _needs_grad(inp):
-        # Gradients through synapses: the differentiable per-step path (the fused
-        # adjoint covers cells without synapses).
+    if (module._edges or solver != "bwd_euler") and _needs_grad(inp):
+        # Gradients through synapses or through Crank-Nicolson steps: the differentiable
+        # per-step path (the fused adjoint covers backward-Euler cells without synapses).
         from .stepwise import integrate_stepwise
 
-        return integrate_stepwise(module, inp, delta_t, checkpoint_lengths, device, dtype, return_states)
-    return _integrate_fused(module, inp, delta_t, checkpoint_lengths, return_states, device, dtype)
+        return integrate_stepwise(module, inp, delta_t, checkpoint_lengths, device, dtype, return_states,
+                                  solver=solver)
+    return _integrate_fused(module, inp, delta_t, checkpoint_lengths, return_states, device, dtype,
+                            solver)
 
 
 def _needs_grad(inp) -> bool:
@@ -230,8 +235,12 @@ class Problem:
     conductances, initial states, stimulus and synapses (see `_prepare`)."""
 
 
-def _prepare(module, inp, dt, device, user_dtype) -> Problem:
-    """Everything the fused kernel needs, independent of the launch shape."""
+def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> Problem:
+    """Everything the fused kernel needs, independent of the launch shape. The tree's
+    coefficients and the stimulus scale use the solve's step (dt / 2 for
+    Crank-Nicolson); gates and synapses advance by dt."""
+    cn = solver == "crank_nicolson"
+    dtv = dt / 2 if cn else dt
     # Element type of gates, rates, parameters and recordings in the kernel.
     dtype = torch.float64 if (user_dtype == torch.float32 and _F32_MODE == "mixed") else user_dtype
     vals = {k: v.to(dtype) for k, v in inp.vals.items()}
@@ -257,7 +266,7 @@ def _prepare(module, inp, dt, device, user_dtype) -> Problem:
     g = axial_conductances(tables, module._comp_node, module._n_nodes, geo, vals["axial_resistivity"], vals["capacitance"])
     cm_node = torch.full((module._n_nodes,), -1.0, dtype=dtype, device=device)
     cm_node[torch.as_tensor(module._comp_node, device=device)] = vals["capacitance"]
-    node_f = node_float_table(st, g, cm_node, dt, user_dtype)
+    node_f = node_float_table(st, g, cm_node, dtv, user_dtype)
 
     # Batch: parameters, states, stimulus.
     chan_keys = [k for k in ALL_PARAMS if k in vals]
@@ -360,7 +369,7 @@ def _prepare(module, inp, dt, device, user_dtype) -> Problem:
         area_node = torch.full((module._n_nodes,), 1.0, dtype=dtype, device=device)
         area_node[torch.as_tensor(module._comp_node, device=device)] = geo["area"]
         sn = torch.as_tensor(inp.stim_nodes, device=device)
-        scale = dt * 1e5 / (area_node[sn] * cm_node[sn])
+        scale = dtv * 1e5 / (area_node[sn] * cm_node[sn])
         order_e = torch.as_tensor(st.stim_order, device=device)
         stim_scale = scale[order_e].contiguous()
     # Synapses.
@@ -388,13 +397,16 @@ def _prepare(module, inp, dt, device, user_dtype) -> Problem:
     P.bparam_for, P.v0, P.gates0, P.stim, P.stim_scale = bparam_for, v0, gates0, stim, stim_scale
     P.syn_params, P.syn_scale, P.syn_s0 = syn_params, syn_scale, syn_s0
     P.B, P.batch, P.T, P.dt, P.dtype, P.user_dtype = B, batch, inp.T, dt, dtype, user_dtype
+    # Launch arguments of the time stepper (0 = backward Euler).
+    P.step_kw = {"dt_v": dtv if cn else 0.0, "rhs_scale": 2.0 if cn else 0.0}
     P.needs_grad, P.device, P.stim_requires_grad = needs_grad, device, (
         inp.stim is not None and inp.stim.requires_grad)
     return P
 
 
-def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device, user_dtype):
-    P = _prepare(module, inp, dt, device, user_dtype)
+def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device, user_dtype,
+                     solver: str = "bwd_euler"):
+    P = _prepare(module, inp, dt, device, user_dtype, solver)
     st, node_f, node_p, bmode, bparam_for = P.st, P.node_f, P.node_p, P.bmode, P.bparam_for
     v0, gates0, stim, stim_scale = P.v0, P.gates0, P.stim, P.stim_scale
     syn_params, syn_scale, syn_s0 = P.syn_params, P.syn_scale, P.syn_s0
@@ -409,7 +421,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
                 run = ForwardLaunch(st, node_f, node_p.detach(),
                                     None if bmode == 0 else bparam_for(c.S).detach(), bmode,
                                     v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
-                                    Tm, dt, B, c, syn_params, syn_scale, syn_s0)
+                                    Tm, dt, B, c, syn_params, syn_scale, syn_s0, **P.step_kw)
                 return _time(run.launch, reps=2)
         except RuntimeError:
             return float("inf")
@@ -430,7 +442,7 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         with torch.no_grad():
             run = ForwardLaunch(st, node_f, node_p.detach(), None if bparam is None else bparam.detach(),
                                 bmode, v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
-                                inp.T, dt, B, cfg, syn_params, syn_scale, syn_s0)
+                                inp.T, dt, B, cfg, syn_params, syn_scale, syn_s0, **P.step_kw)
             run.launch()
             recs = run.recordings()
             final = run
@@ -466,7 +478,8 @@ class _StepRunner:
     step on the current stream and copies them out (no host synchronization, so a step
     can be captured in a CUDA / HIP graph)."""
 
-    def __init__(self, module, params, all_states, param_state, delta_t, ext_nodes, ext_batch):
+    def __init__(self, module, params, all_states, param_state, delta_t, ext_nodes, ext_batch,
+                 solver: str = "bwd_euler"):
         ref = first_float_tensor(params, param_state, all_states)
         device = ref.device if ref is not None else torch.get_default_device()
         dtype = ref.dtype if ref is not None else torch.get_default_dtype()
@@ -482,7 +495,7 @@ class _StepRunner:
         if inp.stim is not None:
             inp.stim = inp.stim[..., :1]
         with torch.no_grad():
-            P = _prepare(module, inp, delta_t, device, dtype)
+            P = _prepare(module, inp, delta_t, device, dtype, solver)
         from .autotune import fused_config
 
         cfg = fused_config(P.st, P.B, P.dtype, torch.tensor([], dtype=dtype).element_size(), False)
@@ -490,7 +503,7 @@ class _StepRunner:
         self.module, self.n_nodes = module, module._n_nodes
         self.run = ForwardLaunch(P.st, P.node_f, P.node_p, None if P.bmode == 0 else P.bparam_for(cfg.S),
                                  P.bmode, P.v0, tuple(P.gates0), P.stim, P.stim_scale, 1, delta_t, P.B, cfg,
-                                 P.syn_params, P.syn_scale, P.syn_s0)
+                                 P.syn_params, P.syn_scale, P.syn_s0, **P.step_kw)
         self.mp = torch.as_tensor(P.st.mp, device=device)
         hh = [k for k in ("HH_m", "HH_h", "HH_n") if k in module._node_data]
         self.gate_names = hh
@@ -558,7 +571,7 @@ def build_init_and_step_fn(module, voltage_solver: str = "jaxley.dhs", solver: s
       `(all_states, all_params)`: the states over nodes (`v`, HH gates with NaN where
       the channel is absent, synapse states) and the parameters to pass to `step_fn`.
     * `step_fn(all_states, all_params, externals, external_inds, delta_t=0.025)` runs one
-      backward-Euler step with the currents `externals["i"]` (`(..., n_inputs)`, nA) into
+      backward-Euler (or Crank-Nicolson) step with the currents `externals["i"]` (`(..., n_inputs)`, nA) into
       the nodes `external_inds["i"]` and returns the new states.
 
     The step is one launch of the fused kernel with T = 1; its buffers are kept between
@@ -568,8 +581,8 @@ def build_init_and_step_fn(module, voltage_solver: str = "jaxley.dhs", solver: s
     voltage_solver = _resolve_voltage_solver(module, voltage_solver)
     if voltage_solver not in _SUPPORTED_SOLVERS:
         raise NotImplementedError(f"voltage_solver={voltage_solver!r} is not part of jaxhip")
-    if solver != "bwd_euler":
-        raise NotImplementedError(f"solver={solver!r}: only 'bwd_euler' is implemented")
+    if solver not in _SUPPORTED_STEPS:
+        raise NotImplementedError(f"solver={solver!r}: supported are {_SUPPORTED_STEPS}")
     external_inds0 = module.external_inds.copy()
     cache: Dict[Any, _StepRunner] = {}
 
@@ -613,7 +626,7 @@ def build_init_and_step_fn(module, voltage_solver: str = "jaxley.dhs", solver: s
             base = {k: torch.broadcast_to(torch.as_tensor(x), (*batch, x.shape[-1])).contiguous()
                     for k, x in all_states.items()}
             runner = _StepRunner(module, all_params["params"], base, all_params["param_state"],
-                                 delta_t, inds, batch)
+                                 delta_t, inds, batch, solver)
             cache.clear()
             cache[key] = runner
         with torch.no_grad():

@@ -108,9 +108,14 @@ class _SynGroup:
 
 
 def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List[int]], device,
-                       dtype, return_states: bool = False):
+                       dtype, return_states: bool = False, solver: str = "bwd_euler"):
     """`integrate` through the differentiable per-step path (see the module docstring).
-    Returns the recordings `(*batch, n_rec, 1 + T)`."""
+    Returns the recordings `(*batch, n_rec, 1 + T)`. `solver` "crank_nicolson" solves
+    with dt / 2 and takes 2 x - v (base.py:3327-3366); gates and synapses advance by dt."""
+    if solver not in ("bwd_euler", "crank_nicolson"):
+        raise NotImplementedError(f"solver={solver!r}")
+    cn = solver == "crank_nicolson"
+    dtv = dt / 2 if cn else dt
     tables, ind = module._ensure_solver()
     n = module._n_nodes
     vals = {k: x.to(dtype) for k, x in inp.vals.items()}
@@ -142,7 +147,7 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
 
     cm = node_array(vals["capacitance"], 1.0)
     area = node_array(geo["area"], 1.0)
-    gdt = dt * flat(g_ax)
+    gdt = dtv * flat(g_ax)
     sinks = torch.as_tensor(np.asarray(tables["sink"], dtype=np.int64), device=device)
     # Channels (HH, Leak) and their compartments.
     chans = {}
@@ -227,12 +232,14 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
         if stim is not None:
             cur = stim[..., t]
             cst = cst.index_add(1, stim_nodes, torch.broadcast_to(cur / area[:, stim_nodes] * 1e5, (B, len(stim_nodes))))
-        diag = diag0 + dt * lin / cm
-        rhs = torch.where(internal_mask, v + dt * cst / cm, torch.zeros_like(v))
+        diag = diag0 + dtv * lin / cm
+        rhs = torch.where(internal_mask, v + dtv * cst / cm, torch.zeros_like(v))
         dd = torch.cat([diag[:, mp], diag.new_ones(B, 1)], 1)
         ss = torch.cat([rhs[:, mp], rhs.new_zeros(B, 1)], 1)
-        x = fused_tree_solve(kern, dd, lo, up, ss)[:, :-1]
-        return x[:, inv], gates, new_s
+        x = fused_tree_solve(kern, dd, lo, up, ss)[:, :-1][:, inv]
+        if cn:
+            x = 2 * x - v
+        return x, gates, new_s
 
     T = inp.T
     K = int(checkpoint_lengths[-1]) if checkpoint_lengths else DEFAULT_CHUNK

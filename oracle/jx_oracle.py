@@ -581,14 +581,21 @@ def simulate(
     voltage_solver: str = "jaxley.dhs.gpu",
     return_all: bool = False,
     record_state: str = "v",
+    solver: str = "bwd_euler",
 ):
     """`jx.integrate` (integrate.py:396-666) + `_step_with_plan` (modules/base.py:3214-3400)
-    for HH / Leak channels and Kinetic / SigmoidRate / Ionotropic synapses, bwd_euler.
+    for HH / Leak channels and Kinetic / SigmoidRate / Ionotropic synapses, with
+    `solver` "bwd_euler" or "crank_nicolson" (modules/base.py:3327-3366: the implicit
+    solve with dt / 2, then v' = 2 x - v at every node; channels and synapses still
+    advance by dt).
 
     `stim`: `(..., n_stim, time)` currents in nA (padded / truncated to
     `int(t_max // dt + 1)` steps as integrate.py:523-541 does). Recordings are
     `(..., n_rec, 1 + steps)`: the initial state and every step."""
     overrides = overrides or {}
+    if solver not in ("bwd_euler", "crank_nicolson"):
+        raise ValueError(f"solver {solver!r}")
+    dtv = dt / 2 if solver == "crank_nicolson" else dt
     p = model.params(overrides)
     n = model.n_nodes
     internal = torch.as_tensor(model.internal)
@@ -634,7 +641,7 @@ def simulate(
     cm = p["capacitance"]
     area = p["area"]
     dense = voltage_solver in ("jax.sparse", "torch.dense")
-    solver = None if dense else DHSSolverOracle(model.indexer, optimize_for_gpu=voltage_solver.endswith("gpu") or voltage_solver == "jaxley.stone")
+    dhs = None if dense else DHSSolverOracle(model.indexer, optimize_for_gpu=voltage_solver.endswith("gpu") or voltage_solver == "jaxley.stone")
     ind = model.indexer
     mp = torch.as_tensor(ind["map_to_solve_order"])
     inv = torch.as_tensor(ind["inv_map_to_solve_order"])
@@ -698,13 +705,13 @@ def simulate(
         lin_t = (lin + slin) / cm
         const_t = (const + i_ext + sconst) / cm
         # _build_diags_and_solves (solver_voltage.py:250-291)
-        gdt = dt * g_ax
+        gdt = dtv * g_ax
         bsh = torch.broadcast_shapes(v.shape[:-1], lin_t.shape[:-1], const_t.shape[:-1], gdt.shape[:-1])
-        diag = torch.broadcast_to(dt * lin_t, (*bsh, n)).clone()
+        diag = torch.broadcast_to(dtv * lin_t, (*bsh, n)).clone()
         diag = diag.index_add(-1, internal, torch.ones((*bsh, len(internal)), dtype=F64))
         if len(sinks):
             diag = diag.index_add(-1, sinks, torch.broadcast_to(gdt, (*bsh, gdt.shape[-1])))
-        vals = v[..., internal] + dt * const_t[..., internal]
+        vals = v[..., internal] + dtv * const_t[..., internal]
         rhs = torch.zeros((*bsh, n), dtype=F64).index_copy(-1, internal, torch.broadcast_to(vals, (*bsh, len(internal))))
         if len(sinks) == 0:
             x = rhs / diag
@@ -726,8 +733,10 @@ def simulate(
             ss = torch.cat([ss, zero], -1)
             lo = torch.cat([lo, zero], -1)
             up = torch.cat([up, zero], -1)
-            x = solver.solve(dd, lo, up, ss)[..., :-1]
+            x = dhs.solve(dd, lo, up, ss)[..., :-1]
             x = x[..., inv]
+        if dtv != dt:
+            x = 2 * x - v
         return x, m, h, nn, syn_states
 
     # Initial synapse currents are only states; they do not change the dynamics.

@@ -151,3 +151,67 @@ def test_x_over_expm1_removable_singularity():
     (am, _), _, (an, _) = O.hh_rates(torch.tensor([-40.0, -55.0], dtype=torch.float64))
     np.testing.assert_allclose(float(am[0]), 1.0, rtol=1e-12)
     np.testing.assert_allclose(float(an[1]), 0.1, rtol=1e-12)
+
+
+# Crank-Nicolson (modules/base.py:3327-3366): the implicit solve with dt / 2, then
+# v' = 2 x - v.
+
+@pytest.mark.parametrize("solver", ["bwd_euler", "crank_nicolson"])
+def test_time_stepper_closed_form(solver):
+    """A passive compartment, dv/dt = -(v - e) / tau: backward Euler multiplies v - e by
+    1 / (1 + a) per step and Crank-Nicolson by (1 - a / 2) / (1 + a / 2), a = dt / tau."""
+    m = O.OracleModel([([-1], [1])])
+    m.leak_comps = np.array([0])
+    m.leak_params["Leak_gLeak"][:] = 1e-3      # tau = cm / (1000 gLeak) = 1 ms
+    m.v0[:] = -50.0
+    v = O.simulate(m, 10.0, 0.025, solver=solver)[0].numpy()
+    a = 0.025 * 1000 * 1e-3
+    R = 1 / (1 + a) if solver == "bwd_euler" else (1 - a / 2) / (1 + a / 2)
+    exact = -70.0 + 20.0 * R ** np.arange(len(v))
+    assert np.abs(v - exact).max() < 1e-10
+
+
+def _passive_cell(solver, dt):
+    m = O.OracleModel([([-1, 0, 0], [2, 2, 2])])
+    m.leak_comps = np.arange(6)
+    m.leak_params["Leak_gLeak"][:] = 1e-3
+    m.length[:] = 50.0
+    m.axial_resistivity[:] = 200.0
+    cur = O.step_current(0.5, 1.0, 0.05, dt, 5.0)
+    v = O.simulate(m, 5.0, dt, stim=torch.as_tensor(cur)[None], stim_comps=[2], rec_comps=[0, 5],
+                   solver=solver).numpy()
+    return np.stack([v[:, int(round(t / dt))] for t in (1.0, 2.0, 4.0)], -1)
+
+
+def test_crank_nicolson_second_order_on_a_branched_cell():
+    """On a passive branched cell (branchpoint included) Crank-Nicolson converges with
+    order 2 and backward Euler with order 1, to the same solution: halving dt divides
+    the error by about 4 and 2, and backward Euler extrapolated by Richardson
+    (2 v(h/2) - v(h)) meets the finest Crank-Nicolson run."""
+    fine = _passive_cell("crank_nicolson", 0.025 / 16)
+    for solver, lo, hi in (("bwd_euler", 1.7, 2.3), ("crank_nicolson", 3.3, 4.6)):
+        errs = [np.abs(_passive_cell(solver, dt) - fine).max() for dt in (0.1, 0.05, 0.025)]
+        for e0, e1 in zip(errs, errs[1:]):
+            assert lo < e0 / e1 < hi, (solver, errs)
+    h = 0.025 / 8
+    rich = 2 * _passive_cell("bwd_euler", h / 2) - _passive_cell("bwd_euler", h)
+    assert np.abs(rich - fine).max() < 1e-4
+
+
+def test_reference_solver_comparison_network():
+    """tests/test_solver.py:29-58 of the reference: two SimpleCell(3, 4) with HH, an
+    IonotropicSynapse from cell 0 (branch 0, comp 0) to cell 1 (branch 0, comp 0), gS
+    0.01, length 50 um, r_a 200; backward Euler and Crank-Nicolson stay within 6 mV."""
+    m = O.OracleModel([([-1, 0, 0], [4, 4, 4])] * 2)
+    m.hh_comps = np.arange(24)
+    m.length[:] = 50.0
+    m.axial_resistivity[:] = 200.0
+    p = dict(O.SYN_DEFAULTS["IonotropicSynapse"], gS=0.01)
+    # cell 1's first compartment is node 13 (12 compartments + 1 branchpoint in cell 0).
+    m.synapses = [{"type": "IonotropicSynapse", "pre": np.array([0]), "post": np.array([13]),
+                   "params": {k: np.array([v]) for k, v in p.items()}, "s0": np.array([0.2])}]
+    cur = O.step_current(0.5, 1.0, 0.1, 0.025, 5.0)
+    out = {s: O.simulate(m, None, 0.025, stim=cur[None], stim_comps=[0], rec_comps=[15], solver=s).numpy()
+           for s in ("bwd_euler", "crank_nicolson")}
+    d = np.abs(out["bwd_euler"] - out["crank_nicolson"]).max()
+    assert 0 < d < 6.0
