@@ -31,6 +31,12 @@ __device__ __forceinline__ float dexpm1(float x) {
 }
 __device__ __forceinline__ float ddiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 #endif
+// A float64 constant materialized in an SGPR pair at its use (volatile asm: not hoisted
+// into a loop-invariant VGPR, which would turn every Horner step into a copy and an fmac).
+__device__ __forceinline__ double kc(double c) {
+  asm volatile("" : "+s"(c));
+  return c;
+}
 // float64 reciprocal: the hardware estimate refined by two Newton steps (within 2 ulp;
 // the operands here are never 0, infinite or subnormal).
 __device__ __forceinline__ double drcp(double x) {
@@ -49,16 +55,16 @@ __device__ __forceinline__ double exp_f64(double x) {
   const double k = rint(x * 1.4426950408889634074);          // log2(e)
   double r = fma(-k, 6.93147180369123816490e-01, x);           // ln2, head
   r = fma(-k, 1.90821492927058770002e-10, r);                  // ln2, tail
-  double p = 2.08767569878680989792e-09;                       // 1/12!
-  p = fma(p, r, 2.50521083854417187751e-08);                   // 1/11!
-  p = fma(p, r, 2.75573192239858906526e-07);                   // 1/10!
-  p = fma(p, r, 2.75573192239858906526e-06);                   // 1/9!
-  p = fma(p, r, 2.48015873015873015873e-05);                   // 1/8!
-  p = fma(p, r, 1.98412698412698412698e-04);                   // 1/7!
-  p = fma(p, r, 1.38888888888888888889e-03);                   // 1/6!
-  p = fma(p, r, 8.33333333333333333333e-03);                   // 1/5!
-  p = fma(p, r, 4.16666666666666666667e-02);                   // 1/4!
-  p = fma(p, r, 1.66666666666666666667e-01);                   // 1/3!
+  double p = kc(2.08767569878680989792e-09);                   // 1/12!
+  p = fma(p, r, kc(2.50521083854417187751e-08));               // 1/11!
+  p = fma(p, r, kc(2.75573192239858906526e-07));               // 1/10!
+  p = fma(p, r, kc(2.75573192239858906526e-06));               // 1/9!
+  p = fma(p, r, kc(2.48015873015873015873e-05));               // 1/8!
+  p = fma(p, r, kc(1.98412698412698412698e-04));               // 1/7!
+  p = fma(p, r, kc(1.38888888888888888889e-03));               // 1/6!
+  p = fma(p, r, kc(8.33333333333333333333e-03));               // 1/5!
+  p = fma(p, r, kc(4.16666666666666666667e-02));               // 1/4!
+  p = fma(p, r, kc(1.66666666666666666667e-01));               // 1/3!
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
@@ -70,12 +76,12 @@ __device__ __forceinline__ double exp_f64(double x) {
 // (the next term is below 6e-20 at |z| = 1/4).
 __device__ __forceinline__ double z_over_expm1_series(double z) {
   const double y = z * z;
-  double p = -5.28419013868749318484e-10;                      // -691/1307674368000
-  p = fma(p, y, 2.08767569878680989792e-08);                   // 1/47900160
-  p = fma(p, y, -8.26719576719576719577e-07);                  // -1/1209600
-  p = fma(p, y, 3.30687830687830687831e-05);                   // 1/30240
-  p = fma(p, y, -1.38888888888888888889e-03);                  // -1/720
-  p = fma(p, y, 8.33333333333333333333e-02);                   // 1/12
+  double p = kc(-5.28419013868749318484e-10);                  // -691/1307674368000
+  p = fma(p, y, kc(2.08767569878680989792e-08));               // 1/47900160
+  p = fma(p, y, kc(-8.26719576719576719577e-07));              // -1/1209600
+  p = fma(p, y, kc(3.30687830687830687831e-05));               // 1/30240
+  p = fma(p, y, kc(-1.38888888888888888889e-03));              // -1/720
+  p = fma(p, y, kc(8.33333333333333333333e-02));               // 1/12
   p = fma(p, y, 1.0);
   return fma(-0.5, z, p);
 }

@@ -855,6 +855,19 @@ __device__ __forceinline__ double apply_update(const Fused<G, L>& a, const Topo&
   return ax;
 }
 
+// The kernel's by-value argument block (the first and only explicit argument, at offset
+// 0 of the kernarg segment) read through a pointer the compiler cannot prove constant:
+// inside a loop its fields are re-read with scalar loads (scalar cache hits) where they
+// are used instead of being held in SGPRs for the whole kernel, which with the float64
+// item code spills SGPRs into VGPR lanes (a v_readlane per use).
+template <typename T>
+__device__ __forceinline__ const T& kernarg() {
+  typedef const __attribute__((address_space(4))) T* KP;
+  KP p = (KP)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return *(const T*)p;
+}
+
 // Streaming forward: every step reads the state of the program (voltages, gates and the
 // axial part of each item's residual) from HBM and writes it back, for programs whose
 // items do not fit the registers of their threads. The axial part is computed from the
@@ -907,6 +920,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     if (tid < nS) {
       ItemState<G> cur = load_state(a, pg, tid);
       for (int i = tid; i < nS; i += nthr) {
+        const Fused<G, L>& a = kernarg<Fused<G, L>>();  // re-read per item (SGPR pressure)
         const int inx = i + nthr < nS ? i + nthr : i;
         const ItemConst<G, L> c = load_const(a, pg, tp, i);
         __builtin_amdgcn_sched_barrier(0);
