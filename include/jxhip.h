@@ -286,6 +286,11 @@ typedef struct jxhip_fused_args {
      those. */
   double dt_v;
   double rhs_scale;
+  /* 1 = forward Euler (step_voltage_explicit, solver_voltage.py:412-502): no tree solve;
+     a compartment advances by dt times the vector field and every branchpoint sits at
+     the conductance-weighted mean of its neighbours (set so at the start of the launch
+     and after every step). 0 = the implicit step above. Forward only. */
+  int32_t explicit_step;
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

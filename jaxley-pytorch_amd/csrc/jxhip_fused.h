@@ -71,6 +71,7 @@ struct Fused {
   G dt;
   G dtv;     // step of the voltage solve (dt, or dt / 2 for Crank-Nicolson)
   G rscale;  // right-hand-side factor (1, or 2 for Crank-Nicolson)
+  int32_t expl;  // forward Euler: no solve (branchpoint rows only)
   G diff;
   int32_t S;
   int32_t log2S;
@@ -120,6 +121,7 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.dt = static_cast<G>(a.dt);
   f.dtv = static_cast<G>(a.dt_v > 0 ? a.dt_v : a.dt);
   f.rscale = static_cast<G>(a.rhs_scale > 0 ? a.rhs_scale : 1.0);
+  f.expl = a.explicit_step;
   f.diff = static_cast<G>(a.diff);
   f.S = a.S;
   int l = 0;
@@ -548,6 +550,65 @@ __device__ __forceinline__ double axial(const Fused<G, L>& a, const Topo& tp, co
   return ax;
 }
 
+// Forward Euler (step_voltage_explicit, solver_voltage.py:412-502) replaces the tree
+// solve: a compartment's increment is its residual (dt times the vector field, with the
+// branchpoints at the conductance-weighted mean of their neighbours), and a branchpoint
+// solves its own row for the neighbours' increments, dv_k = (r_k - lo_k dv_p - sum_c
+// up_c dv_c) / d_k, which puts it at the weighted mean of their new voltages. A
+// branchpoint's neighbours are compartments, so the items do not depend on each other.
+template <int kCS = 0, typename G, typename L>
+__device__ void branchpoint_pass(const Fused<G, L>& a, const Topo& tp, const int32_t* ni,
+                                 const Lds<L>& Ld) {
+  const int lS = a.log2S, S = a.S;
+  const int nS = tp.n << lS;
+  const int32_t* cidx = a.child_idx + tp.cb;
+  for (int i = threadIdx.x; i < nS; i += blockDim.x) {
+    const int k = i >> lS, s = i & (S - 1);
+    const int32_t* nik = ni + k * JXHIP_NODE_ICOLS;
+    if (nik[kNiFlags] & kComp) continue;
+    double num = static_cast<double>(Ld.xs[i]);
+    const int p = nik[kNiParent];
+    if (p >= 0) num -= static_cast<double>(Ld.cl[k << (kCS + 1)]) * static_cast<double>(Ld.xs[(p << lS) + s]);
+    for (int j = nik[kNiChildBeg]; j < nik[kNiChildEnd]; ++j) {
+      const int c = cidx[j];
+      num -= static_cast<double>(Ld.cl[(c << (kCS + 1)) + 1]) * static_cast<double>(Ld.xs[(c << lS) + s]);
+    }
+    Ld.xs[i] = static_cast<L>(num / static_cast<double>(Ld.ds[i]));
+  }
+  __syncthreads();
+}
+
+// Forward Euler: every branchpoint voltage set to the conductance-weighted mean of its
+// neighbours (the value the reference computes from the compartments at every step),
+// before the axial residuals are formed at the start of a launch.
+template <int kCS = 0, typename G, typename L>
+__device__ void branchpoint_means(const Fused<G, L>& a, const Topo& tp, const int32_t* ni,
+                                  const L* cl, double* vg) {
+  const int lS = a.log2S, S = a.S;
+  const int nS = tp.n << lS;
+  const int32_t* cidx = a.child_idx + tp.cb;
+  for (int i = threadIdx.x; i < nS; i += blockDim.x) {
+    const int k = i >> lS, s = i & (S - 1);
+    const int32_t* nik = ni + k * JXHIP_NODE_ICOLS;
+    if (nik[kNiFlags] & kComp) continue;
+    double num = 0.0, den = 0.0;
+    const int p = nik[kNiParent];
+    if (p >= 0) {
+      const double w = static_cast<double>(cl[k << (kCS + 1)]);
+      num += w * vg[(p << lS) + s];
+      den += w;
+    }
+    for (int j = nik[kNiChildBeg]; j < nik[kNiChildEnd]; ++j) {
+      const int c = cidx[j];
+      const double w = static_cast<double>(cl[(c << (kCS + 1)) + 1]);
+      num += w * vg[(c << lS) + s];
+      den += w;
+    }
+    if (den != 0.0) vg[i] = num / den;
+  }
+  __syncthreads();
+}
+
 // System of one (node, simulation) item from its voltage v and the axial part ax of its
 // residual: diagonal d and residual r = b - A v into LDS. The gates are read from /
 // written to the references (registers or memory).
@@ -902,6 +963,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   // thread count is a multiple of S); no barrier is needed between the writes and reads
   // of an item's own state or LDS slots.
   const int s_own = tid & (S - 1);
+  if (a.expl) branchpoint_means<kCS>(a, tp, ni, Ld.cl, vg);
   for (int i = tid; i < nS; i += nthr) {
     const int k = i >> lS, s = i & (S - 1);
     axg[i] = static_cast<G>(axial<kCS>(a, tp, ni + k * JXHIP_NODE_ICOLS, k, Ld.cl, vg, s, vg[i]));
@@ -953,7 +1015,11 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
       }
     }
     __syncthreads();
-    tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+    if (a.expl) {
+      branchpoint_pass<kCS>(a, tp, ni, Ld);
+    } else {
+      tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+    }
     const bool last = step + 1 == nsteps;
     if (s_own < pg.nsim && tid < nS) {
       UpdConst c = load_update_const<kCS>(tp, ni, Ld.cl, tid >> lS);
@@ -1002,6 +1068,7 @@ __global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
   // all-float32 mode).
   double vr[IPT];
   G ar[IPT], mr[IPT], hr[IPT], nr[IPT];
+  if (a.expl) branchpoint_means(a, tp, ni, Ld.cl, vg);
 #pragma unroll 1
   for (int j = 0; j < IPT; ++j) {
     const int i = tid + j * nthr;
@@ -1052,7 +1119,11 @@ __global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
       }
     }
     __syncthreads();
-    tree_solve<G, L, false>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+    if (a.expl) {
+      branchpoint_pass(a, tp, ni, Ld);
+    } else {
+      tree_solve<G, L, false>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+    }
 #pragma unroll 1
     for (int j = 0; j < IPT; ++j) {
       const int i = tid + j * nthr;
@@ -1331,7 +1402,7 @@ inline int check_args(const jxhip_fused_args& a) {
   if (a.S < 1 || a.S > 64 || (a.S & (a.S - 1))) return hipErrorInvalidValue;
   if (a.lds_bytes < 0 || a.lds_bytes > 160 * 1024) return hipErrorInvalidValue;
   if (a.syn_model < -1 || a.syn_model > 2) return hipErrorInvalidValue;
-  if (a.dt_v < 0 || a.rhs_scale < 0) return hipErrorInvalidValue;
+  if (a.dt_v < 0 || a.rhs_scale < 0 || a.explicit_step < 0 || a.explicit_step > 1) return hipErrorInvalidValue;
   return hipSuccess;
 }
 
@@ -1415,7 +1486,8 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (a.syn_model >= 0 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
   if (a.threads > 512) return hipErrorInvalidValue;  // fused_adjoint_kernel's launch bound
   // The reverse pass differentiates the backward-Euler step only.
-  if ((a.dt_v > 0 && a.dt_v != a.dt) || (a.rhs_scale > 0 && a.rhs_scale != 1.0)) return hipErrorInvalidValue;
+  if ((a.dt_v > 0 && a.dt_v != a.dt) || (a.rhs_scale > 0 && a.rhs_scale != 1.0) || a.explicit_step)
+    return hipErrorInvalidValue;
   const Fused<G, L> f = unpack<G, L>(a);
   Adj<G> g;
   g.g_rec = static_cast<const G*>(adj->g_rec);

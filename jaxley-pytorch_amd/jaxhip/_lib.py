@@ -46,7 +46,7 @@ class FusedArgs(C.Structure):
         ("ckpt_v", _p), ("ckpt_m", _p), ("ckpt_h", _p), ("ckpt_n", _p), ("ckpt_s", _p),
         ("ckpt_every", _i64), ("ckpt_stride", _i64), ("ckpt_syn_stride", _i64),
         ("max_items", _i64), ("coef_global", _i32), ("work_ax", _p),
-        ("dt_v", C.c_double), ("rhs_scale", C.c_double),
+        ("dt_v", C.c_double), ("rhs_scale", C.c_double), ("explicit_step", _i32),
     ]
 
 

@@ -317,7 +317,7 @@ class ForwardLaunch:
                  stim_scale: Optional[torch.Tensor], T: int, dt: float, B: int, cfg: LaunchConfig,
                  syn_params: Optional[List[torch.Tensor]] = None, syn_scale: Optional[torch.Tensor] = None,
                  syn_s0: Optional[torch.Tensor] = None, ckpt_every: int = 0,
-                 dt_v: float = 0.0, rhs_scale: float = 0.0):
+                 dt_v: float = 0.0, rhs_scale: float = 0.0, explicit_step: int = 0):
         dev = v0.device
         dtype = gates0[0].dtype
         S = cfg.S
@@ -403,6 +403,8 @@ class ForwardLaunch:
         # doubled right-hand side; 0 = backward Euler.
         a.dt_v = dt_v
         a.rhs_scale = rhs_scale
+        # Forward Euler: no tree solve (branchpoint rows only).
+        a.explicit_step = explicit_step
         a.diff = SYN_DIFF
         a.S = S
         a.threads = cfg.threads

@@ -6,7 +6,8 @@ stimuli `(..., n_inputs, time)` padded / truncated to `int(t_max // delta_t + 1)
 leading batch dimensions of parameters, `data_set` values, stimuli and states broadcast
 against each other; recordings `(..., n_recordings, 1 + steps)` (the initial state and
 every step); `voltage_solver` in {"jaxley.dhs", "jaxley.dhs.gpu", "jaxley.dhs.cpu",
-"jaxley.stone"}; `solver` "bwd_euler" or "crank_nicolson"; gradients through autograd.
+"jaxley.stone"}; `solver` "bwd_euler", "crank_nicolson" or "fwd_euler"; gradients
+through autograd.
 
 Execution: the whole time loop is one fused HIP kernel launch (`jaxhip.fused`), with a
 hand-written adjoint (checkpoints every K steps, recomputation between them) for
@@ -28,9 +29,9 @@ from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, STATEFUL_SYN,
 from .geometry import axial_conductances, comp_geometry
 
 _SUPPORTED_SOLVERS = ("jaxley.dhs.gpu", "jaxley.dhs.cpu", "jaxley.stone")
-# Time steppers on the implicit solve (base.py:3327-3366): Crank-Nicolson solves with
-# dt / 2 and takes v' = 2 x - v.
-_SUPPORTED_STEPS = ("bwd_euler", "crank_nicolson")
+# Time steppers (base.py:3327-3366): backward Euler; Crank-Nicolson solves with dt / 2
+# and takes v' = 2 x - v; forward Euler (solver_voltage.py:412-502) needs no solve.
+_SUPPORTED_STEPS = ("bwd_euler", "crank_nicolson", "fwd_euler")
 
 
 def _resolve_voltage_solver(module, voltage_solver: str) -> str:
@@ -195,8 +196,9 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
         raise ValueError("No recordings are set. Please set them.")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype)
     if (module._edges or solver != "bwd_euler") and _needs_grad(inp):
-        # Gradients through synapses or through Crank-Nicolson steps: the differentiable
-        # per-step path (the fused adjoint covers backward-Euler cells without synapses).
+        # Gradients through synapses or through Crank-Nicolson / forward-Euler steps: the
+        # differentiable per-step path (the fused adjoint covers backward-Euler cells
+        # without synapses).
         from .stepwise import integrate_stepwise
 
         return integrate_stepwise(module, inp, delta_t, checkpoint_lengths, device, dtype, return_states,
@@ -398,7 +400,8 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
     P.syn_params, P.syn_scale, P.syn_s0 = syn_params, syn_scale, syn_s0
     P.B, P.batch, P.T, P.dt, P.dtype, P.user_dtype = B, batch, inp.T, dt, dtype, user_dtype
     # Launch arguments of the time stepper (0 = backward Euler).
-    P.step_kw = {"dt_v": dtv if cn else 0.0, "rhs_scale": 2.0 if cn else 0.0}
+    P.step_kw = {"dt_v": dtv if cn else 0.0, "rhs_scale": 2.0 if cn else 0.0,
+                 "explicit_step": int(solver == "fwd_euler")}
     P.needs_grad, P.device, P.stim_requires_grad = needs_grad, device, (
         inp.stim is not None and inp.stim.requires_grad)
     return P
@@ -571,7 +574,7 @@ def build_init_and_step_fn(module, voltage_solver: str = "jaxley.dhs", solver: s
       `(all_states, all_params)`: the states over nodes (`v`, HH gates with NaN where
       the channel is absent, synapse states) and the parameters to pass to `step_fn`.
     * `step_fn(all_states, all_params, externals, external_inds, delta_t=0.025)` runs one
-      backward-Euler (or Crank-Nicolson) step with the currents `externals["i"]` (`(..., n_inputs)`, nA) into
+      backward-Euler (Crank-Nicolson, forward-Euler) step with the currents `externals["i"]` (`(..., n_inputs)`, nA) into
       the nodes `external_inds["i"]` and returns the new states.
 
     The step is one launch of the fused kernel with T = 1; its buffers are kept between

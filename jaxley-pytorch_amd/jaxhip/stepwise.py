@@ -67,6 +67,33 @@ def _hh_gates(m, h, n, v, dt):
     return _gate(m, dt, am, bm), _gate(h, dt, ah, bh), _gate(n, dt, an, bn)
 
 
+def _explicit_edges(tables, device):
+    """Edge index sets of the forward-Euler vector field: all edges, and the edges into
+    branchpoints (types 3 and 4) with their sinks and sources."""
+    typ = np.asarray(tables["type"])
+    sinks = torch.as_tensor(np.asarray(tables["sink"], dtype=np.int64), device=device)
+    sources = torch.as_tensor(np.asarray(tables["source"], dtype=np.int64), device=device)
+    bp = torch.as_tensor(np.nonzero(np.isin(typ, [3, 4]))[0], device=device)
+    return sinks, sources, bp
+
+
+def _explicit_step(v, lin_t, cst_t, g, edges, dt):
+    """v + dt f(v) with f = -lin v + const plus the axial currents of every edge, the
+    branchpoints at the conductance-weighted mean of their neighbours
+    (`_voltage_vectorfield`, solver_voltage.py:438-502)."""
+    sinks, sources, bp = edges
+    f = -lin_t * v + cst_t
+    vv = v
+    if len(bp):
+        snk, src = sinks[bp], sources[bp]
+        num = vv.index_fill(-1, snk, 0.0).index_add(-1, snk, g[:, bp] * vv[:, src])
+        den = torch.ones_like(vv).index_fill(-1, snk, 0.0).index_add(-1, snk, g[:, bp])
+        vv = num / den
+    if len(sinks):
+        f = f.index_add(-1, sinks, (vv[:, sources] - vv[:, sinks]) * g)
+    return v + dt * f
+
+
 class _SynGroup:
     """One synapse type of the network in the regular layout of the synapse operator."""
 
@@ -111,10 +138,12 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
                        dtype, return_states: bool = False, solver: str = "bwd_euler"):
     """`integrate` through the differentiable per-step path (see the module docstring).
     Returns the recordings `(*batch, n_rec, 1 + T)`. `solver` "crank_nicolson" solves
-    with dt / 2 and takes 2 x - v (base.py:3327-3366); gates and synapses advance by dt."""
-    if solver not in ("bwd_euler", "crank_nicolson"):
+    with dt / 2 and takes 2 x - v (base.py:3327-3366); "fwd_euler" takes v + dt f(v)
+    (solver_voltage.py:412-502); gates and synapses advance by dt."""
+    if solver not in ("bwd_euler", "crank_nicolson", "fwd_euler"):
         raise NotImplementedError(f"solver={solver!r}")
     cn = solver == "crank_nicolson"
+    fe = solver == "fwd_euler"
     dtv = dt / 2 if cn else dt
     tables, ind = module._ensure_solver()
     n = module._n_nodes
@@ -202,6 +231,9 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     if len(sinks):
         diag0 = diag0.index_add(1, sinks, torch.broadcast_to(gdt, (B, gdt.shape[-1])))
 
+    g_full = torch.broadcast_to(flat(g_ax), (B, g_ax.shape[-1]))
+    edges = _explicit_edges(tables, device)
+
     def step(v, gates, syn_s, t):
         lin = torch.zeros(B, n, dtype=dtype, device=device)
         cst = torch.zeros(B, n, dtype=dtype, device=device)
@@ -232,6 +264,8 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
         if stim is not None:
             cur = stim[..., t]
             cst = cst.index_add(1, stim_nodes, torch.broadcast_to(cur / area[:, stim_nodes] * 1e5, (B, len(stim_nodes))))
+        if fe:
+            return _explicit_step(v, lin / cm, cst / cm, g_full, edges, dt), gates, new_s
         diag = diag0 + dtv * lin / cm
         rhs = torch.where(internal_mask, v + dtv * cst / cm, torch.zeros_like(v))
         dd = torch.cat([diag[:, mp], diag.new_ones(B, 1)], 1)

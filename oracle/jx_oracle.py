@@ -570,6 +570,31 @@ class OracleModel:
         return p
 
 
+def explicit_step(v, lin_t, const_t, g_ax, tables: Dict, dt: float):
+    """`step_voltage_explicit` / `_voltage_vectorfield` (solver_voltage.py:412-502):
+    v' = v + dt f(v), f = -lin v + const plus the axial currents of every compartment
+    edge, with each branchpoint at the conductance-weighted mean of its neighbours
+    (edges of types 3 and 4 point into branchpoints)."""
+    typ = np.asarray(tables["type"])
+    sinks = torch.as_tensor(tables["sink"])
+    sources = torch.as_tensor(tables["source"])
+    n = v.shape[-1]
+    batch = torch.broadcast_shapes(v.shape[:-1], lin_t.shape[:-1], const_t.shape[:-1], g_ax.shape[:-1])
+    vv = torch.broadcast_to(v, (*batch, n))
+    g = torch.broadcast_to(g_ax, (*batch, g_ax.shape[-1]))
+    f = torch.broadcast_to(-lin_t * v + const_t, (*batch, n))
+    bp = np.nonzero(np.isin(typ, [3, 4]))[0]
+    if len(bp):
+        ci = torch.as_tensor(bp)
+        snk, src = sinks[ci], sources[ci]
+        num = vv.index_fill(-1, snk, 0.0).index_add(-1, snk, g[..., ci] * vv[..., src])
+        den = torch.ones_like(vv).index_fill(-1, snk, 0.0).index_add(-1, snk, g[..., ci])
+        vv = num / den
+    if np.isin(typ, [0, 1, 2]).any():
+        f = f.index_add(-1, sinks, (vv[..., sources] - vv[..., sinks]) * g)
+    return v + dt * f
+
+
 def simulate(
     model: OracleModel,
     t_max: Optional[float],
@@ -585,15 +610,15 @@ def simulate(
 ):
     """`jx.integrate` (integrate.py:396-666) + `_step_with_plan` (modules/base.py:3214-3400)
     for HH / Leak channels and Kinetic / SigmoidRate / Ionotropic synapses, with
-    `solver` "bwd_euler" or "crank_nicolson" (modules/base.py:3327-3366: the implicit
+    `solver` "bwd_euler", "crank_nicolson" (modules/base.py:3327-3366: the implicit
     solve with dt / 2, then v' = 2 x - v at every node; channels and synapses still
-    advance by dt).
+    advance by dt) or "fwd_euler" (`step_voltage_explicit`, solver_voltage.py:412-502).
 
     `stim`: `(..., n_stim, time)` currents in nA (padded / truncated to
     `int(t_max // dt + 1)` steps as integrate.py:523-541 does). Recordings are
     `(..., n_rec, 1 + steps)`: the initial state and every step."""
     overrides = overrides or {}
-    if solver not in ("bwd_euler", "crank_nicolson"):
+    if solver not in ("bwd_euler", "crank_nicolson", "fwd_euler"):
         raise ValueError(f"solver {solver!r}")
     dtv = dt / 2 if solver == "crank_nicolson" else dt
     p = model.params(overrides)
@@ -713,7 +738,9 @@ def simulate(
             diag = diag.index_add(-1, sinks, torch.broadcast_to(gdt, (*bsh, gdt.shape[-1])))
         vals = v[..., internal] + dtv * const_t[..., internal]
         rhs = torch.zeros((*bsh, n), dtype=F64).index_copy(-1, internal, torch.broadcast_to(vals, (*bsh, len(internal))))
-        if len(sinks) == 0:
+        if solver == "fwd_euler":
+            x = explicit_step(v, lin_t, const_t, g_ax, model.tables, dt)
+        elif len(sinks) == 0:
             x = rhs / diag
         elif dense:
             A = torch.diag_embed(diag)

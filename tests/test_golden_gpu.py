@@ -87,3 +87,23 @@ def test_golden_network_ionotropic(device):
     v = jx.integrate(net, data_stimuli=ds, delta_t=0.025).double().cpu().numpy()
     assert v.shape == (2, 202)
     _check(v, "net_2x_cell_3_2")
+
+
+@pytest.mark.parametrize("solver,voltage_solver", [("fwd_euler", "jaxley.dhs"), ("bwd_euler", "jaxley.stone")])
+def test_golden_branch_uneven_radiuses(device, solver, voltage_solver):
+    """test_basic_modules.py:124-197: an 8-compartment branch, r_a 500, lengths
+    linspace(20, 300, 8), radii linspace(1, 5, 8), 2.0 nA from 0.5 to 1.5 ms into
+    loc(1.0), recorded at loc(0.0); every 50th step against the reference at 1e-5 mV."""
+    branch = jx.Branch(jx.Compartment(), 8)
+    branch.set("axial_resistivity", 500.0)
+    branch.set("length", np.linspace(20, 300, 8))
+    branch.set("radius", np.linspace(1, 5, 8))
+    branch.insert(jx.HH())
+    branch.loc(0.0).record()
+    cur = jx.step_current(0.5, 1.0, 2.0, 0.025, 10.0).to(device)
+    v = jx.integrate(branch, data_stimuli=branch.loc(1.0).data_stimulate(cur), delta_t=0.025,
+                     solver=solver, voltage_solver=voltage_solver).double().cpu().numpy()[0]
+    gold = np.asarray(GOLD[f"branch8_uneven_radiuses_{solver}"]["values"])
+    err = np.abs(v[::50] - gold).max()
+    print(f"branch8_uneven_radiuses {solver} max err vs reference golden {err:.3e}")
+    assert err <= 1e-5

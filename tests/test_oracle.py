@@ -156,17 +156,19 @@ def test_x_over_expm1_removable_singularity():
 # Crank-Nicolson (modules/base.py:3327-3366): the implicit solve with dt / 2, then
 # v' = 2 x - v.
 
-@pytest.mark.parametrize("solver", ["bwd_euler", "crank_nicolson"])
+@pytest.mark.parametrize("solver", ["bwd_euler", "crank_nicolson", "fwd_euler"])
 def test_time_stepper_closed_form(solver):
     """A passive compartment, dv/dt = -(v - e) / tau: backward Euler multiplies v - e by
-    1 / (1 + a) per step and Crank-Nicolson by (1 - a / 2) / (1 + a / 2), a = dt / tau."""
+    1 / (1 + a) per step, Crank-Nicolson by (1 - a / 2) / (1 + a / 2) and forward Euler
+    by 1 - a, a = dt / tau."""
     m = O.OracleModel([([-1], [1])])
     m.leak_comps = np.array([0])
     m.leak_params["Leak_gLeak"][:] = 1e-3      # tau = cm / (1000 gLeak) = 1 ms
     m.v0[:] = -50.0
     v = O.simulate(m, 10.0, 0.025, solver=solver)[0].numpy()
     a = 0.025 * 1000 * 1e-3
-    R = 1 / (1 + a) if solver == "bwd_euler" else (1 - a / 2) / (1 + a / 2)
+    R = {"bwd_euler": 1 / (1 + a), "crank_nicolson": (1 - a / 2) / (1 + a / 2),
+         "fwd_euler": 1 - a}[solver]
     exact = -70.0 + 20.0 * R ** np.arange(len(v))
     assert np.abs(v - exact).max() < 1e-10
 
@@ -185,12 +187,16 @@ def _passive_cell(solver, dt):
 
 def test_crank_nicolson_second_order_on_a_branched_cell():
     """On a passive branched cell (branchpoint included) Crank-Nicolson converges with
-    order 2 and backward Euler with order 1, to the same solution: halving dt divides
-    the error by about 4 and 2, and backward Euler extrapolated by Richardson
+    order 2, backward and forward Euler (branchpoints at the conductance-weighted mean of
+    their neighbours) with order 1, all to the same solution: halving dt divides the
+    error by about 4 and 2, and backward Euler extrapolated by Richardson
     (2 v(h/2) - v(h)) meets the finest Crank-Nicolson run."""
     fine = _passive_cell("crank_nicolson", 0.025 / 16)
-    for solver, lo, hi in (("bwd_euler", 1.7, 2.3), ("crank_nicolson", 3.3, 4.6)):
-        errs = [np.abs(_passive_cell(solver, dt) - fine).max() for dt in (0.1, 0.05, 0.025)]
+    # Forward Euler is stable on this cell for dt <= 0.05 and asymptotic below it.
+    for solver, lo, hi, dts in (("bwd_euler", 1.7, 2.3, (0.1, 0.05, 0.025)),
+                                ("crank_nicolson", 3.3, 4.6, (0.1, 0.05, 0.025)),
+                                ("fwd_euler", 1.8, 2.2, (0.025, 0.0125, 0.00625))):
+        errs = [np.abs(_passive_cell(solver, dt) - fine).max() for dt in dts]
         for e0, e1 in zip(errs, errs[1:]):
             assert lo < e0 / e1 < hi, (solver, errs)
     h = 0.025 / 8
@@ -215,3 +221,22 @@ def test_reference_solver_comparison_network():
            for s in ("bwd_euler", "crank_nicolson")}
     d = np.abs(out["bwd_euler"] - out["crank_nicolson"]).max()
     assert 0 < d < 6.0
+
+
+def _branch8_uneven(solver, voltage_solver="jaxley.dhs.gpu"):
+    m = O.OracleModel([([-1], [8])])
+    m.hh_comps = np.arange(8)
+    m.axial_resistivity[:] = 500.0
+    m.length = np.linspace(20, 300, 8)
+    m.radius = np.linspace(1, 5, 8)
+    cur = O.step_current(0.5, 1.0, 2.0, 0.025, 10.0)
+    return O.simulate(m, None, 0.025, stim=cur[None], stim_comps=[7], rec_comps=[0], solver=solver,
+                      voltage_solver=voltage_solver)[0]
+
+
+@pytest.mark.parametrize("solver,voltage_solver", [("fwd_euler", "jaxley.dhs"), ("bwd_euler", "jaxley.stone")])
+def test_golden_branch_uneven_radiuses(solver, voltage_solver):
+    """test_basic_modules.py:124-197: every 50th step, 1e-5 mV."""
+    v = _branch8_uneven(solver, voltage_solver)
+    gold = np.asarray(GOLD[f"branch8_uneven_radiuses_{solver}"]["values"])
+    assert np.abs(v[::50].numpy() - gold).max() <= 1e-5
