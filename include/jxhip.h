@@ -201,6 +201,7 @@ int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int3
 #define JXHIP_TOPO_COLS 16
 #define JXHIP_NODE_ICOLS 8
 #define JXHIP_NODE_FCOLS 4
+#define JXHIP_SYN_MIXED 4   /* syn_model: the model of every synapse in syn_kind */
 #define JXHIP_NPARAM 8      /* HH gNa gK gLeak eNa eK eLeak, Leak gLeak eLeak */
 #define JXHIP_NSYNPARAM 6
 
@@ -237,7 +238,8 @@ typedef struct jxhip_fused_args {
   /* recordings: per program [t][rec][S] at rec_off, t = 0 .. t_end - t_begin */
   void* rec;
   /* synapses, sorted by target: per target [syn_ptr[t], syn_ptr[t+1]) */
-  int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate, 2 ionotropic */
+  int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate, 2 ionotropic, 3 test
+                               synapse, JXHIP_SYN_MIXED: per synapse in syn_kind */
   const int32_t* syn_pre;   /* topology-relative presynaptic node, at syn_base */
   const int32_t* syn_ptr;   /* n_tgt + 1 per topology at syn_ptr_base */
   const int32_t* tgt_node;  /* topology-relative target node, at tgt_base */
@@ -291,6 +293,11 @@ typedef struct jxhip_fused_args {
      the conductance-weighted mean of its neighbours (set so at the start of the launch
      and after every step). 0 = the implicit step above. Forward only. */
   int32_t explicit_step;
+  /* With syn_model == JXHIP_SYN_MIXED (several synapse types in one network,
+     network.py:350-478 steps every type): the model (0-3) of every synapse at syn_base.
+     syn_param[j] then holds each synapse's j-th parameter in its own model's order, and
+     every synapse has a state slot in syn_s (unused by the sigmoid-rate model). */
+  const int32_t* syn_kind;
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

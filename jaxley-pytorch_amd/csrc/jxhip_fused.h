@@ -68,6 +68,7 @@ struct Fused {
   const G* syn_param[JXHIP_NSYNPARAM];
   const G* syn_scale;
   G* syn_s;
+  const int32_t* syn_kind;
   G dt;
   G dtv;     // step of the voltage solve (dt, or dt / 2 for Crank-Nicolson)
   G rscale;  // right-hand-side factor (1, or 2 for Crank-Nicolson)
@@ -118,6 +119,7 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   for (int j = 0; j < JXHIP_NSYNPARAM; ++j) f.syn_param[j] = static_cast<const G*>(a.syn_param[j]);
   f.syn_scale = static_cast<const G*>(a.syn_scale);
   f.syn_s = static_cast<G*>(a.syn_s);
+  f.syn_kind = a.syn_model == JXHIP_SYN_MIXED ? a.syn_kind : nullptr;
   f.dt = static_cast<G>(a.dt);
   f.dtv = static_cast<G>(a.dt_v > 0 ? a.dt_v : a.dt);
   f.rscale = static_cast<G>(a.rhs_scale > 0 ? a.rhs_scale : 1.0);
@@ -221,7 +223,8 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
     const G vpre = static_cast<G>(v[pre * S + s]);
     const G scale = a.syn_scale[gsyn];
     const G gs = a.syn_param[0][gsyn];
-    if (a.syn_model == 0) {
+    const int model = a.syn_kind != nullptr ? a.syn_kind[gsyn] : a.syn_model;
+    if (model == 0) {
       const int node = a.tgt_node[tp.tgb + tgt];
       const G e_syn = a.syn_param[1][gsyn];
       const G alpha = a.syn_param[2][gsyn];
@@ -240,7 +243,7 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
       const G g = gs * snew * scale;
       sl += g;
       sc += g * static_cast<G>(v[node * S + s] - static_cast<double>(e_syn));
-    } else if (a.syn_model == 2) {
+    } else if (model == 2) {
       // IonotropicSynapse (synapses/ionotropic.py:44-80): u = (v_pre - v_th) / delta,
       // s_inf = 1 / (1 + save_exp(-u)), rate = k_minus (1 + exp(clamp(u, -20, 50))),
       // s' = s e + s_inf (1 - e) with e = save_exp(-rate dt); i = gS s' (v_post - e_syn),
@@ -262,6 +265,21 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
       const G g = gs * snew * scale;
       sl += g;
       sc += g * static_cast<G>(v[node * S + s] - static_cast<double>(e_syn));
+    } else if (model == 3) {
+      // TestSynapse (synapses/test.py:25-59): s_bar = 1 / (1 + save_exp((v_th - v_pre) /
+      // delta)), tau = (1 - s_bar) / k_minus, c' = c e + s_bar (1 - e) with e =
+      // save_exp(-dt / tau); v_th = -35, delta = 10, k_minus = 1 / 40; i = gC c' v_post.
+      const int node = a.tgt_node[tp.tgb + tgt];
+      G* sp = a.syn_s + pg.synoff + static_cast<int64_t>(syn) * S + s;
+      const G st = *sp;
+      const G sbar = G(1) / (G(1) + save_exp((G(-35) - vpre) / G(10)));
+      const G tau = (G(1) - sbar) / G(1.0 / 40.0);
+      const G e = save_exp((G(-1) / tau) * dt);
+      const G snew = st * e + sbar * (G(1) - e);
+      *sp = snew;
+      const G g = gs * snew * scale;
+      sl += g;
+      sc += g * static_cast<G>(v[node * S + s]);
     } else {
       const G xo = a.syn_param[1][gsyn];
       const G slope = a.syn_param[2][gsyn];
@@ -868,7 +886,7 @@ __device__ __forceinline__ void record_step(const Fused<G, L>& a, const Prog& pg
 template <typename G, typename L>
 __device__ __forceinline__ void checkpoint_syn(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
                                                int64_t c) {
-  if ((a.syn_model == 0 || a.syn_model == 2) && a.ckpt_s != nullptr) {
+  if (a.syn_model != 1 && a.ckpt_s != nullptr) {
     const int64_t nsyn = static_cast<int64_t>(tp.n_syn) << a.log2S;
     const int64_t so = pg.synoff + c * a.ckpt_syn_stride;
     for (int64_t i = threadIdx.x; i < nsyn; i += blockDim.x) a.ckpt_s[so + i] = a.syn_s[pg.synoff + i];
@@ -1401,7 +1419,8 @@ inline int check_args(const jxhip_fused_args& a) {
     return hipErrorInvalidValue;
   if (a.S < 1 || a.S > 64 || (a.S & (a.S - 1))) return hipErrorInvalidValue;
   if (a.lds_bytes < 0 || a.lds_bytes > 160 * 1024) return hipErrorInvalidValue;
-  if (a.syn_model < -1 || a.syn_model > 2) return hipErrorInvalidValue;
+  if (a.syn_model < -1 || a.syn_model > JXHIP_SYN_MIXED) return hipErrorInvalidValue;
+  if (a.syn_model == JXHIP_SYN_MIXED && a.syn_kind == nullptr) return hipErrorInvalidValue;
   if (a.dt_v < 0 || a.rhs_scale < 0 || a.explicit_step < 0 || a.explicit_step > 1) return hipErrorInvalidValue;
   return hipSuccess;
 }

@@ -1,22 +1,23 @@
 """jaxhip: the per-time-step hot path of jaxley-pytorch's `jx.integrate` on MI355X.
 
 Drop-in surface (`import jaxhip as jx`): `Compartment`, `Branch`, `Cell`, `Network`,
-`connect`, `integrate`, `build_init_and_step_fn`, `step_current`, `datapoint_to_step_currents`; channels `HH`,
-`Leak`; synapses `KineticSynapse`, `SigmoidRateSynapse`, `IonotropicSynapse`; the GPU tree-solve operator
+`connect`, `fully_connect`, `integrate`, `build_init_and_step_fn`, `step_current`, `datapoint_to_step_currents`; channels `HH`,
+`Leak`; synapses `KineticSynapse`, `SigmoidRateSynapse`, `IonotropicSynapse`, `TestSynapse`; the GPU tree-solve operator
 `jaxhip.kernels.tree_solve.make_tree_solve_kernel`; the synapse operator
 `jaxhip.kernels.synapse_step`. Kernels are hand-written HIP for gfx950 in
 `libjxhip.so` (C ABI: include/jxhip.h).
 """
 from .channels import HH, Leak
 from .integrate import build_init_and_step_fn, integrate
-from .modules import Branch, Cell, Compartment, Network, connect
+from .modules import Branch, Cell, Compartment, Network, connect, fully_connect
 from .stimulus import datapoint_to_step_currents, step_current
-from .synapses import IonotropicSynapse, KineticSynapse, SigmoidRateSynapse
+from .synapses import IonotropicSynapse, KineticSynapse, SigmoidRateSynapse, TestSynapse
 
 __version__ = "0.1.0"
 
 __all__ = [
-    "Branch", "Cell", "Compartment", "Network", "connect", "integrate", "build_init_and_step_fn",
+    "Branch", "Cell", "Compartment", "Network", "connect", "fully_connect", "integrate", "build_init_and_step_fn",
     "step_current",
     "datapoint_to_step_currents", "HH", "Leak", "KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse",
+    "TestSynapse",
 ]

@@ -20,6 +20,7 @@ JXHIP_NODE_ICOLS = 8
 JXHIP_NODE_FCOLS = 4
 JXHIP_NPARAM = 8
 JXHIP_NSYNPARAM = 6
+JXHIP_SYN_MIXED = 4
 
 _p = C.c_void_p
 _i32 = C.c_int32
@@ -47,6 +48,7 @@ class FusedArgs(C.Structure):
         ("ckpt_every", _i64), ("ckpt_stride", _i64), ("ckpt_syn_stride", _i64),
         ("max_items", _i64), ("coef_global", _i32), ("work_ax", _p),
         ("dt_v", C.c_double), ("rhs_scale", C.c_double), ("explicit_step", _i32),
+        ("syn_kind", _p),
     ]
 
 

@@ -29,7 +29,6 @@ ALL_PARAMS = HH_PARAMS + LEAK_PARAMS
 # the batched table.
 BATCHABLE = {"HH_gNa": 0, "HH_gK": 1, "HH_gLeak": 2, "Leak_gLeak": 3}
 FLAG_COMP, FLAG_HH, FLAG_LEAK, FLAG_EXPORT = 1, 2, 4, 8
-STATEFUL_SYN = (0, 2)  # synapse models with a state s (Kinetic, Ionotropic)
 SYN_DIFF = 1e-3  # mV, the finite difference of the synapse slope (network.py:29)
 
 
@@ -138,6 +137,8 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
     if syn is not None:
         for key in ("pre", "ptr", "tgt"):
             syn[key] = t(syn[key])
+        if syn["kind"] is not None:
+            syn["kind"] = t(syn["kind"])
     return Structure(
         n=n, L=sg["L"], mp=mp, kidx_of_node=kidx_of_node, comp_of_k=comp_of_k, topo_row=topo_row,
         seg=t(sg["seg"]), seg_level_ptr=t(sg["seg_level_ptr"]), seg_child=t(sg["seg_child"]),
@@ -154,28 +155,46 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
 
 
 def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray) -> Optional[Dict]:
+    """All synapses of the network as one list sorted by target node (CSR; stable, so the
+    synapses of one target keep the order of the types in the network and of the edges
+    within a type). Several types give the per-synapse model ids of JXHIP_SYN_MIXED."""
     if not module._edges:
         return None
-    if len(module._edges) > 1:
-        raise NotImplementedError("the fused kernel supports one synapse type per network")
-    e = module._edges[0]
-    post = kidx_of_node[e["post"]]
-    pre = kidx_of_node[e["pre"]]
+    groups, pre_l, post_l, kind_l = [], [], [], []
+    off = 0
+    for e in module._edges:
+        m = len(e["pre"])
+        groups.append({"type": e["type"], "offset": off, "n": m})
+        pre_l.append(np.asarray(e["pre"], dtype=np.int64))
+        post_l.append(np.asarray(e["post"], dtype=np.int64))
+        kind_l.append(np.full(m, int(e["type"].kernel_model), dtype=np.int32))
+        off += m
+    post_nodes = np.concatenate(post_l)
+    post = kidx_of_node[post_nodes]
+    pre = kidx_of_node[np.concatenate(pre_l)]
+    kind = np.concatenate(kind_l)
     perm = np.argsort(post, kind="stable")          # synapses sorted by target (CSR)
     targets, counts = np.unique(post[perm], return_counts=True)
     ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     node_i[targets, 6] = np.arange(len(targets), dtype=np.int32)
+    mixed = len(groups) > 1
     return {
-        "model": int(e["type"].kernel_model),
-        "type": e["type"],
+        "model": _lib.JXHIP_SYN_MIXED if mixed else int(groups[0]["type"].kernel_model),
+        "groups": groups,
+        "kind": kind[perm] if mixed else None,
         "perm": perm,
         "pre": pre[perm].astype(np.int32),
         "ptr": ptr,
         "tgt": targets.astype(np.int32),
-        "post_node": e["post"][perm],
+        "post_node": post_nodes[perm],
         "n_tgt": len(targets),
         "n_syn": len(post),
     }
+
+
+def syn_has_state(syn: Dict) -> bool:
+    """Whether the launch keeps a state per synapse (every model but the sigmoid rate)."""
+    return syn["model"] != 1
 
 
 def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor, dt: float,
@@ -347,7 +366,7 @@ class ForwardLaunch:
         self.syn_params = syn_params
         self.syn_scale = syn_scale
         self.syn_s = None
-        if st.syn is not None and st.syn["model"] in STATEFUL_SYN:
+        if st.syn is not None and syn_has_state(st.syn):
             self.syn_s = to_prog(syn_s0, S)
         self.ckpt_every = ckpt_every
         self.n_ckpt = 0
@@ -395,6 +414,8 @@ class ForwardLaunch:
                 a.syn_param[j] = syn_params[min(j, len(syn_params) - 1)].data_ptr()
             a.syn_scale = syn_scale.data_ptr()
             a.syn_s = self.syn_s.data_ptr() if self.syn_s is not None else self._dummy.data_ptr()
+            if st.syn["kind"] is not None:
+                a.syn_kind = st.syn["kind"].data_ptr()
         else:
             a.syn_model = -1
         a.sims_total = B

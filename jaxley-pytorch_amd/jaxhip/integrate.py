@@ -24,8 +24,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, STATEFUL_SYN, ForwardLaunch,
-                    LaunchConfig, build_structure, default_config, node_float_table, to_prog)
+from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, ForwardLaunch, LaunchConfig,
+                    build_structure, default_config, node_float_table, syn_has_state, to_prog)
 from .geometry import axial_conductances, comp_geometry
 
 _SUPPORTED_SOLVERS = ("jaxley.dhs.gpu", "jaxley.dhs.cpu", "jaxley.stone")
@@ -181,14 +181,12 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     tables, _ = module._ensure_solver()
     if voltage_solver == "jaxley.stone" and np.any(np.isin(tables["type"], [1, 2, 3, 4])):
         raise NotImplementedError("The stone solver is not implemented for branched morphologies.")
+    # Device and dtype from the first floating point input, module stimuli included
+    # (integrate.py:480-490).
     ref = first_float_tensor(params, param_state, data_stimuli[1] if data_stimuli else None,
-                             all_states)
+                             all_states, module.externals)
     device = ref.device if ref is not None else torch.get_default_device()
-    if ref is None:
-        ext = first_float_tensor(module.externals)
-        dtype = ext.dtype if ext is not None else torch.get_default_dtype()
-    else:
-        dtype = ref.dtype
+    dtype = ref.dtype if ref is not None else torch.get_default_dtype()
     _require_gpu(device)
     if dtype not in (torch.float32, torch.float64):
         raise TypeError(f"dtype {dtype} is not supported (float32 / float64)")
@@ -377,23 +375,39 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
     # Synapses.
     syn_params = syn_scale = syn_s0 = None
     if st.syn is not None:
-        stype = st.syn["type"]
+        groups = st.syn["groups"]
         perm = torch.as_tensor(st.syn["perm"], device=device)
+        # Parameter slot j holds every synapse's j-th parameter of its own type (in the
+        # concatenated edge order of the types, then sorted by target).
+        n_slots = max(len(g["type"].param_names) for g in groups)
         syn_params = []
-        for name in stype.param_names:
-            x = inp.syn_vals[f"{stype.name}_{name}"].to(dtype)
-            if x.ndim > 1:
-                raise NotImplementedError("batched synapse parameters are not supported by the fused kernel")
-            syn_params.append(x[perm].contiguous())
+        for j in range(n_slots):
+            cols = []
+            for g in groups:
+                names = g["type"].param_names
+                if j < len(names):
+                    x = inp.syn_vals[f"{g['type'].name}_{names[j]}"].to(dtype)
+                    if x.ndim > 1:
+                        raise NotImplementedError("batched synapse parameters are not supported by the fused kernel")
+                    cols.append(x)
+                else:
+                    cols.append(torch.zeros(g["n"], dtype=dtype, device=device))
+            syn_params.append(torch.cat(cols)[perm].contiguous())
         area_node = torch.full((module._n_nodes,), 1.0, dtype=dtype, device=device)
         area_node[torch.as_tensor(module._comp_node, device=device)] = geo["area"]
         syn_scale = (1e5 / area_node[torch.as_tensor(st.syn["post_node"], device=device)]).contiguous()
-        if st.syn["model"] in STATEFUL_SYN:
-            s0 = inp.syn_vals[f"{stype.name}_s"].to(dtype)
-            if inp.all_states is not None and f"{stype.name}_s" in inp.all_states:
-                s0 = inp.all_states[f"{stype.name}_s"].to(dtype)
-            s0 = torch.broadcast_to(s0, (*batch, s0.shape[-1])).reshape(B, -1)
-            syn_s0 = s0[:, perm]
+        if syn_has_state(st.syn):
+            cols = []
+            for g in groups:
+                key = _syn_state_key(g)
+                if key is None:
+                    cols.append(torch.zeros((B, g["n"]), dtype=dtype, device=device))
+                    continue
+                s0 = inp.syn_vals[key].to(dtype)
+                if inp.all_states is not None and key in inp.all_states:
+                    s0 = inp.all_states[key].to(dtype)
+                cols.append(torch.broadcast_to(s0, (*batch, s0.shape[-1])).reshape(B, -1))
+            syn_s0 = torch.cat(cols, 1)[:, perm]
     P = Problem()
     P.module, P.st, P.node_f, P.node_p, P.bmode = module, st, node_f, node_p, bmode
     P.bparam_for, P.v0, P.gates0, P.stim, P.stim_scale = bparam_for, v0, gates0, stim, stim_scale
@@ -468,10 +482,29 @@ def _final_states(module, st, run, batch, B, dtype, device):
         x[:, mp] = xs
         out[name] = x.reshape(*batch, st.n)
     if st.syn is not None and run.syn_s is not None:
-        xs = from_prog(run.syn_s, B)
-        inv = torch.empty_like(torch.as_tensor(st.syn["perm"], device=device))
-        inv[torch.as_tensor(st.syn["perm"], device=device)] = torch.arange(len(inv), device=device)
-        out[f"{st.syn['type'].name}_s"] = xs[:, inv].reshape(*batch, -1)
+        out.update(_syn_states_out(st, from_prog(run.syn_s, B), batch, device))
+    return out
+
+
+def _syn_state_key(group) -> Optional[str]:
+    name = group["type"].state_name
+    return None if name is None else f"{group['type'].name}_{name}"
+
+
+def _syn_states_out(st, xs, batch, device, dtype=None) -> Dict[str, torch.Tensor]:
+    """Per-type synapse states in edge order from the kernel's target-sorted states
+    `xs` (B, n_syn)."""
+    perm = torch.as_tensor(st.syn["perm"], device=device)
+    inv = torch.empty_like(perm)
+    inv[perm] = torch.arange(len(inv), device=device)
+    xe = xs[:, inv]
+    if dtype is not None:
+        xe = xe.to(dtype)
+    out = {}
+    for g in st.syn["groups"]:
+        key = _syn_state_key(g)
+        if key is not None:
+            out[key] = xe[:, g["offset"]:g["offset"] + g["n"]].reshape(*batch, -1)
     return out
 
 
@@ -514,9 +547,6 @@ class _StepRunner:
         self.is_comp = comp_of_k >= 0
         if P.st.syn is not None:
             self.syn_perm = torch.as_tensor(P.st.syn["perm"], device=device)
-            inv = torch.empty_like(self.syn_perm)
-            inv[self.syn_perm] = torch.arange(len(inv), device=device)
-            self.syn_inv = inv
 
     def step(self, all_states, externals):
         from .fused import from_prog
@@ -532,10 +562,21 @@ class _StepRunner:
                 x = torch.nan_to_num(x, nan=0.0)
                 buf.copy_(to_prog(x, run.S))
         if run.syn_s is not None:
-            key = f"{P.st.syn['type'].name}_s"
-            if key in all_states:
-                s = torch.as_tensor(all_states[key], device=self.device).to(run.dtype)
-                s = torch.broadcast_to(s, (*P.batch, s.shape[-1])).reshape(B, -1)[:, self.syn_perm]
+            groups = P.st.syn["groups"]
+            keys = [_syn_state_key(g) for g in groups]
+            if any(k is not None and k in all_states for k in keys):
+                cols = []
+                prev = from_prog(run.syn_s, B)
+                inv = torch.empty_like(self.syn_perm)
+                inv[self.syn_perm] = torch.arange(len(inv), device=self.device)
+                prev = prev[:, inv]
+                for g, key in zip(groups, keys):
+                    if key is not None and key in all_states:
+                        x = torch.as_tensor(all_states[key], device=self.device).to(run.dtype)
+                        cols.append(torch.broadcast_to(x, (*P.batch, x.shape[-1])).reshape(B, -1))
+                    else:
+                        cols.append(prev[:, g["offset"]:g["offset"] + g["n"]])
+                s = torch.cat(cols, 1)[:, self.syn_perm]
                 run.syn_s.copy_(to_prog(s, run.S))
         if P.stim is not None:
             cur = torch.as_tensor(externals["i"], device=self.device).to(run.dtype)
@@ -562,8 +603,7 @@ class _StepRunner:
                     gn = torch.where(torch.isnan(prev), prev, gn)
                 out[name] = gn.reshape(*P.batch, self.n_nodes)
         if run.syn_s is not None:
-            s = from_prog(run.syn_s, B)[:, self.syn_inv].to(self.dtype)
-            out[f"{P.st.syn['type'].name}_s"] = s.reshape(*P.batch, -1)
+            out.update(_syn_states_out(P.st, from_prog(run.syn_s, B), P.batch, self.device, self.dtype))
         return out
 
 
@@ -606,8 +646,10 @@ def build_init_and_step_fn(module, voltage_solver: str = "jaxley.dhs", solver: s
             if k == "v":   # branchpoints start at their parent compartment's voltage
                 out = torch.nan_to_num(out, nan=float(x.reshape(-1)[0]))
             states[k] = out
+        state_keys = {f"{e['type'].name}_{e['type'].state_name}" for e in module._edges
+                      if e["type"].state_name is not None}
         for k, x in inp.syn_vals.items():
-            if k.endswith("_s"):
+            if k in state_keys:
                 states[k] = x
         if all_states is not None:
             states.update({k: torch.as_tensor(v, device=device) for k, v in all_states.items()})

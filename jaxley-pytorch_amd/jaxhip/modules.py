@@ -18,7 +18,7 @@ import torch
 
 from . import schedule as sch
 from .channels import HH, Channel, Leak
-from .synapses import IonotropicSynapse, KineticSynapse, SigmoidRateSynapse, Synapse
+from .synapses import IonotropicSynapse, KineticSynapse, SigmoidRateSynapse, Synapse, TestSynapse
 
 COMP_PARAMS = {"length": 10.0, "radius": 1.0, "axial_resistivity": 5000.0, "capacitance": 1.0}
 COMP_STATES = {"v": -70.0}
@@ -314,7 +314,7 @@ class Module(View):
         """Append synapses between node indices (`Network._append_multiple_synapses`,
         modules/network.py:736-790): one row per (pre, post) pair, parameters and states
         at the synapse type's defaults."""
-        if not isinstance(synapse_type, (KineticSynapse, SigmoidRateSynapse, IonotropicSynapse)):
+        if not isinstance(synapse_type, (KineticSynapse, SigmoidRateSynapse, IonotropicSynapse, TestSynapse)):
             raise NotImplementedError(f"synapse {type(synapse_type).__name__} is not on the jaxhip path")
         pre = np.asarray(pre_nodes, dtype=np.int64).ravel()
         post = np.asarray(post_nodes, dtype=np.int64).ravel()
@@ -401,13 +401,21 @@ class Module(View):
 
 
 class _EdgeView:
-    def __init__(self, base, edges):
+    """The synapses of one type (`net.IonotropicSynapse`), or a subset of them by their
+    index within the type (`.edge([0, 2])`, as `View.edge`, modules/base.py)."""
+
+    def __init__(self, base, edges, idx=None):
         self.base, self.e = base, edges
+        self.idx = np.arange(len(edges["pre"])) if idx is None else np.asarray(idx, dtype=np.int64)
+
+    def edge(self, idx) -> "_EdgeView":
+        sel = self.idx if isinstance(idx, str) and idx == "all" else self.idx[np.atleast_1d(idx)]
+        return _EdgeView(self.base, self.e, sel)
 
     def set(self, key, val):
         for d in (self.e["params"], self.e["states"]):
             if key in d:
-                d[key][:] = np.broadcast_to(np.asarray(val, dtype=float), d[key].shape)
+                d[key][self.idx] = np.broadcast_to(np.asarray(val, dtype=float), self.idx.shape)
                 self.base._version += 1
                 return
         raise KeyError(key)
@@ -487,6 +495,33 @@ def _inherit_from(self, parts):
 
 
 Module._inherit_from = _inherit_from
+
+
+def fully_connect(pre_cell_view: View, post_cell_view: View, synapse_type: Synapse,
+                  random_post_comp: bool = False):
+    """One synapse from branch 0, compartment 0 of every presynaptic cell to every
+    postsynaptic cell (connect.py:80-150): pre-major order, onto branch 0, compartment 0
+    of the postsynaptic cell, or with `random_post_comp` onto compartments drawn from
+    numpy's global random state exactly as the reference draws them (pandas'
+    groupby-sample: for each postsynaptic cell in ascending order, `num_pre` draws with
+    replacement among the cell's compartments in the view)."""
+    base = pre_cell_view.base
+    assert post_cell_view.base is base and isinstance(base, Network), "pre and post must be in the same network"
+    pre_cells = np.unique(base._comp_cell[pre_cell_view._comps])
+    post_cells = np.unique(base._comp_cell[post_cell_view._comps])
+    num_pre, num_post = len(pre_cells), len(post_cells)
+    pre_comps = np.array([base.cell(int(c)).branch(0).comp(0)._comps[0] for c in pre_cells], dtype=np.int64)
+    pre_comps = np.repeat(pre_comps, num_post)
+    if random_post_comp:
+        drawn = []
+        for c in post_cells:
+            comps = post_cell_view._comps[base._comp_cell[post_cell_view._comps] == c]
+            drawn.append(comps[np.random.choice(len(comps), num_pre, replace=True)])
+        post_comps = np.reshape(np.concatenate(drawn), (num_pre, num_post)).T.flatten()
+    else:
+        first = np.array([base.cell(int(c)).branch(0).comp(0)._comps[0] for c in post_cells], dtype=np.int64)
+        post_comps = first[np.tile(np.arange(num_post), num_pre)]
+    base.add_synapses(base._comp_node[pre_comps], base._comp_node[post_comps], synapse_type)
 
 
 def connect(pre: View, post: View, synapse_type: Synapse):

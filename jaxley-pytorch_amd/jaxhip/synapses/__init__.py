@@ -1,6 +1,7 @@
 """Synapses on the hot path: `KineticSynapse` (synapses/kinetic.py:12-63),
-`SigmoidRateSynapse` (synapses/sigmoid_rate.py:12-50) and `IonotropicSynapse`
-(synapses/ionotropic.py:13-80; fused forward only) of jaxley-pytorch.
+`SigmoidRateSynapse` (synapses/sigmoid_rate.py:12-50), `IonotropicSynapse`
+(synapses/ionotropic.py:13-80) and `TestSynapse` (synapses/test.py:13-59) of
+jaxley-pytorch; the last two in the fused forward only. A network may mix types.
 
 As in the reference, a synapse is a point process (conductances in uS, currents in nA);
 its arithmetic lives in the HIP kernels (csrc/jxhip_synapse.hip, jxhip_fused.hip).
@@ -11,6 +12,7 @@ from typing import Optional
 class Synapse:
     kernel_model = None
     param_names = ()
+    state_name: Optional[str] = None   # the synapse's state variable (without the prefix)
     synapse_params: dict = {}
     synapse_states: dict = {}
 
@@ -27,6 +29,7 @@ class KineticSynapse(Synapse):
 
     kernel_model = 0
     param_names = ("gS", "e_syn", "alpha", "beta", "v_th", "delta")
+    state_name = "s"
 
     def __init__(self, name: Optional[str] = None):
         super().__init__(name)
@@ -58,6 +61,7 @@ class IonotropicSynapse(Synapse):
 
     kernel_model = 2
     param_names = ("gS", "e_syn", "k_minus", "v_th", "delta")
+    state_name = "s"
 
     def __init__(self, name: Optional[str] = None):
         super().__init__(name)
@@ -69,4 +73,21 @@ class IonotropicSynapse(Synapse):
         self.synapse_states = {f"{p}_s": 0.2}
 
 
-__all__ = ["Synapse", "KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse"]
+class TestSynapse(Synapse):
+    """The reference's test synapse: c relaxes to s_bar = 1 / (1 + exp((v_th - v_pre) /
+    delta)) with time constant (1 - s_bar) / k_minus (v_th = -35 mV, delta = 10 mV,
+    k_minus = 1/40 per ms); current gC c v_post."""
+
+    __test__ = False   # not a test class for pytest
+    kernel_model = 3
+    param_names = ("gC",)
+    state_name = "c"
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        p = self._name
+        self.synapse_params = {f"{p}_gC": 1e-4}
+        self.synapse_states = {f"{p}_c": 0.2}
+
+
+__all__ = ["Synapse", "KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse", "TestSynapse"]

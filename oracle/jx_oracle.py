@@ -426,15 +426,34 @@ def ionotropic_current(s, pre_v, post_v, p):
     return p["gS"] * s * (post_v - p["e_syn"])
 
 
+def test_synapse_update(c, pre_v, p, dt):
+    """`TestSynapse.update_states` (synapses/test.py:25-48): v_th = -35, delta = 10,
+    k_minus = 1/40."""
+    s_bar = 1.0 / (1.0 + save_exp((-35.0 - pre_v) / 10.0))
+    tau_s = (1.0 - s_bar) / (1.0 / 40.0)
+    exp_term = save_exp((-1.0 / tau_s) * dt)
+    return c * exp_term + s_bar * (1.0 - exp_term)
+
+
+def test_synapse_current(c, pre_v, post_v, p):
+    """`TestSynapse.compute_current` (synapses/test.py:50-59), e_syn = 0."""
+    return p["gC"] * c * (post_v - 0.0)
+
+
+test_synapse_update.__test__ = False
+test_synapse_current.__test__ = False
+
 SYNAPSES = {
     "KineticSynapse": (kinetic_update, kinetic_current, ("gS", "e_syn", "alpha", "beta", "v_th", "delta"), 0.0),
     "SigmoidRateSynapse": (None, sigmoid_rate_current, ("gS", "x_offset", "slope"), None),
     "IonotropicSynapse": (ionotropic_update, ionotropic_current, ("gS", "e_syn", "k_minus", "v_th", "delta"), 0.2),
+    "TestSynapse": (test_synapse_update, test_synapse_current, ("gC",), 0.2),
 }
 SYN_DEFAULTS = {
     "KineticSynapse": {"gS": 1e-4, "e_syn": 0.0, "alpha": 1.0, "beta": 0.2, "v_th": -20.0, "delta": 2.0},
     "SigmoidRateSynapse": {"gS": 1e-4, "x_offset": -60.0, "slope": 0.25},
     "IonotropicSynapse": {"gS": 1e-4, "e_syn": 0.0, "k_minus": 0.025, "v_th": -35.0, "delta": 10.0},
+    "TestSynapse": {"gC": 1e-4},
 }
 HH_DEFAULTS = {"HH_gNa": 0.12, "HH_gK": 0.036, "HH_gLeak": 0.0003, "HH_eNa": 50.0, "HH_eK": -77.0, "HH_eLeak": -54.3}
 LEAK_DEFAULTS = {"Leak_gLeak": 1e-4, "Leak_eLeak": -70.0}
@@ -609,7 +628,7 @@ def simulate(
     solver: str = "bwd_euler",
 ):
     """`jx.integrate` (integrate.py:396-666) + `_step_with_plan` (modules/base.py:3214-3400)
-    for HH / Leak channels and Kinetic / SigmoidRate / Ionotropic synapses, with
+    for HH / Leak channels and Kinetic / SigmoidRate / Ionotropic / Test synapses, with
     `solver` "bwd_euler", "crank_nicolson" (modules/base.py:3327-3366: the implicit
     solve with dt / 2, then v' = 2 x - v at every node; channels and synapses still
     advance by dt) or "fwd_euler" (`step_voltage_explicit`, solver_voltage.py:412-502).

@@ -107,3 +107,35 @@ def test_golden_branch_uneven_radiuses(device, solver, voltage_solver):
     err = np.abs(v[::50] - gold).max()
     print(f"branch8_uneven_radiuses {solver} max err vs reference golden {err:.3e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("which", ["complex", "mixed"])
+def test_golden_net7(device, which):
+    """test_complex_net / test_mixed_net (test_basic_modules.py:412-574): seven cells
+    (SimpleCell(5, 4); or six single-compartment cells and one SimpleCell(5, 4)) with
+    HH, IonotropicSynapse and TestSynapse in one network (the fused kernel's per-synapse
+    models), fully_connect with random postsynaptic compartments after np.random.seed(0),
+    per-edge conductances, module stimuli into three (six) cells; every 40th step
+    against the reference at 1e-8 mV."""
+    morph = jx.Cell(jx.Branch(jx.Compartment(), 4), parents=[-1, 0, 0, 1, 1])
+    cells = [morph] * 7 if which == "complex" else [jx.Cell()] * 6 + [morph]
+    net = jx.Network(cells)
+    net.insert(jx.HH())
+    np.random.seed(0)
+    for pre, post in (([0, 1, 2], [3, 4, 5]), ([3, 4, 5], 6)):
+        jx.fully_connect(net.cell(pre), net.cell(post), jx.IonotropicSynapse(), random_post_comp=True)
+        jx.fully_connect(net.cell(pre), net.cell(post), jx.TestSynapse(), random_post_comp=True)
+    f = 100_000.0 / (2 * np.pi * 10.0)
+    net.set("IonotropicSynapse_gS", 0.44 / f)
+    net.set("TestSynapse_gC", 0.62 / f)
+    net.IonotropicSynapse.edge([0, 2, 4]).set("IonotropicSynapse_gS", 0.32 / f)
+    net.TestSynapse.edge([0, 3, 5]).set("TestSynapse_gC", 0.24 / f)
+    cur = jx.step_current(0.5, 0.5, 0.1, 0.025, 10.0).to(device)
+    for i in range(3 if which == "complex" else 6):
+        net.cell(i).branch(0).loc(0.0).stimulate(cur)
+    net.cell(6).branch(0).loc(0.0).record()
+    v = jx.integrate(net, delta_t=0.025).double().cpu().numpy()
+    gold = np.asarray(GOLD[f"net7_{which}"]["values"])
+    err = np.abs(v[..., ::40] - gold).max()
+    print(f"net7_{which} max err vs reference golden {err:.3e}")
+    assert err <= 1e-8, err

@@ -240,3 +240,54 @@ def test_golden_branch_uneven_radiuses(solver, voltage_solver):
     v = _branch8_uneven(solver, voltage_solver)
     gold = np.asarray(GOLD[f"branch8_uneven_radiuses_{solver}"]["values"])
     assert np.abs(v[::50].numpy() - gold).max() <= 1e-5
+
+
+def _net7_oracle(cells):
+    """test_complex_net / test_mixed_net (test_basic_modules.py:412-574): cells 0-2 ->
+    3-5 and 3-5 -> 6 with IonotropicSynapse and TestSynapse by fully_connect with random
+    postsynaptic compartments (np.random.seed(0); pandas' groupby-sample draws, per
+    postsynaptic cell in ascending order, three compartments with replacement)."""
+    m = O.OracleModel(cells)
+    nc = [int(np.sum(nb)) for _, nb in cells]
+    first = np.concatenate([[0], np.cumsum(nc)])
+    m.hh_comps = np.arange(first[-1])
+    np.random.seed(0)
+
+    def fully(pre_cells, post_cells):
+        drawn = [first[c] + np.random.choice(nc[c], len(pre_cells), replace=True) for c in post_cells]
+        post = np.reshape(np.concatenate(drawn), (len(pre_cells), len(post_cells))).T.flatten()
+        pre = np.repeat(first[pre_cells], len(post_cells))
+        return pre, post
+
+    groups = {"IonotropicSynapse": [], "TestSynapse": []}
+    for pre_cells, post_cells in (([0, 1, 2], [3, 4, 5]), ([3, 4, 5], [6])):
+        for name in ("IonotropicSynapse", "TestSynapse"):
+            groups[name].append(fully(np.array(pre_cells), np.array(post_cells)))
+    f = 100_000.0 / (2 * np.pi * 10.0)
+    vals = {"IonotropicSynapse": ("gS", 0.44 / f, [0, 2, 4], 0.32 / f),
+            "TestSynapse": ("gC", 0.62 / f, [0, 3, 5], 0.24 / f)}
+    m.synapses = []
+    for name, parts in groups.items():
+        pre = np.concatenate([p for p, _ in parts])
+        post = np.concatenate([q for _, q in parts])
+        key, base, edges, val = vals[name]
+        p = {k: np.full(len(pre), v) for k, v in O.SYN_DEFAULTS[name].items()}
+        p[key][:] = base
+        p[key][edges] = val
+        m.synapses.append({"type": name, "pre": m.internal[pre], "post": m.internal[post],
+                           "params": p, "s0": np.full(len(pre), 0.2)})
+    return m, first
+
+
+@pytest.mark.parametrize("which", ["complex", "mixed"])
+def test_golden_net7(which):
+    morph = ([-1, 0, 0, 1, 1], [4] * 5)
+    cells = [morph] * 7 if which == "complex" else [([-1], [1])] * 6 + [morph]
+    m, first = _net7_oracle(cells)
+    cur = O.step_current(0.5, 0.5, 0.1, 0.025, 10.0)
+    n_stim = 3 if which == "complex" else 6
+    v = O.simulate(m, None, 0.025, stim=torch.stack([cur] * n_stim), stim_comps=list(first[:n_stim]),
+                   rec_comps=[first[6]])
+    gold = np.asarray(GOLD[f"net7_{which}"]["values"])
+    err = np.abs(v[..., ::40].numpy() - gold).max()
+    assert err <= 1e-8, err
