@@ -406,6 +406,7 @@ class _EdgeView:
 
     def __init__(self, base, edges, idx=None):
         self.base, self.e = base, edges
+        self._per_edge = idx is not None
         self.idx = np.arange(len(edges["pre"])) if idx is None else np.asarray(idx, dtype=np.int64)
 
     def edge(self, idx) -> "_EdgeView":
@@ -419,6 +420,29 @@ class _EdgeView:
                 self.base._version += 1
                 return
         raise KeyError(key)
+
+    def make_trainable(self, key: str, init_val: Optional[Union[float, list]] = None, verbose: bool = True):
+        """One trainable parameter for all synapses of the type view, or one per synapse of
+        an `.edge(...)` view (modules/base.py:1625-1750); the initial value is the mean
+        of the current values of the synapses it controls."""
+        col = self.e["params"].get(key, self.e["states"].get(key))
+        if col is None:
+            raise KeyError(f"Key '{key}' not found in edges")
+        inds = self.idx[:, None] if self._per_edge else self.idx[None, :]
+        if init_val is None:
+            new = np.array([np.mean(col[r]) for r in inds])
+        elif isinstance(init_val, float):
+            new = np.full(len(inds), init_val)
+        else:
+            new = np.asarray(init_val, dtype=float)
+            assert len(new) == len(inds)
+        b = self.base
+        b.trainable_params.append({key: torch.as_tensor(new, dtype=torch.get_default_dtype())})
+        b.indices_set_by_trainables.append(inds)
+        b.num_trainable_params += len(inds)
+        if verbose:
+            print(f"Number of newly added trainable parameters: {len(inds)}. "
+                  f"Total number of trainable parameters: {b.num_trainable_params}")
 
 
 def _xoe(x, y):

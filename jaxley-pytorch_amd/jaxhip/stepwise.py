@@ -7,7 +7,9 @@ modules/base.py:3214-3400, inside the time loop of integrate.py:624-662), one st
 time with autograd, on the device:
 
 * synapses: the fused synapse-step operator (`kernels.synapse_step`, the HIP kernel
-  pair K3 / K4-K5 with its hand-written backward; network.py:350-478);
+  pair K3 / K4-K5 with its hand-written backward; network.py:350-478) for Kinetic and
+  SigmoidRate synapses in the regular layout; the reference's per-edge tensor operations
+  for IonotropicSynapse and TestSynapse (network.py:480-578);
 * HH / Leak: elementwise tensor operations, as the reference computes them
   (channels/hh.py:37-108, channels/pospischil.py:39-71, solver_gate.py:8-64; the reference
   has no kernel there);
@@ -133,6 +135,89 @@ class _SynGroup:
             self.s0 = torch.zeros(B, E, dtype=dtype, device=device)
         self.state_key = key if self.model == 0 else None
 
+    def apply(self, v, s, lin, cst, dt):
+        s2, _, sl, sc = synapse_step(self.plan, v, s, self.params, self.area, dt, DIFF)
+        return s2, lin.index_add(1, self.targets, sl), cst.index_add(1, self.targets, -sc)
+
+
+def _ionotropic_update(s, pre_v, p, dt):
+    """synapses/ionotropic.py:44-66."""
+    u = (pre_v - p["v_th"]) / p["delta"]
+    s_inf = 1.0 / (1.0 + _save_exp(-u))
+    rate = p["k_minus"] * (1.0 + torch.exp(torch.clamp(u, -20.0, 50.0)))
+    e = _save_exp(-rate * dt)
+    return s * e + s_inf * (1.0 - e)
+
+
+def _test_update(c, pre_v, p, dt):
+    """synapses/test.py:25-48 (v_th = -35, delta = 10, k_minus = 1/40)."""
+    s_bar = 1.0 / (1.0 + _save_exp((-35.0 - pre_v) / 10.0))
+    tau = (1.0 - s_bar) / (1.0 / 40.0)
+    e = _save_exp((-1.0 / tau) * dt)
+    return c * e + s_bar * (1.0 - e)
+
+
+# update(state, v_pre, params, dt) and current(state, v_pre, v_post, params) in nA of the
+# synapse types without an operator kernel (synapses/ionotropic.py, synapses/test.py).
+_TORCH_SYNAPSES = {
+    "IonotropicSynapse": (_ionotropic_update, lambda s, vp, vq, p: p["gS"] * s * (vq - p["e_syn"])),
+    "TestSynapse": (_test_update, lambda s, vp, vq, p: p["gC"] * s * vq),
+}
+
+
+class _TorchSynGroup:
+    """A synapse type without an operator kernel (IonotropicSynapse, TestSynapse), or in
+    a layout the operator does not take: the reference's per-edge step in tensor
+    operations (network.py:350-478, 480-578): state update, current at the unperturbed and
+    at the perturbed voltages (both ends moved by 1e-3 mV), slope and offset added to the
+    postsynaptic nodes."""
+
+    def __init__(self, edge: Dict, syn_vals: Dict[str, torch.Tensor], area_node: torch.Tensor,
+                 B: int, device, dtype):
+        stype = edge["type"]
+        self.name = stype.name
+        if self.name not in _TORCH_SYNAPSES:
+            raise NotImplementedError(f"{self.name}: no differentiable step on the jaxhip path")
+        self.update, self.current = _TORCH_SYNAPSES[self.name]
+        self.pre = torch.as_tensor(np.asarray(edge["pre"], dtype=np.int64), device=device)
+        self.post = torch.as_tensor(np.asarray(edge["post"], dtype=np.int64), device=device)
+        E = len(self.pre)
+        self.params = {}
+        for p in stype.param_names:
+            x = syn_vals[f"{self.name}_{p}"].to(dtype)
+            if x.ndim > 1:
+                raise NotImplementedError("batched synapse parameters are not supported")
+            self.params[p] = x
+        self.area = area_node[self.post]
+        self.state_key = f"{self.name}_{stype.state_name}"
+        s0 = syn_vals[self.state_key].to(dtype)
+        self.s0 = torch.broadcast_to(s0, (*s0.shape[:-1], E)).reshape(-1, E)
+        self.s0 = self.s0.expand(B, E) if self.s0.shape[0] == 1 else self.s0
+        self.inv = torch.arange(E, device=device)
+
+    def apply(self, v, s, lin, cst, dt):
+        pre_v, post_v = v[:, self.pre], v[:, self.post]
+        s2 = self.update(s, pre_v, self.params, dt)
+        d0 = self.current(s2, pre_v, post_v, self.params) / self.area * 1e5
+        d1 = self.current(s2, pre_v + DIFF, post_v + DIFF, self.params) / self.area * 1e5
+        vt = (d1 - d0) / DIFF
+        ct = d0 - vt * post_v
+        return s2, lin.index_add(1, self.post, vt), cst.index_add(1, self.post, -ct)
+
+
+def _syn_group(edge, syn_vals, area_node, n, B, device, dtype):
+    """The operator kernel for Kinetic / SigmoidRate synapses in the regular layout,
+    tensor operations otherwise."""
+    name = edge["type"].name
+    if name in MODELS:
+        post = np.asarray(edge["post"], dtype=np.int64)
+        perm = np.argsort(post, kind="stable")
+        if kernel_plan(name, np.asarray(edge["pre"])[perm], post[perm], n, device) is not None:
+            return _SynGroup(edge, syn_vals, area_node, n, B, device, dtype)
+        raise NotImplementedError("gradients through Kinetic / SigmoidRate synapses need every "
+                                  "target to hold the same number of synapses (regular layout)")
+    return _TorchSynGroup(edge, syn_vals, area_node, B, device, dtype)
+
 
 def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List[int]], device,
                        dtype, return_states: bool = False, solver: str = "bwd_euler"):
@@ -200,7 +285,7 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     if "HH" in chans:
         cidx = chans["HH"][2]
         gates = [torch.broadcast_to(flat(init[k])[:, cidx], (B, len(cidx))) for k in ("HH_m", "HH_h", "HH_n")]
-    groups = [_SynGroup(e, inp.syn_vals, area[0] if area.shape[0] == 1 else area[0], n, B, device, dtype)
+    groups = [_syn_group(e, inp.syn_vals, area[0], n, B, device, dtype)
               for e in module._edges]
     if area.shape[0] > 1 and groups:
         raise NotImplementedError("batched geometry with synapses is not supported")
@@ -257,10 +342,8 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
         cst = cst * 1000.0
         new_s = []
         for grp, s in zip(groups, syn_s):
-            s2, _, sl, sc = synapse_step(grp.plan, v, s, grp.params, grp.area, dt, DIFF)
+            s2, lin, cst = grp.apply(v, s, lin, cst, dt)
             new_s.append(s2)
-            lin = lin.index_add(1, grp.targets, sl)
-            cst = cst.index_add(1, grp.targets, -sc)
         if stim is not None:
             cur = stim[..., t]
             cst = cst.index_add(1, stim_nodes, torch.broadcast_to(cur / area[:, stim_nodes] * 1e5, (B, len(stim_nodes))))

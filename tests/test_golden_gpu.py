@@ -139,3 +139,45 @@ def test_golden_net7(device, which):
     err = np.abs(v[..., ::40] - gold).max()
     print(f"net7_{which} max err vs reference golden {err:.3e}")
     assert err <= 1e-8, err
+
+
+def test_golden_network_gradient(device):
+    """tests/jaxley_identical/test_grad.py:24-105 through the product path: the complex
+    network, `make_trainable` on the module, on cells, on a synapse type and on two
+    synapses, `get_parameters`, loss = sum of every 40th recorded voltage; the gradient
+    runs through the per-step path (the HIP tree-solve operator and its backward kernel,
+    synapse steps as tensor operations). Loss within 1e-8, gradients within 1e-3."""
+    morph = jx.Cell(jx.Branch(jx.Compartment(), 4), parents=[-1, 0, 0, 1, 1])
+    net = jx.Network([morph] * 7)
+    net.insert(jx.HH())
+    np.random.seed(0)
+    for pre, post in (([0, 1, 2], [3, 4, 5]), ([3, 4, 5], 6)):
+        jx.fully_connect(net.cell(pre), net.cell(post), jx.IonotropicSynapse(), random_post_comp=True)
+        jx.fully_connect(net.cell(pre), net.cell(post), jx.TestSynapse(), random_post_comp=True)
+    f = 100_000.0 / (2 * np.pi * 10.0)
+    net.set("IonotropicSynapse_gS", 0.44 / f)
+    net.set("TestSynapse_gC", 0.62 / f)
+    net.IonotropicSynapse.edge([0, 2, 4]).set("IonotropicSynapse_gS", 0.32 / f)
+    net.TestSynapse.edge([0, 3, 5]).set("TestSynapse_gC", 0.24 / f)
+    cur = jx.step_current(0.5, 0.5, 0.1, 0.025, 10.0).to(device)
+    for i in range(3):
+        net.cell(i).branch(0).loc(0.0).stimulate(cur)
+    net.cell(6).branch(0).loc(0.0).record()
+    net.make_trainable("HH_gNa")
+    net.cell([0, 1, 4]).make_trainable("HH_gK")
+    net.cell("all").make_trainable("HH_gLeak")
+    net.IonotropicSynapse.make_trainable("IonotropicSynapse_gS")
+    net.TestSynapse.edge([0, 2]).make_trainable("TestSynapse_gC")
+    params = [{k: v.to(device).requires_grad_()} for p in net.get_parameters() for k, v in p.items()]
+    loss = jx.integrate(net, params=params, voltage_solver="jaxley.dhs.cpu")[0, ::40].sum()
+    loss.backward()
+    print(f"network gradient golden: loss {loss.item():.10f}")
+    assert abs(loss.item() - (-592.26920963)) <= 1e-8
+    gold = [[-1420.82493379], [36.76796492, 14.50837022, 127.31361882],
+            [-1511.83061117, -657.06543355, -276.59064858, -348.48049676, -10613.6754718,
+             -1240.62600873, -268475.23225046], [-149134.15237682], [-75.34438431, -163.50398586]]
+    for p, g in zip(params, gold):
+        (key, val), = p.items()
+        err = np.abs(val.grad.cpu().numpy() - np.asarray(g)).max()
+        print(f"  {key}: max grad err {err:.3e}")
+        assert err <= 1e-3, key

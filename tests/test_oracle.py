@@ -291,3 +291,39 @@ def test_golden_net7(which):
     gold = np.asarray(GOLD[f"net7_{which}"]["values"])
     err = np.abs(v[..., ::40].numpy() - gold).max()
     assert err <= 1e-8, err
+
+
+def test_golden_network_gradient():
+    """tests/jaxley_identical/test_grad.py:24-105: the complex network (seven
+    SimpleCell(5, 4)), loss = sum of every 40th recorded voltage of cell 6, gradients
+    with respect to a shared gNa, gK of cells 0, 1, 4, gLeak per cell, one shared
+    IonotropicSynapse gS (initial value the mean over its synapses) and the TestSynapse gC
+    of edges 0 and 2; loss within 1e-8, gradients within 1e-3 (absolute)."""
+    m, first = _net7_oracle([([-1, 0, 0, 1, 1], [4] * 5)] * 7)
+    nc = 20
+    f = 100_000.0 / (2 * np.pi * 10.0)
+    g_na = torch.tensor([0.12], dtype=torch.float64, requires_grad=True)
+    g_k = torch.tensor([0.036] * 3, dtype=torch.float64, requires_grad=True)
+    g_leak = torch.tensor([0.0003] * 7, dtype=torch.float64, requires_grad=True)
+    iono, test = m.synapses
+    g_s = torch.tensor([float(np.mean(iono["params"]["gS"]))], dtype=torch.float64, requires_grad=True)
+    g_c = torch.tensor([0.24 / f, 0.62 / f], dtype=torch.float64, requires_grad=True)
+    gk_cell = torch.full((7,), 0.036, dtype=torch.float64).index_put((torch.tensor([0, 1, 4]),), g_k)
+    iono["params"] = dict(iono["params"], gS=g_s.expand(len(iono["pre"])))
+    gc = torch.as_tensor(test["params"]["gC"], dtype=torch.float64)
+    test["params"] = dict(test["params"], gC=gc.index_put((torch.tensor([0, 2]),), g_c))
+    over = {"HH_gNa": g_na.expand(7 * nc), "HH_gK": torch.repeat_interleave(gk_cell, nc),
+            "HH_gLeak": torch.repeat_interleave(g_leak, nc)}
+    cur = O.step_current(0.5, 0.5, 0.1, 0.025, 10.0)
+    v = O.simulate(m, None, 0.025, stim=torch.stack([cur] * 3), stim_comps=list(first[:3]),
+                   rec_comps=[first[6]], overrides=over, voltage_solver="jaxley.dhs.cpu")
+    loss = v[0, ::40].sum()
+    loss.backward()
+    assert abs(loss.item() - (-592.26920963)) <= 1e-8
+    gold = {"HH_gNa": [-1420.82493379], "HH_gK": [36.76796492, 14.50837022, 127.31361882],
+            "HH_gLeak": [-1511.83061117, -657.06543355, -276.59064858, -348.48049676,
+                         -10613.6754718, -1240.62600873, -268475.23225046],
+            "IonotropicSynapse_gS": [-149134.15237682], "TestSynapse_gC": [-75.34438431, -163.50398586]}
+    for key, p in (("HH_gNa", g_na), ("HH_gK", g_k), ("HH_gLeak", g_leak), ("IonotropicSynapse_gS", g_s),
+                   ("TestSynapse_gC", g_c)):
+        assert np.abs(p.grad.numpy() - np.asarray(gold[key])).max() <= 1e-3, (key, p.grad)
