@@ -181,3 +181,53 @@ def test_golden_network_gradient(device):
         err = np.abs(val.grad.cpu().numpy() - np.asarray(g)).max()
         print(f"  {key}: max grad err {err:.3e}")
         assert err <= 1e-3, key
+
+
+def _rl_geometry(seed, nb):
+    np.random.seed(seed)
+    r1 = 5 * np.random.rand(2 * nb)
+    r2 = np.random.rand(2 * nb)
+    return [np.flip(r1[2 * b:2 * b + 2]) for b in range(nb)], [np.flip(r2[2 * b:2 * b + 2]) for b in range(nb)]
+
+
+def test_golden_radius_and_length_branch_cell(device):
+    """test_radius_and_length.py:63-146: a 2-compartment branch and a SimpleCell(3, 2)
+    with random (flipped) lengths and radii per branch."""
+    branch = jx.Branch(jx.Compartment(), 2)
+    (l,), (r,) = _rl_geometry(1, 1)
+    branch.set("length", l)
+    branch.set("radius", r)
+    branch.insert(jx.HH())
+    _check(_run(branch, branch.loc(0.0), branch.loc(0.0), device)[0], "radius_and_length_branch")
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 2), parents=[-1, 0, 0])
+    ls, rs = _rl_geometry(1, 3)
+    for b in range(3):
+        cell.branch(b).set("length", ls[b])
+        cell.branch(b).set("radius", rs[b])
+    cell.insert(jx.HH())
+    _check(_run(cell, cell.branch(1).loc(0.0), cell.branch(1).loc(0.0), device)[0], "radius_and_length_cell")
+
+
+def test_golden_radius_and_length_net(device):
+    """test_radius_and_length.py:149-227: two SimpleCell(3, 2) with their own random
+    geometry, an IonotropicSynapse from cell 0 to cell 1 (branch 0, loc 0) with gS =
+    0.5 / (1e5 / area of the postsynaptic compartment)."""
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 2), parents=[-1, 0, 0])
+    net = jx.Network([cell, cell])
+    for c, seed in ((0, 1), (1, 2)):
+        ls, rs = _rl_geometry(seed, 3)
+        for b in range(3):
+            net.cell(c).branch(b).set("length", ls[b])
+            net.cell(c).branch(b).set("radius", rs[b])
+    jx.connect(net.cell(0).branch(0).loc(0.0), net.cell(1).branch(0).loc(0.0), jx.IonotropicSynapse())
+    net.insert(jx.HH())
+    ls, rs = _rl_geometry(2, 3)
+    area = 2 * np.pi * ls[0][0] * rs[0][0]
+    net.set("IonotropicSynapse_gS", 0.5 / (100_000.0 / area))
+    cur = jx.step_current(0.5, 1.0, 0.02, 0.025, 5.0).to(device)
+    ds = None
+    for c in (0, 1):
+        net.cell(c).branch(1).loc(0.0).record()
+        ds = net.cell(c).branch(1).loc(0.0).data_stimulate(cur, ds)
+    v = jx.integrate(net, data_stimuli=ds, delta_t=0.025).double().cpu().numpy()
+    _check(v, "radius_and_length_net")

@@ -327,3 +327,39 @@ def test_golden_network_gradient():
     for key, p in (("HH_gNa", g_na), ("HH_gK", g_k), ("HH_gLeak", g_leak), ("IonotropicSynapse_gS", g_s),
                    ("TestSynapse_gC", g_c)):
         assert np.abs(p.grad.numpy() - np.asarray(gold[key])).max() <= 1e-3, (key, p.grad)
+
+
+def _rl_geometry(seed, nb):
+    """np.random.seed(seed); 2 compartments per branch; per branch the flipped pair of
+    5 * rand lengths and rand radii (test_radius_and_length.py:63-227)."""
+    np.random.seed(seed)
+    r1 = 5 * np.random.rand(2 * nb)
+    r2 = np.random.rand(2 * nb)
+    length = np.concatenate([np.flip(r1[2 * b:2 * b + 2]) for b in range(nb)])
+    radius = np.concatenate([np.flip(r2[2 * b:2 * b + 2]) for b in range(nb)])
+    return length, radius
+
+
+def test_golden_radius_and_length_branch_cell():
+    for key, cells, rec in (("radius_and_length_branch", [([-1], [2])], 0),
+                            ("radius_and_length_cell", [([-1, 0, 0], [2, 2, 2])], 2)):
+        nb = len(cells[0][0])
+        m = O.OracleModel(cells)
+        m.hh_comps = np.arange(2 * nb)
+        m.length, m.radius = _rl_geometry(1, nb)
+        v = O.simulate(m, None, 0.025, stim=CUR[None], stim_comps=[rec], rec_comps=[rec])
+        _check(v, key)
+
+
+def test_golden_radius_and_length_net():
+    m = O.OracleModel([([-1, 0, 0], [2, 2, 2])] * 2)
+    m.hh_comps = np.arange(12)
+    l0, r0 = _rl_geometry(1, 3)
+    l1, r1 = _rl_geometry(2, 3)
+    m.length, m.radius = np.concatenate([l0, l1]), np.concatenate([r0, r1])
+    area_post = 2 * np.pi * m.length[6] * m.radius[6]
+    p = dict(O.SYN_DEFAULTS["IonotropicSynapse"], gS=0.5 / (100_000.0 / area_post))
+    m.synapses = [{"type": "IonotropicSynapse", "pre": m.internal[[0]], "post": m.internal[[6]],
+                   "params": {k: np.array([v]) for k, v in p.items()}, "s0": np.array([0.2])}]
+    v = O.simulate(m, None, 0.025, stim=torch.stack([CUR, CUR]), stim_comps=[2, 8], rec_comps=[2, 8])
+    _check(v, "radius_and_length_net")
