@@ -193,16 +193,30 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     if not module.recordings:
         raise ValueError("No recordings are set. Please set them.")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype)
-    if (module._edges or solver != "bwd_euler") and _needs_grad(inp):
-        # Gradients through synapses or through Crank-Nicolson / forward-Euler steps: the
-        # differentiable per-step path (the fused adjoint covers backward-Euler cells
-        # without synapses).
+    if ((module._edges or solver != "bwd_euler") and _needs_grad(inp)) or _has_tangents(inp):
+        # Gradients through synapses or through Crank-Nicolson / forward-Euler steps, and
+        # forward-mode tangents: the differentiable per-step path (the fused adjoint covers
+        # reverse mode for backward-Euler cells without synapses).
         from .stepwise import integrate_stepwise
 
         return integrate_stepwise(module, inp, delta_t, checkpoint_lengths, device, dtype, return_states,
                                   solver=solver)
     return _integrate_fused(module, inp, delta_t, checkpoint_lengths, return_states, device, dtype,
                             solver)
+
+
+def _has_tangents(inp) -> bool:
+    """Forward-mode differentiation (`torch.autograd.forward_ad` dual inputs): the
+    per-step path, whose tree solve has the forward rule of `_DHSSolveForwardAD`
+    (solver_voltage.py:190-224)."""
+    from .stepwise import has_tangents
+
+    ts = list(inp.vals.values()) + list(inp.syn_vals.values())
+    if inp.stim is not None:
+        ts.append(inp.stim)
+    if inp.all_states is not None:
+        ts += list(inp.all_states.values())
+    return has_tangents(ts)
 
 
 def _needs_grad(inp) -> bool:

@@ -154,9 +154,31 @@ class _TreeSolve(torch.autograd.Function):
     def forward(ctx, diags, lowers, uppers, solves, kernel):
         x = kernel.solve(diags, lowers, uppers, solves)
         ctx.save_for_backward(diags, lowers, uppers, x)
+        ctx.save_for_forward(diags, lowers, uppers, x)
         ctx.kernel = kernel
         ctx.shapes = [t.shape for t in (diags, lowers, uppers, solves)]
         return x
+
+    @staticmethod
+    def jvp(ctx, t_d, t_l, t_u, t_s, _):
+        """Forward mode (`_DHSSolveForwardAD`, solver_voltage.py:190-224): the tangent is
+        a solve with the same matrix, dx = A^{-1} (db - dA x), where row c holds lowers[c]
+        in its parent's column and the parent's row holds uppers[c] in column c."""
+        diags, lowers, uppers, x = ctx.saved_tensors
+        kernel = ctx.kernel
+        par = kernel.parent_glob.long()
+        has = par >= 0
+        ps = torch.where(has, par, torch.zeros_like(par))
+        rhs = t_s if t_s is not None else torch.zeros_like(x)
+        if t_d is not None:
+            rhs = rhs - t_d * x
+        if t_l is not None:
+            rhs = rhs - torch.where(has, t_l * x[..., ps], torch.zeros_like(x))
+        if t_u is not None:
+            ch = torch.nonzero(has).flatten()
+            rhs = torch.broadcast_to(rhs, torch.broadcast_shapes(rhs.shape, t_u.shape, x.shape))
+            rhs = rhs.index_add(-1, par[ch], -(t_u * x)[..., ch])
+        return kernel.solve(diags, lowers, uppers, rhs)
 
     @staticmethod
     def backward(ctx, g):

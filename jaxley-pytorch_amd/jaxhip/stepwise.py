@@ -18,7 +18,9 @@ time with autograd, on the device:
 
 The time loop is checkpointed every K steps (`checkpoint_lengths[-1]`, default 100):
 the backward recomputes each K-step chunk from its start, as the reference's
-`nested_checkpoint_scan` does (utils/torch_utils.py:141-198).
+`nested_checkpoint_scan` does (utils/torch_utils.py:141-198). Forward-mode
+differentiation (inputs with tangents of `torch.autograd.forward_ad`) runs the same steps
+without checkpoints; the tree-solve operator carries its forward rule.
 """
 from __future__ import annotations
 
@@ -26,6 +28,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 import torch
+import torch.autograd.forward_ad as fwAD
 from torch.utils.checkpoint import checkpoint
 
 from .geometry import axial_conductances, comp_geometry
@@ -159,7 +162,18 @@ def _test_update(c, pre_v, p, dt):
 
 # update(state, v_pre, params, dt) and current(state, v_pre, v_post, params) in nA of the
 # synapse types without an operator kernel (synapses/ionotropic.py, synapses/test.py).
+def _kinetic_update(s, pre_v, p, dt):
+    """synapses/kinetic.py:37-53."""
+    a = p["alpha"] * torch.sigmoid((pre_v - p["v_th"]) / p["delta"])
+    rate = a + p["beta"]
+    s_inf = a / rate
+    return s_inf + (s - s_inf) * torch.exp(-dt * rate)
+
+
 _TORCH_SYNAPSES = {
+    "KineticSynapse": (_kinetic_update, lambda s, vp, vq, p: p["gS"] * s * (vq - p["e_syn"])),
+    "SigmoidRateSynapse": (lambda s, vp, p, dt: s,
+                           lambda s, vp, vq, p: -p["gS"] * torch.sigmoid((vp - p["x_offset"]) * p["slope"])),
     "IonotropicSynapse": (_ionotropic_update, lambda s, vp, vq, p: p["gS"] * s * (vq - p["e_syn"])),
     "TestSynapse": (_test_update, lambda s, vp, vq, p: p["gC"] * s * vq),
 }
@@ -189,11 +203,15 @@ class _TorchSynGroup:
                 raise NotImplementedError("batched synapse parameters are not supported")
             self.params[p] = x
         self.area = area_node[self.post]
+        self.inv = torch.arange(E, device=device)
+        if stype.state_name is None:
+            self.state_key = None
+            self.s0 = torch.zeros(B, E, dtype=dtype, device=device)
+            return
         self.state_key = f"{self.name}_{stype.state_name}"
         s0 = syn_vals[self.state_key].to(dtype)
         self.s0 = torch.broadcast_to(s0, (*s0.shape[:-1], E)).reshape(-1, E)
         self.s0 = self.s0.expand(B, E) if self.s0.shape[0] == 1 else self.s0
-        self.inv = torch.arange(E, device=device)
 
     def apply(self, v, s, lin, cst, dt):
         pre_v, post_v = v[:, self.pre], v[:, self.post]
@@ -205,11 +223,12 @@ class _TorchSynGroup:
         return s2, lin.index_add(1, self.post, vt), cst.index_add(1, self.post, -ct)
 
 
-def _syn_group(edge, syn_vals, area_node, n, B, device, dtype):
+def _syn_group(edge, syn_vals, area_node, n, B, device, dtype, forward_ad=False):
     """The operator kernel for Kinetic / SigmoidRate synapses in the regular layout,
-    tensor operations otherwise."""
+    tensor operations otherwise (and under forward-mode differentiation, for which the
+    operator has no rule)."""
     name = edge["type"].name
-    if name in MODELS:
+    if name in MODELS and not forward_ad:
         post = np.asarray(edge["post"], dtype=np.int64)
         perm = np.argsort(post, kind="stable")
         if kernel_plan(name, np.asarray(edge["pre"])[perm], post[perm], n, device) is not None:
@@ -217,6 +236,11 @@ def _syn_group(edge, syn_vals, area_node, n, B, device, dtype):
         raise NotImplementedError("gradients through Kinetic / SigmoidRate synapses need every "
                                   "target to hold the same number of synapses (regular layout)")
     return _TorchSynGroup(edge, syn_vals, area_node, B, device, dtype)
+
+
+def has_tangents(tensors) -> bool:
+    """Whether any tensor carries a forward-mode tangent (torch.autograd.forward_ad)."""
+    return any(fwAD.unpack_dual(t).tangent is not None for t in tensors if isinstance(t, torch.Tensor))
 
 
 def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List[int]], device,
@@ -285,7 +309,10 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     if "HH" in chans:
         cidx = chans["HH"][2]
         gates = [torch.broadcast_to(flat(init[k])[:, cidx], (B, len(cidx))) for k in ("HH_m", "HH_h", "HH_n")]
-    groups = [_syn_group(e, inp.syn_vals, area[0], n, B, device, dtype)
+    forward_ad = has_tangents(list(inp.vals.values()) + list(inp.syn_vals.values()) +
+                              ([inp.stim] if inp.stim is not None else []) +
+                              (list(inp.all_states.values()) if inp.all_states is not None else []))
+    groups = [_syn_group(e, inp.syn_vals, area[0], n, B, device, dtype, forward_ad)
               for e in module._edges]
     if area.shape[0] > 1 and groups:
         raise NotImplementedError("batched geometry with synapses is not supported")
@@ -375,7 +402,7 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     state = (v, *gates, *syn_s)
     for t0 in range(0, T, K):
         t1 = min(t0 + K, T)
-        if torch.is_grad_enabled():
+        if torch.is_grad_enabled() and not forward_ad:
             out = checkpoint(chunk, t0, t1, *state, use_reentrant=False)
         else:
             out = chunk(t0, t1, *state)
