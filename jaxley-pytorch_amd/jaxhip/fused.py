@@ -72,6 +72,8 @@ class Structure:
     rec_comps: np.ndarray
     stim_order: np.ndarray     # stimulus entry e -> stimulus column
     sinks_k: np.ndarray        # sink of every compartment edge, kernel index
+    in_edges: np.ndarray       # (n, max in-degree): incoming edges of every kernel node in
+                               # edge order, -1 padded (fixed summation order)
     lo_edge: np.ndarray        # edge of the lower coefficient of every kernel node
     up_edge: np.ndarray
     parent_k: np.ndarray
@@ -147,6 +149,7 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         rec_node=t(rec_k if len(rec_k) else np.zeros(1, np.int64)),
         syn=syn, n_rec=len(rec_k), rec_comps=rec_comps, stim_order=stim_order,
         sinks_k=kidx_of_node[tables["sink"]],
+        in_edges=_in_edges(kidx_of_node[tables["sink"]], n),
         lo_edge=np.asarray(indexer["map_to_solve_order_lower"])[korder],
         up_edge=np.asarray(indexer["map_to_solve_order_upper"])[korder],
         parent_k=parent_k,
@@ -197,6 +200,18 @@ def syn_has_state(syn: Dict) -> bool:
     return syn["model"] != 1
 
 
+def _in_edges(sinks_k: np.ndarray, n: int) -> np.ndarray:
+    """Incoming edges of every kernel node, in edge order, padded with -1."""
+    sinks_k = np.asarray(sinks_k, dtype=np.int64)
+    deg = np.bincount(sinks_k, minlength=n) if len(sinks_k) else np.zeros(n, np.int64)
+    out = np.full((n, max(int(deg.max()) if n else 0, 1)), -1, dtype=np.int64)
+    fill = np.zeros(n, dtype=np.int64)
+    for e, k in enumerate(sinks_k):
+        out[k, fill[k]] = e
+        fill[k] += 1
+    return out
+
+
 def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor, dt: float,
                      dtype) -> torch.Tensor:
     """lo, up, dbase, inv_cm per kernel node from the (unbatched) axial conductances
@@ -214,7 +229,13 @@ def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor
     up = torch.where(has_par, up, torch.zeros_like(up))
     dbase = torch.as_tensor((st.comp_of_k >= 0).astype(np.float64), device=dev)
     if len(gdt):
-        dbase = dbase.index_add(0, torch.as_tensor(st.sinks_k, device=dev), gdt)
+        # 1 + sum of dt g over the node's incoming edges, added in edge order (as the
+        # reference's index_add on the CPU does; a scatter-add on the GPU would round in a
+        # different order on every call and make simulations differ in the last bits).
+        inc = torch.as_tensor(st.in_edges, device=dev)
+        for j in range(inc.shape[1]):
+            col = inc[:, j]
+            dbase = dbase + torch.where(col >= 0, gdt[torch.clamp(col, min=0)], torch.zeros_like(dbase))
     cm = cm_node.double()[torch.as_tensor(st.mp, device=dev)]
     inv_cm = torch.where(torch.as_tensor(st.comp_of_k >= 0, device=dev), 1.0 / cm, torch.zeros_like(cm))
     return torch.stack([lo, up, dbase, inv_cm], 1).to(dtype).contiguous()

@@ -228,3 +228,23 @@ def test_adjoint_vs_central_differences(device):
     rel = np.abs(g32 - grads).max() / np.abs(grads).max()
     print("fp32 vs fp64 gradient, max error relative to the largest gradient", rel)
     assert rel < 1e-2
+
+
+@pytest.mark.parametrize("shape", ["8,1024,gcoef", "2,256"])
+def test_repeated_launches_bitwise_identical(device, shape, monkeypatch):
+    """Deterministic like the reference's kernels (no atomics): the same simulation run
+    three times gives bitwise identical traces, for the streaming and the register
+    kernel (the node tables are summed in a fixed order on the host)."""
+    from jaxhip import autotune
+
+    B, T = 64, 600
+    cell, _ = build_tree(40, seed=3)
+    cell.branch(0).comp(0).record()
+    prm = c1_params(B, seed=3)
+    ps = data_set_params(cell, prm, device, torch.float64)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(c1_stimulus(T), dtype=torch.float64, device=device))
+    monkeypatch.setenv("JAXHIP_FUSED_CONFIG", shape)
+    autotune._CACHE.clear()
+    runs = [jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT) for _ in range(3)]
+    autotune._CACHE.clear()
+    assert torch.equal(runs[0], runs[1]) and torch.equal(runs[0], runs[2])
