@@ -276,6 +276,7 @@ def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
             nxt = arrivals[i] - (time.perf_counter() - t0) if i < n_req else 5e-4
             time.sleep(max(5e-5, min(5e-4, nxt)))
     makespan = time.perf_counter() - t0
+    srv.close()
     lat = np.array([h.latency for h in handles])
     comp_steps = sum(len(c._comp_node) * h.T for c, h in zip(cells, handles))
     return {"requests": n_req, "arrival_rate_per_s": rate if rate is not None else "all at t=0",

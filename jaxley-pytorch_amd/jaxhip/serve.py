@@ -113,26 +113,70 @@ class SimHandle:
         return None if self.t_done is None else self.t_done - self.t_submit
 
 
+class _Prepared:
+    """A request prepared on the host: schedule, node tables (channel parameters with the
+    request's values merged in), initial state and stimulus, as CPU tensors."""
+
+
+def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_states, dt: float,
+                     user_dtype, f32_mode: str, max_steps: int) -> _Prepared:
+    """`integrate._assemble` + `_prepare` for one served request (batch size 1)."""
+    integ = importlib.import_module("jaxhip.integrate")
+    integ._F32_MODE = f32_mode
+    cpu = torch.device("cpu")
+    inp = _assemble(module, params, param_state, data_stimuli, t_max, dt, all_states, cpu, user_dtype)
+    with torch.no_grad():
+        P = _prepare(module, inp, dt, cpu, user_dtype)
+    if P.B != 1:
+        raise ValueError("a served request is one simulation (batch size 1)")
+    if P.st.syn is not None:
+        raise NotImplementedError("the server runs cells without synapses")
+    if P.T + 1 > max_steps:
+        raise ValueError(f"{P.T} steps exceed the server's max_steps={max_steps}")
+    if P.stim is not None and P.stim.shape[2] != 1:
+        raise ValueError("stimulus with batch dimensions in a served request")
+    st = P.st
+    q = _Prepared()
+    q.n, q.L, q.T, q.n_rec, q.sched = st.n, st.L, P.T, st.n_rec, st.sched
+    q.node_i, q.seg, q.seg_level_ptr, q.seg_child = st.node_i, st.seg, st.seg_level_ptr, st.seg_child
+    q.child_idx, q.rec_node, q.stim_col = st.child_idx, st.rec_node, st.stim_col
+    q.node_f = P.node_f
+    node_p = P.node_p.clone()
+    if P.bmode:
+        bp = P.bparam_for(1)                       # (1, 4) or (1, n, 1, 4)
+        flags = st.node_i[:, 3]
+        for c, j in enumerate((0, 1, 2, 6)):
+            val = bp[0, c] if P.bmode == 1 else bp.reshape(st.n, 4)[:, c]
+            mask = (flags & (2 if j < 6 else 4)) != 0
+            node_p[:, j] = torch.where(mask, val.to(node_p.dtype) * torch.ones_like(node_p[:, j]), node_p[:, j])
+    q.node_p = node_p
+    q.v0 = P.v0[0].double()
+    q.gates0 = [g[0] for g in P.gates0]
+    q.stim = None if P.stim is None else P.stim[:, :, 0].contiguous()
+    q.stim_scale = P.stim_scale
+    return q
+
+
 class _Live:
     """An admitted simulation: its prepared problem and arena placements."""
 
-    def __init__(self, handle: SimHandle, P):
+    def __init__(self, handle: SimHandle, P: _Prepared):
         self.handle, self.P = handle, P
-        st = P.st
-        self.n, self.T, self.n_rec = st.n, P.T, st.n_rec
-        self.nseg = int(st.seg.shape[0])
-        self.ncs = int(st.seg_child.shape[0]) if self.nseg else 0
-        self.nchild = int(st.child_idx.shape[0])
+        self.n, self.T, self.n_rec = P.n, P.T, P.n_rec
+        self.nseg = int(P.seg.shape[0])
+        self.ncs = int(P.seg_child.shape[0]) if self.nseg else 0
+        self.nchild = int(P.child_idx.shape[0])
         self.n_stim = 0 if P.stim is None else int(P.stim_scale.shape[0])
-        self.aux_size = 4 * self.nseg + (st.L + 2) + self.ncs + self.nchild + max(self.n_rec, 1)
+        self.aux_size = 4 * self.nseg + (P.L + 2) + self.ncs + self.nchild + max(self.n_rec, 1)
 
 
 class SimServer:
     """See the module docstring. `dtype` float32 (mode of `integrate.set_float32_mode`)
     or float64; `delta_t` is shared by all requests; `max_steps` bounds a request's
     duration (the stimulus table holds `max_steps` rows). `prep_threads` > 0 prepares
-    requests in that many background threads (0: inside `submit`); with more than one,
-    requests must not share a module object (its schedule cache is built on first use)."""
+    requests in that many background threads (0: inside `submit`; `close()` ends them);
+    with more than one, requests must not share a module object (its schedule cache is
+    built on first use)."""
 
     def __init__(self, device=None, dtype=torch.float32, delta_t: float = 0.025,
                  hbm_budget: int = 2 << 30, max_active: int = 1024, chunk_steps: int = 200,
@@ -193,31 +237,19 @@ class SimServer:
         h = SimHandle(self._next_id, request)
         self._next_id += 1
         self.handles.append(h)
-        if self._pool is None:
-            self._prepare_request(h)
-            self.queue.append(h)
-        else:
-            self.preparing.append((h, self._pool.submit(self._prepare_request, h)))
-        return h
-
-    def _prepare_request(self, h: SimHandle) -> None:
         # Prepared on the host (no GPU work, so a submission never waits for a running
         # chunk); the tables go to the arenas at admission by asynchronous copies.
-        request = h.request
-        m = request.module
-        cpu = torch.device("cpu")
-        inp = _assemble(m, request.params, request.param_state, request.data_stimuli,
-                        request.t_max, self.dt, request.all_states, cpu, self.user_dtype)
-        with torch.no_grad():
-            P = _prepare(m, inp, self.dt, cpu, self.user_dtype)
-        if P.B != 1:
-            raise ValueError("a served request is one simulation (batch size 1)")
-        if P.st.syn is not None:
-            raise NotImplementedError("the server runs cells without synapses")
-        if P.T + 1 > self.max_steps:
-            raise ValueError(f"{P.T} steps exceed the server's max_steps={self.max_steps}")
-        if P.stim is not None and P.stim.shape[2] != 1:
-            raise ValueError("stimulus with batch dimensions in a served request")
+        r = h.request
+        args = (r.module, r.params, r.param_state, r.data_stimuli, r.t_max, r.all_states, self.dt,
+                self.user_dtype, importlib.import_module("jaxhip.integrate")._F32_MODE, self.max_steps)
+        if self._pool is None:
+            self._accept(h, _prepare_payload(*args))
+            self.queue.append(h)
+        else:
+            self.preparing.append((h, self._pool.submit(_prepare_payload, *args)))
+        return h
+
+    def _accept(self, h: SimHandle, P: _Prepared) -> None:
         L = _Live(h, P)
         if (L.n > self.node_cap or L.aux_size + 4 > self.aux.numel()
                 or (L.T + 1) * max(L.n_rec, 1) > self.rec_cap or L.n_stim > self.stim_col.numel()):
@@ -234,10 +266,16 @@ class SimServer:
                 if not wait:
                     return
                 fut.result()
-            fut.result()              # re-raises a preparation error
+            self._accept(h, fut.result())     # re-raises a preparation error
             self.preparing.pop(0)
             self.queue.append(h)
             wait = False
+
+    def close(self) -> None:
+        """Ends the preparation workers (threads or processes)."""
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
 
     def outstanding(self) -> int:
         """Requests not finished: being prepared, queued for admission or running."""
@@ -270,7 +308,7 @@ class SimServer:
         """Copies the request's tables and initial state into its arena places: pinned
         host buffers, asynchronous copies on the upload stream (the compute stream waits
         for it before the next chunk)."""
-        P, st = L.P, L.P.st
+        P = L.P
         n, no = L.n, L.node_off
         pins = []
 
@@ -281,38 +319,31 @@ class SimServer:
                 pins.append(src)
             dst.copy_(src, non_blocking=True)
 
-        ni = st.node_i.clone()
+        ni = P.node_i.clone()
         ni[:, 4:6] += L.stim_off
-        node_p = P.node_p.clone()
-        if P.bmode:
-            bp = P.bparam_for(1)                       # (1, 4) or (1, n, 1, 4)
-            flags = st.node_i[:, 3]
-            for c, j in enumerate((0, 1, 2, 6)):
-                val = bp[0, c] if P.bmode == 1 else bp.reshape(n, 4)[:, c]
-                mask = (flags & (2 if j < 6 else 4)) != 0
-                node_p[:, j] = torch.where(mask, val.to(node_p.dtype) * torch.ones_like(node_p[:, j]), node_p[:, j])
+        node_p = P.node_p
         a = L.aux_off
         sb = a // 4
-        parts = [st.seg.reshape(-1), st.seg_level_ptr[:st.L + 2], st.seg_child[:L.ncs], st.child_idx,
-                 st.rec_node[:max(L.n_rec, 1)]]
+        parts = [P.seg.reshape(-1), P.seg_level_ptr[:P.L + 2], P.seg_child[:L.ncs], P.child_idx,
+                 P.rec_node[:max(L.n_rec, 1)]]
         offs = np.cumsum([a] + [int(x.numel()) for x in parts])
         lb, scb, cb, rb = (int(x) for x in offs[1:5])
-        row = torch.tensor([n, st.L, no, lb, cb, L.n_rec, rb, 0, 0, 0, 0, 0, sb, scb, 0, 0],
+        row = torch.tensor([n, P.L, no, lb, cb, L.n_rec, rb, 0, 0, 0, 0, 0, sb, scb, 0, 0],
                            dtype=torch.int32)
         with torch.cuda.stream(self.up_stream):
             put(self.node_i[no:no + n], ni)
             put(self.node_f[no:no + n], P.node_f.to(self.ldtype))
             put(self.node_p[no:no + n], node_p.to(self.gdtype))
-            put(self.v[no:no + n], P.v0[0].double())
+            put(self.v[no:no + n], P.v0)
             for buf, g in zip((self.gm, self.gh, self.gn), P.gates0):
-                put(buf[no:no + n], g[0].to(self.gdtype))
+                put(buf[no:no + n], g.to(self.gdtype))
             put(self.aux[a:int(offs[-1])], torch.cat([x.to(torch.int32) for x in parts]))
             put(self.topos[L.slot], row)
             if L.n_stim:
                 e0 = L.stim_off
-                put(self.stim_col[e0:e0 + L.n_stim], st.stim_col[:L.n_stim] + e0)
+                put(self.stim_col[e0:e0 + L.n_stim], P.stim_col[:L.n_stim] + e0)
                 put(self.stim_scale[e0:e0 + L.n_stim], P.stim_scale.to(self.gdtype))
-                put(self.stim[:P.T, e0:e0 + P.stim.shape[1], 0], P.stim[:, :, 0].to(self.gdtype))
+                put(self.stim[:P.T, e0:e0 + P.stim.shape[1], 0], P.stim.to(self.gdtype))
         L.pins = pins
 
     def _retire(self, L: _Live) -> None:
@@ -380,7 +411,7 @@ class SimServer:
             if t1 >= L.T:
                 finishing.append(L)
         n_max = max(L.n for L in lives)
-        sched_max = max(L.P.st.sched for L in lives)
+        sched_max = max(L.P.sched for L in lives)
         cfg = self._cfg_cache.get(n_max.bit_length())
         if cfg is None:
             cfg = self._tune(n_max, sched_max, progs) if self.autotune else self._config(n_max, sched_max)

@@ -165,3 +165,4 @@ def test_serve_arena_allocator():
     for off, sz in held:
         b.free(off, sz)
     assert b.free_list == [(0, 1000)] and b.used == 0
+

@@ -70,3 +70,4 @@ def test_served_float32(device):
     _, r64 = _serve(specs, device, chunk_steps=64)
     for a, b in zip(r32, r64):
         assert float((a.double() - b).abs().max()) < 1e-2
+
