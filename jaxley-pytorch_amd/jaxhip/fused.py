@@ -205,10 +205,10 @@ def _in_edges(sinks_k: np.ndarray, n: int) -> np.ndarray:
     sinks_k = np.asarray(sinks_k, dtype=np.int64)
     deg = np.bincount(sinks_k, minlength=n) if len(sinks_k) else np.zeros(n, np.int64)
     out = np.full((n, max(int(deg.max()) if n else 0, 1)), -1, dtype=np.int64)
-    fill = np.zeros(n, dtype=np.int64)
-    for e, k in enumerate(sinks_k):
-        out[k, fill[k]] = e
-        fill[k] += 1
+    order = np.argsort(sinks_k, kind="stable")              # edges grouped by sink, in order
+    ks = sinks_k[order]
+    first = np.concatenate([[0], np.cumsum(deg)[:-1]])
+    out[ks, np.arange(len(order)) - first[ks]] = order
     return out
 
 
