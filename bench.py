@@ -253,9 +253,10 @@ def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
                 else np.cumsum(rng.exponential(1.0 / rate, size=n_req)))
     srv = SimServer(device, dtype=dtype, delta_t=DT, hbm_budget=4 << 30, chunk_steps=chunk,
                     max_steps=4001, max_stim_entries=1024, **server_kw)
-    # Warm the launch path (kernel load, first shapes) on one request.
+    # Warm the launch path (kernel load, first shapes) and every preparation worker.
     cpu = torch.device("cpu")          # request tensors live on the host until admission
-    srv.submit(c2_request(specs[0], cpu, dtype, cells[0]))
+    for _ in range(max(1, server_kw.get("prep_processes", 0))):
+        srv.submit(c2_request(specs[0], cpu, dtype, cells[0]))
     srv.run_until_idle()
     srv.handles.clear()
     handles = []
@@ -284,7 +285,8 @@ def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
             "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
             "makespan_s": makespan, "comp_steps_per_s": comp_steps / makespan,
-            "launches": srv.launches, "dtype": str(dtype).replace("torch.", "")}
+            "launches": srv.launches, "dtype": str(dtype).replace("torch.", ""),
+            "prep_processes": server_kw.get("prep_processes", 0)}
 
 
 def main():
@@ -298,6 +300,8 @@ def main():
     ap.add_argument("--cpu-sims", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--prep-processes", type=int, default=12,
+                    help="request preparation workers of the served runs (0: one thread)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -442,7 +446,9 @@ def main():
         # ---- served: continuous batching of uneven requests ----
         if args.serve_requests > 0:
             for key, rate in (("served", args.serve_rate), ("served_burst", None)):
-                sv = bench_served(device, args.serve_requests, rate, torch.float64, 200)
+                ncpu_s = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+                sv = bench_served(device, args.serve_requests, rate, torch.float64, 200,
+                                  prep_processes=max(0, min(args.prep_processes, ncpu_s - 2)))
                 out[key] = sv
                 log(f"[bench] {key} {sv['requests']} requests at {sv['arrival_rate_per_s']}/s: p50 "
                     f"{sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms, "

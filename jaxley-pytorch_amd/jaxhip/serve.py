@@ -6,7 +6,8 @@ Simulations of different morphologies, durations and parameters arrive as reques
 
 * prepares each request on arrival, on the host (schedule and node tables,
   `integrate._prepare`; request tensors are best given on the CPU), in a background
-  thread (`prep_threads`), so that admissions and launches go on meanwhile,
+  thread (`prep_threads`) or in worker processes (`prep_processes`), so that admissions
+  and launches go on meanwhile,
 * admits it when a slot and room in its HBM arenas are free (`hbm_budget` bytes for
   node tables, states, schedules, stimuli and recordings; first fit),
 * keeps the state of every admitted simulation resident in HBM between launches,
@@ -29,6 +30,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import importlib
+import multiprocessing
 import time
 from dataclasses import dataclass, field
 from typing import Any, Dict, List, Optional
@@ -157,6 +159,36 @@ def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_state
     return q
 
 
+def _to_numpy(x):
+    """Tensors in nested lists / tuples / dicts as numpy arrays (they cross a process
+    boundary as plain bytes; torch tensors would go through shared-memory handles)."""
+    if isinstance(x, torch.Tensor):
+        return ("__tensor__", x.detach().cpu().numpy())
+    if isinstance(x, dict):
+        return {k: _to_numpy(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return type(x)(_to_numpy(v) for v in x)
+    return x
+
+
+def _to_torch(x):
+    if isinstance(x, tuple) and len(x) == 2 and isinstance(x[0], str) and x[0] == "__tensor__":
+        return torch.from_numpy(x[1])
+    if isinstance(x, dict):
+        return {k: _to_torch(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return type(x)(_to_torch(v) for v in x)
+    return x
+
+
+def _prepare_in_worker(args):
+    """Worker-process side of `_prepare_payload`: numpy in, numpy out."""
+    torch.set_num_threads(1)
+    q = _prepare_payload(*_to_torch(args))
+    q.__dict__ = _to_numpy(q.__dict__)
+    return q
+
+
 class _Live:
     """An admitted simulation: its prepared problem and arena placements."""
 
@@ -173,15 +205,19 @@ class _Live:
 class SimServer:
     """See the module docstring. `dtype` float32 (mode of `integrate.set_float32_mode`)
     or float64; `delta_t` is shared by all requests; `max_steps` bounds a request's
-    duration (the stimulus table holds `max_steps` rows). `prep_threads` > 0 prepares
-    requests in that many background threads (0: inside `submit`; `close()` ends them);
-    with more than one, requests must not share a module object (its schedule cache is
-    built on first use)."""
+    duration (the stimulus table holds `max_steps` rows). `prep_processes` > 0 prepares
+    requests in that many worker processes (preparation is Python-bound, so threads do
+    not add throughput; tensors cross the process boundary as numpy arrays; the workers
+    are started with "spawn", so a script that creates the server does its work under
+    `if __name__ == "__main__":`); otherwise `prep_threads` > 0 prepares requests in that
+    many background threads (0: inside `submit`); with more than one thread, requests
+    must not share a module object (its schedule cache is built on first use). `close()`
+    ends the workers."""
 
     def __init__(self, device=None, dtype=torch.float32, delta_t: float = 0.025,
                  hbm_budget: int = 2 << 30, max_active: int = 1024, chunk_steps: int = 200,
                  max_steps: int = 4001, max_stim_entries: int = 4096, autotune: bool = False,
-                 thread_order=(256, 512, 1024), prep_threads: int = 1):
+                 thread_order=(256, 512, 1024), prep_threads: int = 1, prep_processes: int = 0):
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         _require_gpu(device)
         self.device, self.user_dtype, self.dt = device, dtype, float(delta_t)
@@ -220,7 +256,14 @@ class SimServer:
         self.slots = list(range(max_active))[::-1]
         self.queue: List[SimHandle] = []
         self.preparing: List = []     # (handle, future) in submission order
-        self._pool = cf.ThreadPoolExecutor(max_workers=prep_threads) if prep_threads > 0 else None
+        self._procs = prep_processes > 0
+        if self._procs:
+            # Worker processes started fresh ("spawn": nothing of this process's GPU
+            # context is inherited); they import jaxhip and never touch the GPU.
+            self._pool = cf.ProcessPoolExecutor(max_workers=prep_processes,
+                                                mp_context=multiprocessing.get_context("spawn"))
+        else:
+            self._pool = cf.ThreadPoolExecutor(max_workers=prep_threads) if prep_threads > 0 else None
         self.live: Dict[int, _Live] = {}
         self.inflight: List = []      # (event, [live finishing], pinned progs)
         self.handles: List[SimHandle] = []
@@ -245,11 +288,15 @@ class SimServer:
         if self._pool is None:
             self._accept(h, _prepare_payload(*args))
             self.queue.append(h)
+        elif self._procs:
+            self.preparing.append((h, self._pool.submit(_prepare_in_worker, _to_numpy(args))))
         else:
             self.preparing.append((h, self._pool.submit(_prepare_payload, *args)))
         return h
 
     def _accept(self, h: SimHandle, P: _Prepared) -> None:
+        if self._procs:
+            P.__dict__ = _to_torch(P.__dict__)
         L = _Live(h, P)
         if (L.n > self.node_cap or L.aux_size + 4 > self.aux.numel()
                 or (L.T + 1) * max(L.n_rec, 1) > self.rec_cap or L.n_stim > self.stim_col.numel()):

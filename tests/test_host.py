@@ -166,3 +166,30 @@ def test_serve_arena_allocator():
         b.free(off, sz)
     assert b.free_list == [(0, 1000)] and b.used == 0
 
+
+
+def test_serve_preparation_in_worker_processes():
+    """A served request prepared in a spawned worker process (the server's
+    `prep_processes`: numpy arrays across the boundary) equals the same preparation in
+    this process, tensor by tensor."""
+    import concurrent.futures as cf
+    import multiprocessing
+
+    from jaxhip.serve import _prepare_in_worker, _prepare_payload, _to_numpy, _to_torch
+    from models import c2_request, c2_specs
+
+    cpu = torch.device("cpu")
+    reqs = [c2_request(sp, cpu, torch.float64) for sp in c2_specs(3, seed=7)]
+    args = [(r.module, r.params, r.param_state, r.data_stimuli, r.t_max, r.all_states, 0.025,
+             torch.float64, "mixed", 4001) for r in reqs]
+    local = [_prepare_payload(*a) for a in args]
+    with cf.ProcessPoolExecutor(max_workers=2, mp_context=multiprocessing.get_context("spawn")) as pool:
+        remote = list(pool.map(_prepare_in_worker, [_to_numpy(a) for a in args]))
+    for p, q in zip(local, remote):
+        q.__dict__ = _to_torch(q.__dict__)
+        assert (p.n, p.L, p.T, p.n_rec, p.sched) == (q.n, q.L, q.T, q.n_rec, q.sched)
+        for name in ("node_i", "seg", "seg_level_ptr", "seg_child", "child_idx", "rec_node", "stim_col",
+                     "node_f", "node_p", "v0", "stim", "stim_scale"):
+            assert torch.equal(getattr(p, name), getattr(q, name)), name
+        for g, h in zip(p.gates0, q.gates0):
+            assert torch.equal(g, h)

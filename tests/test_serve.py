@@ -71,3 +71,21 @@ def test_served_float32(device):
     for a, b in zip(r32, r64):
         assert float((a.double() - b).abs().max()) < 1e-2
 
+
+
+def test_served_with_preparation_processes(device):
+    """Requests prepared in worker processes (`prep_processes`) give the same results as
+    requests prepared in a thread (request tensors on the host)."""
+    specs = _small_specs(4, seed=5)
+    cpu = torch.device("cpu")
+
+    def run(**kw):
+        srv = SimServer(device, dtype=torch.float64, delta_t=DT, hbm_budget=256 << 20, max_steps=4001,
+                        max_stim_entries=64, chunk_steps=64, **kw)
+        hs = [srv.submit(c2_request(s, cpu, torch.float64)) for s in specs]
+        srv.run_until_idle()
+        srv.close()
+        return [h.result.cpu() for h in hs]
+
+    for a, b in zip(run(prep_threads=1), run(prep_processes=2)):
+        assert torch.equal(a, b)
