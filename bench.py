@@ -88,6 +88,23 @@ def timed(fn, steps, warmup, device):
     return (time.perf_counter() - t0) / steps
 
 
+def copy_bandwidth(device, nbytes=1 << 30, reps=5) -> float:
+    """Measured device-to-device copy bandwidth (bytes read + written per second) of a
+    1 GiB buffer, beside the 8 TB/s peak."""
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device=device)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize(device)
+    sec = a.elapsed_time(b) * 1e-3 / reps
+    del x, y
+    return 2 * nbytes / sec
+
+
 def kernel_time_ms(fn, device, reps=2):
     """Average duration of the fused forward launch measured with HIP events on the
     stream the kernel is launched on (PyTorch's current stream)."""
@@ -361,6 +378,7 @@ def main():
                    "batch": B, "compartments": n_comp, "time_steps": T,
                    "launch": str(cfg[0]) if cfg else None},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "copy_GBps": copy_bandwidth(device) / 1e9,
                      "frac": achieved / HBM_PEAK,
                      "traffic": traffic,
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
