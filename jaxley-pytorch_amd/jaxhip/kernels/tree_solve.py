@@ -18,10 +18,12 @@ Both variants are hand-written HIP kernels (csrc/jxhip_tree_solve.hip):
 from __future__ import annotations
 
 import os
+import weakref
 from typing import Dict, Optional
 
 import numpy as np
 import torch
+import torch.autograd.forward_ad as fwAD
 
 from .. import _lib
 from ..schedule import forest_schedule
@@ -66,6 +68,12 @@ class _KernelBase:
         self.child_idx = t(sched["child_idx"])
         self.parent_glob = t(sched["parent_glob"])
         self.sched = sched
+        # Handle of this kernel for the `jaxhip::tree_solve` operator (an operator's
+        # arguments are tensors and numbers).
+        global _NEXT_HANDLE
+        self.handle = _NEXT_HANDLE
+        _NEXT_HANDLE += 1
+        _KERNELS[self.handle] = self
 
     def _prep(self, *xs):
         ref = xs[0]
@@ -86,8 +94,14 @@ class _KernelBase:
         return x.reshape(*batch, n)
 
     def fused(self, diags, lowers, uppers, solves):
-        """Differentiable solve (backward: transposed solve + elementwise gradients)."""
-        return _TreeSolve.apply(diags, lowers, uppers, solves, self)
+        """Differentiable solve (backward: transposed solve + elementwise gradients): the
+        `jaxhip::tree_solve` operator, which `torch.compile` traces (fake tensor rule,
+        registered backward); inputs with forward-mode tangents take the autograd
+        function with the forward rule instead (as the reference keeps
+        `_DHSSolveForwardAD` apart, solver_voltage.py:190-224)."""
+        if any(fwAD.unpack_dual(t).tangent is not None for t in (diags, lowers, uppers, solves)):
+            return _TreeSolve.apply(diags, lowers, uppers, solves, self)
+        return tree_solve_op(diags, lowers, uppers, solves, self.parent_glob, self.handle)
 
     def grads(self, lam, x, need_d=True, need_l=True, need_u=True):
         rows = lam.reshape(-1, self.num_nodes).contiguous()
@@ -208,6 +222,59 @@ class _TreeSolve(torch.autograd.Function):
                     gi = gi.sum(ax, keepdim=True)
             out.append(gi)
         return (*out, None)
+
+
+_KERNELS: "weakref.WeakValueDictionary[int, _KernelBase]" = weakref.WeakValueDictionary()
+_NEXT_HANDLE = 0
+
+
+@torch.library.custom_op("jaxhip::tree_solve", mutates_args=())
+def tree_solve_op(diags: torch.Tensor, lowers: torch.Tensor, uppers: torch.Tensor,
+                  solves: torch.Tensor, parent: torch.Tensor, handle: int) -> torch.Tensor:
+    """x = A^{-1} b on the GPU by the kernel registered under `handle` (inputs broadcast
+    to `(..., num_nodes)`); `parent` (int32, -1 at roots) is the kernel's node parent,
+    used by the backward."""
+    return _KERNELS[handle].solve(diags, lowers, uppers, solves)
+
+
+@tree_solve_op.register_fake
+def _tree_solve_fake(diags, lowers, uppers, solves, parent, handle):
+    shape = torch.broadcast_shapes(diags.shape, lowers.shape, uppers.shape, solves.shape)
+    return diags.new_empty(shape)
+
+
+def _tree_solve_setup(ctx, inputs, output):
+    diags, lowers, uppers, solves, parent, handle = inputs
+    ctx.save_for_backward(diags, lowers, uppers, output, parent)
+    ctx.kernel = _KERNELS[handle]           # keeps the kernel alive with the graph
+    ctx.handle = handle
+    ctx.shapes = [t.shape for t in (diags, lowers, uppers, solves)]
+
+
+def _sum_to(g, shape):
+    while g.dim() > len(shape):
+        g = g.sum(0)
+    for ax, (a, b) in enumerate(zip(g.shape, shape)):
+        if b == 1 and a != 1:
+            g = g.sum(ax, keepdim=True)
+    return g
+
+
+def _tree_solve_backward(ctx, g):
+    """lam = A^{-T} g (the same operator with lower and upper swapped), g_d = -lam x,
+    g_u[c] = -lam[p] x[c], g_l[c] = -lam[c] x[p], g_b = lam (solver_voltage.py:143-187);
+    tensor operations only, so the backward is differentiable and traceable."""
+    diags, lowers, uppers, x, parent = ctx.saved_tensors
+    lam = tree_solve_op(diags, uppers, lowers, g, parent, ctx.handle)
+    par = parent.long()
+    has = par >= 0
+    ps = torch.where(has, par, torch.zeros_like(par))
+    grads = (-lam * x, torch.where(has, -lam * x[..., ps], torch.zeros_like(x)),
+             torch.where(has, -lam[..., ps] * x, torch.zeros_like(x)), lam)
+    return (*(_sum_to(gi, shp) for gi, shp in zip(grads, ctx.shapes)), None, None)
+
+
+tree_solve_op.register_autograd(_tree_solve_backward, setup_context=_tree_solve_setup)
 
 
 def _env_mode() -> str:

@@ -119,3 +119,31 @@ def test_network_forward_mode_vs_oracle(device):
     print(f"network forward mode: tangent rel err {_rel(tan, ref_t):.3e}, max |tangent| {np.abs(ref_t).max():.3e}")
     assert np.abs(ref_t).max() > 0
     assert _rel(tan, ref_t) < 1e-7
+
+
+def test_tree_solve_under_torch_compile(device):
+    """The solve operator inside `torch.compile(fullgraph=True)` (aot_eager: traced
+    through its fake rule and registered backward, no code generation) gives the eager
+    values and gradients."""
+    cell, _ = build_tree(20, seed=5)
+    parent = np.asarray(cell._dhs_solve_indexer["parent_lookup"])
+    kern = make_tree_solve_kernel(parent, device)
+    n = len(parent)
+    rng = np.random.default_rng(4)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    d0, l0, u0, s0 = (t(4.0 + rng.uniform(0, 1, (2, n))), t(-rng.uniform(0.1, 1, (2, n))),
+                      t(-rng.uniform(0.1, 1, (2, n))), t(rng.normal(size=(2, n))))
+    w = t(rng.normal(size=(2, n)))
+
+    def f(d, l, u, s):
+        return (kern.fused(d, l, u, s) * w).sum()
+
+    outs = []
+    for fn in (f, torch.compile(f, backend="aot_eager", fullgraph=True)):
+        args = [x.clone().requires_grad_() for x in (d0, l0, u0, s0)]
+        val = fn(*args)
+        val.backward()
+        outs.append((val.detach(), [a.grad for a in args]))
+    torch.testing.assert_close(outs[0][0], outs[1][0], rtol=1e-12, atol=0)
+    for ga, gb in zip(outs[0][1], outs[1][1]):
+        torch.testing.assert_close(ga, gb, rtol=1e-12, atol=1e-15)

@@ -193,3 +193,17 @@ def test_serve_preparation_in_worker_processes():
             assert torch.equal(getattr(p, name), getattr(q, name)), name
         for g, h in zip(p.gates0, q.gates0):
             assert torch.equal(g, h)
+
+
+def test_tree_solve_operator_fake_rule():
+    """`jaxhip::tree_solve` propagates shapes and dtypes without running (what
+    `torch.compile` traces with)."""
+    from torch._subclasses.fake_tensor import FakeTensorMode
+
+    from jaxhip.kernels.tree_solve import tree_solve_op
+
+    with FakeTensorMode():
+        d = torch.empty(3, 10, dtype=torch.float64)
+        s = torch.empty(1, 10, dtype=torch.float64)
+        x = tree_solve_op(d, d, d, s, torch.empty(10, dtype=torch.int32), 0)
+        assert x.shape == (3, 10) and x.dtype == torch.float64
