@@ -21,7 +21,7 @@ through the transposed index, no atomics).
 """
 from __future__ import annotations
 
-from typing import Dict, Optional, Sequence
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import ctypes as C
 import numpy as np
@@ -77,59 +77,102 @@ def _param_ptrs(params: Sequence[torch.Tensor]):
     return arr
 
 
-class _SynapseStep(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, plan, dt, diff, v, s, area, *params):
-        B, n_nodes = v.shape
-        E, k = plan["n_syn"], plan["k"]
-        T = E // k
-        dev, dtype = v.device, v.dtype
-        v, s, area = v.contiguous(), s.contiguous(), area.contiguous()
-        params = [p.contiguous() for p in params]
-        s_new = torch.empty((B, E), dtype=dtype, device=dev)
-        i_syn = torch.empty((B, E), dtype=dtype, device=dev)
-        lin = torch.empty((B, T), dtype=dtype, device=dev)
-        cst = torch.empty((B, T), dtype=dtype, device=dev)
-        f = _lib.fn("jxhip_synapse_step", dtype)
-        pp = _param_ptrs(params)
-        _lib.check(f(v.data_ptr(), s.data_ptr(), plan["pre"].data_ptr(), plan["targets"].data_ptr(),
-                     C.cast(pp, C.c_void_p), area.data_ptr(), dt, diff, s_new.data_ptr(), i_syn.data_ptr(),
-                     lin.data_ptr(), cst.data_ptr(), B, n_nodes, E, k, plan["model"],
-                     int(plan["targets_are_nodes"]), _lib.stream_ptr(dev)), "synapse_step")
-        if plan["model"] == 1:
-            s_new.copy_(s)
-        ctx.plan, ctx.dt, ctx.diff = plan, dt, diff
-        ctx.save_for_backward(v, s, area, *params)
-        return s_new, i_syn, lin, cst
+@torch.library.custom_op("jaxhip::synapse_step", mutates_args=())
+def synapse_step_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tensor, params: List[torch.Tensor],
+                    pre: torch.Tensor, targets: torch.Tensor, pre_nodes: torch.Tensor,
+                    inv: torch.Tensor, k: int, model: int, targets_are_nodes: bool, dt: float,
+                    diff: float) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Forward kernel (K3): `(s_new, i_syn, lin, cst)` of a regular synapse group. The
+    plan's index tensors are operator inputs (`pre_nodes`, `inv` serve the backward), so
+    a traced graph holds no tensor constants."""
+    B, n_nodes = v.shape
+    E = pre.shape[0]
+    T = E // k
+    dev, dtype = v.device, v.dtype
+    v, s, area = v.contiguous(), s.contiguous(), area.contiguous()
+    params = [p.contiguous() for p in params]
+    s_new = torch.empty((B, E), dtype=dtype, device=dev)
+    i_syn = torch.empty((B, E), dtype=dtype, device=dev)
+    lin = torch.empty((B, T), dtype=dtype, device=dev)
+    cst = torch.empty((B, T), dtype=dtype, device=dev)
+    f = _lib.fn("jxhip_synapse_step", dtype)
+    pp = _param_ptrs(params)
+    _lib.check(f(v.data_ptr(), s.data_ptr(), pre.data_ptr(), targets.data_ptr(),
+                 C.cast(pp, C.c_void_p), area.data_ptr(), dt, diff, s_new.data_ptr(), i_syn.data_ptr(),
+                 lin.data_ptr(), cst.data_ptr(), B, n_nodes, E, k, model,
+                 int(targets_are_nodes), _lib.stream_ptr(dev)), "synapse_step")
+    if model == 1:
+        s_new.copy_(s)
+    return s_new, i_syn, lin, cst
 
-    @staticmethod
-    def backward(ctx, g_sn, g_i, g_lin, g_cst):
-        plan = ctx.plan
-        v, s, area, *params = ctx.saved_tensors
-        B, n_nodes = v.shape
-        E, k = plan["n_syn"], plan["k"]
-        T = E // k
-        dev, dtype = v.device, v.dtype
-        z = lambda *shape: torch.zeros(shape, dtype=dtype, device=dev)
-        g_lin = z(B, T) if g_lin is None else g_lin.contiguous()
-        g_cst = z(B, T) if g_cst is None else g_cst.contiguous()
-        g_v = torch.empty((B, n_nodes), dtype=dtype, device=dev)
-        g_s = z(B, E)
-        g_p = torch.empty((len(params), E), dtype=dtype, device=dev)
-        work = torch.empty(2 * E * B, dtype=dtype, device=dev)
-        gsn = None if (g_sn is None or plan["model"] == 1) else g_sn.contiguous()
-        gi = None if g_i is None else g_i.contiguous()
-        f = _lib.fn("jxhip_synapse_step_backward", dtype)
-        pp = _param_ptrs(params)
-        _lib.check(f(v.data_ptr(), s.data_ptr(), plan["pre"].data_ptr(), plan["targets"].data_ptr(),
-                     C.cast(pp, C.c_void_p), area.data_ptr(), ctx.dt, ctx.diff, _lib.ptr(gsn),
-                     _lib.ptr(gi), g_lin.data_ptr(), g_cst.data_ptr(), g_v.data_ptr(), g_s.data_ptr(),
-                     g_p.data_ptr(), work.data_ptr(), plan["pre_nodes"].data_ptr(), plan["inv"].data_ptr(),
-                     plan["pre_nodes"].numel(), plan["inv_width"], B, n_nodes, E, k, plan["model"],
-                     int(plan["targets_are_nodes"]), _lib.stream_ptr(dev)), "synapse_step_backward")
-        if plan["model"] == 1 and g_sn is not None:
-            g_s = g_s + g_sn               # s passes through unchanged
-        return (None, None, None, g_v, g_s, None) + tuple(g_p[q] for q in range(len(params)))
+
+@synapse_step_op.register_fake
+def _synapse_step_fake(v, s, area, params, pre, targets, pre_nodes, inv, k, model, targets_are_nodes,
+                       dt, diff):
+    B, E = v.shape[0], pre.shape[0]
+    return v.new_empty((B, E)), v.new_empty((B, E)), v.new_empty((B, E // k)), v.new_empty((B, E // k))
+
+
+@torch.library.custom_op("jaxhip::synapse_step_backward", mutates_args=())
+def synapse_step_backward_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tensor,
+                             params: List[torch.Tensor], pre: torch.Tensor, targets: torch.Tensor,
+                             pre_nodes: torch.Tensor, inv: torch.Tensor, k: int, model: int,
+                             targets_are_nodes: bool, dt: float, diff: float,
+                             g_sn: Optional[torch.Tensor], g_i: Optional[torch.Tensor],
+                             g_lin: torch.Tensor,
+                             g_cst: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Backward kernels (K4-K5): `(g_v, g_s, g_params)`, g_params `(n_params, E)`."""
+    B, n_nodes = v.shape
+    E = pre.shape[0]
+    dev, dtype = v.device, v.dtype
+    g_v = torch.empty((B, n_nodes), dtype=dtype, device=dev)
+    g_s = torch.zeros((B, E), dtype=dtype, device=dev)
+    g_p = torch.empty((len(params), E), dtype=dtype, device=dev)
+    work = torch.empty(2 * E * B, dtype=dtype, device=dev)
+    gsn = None if (g_sn is None or model == 1) else g_sn.contiguous()
+    gi = None if g_i is None else g_i.contiguous()
+    f = _lib.fn("jxhip_synapse_step_backward", dtype)
+    pp = _param_ptrs([p.contiguous() for p in params])
+    _lib.check(f(v.contiguous().data_ptr(), s.contiguous().data_ptr(), pre.data_ptr(), targets.data_ptr(),
+                 C.cast(pp, C.c_void_p), area.contiguous().data_ptr(), dt, diff, _lib.ptr(gsn),
+                 _lib.ptr(gi), g_lin.contiguous().data_ptr(), g_cst.contiguous().data_ptr(),
+                 g_v.data_ptr(), g_s.data_ptr(), g_p.data_ptr(), work.data_ptr(), pre_nodes.data_ptr(),
+                 inv.data_ptr(), pre_nodes.numel(), inv.shape[1], B, n_nodes, E, k, model,
+                 int(targets_are_nodes), _lib.stream_ptr(dev)), "synapse_step_backward")
+    if model == 1 and g_sn is not None:
+        g_s = g_s + g_sn               # s passes through unchanged
+    return g_v, g_s, g_p
+
+
+@synapse_step_backward_op.register_fake
+def _synapse_step_backward_fake(v, s, area, params, pre, targets, pre_nodes, inv, k, model,
+                                targets_are_nodes, dt, diff, g_sn, g_i, g_lin, g_cst):
+    B, E = v.shape[0], pre.shape[0]
+    return v.new_empty(v.shape), v.new_empty((B, E)), v.new_empty((len(params), E))
+
+
+def _synapse_setup(ctx, inputs, output):
+    v, s, area, params, pre, targets, pre_nodes, inv, k, model, targets_are_nodes, dt, diff = inputs
+    ctx.save_for_backward(v, s, area, pre, targets, pre_nodes, inv, *params)
+    ctx.args = (k, model, targets_are_nodes, dt, diff)
+    ctx.n_params = len(params)
+
+
+def _synapse_backward(ctx, g_sn, g_i, g_lin, g_cst):
+    v, s, area, pre, targets, pre_nodes, inv, *params = ctx.saved_tensors
+    k, model, targets_are_nodes, dt, diff = ctx.args
+    B, E = v.shape[0], pre.shape[0]
+    z = lambda *shape: v.new_zeros(shape)
+    g_lin = z(B, E // k) if g_lin is None else g_lin
+    g_cst = z(B, E // k) if g_cst is None else g_cst
+    g_v, g_s, g_p = synapse_step_backward_op(v, s, area, list(params), pre, targets, pre_nodes, inv,
+                                             k, model, targets_are_nodes, dt, diff, g_sn, g_i,
+                                             g_lin, g_cst)
+    return (g_v, g_s, None, [g_p[q] for q in range(ctx.n_params)], None, None, None, None, None,
+            None, None, None, None)
+
+
+synapse_step_op.register_autograd(_synapse_backward, setup_context=_synapse_setup)
 
 
 def synapse_step(plan: Dict, v: torch.Tensor, s: torch.Tensor, params: Sequence[torch.Tensor],
@@ -142,4 +185,6 @@ def synapse_step(plan: Dict, v: torch.Tensor, s: torch.Tensor, params: Sequence[
     names = PARAM_NAMES[plan["model"]]
     if len(params) != len(names):
         raise ValueError(f"expected parameters {names}")
-    return _SynapseStep.apply(plan, float(dt), float(diff), v, s, area, *params)
+    return synapse_step_op(v, s, area, list(params), plan["pre"], plan["targets"], plan["pre_nodes"],
+                           plan["inv"], plan["k"], plan["model"], bool(plan["targets_are_nodes"]),
+                           float(dt), float(diff))

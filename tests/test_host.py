@@ -207,3 +207,29 @@ def test_tree_solve_operator_fake_rule():
         s = torch.empty(1, 10, dtype=torch.float64)
         x = tree_solve_op(d, d, d, s, torch.empty(10, dtype=torch.int32), 0)
         assert x.shape == (3, 10) and x.dtype == torch.float64
+
+
+def test_synapse_step_operators_fake_rules():
+    """`jaxhip::synapse_step` and `jaxhip::synapse_step_backward` propagate shapes and
+    dtypes without running, and the forward's registered backward traces through the
+    backward operator (what `torch.compile` records)."""
+    from torch._subclasses.fake_tensor import FakeTensorMode
+
+    from jaxhip.kernels.synapse_step import synapse_step_backward_op, synapse_step_op
+
+    B, n, k, T = 2, 12, 3, 4
+    E = k * T
+    with FakeTensorMode():
+        f = lambda *shape: torch.empty(shape, dtype=torch.float64)
+        i = lambda *shape: torch.empty(shape, dtype=torch.int32)
+        params = [f(E).requires_grad_() for _ in range(6)]
+        v = f(B, n).requires_grad_()
+        outs = synapse_step_op(v, f(B, E), f(E), params, i(E), i(T), i(5), i(5, 3), k, 0, False,
+                               0.025, 1e-3)
+        assert [tuple(o.shape) for o in outs] == [(B, E), (B, E), (B, T), (B, T)]
+        assert all(o.dtype == torch.float64 for o in outs)
+        g = synapse_step_backward_op(v.detach(), f(B, E), f(E), params, i(E), i(T), i(5), i(5, 3), k, 0,
+                                     False, 0.025, 1e-3, None, f(B, E), f(B, T), f(B, T))
+        assert [tuple(x.shape) for x in g] == [(B, n), (B, E), (6, E)]
+        gv, *gp = torch.autograd.grad(outs[2].sum() + outs[1].sum(), [v] + params)
+        assert gv.shape == (B, n) and all(x.shape == (E,) for x in gp)

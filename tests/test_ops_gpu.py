@@ -90,6 +90,33 @@ def test_synapse_step_forward_backward(device, model):
         assert torch.allclose(a.cpu(), b, rtol=1e-8, atol=1e-10), float((a.cpu() - b).abs().max())
 
 
+@pytest.mark.parametrize("model", [0, 1])
+def test_synapse_step_under_torch_compile(device, model):
+    """The synapse operators inside `torch.compile(fullgraph=True)` (aot_eager: traced
+    through their fake rules and the registered backward, no code generation) give the
+    eager values and gradients."""
+    pre, post, targets, k, prm, v, s, area = _problem(model, 20 + model)
+    plan = kernel_plan(model, pre, post, v.shape[1], device)
+    names = PARAM_NAMES[model]
+    ag = area.to(device)
+    rng = np.random.default_rng(1)
+    w = [torch.tensor(rng.normal(size=sh), device=device)
+         for sh in (tuple(s.shape), tuple(s.shape), (v.shape[0], len(targets)), (v.shape[0], len(targets)))]
+
+    def f(v, s, *p):
+        return sum((wi * o).sum() for wi, o in zip(w, synapse_step(plan, v, s, list(p), ag, 0.025, DIFF)))
+
+    outs = []
+    for fn in (f, torch.compile(f, backend="aot_eager", fullgraph=True)):
+        args = [x.to(device).requires_grad_() for x in [v, s] + [prm[n] for n in names]]
+        val = fn(*args)
+        val.backward()
+        outs.append((val.detach(), [a.grad for a in args]))
+    torch.testing.assert_close(outs[0][0], outs[1][0], rtol=1e-12, atol=0)
+    for ga, gb in zip(outs[0][1], outs[1][1]):
+        torch.testing.assert_close(ga, gb, rtol=1e-12, atol=1e-15)
+
+
 def test_synapse_step_float32(device):
     pre, post, targets, k, prm, v, s, area = _problem(0, 5)
     plan = kernel_plan(0, pre, post, v.shape[1], device)
