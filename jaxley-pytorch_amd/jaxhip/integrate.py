@@ -26,7 +26,7 @@ import torch
 from . import _lib
 from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, ForwardLaunch, LaunchConfig,
                     build_structure, default_config, node_float_table, syn_has_state, to_prog)
-from .geometry import axial_conductances, comp_geometry
+from .geometry import GEOMETRY_KEYS, axial_conductances, comp_geometry, cylinder_geometry
 
 _SUPPORTED_SOLVERS = ("jaxley.dhs.gpu", "jaxley.dhs.cpu", "jaxley.stone")
 # Time steppers (base.py:3327-3366): backward Euler; Crank-Nicolson solves with dt / 2
@@ -105,6 +105,12 @@ def _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_sta
         val = torch.as_tensor(val, dtype=dtype, device=device)
         if key in vals:
             vals[key] = set_at(vals[key], inds, val[..., :, None])
+            if key in ("radius", "length"):
+                # Those compartments become cylinders (modules/base.py:1997-2015).
+                it = torch.as_tensor(np.asarray(inds, dtype=np.int64), device=device)
+                geo = cylinder_geometry(vals["length"][..., it], vals["radius"][..., it])
+                for gk, gv in geo.items():
+                    vals[gk] = set_at(vals[gk], inds, gv)
         elif key in syn_vals:
             syn_vals[key] = set_at(syn_vals[key], inds, val[..., :, None])
         else:
@@ -262,7 +268,7 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
     if rec_states != {"v"}:
         raise NotImplementedError("the fused kernel records voltages ('v') only")
     rec_comps = np.array([c for _, c in module.recordings], dtype=np.int64)
-    geo_keys = ("length", "radius", "axial_resistivity", "capacitance")
+    geo_keys = ("length", "radius", "axial_resistivity", "capacitance") + GEOMETRY_KEYS
     for k in geo_keys:
         if vals[k].ndim > 1:
             raise NotImplementedError(f"batched {k!r} is not supported by the fused kernel")
@@ -276,7 +282,7 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
         st = build_structure(module, rec_comps, inp.stim_nodes, device)
         cache[key] = st
     tables = module._comp_edges
-    geo = comp_geometry(vals["length"], vals["radius"])
+    geo = comp_geometry(vals)
     g = axial_conductances(tables, module._comp_node, module._n_nodes, geo, vals["axial_resistivity"], vals["capacitance"])
     cm_node = torch.full((module._n_nodes,), -1.0, dtype=dtype, device=device)
     cm_node[torch.as_tensor(module._comp_node, device=device)] = vals["capacitance"]

@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from . import schedule as sch
+from .geometry import cylinder_geometry
 from .channels import HH, Channel, Leak
 from .synapses import IonotropicSynapse, KineticSynapse, SigmoidRateSynapse, Synapse, TestSynapse
 
@@ -97,6 +98,28 @@ class View:
             out.append(np.digitize(locs, edges) - 1 + b._cumsum_ncomp[br])
         comps = np.concatenate(out).astype(np.int64)
         return self._sub(comps, "comp", comps)
+
+    def select(self, nodes: Any = None) -> "View":
+        """Compartments by global index (`View.select(nodes=...)`, modules/base.py)."""
+        comps = self._comps if nodes is None else np.asarray(nodes, dtype=np.int64).ravel()
+        return self._sub(comps, "comp", comps)
+
+    def add_to_group(self, group_name: str):
+        """Add the compartments in view to a named group (modules/base.py)."""
+        b = self.base
+        prev = b.groups.get(group_name, np.zeros(0, np.int64))
+        b.groups[group_name] = np.union1d(prev, self._comps)
+
+    def __getattr__(self, name):
+        # `cell.soma` -> the compartments in view that belong to group `soma`.
+        if name.startswith("_") or name == "base":
+            raise AttributeError(name)
+        base = self.__dict__.get("base")
+        groups = base.__dict__.get("groups", {}) if base is not None else {}
+        if name in groups:
+            comps = self._comps[np.isin(self._comps, groups[name])]
+            return self._sub(comps, self._scope, np.zeros(len(comps), dtype=np.int64))
+        raise AttributeError(name)
 
     # -- configuration ----------------------------------------------------------------
     def insert(self, channel: Channel):
@@ -222,6 +245,9 @@ class Module(View):
         self._node_data: Dict[str, np.ndarray] = {
             k: np.full(self._n_comp, v, dtype=float) for k, v in {**COMP_PARAMS, **COMP_STATES}.items()
         }
+        geo = cylinder_geometry(self._node_data["length"], self._node_data["radius"])
+        self._node_data.update({k: np.array(v, dtype=float) for k, v in geo.items()})  # own buffers
+        self.groups: Dict[str, np.ndarray] = {}
         self._channel_mask: Dict[str, np.ndarray] = {}
         self.channels: List[Channel] = []
         self.synapses: List[Synapse] = []
@@ -301,6 +327,12 @@ class Module(View):
             col = self._node_data[key]
             valid = ~np.isnan(col[comps])
             col[comps[valid]] = np.broadcast_to(np.asarray(val, dtype=float), comps.shape)[valid]
+            if key in ("radius", "length"):
+                # Set compartments become cylinders (modules/base.py:1316-1344).
+                rows = comps[valid]
+                geo = cylinder_geometry(self._node_data["length"][rows], self._node_data["radius"][rows])
+                for k, x in geo.items():
+                    self._node_data[k][rows] = x
         elif self._is_synapse_key(key):
             for e in self._edges:
                 for d in (e["params"], e["states"]):
@@ -397,7 +429,23 @@ class Module(View):
         for e in self.__dict__.get("_edges", []):
             if e["type"].name == name:
                 return _EdgeView(self, e)
-        raise AttributeError(name)
+        return View.__getattr__(self, name)
+
+    @property
+    def comb_parents(self) -> np.ndarray:
+        """Parent branch of every branch, cells offset by their first branch
+        (modules/base.py `comb_parents`)."""
+        out = []
+        for c, (p, _) in enumerate(self._cell_specs):
+            off = self._cell_branch_off[c]
+            out.append(np.where(p < 0, -1, p + off))
+        return np.concatenate(out)
+
+    def _set_morphology(self, **cols):
+        """Per-compartment geometry as read from a morphology (no cylinder rule)."""
+        for k, v in cols.items():
+            self._node_data[k] = np.asarray(v, dtype=float).copy()
+        self._version += 1
 
 
 class _EdgeView:
@@ -516,6 +564,11 @@ def _inherit_from(self, parts):
             if k not in self._node_data:
                 self._node_data[k] = np.full(self._n_comp, np.nan)
             self._node_data[k][comps] = col
+        for name, members in mod.groups.items():
+            prev = self.groups.get(name, np.zeros(0, np.int64))
+            self.groups[name] = np.union1d(prev, comps[members])
+        if getattr(mod, "xyzr", None) is not None:
+            self.xyzr = list(getattr(self, "xyzr", None) or []) + list(mod.xyzr)
 
 
 Module._inherit_from = _inherit_from

@@ -262,7 +262,7 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
         raise NotImplementedError("recordings of voltages ('v') only")
     comp_node = torch.as_tensor(module._comp_node, device=device)
     rec_nodes = comp_node[torch.as_tensor([c for _, c in module.recordings], device=device)]
-    geo = comp_geometry(vals["length"], vals["radius"])
+    geo = comp_geometry(vals)
     g_ax = axial_conductances(tables, module._comp_node, n, geo, vals["axial_resistivity"],
                               vals["capacitance"])
     batch_src = [x for x in vals.values() if x.ndim > 1]

@@ -559,6 +559,10 @@ class OracleModel:
         self.hh_params = {k: np.full(nc, v) for k, v in HH_DEFAULTS.items()}
         self.leak_params = {k: np.full(nc, v) for k, v in LEAK_DEFAULTS.items()}
         self.gates0 = None  # (m, h, n) over compartments, default 0.2
+        # Traced geometry of a morphology read from SWC: per compartment "area",
+        # "resistive_load_in", "resistive_load_out" (io/graph.py:351-359 -> module
+        # nodes, io/graph.py:1058-1064); None means cylinders of `length`, `radius`.
+        self.morph: Optional[Dict[str, np.ndarray]] = None
         self.synapses: List[Dict] = []   # {"type", "pre" (node), "post" (node), params, s0}
 
     def node_array(self, comp_vals, fill=-1.0):
@@ -578,9 +582,14 @@ class OracleModel:
             "area": geo["area"],
             "resistive_load_in": geo["rl"],
             "resistive_load_out": geo["rl"],
+        }
+        if self.morph is not None:
+            for k in ("area", "resistive_load_in", "resistive_load_out"):
+                p[k] = self.node_array(self.morph[k])
+        p.update({
             "axial_resistivity": self.node_array(get("axial_resistivity", self.axial_resistivity)),
             "capacitance": self.node_array(get("capacitance", self.capacitance)),
-        }
+        })
         p["g_axial"] = axial_conductances(self.tables, p)
         for k, v in self.hh_params.items():
             p[k] = torch.as_tensor(get(k, v), dtype=F64)

@@ -190,3 +190,14 @@ def c4_stimulus(B: int, n_cells: int, T: int, seed: int, amp: float = 0.05, p: f
     """Bernoulli(p) pulses of `amp` nA per soma and step, (B, n_cells, T)."""
     rng = np.random.default_rng(seed)
     return amp * (rng.uniform(size=(B, n_cells, T)) < p)
+
+
+def oracle_of_swc(cells):
+    """The oracle's model of cells read by `jx.read_swc` (same parents and traced
+    compartment geometry), HH in every compartment."""
+    om = O.OracleModel([(np.asarray(c._cell_specs[0][0]), np.asarray(c._cell_specs[0][1])) for c in cells])
+    cat = lambda k: np.concatenate([c._node_data[k] for c in cells])
+    om.length[:], om.radius[:] = cat("length"), cat("radius")
+    om.morph = {k: cat(k) for k in ("area", "resistive_load_in", "resistive_load_out")}
+    om.hh_comps = np.arange(len(om.internal))
+    return om
