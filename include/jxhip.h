@@ -187,7 +187,8 @@ int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int3
  * Per-node int32 (JXHIP_NODE_ICOLS) at node_base + k: parent (-1 root), child_beg,
  *   child_end (into child_idx at child_base), flags (1 comp, 2 HH, 4 Leak, 8 export:
  *   voltage copied to HBM every step; set for recorded nodes and synapse pre- and
- *   postsynaptic nodes), stim_beg,
+ *   postsynaptic nodes; 16, 32, 64: Pospischil Na, K, Km, channels/pospischil.py:74-250),
+ *   stim_beg,
  *   stim_end (stimulus entries), target index (-1 none), 0
  * Per-node solve type (JXHIP_NODE_FCOLS): lo = -dt g(parent->k), up = -dt g(k->parent),
  *   dbase = [comp] + sum of dt g over edges into k, inv_cm (0 at branchpoints)
@@ -203,6 +204,7 @@ int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int3
 #define JXHIP_NODE_FCOLS 4
 #define JXHIP_SYN_MIXED 4   /* syn_model: the model of every synapse in syn_kind */
 #define JXHIP_NPARAM 8      /* HH gNa gK gLeak eNa eK eLeak, Leak gLeak eLeak */
+#define JXHIP_NPARAM2 7     /* Pospischil: Na gNa, eNa, vt, K gK, eK, Km gKm, taumax */
 #define JXHIP_NSYNPARAM 6
 
 typedef struct jxhip_fused_args {
@@ -298,6 +300,15 @@ typedef struct jxhip_fused_args {
      syn_param[j] then holds each synapse's j-th parameter in its own model's order, and
      every synapse has a state slot in syn_s (unused by the sigmoid-rate model). */
   const int32_t* syn_kind;
+  /* Pospischil channels (node flags 16 Na, 32 K, 64 Km; Pospischil et al. 2008 as in
+     channels/pospischil.py:74-250 of jaxley-pytorch): node_p2 (n_nodes_total,
+     JXHIP_NPARAM2) their parameters per node, unbatched; gate_ext their gates in the
+     program layout of gate_m, Na m at gate_ext, Na h at + gate_ext_stride, K n at
+     + 2 stride, Km p at + 3 stride (elements). null node_p2: no such channels. Forward
+     only (the reverse pass rejects them). */
+  const void* node_p2;
+  void* gate_ext;
+  int64_t gate_ext_stride;
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

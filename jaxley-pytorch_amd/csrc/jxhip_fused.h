@@ -34,7 +34,8 @@ namespace jxhip {
 // Column layout of the tables (see jxhip.h).
 enum : int { kNiParent = 0, kNiChildBeg, kNiChildEnd, kNiFlags, kNiStimBeg, kNiStimEnd, kNiTgt };
 enum : int { kNfLo = 0, kNfUp, kNfDbase, kNfInvCm };
-enum : int32_t { kComp = 1, kHH = 2, kLeak = 4, kExport = 8 };
+enum : int32_t { kComp = 1, kHH = 2, kLeak = 4, kExport = 8, kPNa = 16, kPK = 32, kPKm = 64 };
+constexpr int32_t kPosp = kPNa | kPK | kPKm;
 
 template <typename G, typename L>
 struct Fused {
@@ -84,6 +85,9 @@ struct Fused {
   int64_t ckpt_every;
   int64_t ckpt_stride;
   int64_t ckpt_syn_stride;
+  const G* node_p2;  // (n_nodes_total, JXHIP_NPARAM2) Pospischil parameters, or null
+  G* gx;             // Pospischil gates m, h (Na), n (K), p (Km) at q * gx_stride + item
+  int64_t gx_stride;
 };
 
 template <typename G, typename L>
@@ -118,6 +122,9 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.tgt_node = a.tgt_node;
   for (int j = 0; j < JXHIP_NSYNPARAM; ++j) f.syn_param[j] = static_cast<const G*>(a.syn_param[j]);
   f.syn_scale = static_cast<const G*>(a.syn_scale);
+  f.node_p2 = static_cast<const G*>(a.node_p2);
+  f.gx = static_cast<G*>(a.gate_ext);
+  f.gx_stride = a.gate_ext_stride;
   f.syn_s = static_cast<G*>(a.syn_s);
   f.syn_kind = a.syn_model == JXHIP_SYN_MIXED ? a.syn_kind : nullptr;
   f.dt = static_cast<G>(a.dt);
@@ -627,6 +634,56 @@ __device__ void branchpoint_means(const Fused<G, L>& a, const Topo& tp, const in
   __syncthreads();
 }
 
+// Pospischil et al. (2008) Na, K and Km channels (channels/pospischil.py:74-250) of one
+// item: parameters from the node's row of node_p2 (Na gNa, eNa, vt, K gK, eK, Km gKm,
+// taumax), gates read from and written to gx (m, h of Na, n of K, p of Km; q * gx_stride
+// + item), each advanced at the old voltage by exponential Euler; adds the conductances
+// to g and g (v - E) to cur. Compartments without these channels never enter.
+template <typename G, typename L>
+__device__ __forceinline__ void pospischil_step(const Fused<G, L>& a, int32_t flags, int64_t row,
+                                                int64_t item, double v, G& g, G& cur) {
+  const G* pr = a.node_p2 + row * JXHIP_NPARAM2;
+  const G dt = a.dt;
+  const G vv = static_cast<G>(v);
+  G* gx = a.gx + item;
+  const int64_t st = a.gx_stride;
+  const G u = vv - pr[2];  // v - vt
+  if (flags & kPNa) {
+    const G am = G(0.32) * x_over_expm1(G(-0.25) * (u - G(13)), G(1)) / G(0.25);
+    const G bm = G(0.28) * x_over_expm1(G(0.2) * (u - G(40)), G(1)) / G(0.2);
+    const G ah = G(0.128) * save_exp(-(u - G(17)) / G(18));
+    const G bh = G(4) / (save_exp(-(u - G(40)) / G(5)) + G(1));
+    const G m = gate_step(gx[0], dt, am, bm);
+    const G h = gate_step(gx[st], dt, ah, bh);
+    gx[0] = m;
+    gx[st] = h;
+    const G gna = pr[0] * (m * m * m) * h;
+    g += gna;
+    cur += gna * static_cast<G>(v - static_cast<double>(pr[1]));
+  }
+  if (flags & kPK) {
+    const G an = G(0.032) * x_over_expm1(G(-0.2) * (u - G(15)), G(1)) / G(0.2);
+    const G bn = G(0.5) * save_exp(-(u - G(10)) / G(40));
+    const G n = gate_step(gx[2 * st], dt, an, bn);
+    gx[2 * st] = n;
+    const G n2 = n * n;
+    const G gk = pr[3] * (n2 * n2);
+    g += gk;
+    cur += gk * static_cast<G>(v - static_cast<double>(pr[4]));
+  }
+  if (flags & kPKm) {
+    const G vp = vv + G(35);
+    const G pinf = G(1) / (G(1) + save_exp(G(-0.1) * vp));
+    const G tau = pr[6] / (G(3.3) * save_exp(G(0.05) * vp) + save_exp(G(-0.05) * vp));
+    const G e = save_exp(-G(1) / tau * dt);
+    const G p = gx[3 * st] * e + pinf * (G(1) - e);
+    gx[3 * st] = p;
+    const G gkm = pr[5] * p;
+    g += gkm;
+    cur += gkm * static_cast<G>(v - static_cast<double>(pr[4]));
+  }
+}
+
 // System of one (node, simulation) item from its voltage v and the axial part ax of its
 // residual: diagonal d and residual r = b - A v into LDS. The gates are read from /
 // written to the references (registers or memory).
@@ -668,6 +725,7 @@ __device__ __forceinline__ void build_core(const Fused<G, L>& a, const Prog& pg,
     g += gl;
     cur += gl * static_cast<G>(v - static_cast<double>(prm<7>(a, pg, tp, k, s)));
   }
+  if (flags & kPosp) pospischil_step(a, flags, static_cast<int64_t>(tp.nb) + k, pg.soff + i, v, g, cur);
   G lin = G(1000) * g;
   G net = -G(1000) * cur;  // (cst - lin v) of the channels
   const int tg = nik[kNiTgt];
@@ -772,7 +830,7 @@ __device__ __forceinline__ ItemConst<G, L> load_const(const Fused<G, L>& a, cons
 // the updated gates into `it`.
 template <typename G, typename L>
 __device__ __forceinline__ void build_loaded(const Fused<G, L>& a, const Prog& pg,
-                                             const Lds<L>& Ld, ItemState<G>& it,
+                                             const Topo& tp, const Lds<L>& Ld, ItemState<G>& it,
                                              const ItemConst<G, L>& c, int i, int s,
                                              int64_t t) {
   if (!(c.flags & kComp)) {
@@ -807,6 +865,8 @@ __device__ __forceinline__ void build_loaded(const Fused<G, L>& a, const Prog& p
     g += gl;
     cur += gl * static_cast<G>(v - static_cast<double>(c.p[7]));
   }
+  if (c.flags & kPosp)
+    pospischil_step(a, c.flags, static_cast<int64_t>(tp.nb) + (i >> a.log2S), pg.soff + i, v, g, cur);
   G lin = G(1000) * g;
   G net = -G(1000) * cur;
   if (c.tgt >= 0) {
@@ -1021,7 +1081,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
           }
           vg[i] = cur.v;
           axg[i] = cur.ax;
-          build_loaded(a, pg, Ld, cur, c, i, s_own, t);
+          build_loaded(a, pg, tp, Ld, cur, c, i, s_own, t);
           gm[i] = cur.m;
           gh[i] = cur.h;
           gn[i] = cur.n;
@@ -1422,6 +1482,7 @@ inline int check_args(const jxhip_fused_args& a) {
   if (a.syn_model < -1 || a.syn_model > JXHIP_SYN_MIXED) return hipErrorInvalidValue;
   if (a.syn_model == JXHIP_SYN_MIXED && a.syn_kind == nullptr) return hipErrorInvalidValue;
   if (a.dt_v < 0 || a.rhs_scale < 0 || a.explicit_step < 0 || a.explicit_step > 1) return hipErrorInvalidValue;
+  if (a.node_p2 != nullptr && (a.gate_ext == nullptr || a.gate_ext_stride <= 0)) return hipErrorInvalidValue;
   return hipSuccess;
 }
 
@@ -1503,6 +1564,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (a.n_progs <= 0) return hipSuccess;
   if (int e = check_args(a)) return e;
   if (a.syn_model >= 0 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
+  if (a.node_p2 != nullptr) return hipErrorInvalidValue;  // HH / Leak cells only
   if (a.threads > 512) return hipErrorInvalidValue;  // fused_adjoint_kernel's launch bound
   // The reverse pass differentiates the backward-Euler step only.
   if ((a.dt_v > 0 && a.dt_v != a.dt) || (a.rhs_scale > 0 && a.rhs_scale != 1.0) || a.explicit_step)

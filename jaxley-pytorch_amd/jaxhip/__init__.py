@@ -7,7 +7,7 @@ Drop-in surface (`import jaxhip as jx`): `Compartment`, `Branch`, `Cell`, `Netwo
 `jaxhip.kernels.synapse_step`. Kernels are hand-written HIP for gfx950 in
 `libjxhip.so` (C ABI: include/jxhip.h).
 """
-from .channels import HH, Leak
+from .channels import HH, K, Km, Leak, Na
 from .integrate import build_init_and_step_fn, integrate
 from .modules import Branch, Cell, Compartment, Network, connect, fully_connect
 from .stimulus import datapoint_to_step_currents, step_current
@@ -19,6 +19,6 @@ __version__ = "0.1.0"
 __all__ = [
     "Branch", "Cell", "Compartment", "Network", "connect", "fully_connect", "read_swc", "integrate", "build_init_and_step_fn",
     "step_current",
-    "datapoint_to_step_currents", "HH", "Leak", "KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse",
+    "datapoint_to_step_currents", "HH", "Leak", "Na", "K", "Km", "KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse",
     "TestSynapse",
 ]

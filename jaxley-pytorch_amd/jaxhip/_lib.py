@@ -48,7 +48,7 @@ class FusedArgs(C.Structure):
         ("ckpt_every", _i64), ("ckpt_stride", _i64), ("ckpt_syn_stride", _i64),
         ("max_items", _i64), ("coef_global", _i32), ("work_ax", _p),
         ("dt_v", C.c_double), ("rhs_scale", C.c_double), ("explicit_step", _i32),
-        ("syn_kind", _p),
+        ("syn_kind", _p), ("node_p2", _p), ("gate_ext", _p), ("gate_ext_stride", C.c_int64),
     ]
 
 

@@ -1,5 +1,5 @@
-"""Ion channels on the hot path: `HH` (channels/hh.py:14-108) and `Leak`
-(channels/pospischil.py:39-71) of jaxley-pytorch.
+"""Ion channels on the hot path: `HH` (channels/hh.py:14-108) and the Pospischil et
+al. (2008) `Leak`, `Na`, `K`, `Km` (channels/pospischil.py:39-250) of jaxley-pytorch.
 
 A channel here is a description (parameter and state names with defaults, a kernel id);
 its arithmetic lives in the HIP kernels (csrc/jxhip_common.h, jxhip_fused.hip,
@@ -58,4 +58,50 @@ class Leak(Channel):
         self.current_name = f"i_{p}"
 
 
-__all__ = ["Channel", "HH", "Leak"]
+class _Pospischil(Channel):
+    """Pospischil channels keep their reference names: the kernels know their
+    parameters (`vt`, `eNa`, `eK` are global, shared by the channels that use them)."""
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        if self._name != self.__class__.__name__:
+            raise NotImplementedError("renamed Pospischil channels are not supported by the kernels")
+
+
+class Na(_Pospischil):
+    """Sodium channel (pospischil.py:74-134)."""
+
+    kernel_id = "PNa"
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        self.channel_params = {"Na_gNa": 50e-3, "eNa": 50.0, "vt": -60.0}
+        self.channel_states = {"Na_m": 0.2, "Na_h": 0.2}
+        self.current_name = "i_Na"
+
+
+class K(_Pospischil):
+    """Potassium channel (pospischil.py:137-193)."""
+
+    kernel_id = "PK"
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        self.channel_params = {"K_gK": 5e-3, "eK": -90.0, "vt": -60.0}
+        self.channel_states = {"K_n": 0.2}
+        self.current_name = "i_K"
+
+
+class Km(_Pospischil):
+    """Slow M potassium channel (pospischil.py:196-245)."""
+
+    kernel_id = "PKm"
+
+    def __init__(self, name: Optional[str] = None):
+        super().__init__(name)
+        self.channel_params = {"Km_gKm": 0.004e-3, "Km_taumax": 4000.0, "eK": -90.0}
+        self.channel_states = {"Km_p": 0.2}
+        self.current_name = "i_K"
+
+
+__all__ = ["Channel", "HH", "Leak", "Na", "K", "Km"]

@@ -29,6 +29,11 @@ ALL_PARAMS = HH_PARAMS + LEAK_PARAMS
 # the batched table.
 BATCHABLE = {"HH_gNa": 0, "HH_gK": 1, "HH_gLeak": 2, "Leak_gLeak": 3}
 FLAG_COMP, FLAG_HH, FLAG_LEAK, FLAG_EXPORT = 1, 2, 4, 8
+CHANNEL_FLAGS = {"HH": FLAG_HH, "Leak": FLAG_LEAK, "PNa": 16, "PK": 32, "PKm": 64}
+# Pospischil channels: the columns of their parameter table and their gates, in the
+# kernel's order (include/jxhip.h, JXHIP_NPARAM2 and gate_ext).
+POSP_PARAMS = ("Na_gNa", "eNa", "vt", "K_gK", "eK", "Km_gKm", "Km_taumax")
+POSP_STATES = ("Na_m", "Na_h", "K_n", "Km_p")
 SYN_DIFF = 1e-3  # mV, the finite difference of the synapse slope (network.py:29)
 
 
@@ -103,7 +108,7 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         has = np.zeros(n, dtype=bool)
         ok = comp_of_k >= 0
         has[ok] = mask[comp_of_k[ok]]
-        flags |= np.where(has, FLAG_HH if ch.kernel_id == "HH" else FLAG_LEAK, 0).astype(np.int32)
+        flags |= np.where(has, CHANNEL_FLAGS[ch.kernel_id], 0).astype(np.int32)
     # Children of every kernel node (CSR, in increasing kernel index).
     ch_nodes = np.nonzero(parent_k >= 0)[0]
     srt = np.lexsort((ch_nodes, parent_k[ch_nodes]))
@@ -357,7 +362,8 @@ class ForwardLaunch:
                  stim_scale: Optional[torch.Tensor], T: int, dt: float, B: int, cfg: LaunchConfig,
                  syn_params: Optional[List[torch.Tensor]] = None, syn_scale: Optional[torch.Tensor] = None,
                  syn_s0: Optional[torch.Tensor] = None, ckpt_every: int = 0,
-                 dt_v: float = 0.0, rhs_scale: float = 0.0, explicit_step: int = 0):
+                 dt_v: float = 0.0, rhs_scale: float = 0.0, explicit_step: int = 0,
+                 node_p2: Optional[torch.Tensor] = None, gates_ext0: Optional[Tuple[torch.Tensor, ...]] = None):
         dev = v0.device
         dtype = gates0[0].dtype
         S = cfg.S
@@ -415,6 +421,14 @@ class ForwardLaunch:
         a.bparam_mode = bmode if bparam is not None else 0
         a.v = self.v.data_ptr()
         a.gate_m, a.gate_h, a.gate_n = self.gm.data_ptr(), self.gh.data_ptr(), self.gn.data_ptr()
+        self.gx = self.node_p2 = None
+        if node_p2 is not None:
+            # Pospischil gates: four arrays in the program layout of gate_m, one buffer.
+            self.node_p2 = node_p2
+            self.gx = torch.stack([to_prog(g, S) for g in gates_ext0]).contiguous()
+            a.node_p2 = node_p2.data_ptr()
+            a.gate_ext = self.gx.data_ptr()
+            a.gate_ext_stride = self.gm.numel()
         if stim is not None:
             a.stim = stim.data_ptr()
             a.stim_cols = stim.shape[1]

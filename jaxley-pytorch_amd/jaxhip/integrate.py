@@ -24,7 +24,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, ForwardLaunch, LaunchConfig,
+from .fused import (ALL_PARAMS, BATCHABLE, HH_PARAMS, LEAK_PARAMS, POSP_PARAMS, POSP_STATES, ForwardLaunch,
+                    LaunchConfig,
                     build_structure, default_config, node_float_table, syn_has_state, to_prog)
 from .geometry import GEOMETRY_KEYS, axial_conductances, comp_geometry, cylinder_geometry
 
@@ -199,6 +200,9 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     if not module.recordings:
         raise ValueError("No recordings are set. Please set them.")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype)
+    if has_pospischil(module) and (_needs_grad(inp) or _has_tangents(inp)):
+        raise NotImplementedError("gradients through Pospischil channels are not supported "
+                                  "(forward simulation only)")
     if ((module._edges or solver != "bwd_euler") and _needs_grad(inp)) or _has_tangents(inp):
         # Gradients through synapses or through Crank-Nicolson / forward-Euler steps, and
         # forward-mode tangents: the differentiable per-step path (the fused adjoint covers
@@ -209,6 +213,10 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
                                   solver=solver)
     return _integrate_fused(module, inp, delta_t, checkpoint_lengths, return_states, device, dtype,
                             solver)
+
+
+def has_pospischil(module) -> bool:
+    return any(ch.kernel_id in ("PNa", "PK", "PKm") for ch in module.channels)
 
 
 def _has_tangents(inp) -> bool:
@@ -290,7 +298,7 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
 
     # Batch: parameters, states, stimulus.
     chan_keys = [k for k in ALL_PARAMS if k in vals]
-    state_keys = ["v"] + [k for k in ("HH_m", "HH_h", "HH_n") if k in vals]
+    state_keys = ["v"] + [k for k in ("HH_m", "HH_h", "HH_n") + POSP_STATES if k in vals]
     init = {k: vals[k] for k in state_keys}
     if inp.all_states is not None:
         for k in state_keys:
@@ -376,6 +384,21 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
             gates0.append(torch.broadcast_to(comp_to_kernel(x), (*batch, n)).reshape(B, n))
         else:
             gates0.append(torch.zeros(B, n, dtype=dtype, device=device))
+    # Pospischil channels: their own parameter table and gates (unbatched parameters).
+    posp_kw = {}
+    if has_pospischil(module):
+        for k in POSP_PARAMS:
+            if k in vals and (vals[k].ndim > 1 or vals[k].requires_grad):
+                raise NotImplementedError(f"batched or trainable {k!r} is not supported by the fused kernel")
+        node_p2 = torch.stack([kernel_cols(vals[k]) if k in vals else zero_n for k in POSP_PARAMS], 1)
+        gx = []
+        for k in POSP_STATES:
+            if k in init:
+                x = torch.nan_to_num(init[k], nan=0.0)
+                gx.append(torch.broadcast_to(comp_to_kernel(x), (*batch, n)).reshape(B, n))
+            else:
+                gx.append(torch.zeros(B, n, dtype=dtype, device=device))
+        posp_kw = {"node_p2": node_p2.contiguous(), "gates_ext0": tuple(gx)}
     # Stimulus (T, n_stim, stim_b) and per-entry scale dt * 1e5 / (area cm).
     stim = stim_scale = None
     if inp.stim is not None:
@@ -435,7 +458,7 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
     P.B, P.batch, P.T, P.dt, P.dtype, P.user_dtype = B, batch, inp.T, dt, dtype, user_dtype
     # Launch arguments of the time stepper (0 = backward Euler).
     P.step_kw = {"dt_v": dtv if cn else 0.0, "rhs_scale": 2.0 if cn else 0.0,
-                 "explicit_step": int(solver == "fwd_euler")}
+                 "explicit_step": int(solver == "fwd_euler"), **posp_kw}
     P.needs_grad, P.device, P.stim_requires_grad = needs_grad, device, (
         inp.stim is not None and inp.stim.requires_grad)
     return P
@@ -494,7 +517,10 @@ def _final_states(module, st, run, batch, B, dtype, device):
 
     mp = torch.as_tensor(st.mp, device=device)
     out = {}
-    for name, buf in (("v", run.v), ("HH_m", run.gm), ("HH_h", run.gh), ("HH_n", run.gn)):
+    bufs = [("v", run.v), ("HH_m", run.gm), ("HH_h", run.gh), ("HH_n", run.gn)]
+    if getattr(run, "gx", None) is not None:
+        bufs += list(zip(POSP_STATES, run.gx))
+    for name, buf in bufs:
         if name != "v" and name not in module._node_data:
             continue
         xs = from_prog(buf, B)
@@ -536,6 +562,8 @@ class _StepRunner:
 
     def __init__(self, module, params, all_states, param_state, delta_t, ext_nodes, ext_batch,
                  solver: str = "bwd_euler"):
+        if has_pospischil(module):
+            raise NotImplementedError("the per-step API does not carry Pospischil channel states")
         ref = first_float_tensor(params, param_state, all_states)
         device = ref.device if ref is not None else torch.get_default_device()
         dtype = ref.dtype if ref is not None else torch.get_default_dtype()

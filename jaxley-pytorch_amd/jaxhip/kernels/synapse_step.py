@@ -77,14 +77,8 @@ def _param_ptrs(params: Sequence[torch.Tensor]):
     return arr
 
 
-@torch.library.custom_op("jaxhip::synapse_step", mutates_args=())
-def synapse_step_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tensor, params: List[torch.Tensor],
-                    pre: torch.Tensor, targets: torch.Tensor, pre_nodes: torch.Tensor,
-                    inv: torch.Tensor, k: int, model: int, targets_are_nodes: bool, dt: float,
-                    diff: float) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
-    """Forward kernel (K3): `(s_new, i_syn, lin, cst)` of a regular synapse group. The
-    plan's index tensors are operator inputs (`pre_nodes`, `inv` serve the backward), so
-    a traced graph holds no tensor constants."""
+def _forward_impl(v, s, area, params, pre, targets, k: int, model: int, targets_are_nodes: bool,
+                  dt: float, diff: float):
     B, n_nodes = v.shape
     E = pre.shape[0]
     T = E // k
@@ -106,6 +100,17 @@ def synapse_step_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tensor, params
     return s_new, i_syn, lin, cst
 
 
+@torch.library.custom_op("jaxhip::synapse_step", mutates_args=())
+def synapse_step_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tensor, params: List[torch.Tensor],
+                    pre: torch.Tensor, targets: torch.Tensor, pre_nodes: torch.Tensor,
+                    inv: torch.Tensor, k: int, model: int, targets_are_nodes: bool, dt: float,
+                    diff: float) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Forward kernel (K3): `(s_new, i_syn, lin, cst)` of a regular synapse group. The
+    plan's index tensors are operator inputs (`pre_nodes`, `inv` serve the backward), so
+    a traced graph holds no tensor constants."""
+    return _forward_impl(v, s, area, params, pre, targets, k, model, targets_are_nodes, dt, diff)
+
+
 @synapse_step_op.register_fake
 def _synapse_step_fake(v, s, area, params, pre, targets, pre_nodes, inv, k, model, targets_are_nodes,
                        dt, diff):
@@ -113,15 +118,8 @@ def _synapse_step_fake(v, s, area, params, pre, targets, pre_nodes, inv, k, mode
     return v.new_empty((B, E)), v.new_empty((B, E)), v.new_empty((B, E // k)), v.new_empty((B, E // k))
 
 
-@torch.library.custom_op("jaxhip::synapse_step_backward", mutates_args=())
-def synapse_step_backward_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tensor,
-                             params: List[torch.Tensor], pre: torch.Tensor, targets: torch.Tensor,
-                             pre_nodes: torch.Tensor, inv: torch.Tensor, k: int, model: int,
-                             targets_are_nodes: bool, dt: float, diff: float,
-                             g_sn: Optional[torch.Tensor], g_i: Optional[torch.Tensor],
-                             g_lin: torch.Tensor,
-                             g_cst: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
-    """Backward kernels (K4-K5): `(g_v, g_s, g_params)`, g_params `(n_params, E)`."""
+def _backward_impl(v, s, area, params, pre, targets, pre_nodes, inv, k: int, model: int,
+                   targets_are_nodes: bool, dt: float, diff: float, g_sn, g_i, g_lin, g_cst):
     B, n_nodes = v.shape
     E = pre.shape[0]
     dev, dtype = v.device, v.dtype
@@ -142,6 +140,19 @@ def synapse_step_backward_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tenso
     if model == 1 and g_sn is not None:
         g_s = g_s + g_sn               # s passes through unchanged
     return g_v, g_s, g_p
+
+
+@torch.library.custom_op("jaxhip::synapse_step_backward", mutates_args=())
+def synapse_step_backward_op(v: torch.Tensor, s: torch.Tensor, area: torch.Tensor,
+                             params: List[torch.Tensor], pre: torch.Tensor, targets: torch.Tensor,
+                             pre_nodes: torch.Tensor, inv: torch.Tensor, k: int, model: int,
+                             targets_are_nodes: bool, dt: float, diff: float,
+                             g_sn: Optional[torch.Tensor], g_i: Optional[torch.Tensor],
+                             g_lin: torch.Tensor,
+                             g_cst: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Backward kernels (K4-K5): `(g_v, g_s, g_params)`, g_params `(n_params, E)`."""
+    return _backward_impl(v, s, area, params, pre, targets, pre_nodes, inv, k, model,
+                          targets_are_nodes, dt, diff, g_sn, g_i, g_lin, g_cst)
 
 
 @synapse_step_backward_op.register_fake
@@ -175,6 +186,31 @@ def _synapse_backward(ctx, g_sn, g_i, g_lin, g_cst):
 synapse_step_op.register_autograd(_synapse_backward, setup_context=_synapse_setup)
 
 
+class _SynapseStep(torch.autograd.Function):
+    """The same forward and backward kernels without the operator dispatch (the eager
+    per-step path calls this once per step and synapse type)."""
+
+    @staticmethod
+    def forward(ctx, v, s, area, plan, dt, diff, *params):
+        out = _forward_impl(v, s, area, list(params), plan["pre"], plan["targets"], plan["k"],
+                            plan["model"], bool(plan["targets_are_nodes"]), dt, diff)
+        ctx.save_for_backward(v, s, area, *params)
+        ctx.plan, ctx.dt, ctx.diff = plan, dt, diff
+        return out
+
+    @staticmethod
+    def backward(ctx, g_sn, g_i, g_lin, g_cst):
+        v, s, area, *params = ctx.saved_tensors
+        plan = ctx.plan
+        B, E, k = v.shape[0], plan["n_syn"], plan["k"]
+        g_lin = v.new_zeros((B, E // k)) if g_lin is None else g_lin
+        g_cst = v.new_zeros((B, E // k)) if g_cst is None else g_cst
+        g_v, g_s, g_p = _backward_impl(v, s, area, params, plan["pre"], plan["targets"], plan["pre_nodes"],
+                                       plan["inv"], k, plan["model"], bool(plan["targets_are_nodes"]),
+                                       ctx.dt, ctx.diff, g_sn, g_i, g_lin, g_cst)
+        return (g_v, g_s, None, None, None, None) + tuple(g_p[q] for q in range(len(params)))
+
+
 def synapse_step(plan: Dict, v: torch.Tensor, s: torch.Tensor, params: Sequence[torch.Tensor],
                  area: torch.Tensor, dt: float, diff: float = 1e-3):
     """One synapse step of a regular synapse group; `v` (B, n_nodes), `s` (B, E),
@@ -185,6 +221,8 @@ def synapse_step(plan: Dict, v: torch.Tensor, s: torch.Tensor, params: Sequence[
     names = PARAM_NAMES[plan["model"]]
     if len(params) != len(names):
         raise ValueError(f"expected parameters {names}")
-    return synapse_step_op(v, s, area, list(params), plan["pre"], plan["targets"], plan["pre_nodes"],
-                           plan["inv"], plan["k"], plan["model"], bool(plan["targets_are_nodes"]),
-                           float(dt), float(diff))
+    if torch.compiler.is_compiling():
+        return synapse_step_op(v, s, area, list(params), plan["pre"], plan["targets"], plan["pre_nodes"],
+                               plan["inv"], plan["k"], plan["model"], bool(plan["targets_are_nodes"]),
+                               float(dt), float(diff))
+    return _SynapseStep.apply(v, s, area, plan, float(dt), float(diff), *params)

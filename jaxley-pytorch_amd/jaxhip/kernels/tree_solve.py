@@ -94,14 +94,15 @@ class _KernelBase:
         return x.reshape(*batch, n)
 
     def fused(self, diags, lowers, uppers, solves):
-        """Differentiable solve (backward: transposed solve + elementwise gradients): the
-        `jaxhip::tree_solve` operator, which `torch.compile` traces (fake tensor rule,
-        registered backward); inputs with forward-mode tangents take the autograd
-        function with the forward rule instead (as the reference keeps
-        `_DHSSolveForwardAD` apart, solver_voltage.py:190-224)."""
-        if any(fwAD.unpack_dual(t).tangent is not None for t in (diags, lowers, uppers, solves)):
-            return _TreeSolve.apply(diags, lowers, uppers, solves, self)
-        return tree_solve_op(diags, lowers, uppers, solves, self.parent_glob, self.handle)
+        """Differentiable solve (backward: transposed solve + elementwise gradients).
+        Under `torch.compile` it is the `jaxhip::tree_solve` operator (fake tensor rule,
+        registered backward); eagerly it is the autograd function with the same
+        backward and the forward rule of `_DHSSolveForwardAD` (solver_voltage.py:190-224),
+        which skips the operator dispatch (tens of microseconds per call on the per-step
+        path)."""
+        if torch.compiler.is_compiling():
+            return tree_solve_op(diags, lowers, uppers, solves, self.parent_glob, self.handle)
+        return _TreeSolve.apply(diags, lowers, uppers, solves, self)
 
     def grads(self, lam, x, need_d=True, need_l=True, need_u=True):
         rows = lam.reshape(-1, self.num_nodes).contiguous()

@@ -18,7 +18,7 @@ import torch
 
 from . import schedule as sch
 from .geometry import cylinder_geometry
-from .channels import HH, Channel, Leak
+from .channels import HH, Channel, K, Km, Leak, Na
 from .synapses import IonotropicSynapse, KineticSynapse, SigmoidRateSynapse, Synapse, TestSynapse
 
 COMP_PARAMS = {"length": 10.0, "radius": 1.0, "axial_resistivity": 5000.0, "capacitance": 1.0}
@@ -295,7 +295,7 @@ class Module(View):
 
     # -- mechanisms ---------------------------------------------------------------------
     def _insert(self, channel: Channel, comps: np.ndarray):
-        if not isinstance(channel, (HH, Leak)):
+        if not isinstance(channel, (HH, Leak, Na, K, Km)):
             raise NotImplementedError(f"channel {type(channel).__name__} is not on the jaxhip path")
         name = channel.name
         if name not in self._channel_mask:
@@ -387,6 +387,30 @@ class Module(View):
                 self._node_data["HH_m"][m] = am / (am + bm)
                 self._node_data["HH_h"][m] = ah / (ah + bh)
                 self._node_data["HH_n"][m] = an / (an + bn)
+            elif ch.kernel_id in ("PNa", "PK", "PKm"):
+                # Fixed points of the gates (pospischil.py:117-125, 184-189, 235-238;
+                # Km takes p_inf / (p_inf + tau_p), as the reference does).
+                m = self._channel_mask[ch.name]
+                v = self._node_data["v"][m]
+                d = self._node_data
+                if ch.kernel_id == "PKm":
+                    vp = v + 35.0
+                    pinf = 1.0 / (1.0 + np.exp(np.minimum(-0.1 * vp, 20)))
+                    tau = d["Km_taumax"][m] / (3.3 * np.exp(np.minimum(0.05 * vp, 20)) + np.exp(np.minimum(-0.05 * vp, 20)))
+                    d["Km_p"][m] = pinf / (pinf + tau)
+                    continue
+                u = v - d["vt"][m]
+                if ch.kernel_id == "PNa":
+                    am = 0.32 * _xoe(-0.25 * (u - 13.0), 1.0) / 0.25
+                    bm = 0.28 * _xoe(0.2 * (u - 40.0), 1.0) / 0.2
+                    ah = 0.128 * np.exp(np.minimum(-(u - 17.0) / 18.0, 20))
+                    bh = 4.0 / (np.exp(np.minimum(-(u - 40.0) / 5.0, 20)) + 1.0)
+                    d["Na_m"][m] = am / (am + bm)
+                    d["Na_h"][m] = ah / (ah + bh)
+                else:
+                    an = 0.032 * _xoe(-0.2 * (u - 15.0), 1.0) / 0.2
+                    bn = 0.5 * np.exp(np.minimum(-(u - 10.0) / 40.0, 20))
+                    d["K_n"][m] = an / (an + bn)
         self._version += 1
 
     def get_parameters(self) -> List[Dict[str, torch.Tensor]]:

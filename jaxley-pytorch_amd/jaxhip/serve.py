@@ -124,6 +124,8 @@ def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_state
                      user_dtype, f32_mode: str, max_steps: int) -> _Prepared:
     """`integrate._assemble` + `_prepare` for one served request (batch size 1)."""
     integ = importlib.import_module("jaxhip.integrate")
+    if integ.has_pospischil(module):
+        raise NotImplementedError("served requests carry HH / Leak channels only")
     integ._F32_MODE = f32_mode
     cpu = torch.device("cpu")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, dt, all_states, cpu, user_dtype)

@@ -382,6 +382,72 @@ def hh_current(m, h, n, v, p):
     return gNa * (v - p["HH_eNa"]) + gK * (v - p["HH_eK"]) + p["HH_gLeak"] * (v - p["HH_eLeak"])
 
 
+
+# Pospischil et al. (2008) channels (channels/pospischil.py:74-250): Na (m, h) and K (n)
+# with the global threshold parameter vt, Km (p) with its own time-constant scale.
+def posp_na_rates(v, vt):
+    """`Na.m_gate`, `Na.h_gate` (pospischil.py:120-134)."""
+    va, vb = v - vt - 13.0, v - vt - 40.0
+    am = 0.32 * x_over_expm1(-0.25 * va) / 0.25
+    bm = 0.28 * x_over_expm1(0.2 * vb) / 0.2
+    ha, hb = v - vt - 17.0, v - vt - 40.0
+    ah = 0.128 * save_exp(-ha / 18.0)
+    bh = 4.0 / (save_exp(-hb / 5.0) + 1.0)
+    return am, bm, ah, bh
+
+
+def posp_k_rates(v, vt):
+    """`K.n_gate` (pospischil.py:186-193)."""
+    an = 0.032 * x_over_expm1(-0.2 * (v - vt - 15.0)) / 0.2
+    bn = 0.5 * save_exp(-(v - vt - 10.0) / 40.0)
+    return an, bn
+
+
+def posp_km_gate(v, taumax):
+    """`Km.p_gate` (pospischil.py:240-245): (p_inf, tau_p)."""
+    vp = v + 35.0
+    return 1.0 / (1.0 + save_exp(-0.1 * vp)), taumax / (3.3 * save_exp(0.05 * vp) + save_exp(-0.05 * vp))
+
+
+def gate_exponential(x, dt, alpha, beta):
+    """`solve_gate_exponential` (solver_gate.py:45-64)."""
+    tau = 1.0 / (alpha + beta)
+    e = save_exp(-dt / tau)
+    return x * e + alpha * tau * (1.0 - e)
+
+
+def inf_gate_exponential(x, dt, x_inf, tau):
+    """`solve_inf_gate_exponential` (solver_gate.py:67-87)."""
+    e = save_exp(-1.0 / tau * dt)
+    return x * e + x_inf * (1.0 - e)
+
+
+POSP_PARAMS = {"Na_gNa": 50e-3, "eNa": 50.0, "vt": -60.0, "K_gK": 5e-3, "eK": -90.0,
+               "Km_gKm": 0.004e-3, "Km_taumax": 4000.0}
+POSP_STATES = {"Na": ("Na_m", "Na_h"), "K": ("K_n",), "Km": ("Km_p",)}
+
+
+def posp_update(name, st, v, p, dt):
+    """New gates of one Pospischil channel at voltage v (`update_states`)."""
+    if name == "Na":
+        am, bm, ah, bh = posp_na_rates(v, p["vt"])
+        return {"Na_m": gate_exponential(st["Na_m"], dt, am, bm), "Na_h": gate_exponential(st["Na_h"], dt, ah, bh)}
+    if name == "K":
+        an, bn = posp_k_rates(v, p["vt"])
+        return {"K_n": gate_exponential(st["K_n"], dt, an, bn)}
+    pinf, tau = posp_km_gate(v, p["Km_taumax"])
+    return {"Km_p": inf_gate_exponential(st["Km_p"], dt, pinf, tau)}
+
+
+def posp_current(name, st, v, p):
+    """`compute_current` in mA/cm^2 (pospischil.py:108-115, 176-182, 228-233)."""
+    if name == "Na":
+        return p["Na_gNa"] * st["Na_m"] ** 3 * st["Na_h"] * (v - p["eNa"])
+    if name == "K":
+        return p["K_gK"] * st["K_n"] ** 4 * (v - p["eK"])
+    return p["Km_gKm"] * st["Km_p"] * (v - p["eK"])
+
+
 def hh_init(v):
     """`HH.init_state` (channels/hh.py:69-85)."""
     (am, bm), (ah, bh), (an, bn) = hh_rates(v)
@@ -563,6 +629,11 @@ class OracleModel:
         # "resistive_load_in", "resistive_load_out" (io/graph.py:351-359 -> module
         # nodes, io/graph.py:1058-1064); None means cylinders of `length`, `radius`.
         self.morph: Optional[Dict[str, np.ndarray]] = None
+        # Pospischil channels: compartments per channel ("Na", "K", "Km"), parameters
+        # and initial gates per compartment.
+        self.posp_comps: Dict[str, np.ndarray] = {}
+        self.posp_params = {k: np.full(nc, v) for k, v in POSP_PARAMS.items()}
+        self.posp_states0 = {k: np.full(nc, 0.2) for ks in POSP_STATES.values() for k in ks}
         self.synapses: List[Dict] = []   # {"type", "pre" (node), "post" (node), params, s0}
 
     def node_array(self, comp_vals, fill=-1.0):
@@ -683,6 +754,12 @@ def simulate(
         m, h, nn = (torch.as_tensor(g, dtype=F64)[..., model.hh_comps] for g in model.gates0)
 
     syn_states = [torch.as_tensor(sp["s0"], dtype=F64) if sp.get("s0") is not None else None for sp in model.synapses]
+    posp = {}
+    for name, comps in model.posp_comps.items():
+        comps = np.asarray(comps, dtype=np.int64)
+        prm = {k: torch.as_tensor(overrides.get(k, x), dtype=F64)[..., comps] for k, x in model.posp_params.items()}
+        st0 = {k: torch.as_tensor(model.posp_states0[k], dtype=F64)[..., comps] for k in POSP_STATES[name]}
+        posp[name] = [torch.as_tensor(model.internal[comps]), prm, st0]
     batch = torch.broadcast_shapes(
         v.shape[:-1], p["g_axial"].shape[:-1], *[t.shape[:-1] for t in hhp.values()],
         *(() if stim is None else (stim.shape[:-2],)),
@@ -752,6 +829,17 @@ def simulate(
             ct = I0 - vt * vl
             lin = lin.index_add(-1, leak_nodes, torch.broadcast_to(vt, (*v.shape[:-1], len(leak_nodes))))
             const = const.index_add(-1, leak_nodes, torch.broadcast_to(-ct, (*v.shape[:-1], len(leak_nodes))))
+        for name, entry in posp.items():
+            nodes, prm, st = entry
+            vp = v[..., nodes]
+            st = posp_update(name, st, vp, prm, dt)
+            entry[2] = st
+            I0 = posp_current(name, st, vp, prm)
+            I1 = posp_current(name, st, vp + diff, prm)
+            vt = (I1 - I0) / diff
+            ct = I0 - vt * vp
+            lin = lin.index_add(-1, nodes, torch.broadcast_to(vt, (*v.shape[:-1], len(nodes))))
+            const = const.index_add(-1, nodes, torch.broadcast_to(-ct, (*v.shape[:-1], len(nodes))))
         lin = lin * 1000.0
         const = const * 1000.0
         slin, sconst, syn_states = syn_terms(v, syn_states, True)

@@ -201,3 +201,46 @@ def oracle_of_swc(cells):
     om.morph = {k: cat(k) for k in ("area", "resistive_load_in", "resistive_load_out")}
     om.hh_comps = np.arange(len(om.internal))
     return om
+
+
+def oracle_of_module(mod):
+    """The oracle's model of a `jaxhip` cell or network (structure, geometry, HH, Leak
+    and Pospischil channels with the module's parameters and initial states)."""
+    om = O.OracleModel([(np.asarray(p), np.asarray(nc)) for p, nc in mod._cell_specs])
+    d = mod._node_data
+    for k in ("length", "radius", "axial_resistivity", "capacitance"):
+        getattr(om, k)[:] = d[k]
+    om.morph = {k: d[k].copy() for k in ("area", "resistive_load_in", "resistive_load_out")}
+    om.v0 = d["v"].copy()
+    mask = mod._channel_mask
+    if "HH" in mask:
+        om.hh_comps = np.nonzero(mask["HH"])[0]
+        om.hh_params = {k: np.nan_to_num(d[k]) for k in om.hh_params}
+        om.gates0 = tuple(np.nan_to_num(d[k]) for k in ("HH_m", "HH_h", "HH_n"))
+    if "Leak" in mask:
+        om.leak_comps = np.nonzero(mask["Leak"])[0]
+        om.leak_params = {k: np.nan_to_num(d[k]) for k in om.leak_params}
+    for name in ("Na", "K", "Km"):
+        if name in mask:
+            om.posp_comps[name] = np.nonzero(mask[name])[0]
+    for k in om.posp_params:
+        if k in d:
+            om.posp_params[k] = np.nan_to_num(d[k])
+    for k in om.posp_states0:
+        if k in d:
+            om.posp_states0[k] = np.nan_to_num(d[k])
+    return om
+
+
+def pospischil_cell():
+    """The reference's channel steady-state cell (tests/test_channels.py:181-208) without
+    its calcium channels: two branches of four compartments, Na on branch 0, K on
+    branch 1, Km at branch 1's first compartment, Leak and HH everywhere."""
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 4), parents=[-1, 0])
+    cell.branch(0).loc(0.0).record()
+    cell.branch(0).insert(jx.Na())
+    cell.branch(1).insert(jx.K())
+    cell.branch(1).loc(0.0).insert(jx.Km())
+    cell.insert(jx.Leak())
+    cell.insert(jx.HH())
+    return cell
