@@ -200,13 +200,13 @@ def integrate(module, params: List[Dict[str, torch.Tensor]] = [], *, param_state
     if not module.recordings:
         raise ValueError("No recordings are set. Please set them.")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, delta_t, all_states, device, dtype)
-    if has_pospischil(module) and (_needs_grad(inp) or _has_tangents(inp)):
-        raise NotImplementedError("gradients through Pospischil channels are not supported "
-                                  "(forward simulation only)")
-    if ((module._edges or solver != "bwd_euler") and _needs_grad(inp)) or _has_tangents(inp):
-        # Gradients through synapses or through Crank-Nicolson / forward-Euler steps, and
-        # forward-mode tangents: the differentiable per-step path (the fused adjoint covers
-        # reverse mode for backward-Euler cells without synapses).
+    posp_grad = has_pospischil(module) and (_needs_grad(inp) or any(
+        inp.vals[k].ndim > 1 for k in POSP_PARAMS if k in inp.vals))
+    if ((module._edges or solver != "bwd_euler") and _needs_grad(inp)) or _has_tangents(inp) or posp_grad:
+        # Gradients through synapses, Pospischil channels or Crank-Nicolson /
+        # forward-Euler steps, batched Pospischil parameters, and forward-mode tangents:
+        # the differentiable per-step path (the fused adjoint covers reverse mode for
+        # backward-Euler HH / Leak cells without synapses).
         from .stepwise import integrate_stepwise
 
         return integrate_stepwise(module, inp, delta_t, checkpoint_lengths, device, dtype, return_states,

@@ -10,9 +10,10 @@ time with autograd, on the device:
   pair K3 / K4-K5 with its hand-written backward; network.py:350-478) for Kinetic and
   SigmoidRate synapses in the regular layout; the reference's per-edge tensor operations
   for IonotropicSynapse and TestSynapse (network.py:480-578);
-* HH / Leak: elementwise tensor operations, as the reference computes them
-  (channels/hh.py:37-108, channels/pospischil.py:39-71, solver_gate.py:8-64; the reference
-  has no kernel there);
+* HH, Leak and the Pospischil Na / K / Km channels: elementwise tensor operations, as
+  the reference computes them (channels/hh.py:37-108, channels/pospischil.py:39-245,
+  solver_gate.py:8-87; the reference has no kernel there); cells with Pospischil
+  channels take this path for gradients and for batched Pospischil parameters;
 * the implicit system (solver_voltage.py:250-359) and its solve by the HIP tree-solve
   kernel with its transposed-solve backward (`kernels.tree_solve`, K1 / K2).
 
@@ -70,6 +71,39 @@ def _hh_gates(m, h, n, v, dt):
     an = 0.01 * _x_over_expm1(-(v + 55.0), 10.0)
     bn = 0.125 * _save_exp(-(v + 65.0) / 80.0)
     return _gate(m, dt, am, bm), _gate(h, dt, ah, bh), _gate(n, dt, an, bn)
+
+
+_CHANNEL_PARAMS = {
+    "HH": ("HH_gNa", "HH_gK", "HH_gLeak", "HH_eNa", "HH_eK", "HH_eLeak"),
+    "Leak": ("Leak_gLeak", "Leak_eLeak"),
+    "PNa": ("Na_gNa", "eNa", "vt"), "PK": ("K_gK", "eK", "vt"), "PKm": ("Km_gKm", "Km_taumax", "eK"),
+}
+_CHANNEL_GATES = {"HH": ("HH_m", "HH_h", "HH_n"), "PNa": ("Na_m", "Na_h"), "PK": ("K_n",), "PKm": ("Km_p",)}
+_GATE_ORDER = ("HH", "PNa", "PK", "PKm")
+
+
+def _pospischil(kid, gates, v, p, dt):
+    """Gates at the old voltage, then (conductance, reversal potential) of a Pospischil
+    channel (channels/pospischil.py:74-245; solver_gate.py:45-87)."""
+    if kid == "PKm":
+        vp = v + 35.0
+        pinf = 1.0 / (1.0 + _save_exp(-0.1 * vp))
+        tau = p["Km_taumax"] / (3.3 * _save_exp(0.05 * vp) + _save_exp(-0.05 * vp))
+        e = _save_exp(-1.0 / tau * dt)
+        q = gates[0] * e + pinf * (1.0 - e)
+        return [q], p["Km_gKm"] * q, p["eK"]
+    u = v - p["vt"]
+    if kid == "PNa":
+        am = 0.32 * _x_over_expm1(-0.25 * (u - 13.0), 1.0) / 0.25
+        bm = 0.28 * _x_over_expm1(0.2 * (u - 40.0), 1.0) / 0.2
+        ah = 0.128 * _save_exp(-(u - 17.0) / 18.0)
+        bh = 4.0 / (_save_exp(-(u - 40.0) / 5.0) + 1.0)
+        m, h = _gate(gates[0], dt, am, bm), _gate(gates[1], dt, ah, bh)
+        return [m, h], p["Na_gNa"] * m**3 * h, p["eNa"]
+    an = 0.032 * _x_over_expm1(-0.2 * (u - 15.0), 1.0) / 0.2
+    bn = 0.5 * _save_exp(-(u - 10.0) / 40.0)
+    nn = _gate(gates[0], dt, an, bn)
+    return [nn], p["K_gK"] * nn**4, p["eK"]
 
 
 def _explicit_edges(tables, device):
@@ -287,28 +321,29 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     area = node_array(geo["area"], 1.0)
     gdt = dtv * flat(g_ax)
     sinks = torch.as_tensor(np.asarray(tables["sink"], dtype=np.int64), device=device)
-    # Channels (HH, Leak) and their compartments.
+    # Channels (HH, Leak, Pospischil Na / K / Km) and their compartments.
     chans = {}
     for ch in module.channels:
         comps = np.nonzero(module._channel_mask[ch.name])[0]
         if not len(comps):
             continue
         cidx = torch.as_tensor(comps, device=device)
-        names = ("HH_gNa", "HH_gK", "HH_gLeak", "HH_eNa", "HH_eK", "HH_eLeak") if ch.kernel_id == "HH" \
-            else ("Leak_gLeak", "Leak_eLeak")
+        names = _CHANNEL_PARAMS[ch.kernel_id]
         prm = {k: flat(vals[k])[:, cidx] for k in names}
         chans[ch.kernel_id] = (comp_node[cidx], prm, cidx)
     # Initial states.
-    init = {k: vals[k] for k in vals if k in ("v", "HH_m", "HH_h", "HH_n")}
+    gate_keys = [k for kid in _GATE_ORDER if kid in chans for k in _CHANNEL_GATES[kid]]
+    init = {k: vals[k] for k in vals if k in ["v"] + gate_keys}
     if inp.all_states is not None:
         for k, x in inp.all_states.items():
             if k in init:
                 init[k] = x[..., comp_node]
     v = torch.broadcast_to(node_array(init["v"], 0.0), (B, n))
     gates = []
-    if "HH" in chans:
-        cidx = chans["HH"][2]
-        gates = [torch.broadcast_to(flat(init[k])[:, cidx], (B, len(cidx))) for k in ("HH_m", "HH_h", "HH_n")]
+    for kid in _GATE_ORDER:
+        if kid in chans:
+            cidx = chans[kid][2]
+            gates += [torch.broadcast_to(flat(init[k])[:, cidx], (B, len(cidx))) for k in _CHANNEL_GATES[kid]]
     forward_ad = has_tangents(list(inp.vals.values()) + list(inp.syn_vals.values()) +
                               ([inp.stim] if inp.stim is not None else []) +
                               (list(inp.all_states.values()) if inp.all_states is not None else []))
@@ -349,11 +384,14 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     def step(v, gates, syn_s, t):
         lin = torch.zeros(B, n, dtype=dtype, device=device)
         cst = torch.zeros(B, n, dtype=dtype, device=device)
+        gates = list(gates)
+        gi = 0
         if "HH" in chans:
             nodes, p, _ = chans["HH"]
             vh = v[:, nodes]
-            m, h, nn = _hh_gates(*gates, vh, dt)
-            gates = [m, h, nn]
+            m, h, nn = _hh_gates(*gates[:3], vh, dt)
+            gates[:3] = [m, h, nn]
+            gi = 3
             gNa = p["HH_gNa"] * m**3 * h
             gK = p["HH_gK"] * nn**4
             g = gNa + gK + p["HH_gLeak"]
@@ -365,6 +403,15 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
             nodes, p, _ = chans["Leak"]
             lin = lin.index_add(1, nodes, torch.broadcast_to(p["Leak_gLeak"], (B, len(nodes))))
             cst = cst.index_add(1, nodes, torch.broadcast_to(p["Leak_gLeak"] * p["Leak_eLeak"], (B, len(nodes))))
+        for kid in _GATE_ORDER[1:]:
+            if kid in chans:
+                nodes, p, _ = chans[kid]
+                k = len(_CHANNEL_GATES[kid])
+                new, g, e = _pospischil(kid, gates[gi:gi + k], v[:, nodes], p, dt)
+                gates[gi:gi + k] = new
+                gi += k
+                lin = lin.index_add(1, nodes, torch.broadcast_to(g, (B, len(nodes))))
+                cst = cst.index_add(1, nodes, torch.broadcast_to(g * e, (B, len(nodes))))
         lin = lin * 1000.0
         cst = cst * 1000.0
         new_s = []
@@ -413,11 +460,13 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
         return rec
     v, gates, syn_s = state[0], list(state[1:1 + n_g]), list(state[1 + n_g:])
     out = {"v": v.reshape(*batch, n)}
-    if gates:
-        hh_nodes = chans["HH"][0]
-        for k, g in zip(("HH_m", "HH_h", "HH_n"), gates):
-            full = torch.full((B, n), float("nan"), dtype=dtype, device=device)
-            out[k] = full.index_copy(1, hh_nodes, g).reshape(*batch, n)
+    gi = 0
+    for kid in _GATE_ORDER:
+        if kid in chans:
+            for k in _CHANNEL_GATES[kid]:
+                full = torch.full((B, n), float("nan"), dtype=dtype, device=device)
+                out[k] = full.index_copy(1, chans[kid][0], gates[gi]).reshape(*batch, n)
+                gi += 1
     for grp, s in zip(groups, syn_s):
         if grp.state_key is not None:
             out[grp.state_key] = s[:, grp.inv].reshape(*batch, -1)

@@ -93,8 +93,53 @@ def test_pospischil_steady_state(device):
     assert abs(v[0, 0] - v[0, -1]) < 0.02
 
 
-def test_pospischil_gradients_rejected(device):
-    cell = pospischil_cell()
-    g = torch.tensor([0.05], dtype=torch.float64, device=device, requires_grad=True)
-    with pytest.raises(NotImplementedError):
-        jx.integrate(cell, param_state=cell.data_set("Na_gNa", g, None), t_max=1.0)
+@pytest.mark.parametrize("solver", ["bwd_euler", "crank_nicolson"])
+def test_pospischil_gradients_vs_oracle_autograd(device, solver):
+    """Per-branch Na gNa and K gK (trainable) through the per-step path against autograd
+    through the oracle, and a batch of vt values (batched Pospischil parameters)."""
+    cell, rec = _tree()
+    cell.branch("all").make_trainable("Na_gNa", verbose=False)
+    cell.branch("all").make_trainable("K_gK", verbose=False)
+    rng = np.random.default_rng(5)
+    na = 0.05 * rng.uniform(0.7, 1.3, 16)
+    kk = 0.005 * rng.uniform(0.7, 1.3, 16)
+    params = [{"Na_gNa": torch.tensor(na, device=device, requires_grad=True)},
+              {"K_gK": torch.tensor(kk, device=device, requires_grad=True)}]
+    T = 200
+    cur = c1_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=torch.float64, device=device))
+    v = jx.integrate(cell, params, data_stimuli=ds, delta_t=DT, solver=solver, checkpoint_lengths=[50])
+    w = torch.tensor(rng.normal(size=tuple(v.shape)), device=device)
+    loss = (v * w).sum()
+    loss.backward()
+    ona = torch.tensor(na, requires_grad=True)
+    okk = torch.tensor(kk, requires_grad=True)
+    z = torch.zeros(32, dtype=torch.float64)
+    ov = {"Na_gNa": torch.cat([torch.repeat_interleave(ona, 4), z]),
+          "K_gK": torch.cat([z, torch.repeat_interleave(okk, 4)])}
+    ref = O.simulate(oracle_of_module(cell), None, DT, stim=torch.tensor(cur)[None], stim_comps=[0],
+                     rec_comps=rec, overrides=ov, solver=solver)
+    oloss = (ref * w.cpu()).sum()
+    oloss.backward()
+    print(f"pospischil {solver} loss", loss.item(), oloss.item(), flush=True)
+    np.testing.assert_allclose(loss.item(), oloss.item(), rtol=1e-9)
+    np.testing.assert_allclose(params[0]["Na_gNa"].grad.cpu().numpy(), ona.grad.numpy(), rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(params[1]["K_gK"].grad.cpu().numpy(), okk.grad.numpy(), rtol=1e-6, atol=1e-9)
+
+
+def test_pospischil_batched_parameters(device):
+    """A batch of threshold parameters vt (the per-step path) against the oracle."""
+    cell, rec = _tree()
+    T = 200
+    cur = c1_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=torch.float64, device=device))
+    vts = torch.tensor([[-60.0], [-57.0], [-63.0]], dtype=torch.float64)
+    ps = cell.data_set("vt", vts.to(device), None)
+    v = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT).cpu().numpy()
+    for b in range(3):
+        om = oracle_of_module(cell)
+        om.posp_params["vt"] = np.where(np.isnan(cell._node_data["vt"]), 0.0, float(vts[b, 0]))
+        ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0], rec_comps=rec).numpy()
+        err = np.abs(v[b] - ref).max()
+        print("pospischil batched vt", float(vts[b, 0]), "max err", err, flush=True)
+        assert v[b].shape == ref.shape and err < 1e-6, b
