@@ -57,7 +57,7 @@ BYTES_PER_COMP_STEP = {"f64": 64, "f32": 32}
 # 256 MiB device copy: fetch x 2.0, write x 1.0) for the headline launch shape and
 # workload; reported as roofline.traffic when the autotuner picks that shape.
 MEASURED_TRAFFIC = {
-    ("float64", 4096, 2000, "S=8, threads=1024, stream=True, gcoef=True"): 376.5e9 + 372.3e9,
+    ("float64", 4096, 2000, "S=8, threads=1024, stream=True, gcoef=True"): 376.4e9 + 372.3e9,
 }
 
 
