@@ -562,8 +562,6 @@ class _StepRunner:
 
     def __init__(self, module, params, all_states, param_state, delta_t, ext_nodes, ext_batch,
                  solver: str = "bwd_euler"):
-        if has_pospischil(module):
-            raise NotImplementedError("the per-step API does not carry Pospischil channel states")
         ref = first_float_tensor(params, param_state, all_states)
         device = ref.device if ref is not None else torch.get_default_device()
         dtype = ref.dtype if ref is not None else torch.get_default_dtype()
@@ -589,8 +587,12 @@ class _StepRunner:
                                  P.bmode, P.v0, tuple(P.gates0), P.stim, P.stim_scale, 1, delta_t, P.B, cfg,
                                  P.syn_params, P.syn_scale, P.syn_s0, **P.step_kw)
         self.mp = torch.as_tensor(P.st.mp, device=device)
-        hh = [k for k in ("HH_m", "HH_h", "HH_n") if k in module._node_data]
-        self.gate_names = hh
+        run = self.run
+        bufs = [(run.gm, "HH_m"), (run.gh, "HH_h"), (run.gn, "HH_n")]
+        if run.gx is not None:
+            bufs += [(run.gx[q], name) for q, name in enumerate(POSP_STATES)]
+        self.gate_bufs = [(b, k) for b, k in bufs if k in module._node_data]
+        self.gate_names = [k for _, k in self.gate_bufs]
         comp_of_k = torch.as_tensor(P.st.comp_of_k, device=device)
         self.is_comp = comp_of_k >= 0
         if P.st.syn is not None:
@@ -603,8 +605,8 @@ class _StepRunner:
         v = torch.as_tensor(all_states["v"], dtype=torch.float64, device=self.device)
         v = torch.broadcast_to(v, (*P.batch, self.n_nodes)).reshape(B, self.n_nodes)
         run.v.copy_(to_prog(v[:, self.mp], run.S))
-        for buf, name in zip((run.gm, run.gh, run.gn), ("HH_m", "HH_h", "HH_n")):
-            if name in self.gate_names and name in all_states:
+        for buf, name in self.gate_bufs:
+            if name in all_states:
                 x = torch.as_tensor(all_states[name], device=self.device).to(run.dtype)
                 x = torch.broadcast_to(x, (*P.batch, self.n_nodes)).reshape(B, self.n_nodes)[:, self.mp]
                 x = torch.nan_to_num(x, nan=0.0)
@@ -639,17 +641,16 @@ class _StepRunner:
         vn = torch.empty((B, self.n_nodes), dtype=self.dtype, device=self.device)
         vn[:, self.mp] = vk.to(self.dtype)
         out["v"] = vn.reshape(*P.batch, self.n_nodes)
-        for buf, name in zip((run.gm, run.gh, run.gn), ("HH_m", "HH_h", "HH_n")):
-            if name in self.gate_names:
-                gk = from_prog(buf, B).to(self.dtype)
-                gk = torch.where(self.is_comp[None], gk, torch.full_like(gk, float("nan")))
-                gn = torch.full((B, self.n_nodes), float("nan"), dtype=self.dtype, device=self.device)
-                gn[:, self.mp] = gk
-                if name in all_states:
-                    prev = torch.as_tensor(all_states[name], dtype=self.dtype, device=self.device)
-                    prev = torch.broadcast_to(prev, (*P.batch, self.n_nodes)).reshape(B, self.n_nodes)
-                    gn = torch.where(torch.isnan(prev), prev, gn)
-                out[name] = gn.reshape(*P.batch, self.n_nodes)
+        for buf, name in self.gate_bufs:
+            gk = from_prog(buf, B).to(self.dtype)
+            gk = torch.where(self.is_comp[None], gk, torch.full_like(gk, float("nan")))
+            gn = torch.full((B, self.n_nodes), float("nan"), dtype=self.dtype, device=self.device)
+            gn[:, self.mp] = gk
+            if name in all_states:
+                prev = torch.as_tensor(all_states[name], dtype=self.dtype, device=self.device)
+                prev = torch.broadcast_to(prev, (*P.batch, self.n_nodes)).reshape(B, self.n_nodes)
+                gn = torch.where(torch.isnan(prev), prev, gn)
+            out[name] = gn.reshape(*P.batch, self.n_nodes)
         if run.syn_s is not None:
             out.update(_syn_states_out(P.st, from_prog(run.syn_s, B), P.batch, self.device, self.dtype))
         return out
@@ -687,7 +688,7 @@ def build_init_and_step_fn(module, voltage_solver: str = "jaxley.dhs", solver: s
         states = {}
         inp = _assemble(module, params, param_state, None, delta_t, delta_t, None, device, dtype,
                         module_stimuli=False)
-        for k in ["v"] + [k for k in ("HH_m", "HH_h", "HH_n") if k in inp.vals]:
+        for k in ["v"] + [k for k in ("HH_m", "HH_h", "HH_n") + POSP_STATES if k in inp.vals]:
             x = inp.vals[k]
             out = torch.full((*x.shape[:-1], n), float("nan"), dtype=dtype, device=device)
             out[..., comp_node] = x

@@ -143,3 +143,26 @@ def test_pospischil_batched_parameters(device):
         err = np.abs(v[b] - ref).max()
         print("pospischil batched vt", float(vts[b, 0]), "max err", err, flush=True)
         assert v[b].shape == ref.shape and err < 1e-6, b
+
+
+def test_pospischil_step_loop_matches_integrate(device):
+    """`build_init_and_step_fn` carries the Pospischil gates between steps: T single
+    steps equal one `integrate` of T steps."""
+    cell, rec = _tree()
+    T = 120
+    cur = torch.tensor(c1_stimulus(T), dtype=torch.float64, device=device)
+    init_fn, step_fn = jx.build_init_and_step_fn(cell)
+    ps = cell.data_set("HH_gK", torch.tensor([[0.04]], dtype=torch.float64, device=device), None)
+    states, all_params = init_fn([], param_state=ps)
+    assert "Na_m" in states and "Km_p" in states
+    node = int(cell._comp_node[0])
+    out = [states["v"][..., node]]
+    for t in range(T):
+        states = step_fn(states, all_params, {"i": cur[t:t + 1]}, {"i": np.array([node])}, delta_t=DT)
+        out.append(states["v"][..., node])
+    v_step = torch.stack([x.reshape(-1)[0] for x in out])
+    ds = cell.branch(0).comp(0).data_stimulate(cur)
+    v_int = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT)[..., 0, :].reshape(T + 1)
+    err = float((v_step - v_int).abs().max())
+    print("pospischil step loop vs integrate", err, flush=True)
+    assert err < 1e-9
