@@ -330,7 +330,7 @@ def _compartments(g: _SwcGraph, branches, types, ignore, bp, ncomp: int, min_rad
     return comp, n, cedges
 
 
-def _solve_order(n_comp: int, cedges, ncomp: int):
+def _solve_order(n_comp: int, cedges):
     """Depth-first numbering of the compartment graph from its lowest-numbered tip:
     new compartment index per old one, new branch index per old compartment, the
     parent branch of every new branch and which compartments were walked against the
@@ -397,7 +397,7 @@ def read_swc(fname: str, ncomp: int, max_branch_len: Optional[float] = None,
     g, branches, types, ignore, bp = _trace_branches(g, max_branch_len, ignore_swc_tracing_interruptions,
                                                      relevant_type_ids)
     comp, n, cedges = _compartments(g, branches, types, ignore, bp, ncomp, min_radius)
-    new_comp, branch, parents, flipped = _solve_order(n, cedges, ncomp)
+    new_comp, branch, parents, flipped = _solve_order(n, cedges)
     order = np.empty(n, dtype=np.int64)          # old compartment at each new index
     for old, new in new_comp.items():
         order[new] = old
