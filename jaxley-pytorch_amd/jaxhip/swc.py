@@ -123,7 +123,7 @@ def morph_attrs_from_xyzr(xyzr: np.ndarray, min_radius: Optional[float], ncomp: 
 
 class _SwcGraph:
     """Nodes in insertion order with attributes; directed edges parent -> child with
-    length `l`; `adj` the undirected neighbour sets."""
+    their length; `undirected()` gives neighbour sets and lengths both ways."""
 
     def __init__(self):
         self.nodes: Dict[int, Dict] = {}

@@ -156,3 +156,16 @@ def test_axial_conductances_match_oracle(n_cells):
     assert np.array_equal(mod._comp_node, om.internal)
     assert all(np.array_equal(np.asarray(t[k]), np.asarray(om.tables[k])) for k in ("source", "sink", "type", "ordered"))
     np.testing.assert_allclose(g, om.params()["g_axial"].numpy(), rtol=1e-13)
+
+
+def test_single_branch_with_repeated_point():
+    """tests/test_swc.py:161-169's morphology: one branch whose last two points coincide
+    (a zero-length segment): finite, positive geometry for any compartment count."""
+    for ncomp in (1, 3, 4):
+        df = jx.read_swc(swc("morph_single_branch.swc"), ncomp=ncomp).nodes
+        assert len(df) == ncomp
+        for k in ("area", "resistive_load_in", "resistive_load_out", "radius", "length"):
+            x = df[k].to_numpy()
+            assert np.all(np.isfinite(x)) and np.all(x > 0), (ncomp, k)
+        np.testing.assert_allclose(df.radius.to_numpy(), 8.119, rtol=1e-12)
+        np.testing.assert_allclose(df.area.sum(), 2 * math.pi * 8.119 * 2.0, rtol=1e-9)

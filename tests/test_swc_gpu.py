@@ -124,3 +124,13 @@ def test_swc_gradients_vs_oracle_autograd(device):
     print("swc loss", loss.item(), oloss.item(), flush=True)
     np.testing.assert_allclose(loss.item(), oloss.item(), rtol=1e-9)
     np.testing.assert_allclose(params[0]["HH_gNa"].grad.cpu().numpy(), ov.grad.numpy(), rtol=1e-6, atol=1e-9)
+
+
+def test_single_branch_swc_no_nan(device):
+    """tests/test_swc.py:161-169 (without set_ncomp): no NaN in a 1 ms simulation."""
+    cell = jx.read_swc(swc("morph_single_branch.swc"), ncomp=4)
+    cell.insert(jx.HH())
+    cell.branch(0).comp(0).record()
+    ds = cell.branch(0).comp(0).data_stimulate(torch.zeros(41, dtype=torch.float64, device=device))
+    v = jx.integrate(cell, data_stimuli=ds, delta_t=0.025)
+    assert not torch.isnan(v).any()
