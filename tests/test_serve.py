@@ -89,3 +89,30 @@ def test_served_with_preparation_processes(device):
 
     for a, b in zip(run(prep_threads=1), run(prep_processes=2)):
         assert torch.equal(a, b)
+
+
+def test_served_swc_cells(device):
+    """Requests of cells read from SWC files (traced geometry: input and output loads
+    differ) served together equal `integrate` of each."""
+    from jaxhip.serve import SimRequest
+    from test_swc import swc
+
+    srv = SimServer(device, dtype=torch.float64, delta_t=DT, hbm_budget=256 << 20, max_steps=4001,
+                    max_stim_entries=64, chunk_steps=64)
+    reqs = []
+    for f, nc, t_max in (("morph_ca1_n120_250.swc", 2, 8.0), ("morph_ca1_n120.swc", 2, 5.0),
+                         ("morph_ca1_n120_single_point_soma.swc", 1, 6.0)):
+        cell = jx.read_swc(swc(f), ncomp=nc, max_branch_len=300.0)
+        cell.insert(jx.HH())
+        cell.soma.branch(0).loc(1.0).record()
+        T = int(t_max // DT + 1)
+        cur = torch.tensor(c1_stimulus(T), dtype=torch.float64, device=device) * 2
+        ds = cell.soma.branch(0).loc(1.0).data_stimulate(cur)
+        reqs.append((cell, ds, t_max))
+    hs = [srv.submit(SimRequest(c, t_max=t, data_stimuli=ds)) for c, ds, t in reqs]
+    srv.run_until_idle()
+    for h, (c, ds, t) in zip(hs, reqs):
+        v = jx.integrate(c, data_stimuli=ds, t_max=t, delta_t=DT)
+        err = float((v[0].cpu() - h.result.cpu()).abs().max())
+        print("served swc vs integrate", err, flush=True)
+        assert v.shape[-1] == h.result.shape[-1] and err < 1e-8
