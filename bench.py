@@ -22,6 +22,8 @@ Also reported (not the headline value):
               trainable conductances, float64);
   network   - 1024 ball-and-stick cells with 32768 KineticSynapses, B = 64, 4000 steps;
               network_fwd_adj: forward + adjoint with every gS trainable;
+  swc       - the CA1 morphology of the reference's SWC tests (4 compartments per
+              branch), B = 2048, T = 2000, float64;
   served    - 512 requests of random trees (20-2000 compartments, t_max 10-100 ms)
               arriving as a Poisson stream, continuous batching: p50 / p99 latency and
               throughput; served_burst: the same requests all at time 0 (capacity);
@@ -215,6 +217,35 @@ def bench_network(device, n_cells=1024, k=32, B=64, T=4000, dtype=torch.float32)
            "dtype": str(dtype).replace("torch.", ""),
            "mean_spikes_per_soma": float(spikes(v.double()).float().mean())}
     log(f"[bench] network {n_cells} cells x {k} synapses, B={B}, T={T}: {sec*1e3:.1f} ms, "
+        f"{res['comp_steps_per_s']:.3e} comp-steps/s")
+    return res
+
+
+def bench_swc(device, ncomp=4, B=2048, T=2000, dtype=torch.float64):
+    """A traced morphology: the CA1 pyramidal cell of the reference's SWC tests
+    (tests/golden/swc/morph_ca1_n120.swc, read by jx.read_swc with `ncomp` compartments per
+    branch), HH everywhere, per-simulation gNa (B values), a 0.2 nA step into the soma,
+    T steps, the soma recorded; float64."""
+    cell = jx.read_swc(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "swc",
+                                    "morph_ca1_n120.swc"), ncomp=ncomp)
+    cell.insert(jx.HH())
+    cell.soma.branch(0).loc(1.0).record()
+    gna = torch.tensor(np.random.default_rng(3).uniform(0.09, 0.15, (B, 1)), dtype=dtype, device=device)
+    ps = cell.data_set("HH_gNa", gna, None)
+    cur = jx.step_current(1.0, 2.0, 0.2, DT, (T - 1) * DT).to(device=device, dtype=dtype)
+    ds = cell.soma.branch(0).loc(1.0).data_stimulate(cur)
+
+    def run():
+        with torch.no_grad():
+            return jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT)
+
+    sec = timed(run, 1, 1, device)
+    n_comp = cell._n_comp
+    T = int(run().shape[-1]) - 1
+    res = {"comp_steps_per_s": n_comp * T * B / sec, "ms": sec * 1e3, "batch": B, "compartments": n_comp,
+           "branches": len(cell.comb_parents), "time_steps": T, "dtype": str(dtype).replace("torch.", ""),
+           "morphology": "morph_ca1_n120.swc"}
+    log(f"[bench] swc CA1 {n_comp} compartments, B={B}, T={T}: {sec*1e3:.1f} ms, "
         f"{res['comp_steps_per_s']:.3e} comp-steps/s")
     return res
 
@@ -460,6 +491,8 @@ def main():
         out["fwd_adj_sweep"] = bench_fwd_adj_sweep(device, B, T)
         # ---- network: 1024 ball-and-stick cells, 32768 KineticSynapses, B = 64 ----
         out["network"] = bench_network(device)
+        # ---- a traced morphology (SWC): the CA1 cell, B = 2048, float64 ----
+        out["swc"] = bench_swc(device)
         out["network_fwd_adj"] = bench_network_fwd_adj(device)
         # ---- served: continuous batching of uneven requests ----
         if args.serve_requests > 0:
