@@ -169,3 +169,23 @@ def test_single_branch_with_repeated_point():
             assert np.all(np.isfinite(x)) and np.all(x > 0), (ncomp, k)
         np.testing.assert_allclose(df.radius.to_numpy(), 8.119, rtol=1e-12)
         np.testing.assert_allclose(df.area.sum(), 2 * math.pi * 8.119 * 2.0, rtol=1e-9)
+
+
+def test_read_swc_options():
+    """`min_radius` clips compartment radii (io/graph.py:405-411, morph_attributes.py:
+    63-75), `assign_groups=False` creates no groups, `relevant_type_ids` merges neurites
+    of ignored types into one branch (io/graph.py:36-55), and an unknown backend raises."""
+    base = jx.read_swc(swc("morph_ca1_n120_250.swc"), ncomp=2)
+    r = base.nodes.radius.to_numpy()
+    clipped = jx.read_swc(swc("morph_ca1_n120_250.swc"), ncomp=2, min_radius=1.0).nodes.radius.to_numpy()
+    np.testing.assert_array_equal(clipped, np.maximum(r, 1.0))
+    assert np.any(r < 1.0)
+    assert jx.read_swc(swc("morph_ca1_n120_250.swc"), ncomp=2, assign_groups=False).groups == {}
+    # An unbranched line of basal (3), soma (1), basal points: three branches, or one when
+    # no type is relevant.
+    assert len(jx.read_swc(swc("morph_soma_both_ends.swc"), ncomp=1).comb_parents) == 3
+    one = jx.read_swc(swc("morph_soma_both_ends.swc"), ncomp=1, relevant_type_ids=[])
+    assert len(one.comb_parents) == 1
+    np.testing.assert_allclose(one.nodes.length.to_numpy(), [2.2 + 1.0 + 2.6])
+    with pytest.raises(ValueError):
+        jx.read_swc(swc("morph_3_types.swc"), ncomp=1, backend="custom")
