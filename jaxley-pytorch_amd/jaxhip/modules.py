@@ -493,6 +493,16 @@ class _EdgeView:
                 return
         raise KeyError(key)
 
+    def record(self, state: str, verbose: bool = False):
+        """Record a synapse state or current (`i_<synapse>`) of the synapses in view, by
+        their index within the type (modules/base.py:2544-2563)."""
+        stype = self.e["type"]
+        known = {f"{stype.name}_{stype.state_name}"} if stype.state_name else set()
+        if state not in known | {f"i_{stype.name}"}:
+            raise KeyError(f"{state} is not a recognized state in this module.")
+        for i in self.idx:
+            self.base.recordings.append((state, int(i)))
+
     def make_trainable(self, key: str, init_val: Optional[Union[float, list]] = None, verbose: bool = True):
         """One trainable parameter for all synapses of the type view, or one per synapse of
         an `.edge(...)` view (modules/base.py:1625-1750); the initial value is the mean

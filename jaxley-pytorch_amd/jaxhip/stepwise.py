@@ -173,7 +173,8 @@ class _SynGroup:
         self.state_key = key if self.model == 0 else None
 
     def apply(self, v, s, lin, cst, dt):
-        s2, _, sl, sc = synapse_step(self.plan, v, s, self.params, self.area, dt, DIFF)
+        s2, i_syn, sl, sc = synapse_step(self.plan, v, s, self.params, self.area, dt, DIFF)
+        self.last_current = i_syn          # nA per synapse (operator order)
         return s2, lin.index_add(1, self.targets, sl), cst.index_add(1, self.targets, -sc)
 
 
@@ -254,6 +255,7 @@ class _TorchSynGroup:
         d1 = self.current(s2, pre_v + DIFF, post_v + DIFF, self.params) / self.area * 1e5
         vt = (d1 - d0) / DIFF
         ct = d0 - vt * post_v
+        self.last_current = d0 * self.area / 1e5     # nA per synapse
         return s2, lin.index_add(1, self.post, vt), cst.index_add(1, self.post, -ct)
 
 
@@ -270,6 +272,89 @@ def _syn_group(edge, syn_vals, area_node, n, B, device, dtype, forward_ad=False)
         raise NotImplementedError("gradients through Kinetic / SigmoidRate synapses need every "
                                   "target to hold the same number of synapses (regular layout)")
     return _TorchSynGroup(edge, syn_vals, area_node, B, device, dtype)
+
+
+class _Recorder:
+    """The reference's recorder (`_build_recorder`, integrate.py:53-84) over the per-step
+    path's states. Recordable: `v`; channel gates (NaN where the channel is absent);
+    channel membrane currents `i_<channel>` (mA / cm^2 at the step's starting voltage
+    with the new gates, summed per current name, zero where absent; base.py:3445-3502);
+    synapse states and synapse currents `i_<synapse>` (nA, after the state update;
+    network.py:462-465), indexed by the synapse's index within its type. Each state is
+    gathered with one index operation; the result keeps the order of the recordings."""
+
+    def __init__(self, module, chans, groups, comp_node, gate_keys, B, n, device, dtype):
+        self.B, self.dtype, self.device = B, dtype, device
+        names = [s for s, _ in module.recordings]
+        idx = np.array([i for _, i in module.recordings], dtype=np.int64)
+        self.n_rec = len(names)
+        cur_of_kid = {ch.kernel_id: ch.current_name for ch in module.channels}
+        gate_pos = {}
+        gi = 0
+        for kid in _GATE_ORDER:
+            if kid in chans:
+                for k in _CHANNEL_GATES[kid]:
+                    gate_pos[k] = (gi, chans[kid][2].cpu().numpy())
+                    gi += 1
+        syn_state = {g.state_key: j for j, g in enumerate(groups) if g.state_key is not None}
+        syn_cur = {f"i_{g.name}": j for j, g in enumerate(groups)}
+        self.channel_current = {}
+        self.aux_names = []
+        self.getters = []
+        order = []
+        for name in dict.fromkeys(names):
+            pos = np.nonzero(np.array(names) == name)[0]
+            order.append(pos)
+            ind = idx[pos]
+            if name == "v":
+                nodes = comp_node[torch.as_tensor(ind, device=device)]
+                self.getters.append(lambda v, g, s, a, nodes=nodes: v[:, nodes])
+            elif name in gate_pos:
+                gi, cidx = gate_pos[name]
+                col = np.full(int(module._n_comp), -1, dtype=np.int64)
+                col[cidx] = np.arange(len(cidx))
+                c = col[ind]
+                cols = torch.as_tensor(np.maximum(c, 0), device=device)
+                has = torch.as_tensor(c >= 0, device=device)
+                self.getters.append(lambda v, g, s, a, gi=gi, cols=cols, has=has: torch.where(
+                    has, g[gi][:, cols], torch.full_like(g[gi][:, cols], float("nan"))))
+            elif name in cur_of_kid.values():
+                for kid, cn in cur_of_kid.items():
+                    if cn == name:
+                        self.channel_current[kid] = name
+                self.aux_names.append(name)
+                nodes = comp_node[torch.as_tensor(ind, device=device)]
+                self.getters.append(lambda v, g, s, a, name=name, nodes=nodes: self._aux(a, name, nodes))
+            elif name in syn_state:
+                j = syn_state[name]
+                sel = groups[j].inv[torch.as_tensor(ind, device=device)]
+                self.getters.append(lambda v, g, s, a, j=j, sel=sel: s[j][:, sel])
+            elif name in syn_cur:
+                self.aux_names.append(name)
+                sel = torch.as_tensor(ind, device=device)
+                self.getters.append(lambda v, g, s, a, name=name, sel=sel: self._aux(a, name, sel))
+            else:
+                raise KeyError(f"{name} is not a recognized state in this module.")
+        self.needs_aux = bool(self.aux_names)
+        self.single = len(order) == 1
+        self.inverse = torch.as_tensor(np.argsort(np.concatenate(order)), device=device)
+
+    def needs(self, name: str) -> bool:
+        return name in self.aux_names
+
+    def _aux(self, aux, name, sel):
+        if aux is None or name not in aux:
+            return torch.zeros(1, len(sel), dtype=self.dtype, device=self.device)
+        return aux[name][:, sel]
+
+    def __call__(self, v, gates, syn_s, aux):
+        recs = []
+        for f in self.getters:
+            r = f(v, gates, syn_s, aux)
+            recs.append(r.expand(self.B, -1) if r.shape[0] != self.B else r)
+        if self.single:
+            return recs[0]
+        return torch.cat(recs, 1)[:, self.inverse]
 
 
 def has_tangents(tensors) -> bool:
@@ -291,11 +376,7 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     tables, ind = module._ensure_solver()
     n = module._n_nodes
     vals = {k: x.to(dtype) for k, x in inp.vals.items()}
-    rec_states = {s for s, _ in module.recordings}
-    if rec_states != {"v"}:
-        raise NotImplementedError("recordings of voltages ('v') only")
     comp_node = torch.as_tensor(module._comp_node, device=device)
-    rec_nodes = comp_node[torch.as_tensor([c for _, c in module.recordings], device=device)]
     geo = comp_geometry(vals)
     g_ax = axial_conductances(tables, module._comp_node, n, geo, vals["axial_resistivity"],
                               vals["capacitance"])
@@ -381,7 +462,17 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     g_full = torch.broadcast_to(flat(g_ax), (B, g_ax.shape[-1]))
     edges = _explicit_edges(tables, device)
 
-    def step(v, gates, syn_s, t):
+    rec = _Recorder(module, chans, groups, comp_node, gate_keys, B, n, device, dtype)
+
+    def add_current(aux, kid, nodes, g, gv, vn):
+        """The channel's membrane current g v - gv (mA / cm^2) at the step's starting
+        voltage with the new gates, summed per current name (base.py:3445-3502)."""
+        name = rec.channel_current.get(kid)
+        if name is not None:
+            prev = aux.get(name, torch.zeros(B, n, dtype=dtype, device=device))
+            aux[name] = prev.index_add(1, nodes, torch.broadcast_to(g * vn - gv, (B, len(nodes))))
+
+    def step(v, gates, syn_s, t, aux=None):
         lin = torch.zeros(B, n, dtype=dtype, device=device)
         cst = torch.zeros(B, n, dtype=dtype, device=device)
         gates = list(gates)
@@ -395,14 +486,21 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
             gNa = p["HH_gNa"] * m**3 * h
             gK = p["HH_gK"] * nn**4
             g = gNa + gK + p["HH_gLeak"]
+            gv = gNa * p["HH_eNa"] + gK * p["HH_eK"] + p["HH_gLeak"] * p["HH_eLeak"]
             # i = g v - (gNa eNa + gK eK + gL eL): the slope of a current linear in v is g
             # (the reference's finite difference up to round-off).
             lin = lin.index_add(1, nodes, g)
-            cst = cst.index_add(1, nodes, gNa * p["HH_eNa"] + gK * p["HH_eK"] + p["HH_gLeak"] * p["HH_eLeak"])
+            cst = cst.index_add(1, nodes, gv)
+            if aux is not None:
+                add_current(aux, "HH", nodes, g, gv, vh)
         if "Leak" in chans:
             nodes, p, _ = chans["Leak"]
-            lin = lin.index_add(1, nodes, torch.broadcast_to(p["Leak_gLeak"], (B, len(nodes))))
-            cst = cst.index_add(1, nodes, torch.broadcast_to(p["Leak_gLeak"] * p["Leak_eLeak"], (B, len(nodes))))
+            gl = torch.broadcast_to(p["Leak_gLeak"], (B, len(nodes)))
+            gv = torch.broadcast_to(p["Leak_gLeak"] * p["Leak_eLeak"], (B, len(nodes)))
+            lin = lin.index_add(1, nodes, gl)
+            cst = cst.index_add(1, nodes, gv)
+            if aux is not None:
+                add_current(aux, "Leak", nodes, gl, gv, v[:, nodes])
         for kid in _GATE_ORDER[1:]:
             if kid in chans:
                 nodes, p, _ = chans[kid]
@@ -412,12 +510,16 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
                 gi += k
                 lin = lin.index_add(1, nodes, torch.broadcast_to(g, (B, len(nodes))))
                 cst = cst.index_add(1, nodes, torch.broadcast_to(g * e, (B, len(nodes))))
+                if aux is not None:
+                    add_current(aux, kid, nodes, g, g * e, v[:, nodes])
         lin = lin * 1000.0
         cst = cst * 1000.0
         new_s = []
         for grp, s in zip(groups, syn_s):
             s2, lin, cst = grp.apply(v, s, lin, cst, dt)
             new_s.append(s2)
+            if aux is not None and rec.needs(f"i_{grp.name}"):
+                aux[f"i_{grp.name}"] = grp.last_current[:, grp.inv]
         if stim is not None:
             cur = stim[..., t]
             cst = cst.index_add(1, stim_nodes, torch.broadcast_to(cur / area[:, stim_nodes] * 1e5, (B, len(stim_nodes))))
@@ -441,11 +543,20 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
         gates, syn_s = list(rest[:n_g]), list(rest[n_g:])
         recs = []
         for t in range(t0, t1):
-            v, gates, syn_s = step(v, gates, syn_s, t)
-            recs.append(v[:, rec_nodes])
+            aux = {} if rec.needs_aux else None
+            v, gates, syn_s = step(v, gates, syn_s, t, aux)
+            recs.append(rec(v, gates, syn_s, aux))
         return (v, *gates, *syn_s, torch.stack(recs, -1))
 
-    recs = [v[:, rec_nodes][..., None]]
+    init_aux = None
+    if rec.needs_aux:
+        init_aux = {}
+        for name in rec.aux_names:
+            x0 = inp.all_states.get(name) if inp.all_states is not None else None
+            if x0 is not None:
+                init_aux[name] = torch.broadcast_to(torch.as_tensor(x0, dtype=dtype, device=device),
+                                                    (*batch, x0.shape[-1])).reshape(B, -1)
+    recs = [rec(v, gates, syn_s, init_aux)[..., None]]
     state = (v, *gates, *syn_s)
     for t0 in range(0, T, K):
         t1 = min(t0 + K, T)
@@ -455,7 +566,7 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
             out = chunk(t0, t1, *state)
         state, r = out[:-1], out[-1]
         recs.append(r)
-    rec = torch.cat(recs, -1).reshape(*batch, len(rec_nodes), T + 1)
+    rec = torch.cat(recs, -1).reshape(*batch, rec.n_rec, T + 1)
     if not return_states:
         return rec
     v, gates, syn_s = state[0], list(state[1:1 + n_g]), list(state[1 + n_g:])

@@ -20,11 +20,13 @@ Pinning (what checks each part; tests/test_oracle.py, tests/test_host.py):
     indexer of tests/test_solver.py:78-86;
   * tree solve: dense `numpy.linalg.solve` (closed form) and its transpose;
   * IonotropicSynapse + network indexer: golden `test_net` trace;
-  * KineticSynapse / SigmoidRateSynapse: closed-form single-synapse step (the
-    exponential-Euler solution of a linear ODE with constant coefficients) and the
-    golden network path through IonotropicSynapse, which exercises the same
-    finite-difference synapse linearization. Their exact network traces are
-    "parity unpinned" beyond that (no reference test holds such a trace).
+  * KineticSynapse / SigmoidRateSynapse (tests/closed_forms.py): the state update
+    against the analytic solution of its linear ODE at a constant presynaptic voltage
+    (400 steps) and first-order convergence to a Runge-Kutta solution with a varying
+    one; the sigmoid's symmetry and midpoint; the finite-difference slope against the
+    analytic derivative; whole two-cell networks through `simulate` against the
+    closed-form backward-Euler recurrence of the postsynaptic membrane; the golden
+    network path through IonotropicSynapse exercises the same linearization.
 """
 from __future__ import annotations
 
@@ -521,6 +523,28 @@ SYN_DEFAULTS = {
     "IonotropicSynapse": {"gS": 1e-4, "e_syn": 0.0, "k_minus": 0.025, "v_th": -35.0, "delta": 10.0},
     "TestSynapse": {"gC": 1e-4},
 }
+
+
+def synapse_linearization(kind, s, pre_v, post_v, prm, area_post, dt, diff=1e-3, update=True):
+    """One synapse type's part of a step (modules/network.py:382-478): the state update
+    at the old voltages (when `update` and the type has a state), then the finite
+    difference linearization of i / area * 1e5 with BOTH voltages perturbed by `diff`
+    (network.py:452-457). Returns `(s_new, vt, ct, i)`: d0 = vt v_post + ct is the current
+    per unit area (uA / cm^2), i the current in nA (the `i_<synapse>` state,
+    network.py:462-465)."""
+    upd, cur, _, _ = SYNAPSES[kind]
+    if update and upd is not None:
+        s = upd(s, pre_v, prm, dt)
+    i0 = cur(s, pre_v, post_v, prm)
+    i1 = cur(s, pre_v + diff, post_v + diff, prm)
+    shape = torch.broadcast_shapes(i0.shape, i1.shape, post_v.shape)
+    d0 = torch.broadcast_to(i0, shape) / area_post * 100_000
+    d1 = torch.broadcast_to(i1, shape) / area_post * 100_000
+    vt = (d1 - d0) / diff
+    ct = d0 - vt * post_v
+    return s, vt, ct, torch.broadcast_to(i0, shape)
+
+
 HH_DEFAULTS = {"HH_gNa": 0.12, "HH_gK": 0.036, "HH_gLeak": 0.0003, "HH_eNa": 50.0, "HH_eK": -77.0, "HH_eLeak": -54.3}
 LEAK_DEFAULTS = {"Leak_gLeak": 1e-4, "Leak_eLeak": -70.0}
 
@@ -694,6 +718,45 @@ def explicit_step(v, lin_t, const_t, g_ax, tables: Dict, dt: float):
     return v + dt * f
 
 
+def _recorder(model, record, internal):
+    """`_build_recorder` (integrate.py:53-84) for `(state, index)` recordings: `v` by
+    compartment; HH gates by compartment (NaN where HH is absent); `i_HH` / `i_Leak`,
+    the channel's membrane current in mA / cm^2 at the step's starting voltage with the
+    new gates (base.py:3445-3502; 0 where absent and before the first step); synapse
+    states `<type>_s` / `<type>_c` and currents `i_<type>` in nA (network.py:462-465) by
+    index within the synapse type."""
+    hh_pos = {int(c): j for j, c in enumerate(model.hh_comps)}
+    leak_pos = {int(c): j for j, c in enumerate(model.leak_comps)}
+    syn_of = {sp["type"]: j for j, sp in enumerate(model.synapses)}
+
+    def get(state, idx, v, m, h, nn, syn_states, currents):
+        nan = torch.full(v.shape[:-1], float("nan"), dtype=F64)
+        zero = torch.zeros(v.shape[:-1], dtype=F64)
+        if state == "v":
+            return v[..., int(internal[idx])]
+        if state in ("HH_m", "HH_h", "HH_n"):
+            g = {"HH_m": m, "HH_h": h, "HH_n": nn}[state]
+            return torch.broadcast_to(g[..., hh_pos[idx]], zero.shape) if idx in hh_pos else nan
+        if state in ("i_HH", "i_Leak"):
+            pos = hh_pos if state == "i_HH" else leak_pos
+            if currents is None or state not in currents or idx not in pos:
+                return zero
+            return torch.broadcast_to(currents[state][..., pos[idx]], zero.shape)
+        if state.startswith("i_") and state[2:] in syn_of:
+            if currents is None or state not in currents:
+                return zero
+            return torch.broadcast_to(currents[state][..., idx], zero.shape)
+        kind = state.rsplit("_", 1)[0]
+        if kind in syn_of:
+            return torch.broadcast_to(syn_states[syn_of[kind]][..., idx], zero.shape)
+        raise KeyError(state)
+
+    def rec(v, m, h, nn, syn_states, currents=None):
+        return torch.stack([get(st, int(i), v, m, h, nn, syn_states, currents) for st, i in record], -1)
+
+    return rec
+
+
 def simulate(
     model: OracleModel,
     t_max: Optional[float],
@@ -704,7 +767,7 @@ def simulate(
     overrides: Optional[Dict] = None,
     voltage_solver: str = "jaxley.dhs.gpu",
     return_all: bool = False,
-    record_state: str = "v",
+    record: Optional[Sequence] = None,
     solver: str = "bwd_euler",
 ):
     """`jx.integrate` (integrate.py:396-666) + `_step_with_plan` (modules/base.py:3214-3400)
@@ -715,7 +778,8 @@ def simulate(
 
     `stim`: `(..., n_stim, time)` currents in nA (padded / truncated to
     `int(t_max // dt + 1)` steps as integrate.py:523-541 does). Recordings are
-    `(..., n_rec, 1 + steps)`: the initial state and every step."""
+    `(..., n_rec, 1 + steps)`: the initial state and every step; `record` (a list of
+    `(state, index)`, see `_recorder`) replaces the voltages at `rec_comps`."""
     overrides = overrides or {}
     if solver not in ("bwd_euler", "crank_nicolson", "fwd_euler"):
         raise ValueError(f"solver {solver!r}")
@@ -784,25 +848,19 @@ def simulate(
         const = torch.zeros_like(v)
         new_states = []
         for sp, s in zip(model.synapses, syn_states):
-            upd, cur, names, _ = SYNAPSES[sp["type"]]
+            _, _, names, _ = SYNAPSES[sp["type"]]
             prm = {k: torch.as_tensor(sp["params"][k], dtype=F64) for k in names}
             pre = torch.as_tensor(sp["pre"])
             post = torch.as_tensor(sp["post"])
-            pre_v, post_v = v[..., pre], v[..., post]
-            if update and upd is not None:
-                s = upd(s, pre_v, prm, dt)
+            s, vt, ct, i0 = synapse_linearization(sp["type"], s, v[..., pre], v[..., post], prm,
+                                                  area[..., post], dt, diff, update)
             new_states.append(s)
-            i0 = cur(s, pre_v, post_v, prm)
-            i1 = cur(s, pre_v + diff, post_v + diff, prm)
-            shape = torch.broadcast_shapes(i0.shape, i1.shape, post_v.shape)
-            a = area[..., post]
-            d0 = torch.broadcast_to(i0, shape) / a * 100_000
-            d1 = torch.broadcast_to(i1, shape) / a * 100_000
-            vt = (d1 - d0) / diff
-            ct = d0 - vt * post_v
+            currents[f"i_{sp['type']}"] = i0
             lin = lin.index_add(-1, post, torch.broadcast_to(vt, (*v.shape[:-1], len(post))))
             const = const.index_add(-1, post, torch.broadcast_to(-ct, (*v.shape[:-1], len(post))))
         return lin, const, new_states
+
+    currents: Dict[str, torch.Tensor] = {}
 
     def one_step(v, m, h, nn, syn_states, t):
         if stim_nodes is not None:
@@ -816,6 +874,7 @@ def simulate(
             vh = v[..., hh_nodes]
             m, h, nn = hh_update(m, h, nn, vh, dt)
             I0 = hh_current(m, h, nn, vh, hhp)
+            currents["i_HH"] = I0
             I1 = hh_current(m, h, nn, vh + diff, hhp)
             vt = (I1 - I0) / diff
             ct = I0 - vt * vh
@@ -824,6 +883,7 @@ def simulate(
         if len(model.leak_comps):
             vl = v[..., leak_nodes]
             I0 = leak_current(vl, lkp)
+            currents["i_Leak"] = I0
             I1 = leak_current(vl + diff, lkp)
             vt = (I1 - I0) / diff
             ct = I0 - vt * vl
@@ -883,10 +943,14 @@ def simulate(
         return x, m, h, nn, syn_states
 
     # Initial synapse currents are only states; they do not change the dynamics.
-    recs = [v[..., rec_nodes]]
+    if record is None:
+        rec_fn = lambda v, m, h, nn, syn_states, currents=None: v[..., rec_nodes]
+    else:
+        rec_fn = _recorder(model, record, internal)
+    recs = [rec_fn(v, m, h, nn, syn_states)]
     for t in range(T):
         v, m, h, nn, syn_states = one_step(v, m, h, nn, syn_states, t)
-        recs.append(v[..., rec_nodes])
+        recs.append(rec_fn(v, m, h, nn, syn_states, currents))
     out = torch.stack(recs, -1)
     if return_all:
         return out, {"v": v, "m": m, "h": h, "n": nn, "syn": syn_states}

@@ -92,3 +92,56 @@ def test_network_gradients_vs_oracle_autograd(device, synapse):
     assert err_v < 1e-6
     assert float(g1o.abs().max()) > 0 and float(g2o.abs().max()) > 0
     assert e1 < 1e-6 and e2 < 1e-6
+
+
+def _two_cell_net(synapse):
+    """Cell 0 held at -10 mV (Leak reversal, no input), cell 1 at -70 mV with Leak
+    (reversal -65 mV), one synapse 0 -> 1; compartments 10 um long, radius 1 um."""
+    cell = jx.Cell(jx.Branch(jx.Compartment(), 1), [-1])
+    cell.insert(jx.Leak())
+    net = jx.Network([cell, cell])
+    net.cell(0).set("Leak_eLeak", -10.0)
+    net.cell(0).set("v", -10.0)
+    net.cell(1).set("Leak_eLeak", -65.0)
+    net.cell(1).set("v", -70.0)
+    jx.connect(net.cell(0).branch(0).comp(0), net.cell(1).branch(0).comp(0), synapse)
+    net.cell(1).branch(0).comp(0).record()
+    return net
+
+
+def _zero_input(net, dtype, device):
+    """A zero current into cell 0 (fixes the device and dtype of the simulation)."""
+    return net.cell(0).branch(0).comp(0).data_stimulate(torch.zeros(401, dtype=dtype, device=device))
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-9), (torch.float32, 1e-3)])
+def test_two_cell_kinetic_closed_form(device, dtype, tol):
+    """The fused network step with a KineticSynapse against the closed-form backward
+    Euler recurrence of the postsynaptic membrane (exact synapse state at a constant
+    presynaptic voltage; tests/closed_forms.py): pins the state update, current, scaling
+    and implicit step independently of the oracle."""
+    import closed_forms as CF
+
+    net = _two_cell_net(jx.KineticSynapse())
+    net.KineticSynapse.set("KineticSynapse_gS", 1e-3)
+    net.KineticSynapse.set("KineticSynapse_s", 0.2)
+    v = jx.integrate(net, t_max=10.0, delta_t=DT, data_stimuli=_zero_input(net, dtype, device))
+    ref = CF.two_cell_kinetic(400, DT, -10.0, -70.0, 0.2, 1e-3, 0.0, 1e-4, -65.0, CF.cylinder_area(10.0, 1.0))
+    err = float(np.abs(v[0].double().cpu().numpy() - ref).max())
+    print(dtype, "kinetic closed form max err", err)
+    assert err < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-9), (torch.float32, 1e-3)])
+def test_two_cell_sigmoid_rate_closed_form(device, dtype, tol):
+    import closed_forms as CF
+
+    net = _two_cell_net(jx.SigmoidRateSynapse())
+    net.SigmoidRateSynapse.set("SigmoidRateSynapse_gS", 2e-3)
+    net.SigmoidRateSynapse.set("SigmoidRateSynapse_x_offset", -12.0)
+    v = jx.integrate(net, t_max=10.0, delta_t=DT, data_stimuli=_zero_input(net, dtype, device))
+    ref = CF.two_cell_sigmoid_rate(400, DT, -10.0, -70.0, 2e-3, -12.0, 0.25, 1e-4, -65.0,
+                                   CF.cylinder_area(10.0, 1.0))
+    err = float(np.abs(v[0].double().cpu().numpy() - ref).max())
+    print(dtype, "sigmoid-rate closed form max err", err)
+    assert err < tol

@@ -1,5 +1,6 @@
 """The oracle pinned against the reference's golden traces, closed forms and networkx."""
 import json
+import math
 from pathlib import Path
 
 import networkx as nx
@@ -8,6 +9,8 @@ import pytest
 import torch
 
 from oracle import jx_oracle as O
+
+import closed_forms as CF
 
 GOLD = json.loads((Path(__file__).parent / "golden" / "reference_traces.json").read_text())
 CUR = O.step_current(0.5, 1.0, 0.02, 0.025, 5.0)
@@ -134,16 +137,126 @@ def test_random_tree_solve_vs_dense():
         np.testing.assert_allclose(x.numpy(), np.linalg.solve(O.dense_matrix(d, l, u, pl), b.numpy()), rtol=1e-10)
 
 
-def test_kinetic_closed_form():
-    """Exponential Euler is exact for ds/dt = a (1 - s) - beta s at a fixed v_pre."""
-    p = {k: torch.tensor(v, dtype=torch.float64) for k, v in O.SYN_DEFAULTS["KineticSynapse"].items()}
-    vpre = torch.tensor(-10.0, dtype=torch.float64)
-    s = torch.tensor(0.1, dtype=torch.float64)
-    dt = 0.025
-    a = p["alpha"] / (1 + np.exp(-(vpre - p["v_th"]) / p["delta"]))
-    r = a + p["beta"]
-    exact = a / r + (s - a / r) * np.exp(-r * dt)
-    np.testing.assert_allclose(float(O.kinetic_update(s, vpre, p, dt)), float(exact), rtol=1e-14)
+KIN = {k: torch.tensor(v, dtype=torch.float64) for k, v in O.SYN_DEFAULTS["KineticSynapse"].items()}
+SIG = {k: torch.tensor(v, dtype=torch.float64) for k, v in O.SYN_DEFAULTS["SigmoidRateSynapse"].items()}
+
+
+@pytest.mark.parametrize("v_pre", [-70.0, -21.0, -10.0, 30.0])
+def test_kinetic_exact_at_constant_presynaptic_voltage(v_pre):
+    """Exponential Euler solves the linear ODE of a KineticSynapse exactly while v_pre is
+    constant: 400 steps land on the analytic s(t) at t = 400 dt (a wrong rate, sign or
+    steady state breaks it)."""
+    dt, s = 0.025, torch.tensor(0.05, dtype=torch.float64)
+    for _ in range(400):
+        s = O.kinetic_update(s, torch.tensor(v_pre, dtype=torch.float64), KIN, dt)
+    np.testing.assert_allclose(float(s), CF.kinetic_exact(0.05, v_pre, 400 * dt), rtol=1e-12)
+
+
+def test_kinetic_fixed_point():
+    v = torch.tensor(-12.0, dtype=torch.float64)
+    a = 1.0 * CF.logistic((-12.0 + 20.0) / 2.0)
+    s_inf = torch.tensor(a / (a + 0.2), dtype=torch.float64)
+    np.testing.assert_allclose(float(O.kinetic_update(s_inf, v, KIN, 0.025)), float(s_inf), rtol=1e-14)
+
+
+def test_kinetic_first_order_in_time():
+    """With a time-varying v_pre (held at its value at the start of each step, as the
+    reference's step does) the state converges to the ODE's solution (classical RK4 at
+    dt / 1000) at first order: halving dt halves the error."""
+    vp = lambda t: -20.0 + 15.0 * math.sin(2.0 * t)
+    t_end = 4.0
+    errs = []
+    for dt in (0.05, 0.025, 0.0125):
+        s = torch.tensor(0.1, dtype=torch.float64)
+        for i in range(int(round(t_end / dt))):
+            s = O.kinetic_update(s, torch.tensor(vp(i * dt), dtype=torch.float64), KIN, dt)
+        errs.append(abs(float(s) - CF.kinetic_rk4(0.1, vp, t_end, dt / 1000)))
+    assert errs[0] > 1e-5
+    for a, b in zip(errs, errs[1:]):
+        assert 1.8 < a / b < 2.2, errs
+
+
+def test_sigmoid_rate_linearization():
+    """SigmoidRateSynapse: at v_pre = x_offset the current is -gS / 2 (logistic(0) = 1/2);
+    currents mirrored about x_offset sum to -gS (logistic(-x) = 1 - logistic(x)); the
+    finite-difference slope (both voltages perturbed, network.py:452-457) equals the
+    derivative -gS slope s (1 - s) 1e5 / area to O(diff); nothing depends on v_post."""
+    gS, area = 2e-3, 150.0
+    prm = dict(SIG, gS=torch.tensor(gS, dtype=torch.float64))
+    x0 = float(SIG["x_offset"])
+    i_mid = O.sigmoid_rate_current(None, torch.tensor(x0, dtype=torch.float64), None, prm)
+    np.testing.assert_allclose(float(i_mid), -gS / 2, rtol=1e-15)
+    for u in (0.3, 7.0, 25.0):
+        up = O.sigmoid_rate_current(None, torch.tensor(x0 + u, dtype=torch.float64), None, prm)
+        dn = O.sigmoid_rate_current(None, torch.tensor(x0 - u, dtype=torch.float64), None, prm)
+        np.testing.assert_allclose(float(up + dn), -gS, rtol=1e-14)
+    for v_pre in (-75.0, -60.0, -52.0):
+        vts = []
+        for v_post in (-80.0, -30.0):
+            _, vt, ct, _ = O.synapse_linearization("SigmoidRateSynapse", None, torch.tensor(v_pre, dtype=torch.float64),
+                                                torch.tensor(v_post, dtype=torch.float64), prm,
+                                                torch.tensor(area, dtype=torch.float64), 0.025)
+            vts.append(float(vt))
+            d0 = -gS * CF.logistic((v_pre - x0) * 0.25) * 1e5 / area
+            np.testing.assert_allclose(float(vt) * v_post + float(ct), d0, rtol=1e-12)
+        sg = CF.logistic((v_pre - x0) * 0.25)
+        deriv = -gS * 0.25 * sg * (1 - sg) * 1e5 / area
+        assert vts[0] == vts[1]
+        assert abs(vts[0] - deriv) <= 1e-3 * abs(deriv) + 1e-12
+
+
+def test_kinetic_linearization_is_the_conductance():
+    """KineticSynapse: i = gS s (v_post - e_syn) is linear in v_post and independent of
+    v_pre, so the finite difference returns the conductance gS s 1e5 / area exactly (up
+    to round-off) and the state advances before the current is taken."""
+    gS, area, e_syn = 5e-4, 80.0, -80.0
+    prm = dict(KIN, gS=torch.tensor(gS, dtype=torch.float64), e_syn=torch.tensor(e_syn, dtype=torch.float64))
+    s0, v_pre, v_post = 0.3, -5.0, -61.0
+    s, vt, ct, _ = O.synapse_linearization("KineticSynapse", torch.tensor(s0, dtype=torch.float64),
+                                        torch.tensor(v_pre, dtype=torch.float64),
+                                        torch.tensor(v_post, dtype=torch.float64), prm,
+                                        torch.tensor(area, dtype=torch.float64), 0.025)
+    s_new = CF.kinetic_exact(s0, v_pre, 0.025)
+    np.testing.assert_allclose(float(s), s_new, rtol=1e-13)
+    np.testing.assert_allclose(float(vt), gS * s_new * 1e5 / area, rtol=1e-9)
+    np.testing.assert_allclose(float(ct), -gS * s_new * 1e5 / area * e_syn, rtol=1e-9)
+
+
+def _two_cells(kind, prm, s0=None):
+    """Two one-compartment cells (length 10 um, radius 1 um), Leak in both, cell 0 at
+    -10 mV (its Leak reversal), a synapse 0 -> 1."""
+    om = O.OracleModel([([-1], [1])] * 2)
+    om.leak_comps = np.array([0, 1])
+    om.leak_params = {"Leak_gLeak": np.array([1e-4, 1e-4]), "Leak_eLeak": np.array([-10.0, -65.0])}
+    om.v0 = np.array([-10.0, -70.0])
+    om.synapses = [{"type": kind, "pre": om.internal[[0]], "post": om.internal[[1]],
+                    "params": {k: np.array([v]) for k, v in prm.items()},
+                    "s0": None if s0 is None else np.array([s0])}]
+    return om
+
+
+def test_kinetic_network_closed_form():
+    """The oracle's network step with a KineticSynapse (state update, current, the
+    finite-difference linearization, the 1e5 / area scaling and the implicit step)
+    against the closed-form recurrence of the postsynaptic membrane."""
+    prm = dict(O.SYN_DEFAULTS["KineticSynapse"], gS=1e-3, e_syn=0.0)
+    om = _two_cells("KineticSynapse", prm, s0=0.2)
+    v = O.simulate(om, 10.0, 0.025, rec_comps=[0, 1])
+    ref = CF.two_cell_kinetic(400, 0.025, -10.0, -70.0, 0.2, 1e-3, 0.0, 1e-4, -65.0,
+                              CF.cylinder_area(10.0, 1.0))
+    np.testing.assert_allclose(v[1].numpy(), ref, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(v[0].numpy(), -10.0, rtol=0, atol=1e-12)
+    assert ref[-1] > -30.0          # the synapse moved the cell far from rest
+
+
+def test_sigmoid_rate_network_closed_form():
+    prm = dict(O.SYN_DEFAULTS["SigmoidRateSynapse"], gS=2e-3, x_offset=-12.0)
+    om = _two_cells("SigmoidRateSynapse", prm)
+    v = O.simulate(om, 10.0, 0.025, rec_comps=[1])
+    ref = CF.two_cell_sigmoid_rate(400, 0.025, -10.0, -70.0, 2e-3, -12.0, 0.25, 1e-4, -65.0,
+                                   CF.cylinder_area(10.0, 1.0))
+    np.testing.assert_allclose(v[0].numpy(), ref, rtol=0, atol=1e-10)
+    assert ref[-1] > -60.0          # gS > 0 depolarizes (synapses/sigmoid_rate.py:15-17)
 
 
 def test_x_over_expm1_removable_singularity():
@@ -363,3 +476,24 @@ def test_golden_radius_and_length_net():
                    "params": {k: np.array([v]) for k, v in p.items()}, "s0": np.array([0.2])}]
     v = O.simulate(m, None, 0.025, stim=torch.stack([CUR, CUR]), stim_comps=[2, 8], rec_comps=[2, 8])
     _check(v, "radius_and_length_net")
+
+
+def test_recorded_gates_and_currents_are_consistent():
+    """Recordings of gates and the HH membrane current (base.py:3445-3502: mA / cm^2 at
+    the step's starting voltage with the new gates) obey the backward-Euler step of a
+    single HH compartment: v' - v = -dt 1000 (i + g (v' - v)) / cm, g the total HH
+    conductance from the recorded gates (a current taken at v', in uA, or with the old
+    gates breaks it)."""
+    om = O.OracleModel([([-1], [1])])
+    om.hh_comps = np.array([0])
+    rec = [("v", 0), ("HH_m", 0), ("HH_h", 0), ("HH_n", 0), ("i_HH", 0)]
+    cur = O.step_current(0.5, 2.0, 0.05, 0.025, 10.0)[None]
+    out = O.simulate(om, 10.0, 0.025, stim=cur * 0, stim_comps=[0], record=rec).numpy()
+    v, m, h, n, i = out
+    dt = 0.025
+    g = 0.12 * m[1:] ** 3 * h[1:] + 0.036 * n[1:] ** 4 + 0.0003
+    dv = v[1:] - v[:-1]
+    np.testing.assert_allclose(dv, -dt * 1000.0 * (i[1:] + g * dv), rtol=0, atol=1e-11)
+    assert i[0] == 0.0 and np.abs(i[1:]).max() > 1e-4
+    # The initial gates are the reference's defaults (hh.py:30-34).
+    np.testing.assert_allclose([m[0], h[0], n[0]], 0.2)
