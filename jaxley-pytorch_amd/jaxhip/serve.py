@@ -42,6 +42,8 @@ from . import _lib
 from .fused import LaunchConfig, items_per_thread, lds_bytes
 from .integrate import _assemble, _prepare, _require_gpu
 
+_LDS_MAX = 160 * 1024   # bytes of LDS one workgroup may use on gfx950
+
 
 class _Arena:
     """First-fit allocator over [0, capacity) in units of one element."""
@@ -303,6 +305,8 @@ class SimServer:
         if (L.n > self.node_cap or L.aux_size + 4 > self.aux.numel()
                 or (L.T + 1) * max(L.n_rec, 1) > self.rec_cap or L.n_stim > self.stim_col.numel()):
             raise ValueError("the request does not fit the server's HBM budget")
+        if not self._candidates(L.n, P.sched):
+            raise ValueError(f"the request's tree ({L.n} nodes) does not fit a workgroup's LDS")
         h.T = P.T
         h._P = P
 
@@ -408,18 +412,33 @@ class SimServer:
         h._P = None
 
     # ---------------------------------------------------------------- launches
+    def _candidates(self, n_max: int, sched_max: int) -> List[LaunchConfig]:
+        """Launch shapes that can run a chunk whose largest tree has `n_max` nodes and a
+        staged schedule of `sched_max` bytes: register-resident shapes whose registers
+        hold the tree and whose LDS (schedule, coefficients, system) fits, then the
+        streaming kernel (coupling coefficients in LDS, or read from L2 when only that
+        fits)."""
+        out = []
+        regs_lds = lds_bytes(n_max, 0, 1, self.lsize, sched_max)
+        for thr in self.thread_order:
+            if items_per_thread(n_max, 1, thr, self.gsize, self.lsize) and regs_lds <= _LDS_MAX:
+                out.append(LaunchConfig(1, thr))
+        g = regs_lds > _LDS_MAX
+        if lds_bytes(n_max, 0, 1, self.lsize, sched_max, g) <= _LDS_MAX:
+            out.append(LaunchConfig(1, 1024, stream=True, gcoef=g))
+        return out
+
+    def _fits(self, cfg: LaunchConfig, n_max: int, sched_max: int) -> bool:
+        return any(c == cfg for c in self._candidates(n_max, sched_max))
+
     def _config(self, n_max: int, sched_max: int) -> LaunchConfig:
         # A program is one simulation: the first thread count of `thread_order` whose
         # registers hold the largest tree (small workgroups let several programs share a
-        # CU, which hides the latency of each program's tree solve).
-        for thr in self.thread_order:
-            if items_per_thread(n_max, 1, thr, self.gsize, self.lsize):
-                cfg = LaunchConfig(1, thr)
-                break
-        else:
-            g = lds_bytes(n_max, 0, 1, self.lsize, sched_max) > 160 * 1024
-            cfg = LaunchConfig(1, 1024, stream=True, gcoef=g)
-        return cfg
+        # CU, which hides the latency of each program's tree solve), else streaming.
+        cands = self._candidates(n_max, sched_max)
+        if not cands:
+            raise ValueError(f"a tree of {n_max} nodes does not fit a workgroup's LDS")
+        return cands[0]
 
     def _args(self, progs_dev, n_progs, cfg, n_max, sched_max):
         a = _lib.FusedArgs()
@@ -456,20 +475,25 @@ class SimServer:
             t0 = L.handle.steps_done
             t1 = min(t0 + self.chunk_steps, L.T)
             progs[r] = [L.slot, 1, t0, t1, L.node_off, L.slot, L.rec_off + t0 * max(L.n_rec, 1), 0]
-            L.handle.steps_done = t1
             if t1 >= L.T:
                 finishing.append(L)
         n_max = max(L.n for L in lives)
         sched_max = max(L.P.sched for L in lives)
-        cfg = self._cfg_cache.get(n_max.bit_length())
-        if cfg is None:
+        # Shapes are cached per size class; a cached shape is used only if it still fits
+        # (a larger tree of the same class may need the streaming kernel).
+        key = n_max.bit_length()
+        cfg = self._cfg_cache.get(key)
+        if cfg is None or not self._fits(cfg, n_max, sched_max):
             cfg = self._tune(n_max, sched_max, progs) if self.autotune else self._config(n_max, sched_max)
-            self._cfg_cache[n_max.bit_length()] = cfg
+            self._cfg_cache[key] = cfg
         self.stream.wait_stream(self.up_stream)
         host = torch.from_numpy(progs).pin_memory()
         dev = host.to(self.device, non_blocking=True)
         a = self._args(dev, len(lives), cfg, n_max, sched_max)
         _lib.check(self._kernel()(a, _lib.stream_ptr(self.device)), "fused_forward (served chunk)")
+        # Only a launched chunk advances its simulations.
+        for r, L in enumerate(lives):
+            L.handle.steps_done = int(progs[r, 3])
         ev = torch.cuda.Event()
         ev.record(self.stream)
         self.inflight.append((ev, finishing, host, dev))
@@ -480,12 +504,9 @@ class SimServer:
 
     def _tune(self, n_max, sched_max, progs) -> LaunchConfig:
         """Measure the candidate shapes on copies of the live state (a few steps)."""
-        cands = []
-        for thr in self.thread_order:
-            if items_per_thread(n_max, 1, thr, self.gsize, self.lsize):
-                cands.append(LaunchConfig(1, thr))
-        g = lds_bytes(n_max, 0, 1, self.lsize, sched_max) > 160 * 1024
-        cands.append(LaunchConfig(1, 1024, stream=True, gcoef=g))
+        cands = self._candidates(n_max, sched_max)
+        if not cands:
+            raise ValueError(f"a tree of {n_max} nodes does not fit a workgroup's LDS")
         saved = [t.clone() for t in (self.v, self.gm, self.gh, self.gn, self.rec)]
         p = progs.copy()
         p[:, 3] = np.minimum(p[:, 2] + 16, p[:, 3])

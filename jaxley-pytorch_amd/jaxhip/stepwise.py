@@ -566,9 +566,9 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
             out = chunk(t0, t1, *state)
         state, r = out[:-1], out[-1]
         recs.append(r)
-    rec = torch.cat(recs, -1).reshape(*batch, rec.n_rec, T + 1)
+    out_rec = torch.cat(recs, -1).reshape(*batch, rec.n_rec, T + 1)
     if not return_states:
-        return rec
+        return out_rec
     v, gates, syn_s = state[0], list(state[1:1 + n_g]), list(state[1 + n_g:])
     out = {"v": v.reshape(*batch, n)}
     gi = 0
@@ -581,4 +581,4 @@ def integrate_stepwise(module, inp, dt: float, checkpoint_lengths: Optional[List
     for grp, s in zip(groups, syn_s):
         if grp.state_key is not None:
             out[grp.state_key] = s[:, grp.inv].reshape(*batch, -1)
-    return rec, out
+    return out_rec, out

@@ -233,3 +233,28 @@ def test_synapse_step_operators_fake_rules():
         assert [tuple(x.shape) for x in g] == [(B, n), (B, E), (6, E)]
         gv, *gp = torch.autograd.grad(outs[2].sum() + outs[1].sum(), [v] + params)
         assert gv.shape == (B, n) and all(x.shape == (E,) for x in gp)
+
+
+def test_server_launch_shapes_respect_lds():
+    """The server's launch shapes for a float64 tree of ~5.5k nodes: no register-resident
+    shape (its coefficients and system exceed the LDS), the streaming kernel with the
+    coupling coefficients read from L2; a tree whose streaming system exceeds the LDS
+    has no shape (and is rejected at acceptance)."""
+    from types import SimpleNamespace
+
+    from jaxhip.fused import lds_bytes
+    from jaxhip.serve import SimServer
+
+    fake = SimpleNamespace(gsize=8, lsize=8, thread_order=(256, 512, 1024))
+    fake._candidates = lambda n, s: SimServer._candidates(fake, n, s)
+    small = SimServer._candidates(fake, 900, 4096)
+    assert small and not small[0].stream
+    n, sched = 5500, 16 * 1400
+    assert lds_bytes(n, 0, 1, 8, sched) > 160 * 1024
+    cands = SimServer._candidates(fake, n, sched)
+    assert [c.stream for c in cands] == [True] and cands[0].gcoef
+    assert lds_bytes(n, 0, 1, 8, sched, True) <= 160 * 1024
+    assert SimServer._config(fake, n, sched) == cands[0]
+    assert SimServer._fits(fake, small[0], n, sched) is False
+    big = 12000
+    assert SimServer._candidates(fake, big, 16 * 3000) == []
