@@ -318,6 +318,18 @@ int jxhip_fused_forward_f64(const jxhip_fused_args* args, void* stream);
  * and LDS in float32). Meets float64-level trace accuracy at float32 solve cost. */
 int jxhip_fused_forward_mixed(const jxhip_fused_args* args, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc). The same
+ * forward as jxhip_fused_forward_f64, but the streaming kernel's scratch (work_ax, which
+ * must be NULL here) is a 64 MiB array inside the library, one per device, held until the
+ * process ends: state_elems (the number of elements of the state arrays, i.e. the
+ * largest state_off + n * S over the programs) must be <= 8388608, else the call fails
+ * with hipErrorInvalidValue. Calls nothing that allocates. The first call on a device
+ * looks the array up (hipGetSymbolAddress) with that device current; make it outside a
+ * graph capture. Calls that overlap on one device share the array and race: order them
+ * on one stream. */
+int jxhip_fused_forward_noalloc_f64(const jxhip_fused_args* args, int64_t state_elems, void* stream);
+
 /* Reverse pass of jxhip_fused_forward_* for HH / Leak cells (no synapses, no stimulus
  * gradient): for every K-step segment, from the last to the first, recomputes the
  * segment from its checkpoint (storing the trajectory in work_*), then runs the adjoint

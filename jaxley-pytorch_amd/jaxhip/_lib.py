@@ -77,8 +77,10 @@ _SIGS = {
     "jxhip_fused_adjoint": ([C.POINTER(FusedArgs), C.POINTER(AdjointArgs), _p], C.c_int),
     "jxhip_fused_forward_mixed": ([C.POINTER(FusedArgs), _p], C.c_int),
     "jxhip_fused_adjoint_mixed": ([C.POINTER(FusedArgs), C.POINTER(AdjointArgs), _p], C.c_int),
+    "jxhip_fused_forward_noalloc_f64": ([C.POINTER(FusedArgs), _i64, _p], C.c_int),
 }
-UNTYPED = ("jxhip_version", "jxhip_error_string", "jxhip_fused_forward_mixed", "jxhip_fused_adjoint_mixed")
+UNTYPED = ("jxhip_version", "jxhip_error_string", "jxhip_fused_forward_mixed", "jxhip_fused_adjoint_mixed",
+           "jxhip_fused_forward_noalloc_f64")
 TYPED = {k for k in _SIGS if k not in UNTYPED}
 
 
