@@ -342,9 +342,13 @@ int jxhip_fused_forward_noalloc_f64(const jxhip_fused_args* args, int64_t state_
  *   g_v0:    gradient of the initial voltages, program layout
  *   work_*:  per program [K + 1][node][S] at (program index) * work_stride
  *   lam_*:   adjoint gates, program layout (scratch)
- * Launch limits of the reverse pass: fwd->threads <= 512 and
- * fwd->lds_bytes = (3 * n * S + 2 * n) * sizeof(solve type) <= 160 KiB; the forward that
- * wrote the checkpoints must have used the same S. */
+ * Launch of the reverse pass (fields of fwd): threads <= 512; items_per_thread = the voltage
+ * adjoints every thread keeps in VGPRs, 1, 2, 4, 8, 12 or 16 with n * S <= items_per_thread
+ * * threads, or 0 to keep them in lam_v; coef_global = 1 reads the tree's coupling
+ * coefficients from node_f (L2) instead of an LDS copy; lds_bytes = staged schedule +
+ * 2 * n * S * sizeof(solve type) + 4 * n (+ 2 * n * sizeof(solve type) with LDS
+ * coefficients) <= 160 KiB. The forward that wrote the checkpoints must have used the
+ * same S. */
 typedef struct jxhip_adjoint_args {
   const void* g_rec;
   void* g_param[JXHIP_NPARAM];
@@ -358,6 +362,8 @@ typedef struct jxhip_adjoint_args {
   void* lam_n;
   int64_t work_stride;
   int64_t n_steps;          /* total steps of the forward */
+  void* lam_v;              /* adjoint of the voltages, program layout (scratch); required
+                               when fwd->items_per_thread == 0, unused otherwise */
 } jxhip_adjoint_args;
 
 int jxhip_fused_adjoint_f32(const jxhip_fused_args* fwd, const jxhip_adjoint_args* adj,

@@ -1245,6 +1245,7 @@ struct Adj {
   G* lm;
   G* lh;
   G* ln;
+  G* lv;  // adjoint of the voltages when not kept in VGPRs (IPT == 0)
   int64_t work_stride;
 };
 
@@ -1276,26 +1277,45 @@ __device__ __forceinline__ T d_gate_dv(T x, T dt, T al, T be, T dal, T dbe, T& e
   return dxa * dal + dxb * dbe;
 }
 
-template <typename G, typename L>
+// Reverse pass. Every thread owns the items i = tid + j * blockDim for the whole launch
+// and keeps their voltage adjoints in VGPRs (IPT > 0 items per thread) or in HBM
+// (IPT == 0: g.lv, program layout, for programs too large for the registers); LDS holds
+// the staged schedule,
+// the tree's coupling coefficients (or, kGC, they are read from the node table in L2),
+// the system of the transposed solve and, per node, its first recording. Per K-step
+// segment (last first): recompute the segment from its checkpoint into the work arrays,
+// then run the adjoint of each step backwards.
+template <typename G, typename L, int IPT, bool kGC>
 __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
   const int lS = a.log2S, S = a.S;
   const int nS = tp.n << lS;
-  // The schedule and the coefficient pairs first (aligned for their vector loads), then
-  // the system and the adjoint of the voltages.
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (IPT > 0 && nS > IPT * nthr) return;  // the host sizes IPT for every program
+  constexpr int kJ = IPT > 0 ? IPT : 0x7fffffff;  // item loop bound (items end at nS)
+  constexpr int kCS = kGC ? 1 : 0;
   const SchedSize z = schedule_size(a, tp);
   Lds<L> Ld;
   Ld.sc = stage_schedule(a, tp, z, smem_raw);
-  Ld.cl = reinterpret_cast<L*>(smem_raw + z.bytes);
-  Ld.xs = Ld.cl + 2 * tp.n;
+  unsigned char* lp = smem_raw + z.bytes;
+  if constexpr (kGC) {
+    Ld.cl = const_cast<L*>(a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS);
+  } else {
+    Ld.cl = reinterpret_cast<L*>(lp);
+    lp += 2 * static_cast<size_t>(tp.n) * sizeof(L);
+    load_coefficients(a, tp, Ld.cl);
+  }
+  Ld.xs = reinterpret_cast<L*>(lp);
   Ld.ds = Ld.xs + nS;
-  L* lv = Ld.ds + nS;  // adjoint of the voltages
-  Ld.tl = lv;          // unused (no synapses in the adjoint)
-  Ld.tc = lv;
-  load_coefficients(a, tp, Ld.cl);
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  int* recf = reinterpret_cast<int*>(Ld.ds + nS);  // per node: first recording, or kNone
+  Ld.tl = Ld.xs;  // unused (no synapses in the reverse pass)
+  Ld.tc = Ld.xs;
+  constexpr int kNone = 0x7fffffff;
+  for (int k = tid; k < tp.n; k += nthr) recf[k] = kNone;
+  __syncthreads();
+  for (int r = tid; r < tp.n_rec; r += nthr) atomicMin(&recf[a.rec_node[tp.rb + r]], r);
   const G dt = a.dt;
   const int64_t nsteps = pg.t1 - pg.t0;
   const int64_t K = a.ckpt_every;
@@ -1304,12 +1324,40 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   G* lm = g.lm + pg.soff;
   G* lh = g.lh + pg.soff;
   G* ln = g.ln + pg.soff;
-  for (int i = tid; i < nS; i += nthr) {
-    lv[i] = L(0);
+  L lvr[IPT > 0 ? IPT : 1];  // adjoint of the voltages of the thread's items (IPT > 0)
+  G* lvg = g.lv + pg.soff;    // or in HBM (IPT == 0)
+  auto lv_get = [&](int jj, int i) -> L {
+    if constexpr (IPT > 0) {
+      return lvr[jj];
+    } else {
+      return static_cast<L>(lvg[i]);
+    }
+  };
+  auto lv_set = [&](int jj, int i, L x) {
+    if constexpr (IPT > 0) {
+      lvr[jj] = x;
+    } else {
+      lvg[i] = static_cast<G>(x);
+    }
+  };
+#pragma unroll 1
+  for (int j = 0; j < kJ; ++j) {
+    const int i = tid + j * nthr;
+    if (i >= nS) break;
+    lv_set(j, i, L(0));
     lm[i] = G(0);
     lh[i] = G(0);
     ln[i] = G(0);
   }
+  // Adds the gradient of the recordings at slot t1 to the voltage adjoint of item i.
+  auto add_rec = [&](int i, int64_t t1, L& lv) {
+    const int k = i >> lS, s = i & (S - 1);
+    int r = recf[k];
+    if (r == kNone) return;
+    for (; r < tp.n_rec; ++r) {
+      if (a.rec_node[tp.rb + r] == k) lv += static_cast<L>(g.g_rec[pg.roff + (t1 * tp.n_rec + r) * S + s]);
+    }
+  };
   __syncthreads();
   double* wv = g.wv + wbase;
   G* wm = g.wm + wbase;
@@ -1333,7 +1381,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         const int k = i >> lS, s = i & (S - 1);
         G m = wm[j * nS + i], h = wh[j * nS + i], n = wn[j * nS + i];
         if (s < pg.nsim) {
-          build_item(a, pg, tp, Ld, wv + j * nS, k, s, wv[j * nS + i], pg.t0 + ts + j, m, h, n);
+          build_item<kCS>(a, pg, tp, Ld, wv + j * nS, k, s, wv[j * nS + i], pg.t0 + ts + j, m, h, n);
         } else {
           idle_item(a, tp, Ld, k, i);
         }
@@ -1342,23 +1390,21 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         wn[(j + 1) * nS + i] = n;
       }
       __syncthreads();
-      tree_solve<G, L, false>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+      tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
       for (int i = tid; i < nS; i += nthr) wv[(j + 1) * nS + i] = wv[j * nS + i] + static_cast<double>(Ld.xs[i]);
       __syncthreads();
     }
     for (int64_t j = te - ts - 1; j >= 0; --j) {
       const int64_t t1 = ts + j + 1;
-      for (int s = tid; s < S; s += nthr) {
-        for (int r = 0; r < tp.n_rec; ++r) {
-          const int node = a.rec_node[tp.rb + r];
-          lv[(node << lS) + s] += static_cast<L>(g.g_rec[pg.roff + (t1 * tp.n_rec + r) * S + s]);
-        }
-      }
-      __syncthreads();
       const G* m1 = wm + (j + 1) * nS;
       const G* h1 = wh + (j + 1) * nS;
       const G* n1 = wn + (j + 1) * nS;
-      for (int i = tid; i < nS; i += nthr) {
+      // The transposed system: the diagonal at the gates after the step, the right-hand
+      // side the voltage adjoint of the step's result (recordings included).
+#pragma unroll 1
+      for (int jj = 0; jj < kJ; ++jj) {
+        const int i = tid + jj * nthr;
+        if (i >= nS) break;
         const int k = i >> lS, s = i & (S - 1);
         const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
         const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
@@ -1376,22 +1422,27 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         } else {
           Ld.ds[i] = nfk[kNfDbase] + ((flags & kComp) ? L(1) : L(0));
         }
-        Ld.xs[i] = lv[i];
+        L lv = lv_get(jj, i);
+        add_rec(i, t1, lv);
+        Ld.xs[i] = lv;
       }
       __syncthreads();
-      tree_solve<G, L, true>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
+      tree_solve<G, L, true, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
       const double* v0 = wv + j * nS;
       const double* v1 = wv + (j + 1) * nS;
       const G* m0 = wm + j * nS;
       const G* h0 = wh + j * nS;
       const G* n0 = wn + j * nS;
-      for (int i = tid; i < nS; i += nthr) {
+#pragma unroll 1
+      for (int jj = 0; jj < kJ; ++jj) {
+        const int i = tid + jj * nthr;
+        if (i >= nS) break;
         const int k = i >> lS, s = i & (S - 1);
         const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
         const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
         const int32_t flags = nik[kNiFlags];
         if (!(flags & kComp) || s >= pg.nsim) {
-          lv[i] = L(0);
+          lv_set(jj, i, L(0));
           continue;
         }
         const G inv_cm = static_cast<G>(nfk[kNfInvCm]);
@@ -1452,19 +1503,21 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
           if (g.g_param[6]) g.g_param[6][pi] += G(1000) * (gl + gc * eLk);
           if (g.g_param[7]) g.g_param[7][pi] += G(1000) * gc * prm<6>(a, pg, tp, k, s);
         }
-        lv[i] = static_cast<L>(lvn);
+        lv_set(jj, i, static_cast<L>(lvn));
       }
+      // The next step's loops write only the thread's own slots of xs / ds; the solve
+      // that reads all of them starts after the barrier at its head.
       __syncthreads();
     }
   }
-  for (int s = tid; s < S; s += nthr) {
-    for (int r = 0; r < tp.n_rec; ++r) {
-      const int node = a.rec_node[tp.rb + r];
-      lv[(node << lS) + s] += static_cast<L>(g.g_rec[pg.roff + r * S + s]);
-    }
+#pragma unroll 1
+  for (int jj = 0; jj < kJ; ++jj) {
+    const int i = tid + jj * nthr;
+    if (i >= nS) break;
+    L lv = lv_get(jj, i);
+    add_rec(i, 0, lv);
+    g.g_v0[pg.soff + i] = static_cast<G>(lv);
   }
-  __syncthreads();
-  for (int i = tid; i < nS; i += nthr) g.g_v0[pg.soff + i] = static_cast<G>(lv[i]);
 }
 
 template <typename K>
@@ -1569,6 +1622,10 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   // The reverse pass differentiates the backward-Euler step only.
   if ((a.dt_v > 0 && a.dt_v != a.dt) || (a.rhs_scale > 0 && a.rhs_scale != 1.0) || a.explicit_step)
     return hipErrorInvalidValue;
+  if (a.items_per_thread < 0) return hipErrorInvalidValue;
+  if (a.items_per_thread > 0 && a.max_items > static_cast<int64_t>(a.items_per_thread) * a.threads)
+    return hipErrorInvalidValue;
+  if (a.items_per_thread == 0 && adj->lam_v == nullptr) return hipErrorInvalidValue;
   const Fused<G, L> f = unpack<G, L>(a);
   Adj<G> g;
   g.g_rec = static_cast<const G*>(adj->g_rec);
@@ -1581,13 +1638,24 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   g.lm = static_cast<G*>(adj->lam_m);
   g.lh = static_cast<G*>(adj->lam_h);
   g.ln = static_cast<G*>(adj->lam_n);
+  g.lv = static_cast<G*>(adj->lam_v);
   g.work_stride = adj->work_stride;
-  const dim3 grid(static_cast<unsigned>(a.n_progs)), block(a.threads);
-  const size_t lds = static_cast<size_t>(a.lds_bytes);
-  auto kern = fused_adjoint_kernel<G, L>;
-  if (hipError_t e = set_lds(kern, lds)) return e;
-  hipLaunchKernelGGL(kern, grid, block, lds, stream, f, g);
-  return hipGetLastError();
+#define JXHIP_ADJ(IPT)                                                                         \
+  case IPT:                                                                                    \
+    return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true>, a, stream, f, g) \
+                         : launch_kernel(fused_adjoint_kernel<G, L, IPT, false>, a, stream, f, g);
+  switch (a.items_per_thread) {
+    JXHIP_ADJ(0)
+    JXHIP_ADJ(1)
+    JXHIP_ADJ(2)
+    JXHIP_ADJ(4)
+    JXHIP_ADJ(8)
+    JXHIP_ADJ(12)
+    JXHIP_ADJ(16)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef JXHIP_ADJ
 }
 
 }  // namespace jxhip

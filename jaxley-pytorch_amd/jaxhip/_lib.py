@@ -59,6 +59,7 @@ class AdjointArgs(C.Structure):
         ("g_rec", _p), ("g_param", _p * JXHIP_NPARAM), ("g_v0", _p),
         ("work_v", _p), ("work_m", _p), ("work_h", _p), ("work_n", _p),
         ("lam_m", _p), ("lam_h", _p), ("lam_n", _p), ("work_stride", _i64), ("n_steps", _i64),
+        ("lam_v", _p),
     ]
 
 

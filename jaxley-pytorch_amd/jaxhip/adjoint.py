@@ -17,13 +17,14 @@ from typing import Dict, List, Tuple
 import torch
 
 from . import _lib
-from .fused import ALL_PARAMS, BATCHABLE, ForwardLaunch, adjoint_lds_bytes, from_prog, to_prog
+from .fused import (ALL_PARAMS, BATCHABLE, ForwardLaunch, adjoint_config, adjoint_lds_bytes, from_prog,
+                    to_prog)
 
 
 class _FusedIntegrate(torch.autograd.Function):
     @staticmethod
     def forward(ctx, holder, v0, node_p, bparam):
-        st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K = holder["args"]
+        st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K, _ = holder["args"]
         run = ForwardLaunch(st, node_f, node_p.detach(), None if bparam is None else bparam.detach(),
                             bmode, v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
                             T, dt, B, cfg, ckpt_every=K)
@@ -50,8 +51,11 @@ class _FusedIntegrate(torch.autograd.Function):
         lam = [torch.zeros((P, n, S), dtype=dtype, device=dev) for _ in range(3)]
         g_v0 = torch.zeros((P, n, S), dtype=dtype, device=dev)
         need_p = ctx.needs_input_grad[2] or ctx.needs_input_grad[3]
-        gpar = [torch.zeros((P, n, S), dtype=dtype, device=dev) if need_p else None
-                for _ in ALL_PARAMS]
+        # Only the parameters that require gradients get an accumulator (each one is a
+        # read-modify-write per item and step in the reverse pass).
+        wanted = holder["args"][-1]
+        gpar = [torch.zeros((P, n, S), dtype=dtype, device=dev) if need_p and k in wanted else None
+                for k in ALL_PARAMS]
         adj = _lib.AdjointArgs()
         adj.g_rec = gr.data_ptr()
         for j in range(len(ALL_PARAMS)):
@@ -60,16 +64,23 @@ class _FusedIntegrate(torch.autograd.Function):
         adj.work_v, adj.work_m, adj.work_h, adj.work_n = (w.data_ptr() for w in work)
         adj.lam_m, adj.lam_h, adj.lam_n = (l.data_ptr() for l in lam)
         adj.work_stride = (K + 1) * n * S
+        lam_v = torch.empty((P, n, S), dtype=dtype, device=dev)   # used when not in VGPRs
+        adj.lam_v = lam_v.data_ptr()
         adj.n_steps = T
         args = run.args
-        saved_lds = args.lds_bytes
-        args.lds_bytes = adjoint_lds_bytes(n, S, run.node_f.element_size(), st.sched)
+        # The reverse pass's own launch shape at the forward's S.
+        acfg = adjoint_config(n, run.B, run.node_f.element_size(), st.sched, S=S)
+        saved = (args.lds_bytes, args.threads, args.items_per_thread, args.coef_global)
+        args.lds_bytes = adjoint_lds_bytes(n, S, run.node_f.element_size(), st.sched, acfg.gcoef)
+        args.threads, args.items_per_thread, args.coef_global = acfg.threads, acfg.ipt, int(acfg.gcoef)
         f = run.kernel_name("jxhip_fused_adjoint")
         try:
             _lib.check(f(args, adj, _lib.stream_ptr(dev)), "fused_adjoint")
         finally:
-            args.lds_bytes = saved_lds
+            args.lds_bytes, args.threads, args.items_per_thread, args.coef_global = saved
         g_node_p = g_bparam = None
+        zero = torch.zeros((P, n, S), dtype=dtype, device=dev)
+        gpar = [zero if g is None else g for g in gpar]
         if ctx.needs_input_grad[2]:
             g_node_p = torch.stack([g.sum(dim=(0, 2)) for g in gpar], 1)
         if ctx.needs_input_grad[3]:
@@ -84,11 +95,12 @@ class _FusedIntegrate(torch.autograd.Function):
 
 
 def fused_integrate_with_adjoint(st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale,
-                                 T, dt, B, cfg, K):
-    """Differentiable fused simulation: returns `(recordings (B, n_rec, T+1), run)`."""
+                                 T, dt, B, cfg, K, grad_keys=ALL_PARAMS):
+    """Differentiable fused simulation: returns `(recordings (B, n_rec, T+1), run)`.
+    `grad_keys`: the channel parameters whose gradients are wanted (others are zero)."""
     if any(g.requires_grad for g in gates0):
         raise NotImplementedError("gradients with respect to initial gate states are not supported")
     K = int(max(1, min(K, T)))
-    holder = {"args": (st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K)}
+    holder = {"args": (st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K, frozenset(grad_keys))}
     recs = _FusedIntegrate.apply(holder, v0, node_p, bparam)
     return recs, holder.get("run")

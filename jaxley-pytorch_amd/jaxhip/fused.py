@@ -329,28 +329,56 @@ def default_config(n: int, n_tgt: int, B: int, esize: int, gsize: int = 0,
 
 
 ADJOINT_THREADS_MAX = 512
+# Voltage adjoints per thread the reverse pass keeps in VGPRs (its kernels' instances);
+# 0: in HBM, for programs with more items than that.
+ADJ_IPT = (1, 2, 4, 8, 12, 16)
 
 
-def adjoint_lds_bytes(n: int, S: int, esize: int, sched: int) -> int:
+@dataclass
+class AdjointConfig:
+    S: int
+    threads: int
+    ipt: int          # items (voltage adjoints) per thread
+    gcoef: bool       # coupling coefficients read from the node table (L2) instead of LDS
+
+
+def adjoint_lds_bytes(n: int, S: int, esize: int, sched: int, gcoef: bool = False) -> int:
     """LDS of one reverse-pass program: the staged schedule, the tree's two coupling
-    coefficients per node, right-hand side, pivots and voltage adjoints per (node,
-    simulation)."""
-    return sched + (3 * n * S + 2 * n) * esize
+    coefficients per node (unless read from L2), the transposed system (right-hand side
+    and pivots per (node, simulation)) and the first recording of every node."""
+    return sched + (0 if gcoef else 2 * n * esize) + 2 * n * S * esize + 4 * n
+
+
+def adjoint_config(n: int, B: int, esize: int, sched: int = 0, S: Optional[int] = None) -> AdjointConfig:
+    """The reverse pass's launch shape: the largest S (at least 256 programs when the
+    batch allows) whose transposed system fits the LDS, the coefficients in LDS when they
+    fit too, up to 512 threads, and VGPRs for the thread's voltage adjoints; or the shape
+    for a given S (the S of the checkpointed forward)."""
+    if S is None:
+        S = 1
+        while S < 64 and (S * 2) * 256 <= max(B, 256):
+            S *= 2
+        cands = [S >> k for k in range(S.bit_length())]
+    else:
+        cands = [S]
+    for s in cands:
+        for gcoef in (False, True):
+            if adjoint_lds_bytes(n, s, esize, sched, gcoef) > 160 * 1024:
+                continue
+            threads = int(min(ADJOINT_THREADS_MAX, max(64, 64 * -(-n * s // 64))))
+            need = -(-n * s // threads)
+            ipt = next((c for c in ADJ_IPT if c >= need), 0)
+            return AdjointConfig(s, threads, ipt, gcoef)
+    raise NotImplementedError(f"a topology of {n} nodes does not fit the reverse pass's LDS")
 
 
 def grad_config(n: int, B: int, esize: int, sched: int = 0) -> LaunchConfig:
-    """Launch shape shared by the checkpointed forward and the reverse pass: S bounded by
-    the reverse pass's LDS, threads by its launch bound."""
-    S = 1
-    while (S < 64 and (S * 2) * 256 <= max(B, 256)
-           and adjoint_lds_bytes(n, S * 2, esize, sched) <= 160 * 1024):
-        S *= 2
-    if adjoint_lds_bytes(n, S, esize, sched) > 160 * 1024:
-        raise NotImplementedError(f"a topology of {n} nodes does not fit the reverse pass's LDS")
-    # As many threads as the reverse pass allows: its per-step work is latency-bound, so
-    # fewer items per thread shorten every step.
-    threads = int(min(ADJOINT_THREADS_MAX, max(64, 64 * -(-n * S // 64))))
-    return LaunchConfig(S, threads)
+    """A forward launch shape for the checkpointed forward when none is measured: the S
+    of the reverse pass (the checkpoints are in its program layout), the streaming kernel
+    (coefficients from L2 when the LDS copy does not fit)."""
+    S = adjoint_config(n, B, esize, sched).S
+    fits = lds_bytes(n, 0, S, esize, sched) <= 160 * 1024
+    return LaunchConfig(S, 1024, stream=True, gcoef=not fits)
 
 
 class ForwardLaunch:

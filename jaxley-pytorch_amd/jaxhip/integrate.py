@@ -525,8 +525,9 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
             raise RuntimeError("the hand-written adjoint covers HH / Leak cells without "
                                "synapses (integrate routes other gradients to the per-step path)")
         K = checkpoint_lengths[-1] if checkpoint_lengths else 100
+        grad_keys = [k for k in ALL_PARAMS if k in inp.vals and inp.vals[k].requires_grad]
         recs, final = fused_integrate_with_adjoint(
-            st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K)
+            st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K, grad_keys)
     else:
         with torch.no_grad():
             run = ForwardLaunch(st, node_f, node_p.detach(), None if bparam is None else bparam.detach(),
