@@ -176,6 +176,33 @@ __device__ __forceinline__ void hh_rates_f64(double v, double& am, double& bm, d
   an = fabs(zn) < 0.25 ? an_s : an_d;
 }
 
+// dx/dv of x / expm1(x / 10) (the HH alpha_m / alpha_n shape, solver_gate.py:14-30) at
+// z = x / 10 with q = e^z - 1 given: 1 / q - x e^z / (10 q^2), or for |z| < 1/4 the
+// derivative of its Bernoulli series, -1/2 + z/6 - z^3/180 + z^5/5040 - z^7/151200
+// + z^9/4790016 - 691 z^11/108972864000 (next term below 3e-18), selected without a
+// branch.
+__device__ __forceinline__ double d_x_over_expm1_10(double x, double z, double q) {
+  const double y = z * z;
+  double p = kc(-6.34101567625469022e-09);                     // -691/108972864000
+  p = fma(p, y, kc(2.08767569878680989792e-07));               // 1/4790016
+  p = fma(p, y, kc(-6.61375661375661375661e-06));              // -1/151200
+  p = fma(p, y, kc(1.98412698412698412698e-04));               // 1/5040
+  p = fma(p, y, kc(-5.55555555555555555556e-03));              // -1/180
+  p = fma(p, y, kc(1.66666666666666666667e-01));               // 1/6
+  const double ser = fma(p, z, -0.5);
+  const double rq = drcp(q);
+  const double dir = rq - x * (q + 1.0) * rq * rq * 0.1;
+  return fabs(z) < 0.25 ? ser : dir;
+}
+
+// HH rates and their derivatives with respect to v (channels/hh.py:87-108 as autograd
+// differentiates them), float64, from one exponential as hh_rates_f64; other voltages
+// (v < -235 mV) take the direct formulas.
+__device__ __forceinline__ void hh_rates_dv_f64(double v, double& am, double& bm, double& ah,
+                                                double& bh, double& an, double& bn, double& dam,
+                                                double& dbm, double& dah, double& dbh,
+                                                double& dan, double& dbn);
+
 // The rates as the fused kernels evaluate them: the shared-exponential form in float64,
 // the direct form in float32 (where powers of E would lose accuracy).
 __device__ __forceinline__ void hh_rates_step(float v, float& am, float& bm, float& ah, float& bh,
@@ -185,6 +212,95 @@ __device__ __forceinline__ void hh_rates_step(float v, float& am, float& bm, flo
 __device__ __forceinline__ void hh_rates_step(double v, double& am, double& bm, double& ah,
                                               double& bh, double& an, double& bn) {
   hh_rates_f64(v, am, bm, ah, bh, an, bn);
+}
+
+// d/dx of x_over_expm1(x, y) as autograd differentiates solver_gate.py:14-30.
+template <typename T>
+__device__ __forceinline__ T d_x_over_expm1(T x, T y) {
+  const T z = x / y;
+  const T zc = z < T(20) ? z : T(20);
+  const T pass = z <= T(20) ? T(1) : T(0);
+  if (fabs(zc) < T(1e-4)) return pass * (T(-0.5) + zc / T(6));
+  const T em1 = dexpm1(zc);
+  return T(1) / em1 - x * (em1 + T(1)) / (em1 * em1) * pass / y;
+}
+
+template <typename T>
+__device__ __forceinline__ T d_save_exp(T u) {
+  return u <= T(20) ? dexp(u) : T(0);
+}
+
+// HH rates and their derivatives with respect to v by the reference's formulas (any
+// type, any voltage).
+template <typename T>
+__device__ __forceinline__ void hh_rates_dv(T v, T& am, T& bm, T& ah, T& bh, T& an, T& bn, T& dam,
+                                            T& dbm, T& dah, T& dbh, T& dan, T& dbn) {
+  const T xm = -(v + T(40)), xn = -(v + T(55));
+  const T um = -(v + T(65)) / T(18), uh = -(v + T(65)) / T(20);
+  const T ub = -(v + T(35)) / T(10), un = -(v + T(65)) / T(80);
+  am = T(0.1) * x_over_expm1(xm, T(10));
+  dam = -T(0.1) * d_x_over_expm1(xm, T(10));
+  bm = T(4) * save_exp(um);
+  dbm = -T(4) / T(18) * d_save_exp(um);
+  ah = T(0.07) * save_exp(uh);
+  dah = -T(0.07) / T(20) * d_save_exp(uh);
+  const T Eb = save_exp(ub);
+  bh = T(1) / (Eb + T(1));
+  dbh = d_save_exp(ub) / (T(10) * (Eb + T(1)) * (Eb + T(1)));
+  an = T(0.01) * x_over_expm1(xn, T(10));
+  dan = -T(0.01) * d_x_over_expm1(xn, T(10));
+  bn = T(0.125) * save_exp(un);
+  dbn = -T(0.125) / T(80) * d_save_exp(un);
+}
+
+__device__ __forceinline__ void hh_rates_dv_f64(double v, double& am, double& bm, double& ah,
+                                                double& bh, double& an, double& bn, double& dam,
+                                                double& dbm, double& dah, double& dbh,
+                                                double& dan, double& dbn) {
+  const double u = -(v + 65.0);
+  if (!(u <= 170.0)) {
+    hh_rates_dv<double>(v, am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn);
+    return;
+  }
+  const double e1 = exp_f64(u * (1.0 / 720));
+  const double e2 = e1 * e1, e4 = e2 * e2, e8 = e4 * e4, e9 = e8 * e1;
+  const double e18 = e9 * e9, e36 = e18 * e18, e40 = e36 * e4, e72 = e36 * e36;
+  constexpr double kExp1 = 2.718281828459045235;    // e
+  constexpr double kExp25 = 12.18249396070347343;   // e^(5/2)
+  constexpr double kExp3 = 20.08553692318766774;    // e^3
+  bm = 4.0 * e40;
+  dbm = (-4.0 / 18.0) * e40;
+  ah = 0.07 * e36;
+  dah = (-0.07 / 20.0) * e36;
+  bn = 0.125 * e9;
+  dbn = (-0.125 / 80.0) * e9;
+  const double Eb = e72 * kExp3;                    // exp(-(v + 35) / 10)
+  bh = drcp(Eb + 1.0);
+  dbh = bh * bh * Eb * 0.1;
+  const double xm = u + 25.0, xn = u + 10.0;        // -(v + 40), -(v + 55)
+  const double zm = xm * 0.1, zn = xn * 0.1;
+  const double qm = e72 * kExp25 - 1.0, qn = e72 * kExp1 - 1.0;
+  const double am_s = z_over_expm1_series(zm);
+  const double an_s = 0.1 * z_over_expm1_series(zn);
+  const double am_d = 0.1 * xm * drcp(qm);
+  const double an_d = 0.01 * xn * drcp(qn);
+  am = fabs(zm) < 0.25 ? am_s : am_d;
+  an = fabs(zn) < 0.25 ? an_s : an_d;
+  dam = -0.1 * d_x_over_expm1_10(xm, zm, qm);
+  dan = -0.01 * d_x_over_expm1_10(xn, zn, qn);
+}
+
+// The rates and derivatives the reverse pass evaluates (the forward's forms).
+__device__ __forceinline__ void hh_rates_dv_step(float v, float& am, float& bm, float& ah, float& bh,
+                                                 float& an, float& bn, float& dam, float& dbm,
+                                                 float& dah, float& dbh, float& dan, float& dbn) {
+  hh_rates_dv<float>(v, am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn);
+}
+__device__ __forceinline__ void hh_rates_dv_step(double v, double& am, double& bm, double& ah,
+                                                 double& bh, double& an, double& bn, double& dam,
+                                                 double& dbm, double& dah, double& dbh, double& dan,
+                                                 double& dbn) {
+  hh_rates_dv_f64(v, am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn);
 }
 
 }  // namespace jxhip

@@ -1249,22 +1249,6 @@ struct Adj {
   int64_t work_stride;
 };
 
-// d/dx of x_over_expm1(x, y) as autograd differentiates solver_gate.py:14-30.
-template <typename T>
-__device__ __forceinline__ T d_x_over_expm1(T x, T y) {
-  const T z = x / y;
-  const T zc = z < T(20) ? z : T(20);
-  const T pass = z <= T(20) ? T(1) : T(0);
-  if (fabs(zc) < T(1e-4)) return pass * (T(-0.5) + zc / T(6));
-  const T em1 = dexpm1(zc);
-  return T(1) / em1 - x * (em1 + T(1)) / (em1 * em1) * pass / y;
-}
-
-template <typename T>
-__device__ __forceinline__ T d_save_exp(T u) {
-  return u <= T(20) ? dexp(u) : T(0);
-}
-
 // dx'/dv of the exponential-Euler gate step through alpha(v), beta(v); e = dx'/dx.
 template <typename T>
 __device__ __forceinline__ T d_gate_dv(T x, T dt, T al, T be, T dal, T dbe, T& e) {
@@ -1326,6 +1310,22 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   G* ln = g.ln + pg.soff;
   L lvr[IPT > 0 ? IPT : 1];  // adjoint of the voltages of the thread's items (IPT > 0)
   G* lvg = g.lv + pg.soff;    // or in HBM (IPT == 0)
+  // Gate adjoints: in VGPRs with the voltage adjoints (IPT > 0), else in HBM (lm, lh, ln).
+  G lmr[IPT > 0 ? IPT : 1], lhr[IPT > 0 ? IPT : 1], lnr[IPT > 0 ? IPT : 1];
+  // Conductances batched per simulation (bparam mode 1): every item of a thread belongs
+  // to simulation tid mod S, so the thread sums the gradients of gNa, gK, gLeak (HH) and
+  // gLeak (Leak) over its items and all steps in VGPRs; the sums are reduced across the
+  // threads of a simulation in a fixed order at the end (no atomics).
+  const bool simgrad = a.bmode == 1;
+  G acc[4] = {G(0), G(0), G(0), G(0)};
+  auto grad = [&](int j, int q, int64_t pi, G x) {
+    if (g.g_param[j] == nullptr) return;
+    if (q >= 0 && simgrad) {
+      acc[q] += x;
+    } else {
+      g.g_param[j][pi] += x;
+    }
+  };
   auto lv_get = [&](int jj, int i) -> L {
     if constexpr (IPT > 0) {
       return lvr[jj];
@@ -1345,9 +1345,13 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
     const int i = tid + j * nthr;
     if (i >= nS) break;
     lv_set(j, i, L(0));
-    lm[i] = G(0);
-    lh[i] = G(0);
-    ln[i] = G(0);
+    if constexpr (IPT > 0) {
+      lmr[j] = lhr[j] = lnr[j] = G(0);
+    } else {
+      lm[i] = G(0);
+      lh[i] = G(0);
+      ln[i] = G(0);
+    }
   }
   // Adds the gradient of the recordings at slot t1 to the voltage adjoint of item i.
   auto add_rec = [&](int i, int64_t t1, L& lv) {
@@ -1394,45 +1398,51 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       for (int i = tid; i < nS; i += nthr) wv[(j + 1) * nS + i] = wv[j * nS + i] + static_cast<double>(Ld.xs[i]);
       __syncthreads();
     }
-    for (int64_t j = te - ts - 1; j >= 0; --j) {
-      const int64_t t1 = ts + j + 1;
-      const G* m1 = wm + (j + 1) * nS;
-      const G* h1 = wh + (j + 1) * nS;
-      const G* n1 = wn + (j + 1) * nS;
-      // The transposed system: the diagonal at the gates after the step, the right-hand
-      // side the voltage adjoint of the step's result (recordings included).
+    // The transposed system of a step: its diagonal at the gates after the step, its
+    // right-hand side the voltage adjoint at the step's end (recordings included). The
+    // segment's last step takes it from the recomputed trajectory; every other step's
+    // system is written by the previous (later) step's item pass.
+    auto system = [&](int i, int64_t t1, const G* mg, const G* hg, const G* ng, L lv) {
+      const int k = i >> lS, s = i & (S - 1);
+      const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
+      const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
+      const int32_t flags = nik[kNiFlags];
+      G lin = G(0);
+      if ((flags & kComp) && s < pg.nsim) {
+        if (flags & kHH) {
+          const G mm = mg[i], hh = hg[i], nn = ng[i];
+          const G n2 = nn * nn;
+          lin += G(1000) * (prm<0>(a, pg, tp, k, s) * (mm * mm * mm) * hh +
+                            prm<1>(a, pg, tp, k, s) * (n2 * n2) + prm<2>(a, pg, tp, k, s));
+        }
+        if (flags & kLeak) lin += G(1000) * prm<6>(a, pg, tp, k, s);
+        Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * static_cast<G>(nfk[kNfInvCm]) * lin);
+      } else {
+        Ld.ds[i] = nfk[kNfDbase] + ((flags & kComp) ? L(1) : L(0));
+      }
+      add_rec(i, t1, lv);
+      Ld.xs[i] = lv;
+    };
+    {
+      const int64_t jl = te - ts;
 #pragma unroll 1
       for (int jj = 0; jj < kJ; ++jj) {
         const int i = tid + jj * nthr;
         if (i >= nS) break;
-        const int k = i >> lS, s = i & (S - 1);
-        const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
-        const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
-        const int32_t flags = nik[kNiFlags];
-        G lin = G(0);
-        if ((flags & kComp) && s < pg.nsim) {
-          if (flags & kHH) {
-            const G mm = m1[i], hh = h1[i], nn = n1[i];
-            const G n2 = nn * nn;
-            lin += G(1000) * (prm<0>(a, pg, tp, k, s) * (mm * mm * mm) * hh +
-                              prm<1>(a, pg, tp, k, s) * (n2 * n2) + prm<2>(a, pg, tp, k, s));
-          }
-          if (flags & kLeak) lin += G(1000) * prm<6>(a, pg, tp, k, s);
-          Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * static_cast<G>(nfk[kNfInvCm]) * lin);
-        } else {
-          Ld.ds[i] = nfk[kNfDbase] + ((flags & kComp) ? L(1) : L(0));
-        }
-        L lv = lv_get(jj, i);
-        add_rec(i, t1, lv);
-        Ld.xs[i] = lv;
+        system(i, te, wm + jl * nS, wh + jl * nS, wn + jl * nS, lv_get(jj, i));
       }
       __syncthreads();
+    }
+    for (int64_t j = te - ts - 1; j >= 0; --j) {
       tree_solve<G, L, true, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
       const double* v0 = wv + j * nS;
       const double* v1 = wv + (j + 1) * nS;
       const G* m0 = wm + j * nS;
       const G* h0 = wh + j * nS;
       const G* n0 = wn + j * nS;
+      const G* m1 = wm + (j + 1) * nS;
+      const G* h1 = wh + (j + 1) * nS;
+      const G* n1 = wn + (j + 1) * nS;
 #pragma unroll 1
       for (int jj = 0; jj < kJ; ++jj) {
         const int i = tid + jj * nthr;
@@ -1441,72 +1451,77 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
         const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
         const int32_t flags = nik[kNiFlags];
-        if (!(flags & kComp) || s >= pg.nsim) {
-          lv_set(jj, i, L(0));
-          continue;
+        G lvn = G(0);
+        if ((flags & kComp) && s < pg.nsim) {
+          const G inv_cm = static_cast<G>(nfk[kNfInvCm]);
+          const G mu = static_cast<G>(Ld.xs[i]);
+          const G gl = -mu * static_cast<G>(v1[i]) * dt * inv_cm;  // dL/dlin
+          const G gc = mu * dt * inv_cm;                           // dL/dcst
+          lvn = mu;
+          if (flags & kHH) {
+            const G gNa = prm<0>(a, pg, tp, k, s);
+            const G gK = prm<1>(a, pg, tp, k, s);
+            const G eNa = prm<3>(a, pg, tp, k, s);
+            const G eK = prm<4>(a, pg, tp, k, s);
+            const G eL = prm<5>(a, pg, tp, k, s);
+            const G mm = m1[i], hh = h1[i], nn = n1[i];
+            const G m3 = mm * mm * mm, n3 = nn * nn * nn, n4 = n3 * nn;
+            const G wNa = G(1000) * (gl + gc * eNa);
+            const G wK = G(1000) * (gl + gc * eK);
+            const G wL = G(1000) * (gl + gc * eL);
+            const int64_t pi = pg.soff + i;
+            grad(0, 0, pi, wNa * m3 * hh);
+            grad(1, 1, pi, wK * n4);
+            grad(2, 2, pi, wL);
+            grad(3, -1, pi, G(1000) * gc * gNa * m3 * hh);
+            grad(4, -1, pi, G(1000) * gc * gK * n4);
+            grad(5, -1, pi, G(1000) * gc * prm<2>(a, pg, tp, k, s));
+            G lam_m0, lam_h0, lam_n0;
+            if constexpr (IPT > 0) {
+              lam_m0 = lmr[jj];
+              lam_h0 = lhr[jj];
+              lam_n0 = lnr[jj];
+            } else {
+              lam_m0 = lm[i];
+              lam_h0 = lh[i];
+              lam_n0 = ln[i];
+            }
+            const G lam_m1 = lam_m0 + wNa * gNa * G(3) * mm * mm * hh;
+            const G lam_h1 = lam_h0 + wNa * gNa * m3;
+            const G lam_n1 = lam_n0 + wK * gK * G(4) * n3;
+            G am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn;
+            hh_rates_dv_step(static_cast<G>(v0[i]), am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn);
+            G em, eh, en;
+            const G dmv = d_gate_dv(m0[i], dt, am, bm, dam, dbm, em);
+            const G dhv = d_gate_dv(h0[i], dt, ah, bh, dah, dbh, eh);
+            const G dnv = d_gate_dv(n0[i], dt, an, bn, dan, dbn, en);
+            lvn += lam_m1 * dmv + lam_h1 * dhv + lam_n1 * dnv;
+            if constexpr (IPT > 0) {
+              lmr[jj] = lam_m1 * em;
+              lhr[jj] = lam_h1 * eh;
+              lnr[jj] = lam_n1 * en;
+            } else {
+              lm[i] = lam_m1 * em;
+              lh[i] = lam_h1 * eh;
+              ln[i] = lam_n1 * en;
+            }
+          }
+          if (flags & kLeak) {
+            const G eLk = prm<7>(a, pg, tp, k, s);
+            const int64_t pi = pg.soff + i;
+            grad(6, 3, pi, G(1000) * (gl + gc * eLk));
+            grad(7, -1, pi, G(1000) * gc * prm<6>(a, pg, tp, k, s));
+          }
         }
-        const G inv_cm = static_cast<G>(nfk[kNfInvCm]);
-        const G mu = static_cast<G>(Ld.xs[i]);
-        const G gl = -mu * static_cast<G>(v1[i]) * dt * inv_cm;  // dL/dlin
-        const G gc = mu * dt * inv_cm;                           // dL/dcst
-        G lvn = mu;
-        if (flags & kHH) {
-          const G gNa = prm<0>(a, pg, tp, k, s);
-          const G gK = prm<1>(a, pg, tp, k, s);
-          const G eNa = prm<3>(a, pg, tp, k, s);
-          const G eK = prm<4>(a, pg, tp, k, s);
-          const G eL = prm<5>(a, pg, tp, k, s);
-          const G mm = m1[i], hh = h1[i], nn = n1[i];
-          const G m3 = mm * mm * mm, n3 = nn * nn * nn, n4 = n3 * nn;
-          const G wNa = G(1000) * (gl + gc * eNa);
-          const G wK = G(1000) * (gl + gc * eK);
-          const G wL = G(1000) * (gl + gc * eL);
-          const int64_t pi = pg.soff + i;
-          if (g.g_param[0]) g.g_param[0][pi] += wNa * m3 * hh;
-          if (g.g_param[1]) g.g_param[1][pi] += wK * n4;
-          if (g.g_param[2]) g.g_param[2][pi] += wL;
-          if (g.g_param[3]) g.g_param[3][pi] += G(1000) * gc * gNa * m3 * hh;
-          if (g.g_param[4]) g.g_param[4][pi] += G(1000) * gc * gK * n4;
-          if (g.g_param[5]) g.g_param[5][pi] += G(1000) * gc * prm<2>(a, pg, tp, k, s);
-          const G lam_m1 = lm[i] + wNa * gNa * G(3) * mm * mm * hh;
-          const G lam_h1 = lh[i] + wNa * gNa * m3;
-          const G lam_n1 = ln[i] + wK * gK * G(4) * n3;
-          const G v = static_cast<G>(v0[i]);
-          const G xm = -(v + G(40)), xn = -(v + G(55));
-          const G um = -(v + G(65)) / G(18), uh = -(v + G(65)) / G(20);
-          const G ub = -(v + G(35)) / G(10), un = -(v + G(65)) / G(80);
-          const G am = G(0.1) * x_over_expm1(xm, G(10));
-          const G dam = -G(0.1) * d_x_over_expm1(xm, G(10));
-          const G bm = G(4) * save_exp(um);
-          const G dbm = -G(4) / G(18) * d_save_exp(um);
-          const G ah = G(0.07) * save_exp(uh);
-          const G dah = -G(0.07) / G(20) * d_save_exp(uh);
-          const G Eb = save_exp(ub);
-          const G bh = G(1) / (Eb + G(1));
-          const G dbh = d_save_exp(ub) / (G(10) * (Eb + G(1)) * (Eb + G(1)));
-          const G an = G(0.01) * x_over_expm1(xn, G(10));
-          const G dan = -G(0.01) * d_x_over_expm1(xn, G(10));
-          const G bn = G(0.125) * save_exp(un);
-          const G dbn = -G(0.125) / G(80) * d_save_exp(un);
-          G em, eh, en;
-          const G dmv = d_gate_dv(m0[i], dt, am, bm, dam, dbm, em);
-          const G dhv = d_gate_dv(h0[i], dt, ah, bh, dah, dbh, eh);
-          const G dnv = d_gate_dv(n0[i], dt, an, bn, dan, dbn, en);
-          lvn += lam_m1 * dmv + lam_h1 * dhv + lam_n1 * dnv;
-          lm[i] = lam_m1 * em;
-          lh[i] = lam_h1 * eh;
-          ln[i] = lam_n1 * en;
+        // The item's own slots of the earlier step's system (the solve above read every
+        // slot before its closing barrier); at the segment's first step the adjoint waits
+        // for the next segment in VGPRs / HBM.
+        if (j > 0) {
+          system(i, ts + j, m0, h0, n0, static_cast<L>(lvn));
+        } else {
+          lv_set(jj, i, static_cast<L>(lvn));
         }
-        if (flags & kLeak) {
-          const G eLk = prm<7>(a, pg, tp, k, s);
-          const int64_t pi = pg.soff + i;
-          if (g.g_param[6]) g.g_param[6][pi] += G(1000) * (gl + gc * eLk);
-          if (g.g_param[7]) g.g_param[7][pi] += G(1000) * gc * prm<6>(a, pg, tp, k, s);
-        }
-        lv_set(jj, i, static_cast<L>(lvn));
       }
-      // The next step's loops write only the thread's own slots of xs / ds; the solve
-      // that reads all of them starts after the barrier at its head.
       __syncthreads();
     }
   }
@@ -1517,6 +1532,23 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
     L lv = lv_get(jj, i);
     add_rec(i, 0, lv);
     g.g_v0[pg.soff + i] = static_cast<G>(lv);
+  }
+  if (simgrad) {
+    // Per-simulation sums: thread partials into LDS (the system's space is free now),
+    // then thread s < S adds the partials of simulation s in thread order and stores
+    // them at its node-0 item (the caller sums each accumulator over the nodes).
+    G* red = reinterpret_cast<G*>(smem_raw);
+    constexpr int kQ[4] = {0, 1, 2, 6};
+    for (int q = 0; q < 4; ++q) {
+      __syncthreads();
+      red[tid] = acc[q];
+      __syncthreads();
+      if (tid < S && g.g_param[kQ[q]] != nullptr) {
+        G sum = G(0);
+        for (int t = tid; t < nthr; t += S) sum += red[t];
+        g.g_param[kQ[q]][pg.soff + tid] += sum;
+      }
+    }
   }
 }
 

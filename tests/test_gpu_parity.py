@@ -248,3 +248,35 @@ def test_repeated_launches_bitwise_identical(device, shape, monkeypatch):
     runs = [jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT) for _ in range(3)]
     autotune._CACHE.clear()
     assert torch.equal(runs[0], runs[1]) and torch.equal(runs[0], runs[2])
+
+
+def test_adjoint_per_simulation_conductances(device):
+    """Per-simulation trainable gNa / gK / gLeak (`data_set` of shape (B, 1), the batched
+    sweep's parameters: the reverse pass sums their gradients per simulation in
+    registers) against autograd through the oracle, B = 19 (several programs and
+    padding simulations)."""
+    B, T, nb = 19, 300, 20
+    cell, om = build_tree(nb, seed=13)
+    cell.branch(0).comp(0).record()
+    cell.branch(3).comp(2).record()
+    prm = c1_params(B, seed=13)
+    leaves = {k: torch.tensor(v, dtype=torch.float64, device=device, requires_grad=True) for k, v in prm.items()}
+    ps = None
+    for k, v in leaves.items():
+        ps = cell.data_set(k, v, ps)
+    cur = c1_stimulus(T)
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=torch.float64, device=device))
+    v = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[3, 100])
+    w = torch.tensor(np.random.default_rng(5).normal(size=(B, 2, T + 1)), dtype=torch.float64)
+    (v.cpu() * w).sum().backward()
+    ol = {k: torch.tensor(x, dtype=torch.float64, requires_grad=True) for k, x in prm.items()}
+    nc = 4 * nb
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=[0], rec_comps=[0, 14],
+                     overrides={k: x.expand(B, nc) for k, x in ol.items()})
+    (ref * w).sum().backward()
+    assert float((v.detach().cpu() - ref.detach()).abs().max()) < 1e-6
+    for k in prm:
+        g, g_ref = leaves[k].grad.cpu(), ol[k].grad
+        rel = float((g - g_ref).abs().max() / g_ref.abs().max())
+        print(k, "relative gradient error", rel)
+        assert rel < 1e-8
