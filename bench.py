@@ -45,19 +45,23 @@ import numpy as np
 import torch
 
 ROOT = Path(__file__).resolve().parent
-for _p in (ROOT, ROOT / "jaxley-pytorch_amd", ROOT / "tests"):
+for _p in (ROOT, ROOT / "jaxley-pytorch_amd"):
     if str(_p) not in sys.path:
         sys.path.insert(0, str(_p))
 
 import jaxhip as jx  # noqa: E402
-from models import DT, build_tree, c1_params, c1_stimulus, c2_request, c2_specs  # noqa: E402
+from jaxhip.workloads import (DT, c1_params, c1_stimulus, c2_request, c2_specs, c4_network,  # noqa: E402
+                              c4_stimulus, random_tree, tree_cell)
 
 NBRANCH, SEED = 250, 11
 HBM_PEAK = 8.0e12
 BYTES_PER_COMP_STEP = {"f64": 64, "f32": 32}
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
-# 256 MiB device copy: fetch x 2.0, write x 1.0) for the headline launch shape and
-# workload; reported as roofline.traffic when the autotuner picks that shape.
+# HBM bytes per launch of the headline kernel, measured with rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE (scaled by a 256 MiB device copy: fetch x 2.0, write x 1.0) for the headline
+# launch shape and workload (profiles/fused_stream_f64_c1_b4096_t2000.md); reported as
+# roofline.traffic, with that source, when the autotuner picks that shape. Counters are
+# not collected inside this run (a counter pass is a separate rocprofv3 run).
+TRAFFIC_SOURCE = "profiles/fused_stream_f64_c1_b4096_t2000.md"
 MEASURED_TRAFFIC = {
     ("float64", 4096, 2000, "S=8, threads=1024, stream=True, gcoef=True"): 376.4e9 + 372.3e9,
 }
@@ -68,7 +72,7 @@ def log(msg):
 
 
 def setup(B: int, T: int, dtype, device):
-    cell, om = build_tree(NBRANCH, SEED)
+    cell = tree_cell(NBRANCH, SEED)[0]
     cell.branch(0).comp(0).record()
     prm = c1_params(B, SEED)
     ps = None
@@ -76,7 +80,7 @@ def setup(B: int, T: int, dtype, device):
         ps = cell.data_set(k, torch.as_tensor(v, dtype=dtype, device=device), ps)
     cur = torch.tensor(c1_stimulus(T), dtype=dtype, device=device)
     ds = cell.branch(0).comp(0).data_stimulate(cur)
-    return cell, om, ps, ds, prm
+    return cell, ps, ds, prm
 
 
 def timed(fn, steps, warmup, device):
@@ -138,35 +142,98 @@ def spikes(v: torch.Tensor) -> torch.Tensor:
     return ((v[..., 1:] >= 0.0) & (v[..., :-1] < 0.0)).sum(-1)
 
 
-def cpu_baseline(prm, T_sample, B_sample, threads):
-    """The CPU oracle (a from-scratch restatement of the reference's level-parallel CPU
-    step, float64, PyTorch on the host) on a bounded sample of the same workload."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def physical_cores() -> int:
+    """Physical cores available to this process (hyperthreads not counted)."""
+    try:
+        import psutil
+
+        n = psutil.cpu_count(logical=False)
+    except Exception:
+        n = None
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    logical = os.cpu_count() or avail
+    if n:
+        n = max(1, n * avail // max(logical, 1))
+    return max(1, min(n or avail, avail))
+
+
+def cpu_baseline(threads: int, samples=((64, 40), (4096, 2)), dtypes=(torch.float64, torch.float32)):
+    """The CPU oracle (this build's from-scratch restatement of the reference's
+    level-parallel CPU step, PyTorch on the host) on bounded samples of the headline
+    workload: for each batch B and dtype, the median of 3 timed runs of `steps` steps after
+    one warm-up run; compartment-steps/s."""
     from oracle import jx_oracle as O
 
     torch.set_num_threads(threads)
-    _, om = build_tree(NBRANCH, SEED)
+    parents, blen, rad = random_tree(NBRANCH, SEED)
+    om = O.OracleModel([(parents, [4] * NBRANCH)])
+    om.length, om.radius = np.repeat(blen / 4, 4), np.repeat(rad, 4)
+    om.hh_comps = np.arange(4 * NBRANCH)
+    nc = len(om.internal)
+    out = {}
+    for B, steps in samples:
+        prm = c1_params(B, SEED)
+        ov = {k: np.broadcast_to(v, (B, nc)).copy() for k, v in prm.items()}
+        for dt_ in dtypes:
+            cur = torch.tensor(c1_stimulus(steps), dtype=dt_)[None]
+            run = lambda: O.simulate(om, None, DT, stim=cur, stim_comps=[0], rec_comps=[0], overrides=ov,
+                                     dtype=dt_)
+            run()
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                run()
+                times.append(time.perf_counter() - t0)
+            sec = float(np.median(times))
+            key = f"B{B}_{str(dt_).replace('torch.float', 'f')}"
+            out[key] = {"comp_steps_per_s": nc * steps * B / sec, "s_per_run": sec, "steps": steps, "batch": B}
+            log(f"[bench] cpu {key}: {nc * steps * B / sec:.3e} comp-steps/s ({sec:.2f} s per run)")
+    return out
+
+
+def cpu_reference_sample(prm, T_sample, B_sample, threads):
+    """The CPU oracle's float64 traces of the first simulations (checks the GPU result)."""
+    from oracle import jx_oracle as O
+
+    torch.set_num_threads(threads)
+    parents, blen, rad = random_tree(NBRANCH, SEED)
+    om = O.OracleModel([(parents, [4] * NBRANCH)])
+    om.length, om.radius = np.repeat(blen / 4, 4), np.repeat(rad, 4)
+    om.hh_comps = np.arange(4 * NBRANCH)
     nc = len(om.internal)
     ov = {k: np.broadcast_to(v[:B_sample], (B_sample, nc)).copy() for k, v in prm.items()}
     cur = torch.tensor(c1_stimulus(T_sample))[None]
-    t0 = time.perf_counter()
-    ref = O.simulate(om, None, DT, stim=cur, stim_comps=[0], rec_comps=[0], overrides=ov)
-    el = time.perf_counter() - t0
-    return ref, el, nc * T_sample * B_sample / el
+    return O.simulate(om, None, DT, stim=cur, stim_comps=[0], rec_comps=[0], overrides=ov)
 
 
-def bench_fwd_adj_sweep(device, B, T, K=100):
+def bench_fwd_adj_sweep(device, B, T, K=100, dtype=torch.float64, mode=None):
     """Forward + hand-written adjoint of the batched sweep (B simulations, per-simulation
-    trainable gNa, gK, gLeak), float64: MSE of the root voltage against target traces
-    simulated from the seeded sweep parameters, checkpoints every K steps."""
-    cell, _, _, ds, prm = setup(B, T, torch.float64, device)
+    trainable gNa, gK, gLeak): MSE of the root voltage against target traces simulated
+    from the seeded sweep parameters, checkpoints every K steps; float64, or float32 in
+    the given mode of `set_float32_mode`."""
+    from jaxhip.integrate import set_float32_mode
+
+    if mode is not None:
+        set_float32_mode(mode)
+    cell, _, ds, prm = setup(B, T, dtype, device)
     rng = np.random.default_rng(12)
     guess = {k: v * rng.uniform(0.9, 1.1, size=v.shape) for k, v in prm.items()}
     with torch.no_grad():
         ps_t = None
         for k, v in prm.items():
-            ps_t = cell.data_set(k, torch.as_tensor(v, dtype=torch.float64, device=device), ps_t)
+            ps_t = cell.data_set(k, torch.as_tensor(v, dtype=dtype, device=device), ps_t)
         target = jx.integrate(cell, param_state=ps_t, data_stimuli=ds, delta_t=DT)[:, 0]
-    leaves = {k: torch.tensor(v, dtype=torch.float64, device=device, requires_grad=True)
+    leaves = {k: torch.tensor(v, dtype=dtype, device=device, requires_grad=True)
               for k, v in guess.items()}
 
     def run():
@@ -179,12 +246,16 @@ def bench_fwd_adj_sweep(device, B, T, K=100):
         loss.backward()
 
     sec = timed(run, 1, 1, device)
+    if mode is not None:
+        set_float32_mode("mixed")
     n_comp = cell._n_comp
     cs = n_comp * T * B / sec
-    res = {"comp_steps_per_s": cs, "ms": sec * 1e3, "batch": B, "checkpoint_every": K, "dtype": "f64",
-           "trainable_per_sim": 3, "bytes_per_comp_step": 224,
-           "frac_of_hbm_roofline": 224 * cs / HBM_PEAK}
-    log(f"[bench] forward+adjoint sweep B={B} f64: {sec*1e3:.1f} ms, {cs:.3e} comp-steps/s")
+    bpc = 224 if dtype == torch.float64 or mode == "mixed" else 112
+    name = "f64" if dtype == torch.float64 else f"f32 ({mode})"
+    res = {"comp_steps_per_s": cs, "ms": sec * 1e3, "batch": B, "checkpoint_every": K, "dtype": name,
+           "trainable_per_sim": 3, "bytes_per_comp_step": bpc,
+           "frac_of_hbm_roofline": bpc * cs / HBM_PEAK}
+    log(f"[bench] forward+adjoint sweep B={B} {name}: {sec*1e3:.1f} ms, {cs:.3e} comp-steps/s")
     return res
 
 
@@ -194,9 +265,7 @@ def bench_network(device, n_cells=1024, k=32, B=64, T=4000, dtype=torch.float32)
     0.05 nA pulses into every soma), B simulations, T steps, somata recorded; float32
     (the float32 mode of integrate). One program holds the whole network (10240 solve
     nodes): in float64 its system exceeds the LDS of a workgroup."""
-    from models import build_c4, c4_stimulus
-
-    net, _, info = build_c4(n_cells, k, seed=9)
+    net, info = c4_network(n_cells, k, seed=9)
     soma = info["soma_comps"]
     for c in soma:
         net.cell(int(c) // 9).branch(0).comp(0).record()
@@ -256,9 +325,7 @@ def bench_network_fwd_adj(device, n_cells=1024, k=32, B=64, T=1000, K=100, dtype
     through the differentiable per-step path (synapse operator with its hand-written
     backward, tree solve with its transposed solve). T = 1000 of the workload's 4000
     steps keeps the benchmark short; the rate per step does not depend on T."""
-    from models import build_c4, c4_stimulus
-
-    net, _, info = build_c4(n_cells, k, seed=9)
+    net, info = c4_network(n_cells, k, seed=9)
     soma = info["soma_comps"]
     for c in soma:
         net.cell(int(c) // 9).branch(0).comp(0).record()
@@ -292,7 +359,7 @@ def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
     specs = c2_specs(n_req, seed=7)
     cells = []
     for s in specs:                       # morphologies are built before the clock starts
-        cell, _ = build_tree(s["nbranch"], s["seed"])
+        cell = tree_cell(s["nbranch"], s["seed"])[0]
         cell.branch(0).comp(0).record()
         cells.append(cell)
     rng = np.random.default_rng(8)
@@ -372,7 +439,7 @@ def main():
     _integ = sys.modules["jaxhip.integrate"]
 
     # ---- headline: float64 (meets <= 1e-2 mV vs float64 on every simulation) ----
-    cell64, om, ps64, ds64, prm = setup(B, T, torch.float64, device)
+    cell64, ps64, ds64, prm = setup(B, T, torch.float64, device)
     n_comp = cell64._n_comp
 
     def run64():
@@ -413,6 +480,7 @@ def main():
                      "frac": achieved / HBM_PEAK,
                      "traffic": traffic,
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "traffic_source": TRAFFIC_SOURCE if traffic else None,
                      "traffic_GBps": traffic / (ktime * 1e-3) / 1e9 if traffic else None,
                      "kernel": "fused_stream_kernel / fused_regs_kernel (float64)", "kernel_ms": ktime,
                      "bytes_per_comp_step": bpc},
@@ -420,7 +488,27 @@ def main():
     rec64 = run64()
     if not args.no_extras:
         # ---- float32 modes vs float64 on the GPU (all simulations) ----
-        cell32, _, ps32, ds32, _ = setup(B, T, torch.float32, device)
+        cell32, ps32, ds32, _ = setup(B, T, torch.float32, device)
+        # float64 on the float32 inputs (geometry, conductances, stimulus rounded): the
+        # float32 modes' reference, and the simulations' own sensitivity to that rounding.
+        parents, blen, rad = random_tree(NBRANCH, SEED)
+        cr = jx.Cell(jx.Branch(jx.Compartment(), 4), parents)
+        cr.set("length", np.repeat(blen / 4, 4).astype(np.float32).astype(np.float64))
+        cr.set("radius", np.repeat(rad, 4).astype(np.float32).astype(np.float64))
+        cr.insert(jx.HH())
+        cr.branch(0).comp(0).record()
+        psr = None
+        for k, v in prm.items():
+            psr = cr.data_set(k, torch.as_tensor(v.astype(np.float32).astype(np.float64), device=device), psr)
+        curr = torch.tensor(c1_stimulus(T) * float(np.float32(0.2)) / 0.2, dtype=torch.float64, device=device)
+        with torch.no_grad():
+            rec64r = jx.integrate(cr, param_state=psr, data_stimuli=cr.branch(0).comp(0).data_stimulate(curr),
+                                  delta_t=DT)
+        sens = (rec64r - rec64).abs().amax(dim=(1, 2))
+        good = sens <= 3e-3
+        out["f32_input_rounding"] = {"max_shift_of_f64_mV": float(sens.max()),
+                                     "well_conditioned_sims": int(good.sum()),
+                                     "criterion": "float64 trace moves <= 3e-3 mV when the inputs are rounded to float32"}
 
         def run32():
             with torch.no_grad():
@@ -433,17 +521,20 @@ def main():
             k32 = kernel_time_ms(run32, device)
             r32 = run32()
             err = (r32.double() - rec64).abs().amax(dim=(1, 2))
+            err_r = (r32.double() - rec64r).abs().amax(dim=(1, 2))
             spikes_equal = bool((spikes(r32.double()) == spikes(rec64)).all())
             out[f"f32_{mode}"] = {"value": comp_steps / s32, "kernel_ms": k32,
                                   "max_err_vs_f64_gpu_mV": float(err.max()),
                                   "frac_sims_within_1e-2_mV": float((err <= 1e-2).double().mean()),
+                                  "max_err_vs_f64_same_inputs_mV": float(err_r.max()),
+                                  "max_err_well_conditioned_mV": float(err_r[good].max()),
                                   "spike_counts_identical": spikes_equal}
             log(f"[bench] f32 {mode}: {comp_steps / s32:.3e} comp-steps/s, max err {float(err.max()):.3e} mV, "
                 f"{float((err <= 1e-2).double().mean()):.4f} within 1e-2")
         set_float32_mode(saved_mode)
         # ---- plain: one simulation per launch ----
         n_plain = 4
-        cp, _, _, dsp, _ = setup(1, T, torch.float64, device)
+        cp, _, dsp, _ = setup(1, T, torch.float64, device)
 
         def run_plain():
             with torch.no_grad():
@@ -460,7 +551,7 @@ def main():
         log(f"[bench] plain {plain:.3e} comp-steps/s ({value / plain:.0f}x)")
         # ---- forward + adjoint: per-branch trainable conductances, B = 256 ----
         Bg = 256
-        cg, _, _, dsg, _ = setup(1, T, torch.float32, device)
+        cg, _, dsg, _ = setup(1, T, torch.float32, device)
         for k in ("HH_gNa", "HH_gK", "HH_gLeak"):
             cg.branch("all").make_trainable(k, verbose=False)
         rng = np.random.default_rng(5)
@@ -487,8 +578,10 @@ def main():
                           "bytes_per_comp_step": 112,
                           "frac_of_hbm_roofline": 112 * n_comp * T * Bg / sg / HBM_PEAK}
         log(f"[bench] forward+adjoint B={Bg}: {sg*1e3:.1f} ms, {n_comp * T * Bg / sg:.3e} comp-steps/s")
-        # ---- forward + adjoint of the batched sweep itself: B = 4096, float64 ----
+        # ---- forward + adjoint of the batched sweep itself: B = 4096, float64 and float32 ----
         out["fwd_adj_sweep"] = bench_fwd_adj_sweep(device, B, T)
+        out["fwd_adj_sweep_f32"] = bench_fwd_adj_sweep(device, B, T, dtype=torch.float32, mode="fast")
+        out["fwd_adj_sweep_f32_mixed"] = bench_fwd_adj_sweep(device, B, T, dtype=torch.float32, mode="mixed")
         # ---- network: 1024 ball-and-stick cells, 32768 KineticSynapses, B = 64 ----
         out["network"] = bench_network(device)
         # ---- a traced morphology (SWC): the CA1 cell, B = 2048, float64 ----
@@ -504,19 +597,23 @@ def main():
                 log(f"[bench] {key} {sv['requests']} requests at {sv['arrival_rate_per_s']}/s: p50 "
                     f"{sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms, "
                     f"{sv['comp_steps_per_s']:.3e} comp-steps/s")
-    # ---- CPU baseline on a bounded sample (rank 0, N = 1) ----
-    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(16, ncpu))
-    Bs, Ts = args.cpu_sims, T
-    log(f"[bench] cpu baseline: oracle on {Bs} simulations x {Ts} steps, {threads} threads")
-    ref, el, cps = cpu_baseline(prm, Ts, Bs, threads)
-    out["cpu_baseline"] = {"value": cps, "unit": "compartment-steps/s", "cores": threads, "kind": "port",
-                           "sample": f"{Bs} simulations x {Ts} steps x {n_comp} compartments of the same "
-                                     f"workload, float64, {el:.1f} s; CPU {platform.processor() or platform.machine()}"}
+    # ---- CPU baseline on bounded samples (rank 0, N = 1) ----
+    threads = physical_cores()
+    log(f"[bench] cpu baseline: the oracle on {threads} threads (physical cores)")
+    cpu = cpu_baseline(threads)
+    head = cpu["B4096_f64"]
+    out["cpu_baseline"] = {"value": head["comp_steps_per_s"], "unit": "compartment-steps/s", "cores": threads,
+                           "kind": "port",
+                           "sample": f"the headline workload (random tree, {n_comp} compartments, per-simulation "
+                                     f"HH conductances) at B=4096 for {head['steps']} steps, float64, median of 3 "
+                                     f"runs after 1 warm-up; CPU {cpu_model()}",
+                           "runs": cpu}
+    Bs = args.cpu_sims
+    ref = cpu_reference_sample(prm, T, Bs, threads)
     err = float((rec64[:Bs].cpu() - ref).abs().max())
     out["max_err_f64_vs_oracle_mV"] = err
     out["spike_counts_f64_vs_oracle_identical"] = bool((spikes(rec64[:Bs].cpu()) == spikes(ref)).all())
-    log(f"[bench] cpu {cps:.3e} comp-steps/s; float64 GPU vs oracle {err:.3e} mV")
+    log(f"[bench] float64 GPU vs oracle on {Bs} simulations: {err:.3e} mV")
     print(json.dumps(out), flush=True)
 
 

@@ -757,7 +757,20 @@ def _recorder(model, record, internal):
     return rec
 
 
-def simulate(
+def simulate(*args, dtype=torch.float64, **kwargs):
+    """`_simulate` in `dtype`: float64 (the oracle) or float32, every tensor operation in
+    that type as the reference computes when its inputs are float32 (integrate.py:480-490
+    takes the dtype of the first floating point input)."""
+    global F64
+    saved = F64
+    F64 = dtype
+    try:
+        return _simulate(*args, **kwargs)
+    finally:
+        F64 = saved
+
+
+def _simulate(
     model: OracleModel,
     t_max: Optional[float],
     dt: float = 0.025,
