@@ -322,9 +322,10 @@ def bench_swc(device, ncomp=4, B=2048, T=2000, dtype=torch.float64):
 def bench_network_fwd_adj(device, n_cells=1024, k=32, B=64, T=1000, K=100, dtype=torch.float32):
     """Forward + adjoint of the network workload with every synapse's gS trainable (shared
     by the B simulations): a loss on the somatic voltages, checkpoints every K steps,
-    through the differentiable per-step path (synapse operator with its hand-written
-    backward, tree solve with its transposed solve). T = 1000 of the workload's 4000
-    steps keeps the benchmark short; the rate per step does not depend on T."""
+    through the fused kernels (checkpointed forward, then the hand-written reverse pass
+    with the synapse adjoint and the presynaptic shares in the kernel). T = 1000 of the
+    workload's 4000 steps keeps the benchmark short; the rate per step does not depend
+    on T."""
     net, info = c4_network(n_cells, k, seed=9)
     soma = info["soma_comps"]
     for c in soma:
@@ -347,7 +348,7 @@ def bench_network_fwd_adj(device, n_cells=1024, k=32, B=64, T=1000, K=100, dtype
     res = {"comp_steps_per_s": n_comp * T * B / sec, "ms": sec * 1e3, "batch": B, "cells": n_cells,
            "time_steps": T, "trainable": n_cells * k, "checkpoint_every": K,
            "dtype": str(dtype).replace("torch.", ""),
-           "path": "per-step (synapse operator + tree-solve operator with their backward kernels)"}
+           "path": "fused checkpointed forward + fused reverse pass (synapse adjoint in the kernel)"}
     log(f"[bench] network forward+adjoint B={B}, T={T}: {sec*1e3:.1f} ms, {res['comp_steps_per_s']:.3e} comp-steps/s")
     return res
 

@@ -180,7 +180,7 @@ int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int3
  * Topologies: int32 (n_topo, JXHIP_TOPO_COLS):
  *   n (nodes), L (max segment level), node_base, seg_level_base, child_base, n_rec,
  *   rec_base, n_tgt, tgt_base, syn_ptr_base, syn_base, n_syn, seg_base, seg_child_base,
- *   0, 0
+ *   pre_ptr_base, pre_syn_base (the reverse pass's synapses by presynaptic node)
  * Nodes are in "kernel order": segments (maximal single-child chains, top first) grouped
  * by segment level, leaf segments first. A segment's level is 0 if its bottom node has
  * no children, else 1 + the largest level of its child segments.
@@ -239,11 +239,13 @@ typedef struct jxhip_fused_args {
   int64_t stim_b;
   /* recordings: per program [t][rec][S] at rec_off, t = 0 .. t_end - t_begin */
   void* rec;
-  /* synapses, sorted by target: per target [syn_ptr[t], syn_ptr[t+1]) */
+  /* synapses in jagged-diagonal order: targets numbered by non-increasing in-degree; the
+     j-th synapse of target t is syn_ptr[j] + t while t < syn_ptr[j+1] - syn_ptr[j] (the
+     pointer list ends with a repeated entry) */
   int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate, 2 ionotropic, 3 test
                                synapse, JXHIP_SYN_MIXED: per synapse in syn_kind */
   const int32_t* syn_pre;   /* topology-relative presynaptic node, at syn_base */
-  const int32_t* syn_ptr;   /* n_tgt + 1 per topology at syn_ptr_base */
+  const int32_t* syn_ptr;   /* max in-degree + 2 per topology at syn_ptr_base */
   const int32_t* tgt_node;  /* topology-relative target node, at tgt_base */
   const void* syn_param[JXHIP_NSYNPARAM]; /* per synapse at syn_base */
   const void* syn_scale;    /* 1e5 / area(post) per synapse at syn_base */
@@ -330,11 +332,14 @@ int jxhip_fused_forward_mixed(const jxhip_fused_args* args, void* stream);
  * on one stream. */
 int jxhip_fused_forward_noalloc_f64(const jxhip_fused_args* args, int64_t state_elems, void* stream);
 
-/* Reverse pass of jxhip_fused_forward_* for HH / Leak cells (no synapses, no stimulus
+/* Reverse pass of jxhip_fused_forward_* for HH / Leak cells and networks of them with
+ * one synapse type, KineticSynapse (syn_model 0) or SigmoidRateSynapse (1) (no stimulus
  * gradient): for every K-step segment, from the last to the first, recomputes the
  * segment from its checkpoint (storing the trajectory in work_*), then runs the adjoint
  * of each step backwards: transposed tree solve, adjoint of the linearized currents and
- * of the exponential-Euler gates.
+ * of the exponential-Euler gates, and of the synapse step (state update, current and its
+ * finite-difference slope; the presynaptic voltages' share summed per presynaptic node
+ * over pre_syn, without atomics).
  *   g_rec:   gradient of the recordings, per program [t][rec][S] at rec_off
  *   g_param: per (node, simulation) contributions to the parameter gradients, program
  *            layout [node][S] at state_off (the caller reduces them per parameter mode);
@@ -346,9 +351,9 @@ int jxhip_fused_forward_noalloc_f64(const jxhip_fused_args* args, int64_t state_
  * adjoints every thread keeps in VGPRs, 1, 2, 4, 8, 12 or 16 with n * S <= items_per_thread
  * * threads, or 0 to keep them in lam_v; coef_global = 1 reads the tree's coupling
  * coefficients from node_f (L2) instead of an LDS copy; lds_bytes = staged schedule +
- * 2 * n * S * sizeof(solve type) + 4 * n (+ 2 * n * sizeof(solve type) with LDS
+ * (2 * n + 2 * n_tgt) * S * sizeof(solve type) (+ 2 * n * sizeof(solve type) with LDS
  * coefficients) <= 160 KiB. The forward that wrote the checkpoints must have used the
- * same S. */
+ * same S and written the synapse states' checkpoints (ckpt_s, KineticSynapse). */
 typedef struct jxhip_adjoint_args {
   const void* g_rec;
   void* g_param[JXHIP_NPARAM];
@@ -364,6 +369,27 @@ typedef struct jxhip_adjoint_args {
   int64_t n_steps;          /* total steps of the forward */
   void* lam_v;              /* adjoint of the voltages, program layout (scratch); required
                                when fwd->items_per_thread == 0, unused otherwise */
+  const int32_t* rec_first; /* per node (n_nodes_total): the first recording of the node
+                               (topology-relative index), INT32_MAX if none */
+  const int32_t* rec_next;  /* per recording (at rec_base): the node's next recording,
+                               INT32_MAX after its last */
+  /* Synapses (fwd->syn_model 0 or 1): */
+  const int32_t* pre_ptr;   /* per topology at pre_ptr_base: n + 1 offsets into pre_syn */
+  const int32_t* pre_syn;   /* topology-relative synapse indices grouped by presynaptic
+                               node, at pre_syn_base */
+  void* work_s;             /* recomputed synapse states, per program [K + 1][syn][S] at
+                               (program index) * work_syn_stride */
+  int64_t work_syn_stride;
+  void* lam_s;              /* adjoint of the synapse states, per program [syn][S] at syn_off
+                               (scratch) */
+  void* work_q;             /* per program [syn][S] at syn_off (scratch): each synapse's
+                               share of the presynaptic voltage's adjoint */
+  void* g_syn_gs;           /* gradient of gS per (synapse, simulation), [syn][S] at syn_off,
+                               added to; NULL = not wanted */
+  void* work_lin;           /* recomputed synaptic slope of every (target, simulation), per
+                               program [K][n_tgt][S] at (program index) * work_lin_stride:
+                               the transposed system's diagonal */
+  int64_t work_lin_stride;
 } jxhip_adjoint_args;
 
 int jxhip_fused_adjoint_f32(const jxhip_fused_args* fwd, const jxhip_adjoint_args* adj,

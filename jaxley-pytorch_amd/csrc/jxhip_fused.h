@@ -152,7 +152,7 @@ struct Prog {
   int64_t t0, t1, soff, simoff, roff, synoff;
 };
 struct Topo {
-  int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb;
+  int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb, ppb, psb;
 };
 
 __device__ __forceinline__ Prog load_prog(const int64_t* progs, int64_t p) {
@@ -186,6 +186,8 @@ __device__ __forceinline__ Topo load_topo(const int32_t* topos, int32_t t) {
   o.n_syn = r[11];
   o.sb = r[12];
   o.scb = r[13];
+  o.ppb = r[14];
+  o.psb = r[15];
   return o;
 }
 
@@ -216,15 +218,22 @@ __device__ __forceinline__ G prm(const Fused<G, L>& a, const Prog& pg, const Top
 // kernels/synapse_step.py:121-182): `lin` = summed slope of i / area * 1e5 (the
 // reference's finite difference with both voltages perturbed by `diff`), `cur` = summed
 // i / area * 1e5 at the unperturbed voltages.
+// `v`: the program's voltages (program layout); `s_in` / `s_out`: the program's synapse
+// states before / after the step ([syn][S]; the same array in the forward).
 template <typename G, typename L>
 __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog& pg,
-                                               const Topo& tp, int tgt, int s, G& lin, G& cur) {
+                                               const Topo& tp, int tgt, int s, G& lin, G& cur,
+                                               const double* v, const G* s_in, G* s_out) {
   G sl = G(0), sc = G(0);
   const G dt = a.dt;
-  const int32_t* sptr = a.syn_ptr + tp.spb;
+  const int32_t* jdp = a.syn_ptr + tp.spb;
   const int S = a.S;
-  const double* v = a.v + pg.soff;
-  for (int syn = sptr[tgt]; syn < sptr[tgt + 1]; ++syn) {
+  for (int j = 0;; ++j) {
+    // The target's j-th synapse (jagged-diagonal order: consecutive targets, consecutive
+    // synapses).
+    const int beg = jdp[j];
+    if (tgt >= jdp[j + 1] - beg) break;
+    const int syn = beg + tgt;
     const int64_t gsyn = tp.synb + syn;
     const int pre = a.syn_pre[gsyn];
     const G vpre = static_cast<G>(v[pre * S + s]);
@@ -238,14 +247,14 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
       const G beta = a.syn_param[3][gsyn];
       const G v_th = a.syn_param[4][gsyn];
       const G delta = a.syn_param[5][gsyn];
-      G* sp = a.syn_s + pg.synoff + static_cast<int64_t>(syn) * S + s;
-      const G st = *sp;
+      const int64_t si = static_cast<int64_t>(syn) * S + s;
+      const G st = s_in[si];
       const G tr = sigmoid((vpre - v_th) / delta);
       const G aa = alpha * tr;
       const G rate = aa + beta;
       const G sinf = aa / rate;
       const G snew = sinf + (st - sinf) * dexp(-dt * rate);
-      *sp = snew;
+      s_out[si] = snew;
       // i = gS s (v_post - e_syn) is linear in v_post: the slope is gS s.
       const G g = gs * snew * scale;
       sl += g;
@@ -260,15 +269,15 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
       const G k_minus = a.syn_param[2][gsyn];
       const G v_th = a.syn_param[3][gsyn];
       const G delta = a.syn_param[4][gsyn];
-      G* sp = a.syn_s + pg.synoff + static_cast<int64_t>(syn) * S + s;
-      const G st = *sp;
+      const int64_t si = static_cast<int64_t>(syn) * S + s;
+      const G st = s_in[si];
       const G u = (vpre - v_th) / delta;
       const G sinf = G(1) / (G(1) + save_exp(-u));
       const G uc = u < G(-20) ? G(-20) : (u > G(50) ? G(50) : u);
       const G rate = k_minus * (G(1) + dexp(uc));
       const G e = save_exp(-rate * dt);
       const G snew = st * e + sinf * (G(1) - e);
-      *sp = snew;
+      s_out[si] = snew;
       const G g = gs * snew * scale;
       sl += g;
       sc += g * static_cast<G>(v[node * S + s] - static_cast<double>(e_syn));
@@ -277,13 +286,13 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
       // delta)), tau = (1 - s_bar) / k_minus, c' = c e + s_bar (1 - e) with e =
       // save_exp(-dt / tau); v_th = -35, delta = 10, k_minus = 1 / 40; i = gC c' v_post.
       const int node = a.tgt_node[tp.tgb + tgt];
-      G* sp = a.syn_s + pg.synoff + static_cast<int64_t>(syn) * S + s;
-      const G st = *sp;
+      const int64_t si = static_cast<int64_t>(syn) * S + s;
+      const G st = s_in[si];
       const G sbar = G(1) / (G(1) + save_exp((G(-35) - vpre) / G(10)));
       const G tau = (G(1) - sbar) / G(1.0 / 40.0);
       const G e = save_exp((G(-1) / tau) * dt);
       const G snew = st * e + sbar * (G(1) - e);
-      *sp = snew;
+      s_out[si] = snew;
       const G g = gs * snew * scale;
       sl += g;
       sc += g * static_cast<G>(v[node * S + s]);
@@ -901,7 +910,10 @@ __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& 
   for (int i = threadIdx.x; i < (tp.n_tgt << lS); i += blockDim.x) {
     const int tgt = i >> lS, s = i & (S - 1);
     G lin = G(0), cur = G(0);
-    if (s < pg.nsim) synapse_target(a, pg, tp, tgt, s, lin, cur);
+    if (s < pg.nsim) {
+      G* st = a.syn_s + pg.synoff;
+      synapse_target(a, pg, tp, tgt, s, lin, cur, a.v + pg.soff, st, st);
+    }
     Ld.tl[i] = static_cast<L>(lin);
     Ld.tc[i] = static_cast<L>(cur);
   }
@@ -1247,7 +1259,37 @@ struct Adj {
   G* ln;
   G* lv;  // adjoint of the voltages when not kept in VGPRs (IPT == 0)
   int64_t work_stride;
+  const int32_t* rec_first;  // per node: first recording, or INT32_MAX
+  const int32_t* rec_next;   // per recording: the node's next one, or INT32_MAX
+  // Synapses: by presynaptic node (CSR), recomputed states, state adjoints, presynaptic
+  // shares, gradient of gS.
+  const int32_t* pre_ptr;
+  const int32_t* pre_syn;
+  G* ws;
+  int64_t work_syn_stride;
+  G* lam_s;
+  G* wq;
+  G* g_gs;
+  G* wl;  // per program [K][n_tgt][S]: the synaptic slope of each recomputed step
+  int64_t work_lin_stride;
 };
+
+// Derivatives of a KineticSynapse step (synapses/kinetic.py:37-53) at the presynaptic
+// voltage vpre: the new state, E = ds'/ds and ds'/dvpre.
+template <typename G>
+__device__ __forceinline__ void kinetic_step_d(G st, G vpre, G alpha, G beta, G v_th, G delta, G dt,
+                                               G& snew, G& E, G& dsdv) {
+  const G tr = sigmoid((vpre - v_th) / delta);
+  const G aa = alpha * tr;
+  const G rate = aa + beta;
+  const G rinv = G(1) / rate;
+  const G sinf = aa * rinv;
+  E = dexp(-dt * rate);
+  snew = sinf + (st - sinf) * E;
+  const G daa = aa * (G(1) - tr) / delta;     // d(alpha T) / dvpre
+  const G dsinf = beta * rinv * rinv * daa;    // d s_inf / dvpre
+  dsdv = dsinf * (G(1) - E) + (st - sinf) * (-dt * E * daa);
+}
 
 // dx'/dv of the exponential-Euler gate step through alpha(v), beta(v); e = dx'/dx.
 template <typename T>
@@ -1269,7 +1311,7 @@ __device__ __forceinline__ T d_gate_dv(T x, T dt, T al, T be, T dal, T dbe, T& e
 // the system of the transposed solve and, per node, its first recording. Per K-step
 // segment (last first): recompute the segment from its checkpoint into the work arrays,
 // then run the adjoint of each step backwards.
-template <typename G, typename L, int IPT, bool kGC>
+template <typename G, typename L, int IPT, bool kGC, bool kSyn>
 __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
@@ -1293,13 +1335,22 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   }
   Ld.xs = reinterpret_cast<L*>(lp);
   Ld.ds = Ld.xs + nS;
-  int* recf = reinterpret_cast<int*>(Ld.ds + nS);  // per node: first recording, or kNone
-  Ld.tl = Ld.xs;  // unused (no synapses in the reverse pass)
-  Ld.tc = Ld.xs;
+  // Synapses: per (target, simulation) the slope and current of the recomputed forward;
+  // in the reverse, the target's dL/dlin and dL/dcst, then its postsynaptic share.
+  Ld.tl = Ld.ds + nS;
+  Ld.tc = Ld.tl + (tp.n_tgt << lS);
   constexpr int kNone = 0x7fffffff;
-  for (int k = tid; k < tp.n; k += nthr) recf[k] = kNone;
-  __syncthreads();
-  for (int r = tid; r < tp.n_rec; r += nthr) atomicMin(&recf[a.rec_node[tp.rb + r]], r);
+  const int ntS = tp.n_tgt << lS;
+  const int64_t nsynS = static_cast<int64_t>(tp.n_syn) << lS;
+  G* ws = kSyn ? g.ws + static_cast<int64_t>(blockIdx.x) * g.work_syn_stride : nullptr;
+  G* lam_s = kSyn ? g.lam_s + pg.synoff : nullptr;
+  G* wq = kSyn ? g.wq + pg.synoff : nullptr;
+  G* wl = kSyn ? g.wl + static_cast<int64_t>(blockIdx.x) * g.work_lin_stride : nullptr;
+  if constexpr (kSyn) {
+    if (a.syn_model == 0) {
+      for (int64_t q = tid; q < nsynS; q += nthr) lam_s[q] = G(0);
+    }
+  }
   const G dt = a.dt;
   const int64_t nsteps = pg.t1 - pg.t0;
   const int64_t K = a.ckpt_every;
@@ -1356,10 +1407,8 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   // Adds the gradient of the recordings at slot t1 to the voltage adjoint of item i.
   auto add_rec = [&](int i, int64_t t1, L& lv) {
     const int k = i >> lS, s = i & (S - 1);
-    int r = recf[k];
-    if (r == kNone) return;
-    for (; r < tp.n_rec; ++r) {
-      if (a.rec_node[tp.rb + r] == k) lv += static_cast<L>(g.g_rec[pg.roff + (t1 * tp.n_rec + r) * S + s]);
+    for (int r = g.rec_first[tp.nb + k]; r != kNone; r = g.rec_next[tp.rb + r]) {
+      lv += static_cast<L>(g.g_rec[pg.roff + (t1 * tp.n_rec + r) * S + s]);
     }
   };
   __syncthreads();
@@ -1379,8 +1428,27 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       wh[i] = a.ckpt_h[coff + i];
       wn[i] = a.ckpt_n[coff + i];
     }
+    if constexpr (kSyn) {
+      if (a.syn_model == 0) {
+        const G* cs = a.ckpt_s + pg.synoff + c * a.ckpt_syn_stride;
+        for (int64_t q = tid; q < nsynS; q += nthr) ws[q] = cs[q];
+      }
+    }
     __syncthreads();
     for (int64_t j = 0; j < te - ts; ++j) {
+      if constexpr (kSyn) {
+        // The synapse step of the recomputed forward (states into work_s[j + 1]).
+        for (int i2 = tid; i2 < ntS; i2 += nthr) {
+          const int tgt = i2 >> lS, s = i2 & (S - 1);
+          G lin = G(0), cur = G(0);
+          if (s < pg.nsim)
+            synapse_target(a, pg, tp, tgt, s, lin, cur, wv + j * nS, ws + j * nsynS, ws + (j + 1) * nsynS);
+          Ld.tl[i2] = static_cast<L>(lin);
+          Ld.tc[i2] = static_cast<L>(cur);
+          wl[j * ntS + i2] = lin;
+        }
+        __syncthreads();
+      }
       for (int i = tid; i < nS; i += nthr) {
         const int k = i >> lS, s = i & (S - 1);
         G m = wm[j * nS + i], h = wh[j * nS + i], n = wn[j * nS + i];
@@ -1416,6 +1484,11 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
                             prm<1>(a, pg, tp, k, s) * (n2 * n2) + prm<2>(a, pg, tp, k, s));
         }
         if (flags & kLeak) lin += G(1000) * prm<6>(a, pg, tp, k, s);
+        if constexpr (kSyn) {
+          // The synaptic slope of the step ending at t1 (a diagonal term of its system).
+          const int tg = nik[kNiTgt];
+          if (tg >= 0) lin += wl[(t1 - ts - 1) * ntS + (tg << lS) + s];
+        }
         Ld.ds[i] = nfk[kNfDbase] + static_cast<L>(dt * static_cast<G>(nfk[kNfInvCm]) * lin);
       } else {
         Ld.ds[i] = nfk[kNfDbase] + ((flags & kComp) ? L(1) : L(0));
@@ -1512,17 +1585,107 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
             grad(6, 3, pi, G(1000) * (gl + gc * eLk));
             grad(7, -1, pi, G(1000) * gc * prm<6>(a, pg, tp, k, s));
           }
+          if constexpr (kSyn) {
+            const int tg = nik[kNiTgt];
+            if (tg >= 0) {
+              Ld.tl[(tg << lS) + s] = static_cast<L>(gl);
+              Ld.tc[(tg << lS) + s] = static_cast<L>(gc);
+            }
+          }
         }
-        // The item's own slots of the earlier step's system (the solve above read every
-        // slot before its closing barrier); at the segment's first step the adjoint waits
-        // for the next segment in VGPRs / HBM.
-        if (j > 0) {
+        if constexpr (kSyn) {
+          lv_set(jj, i, static_cast<L>(lvn));  // completed by the synapse shares below
+        } else if (j > 0) {
+          // The item's own slots of the earlier step's system (the solve above read every
+          // slot before its closing barrier); at the segment's first step the adjoint
+          // waits for the next segment in VGPRs / HBM.
           system(i, ts + j, m0, h0, n0, static_cast<L>(lvn));
         } else {
           lv_set(jj, i, static_cast<L>(lvn));
         }
       }
       __syncthreads();
+      if constexpr (kSyn) {
+        // Synapse adjoint per (target, simulation) (kernels/synapse_step.py:184-330 of
+        // the reference for its semantics): dL/dslope = gl + gc v_post and dL/dcurrent =
+        // -gc at the target; each synapse's state adjoint, gS gradient and share of its
+        // presynaptic voltage's adjoint (work_q); the sigmoid-rate synapses' share of the
+        // postsynaptic voltage's adjoint (the slope multiplies v_post in the right-hand
+        // side) into tl.
+        const int32_t* jdp = a.syn_ptr + tp.spb;
+        const double* v0 = wv + j * nS;
+        const G* s0 = ws + j * nsynS;
+        for (int i2 = tid; i2 < ntS; i2 += nthr) {
+          const int tgt = i2 >> lS, s = i2 & (S - 1);
+          G post = G(0);
+          if (s < pg.nsim) {
+            const G gl = static_cast<G>(Ld.tl[i2]), gc = static_cast<G>(Ld.tc[i2]);
+            const int qn = a.tgt_node[tp.tgb + tgt];
+            for (int q = 0;; ++q) {
+              const int beg = jdp[q];
+              if (tgt >= jdp[q + 1] - beg) break;
+              const int syn = beg + tgt;
+              const int64_t gsyn = tp.synb + syn;
+              const int64_t si = static_cast<int64_t>(syn) * S + s;
+              const G vpre = static_cast<G>(v0[a.syn_pre[gsyn] * S + s]);
+              const G scale = a.syn_scale[gsyn];
+              const G gs = a.syn_param[0][gsyn];
+              if (a.syn_model == 0) {
+                G snew, E, dsdv;
+                kinetic_step_d(s0[si], vpre, a.syn_param[2][gsyn], a.syn_param[3][gsyn],
+                               a.syn_param[4][gsyn], a.syn_param[5][gsyn], dt, snew, E, dsdv);
+                const G dg = gl + gc * a.syn_param[1][gsyn];  // dL/d(gS s' scale)
+                const G lsp = lam_s[si] + dg * gs * scale;     // dL/ds'
+                if (g.g_gs) g.g_gs[pg.synoff + si] += dg * snew * scale;
+                lam_s[si] = lsp * E;
+                wq[si] = lsp * dsdv;
+              } else {
+                const G xo = a.syn_param[1][gsyn];
+                const G slope = a.syn_param[2][gsyn];
+                const G za = (vpre - xo) * slope;
+                const G zb = ((vpre + a.diff) - xo) * slope;
+                const G sa = sigmoid(za);
+                const G sb = sigmoid(zb);
+                const G dsig = (G(1) - sa) * sb * (-dexpm1(-(zb - za)));
+                const G vt = -gs * dsig * scale / a.diff;
+                const G dvt = gl + gc * static_cast<G>(v0[qn * S + s]);  // dL/dslope
+                const G dd0 = -gc;                                       // dL/dcurrent
+                post += gc * vt;
+                const G dvt_dv = -gs * scale / a.diff * slope * (sb * (G(1) - sb) - sa * (G(1) - sa));
+                const G dd0_dv = -gs * scale * slope * sa * (G(1) - sa);
+                wq[si] = dvt * dvt_dv + dd0 * dd0_dv;
+                if (g.g_gs) g.g_gs[pg.synoff + si] += dvt * (-dsig * scale / a.diff) + dd0 * (-sa * scale);
+              }
+            }
+          }
+          Ld.tl[i2] = static_cast<L>(post);
+        }
+        __syncthreads();
+        // The voltage adjoints completed by the synapse shares: the target's own, and the
+        // presynaptic shares of every synapse leaving the node (by presynaptic node).
+        const int32_t* pptr = g.pre_ptr + tp.ppb;
+        const int32_t* psyn = g.pre_syn + tp.psb;
+#pragma unroll 1
+        for (int jj = 0; jj < kJ; ++jj) {
+          const int i = tid + jj * nthr;
+          if (i >= nS) break;
+          const int k = i >> lS, s = i & (S - 1);
+          L lv = lv_get(jj, i);
+          if (s < pg.nsim) {
+            G add = G(0);
+            const int tg = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiTgt];
+            if (tg >= 0) add += static_cast<G>(Ld.tl[(tg << lS) + s]);
+            for (int e = pptr[k]; e < pptr[k + 1]; ++e) add += wq[static_cast<int64_t>(psyn[e]) * S + s];
+            lv = static_cast<L>(static_cast<G>(lv) + add);
+          }
+          if (j > 0) {
+            system(i, ts + j, m0, h0, n0, lv);
+          } else {
+            lv_set(jj, i, lv);
+          }
+        }
+        __syncthreads();
+      }
     }
   }
 #pragma unroll 1
@@ -1648,7 +1811,14 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   const jxhip_fused_args& a = *args;
   if (a.n_progs <= 0) return hipSuccess;
   if (int e = check_args(a)) return e;
-  if (a.syn_model >= 0 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
+  if (a.syn_model > 1 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
+  const bool syn = a.syn_model >= 0;
+  if (adj->rec_first == nullptr || adj->rec_next == nullptr) return hipErrorInvalidValue;
+  if (syn && (adj->pre_ptr == nullptr || adj->pre_syn == nullptr || adj->work_q == nullptr ||
+              adj->work_lin == nullptr || adj->work_lin_stride <= 0))
+    return hipErrorInvalidValue;
+  if (syn && a.syn_model == 0 && (adj->work_s == nullptr || adj->lam_s == nullptr || a.ckpt_s == nullptr))
+    return hipErrorInvalidValue;
   if (a.node_p2 != nullptr) return hipErrorInvalidValue;  // HH / Leak cells only
   if (a.threads > 512) return hipErrorInvalidValue;  // fused_adjoint_kernel's launch bound
   // The reverse pass differentiates the backward-Euler step only.
@@ -1672,10 +1842,24 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   g.ln = static_cast<G*>(adj->lam_n);
   g.lv = static_cast<G*>(adj->lam_v);
   g.work_stride = adj->work_stride;
-#define JXHIP_ADJ(IPT)                                                                         \
-  case IPT:                                                                                    \
-    return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true>, a, stream, f, g) \
-                         : launch_kernel(fused_adjoint_kernel<G, L, IPT, false>, a, stream, f, g);
+  g.rec_first = adj->rec_first;
+  g.rec_next = adj->rec_next;
+  g.pre_ptr = adj->pre_ptr;
+  g.pre_syn = adj->pre_syn;
+  g.ws = static_cast<G*>(adj->work_s);
+  g.work_syn_stride = adj->work_syn_stride;
+  g.lam_s = static_cast<G*>(adj->lam_s);
+  g.wq = static_cast<G*>(adj->work_q);
+  g.g_gs = static_cast<G*>(adj->g_syn_gs);
+  g.wl = static_cast<G*>(adj->work_lin);
+  g.work_lin_stride = adj->work_lin_stride;
+#define JXHIP_ADJ(IPT)                                                                              \
+  case IPT:                                                                                         \
+    if (syn)                                                                                        \
+      return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true, true>, a, stream, f, g) \
+                           : launch_kernel(fused_adjoint_kernel<G, L, IPT, false, true>, a, stream, f, g); \
+    return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true, false>, a, stream, f, g) \
+                         : launch_kernel(fused_adjoint_kernel<G, L, IPT, false, false>, a, stream, f, g);
   switch (a.items_per_thread) {
     JXHIP_ADJ(0)
     JXHIP_ADJ(1)

@@ -23,11 +23,13 @@ from .fused import (ALL_PARAMS, BATCHABLE, ForwardLaunch, adjoint_config, adjoin
 
 class _FusedIntegrate(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, holder, v0, node_p, bparam):
+    def forward(ctx, holder, v0, node_p, bparam, syn_gs):
         st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K, _ = holder["args"]
+        syn_params, syn_scale, syn_s0 = holder["syn"]
         run = ForwardLaunch(st, node_f, node_p.detach(), None if bparam is None else bparam.detach(),
                             bmode, v0.detach(), tuple(g.detach() for g in gates0), stim, stim_scale,
-                            T, dt, B, cfg, ckpt_every=K)
+                            T, dt, B, cfg, None if syn_params is None else [x.detach() for x in syn_params],
+                            syn_scale, None if syn_s0 is None else syn_s0.detach(), ckpt_every=K)
         run.launch()
         holder["run"] = run
         ctx.holder = holder
@@ -66,12 +68,39 @@ class _FusedIntegrate(torch.autograd.Function):
         adj.work_stride = (K + 1) * n * S
         lam_v = torch.empty((P, n, S), dtype=dtype, device=dev)   # used when not in VGPRs
         adj.lam_v = lam_v.data_ptr()
+        adj.rec_first = st.rec_first.data_ptr()
+        adj.rec_next = st.rec_next.data_ptr()
         adj.n_steps = T
+        g_gs = None
+        n_tgt = 0
+        if st.syn is not None:
+            # Synapses: recomputed states per step, state adjoints, presynaptic shares and
+            # the gradient of gS per (synapse, simulation).
+            n_syn, n_tgt = st.syn["n_syn"], st.syn["n_tgt"]
+            kinetic = st.syn["model"] == 0
+            syn_bufs = {
+                "work_s": torch.empty((P, K + 1, n_syn, S) if kinetic else (1,), dtype=dtype, device=dev),
+                "lam_s": torch.empty((P, n_syn, S) if kinetic else (1,), dtype=dtype, device=dev),
+                "work_q": torch.empty((P, n_syn, S), dtype=dtype, device=dev),
+                "work_lin": torch.empty((P, K, n_tgt, S), dtype=dtype, device=dev),
+            }
+            holder["syn_bufs"] = syn_bufs
+            adj.pre_ptr = st.syn["pre_ptr"].data_ptr()
+            adj.pre_syn = st.syn["pre_syn"].data_ptr()
+            adj.work_s = syn_bufs["work_s"].data_ptr()
+            adj.work_syn_stride = (K + 1) * n_syn * S
+            adj.lam_s = syn_bufs["lam_s"].data_ptr()
+            adj.work_q = syn_bufs["work_q"].data_ptr()
+            adj.work_lin = syn_bufs["work_lin"].data_ptr()
+            adj.work_lin_stride = K * n_tgt * S
+            if ctx.needs_input_grad[4]:
+                g_gs = torch.zeros((P, n_syn, S), dtype=dtype, device=dev)
+                adj.g_syn_gs = g_gs.data_ptr()
         args = run.args
         # The reverse pass's own launch shape at the forward's S.
-        acfg = adjoint_config(n, run.B, run.node_f.element_size(), st.sched, S=S)
+        acfg = adjoint_config(n, run.B, run.node_f.element_size(), st.sched, S=S, n_tgt=n_tgt)
         saved = (args.lds_bytes, args.threads, args.items_per_thread, args.coef_global)
-        args.lds_bytes = adjoint_lds_bytes(n, S, run.node_f.element_size(), st.sched, acfg.gcoef)
+        args.lds_bytes = adjoint_lds_bytes(n, S, run.node_f.element_size(), st.sched, acfg.gcoef, n_tgt)
         args.threads, args.items_per_thread, args.coef_global = acfg.threads, acfg.ipt, int(acfg.gcoef)
         f = run.kernel_name("jxhip_fused_adjoint")
         try:
@@ -90,17 +119,25 @@ class _FusedIntegrate(torch.autograd.Function):
             else:
                 g_bparam = torch.stack(cols, -1)
         gv = from_prog(g_v0, run.B) if ctx.needs_input_grad[1] else None
+        # gS is shared by the simulations: its gradient is the sum over them.
+        g_syn = from_prog(g_gs, run.B).sum(0) if g_gs is not None else None
         holder.pop("run", None)
-        return None, gv, g_node_p, g_bparam
+        holder.pop("syn_bufs", None)
+        return None, gv, g_node_p, g_bparam, g_syn
 
 
 def fused_integrate_with_adjoint(st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale,
-                                 T, dt, B, cfg, K, grad_keys=ALL_PARAMS):
+                                 T, dt, B, cfg, K, grad_keys=ALL_PARAMS, syn_params=None, syn_scale=None,
+                                 syn_s0=None):
     """Differentiable fused simulation: returns `(recordings (B, n_rec, T+1), run)`.
-    `grad_keys`: the channel parameters whose gradients are wanted (others are zero)."""
+    `grad_keys`: the channel parameters whose gradients are wanted (others are zero).
+    Networks (one Kinetic or SigmoidRate synapse type): `syn_params` in the kernel's
+    synapse order; gradients flow to `syn_params[0]` (gS)."""
     if any(g.requires_grad for g in gates0):
         raise NotImplementedError("gradients with respect to initial gate states are not supported")
     K = int(max(1, min(K, T)))
-    holder = {"args": (st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K, frozenset(grad_keys))}
-    recs = _FusedIntegrate.apply(holder, v0, node_p, bparam)
+    holder = {"args": (st, node_f, bmode, gates0, stim, stim_scale, T, dt, B, cfg, K, frozenset(grad_keys)),
+              "syn": (syn_params, syn_scale, syn_s0)}
+    syn_gs = syn_params[0] if syn_params is not None else None
+    recs = _FusedIntegrate.apply(holder, v0, node_p, bparam, syn_gs)
     return recs, holder.get("run")

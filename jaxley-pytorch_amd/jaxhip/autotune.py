@@ -103,13 +103,13 @@ def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False, measure=
         # program layout); among the forward shapes with that S the fastest is measured.
         from .fused import adjoint_config
 
-        S = adjoint_config(st.n, B, lsz, st.sched).S
+        S = adjoint_config(st.n, B, lsz, st.sched, n_tgt=n_tgt).S
         cands = [c for c in config_candidates(st.n, n_tgt, B, esize, lsz, st.sched) if c.S == S]
         if measure is not None and _enabled() and len(cands) > 1:
             times = [(measure(c), i) for i, c in enumerate(cands)]
             cfg = cands[min(times)[1]]
         else:
-            cfg = grad_config(st.n, B, lsz, st.sched)
+            cfg = grad_config(st.n, B, lsz, st.sched, n_tgt)
     elif env:
         cfg = parse_config(env)
     elif measure is not None and _enabled():

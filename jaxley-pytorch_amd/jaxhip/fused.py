@@ -83,6 +83,8 @@ class Structure:
     up_edge: np.ndarray
     parent_k: np.ndarray
     sched: int = 0             # bytes of the solve schedule staged in LDS
+    rec_first: Optional[torch.Tensor] = None   # per kernel node: its first recording (reverse pass)
+    rec_next: Optional[torch.Tensor] = None    # per recording: the node's next recording
 
 
 def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, device) -> Structure:
@@ -138,10 +140,24 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         node_i[syn["pre"], 3] |= FLAG_EXPORT
         node_i[syn["tgt"], 3] |= FLAG_EXPORT
     topo_row = np.zeros(_lib.JXHIP_TOPO_COLS, dtype=np.int32)
-    topo_row[:14] = [n, sg["L"], 0, 0, 0, len(rec_k), 0, syn["n_tgt"] if syn else 0, 0, 0, 0,
-                     syn["n_syn"] if syn else 0, 0, 0]
+    topo_row[:16] = [n, sg["L"], 0, 0, 0, len(rec_k), 0, syn["n_tgt"] if syn else 0, 0, 0, 0,
+                     syn["n_syn"] if syn else 0, 0, 0, 0, 0]
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=device)
+    # The recordings of every node as a list (first, next; the reverse pass adds their
+    # gradients).
+    none = np.iinfo(np.int32).max
+    rec_first = np.full(n, none, dtype=np.int64)
+    rec_next = np.full(max(len(rec_k), 1), none, dtype=np.int64)
+    for r in range(len(rec_k) - 1, -1, -1):
+        rec_next[r] = rec_first[rec_k[r]]
+        rec_first[rec_k[r]] = r
     if syn is not None:
+        # The synapses by presynaptic node (CSR over the target-sorted synapse list): the
+        # reverse pass sums each node's presynaptic shares without atomics.
+        pre_np = np.asarray(syn["pre"], dtype=np.int64)
+        order = np.argsort(pre_np, kind="stable")
+        syn["pre_ptr"] = t(np.concatenate([[0], np.cumsum(np.bincount(pre_np, minlength=n))]))
+        syn["pre_syn"] = t(order if len(order) else np.zeros(1, np.int64))
         for key in ("pre", "ptr", "tgt"):
             syn[key] = t(syn[key])
         if syn["kind"] is not None:
@@ -159,13 +175,19 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         up_edge=np.asarray(indexer["map_to_solve_order_upper"])[korder],
         parent_k=parent_k,
         sched=sched_bytes(len(sg["seg"]), sg["L"], int(sg["seg"][-1, 3]) if len(sg["seg"]) else 0),
+        rec_first=t(rec_first), rec_next=t(rec_next),
     )
 
 
 def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray) -> Optional[Dict]:
-    """All synapses of the network as one list sorted by target node (CSR; stable, so the
-    synapses of one target keep the order of the types in the network and of the edges
-    within a type). Several types give the per-synapse model ids of JXHIP_SYN_MIXED."""
+    """All synapses of the network as one list in jagged-diagonal order: the targets
+    (postsynaptic nodes) numbered by non-increasing in-degree (ties by node), and the
+    list holding every target's first synapse, then every target's second one, and so
+    on, so that the j-th synapse of target t is `ptr[j] + t` (for t below
+    `ptr[j + 1] - ptr[j]`; `ptr` ends with a repeated entry). Threads of consecutive
+    targets then read consecutive synapses. A target's synapses keep the order of the
+    types in the network and of the edges within a type (the order the currents are
+    summed in). Several types give the per-synapse model ids of JXHIP_SYN_MIXED."""
     if not module._edges:
         return None
     groups, pre_l, post_l, kind_l = [], [], [], []
@@ -181,9 +203,16 @@ def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray
     post = kidx_of_node[post_nodes]
     pre = kidx_of_node[np.concatenate(pre_l)]
     kind = np.concatenate(kind_l)
-    perm = np.argsort(post, kind="stable")          # synapses sorted by target (CSR)
-    targets, counts = np.unique(post[perm], return_counts=True)
-    ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    order = np.argsort(post, kind="stable")         # synapses grouped by target
+    targets, counts = np.unique(post[order], return_counts=True)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    rank = np.argsort(-counts, kind="stable")        # targets by non-increasing in-degree
+    targets, counts, first = targets[rank], counts[rank], first[rank]
+    per_j = np.array([int((counts > j).sum()) for j in range(int(counts.max()))], dtype=np.int64)
+    ptr = np.concatenate([[0], np.cumsum(per_j), [len(post)]]).astype(np.int32)
+    perm = np.empty(len(post), dtype=np.int64)
+    for j, cnt in enumerate(per_j):
+        perm[ptr[j]:ptr[j] + cnt] = order[first[:cnt] + j]
     node_i[targets, 6] = np.arange(len(targets), dtype=np.int32)
     mixed = len(groups) > 1
     return {
@@ -342,14 +371,15 @@ class AdjointConfig:
     gcoef: bool       # coupling coefficients read from the node table (L2) instead of LDS
 
 
-def adjoint_lds_bytes(n: int, S: int, esize: int, sched: int, gcoef: bool = False) -> int:
+def adjoint_lds_bytes(n: int, S: int, esize: int, sched: int, gcoef: bool = False, n_tgt: int = 0) -> int:
     """LDS of one reverse-pass program: the staged schedule, the tree's two coupling
     coefficients per node (unless read from L2), the transposed system (right-hand side
-    and pivots per (node, simulation)) and the first recording of every node."""
-    return sched + (0 if gcoef else 2 * n * esize) + 2 * n * S * esize + 4 * n
+    and pivots per (node, simulation)) and two synapse terms per (target, simulation)."""
+    return sched + (0 if gcoef else 2 * n * esize) + (2 * n + 2 * n_tgt) * S * esize
 
 
-def adjoint_config(n: int, B: int, esize: int, sched: int = 0, S: Optional[int] = None) -> AdjointConfig:
+def adjoint_config(n: int, B: int, esize: int, sched: int = 0, S: Optional[int] = None,
+                   n_tgt: int = 0) -> AdjointConfig:
     """The reverse pass's launch shape: the largest S (at least 256 programs when the
     batch allows) whose transposed system fits the LDS, the coefficients in LDS when they
     fit too, up to 512 threads, and VGPRs for the thread's voltage adjoints; or the shape
@@ -363,7 +393,7 @@ def adjoint_config(n: int, B: int, esize: int, sched: int = 0, S: Optional[int] 
         cands = [S]
     for s in cands:
         for gcoef in (False, True):
-            if adjoint_lds_bytes(n, s, esize, sched, gcoef) > 160 * 1024:
+            if adjoint_lds_bytes(n, s, esize, sched, gcoef, n_tgt) > 160 * 1024:
                 continue
             threads = int(min(ADJOINT_THREADS_MAX, max(64, 64 * -(-n * s // 64))))
             need = -(-n * s // threads)
@@ -372,12 +402,12 @@ def adjoint_config(n: int, B: int, esize: int, sched: int = 0, S: Optional[int] 
     raise NotImplementedError(f"a topology of {n} nodes does not fit the reverse pass's LDS")
 
 
-def grad_config(n: int, B: int, esize: int, sched: int = 0) -> LaunchConfig:
+def grad_config(n: int, B: int, esize: int, sched: int = 0, n_tgt: int = 0) -> LaunchConfig:
     """A forward launch shape for the checkpointed forward when none is measured: the S
     of the reverse pass (the checkpoints are in its program layout), the streaming kernel
     (coefficients from L2 when the LDS copy does not fit)."""
-    S = adjoint_config(n, B, esize, sched).S
-    fits = lds_bytes(n, 0, S, esize, sched) <= 160 * 1024
+    S = adjoint_config(n, B, esize, sched, n_tgt=n_tgt).S
+    fits = lds_bytes(n, n_tgt, S, esize, sched) <= 160 * 1024
     return LaunchConfig(S, 1024, stream=True, gcoef=not fits)
 
 

@@ -231,10 +231,20 @@ def _per_step_path(module, inp, solver: str) -> bool:
         return True
     if not _needs_grad(inp):
         return False
-    if module._edges or solver != "bwd_euler" or has_pospischil(module):
+    if solver != "bwd_euler" or has_pospischil(module):
         return True
+    syn_grad = []
+    if module._edges:
+        # The fused adjoint covers one KineticSynapse or SigmoidRateSynapse type and the
+        # gradient of its gS.
+        types = {e["type"].name for e in module._edges}
+        kinds = {int(e["type"].kernel_model) for e in module._edges}
+        if len(types) != 1 or not kinds <= {0, 1}:
+            return True
+        name = types.pop()
+        syn_grad = [x for k, x in inp.syn_vals.items() if k != f"{name}_gS"]
     # The fused adjoint: gradients of the HH / Leak parameters and the initial voltages.
-    others = [inp.vals[k] for k in inp.vals if k not in ALL_PARAMS and k != "v"]
+    others = [inp.vals[k] for k in inp.vals if k not in ALL_PARAMS and k != "v"] + syn_grad
     if inp.stim is not None:
         others.append(inp.stim)
     if inp.all_states is not None:
@@ -515,19 +525,30 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         except RuntimeError:
             return float("inf")
 
-    cfg = fused_config(st, B, dtype, torch.tensor([], dtype=user_dtype).element_size(), needs_grad,
-                       measure=measure)
+    try:
+        cfg = fused_config(st, B, dtype, torch.tensor([], dtype=user_dtype).element_size(), needs_grad,
+                           measure=measure)
+    except NotImplementedError:
+        if not needs_grad:
+            raise
+        # A reverse pass whose system does not fit the LDS: the per-step path.
+        from .stepwise import integrate_stepwise
+
+        return integrate_stepwise(module, inp, dt, checkpoint_lengths, device, user_dtype, return_states,
+                                  solver=solver)
     bparam = bparam_for(cfg.S)
     if needs_grad:
         from .adjoint import fused_integrate_with_adjoint
 
-        if st.syn is not None or P.stim_requires_grad:
-            raise RuntimeError("the hand-written adjoint covers HH / Leak cells without "
-                               "synapses (integrate routes other gradients to the per-step path)")
+        if P.stim_requires_grad or (st.syn is not None and st.syn["model"] not in (0, 1)):
+            raise RuntimeError("the hand-written adjoint covers HH / Leak cells and networks with one "
+                               "Kinetic or SigmoidRate synapse type (integrate routes other gradients "
+                               "to the per-step path)")
         K = checkpoint_lengths[-1] if checkpoint_lengths else 100
         grad_keys = [k for k in ALL_PARAMS if k in inp.vals and inp.vals[k].requires_grad]
         recs, final = fused_integrate_with_adjoint(
-            st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K, grad_keys)
+            st, node_f, node_p, bparam, bmode, v0, gates0, stim, stim_scale, inp.T, dt, B, cfg, K, grad_keys,
+            syn_params=syn_params, syn_scale=syn_scale, syn_s0=syn_s0)
     else:
         with torch.no_grad():
             run = ForwardLaunch(st, node_f, node_p.detach(), None if bparam is None else bparam.detach(),

@@ -31,6 +31,45 @@ def _gate(x, dt, a, b):
     return x * e + a * tau * (1 - e)
 
 
+def _sigmoid(x):
+    return 1.0 / (1.0 + np.exp(-x))
+
+
+def _synapse_phase(syn, prm, scale, V, Sst, dt, diff):
+    """Slope and current of every (target, simulation) and the new synapse states, the
+    synapses visited in the kernel's jagged-diagonal order (KineticSynapse and
+    SigmoidRateSynapse)."""
+    ptr, pre, tgtn = syn["ptr"], syn["pre"], syn["tgt"]
+    n_tgt = len(tgtn)
+    TL = np.zeros((n_tgt, V.shape[1]))
+    TC = np.zeros((n_tgt, V.shape[1]))
+    for tgt in range(n_tgt):
+        node = tgtn[tgt]
+        j = 0
+        while tgt < ptr[j + 1] - ptr[j]:
+            q = ptr[j] + tgt
+            vpre = V[pre[q]]
+            if syn["model"] == 0:
+                e_syn, alpha, beta, v_th, delta = (prm[i][q] for i in range(1, 6))
+                a = alpha * _sigmoid((vpre - v_th) / delta)
+                rate = a + beta
+                sinf = a / rate
+                Sst[q] = sinf + (Sst[q] - sinf) * np.exp(-dt * rate)
+                g = prm[0][q] * Sst[q] * scale[q]
+                TL[tgt] += g
+                TC[tgt] += g * (V[node] - e_syn)
+            elif syn["model"] == 1:
+                xo, slope = prm[1][q], prm[2][q]
+                sa = _sigmoid((vpre - xo) * slope)
+                sb = _sigmoid(((vpre + diff) - xo) * slope)
+                TL[tgt] += -prm[0][q] * (sb - sa) * scale[q] / diff
+                TC[tgt] += -prm[0][q] * sa * scale[q]
+            else:
+                raise NotImplementedError("the CPU model covers Kinetic and SigmoidRate synapses")
+            j += 1
+    return TL, TC
+
+
 def run_forward_model(run):
     """Fill `run.rec`, `run.v` and the gates as the kernel would (in place)."""
     from jaxhip.fused import ALL_PARAMS
@@ -61,6 +100,14 @@ def run_forward_model(run):
     scale = None if run.stim_scale is None else _np(run.stim_scale).astype(np.float64)
     scol = _np(st.stim_col)
     ss = np.arange(S)
+    syn = None
+    if st.syn is not None:
+        syn = {"model": st.syn["model"], "ptr": _np(st.syn["ptr"]), "pre": _np(st.syn["pre"]),
+               "tgt": _np(st.syn["tgt"])}
+        sprm = [_np(x).astype(np.float64) for x in run.syn_params]
+        sscale = _np(run.syn_scale).astype(np.float64)
+        n_syn = st.syn["n_syn"]
+        s_all = None if run.syn_s is None else _np(run.syn_s).astype(np.float64).reshape(-1)
 
     def P(j, pg, k):
         if bmode and j in bcol:
@@ -78,10 +125,16 @@ def run_forward_model(run):
         Gn = gn[soff:soff + n * S].reshape(n, S).copy()
         for r in range(n_rec):
             rec[roff + r * S:roff + (r + 1) * S] = V[rec_node[r]]
+        if syn is not None:
+            synoff = pg[7]
+            Sst = (np.zeros((n_syn, S)) if s_all is None
+                   else s_all[synoff:synoff + n_syn * S].reshape(n_syn, S).copy())
         for step in range(t1 - t0):
             t = t0 + step
             D = np.zeros((n, S))
             X = np.zeros((n, S))
+            if syn is not None:
+                TL, TC = _synapse_phase(syn, sprm, sscale, V, Sst, dt, run.args.diff)
             for k in range(n):
                 flags = ni[k, 3]
                 p = ni[k, 0]
@@ -116,12 +169,15 @@ def run_forward_model(run):
                     gl = P(6, pg, k)
                     g += gl
                     cur += gl * (vv - P(7, pg, k))
-                r = ax + dt * nf[k, 3] * (-1000 * cur)
+                lin, net = 1000 * g, -1000 * cur
+                if syn is not None and ni[k, 6] >= 0:
+                    lin, net = lin + TL[ni[k, 6]], net - TC[ni[k, 6]]
+                r = ax + dt * nf[k, 3] * net
                 for e in range(ni[k, 4], ni[k, 5]):
                     sb = stim.shape[2]
                     b = simoff + ss if sb > 1 else np.zeros(S, np.int64)
                     r = r + stim[t, scol[e], b] * scale[e]
-                D[k] = np.where(act, nf[k, 2] + dt * nf[k, 3] * 1000 * g, nf[k, 2] + 1)
+                D[k] = np.where(act, nf[k, 2] + dt * nf[k, 3] * lin, nf[k, 2] + 1)
                 X[k] = np.where(act, r, 0.0)
             # Elimination by segment level, chains bottom-up; 1/d kept in D.
             for lv in range(Lmax + 1):

@@ -258,3 +258,84 @@ def test_server_launch_shapes_respect_lds():
     assert SimServer._fits(fake, small[0], n, sched) is False
     big = 12000
     assert SimServer._candidates(fake, big, 16 * 3000) == []
+
+
+def _irregular_network(synapse: str):
+    """Six ball-and-stick cells (the network workload's cells) with irregular in-degree:
+    cell c receives 1 + (c mod 4) synapses from seeded presynaptic somata, spread over
+    two dendritic compartments for some cells; the oracle model of the same network."""
+    from jaxhip.workloads import C4_LENGTH, C4_RADIUS
+
+    rng = np.random.default_rng(3)
+    n_cells = 6
+    cell = jx.Cell([jx.Branch(jx.Compartment(), 1), jx.Branch(jx.Compartment(), 8)], [-1, 0])
+    cell.set("length", C4_LENGTH)
+    cell.set("radius", C4_RADIUS)
+    cell.branch(0).insert(jx.HH())
+    cell.branch(1).insert(jx.Leak())
+    net = jx.Network([cell] * n_cells)
+    pre_comp, post_comp = [], []
+    for c in range(n_cells):
+        for j in range(1 + c % 4):
+            pre_comp.append(9 * int(rng.integers(0, n_cells)))
+            post_comp.append(9 * c + (3 if j % 2 == 0 or c < 3 else 6))
+    pre_comp, post_comp = np.array(pre_comp), np.array(post_comp)
+    syn = getattr(jx, synapse)()
+    net.add_synapses(net._comp_node[pre_comp], net._comp_node[post_comp], syn)
+    E = len(pre_comp)
+    gs = rng.uniform(0.0, 2e-2, size=E)
+    getattr(net, syn.name).set(f"{syn.name}_gS", gs)
+    om = O.OracleModel([([-1, 0], [1, 8])] * n_cells)
+    om.length = np.tile(C4_LENGTH, n_cells)
+    om.radius = np.tile(C4_RADIUS, n_cells)
+    om.hh_comps = np.arange(n_cells) * 9
+    om.leak_comps = np.setdiff1d(np.arange(9 * n_cells), om.hh_comps)
+    edge = net._edges[0]
+    params = {name: edge["params"][f"{syn.name}_{name}"].copy() for name in syn.param_names}
+    s0 = edge["states"][f"{syn.name}_s"].copy() if synapse == "KineticSynapse" else None
+    om.synapses = [{"type": synapse, "pre": om.internal[pre_comp], "post": om.internal[post_comp],
+                    "params": params, "s0": s0}]
+    return net, om, pre_comp, post_comp
+
+
+def test_synapse_jagged_diagonal_order():
+    """The fused kernels' synapse list: targets by non-increasing in-degree, the j-th
+    synapse of target t at ptr[j] + t, and each target's synapses in edge order."""
+    from jaxhip.fused import build_structure
+
+    net, _, pre_comp, post_comp = _irregular_network("KineticSynapse")
+    st = build_structure(net, np.array([0]), np.zeros(0, np.int64), torch.device("cpu"))
+    syn = st.syn
+    ptr, perm, tgt = syn["ptr"].numpy(), np.asarray(syn["perm"]), syn["tgt"].numpy()
+    post_k = st.kidx_of_node[net._comp_node[post_comp]]
+    deg = []
+    for t in range(syn["n_tgt"]):
+        j, mine = 0, []
+        while t < ptr[j + 1] - ptr[j]:
+            mine.append(perm[ptr[j] + t])
+            j += 1
+        deg.append(j)
+        assert mine == [e for e in range(len(post_k)) if post_k[e] == tgt[t]]
+    assert deg == sorted(deg, reverse=True) and sum(deg) == len(post_k) and len(set(deg)) > 1
+    assert sorted(perm.tolist()) == list(range(len(post_k)))
+
+
+@pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse"])
+def test_host_tables_network(cpu_model, synapse):
+    """A network with irregular in-degree through the host tables and the CPU model of
+    the fused kernel (synapse phase in jagged-diagonal order) against the oracle."""
+    from models import DT
+
+    net, om, _, _ = _irregular_network(synapse)
+    soma = np.arange(6) * 9
+    for c in range(6):
+        net.cell(c).branch(0).comp(0).record()
+    T = 80
+    cur = np.random.default_rng(5).uniform(size=(6, T)) < 0.2
+    cur = cur * 1.5
+    ds = None
+    for c in range(6):
+        ds = net.cell(c).branch(0).comp(0).data_stimulate(torch.tensor(cur[c], dtype=torch.float64), ds)
+    v = jx.integrate(net, data_stimuli=ds, delta_t=DT)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=list(soma), rec_comps=list(soma))
+    assert np.abs(v.numpy() - ref[0].numpy()).max() < 1e-9

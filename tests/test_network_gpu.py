@@ -47,12 +47,22 @@ def test_network_float32(device):
     assert err < 1e-2
 
 
+@pytest.mark.parametrize("path", ["fused", "stepwise"])
 @pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse"])
-def test_network_gradients_vs_oracle_autograd(device, synapse):
-    """Gradients through a network (the differentiable per-step path: synapse operator
-    with its hand-written backward, tree solve with its transposed solve, checkpoints
-    every 40 steps) with respect to every synapse's gS and the somata's gNa, against
-    autograd through the float64 CPU oracle; the recordings against the oracle too."""
+def test_network_gradients_vs_oracle_autograd(device, synapse, path, monkeypatch):
+    """Gradients through a network with respect to every synapse's gS and the somata's
+    gNa (checkpoints every 40 steps), against autograd through the float64 CPU oracle; the
+    recordings against the oracle too. "fused": the hand-written reverse pass of the fused
+    kernel (synapse adjoint and presynaptic shares in the kernel); "stepwise": the
+    differentiable per-step path (synapse operator with its hand-written backward, tree
+    solve with its transposed solve), taken here because the stimulus requires a gradient
+    too."""
+    import jaxhip.stepwise as stepwise
+
+    if path == "fused":
+        def refuse(*a, **k):
+            raise AssertionError("network gradients took the per-step path")
+        monkeypatch.setattr(stepwise, "integrate_stepwise", refuse)
     n_cells, k, T, seed = 8, 3, 240, 6
     net, om, info = build_c4(n_cells, k, seed, synapse)
     soma = info["soma_comps"]
@@ -62,9 +72,9 @@ def test_network_gradients_vs_oracle_autograd(device, synapse):
     ds = None
     for i, c in enumerate(soma):
         ds = net.cell(int(c) // 9).branch(0).comp(0).data_stimulate(
-            torch.tensor(cur[0, i, :], dtype=torch.float64, device=device), ds)
+            torch.tensor(cur[0, i, :], dtype=torch.float64, device=device, requires_grad=path == "stepwise"),
+            ds)
     name = synapse
-    E = n_cells * k
     gs0 = np.asarray(om.synapses[0]["params"]["gS"], dtype=np.float64) * 20.0
     gs = torch.tensor(gs0, dtype=torch.float64, device=device, requires_grad=True)
     gna = torch.full((n_cells,), 0.12, dtype=torch.float64, device=device, requires_grad=True)
@@ -87,10 +97,53 @@ def test_network_gradients_vs_oracle_autograd(device, synapse):
     g2, g2o = gna.grad.cpu(), gna_o.grad[torch.as_tensor(soma)]
     e1 = float((g1 - g1o).abs().max() / g1o.abs().max())
     e2 = float((g2 - g2o).abs().max() / g2o.abs().max())
-    print(f"{synapse}: v err {err_v:.2e}, rel grad err gS {e1:.2e}, gNa {e2:.2e}, "
+    print(f"{synapse} ({path}): v err {err_v:.2e}, rel grad err gS {e1:.2e}, gNa {e2:.2e}, "
           f"|dL/dgS| max {float(g1o.abs().max()):.3e}")
     assert err_v < 1e-6
     assert float(g1o.abs().max()) > 0 and float(g2o.abs().max()) > 0
+    assert e1 < 1e-6 and e2 < 1e-6
+
+
+@pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse"])
+def test_network_fused_adjoint_batched(device, synapse, monkeypatch):
+    """The fused reverse pass on two simulations with per-simulation gNa (gS shared, its
+    gradient summed over the simulations) and a loss on every soma: against autograd
+    through the oracle, relative 1e-6 of the largest gradient."""
+    import jaxhip.stepwise as stepwise
+
+    def refuse(*a, **k):
+        raise AssertionError("network gradients took the per-step path")
+    monkeypatch.setattr(stepwise, "integrate_stepwise", refuse)
+    n_cells, k, T, seed, B = 16, 4, 200, 8, 2
+    net, om, info = build_c4(n_cells, k, seed, synapse)
+    soma = info["soma_comps"]
+    for c in soma:
+        net.cell(int(c) // 9).branch(0).comp(0).record()
+    cur = c4_stimulus(B, n_cells, T, seed + 1) * 4.0
+    ds = None
+    for i, c in enumerate(soma):
+        ds = net.cell(int(c) // 9).branch(0).comp(0).data_stimulate(
+            torch.tensor(cur[:, i, :], dtype=torch.float64, device=device), ds)
+    gs0 = np.asarray(om.synapses[0]["params"]["gS"], dtype=np.float64) * 20.0
+    gna0 = 0.12 * np.random.default_rng(seed).uniform(0.8, 1.2, size=(B, 1))
+    gs = torch.tensor(gs0, dtype=torch.float64, device=device, requires_grad=True)
+    gna = torch.tensor(gna0, dtype=torch.float64, device=device, requires_grad=True)
+    ps = net.data_set(f"{synapse}_gS", gs, None)
+    ps = net.data_set("HH_gNa", gna, ps)
+    w = torch.tensor(np.random.default_rng(seed + 2).normal(size=(B, n_cells, T + 1)), dtype=torch.float64)
+    v = jx.integrate(net, param_state=ps, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[5, 40])
+    (v.cpu() * w).sum().backward()
+    gs_o = torch.tensor(gs0, dtype=torch.float64, requires_grad=True)
+    om.synapses[0]["params"]["gS"] = gs_o
+    gna_o = torch.tensor(gna0, dtype=torch.float64, requires_grad=True)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur), stim_comps=list(soma), rec_comps=list(soma),
+                     overrides={"HH_gNa": gna_o.expand(B, len(om.internal))})
+    (ref * w).sum().backward()
+    err_v = float((v.detach().cpu() - ref.detach()).abs().max())
+    e1 = float((gs.grad.cpu() - gs_o.grad).abs().max() / gs_o.grad.abs().max())
+    e2 = float((gna.grad.cpu() - gna_o.grad).abs().max() / gna_o.grad.abs().max())
+    print(f"{synapse} batched: v err {err_v:.2e}, rel grad err gS {e1:.2e}, gNa {e2:.2e}")
+    assert err_v < 1e-6
     assert e1 < 1e-6 and e2 < 1e-6
 
 
