@@ -180,7 +180,7 @@ int jxhip_synapse_step_backward_f64(const double* v, const double* s, const int3
  * Topologies: int32 (n_topo, JXHIP_TOPO_COLS):
  *   n (nodes), L (max segment level), node_base, seg_level_base, child_base, n_rec,
  *   rec_base, n_tgt, tgt_base, syn_ptr_base, syn_base, n_syn, seg_base, seg_child_base,
- *   pre_ptr_base, pre_syn_base (the reverse pass's synapses by presynaptic node)
+ *   pre_ptr_base (the reverse pass's synapses by presynaptic node), 0
  * Nodes are in "kernel order": segments (maximal single-child chains, top first) grouped
  * by segment level, leaf segments first. A segment's level is 0 if its bottom node has
  * no children, else 1 + the largest level of its child segments.
@@ -338,8 +338,9 @@ int jxhip_fused_forward_noalloc_f64(const jxhip_fused_args* args, int64_t state_
  * segment from its checkpoint (storing the trajectory in work_*), then runs the adjoint
  * of each step backwards: transposed tree solve, adjoint of the linearized currents and
  * of the exponential-Euler gates, and of the synapse step (state update, current and its
- * finite-difference slope; the presynaptic voltages' share summed per presynaptic node
- * over pre_syn, without atomics).
+ * finite-difference slope; each synapse's share of its presynaptic voltage's adjoint
+ * is stored at its slot of the presynaptic grouping, pre_pos, and summed per node over
+ * the node's contiguous slots, without atomics).
  *   g_rec:   gradient of the recordings, per program [t][rec][S] at rec_off
  *   g_param: per (node, simulation) contributions to the parameter gradients, program
  *            layout [node][S] at state_off (the caller reduces them per parameter mode);
@@ -374,16 +375,16 @@ typedef struct jxhip_adjoint_args {
   const int32_t* rec_next;  /* per recording (at rec_base): the node's next recording,
                                INT32_MAX after its last */
   /* Synapses (fwd->syn_model 0 or 1): */
-  const int32_t* pre_ptr;   /* per topology at pre_ptr_base: n + 1 offsets into pre_syn */
-  const int32_t* pre_syn;   /* topology-relative synapse indices grouped by presynaptic
-                               node, at pre_syn_base */
+  const int32_t* pre_ptr;   /* per topology at pre_ptr_base: n + 1 offsets, the slots of
+                               the synapses leaving each node */
+  const int32_t* pre_pos;   /* per synapse at syn_base: its slot (topology-relative) */
   void* work_s;             /* recomputed synapse states, per program [K + 1][syn][S] at
                                (program index) * work_syn_stride */
   int64_t work_syn_stride;
   void* lam_s;              /* adjoint of the synapse states, per program [syn][S] at syn_off
                                (scratch) */
-  void* work_q;             /* per program [syn][S] at syn_off (scratch): each synapse's
-                               share of the presynaptic voltage's adjoint */
+  void* work_q;             /* per program [slot][S] at syn_off (scratch): each synapse's
+                               share of the presynaptic voltage's adjoint, at its slot */
   void* g_syn_gs;           /* gradient of gS per (synapse, simulation), [syn][S] at syn_off,
                                added to; NULL = not wanted */
   void* work_lin;           /* recomputed synaptic slope of every (target, simulation), per

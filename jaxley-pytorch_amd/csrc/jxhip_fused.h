@@ -152,7 +152,7 @@ struct Prog {
   int64_t t0, t1, soff, simoff, roff, synoff;
 };
 struct Topo {
-  int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb, ppb, psb;
+  int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb, ppb;
 };
 
 __device__ __forceinline__ Prog load_prog(const int64_t* progs, int64_t p) {
@@ -187,7 +187,6 @@ __device__ __forceinline__ Topo load_topo(const int32_t* topos, int32_t t) {
   o.sb = r[12];
   o.scb = r[13];
   o.ppb = r[14];
-  o.psb = r[15];
   return o;
 }
 
@@ -1264,7 +1263,7 @@ struct Adj {
   // Synapses: by presynaptic node (CSR), recomputed states, state adjoints, presynaptic
   // shares, gradient of gS.
   const int32_t* pre_ptr;
-  const int32_t* pre_syn;
+  const int32_t* pre_pos;  // per synapse: its slot in the presynaptic grouping
   G* ws;
   int64_t work_syn_stride;
   G* lam_s;
@@ -1638,7 +1637,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
                 const G lsp = lam_s[si] + dg * gs * scale;     // dL/ds'
                 if (g.g_gs) g.g_gs[pg.synoff + si] += dg * snew * scale;
                 lam_s[si] = lsp * E;
-                wq[si] = lsp * dsdv;
+                wq[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = lsp * dsdv;
               } else {
                 const G xo = a.syn_param[1][gsyn];
                 const G slope = a.syn_param[2][gsyn];
@@ -1653,7 +1652,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
                 post += gc * vt;
                 const G dvt_dv = -gs * scale / a.diff * slope * (sb * (G(1) - sb) - sa * (G(1) - sa));
                 const G dd0_dv = -gs * scale * slope * sa * (G(1) - sa);
-                wq[si] = dvt * dvt_dv + dd0 * dd0_dv;
+                wq[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = dvt * dvt_dv + dd0 * dd0_dv;
                 if (g.g_gs) g.g_gs[pg.synoff + si] += dvt * (-dsig * scale / a.diff) + dd0 * (-sa * scale);
               }
             }
@@ -1664,7 +1663,6 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         // The voltage adjoints completed by the synapse shares: the target's own, and the
         // presynaptic shares of every synapse leaving the node (by presynaptic node).
         const int32_t* pptr = g.pre_ptr + tp.ppb;
-        const int32_t* psyn = g.pre_syn + tp.psb;
 #pragma unroll 1
         for (int jj = 0; jj < kJ; ++jj) {
           const int i = tid + jj * nthr;
@@ -1675,7 +1673,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
             G add = G(0);
             const int tg = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiTgt];
             if (tg >= 0) add += static_cast<G>(Ld.tl[(tg << lS) + s]);
-            for (int e = pptr[k]; e < pptr[k + 1]; ++e) add += wq[static_cast<int64_t>(psyn[e]) * S + s];
+            for (int e = pptr[k]; e < pptr[k + 1]; ++e) add += wq[static_cast<int64_t>(e) * S + s];
             lv = static_cast<L>(static_cast<G>(lv) + add);
           }
           if (j > 0) {
@@ -1814,7 +1812,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (a.syn_model > 1 || a.ckpt_v == nullptr || a.ckpt_every <= 0) return hipErrorInvalidValue;
   const bool syn = a.syn_model >= 0;
   if (adj->rec_first == nullptr || adj->rec_next == nullptr) return hipErrorInvalidValue;
-  if (syn && (adj->pre_ptr == nullptr || adj->pre_syn == nullptr || adj->work_q == nullptr ||
+  if (syn && (adj->pre_ptr == nullptr || adj->pre_pos == nullptr || adj->work_q == nullptr ||
               adj->work_lin == nullptr || adj->work_lin_stride <= 0))
     return hipErrorInvalidValue;
   if (syn && a.syn_model == 0 && (adj->work_s == nullptr || adj->lam_s == nullptr || a.ckpt_s == nullptr))
@@ -1845,7 +1843,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   g.rec_first = adj->rec_first;
   g.rec_next = adj->rec_next;
   g.pre_ptr = adj->pre_ptr;
-  g.pre_syn = adj->pre_syn;
+  g.pre_pos = adj->pre_pos;
   g.ws = static_cast<G*>(adj->work_s);
   g.work_syn_stride = adj->work_syn_stride;
   g.lam_s = static_cast<G*>(adj->lam_s);

@@ -60,7 +60,7 @@ class AdjointArgs(C.Structure):
         ("work_v", _p), ("work_m", _p), ("work_h", _p), ("work_n", _p),
         ("lam_m", _p), ("lam_h", _p), ("lam_n", _p), ("work_stride", _i64), ("n_steps", _i64),
         ("lam_v", _p), ("rec_first", _p), ("rec_next", _p),
-        ("pre_ptr", _p), ("pre_syn", _p), ("work_s", _p), ("work_syn_stride", _i64),
+        ("pre_ptr", _p), ("pre_pos", _p), ("work_s", _p), ("work_syn_stride", _i64),
         ("lam_s", _p), ("work_q", _p), ("g_syn_gs", _p), ("work_lin", _p),
         ("work_lin_stride", _i64),
     ]

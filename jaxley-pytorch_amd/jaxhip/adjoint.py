@@ -86,7 +86,7 @@ class _FusedIntegrate(torch.autograd.Function):
             }
             holder["syn_bufs"] = syn_bufs
             adj.pre_ptr = st.syn["pre_ptr"].data_ptr()
-            adj.pre_syn = st.syn["pre_syn"].data_ptr()
+            adj.pre_pos = st.syn["pre_pos"].data_ptr()
             adj.work_s = syn_bufs["work_s"].data_ptr()
             adj.work_syn_stride = (K + 1) * n_syn * S
             adj.lam_s = syn_bufs["lam_s"].data_ptr()

@@ -152,12 +152,15 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         rec_next[r] = rec_first[rec_k[r]]
         rec_first[rec_k[r]] = r
     if syn is not None:
-        # The synapses by presynaptic node (CSR over the target-sorted synapse list): the
-        # reverse pass sums each node's presynaptic shares without atomics.
+        # The synapses grouped by presynaptic node: every synapse's slot in the grouping
+        # (the reverse pass stores its presynaptic share there and sums each node's
+        # contiguous slots, without atomics).
         pre_np = np.asarray(syn["pre"], dtype=np.int64)
         order = np.argsort(pre_np, kind="stable")
+        pos = np.empty(len(order), dtype=np.int64)
+        pos[order] = np.arange(len(order))
         syn["pre_ptr"] = t(np.concatenate([[0], np.cumsum(np.bincount(pre_np, minlength=n))]))
-        syn["pre_syn"] = t(order if len(order) else np.zeros(1, np.int64))
+        syn["pre_pos"] = t(pos if len(pos) else np.zeros(1, np.int64))
         for key in ("pre", "ptr", "tgt"):
             syn[key] = t(syn[key])
         if syn["kind"] is not None:
