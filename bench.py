@@ -319,13 +319,13 @@ def bench_swc(device, ncomp=4, B=2048, T=2000, dtype=torch.float64):
     return res
 
 
-def bench_network_fwd_adj(device, n_cells=1024, k=32, B=64, T=1000, K=100, dtype=torch.float32):
+def bench_network_fwd_adj(device, n_cells=1024, k=32, B=64, T=4000, K=100, dtype=torch.float32):
     """Forward + adjoint of the network workload with every synapse's gS trainable (shared
     by the B simulations): a loss on the somatic voltages, checkpoints every K steps,
     through the fused kernels (checkpointed forward, then the hand-written reverse pass
-    with the synapse adjoint and the presynaptic shares in the kernel). T = 1000 of the
-    workload's 4000 steps keeps the benchmark short; the rate per step does not depend
-    on T."""
+    with the synapse adjoint and the presynaptic shares in the kernel); the network is
+    split into parts that advance in lockstep, so the B = 64 simulations fill the
+    device's compute units."""
     net, info = c4_network(n_cells, k, seed=9)
     soma = info["soma_comps"]
     for c in soma:

@@ -244,7 +244,8 @@ typedef struct jxhip_fused_args {
      pointer list ends with a repeated entry) */
   int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate, 2 ionotropic, 3 test
                                synapse, JXHIP_SYN_MIXED: per synapse in syn_kind */
-  const int32_t* syn_pre;   /* topology-relative presynaptic node, at syn_base */
+  const int32_t* syn_pre;   /* presynaptic node (of the group's nodes: the topology's own
+                               unless the network is split), at syn_base */
   const int32_t* syn_ptr;   /* max in-degree + 2 per topology at syn_ptr_base */
   const int32_t* tgt_node;  /* topology-relative target node, at tgt_base */
   const void* syn_param[JXHIP_NSYNPARAM]; /* per synapse at syn_base */
@@ -311,6 +312,21 @@ typedef struct jxhip_fused_args {
   const void* node_p2;
   void* gate_ext;
   int64_t gate_ext_stride;
+  /* Networks split into parts (groups of whole cells, one topology each, node_base the
+     part's first node): a group of S simulations is run by `parts` consecutive programs,
+     one per part, whose [node][S] state blocks lie back to back (group_nodes nodes in
+     all, group block at state_off - node_base * S); syn_pre indexes the group's nodes.
+     parts 0 or 1: one program per group. With coop = 1 (synapses between parts) the
+     programs of a group advance in lockstep: the launch is cooperative (in consecutive
+     launches of whole groups when not all programs fit the device at once), every step
+     ends with a grid-wide barrier, and the presynaptic voltages are read from vx, the
+     exported voltages in the state layout by step parity ([2][vx_stride], float64).
+     Streaming kernel only (items_per_thread 0). */
+  int32_t parts;
+  int32_t coop;
+  int64_t group_nodes;
+  void* vx;
+  int64_t vx_stride;
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);
@@ -346,7 +362,9 @@ int jxhip_fused_forward_noalloc_f64(const jxhip_fused_args* args, int64_t state_
  *            layout [node][S] at state_off (the caller reduces them per parameter mode);
  *            NULL entries are skipped
  *   g_v0:    gradient of the initial voltages, program layout
- *   work_*:  per program [K + 1][node][S] at (program index) * work_stride
+ *   work_*:  per group of simulations [K + 1][group node][S] at (group index) * work_stride
+ *            (a group's programs: the parts of a split network, else one program;
+ *            the index is relative to the launch)
  *   lam_*:   adjoint gates, program layout (scratch)
  * Launch of the reverse pass (fields of fwd): threads <= 512; items_per_thread = the voltage
  * adjoints every thread keeps in VGPRs, 1, 2, 4, 8, 12 or 16 with n * S <= items_per_thread
@@ -391,6 +409,8 @@ typedef struct jxhip_adjoint_args {
                                program [K][n_tgt][S] at (program index) * work_lin_stride:
                                the transposed system's diagonal */
   int64_t work_lin_stride;
+  int64_t work_q_stride;    /* elements between work_q's two step-parity buffers (split
+                               networks: parts write each other's slots); 0 = one buffer */
 } jxhip_adjoint_args;
 
 int jxhip_fused_adjoint_f32(const jxhip_fused_args* fwd, const jxhip_adjoint_args* adj,

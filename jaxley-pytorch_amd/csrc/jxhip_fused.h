@@ -26,6 +26,8 @@
 // whole run, so gates and voltages of its items stay in VGPRs; a float64 copy of the
 // voltages in HBM gives neighbours' voltages to the axial currents.
 #pragma once
+#include <hip/hip_cooperative_groups.h>
+
 #include "jxhip_common.h"
 #include "../../include/jxhip.h"
 
@@ -88,6 +90,13 @@ struct Fused {
   const G* node_p2;  // (n_nodes_total, JXHIP_NPARAM2) Pospischil parameters, or null
   G* gx;             // Pospischil gates m, h (Na), n (K), p (Km) at q * gx_stride + item
   int64_t gx_stride;
+  // Split networks: programs per group of S simulations, nodes of a group, and with coop
+  // the grid-wide barrier per step and the exported voltages by step parity.
+  int32_t parts;
+  int32_t coop;
+  int64_t group_nodes;
+  double* vx;
+  int64_t vx_stride;
 };
 
 template <typename G, typename L>
@@ -144,6 +153,11 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.ckpt_every = a.ckpt_every;
   f.ckpt_stride = a.ckpt_stride;
   f.ckpt_syn_stride = a.ckpt_syn_stride;
+  f.parts = a.parts > 1 ? a.parts : 1;
+  f.coop = a.coop;
+  f.group_nodes = a.group_nodes;
+  f.vx = static_cast<double*>(a.vx);
+  f.vx_stride = a.vx_stride;
   return f;
 }
 
@@ -151,6 +165,10 @@ struct Prog {
   int32_t topo, nsim;
   int64_t t0, t1, soff, simoff, roff, synoff;
 };
+
+// Barrier of every workgroup of a cooperative launch (the programs of split networks):
+// the writes of all workgroups before it are visible to all after it.
+__device__ __forceinline__ void grid_barrier() { cooperative_groups::this_grid().sync(); }
 struct Topo {
   int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb, ppb;
 };
@@ -222,7 +240,8 @@ __device__ __forceinline__ G prm(const Fused<G, L>& a, const Prog& pg, const Top
 template <typename G, typename L>
 __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog& pg,
                                                const Topo& tp, int tgt, int s, G& lin, G& cur,
-                                               const double* v, const G* s_in, G* s_out) {
+                                               const double* vpre_g, const double* v,
+                                               const G* s_in, G* s_out) {
   G sl = G(0), sc = G(0);
   const G dt = a.dt;
   const int32_t* jdp = a.syn_ptr + tp.spb;
@@ -235,7 +254,7 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
     const int syn = beg + tgt;
     const int64_t gsyn = tp.synb + syn;
     const int pre = a.syn_pre[gsyn];
-    const G vpre = static_cast<G>(v[pre * S + s]);
+    const G vpre = static_cast<G>(vpre_g[static_cast<int64_t>(pre) * S + s]);
     const G scale = a.syn_scale[gsyn];
     const G gs = a.syn_param[0][gsyn];
     const int model = a.syn_kind != nullptr ? a.syn_kind[gsyn] : a.syn_model;
@@ -901,9 +920,12 @@ __device__ __forceinline__ void idle_item(const Fused<G, L>& a, const Topo& tp, 
   Ld.xs[i] = L(0);
 }
 
+// Synapse step of every (target, simulation) of a program; the presynaptic voltages are
+// read from the group block vpre_g (the program's own block of a.v, or the exported
+// voltages of all parts of a split network).
 template <typename G, typename L>
 __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
-                                              const Lds<L>& Ld) {
+                                              const Lds<L>& Ld, const double* vpre_g) {
   if (a.syn_model < 0) return;
   const int lS = a.log2S, S = a.S;
   for (int i = threadIdx.x; i < (tp.n_tgt << lS); i += blockDim.x) {
@@ -911,7 +933,7 @@ __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& 
     G lin = G(0), cur = G(0);
     if (s < pg.nsim) {
       G* st = a.syn_s + pg.synoff;
-      synapse_target(a, pg, tp, tgt, s, lin, cur, a.v + pg.soff, st, st);
+      synapse_target(a, pg, tp, tgt, s, lin, cur, vpre_g, a.v + pg.soff, st, st);
     }
     Ld.tl[i] = static_cast<L>(lin);
     Ld.tc[i] = static_cast<L>(cur);
@@ -1058,6 +1080,14 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     axg[i] = static_cast<G>(axial<kCS>(a, tp, ni + k * JXHIP_NODE_ICOLS, k, Ld.cl, vg, s, vg[i]));
   }
   record_step(a, pg, tp, vg, 0);
+  // The group's block of the state layout (the programs of a split network share it).
+  const int64_t gbase = pg.soff - (static_cast<int64_t>(tp.nb) << lS);
+  if (kSyn && a.coop) {
+    // Split network: every part's voltages at the start into the exported copy of step 0.
+    for (int i = tid; i < nS; i += nthr) a.vx[pg.soff + i] = vg[i];
+    __syncthreads();
+    grid_barrier();
+  }
   const int64_t nsteps = pg.t1 - pg.t0;
   for (int64_t step = 0; step < nsteps; ++step) {
     const bool pend = step > 0;  // the last solve's increments wait in LDS
@@ -1066,7 +1096,9 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     if constexpr (kSyn && kCkpt) {
       if (ck) checkpoint_syn(a, pg, tp, step / a.ckpt_every);
     }
-    if constexpr (kSyn) synapse_phase(a, pg, tp, Ld);
+    if constexpr (kSyn) {
+      synapse_phase(a, pg, tp, Ld, a.coop ? a.vx + (step & 1) * a.vx_stride + gbase : a.v + gbase);
+    }
     const int64_t t = pg.t0 + step;
     if (tid < nS) {
       ItemState<G> cur = load_state(a, pg, tid);
@@ -1117,12 +1149,18 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
         const int inx = i + nthr < nS ? i + nthr : i;
         const UpdConst cn = load_update_const<kCS>(tp, ni, Ld.cl, inx >> lS);  // next item's
         const double dv = static_cast<double>(Ld.xs[i]);
-        if (last || (c.r0.w & kExport)) vg[i] += dv;
+        if (last || (c.r0.w & kExport)) {
+          vg[i] += dv;
+          if (kSyn && a.coop) a.vx[((step + 1) & 1) * a.vx_stride + pg.soff + i] = vg[i];
+        }
         if (!last) Ld.ds[i] = static_cast<L>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
         c = cn;
       }
     }
     __syncthreads();
+    // Split network: every part's exported voltages of the step written before any part
+    // starts the next step's synapses.
+    if (kSyn && a.coop) grid_barrier();
     record_step(a, pg, tp, vg, step + 1);
   }
 }
@@ -1192,7 +1230,7 @@ __global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
         if constexpr (kSyn) checkpoint_syn(a, pg, tp, c);
       }
     }
-    if constexpr (kSyn) synapse_phase(a, pg, tp, Ld);
+    if constexpr (kSyn) synapse_phase(a, pg, tp, Ld, a.v + pg.soff - (static_cast<int64_t>(tp.nb) << lS));
     const int64_t t = pg.t0 + step;
 #pragma unroll 1
     for (int j = 0; j < IPT; ++j) {
@@ -1271,6 +1309,7 @@ struct Adj {
   G* g_gs;
   G* wl;  // per program [K][n_tgt][S]: the synaptic slope of each recomputed step
   int64_t work_lin_stride;
+  int64_t wq_stride;  // elements between the two step-parity buffers of wq (0: one buffer)
 };
 
 // Derivatives of a KineticSynapse step (synapses/kinetic.py:37-53) at the presynaptic
@@ -1343,7 +1382,9 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   const int64_t nsynS = static_cast<int64_t>(tp.n_syn) << lS;
   G* ws = kSyn ? g.ws + static_cast<int64_t>(blockIdx.x) * g.work_syn_stride : nullptr;
   G* lam_s = kSyn ? g.lam_s + pg.synoff : nullptr;
-  G* wq = kSyn ? g.wq + pg.synoff : nullptr;
+  // Presynaptic shares: per group of simulations, slots of the group's presynaptic
+  // grouping (a split network's parts write each other's), by step parity.
+  G* wq = kSyn ? g.wq + pg.synoff - (static_cast<int64_t>(tp.synb) << lS) : nullptr;
   G* wl = kSyn ? g.wl + static_cast<int64_t>(blockIdx.x) * g.work_lin_stride : nullptr;
   if constexpr (kSyn) {
     if (a.syn_model == 0) {
@@ -1354,7 +1395,12 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   const int64_t nsteps = pg.t1 - pg.t0;
   const int64_t K = a.ckpt_every;
   const int64_t C = (nsteps + K - 1) / K;
-  const int64_t wbase = static_cast<int64_t>(blockIdx.x) * g.work_stride;
+  // Recomputed trajectory: per group of simulations [K + 1][group nodes][S] (the parts of
+  // a split network side by side, so presynaptic voltages of other parts are in reach);
+  // the program's own nodes at its node base.
+  const int64_t rowS = (a.group_nodes > 0 ? a.group_nodes : static_cast<int64_t>(tp.n)) << lS;
+  const int64_t gwork = static_cast<int64_t>(blockIdx.x / a.parts) * g.work_stride;
+  const int64_t wbase = gwork + (static_cast<int64_t>(tp.nb) << lS);
   G* lm = g.lm + pg.soff;
   G* lh = g.lh + pg.soff;
   G* ln = g.ln + pg.soff;
@@ -1412,6 +1458,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
   };
   __syncthreads();
   double* wv = g.wv + wbase;
+  const double* wvg = g.wv + gwork;  // the group's block (presynaptic voltages)
   G* wm = g.wm + wbase;
   G* wh = g.wh + wbase;
   G* wn = g.wn + wbase;
@@ -1434,6 +1481,9 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       }
     }
     __syncthreads();
+    // Split network: every part's checkpoint loaded before any part reads presynaptic
+    // voltages from it (the same after every recomputed step).
+    if (kSyn && a.coop) grid_barrier();
     for (int64_t j = 0; j < te - ts; ++j) {
       if constexpr (kSyn) {
         // The synapse step of the recomputed forward (states into work_s[j + 1]).
@@ -1441,7 +1491,8 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
           const int tgt = i2 >> lS, s = i2 & (S - 1);
           G lin = G(0), cur = G(0);
           if (s < pg.nsim)
-            synapse_target(a, pg, tp, tgt, s, lin, cur, wv + j * nS, ws + j * nsynS, ws + (j + 1) * nsynS);
+            synapse_target(a, pg, tp, tgt, s, lin, cur, wvg + j * rowS, wv + j * rowS, ws + j * nsynS,
+                           ws + (j + 1) * nsynS);
           Ld.tl[i2] = static_cast<L>(lin);
           Ld.tc[i2] = static_cast<L>(cur);
           wl[j * ntS + i2] = lin;
@@ -1450,20 +1501,21 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       }
       for (int i = tid; i < nS; i += nthr) {
         const int k = i >> lS, s = i & (S - 1);
-        G m = wm[j * nS + i], h = wh[j * nS + i], n = wn[j * nS + i];
+        G m = wm[j * rowS + i], h = wh[j * rowS + i], n = wn[j * rowS + i];
         if (s < pg.nsim) {
-          build_item<kCS>(a, pg, tp, Ld, wv + j * nS, k, s, wv[j * nS + i], pg.t0 + ts + j, m, h, n);
+          build_item<kCS>(a, pg, tp, Ld, wv + j * rowS, k, s, wv[j * rowS + i], pg.t0 + ts + j, m, h, n);
         } else {
           idle_item(a, tp, Ld, k, i);
         }
-        wm[(j + 1) * nS + i] = m;
-        wh[(j + 1) * nS + i] = h;
-        wn[(j + 1) * nS + i] = n;
+        wm[(j + 1) * rowS + i] = m;
+        wh[(j + 1) * rowS + i] = h;
+        wn[(j + 1) * rowS + i] = n;
       }
       __syncthreads();
       tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
-      for (int i = tid; i < nS; i += nthr) wv[(j + 1) * nS + i] = wv[j * nS + i] + static_cast<double>(Ld.xs[i]);
+      for (int i = tid; i < nS; i += nthr) wv[(j + 1) * rowS + i] = wv[j * rowS + i] + static_cast<double>(Ld.xs[i]);
       __syncthreads();
+      if (kSyn && a.coop) grid_barrier();
     }
     // The transposed system of a step: its diagonal at the gates after the step, its
     // right-hand side the voltage adjoint at the step's end (recordings included). The
@@ -1501,20 +1553,20 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       for (int jj = 0; jj < kJ; ++jj) {
         const int i = tid + jj * nthr;
         if (i >= nS) break;
-        system(i, te, wm + jl * nS, wh + jl * nS, wn + jl * nS, lv_get(jj, i));
+        system(i, te, wm + jl * rowS, wh + jl * rowS, wn + jl * rowS, lv_get(jj, i));
       }
       __syncthreads();
     }
     for (int64_t j = te - ts - 1; j >= 0; --j) {
       tree_solve<G, L, true, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
-      const double* v0 = wv + j * nS;
-      const double* v1 = wv + (j + 1) * nS;
-      const G* m0 = wm + j * nS;
-      const G* h0 = wh + j * nS;
-      const G* n0 = wn + j * nS;
-      const G* m1 = wm + (j + 1) * nS;
-      const G* h1 = wh + (j + 1) * nS;
-      const G* n1 = wn + (j + 1) * nS;
+      const double* v0 = wv + j * rowS;
+      const double* v1 = wv + (j + 1) * rowS;
+      const G* m0 = wm + j * rowS;
+      const G* h0 = wh + j * rowS;
+      const G* n0 = wn + j * rowS;
+      const G* m1 = wm + (j + 1) * rowS;
+      const G* h1 = wh + (j + 1) * rowS;
+      const G* n1 = wn + (j + 1) * rowS;
 #pragma unroll 1
       for (int jj = 0; jj < kJ; ++jj) {
         const int i = tid + jj * nthr;
@@ -1612,8 +1664,10 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         // postsynaptic voltage's adjoint (the slope multiplies v_post in the right-hand
         // side) into tl.
         const int32_t* jdp = a.syn_ptr + tp.spb;
-        const double* v0 = wv + j * nS;
+        const double* v0 = wv + j * rowS;
+        const double* vg0 = wvg + j * rowS;
         const G* s0 = ws + j * nsynS;
+        G* wqp = wq + (j & 1) * g.wq_stride;
         for (int i2 = tid; i2 < ntS; i2 += nthr) {
           const int tgt = i2 >> lS, s = i2 & (S - 1);
           G post = G(0);
@@ -1626,7 +1680,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
               const int syn = beg + tgt;
               const int64_t gsyn = tp.synb + syn;
               const int64_t si = static_cast<int64_t>(syn) * S + s;
-              const G vpre = static_cast<G>(v0[a.syn_pre[gsyn] * S + s]);
+              const G vpre = static_cast<G>(vg0[static_cast<int64_t>(a.syn_pre[gsyn]) * S + s]);
               const G scale = a.syn_scale[gsyn];
               const G gs = a.syn_param[0][gsyn];
               if (a.syn_model == 0) {
@@ -1637,7 +1691,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
                 const G lsp = lam_s[si] + dg * gs * scale;     // dL/ds'
                 if (g.g_gs) g.g_gs[pg.synoff + si] += dg * snew * scale;
                 lam_s[si] = lsp * E;
-                wq[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = lsp * dsdv;
+                wqp[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = lsp * dsdv;
               } else {
                 const G xo = a.syn_param[1][gsyn];
                 const G slope = a.syn_param[2][gsyn];
@@ -1652,7 +1706,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
                 post += gc * vt;
                 const G dvt_dv = -gs * scale / a.diff * slope * (sb * (G(1) - sb) - sa * (G(1) - sa));
                 const G dd0_dv = -gs * scale * slope * sa * (G(1) - sa);
-                wq[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = dvt * dvt_dv + dd0 * dd0_dv;
+                wqp[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = dvt * dvt_dv + dd0 * dd0_dv;
                 if (g.g_gs) g.g_gs[pg.synoff + si] += dvt * (-dsig * scale / a.diff) + dd0 * (-sa * scale);
               }
             }
@@ -1660,6 +1714,9 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
           Ld.tl[i2] = static_cast<L>(post);
         }
         __syncthreads();
+        // Split network: the presynaptic shares of every part written before any part
+        // sums its nodes' slots.
+        if (a.coop) grid_barrier();
         // The voltage adjoints completed by the synapse shares: the target's own, and the
         // presynaptic shares of every synapse leaving the node (by presynaptic node).
         const int32_t* pptr = g.pre_ptr + tp.ppb;
@@ -1673,7 +1730,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
             G add = G(0);
             const int tg = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiTgt];
             if (tg >= 0) add += static_cast<G>(Ld.tl[(tg << lS) + s]);
-            for (int e = pptr[k]; e < pptr[k + 1]; ++e) add += wq[static_cast<int64_t>(e) * S + s];
+            for (int e = pptr[k]; e < pptr[k + 1]; ++e) add += wqp[static_cast<int64_t>(e) * S + s];
             lv = static_cast<L>(static_cast<G>(lv) + add);
           }
           if (j > 0) {
@@ -1729,16 +1786,42 @@ inline int check_args(const jxhip_fused_args& a) {
   if (a.syn_model == JXHIP_SYN_MIXED && a.syn_kind == nullptr) return hipErrorInvalidValue;
   if (a.dt_v < 0 || a.rhs_scale < 0 || a.explicit_step < 0 || a.explicit_step > 1) return hipErrorInvalidValue;
   if (a.node_p2 != nullptr && (a.gate_ext == nullptr || a.gate_ext_stride <= 0)) return hipErrorInvalidValue;
+  if (a.parts < 0 || a.coop < 0 || a.coop > 1 || a.group_nodes < 0) return hipErrorInvalidValue;
+  // Lockstep programs: whole groups of parts.
+  if (a.coop && (a.parts < 2 || a.n_progs % a.parts)) return hipErrorInvalidValue;
   return hipSuccess;
 }
 
-template <typename K, typename... Args>
-int launch_kernel(K kern, const jxhip_fused_args& a, hipStream_t stream, Args... args) {
-  const dim3 grid(static_cast<unsigned>(a.n_progs)), block(a.threads);
+// One launch of the programs; with a.coop a cooperative launch (every workgroup resident,
+// so the grid-wide barriers complete), in consecutive launches of as many whole groups as
+// fit the device at once when the programs do not.
+template <typename K, typename F, typename... Rest>
+int launch_kernel(K kern, const jxhip_fused_args& a, hipStream_t stream, F f, Rest... rest) {
+  const dim3 block(a.threads);
   const size_t lds = static_cast<size_t>(a.lds_bytes);
   if (hipError_t e = set_lds(kern, lds)) return e;
-  hipLaunchKernelGGL(kern, grid, block, lds, stream, args...);
-  return hipGetLastError();
+  if (!a.coop) {
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(a.n_progs)), block, lds, stream, f, rest...);
+    return hipGetLastError();
+  }
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, a.threads, lds)) return e;
+  if (hipError_t e = hipGetDevice(&dev)) return e;
+  if (hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) return e;
+  int64_t cap = static_cast<int64_t>(per_cu) * cus;
+  cap -= cap % a.parts;
+  if (cap <= 0) return hipErrorCooperativeLaunchTooLarge;
+  for (int64_t p0 = 0; p0 < a.n_progs; p0 += cap) {
+    const int64_t cnt = a.n_progs - p0 < cap ? a.n_progs - p0 : cap;
+    F fc = f;
+    fc.progs = f.progs + p0 * JXHIP_PROG_COLS;
+    void* kargs[] = {&fc, &rest...};
+    if (hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern),
+                                                  dim3(static_cast<unsigned>(cnt)), block, kargs,
+                                                  static_cast<unsigned>(lds), stream))
+      return e;
+  }
+  return hipSuccess;
 }
 
 // Register-resident item counts with a kernel, by state element size: float gates
@@ -1747,10 +1830,11 @@ template <typename G, typename L, bool kSyn, bool kCkpt>
 int launch_forward_feat(const jxhip_fused_args& a, const Fused<G, L>& f, hipStream_t st) {
   if (a.items_per_thread == 0) {
     if (a.work_ax == nullptr) return hipErrorInvalidValue;
+    if (kSyn && a.coop && (a.vx == nullptr || a.vx_stride <= 0)) return hipErrorInvalidValue;
     if (a.coef_global) return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, true>, a, st, f);
     return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, false>, a, st, f);
   }
-  if (a.coef_global) return hipErrorInvalidValue;
+  if (a.coef_global || a.coop) return hipErrorInvalidValue;  // lockstep programs: streaming only
   if (a.max_items > static_cast<int64_t>(a.items_per_thread) * a.threads) return hipErrorInvalidValue;
   constexpr bool kF = sizeof(G) == 4;
 #define JXHIP_REGS(IPT, MAXT)                                                            \
@@ -1851,6 +1935,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   g.g_gs = static_cast<G*>(adj->g_syn_gs);
   g.wl = static_cast<G*>(adj->work_lin);
   g.work_lin_stride = adj->work_lin_stride;
+  g.wq_stride = adj->work_q_stride;
 #define JXHIP_ADJ(IPT)                                                                              \
   case IPT:                                                                                         \
     if (syn)                                                                                        \

@@ -49,6 +49,7 @@ class FusedArgs(C.Structure):
         ("max_items", _i64), ("coef_global", _i32), ("work_ax", _p),
         ("dt_v", C.c_double), ("rhs_scale", C.c_double), ("explicit_step", _i32),
         ("syn_kind", _p), ("node_p2", _p), ("gate_ext", _p), ("gate_ext_stride", C.c_int64),
+        ("parts", _i32), ("coop", _i32), ("group_nodes", _i64), ("vx", _p), ("vx_stride", _i64),
     ]
 
 
@@ -62,7 +63,7 @@ class AdjointArgs(C.Structure):
         ("lam_v", _p), ("rec_first", _p), ("rec_next", _p),
         ("pre_ptr", _p), ("pre_pos", _p), ("work_s", _p), ("work_syn_stride", _i64),
         ("lam_s", _p), ("work_q", _p), ("g_syn_gs", _p), ("work_lin", _p),
-        ("work_lin_stride", _i64),
+        ("work_lin_stride", _i64), ("work_q_stride", _i64),
     ]
 
 

@@ -43,11 +43,10 @@ class _FusedIntegrate(torch.autograd.Function):
         st, K = run.st, run.ckpt_every
         P, S, n, T = run.P, run.S, st.n, run.T
         dev, dtype = run.v.device, run.dtype
-        # (B, n_rec, T+1) -> program layout (P, T+1, n_rec, S)
-        gr = g_rec.to(dtype)
-        if P * S != run.B:
-            gr = torch.cat([gr, gr.new_zeros(P * S - run.B, *gr.shape[1:])], 0)
-        gr = gr.reshape(P, S, st.n_rec, T + 1).permute(0, 3, 2, 1).contiguous()
+        # (B, n_rec, T+1) -> the layout of the recordings (per group, part blocks)
+        gr = run.recordings_to_prog(g_rec.to(dtype))
+        # Recomputed trajectories per group of simulations (a split network's parts side
+        # by side).
         work = [torch.empty((P, K + 1, n, S), dtype=torch.float64, device=dev)]
         work += [torch.empty((P, K + 1, n, S), dtype=dtype, device=dev) for _ in range(3)]
         lam = [torch.zeros((P, n, S), dtype=dtype, device=dev) for _ in range(3)]
@@ -76,21 +75,26 @@ class _FusedIntegrate(torch.autograd.Function):
         if st.syn is not None:
             # Synapses: recomputed states per step, state adjoints, presynaptic shares and
             # the gradient of gS per (synapse, simulation).
-            n_syn, n_tgt = st.syn["n_syn"], st.syn["n_tgt"]
+            n_syn, n_tgt = st.syn["n_syn"], st.tgt_max
+            Pp = P * st.parts                                # programs
+            syn_max = st.syn_max
             kinetic = st.syn["model"] == 0
             syn_bufs = {
-                "work_s": torch.empty((P, K + 1, n_syn, S) if kinetic else (1,), dtype=dtype, device=dev),
+                "work_s": torch.empty((Pp, K + 1, syn_max, S) if kinetic else (1,), dtype=dtype, device=dev),
                 "lam_s": torch.empty((P, n_syn, S) if kinetic else (1,), dtype=dtype, device=dev),
-                "work_q": torch.empty((P, n_syn, S), dtype=dtype, device=dev),
-                "work_lin": torch.empty((P, K, n_tgt, S), dtype=dtype, device=dev),
+                # Presynaptic shares by step parity (the parts of a split network write
+                # each other's slots).
+                "work_q": torch.empty((2 if run.coop else 1, P, n_syn, S), dtype=dtype, device=dev),
+                "work_lin": torch.empty((Pp, K, n_tgt, S), dtype=dtype, device=dev),
             }
             holder["syn_bufs"] = syn_bufs
             adj.pre_ptr = st.syn["pre_ptr"].data_ptr()
             adj.pre_pos = st.syn["pre_pos"].data_ptr()
             adj.work_s = syn_bufs["work_s"].data_ptr()
-            adj.work_syn_stride = (K + 1) * n_syn * S
+            adj.work_syn_stride = (K + 1) * syn_max * S
             adj.lam_s = syn_bufs["lam_s"].data_ptr()
             adj.work_q = syn_bufs["work_q"].data_ptr()
+            adj.work_q_stride = P * n_syn * S if run.coop else 0
             adj.work_lin = syn_bufs["work_lin"].data_ptr()
             adj.work_lin_stride = K * n_tgt * S
             if ctx.needs_input_grad[4]:
@@ -98,9 +102,10 @@ class _FusedIntegrate(torch.autograd.Function):
                 adj.g_syn_gs = g_gs.data_ptr()
         args = run.args
         # The reverse pass's own launch shape at the forward's S.
-        acfg = adjoint_config(n, run.B, run.node_f.element_size(), st.sched, S=S, n_tgt=n_tgt)
+        nmax = st.n_max
+        acfg = adjoint_config(nmax, run.B, run.node_f.element_size(), st.sched, S=S, n_tgt=n_tgt)
         saved = (args.lds_bytes, args.threads, args.items_per_thread, args.coef_global)
-        args.lds_bytes = adjoint_lds_bytes(n, S, run.node_f.element_size(), st.sched, acfg.gcoef, n_tgt)
+        args.lds_bytes = adjoint_lds_bytes(nmax, S, run.node_f.element_size(), st.sched, acfg.gcoef, n_tgt)
         args.threads, args.items_per_thread, args.coef_global = acfg.threads, acfg.ipt, int(acfg.gcoef)
         f = run.kernel_name("jxhip_fused_adjoint")
         try:

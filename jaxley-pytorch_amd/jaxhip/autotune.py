@@ -71,7 +71,7 @@ def choose_tree_kernel(parent, device, forest, levels):
 
 def _key(st, B, dtype, lsize, grad):
     n_tgt = st.syn["n_tgt"] if st.syn else 0
-    return ("fused", st.n, st.L, n_tgt, int(st.topo_row.sum()), B, str(dtype), lsize, bool(grad))
+    return ("fused", st.n, st.L, n_tgt, st.parts, int(st.topo_row.sum()), B, str(dtype), lsize, bool(grad))
 
 
 def parse_config(text: str):
@@ -91,7 +91,9 @@ def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False, measure=
     fastest candidate is chosen, else a heuristic one."""
     from .fused import config_candidates, default_config, grad_config
 
-    n_tgt = st.syn["n_tgt"] if st.syn else 0
+    # The largest part of a split network sizes the LDS and the threads of a program.
+    n, n_tgt, sched = st.n_max, st.tgt_max, st.sched
+    stream_only = st.parts > 1 and st.syn is not None
     esize = torch.tensor([], dtype=dtype).element_size()
     key = _key(st, B, dtype, lsize, grad)
     if key in _CACHE:
@@ -103,21 +105,21 @@ def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False, measure=
         # program layout); among the forward shapes with that S the fastest is measured.
         from .fused import adjoint_config
 
-        S = adjoint_config(st.n, B, lsz, st.sched, n_tgt=n_tgt).S
-        cands = [c for c in config_candidates(st.n, n_tgt, B, esize, lsz, st.sched) if c.S == S]
+        S = adjoint_config(n, B, lsz, sched, n_tgt=n_tgt).S
+        cands = [c for c in config_candidates(n, n_tgt, B, esize, lsz, sched, stream_only) if c.S == S]
         if measure is not None and _enabled() and len(cands) > 1:
             times = [(measure(c), i) for i, c in enumerate(cands)]
             cfg = cands[min(times)[1]]
         else:
-            cfg = grad_config(st.n, B, lsz, st.sched, n_tgt)
+            cfg = grad_config(n, B, lsz, sched, n_tgt)
     elif env:
         cfg = parse_config(env)
     elif measure is not None and _enabled():
-        cands = config_candidates(st.n, n_tgt, B, esize, lsz, st.sched)
+        cands = config_candidates(n, n_tgt, B, esize, lsz, sched, stream_only)
         times = [(measure(c), i) for i, c in enumerate(cands)]
         cfg = cands[min(times)[1]]
     else:
-        cfg = default_config(st.n, n_tgt, B, lsz, esize, st.sched)
+        cfg = default_config(n, n_tgt, B, lsz, esize, sched, stream_only)
     _CACHE[key] = cfg
     return cfg
 

@@ -57,14 +57,18 @@ def from_prog(x: torch.Tensor, B: int) -> torch.Tensor:
 
 @dataclass
 class Structure:
-    """Integer tables of one topology in kernel order (cached per module structure)."""
+    """Integer tables of one topology in kernel order (cached per module structure). A
+    network split into parts has one topology per part (consecutive cells): every part's
+    nodes are a contiguous range of the kernel order, with its own solve schedule,
+    recordings and synapses (those onto its nodes); the per-part values are in the
+    `part_*` arrays and in the rows of `topo_row`."""
 
     n: int
     L: int
     mp: np.ndarray             # kernel index -> node
     kidx_of_node: np.ndarray   # node -> kernel index
     comp_of_k: np.ndarray      # kernel index -> compartment (-1 branchpoint)
-    topo_row: np.ndarray
+    topo_row: np.ndarray       # (parts, JXHIP_TOPO_COLS)
     seg: torch.Tensor
     seg_level_ptr: torch.Tensor
     seg_child: torch.Tensor
@@ -82,12 +86,61 @@ class Structure:
     lo_edge: np.ndarray        # edge of the lower coefficient of every kernel node
     up_edge: np.ndarray
     parent_k: np.ndarray
-    sched: int = 0             # bytes of the solve schedule staged in LDS
+    sched: int = 0             # bytes of the solve schedule staged in LDS (largest part)
     rec_first: Optional[torch.Tensor] = None   # per kernel node: its first recording (reverse pass)
     rec_next: Optional[torch.Tensor] = None    # per recording: the node's next recording
+    parts: int = 1
+    part_nb: Optional[np.ndarray] = None       # first node of every part
+    part_n: Optional[np.ndarray] = None        # nodes of every part
+    part_rb: Optional[np.ndarray] = None       # first recording of every part (part-major)
+    part_nrec: Optional[np.ndarray] = None
+    rec_order: Optional[np.ndarray] = None     # part-major recording -> recording
+
+    @property
+    def n_max(self) -> int:
+        """Nodes of the largest part (the LDS and the threads of a program)."""
+        return int(self.part_n.max()) if self.part_n is not None else self.n
+
+    @property
+    def tgt_max(self) -> int:
+        return int(self.topo_row[:, 7].max()) if self.syn else 0
+
+    @property
+    def syn_max(self) -> int:
+        return int(self.topo_row[:, 11].max()) if self.syn else 0
 
 
-def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, device) -> Structure:
+def cell_sizes(module) -> np.ndarray:
+    """Solve nodes of every tree of the module's forest (its cells), in solve order of
+    their roots."""
+    tables, indexer = module._ensure_solver()
+    n = tables["n_nodes"]
+    parent = np.asarray(indexer["parent_lookup"][:n], dtype=np.int64)
+    root = _roots(parent)
+    _, counts = np.unique(root, return_counts=True)
+    return counts
+
+
+def _roots(parent: np.ndarray) -> np.ndarray:
+    """Root of every node of a forest whose parents precede their children."""
+    root = np.arange(len(parent))
+    has = parent >= 0
+    root[has] = parent[has]
+    while True:
+        nxt = root[root]
+        if np.array_equal(nxt, root):
+            return root
+        root = nxt
+
+
+def _split(sizes: np.ndarray, parts: int) -> np.ndarray:
+    """Part of every tree: consecutive trees, the parts' node counts balanced."""
+    cum = np.cumsum(sizes) - sizes                    # nodes before each tree
+    total = float(sizes.sum())
+    return np.minimum((cum * parts / total).astype(np.int64), parts - 1)
+
+
+def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, device, parts: int = 1) -> Structure:
     tables, indexer = module._ensure_solver()
     n = tables["n_nodes"]
     mp_solve = np.asarray(indexer["map_to_solve_order"], dtype=np.int64)
@@ -95,11 +148,30 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
     parent_solve = np.asarray(indexer["parent_lookup"][:n], dtype=np.int64)
     if check_forest(parent_solve) is None:
         raise ValueError("the solve order must be a forest with parents before children")
-    sg = segments(parent_solve)
-    korder = sg["korder"]                           # kernel -> solve
-    kidx_solve = sg["kidx"]                         # solve -> kernel
-    mp = mp_solve[korder]                           # kernel -> node
-    kidx_of_node = kidx_solve[inv_solve]            # node -> kernel
+    # Parts: consecutive trees (cells); each part's kernel order from its own schedule.
+    root = _roots(parent_solve)
+    roots, tree_of = np.unique(root, return_inverse=True)
+    parts = max(1, min(int(parts), len(roots)))
+    part_of_tree = _split(np.bincount(tree_of), parts) if parts > 1 else np.zeros(len(roots), np.int64)
+    part_of_solve = part_of_tree[tree_of]
+    korder_l, sgs = [], []
+    for q in range(parts):
+        nodes = np.nonzero(part_of_solve == q)[0]
+        loc = np.full(n, -1, dtype=np.int64)
+        loc[nodes] = np.arange(len(nodes))
+        par = parent_solve[nodes]
+        sg = segments(np.where(par >= 0, loc[np.maximum(par, 0)], -1))
+        korder_l.append(nodes[sg["korder"]])
+        sgs.append(sg)
+    korder = np.concatenate(korder_l)                # kernel -> solve
+    kidx_solve = np.empty(n, dtype=np.int64)         # solve -> kernel
+    kidx_solve[korder] = np.arange(n)
+    part_n = np.array([len(x) for x in korder_l], dtype=np.int64)
+    part_nb = np.concatenate([[0], np.cumsum(part_n)[:-1]]).astype(np.int64)
+    part_of_k = np.repeat(np.arange(parts), part_n)
+    nb_k = part_nb[part_of_k]                        # first node of every node's part
+    mp = mp_solve[korder]                            # kernel -> node
+    kidx_of_node = kidx_solve[inv_solve]             # node -> kernel
     parent_k = np.where(parent_solve[korder] >= 0, kidx_solve[np.maximum(parent_solve[korder], 0)], -1)
     comp_of_node = -np.ones(n, dtype=np.int64)
     comp_of_node[module._comp_node] = np.arange(module._n_comp)
@@ -111,13 +183,15 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         ok = comp_of_k >= 0
         has[ok] = mask[comp_of_k[ok]]
         flags |= np.where(has, CHANNEL_FLAGS[ch.kernel_id], 0).astype(np.int32)
-    # Children of every kernel node (CSR, in increasing kernel index).
+    # Children of every kernel node (CSR, in increasing kernel index; a part's nodes and
+    # their children are contiguous, the indices part-relative).
     ch_nodes = np.nonzero(parent_k >= 0)[0]
     srt = np.lexsort((ch_nodes, parent_k[ch_nodes]))
     ch_nodes = ch_nodes[srt]
     cnt = np.bincount(parent_k[ch_nodes], minlength=n)
     cbeg = np.zeros(n, dtype=np.int64)
     np.cumsum(cnt[:-1], out=cbeg[1:])
+    part_cb = cbeg[part_nb] if n else np.zeros(parts, np.int64)
     # Stimulus entries grouped per kernel node; entry e reads stimulus column stim_order[e].
     stim_k = kidx_of_node[stim_nodes] if len(stim_nodes) else np.zeros(0, np.int64)
     stim_order = np.argsort(stim_k, kind="stable")
@@ -125,36 +199,68 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
     sbeg = np.zeros(n, dtype=np.int64)
     np.cumsum(scnt[:-1], out=sbeg[1:])
     node_i = np.zeros((n, _lib.JXHIP_NODE_ICOLS), dtype=np.int32)
-    node_i[:, 0] = parent_k
-    node_i[:, 1] = cbeg
-    node_i[:, 2] = cbeg + cnt
+    node_i[:, 0] = np.where(parent_k >= 0, parent_k - nb_k, -1)
+    node_i[:, 1] = cbeg - part_cb[part_of_k]
+    node_i[:, 2] = cbeg + cnt - part_cb[part_of_k]
     node_i[:, 3] = flags
     node_i[:, 4] = sbeg
     node_i[:, 5] = sbeg + scnt
     node_i[:, 6] = -1
-    syn = _build_synapses(module, kidx_of_node, n, node_i)
+    syn = _build_synapses(module, kidx_of_node, n, node_i, part_of_k, nb_k, parts)
     rec_k = kidx_of_node[module._comp_node[rec_comps]] if len(rec_comps) else np.zeros(0, np.int64)
     # Nodes whose voltage other threads read from HBM during the time loop.
     node_i[rec_k, 3] |= FLAG_EXPORT
     if syn is not None:
         node_i[syn["pre"], 3] |= FLAG_EXPORT
-        node_i[syn["tgt"], 3] |= FLAG_EXPORT
-    topo_row = np.zeros(_lib.JXHIP_TOPO_COLS, dtype=np.int32)
-    topo_row[:16] = [n, sg["L"], 0, 0, 0, len(rec_k), 0, syn["n_tgt"] if syn else 0, 0, 0, 0,
-                     syn["n_syn"] if syn else 0, 0, 0, 0, 0]
-    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=device)
-    # The recordings of every node as a list (first, next; the reverse pass adds their
-    # gradients).
+        node_i[syn["tgt"] + syn["tgt_nb"], 3] |= FLAG_EXPORT
+    # Recordings grouped by part (part-major order; rec_order maps back).
+    rec_part = part_of_k[rec_k] if len(rec_k) else np.zeros(0, np.int64)
+    rec_order = np.argsort(rec_part, kind="stable")
+    part_nrec = np.bincount(rec_part, minlength=parts).astype(np.int64)
+    part_rb = np.concatenate([[0], np.cumsum(part_nrec)[:-1]]).astype(np.int64)
+    rec_local = rec_k[rec_order] - nb_k[rec_k[rec_order]] if len(rec_k) else rec_k
+    # The recordings of every node as a list (first, next; part-relative indices; the
+    # reverse pass adds their gradients).
     none = np.iinfo(np.int32).max
     rec_first = np.full(n, none, dtype=np.int64)
     rec_next = np.full(max(len(rec_k), 1), none, dtype=np.int64)
     for r in range(len(rec_k) - 1, -1, -1):
-        rec_next[r] = rec_first[rec_k[r]]
-        rec_first[rec_k[r]] = r
+        kk = rec_k[rec_order[r]]
+        rb = part_rb[part_of_k[kk]]
+        rec_next[r] = rec_first[kk]
+        rec_first[kk] = r - rb
+    # Segment tables of the parts, concatenated (part-relative node indices).
+    seg = np.concatenate([sg["seg"] for sg in sgs]) if n else np.zeros((0, 4), np.int32)
+    lptr = np.concatenate([sg["seg_level_ptr"] for sg in sgs])
+    schild = np.concatenate([sg["seg_child"] for sg in sgs])
+    nseg = np.array([len(sg["seg"]) for sg in sgs], dtype=np.int64)
+    nch = np.array([len(sg["seg_child"]) for sg in sgs], dtype=np.int64)
+    nlp = np.array([len(sg["seg_level_ptr"]) for sg in sgs], dtype=np.int64)
+    base = lambda c: np.concatenate([[0], np.cumsum(c)[:-1]]).astype(np.int64)
+    Ls = np.array([sg["L"] for sg in sgs], dtype=np.int64)
+    sched = [sched_bytes(len(sg["seg"]), sg["L"], int(sg["seg"][-1, 3]) if len(sg["seg"]) else 0) for sg in sgs]
+    topo_row = np.zeros((parts, _lib.JXHIP_TOPO_COLS), dtype=np.int32)
+    topo_row[:, 0] = part_n
+    topo_row[:, 1] = Ls
+    topo_row[:, 2] = part_nb
+    topo_row[:, 3] = base(nlp)
+    topo_row[:, 4] = part_cb
+    topo_row[:, 5] = part_nrec
+    topo_row[:, 6] = part_rb
     if syn is not None:
-        # The synapses grouped by presynaptic node: every synapse's slot in the grouping
-        # (the reverse pass stores its presynaptic share there and sums each node's
-        # contiguous slots, without atomics).
+        topo_row[:, 7] = syn["part_ntgt"]
+        topo_row[:, 8] = syn["part_tgb"]
+        topo_row[:, 9] = syn["part_spb"]
+        topo_row[:, 10] = syn["part_synb"]
+        topo_row[:, 11] = syn["part_nsyn"]
+    topo_row[:, 12] = base(nseg)
+    topo_row[:, 13] = base(nch)
+    topo_row[:, 14] = part_nb                        # pre_ptr is indexed by the group's nodes
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=device)
+    if syn is not None:
+        # The synapses grouped by presynaptic node (over the group's nodes): every
+        # synapse's slot in the grouping (the reverse pass stores its presynaptic share
+        # there and sums each node's contiguous slots, without atomics).
         pre_np = np.asarray(syn["pre"], dtype=np.int64)
         order = np.argsort(pre_np, kind="stable")
         pos = np.empty(len(order), dtype=np.int64)
@@ -166,31 +272,36 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
         if syn["kind"] is not None:
             syn["kind"] = t(syn["kind"])
     return Structure(
-        n=n, L=sg["L"], mp=mp, kidx_of_node=kidx_of_node, comp_of_k=comp_of_k, topo_row=topo_row,
-        seg=t(sg["seg"]), seg_level_ptr=t(sg["seg_level_ptr"]), seg_child=t(sg["seg_child"]),
-        node_i=t(node_i), child_idx=t(ch_nodes if len(ch_nodes) else np.zeros(1, np.int64)),
+        n=n, L=int(Ls.max()), mp=mp, kidx_of_node=kidx_of_node, comp_of_k=comp_of_k, topo_row=topo_row,
+        seg=t(seg if len(seg) else np.zeros((1, 4), np.int32)), seg_level_ptr=t(lptr), seg_child=t(schild),
+        node_i=t(node_i), child_idx=t(ch_nodes - nb_k[ch_nodes] if len(ch_nodes) else np.zeros(1, np.int64)),
         stim_col=t(stim_order if len(stim_order) else np.zeros(1, np.int64)),
-        rec_node=t(rec_k if len(rec_k) else np.zeros(1, np.int64)),
+        rec_node=t(rec_local if len(rec_k) else np.zeros(1, np.int64)),
         syn=syn, n_rec=len(rec_k), rec_comps=rec_comps, stim_order=stim_order,
         sinks_k=kidx_of_node[tables["sink"]],
         in_edges=_in_edges(kidx_of_node[tables["sink"]], n),
         lo_edge=np.asarray(indexer["map_to_solve_order_lower"])[korder],
         up_edge=np.asarray(indexer["map_to_solve_order_upper"])[korder],
         parent_k=parent_k,
-        sched=sched_bytes(len(sg["seg"]), sg["L"], int(sg["seg"][-1, 3]) if len(sg["seg"]) else 0),
+        sched=int(max(sched)),
         rec_first=t(rec_first), rec_next=t(rec_next),
+        parts=parts, part_nb=part_nb, part_n=part_n, part_rb=part_rb, part_nrec=part_nrec,
+        rec_order=rec_order,
     )
 
 
-def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray) -> Optional[Dict]:
-    """All synapses of the network as one list in jagged-diagonal order: the targets
-    (postsynaptic nodes) numbered by non-increasing in-degree (ties by node), and the
-    list holding every target's first synapse, then every target's second one, and so
-    on, so that the j-th synapse of target t is `ptr[j] + t` (for t below
-    `ptr[j + 1] - ptr[j]`; `ptr` ends with a repeated entry). Threads of consecutive
-    targets then read consecutive synapses. A target's synapses keep the order of the
-    types in the network and of the edges within a type (the order the currents are
-    summed in). Several types give the per-synapse model ids of JXHIP_SYN_MIXED."""
+def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray,
+                    part_of_k: np.ndarray, nb_k: np.ndarray, parts: int) -> Optional[Dict]:
+    """All synapses of the network, grouped by the part of their target, each part's in
+    jagged-diagonal order: the part's targets (postsynaptic nodes) numbered by
+    non-increasing in-degree (ties by node), and the list holding every target's first
+    synapse, then every target's second one, and so on, so that the j-th synapse of
+    target t is `ptr[j] + t` (for t below `ptr[j + 1] - ptr[j]`; a part's `ptr` ends with
+    a repeated entry). Threads of consecutive targets then read consecutive synapses. A
+    target's synapses keep the order of the types in the network and of the edges within
+    a type (the order the currents are summed in). Several types give the per-synapse
+    model ids of JXHIP_SYN_MIXED. Presynaptic nodes index the group's nodes, targets are
+    part-relative."""
     if not module._edges:
         return None
     groups, pre_l, post_l, kind_l = [], [], [], []
@@ -206,17 +317,37 @@ def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray
     post = kidx_of_node[post_nodes]
     pre = kidx_of_node[np.concatenate(pre_l)]
     kind = np.concatenate(kind_l)
-    order = np.argsort(post, kind="stable")         # synapses grouped by target
-    targets, counts = np.unique(post[order], return_counts=True)
-    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
-    rank = np.argsort(-counts, kind="stable")        # targets by non-increasing in-degree
-    targets, counts, first = targets[rank], counts[rank], first[rank]
-    per_j = np.array([int((counts > j).sum()) for j in range(int(counts.max()))], dtype=np.int64)
-    ptr = np.concatenate([[0], np.cumsum(per_j), [len(post)]]).astype(np.int32)
-    perm = np.empty(len(post), dtype=np.int64)
-    for j, cnt in enumerate(per_j):
-        perm[ptr[j]:ptr[j] + cnt] = order[first[:cnt] + j]
-    node_i[targets, 6] = np.arange(len(targets), dtype=np.int32)
+    perm_l, ptr_l, tgt_l, tgt_g = [], [], [], []
+    part_ntgt = np.zeros(parts, np.int64)
+    part_nsyn = np.zeros(parts, np.int64)
+    part_nptr = np.zeros(parts, np.int64)
+    for q in range(parts):
+        mine = np.nonzero(part_of_k[post] == q)[0]
+        part_nsyn[q] = len(mine)
+        if not len(mine):
+            ptr_l.append(np.zeros(2, np.int64))
+            part_nptr[q] = 2
+            continue
+        order = mine[np.argsort(post[mine], kind="stable")]   # grouped by target
+        targets, counts = np.unique(post[order], return_counts=True)
+        first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        rank = np.argsort(-counts, kind="stable")              # non-increasing in-degree
+        targets, counts, first = targets[rank], counts[rank], first[rank]
+        per_j = np.array([int((counts > j).sum()) for j in range(int(counts.max()))], dtype=np.int64)
+        ptr = np.concatenate([[0], np.cumsum(per_j), [len(mine)]]).astype(np.int64)
+        perm = np.empty(len(mine), dtype=np.int64)
+        for j, c in enumerate(per_j):
+            perm[ptr[j]:ptr[j] + c] = order[first[:c] + j]
+        node_i[targets, 6] = np.arange(len(targets), dtype=np.int32)
+        perm_l.append(perm)
+        ptr_l.append(ptr)
+        tgt_l.append(targets - nb_k[targets])
+        tgt_g.append(targets)
+        part_ntgt[q] = len(targets)
+        part_nptr[q] = len(ptr)
+    perm = np.concatenate(perm_l)
+    tgt_local = np.concatenate(tgt_l)
+    part_tgb = np.concatenate([[0], np.cumsum(part_ntgt)[:-1]]).astype(np.int64)
     mixed = len(groups) > 1
     return {
         "model": _lib.JXHIP_SYN_MIXED if mixed else int(groups[0]["type"].kernel_model),
@@ -224,11 +355,16 @@ def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray
         "kind": kind[perm] if mixed else None,
         "perm": perm,
         "pre": pre[perm].astype(np.int32),
-        "ptr": ptr,
-        "tgt": targets.astype(np.int32),
+        "ptr": np.concatenate(ptr_l).astype(np.int32),
+        "tgt": tgt_local.astype(np.int32),
+        "tgt_nb": nb_k[np.concatenate(tgt_g)] if tgt_g else np.zeros(0, np.int64),
         "post_node": post_nodes[perm],
-        "n_tgt": len(targets),
+        "n_tgt": int(part_ntgt.sum()),
         "n_syn": len(post),
+        "part_ntgt": part_ntgt, "part_tgb": part_tgb,
+        "part_spb": np.concatenate([[0], np.cumsum(part_nptr)[:-1]]).astype(np.int64),
+        "part_synb": np.concatenate([[0], np.cumsum(part_nsyn)[:-1]]).astype(np.int64),
+        "part_nsyn": part_nsyn,
     }
 
 
@@ -321,11 +457,13 @@ def items_per_thread(n: int, S: int, threads: int, gsize: int, lsize: int) -> in
 
 
 def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
-                      sched: int = 0) -> List[LaunchConfig]:
+                      sched: int = 0, stream_only: bool = False) -> List[LaunchConfig]:
     """Launch shapes worth measuring: S simulations per program (a power of two, LDS
     permitting, at least 256 programs when the batch allows) with every thread count
     whose registers hold the items of a program, and the streaming kernel at each S
-    (with the coupling coefficients in HBM / L2 when only that fits the LDS)."""
+    (with the coupling coefficients in HBM / L2 when only that fits the LDS). n, n_tgt
+    and sched are those of the largest part; `stream_only` for the lockstep programs
+    of a split network."""
     out = []
     S = 1
     while S <= 64:
@@ -335,7 +473,7 @@ def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
         fits_g = lds_bytes(n, n_tgt, S, lsize, sched, gcoef=True) <= 160 * 1024
         if not fits_g:
             break
-        if fits:
+        if fits and not stream_only:
             for thr in (256, 512, 1024):
                 if items_per_thread(n, S, thr, gsize, lsize):
                     out.append(LaunchConfig(S, thr))
@@ -349,11 +487,11 @@ def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
 
 
 def default_config(n: int, n_tgt: int, B: int, esize: int, gsize: int = 0,
-                   sched: int = 0) -> LaunchConfig:
+                   sched: int = 0, stream_only: bool = False) -> LaunchConfig:
     """A launch shape without measurement: the largest S with a register-resident shape
     (fewest threads), else the largest streaming S."""
     gsize = gsize or esize
-    cands = config_candidates(n, n_tgt, B, gsize, esize, sched)
+    cands = config_candidates(n, n_tgt, B, gsize, esize, sched, stream_only)
     regs = [c for c in cands if not c.stream]
     if regs:
         return max(regs, key=lambda c: (c.S, -c.threads))
@@ -429,22 +567,31 @@ class ForwardLaunch:
         dtype = gates0[0].dtype
         S = cfg.S
         self.st, self.cfg, self.B, self.T, self.S, self.dtype = st, cfg, B, T, S, dtype
+        # P groups of S simulations; a group is one program per part of the structure.
         self.P = P = (B + S - 1) // S
         n = st.n
+        parts = st.parts
+        # Lockstep programs (grid-wide barrier per step): parts joined by synapses.
+        self.coop = coop = parts > 1 and st.syn is not None
         self.v = to_prog(v0.double(), S)
         self.gm, self.gh, self.gn = (to_prog(g, S) for g in gates0)
-        self.rec = torch.empty((P, T + 1, max(st.n_rec, 1), S), dtype=dtype, device=dev)
-        progs = np.zeros((P, _lib.JXHIP_PROG_COLS), dtype=np.int64)
-        p = np.arange(P)
-        progs[:, 1] = np.minimum(S, B - p * S)
+        nrec = max(st.n_rec, 1)
+        # Recordings per group: every part's block [T + 1][its recordings][S].
+        self.rec = torch.empty((P, (T + 1) * nrec * S), dtype=dtype, device=dev)
+        g_i = np.repeat(np.arange(P), parts)
+        q_i = np.tile(np.arange(parts), P)
+        progs = np.zeros((P * parts, _lib.JXHIP_PROG_COLS), dtype=np.int64)
+        progs[:, 0] = q_i
+        progs[:, 1] = np.minimum(S, B - g_i * S)
         progs[:, 3] = T
-        progs[:, 4] = p * n * S
-        progs[:, 5] = p * S
-        progs[:, 6] = p * (T + 1) * max(st.n_rec, 1) * S
+        progs[:, 4] = (g_i * n + st.part_nb[q_i]) * S
+        progs[:, 5] = g_i * S
+        progs[:, 6] = g_i * (T + 1) * nrec * S + (T + 1) * st.part_rb[q_i] * S
         n_syn = st.syn["n_syn"] if st.syn else 0
-        progs[:, 7] = p * n_syn * S
+        if st.syn:
+            progs[:, 7] = (g_i * n_syn + st.syn["part_synb"][q_i]) * S
         self.progs = torch.as_tensor(progs, device=dev)
-        self.topos = torch.as_tensor(st.topo_row[None], device=dev)
+        self.topos = torch.as_tensor(st.topo_row, device=dev)
         self.node_f = node_f
         self.node_p = node_p
         self.bparam = bparam
@@ -466,7 +613,16 @@ class ForwardLaunch:
             self.ckpt_s = torch.empty((C,) + self.syn_s.shape, dtype=dtype, device=dev) if self.syn_s is not None else None
         a = _lib.FusedArgs()
         a.progs = self.progs.data_ptr()
-        a.n_progs = P
+        a.n_progs = P * parts
+        a.parts = parts
+        a.group_nodes = n
+        a.coop = int(coop)
+        self.vx = None
+        if coop:
+            # Exported voltages of every part by step parity (presynaptic reads).
+            self.vx = torch.empty((2,) + self.v.shape, dtype=torch.float64, device=dev)
+            a.vx = self.vx.data_ptr()
+            a.vx_stride = self.v.numel()
         a.topos = self.topos.data_ptr()
         a.seg = st.seg.data_ptr()
         a.seg_level_ptr = st.seg_level_ptr.data_ptr()
@@ -525,15 +681,17 @@ class ForwardLaunch:
         a.diff = SYN_DIFF
         a.S = S
         a.threads = cfg.threads
-        a.items_per_thread = 0 if cfg.stream else items_per_thread(
-            n, S, cfg.threads, self.gm.element_size(), node_f.element_size())
-        n_tgt = st.syn["n_tgt"] if st.syn else 0
-        a.coef_global = int(cfg.stream and cfg.gcoef)
-        a.lds_bytes = lds_bytes(n, n_tgt, S, node_f.element_size(), st.sched, bool(a.coef_global))
+        nmax, n_tgt = st.n_max, st.tgt_max
+        stream = cfg.stream or coop
+        a.items_per_thread = 0 if stream else items_per_thread(
+            nmax, S, cfg.threads, self.gm.element_size(), node_f.element_size())
+        gcoef = cfg.gcoef or (stream and lds_bytes(nmax, n_tgt, S, node_f.element_size(), st.sched) > 160 * 1024)
+        a.coef_global = int(stream and gcoef)
+        a.lds_bytes = lds_bytes(nmax, n_tgt, S, node_f.element_size(), st.sched, bool(a.coef_global))
         # Scratch of the streaming kernel: the axial part of every item's residual.
         self.work_ax = torch.empty_like(self.gm) if a.items_per_thread == 0 else None
         a.work_ax = self.work_ax.data_ptr() if self.work_ax is not None else 0
-        a.max_items = n * S
+        a.max_items = nmax * S
         if ckpt_every > 0:
             a.ckpt_v, a.ckpt_m, a.ckpt_h, a.ckpt_n = (c.data_ptr() for c in self.ckpt)
             a.ckpt_s = self.ckpt_s.data_ptr() if self.ckpt_s is not None else 0
@@ -552,7 +710,31 @@ class ForwardLaunch:
         _lib.check(f(self.args, _lib.stream_ptr(self.v.device)), "fused_forward")
 
     def recordings(self) -> torch.Tensor:
-        """(B, n_rec, T + 1)."""
-        r = self.rec[..., : self.st.n_rec, :]                 # (P, T+1, n_rec, S)
-        r = r.permute(0, 3, 2, 1).reshape(self.P * self.S, self.st.n_rec, self.T + 1)
+        """(B, n_rec, T + 1), the recordings in their order."""
+        st, P, S, T1 = self.st, self.P, self.S, self.T + 1
+        blocks = []
+        for q in range(st.parts):
+            rb, nr = int(st.part_rb[q]), int(st.part_nrec[q])
+            b = self.rec[:, T1 * rb * S:T1 * (rb + nr) * S].reshape(P, T1, nr, S)
+            blocks.append(b.permute(0, 3, 2, 1).reshape(P * S, nr, T1))
+        r = torch.cat(blocks, 1) if len(blocks) > 1 else blocks[0]
+        if st.parts > 1:
+            r = r[:, torch.as_tensor(np.argsort(st.rec_order), device=r.device)]
         return r[: self.B]
+
+    def recordings_to_prog(self, g: torch.Tensor) -> torch.Tensor:
+        """(B, n_rec, T + 1) -> the layout of the recordings buffer (gradients)."""
+        st, P, S, T1 = self.st, self.P, self.S, self.T + 1
+        if P * S != self.B:
+            g = torch.cat([g, g.new_zeros(P * S - self.B, *g.shape[1:])], 0)
+        if st.parts > 1:
+            g = g[:, torch.as_tensor(st.rec_order, device=g.device)]
+        blocks = []
+        for q in range(st.parts):
+            rb, nr = int(st.part_rb[q]), int(st.part_nrec[q])
+            b = g[:, rb:rb + nr].reshape(P, S, nr, T1).permute(0, 3, 2, 1)
+            blocks.append(b.reshape(P, -1))
+        out = torch.cat(blocks, 1) if len(blocks) > 1 else blocks[0]
+        if out.shape[1] < self.rec.shape[1]:
+            out = torch.cat([out, out.new_zeros(P, self.rec.shape[1] - out.shape[1])], 1)
+        return out.contiguous()

@@ -305,10 +305,43 @@ class Problem:
     conductances, initial states, stimulus and synapses (see `_prepare`)."""
 
 
-def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> Problem:
+def _structure(module, rec_comps, stim_nodes, device, B: int, lsize: int, grad: bool):
+    """The module's kernel tables (cached), split into parts for a network: one part per
+    cell up to one program per compute unit when B simulations leave units idle (at one
+    simulation per program), more parts while the largest does not fit the LDS (of the
+    forward, and of the reverse pass with `grad`). B = 0: one part."""
+    from .fused import adjoint_lds_bytes, cell_sizes, lds_bytes
+
+    ncell = len(cell_sizes(module)) if B else 1
+    dev = torch.device(device)
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count if dev.type == "cuda" else 256
+    parts = max(1, min(ncell, cus // max(B, 1))) if ncell > 1 else 1
+    cache = module.__dict__.setdefault("_fused_cache", {})
+    base = (module._version, tuple(rec_comps.tolist()), tuple(np.asarray(stim_nodes).tolist()), str(device))
+    for k in list(cache):
+        if k[:4] != base:
+            del cache[k]
+    while True:
+        key = base + (parts,)
+        st = cache.get(key)
+        if st is None:
+            st = build_structure(module, rec_comps, stim_nodes, device, parts)
+            cache[key] = st
+        n, nt = st.n_max, st.tgt_max
+        fits = lds_bytes(n, nt, 1, lsize, st.sched, gcoef=True) <= 160 * 1024
+        if grad:
+            fits = fits and adjoint_lds_bytes(n, 1, lsize, st.sched, True, nt) <= 160 * 1024
+        if fits or parts >= ncell:
+            return st
+        parts = min(ncell, 2 * parts)
+
+
+def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler", split: bool = True) -> Problem:
     """Everything the fused kernel needs, independent of the launch shape. The tree's
     coefficients and the stimulus scale use the solve's step (dt / 2 for
-    Crank-Nicolson); gates and synapses advance by dt."""
+    Crank-Nicolson); gates and synapses advance by dt. With `split`, a network is split
+    into parts (consecutive cells, one program each per group of simulations) when its
+    batch leaves the device's compute units idle or a part of it is all the LDS holds."""
     cn = solver == "crank_nicolson"
     dtv = dt / 2 if cn else dt
     # Element type of gates, rates, parameters and recordings in the kernel.
@@ -323,20 +356,6 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
             raise NotImplementedError(f"batched {k!r} is not supported by the fused kernel")
         if vals[k].requires_grad:
             raise NotImplementedError(f"gradients with respect to {k!r} take the per-step path")
-    key = (module._version, tuple(rec_comps.tolist()), tuple(inp.stim_nodes.tolist()), str(device))
-    cache = module.__dict__.setdefault("_fused_cache", {})
-    st = cache.get(key)
-    if st is None:
-        cache.clear()
-        st = build_structure(module, rec_comps, inp.stim_nodes, device)
-        cache[key] = st
-    tables = module._comp_edges
-    geo = comp_geometry(vals)
-    g = axial_conductances(tables, module._comp_node, module._n_nodes, geo, vals["axial_resistivity"], vals["capacitance"])
-    cm_node = torch.full((module._n_nodes,), -1.0, dtype=dtype, device=device)
-    cm_node[torch.as_tensor(module._comp_node, device=device)] = vals["capacitance"]
-    node_f = node_float_table(st, g, cm_node, dtv, user_dtype)
-
     # Batch: parameters, states, stimulus.
     chan_keys = [k for k in ALL_PARAMS if k in vals]
     state_keys = ["v"] + [k for k in ("HH_m", "HH_h", "HH_n") + POSP_STATES if k in vals]
@@ -353,6 +372,16 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
     batch_t += [inp.syn_vals[k] for k in syn_keys]
     batch = _batch_of(*batch_t)
     B = int(np.prod(batch)) if batch else 1
+    needs_grad = _needs_grad(inp)
+    st = _structure(module, rec_comps, inp.stim_nodes, device, B if split else 0,
+                    torch.tensor([], dtype=user_dtype).element_size(), needs_grad)
+    tables = module._comp_edges
+    geo = comp_geometry(vals)
+    g = axial_conductances(tables, module._comp_node, module._n_nodes, geo, vals["axial_resistivity"], vals["capacitance"])
+    cm_node = torch.full((module._n_nodes,), -1.0, dtype=dtype, device=device)
+    cm_node[torch.as_tensor(module._comp_node, device=device)] = vals["capacitance"]
+    node_f = node_float_table(st, g, cm_node, dtv, user_dtype)
+
     n = st.n
     comp_of_k = torch.as_tensor(st.comp_of_k, device=device)
     is_comp = comp_of_k >= 0
@@ -367,7 +396,6 @@ def _prepare(module, inp, dt, device, user_dtype, solver: str = "bwd_euler") -> 
         y = x[..., cs_safe]
         return torch.where(is_comp, y, torch.zeros_like(y))
 
-    needs_grad = _needs_grad(inp)
     # Channel parameters: a per-node table (kernel order) and, when some conductance
     # carries batch dimensions, a batched table of the four conductances.
     def kernel_cols(x):
@@ -628,7 +656,9 @@ class _StepRunner:
         if inp.stim is not None:
             inp.stim = inp.stim[..., :1]
         with torch.no_grad():
-            P = _prepare(module, inp, delta_t, device, dtype, solver)
+            # One program per group (a step may be captured in a graph: no cooperative
+            # launch).
+            P = _prepare(module, inp, delta_t, device, dtype, solver, split=False)
         from .autotune import fused_config
 
         cfg = fused_config(P.st, P.B, P.dtype, torch.tensor([], dtype=dtype).element_size(), False)

@@ -132,7 +132,7 @@ def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_state
     cpu = torch.device("cpu")
     inp = _assemble(module, params, param_state, data_stimuli, t_max, dt, all_states, cpu, user_dtype)
     with torch.no_grad():
-        P = _prepare(module, inp, dt, cpu, user_dtype)
+        P = _prepare(module, inp, dt, cpu, user_dtype, split=False)
     if P.B != 1:
         raise ValueError("a served request is one simulation (batch size 1)")
     if P.st.syn is not None:

@@ -298,25 +298,40 @@ def _irregular_network(synapse: str):
     return net, om, pre_comp, post_comp
 
 
-def test_synapse_jagged_diagonal_order():
-    """The fused kernels' synapse list: targets by non-increasing in-degree, the j-th
-    synapse of target t at ptr[j] + t, and each target's synapses in edge order."""
+@pytest.mark.parametrize("parts", [1, 3])
+def test_synapse_jagged_diagonal_order(parts):
+    """The fused kernels' synapse lists: per part of the network (consecutive cells), the
+    synapses onto the part's nodes with the targets by non-increasing in-degree, the j-th
+    synapse of target t at ptr[j] + t, and each target's synapses in edge order;
+    presynaptic nodes index the whole network, targets the part's nodes."""
     from jaxhip.fused import build_structure
 
     net, _, pre_comp, post_comp = _irregular_network("KineticSynapse")
-    st = build_structure(net, np.array([0]), np.zeros(0, np.int64), torch.device("cpu"))
+    st = build_structure(net, np.array([0]), np.zeros(0, np.int64), torch.device("cpu"), parts)
+    assert st.parts == parts and st.part_n.sum() == st.n
     syn = st.syn
-    ptr, perm, tgt = syn["ptr"].numpy(), np.asarray(syn["perm"]), syn["tgt"].numpy()
+    ptr_all, perm, tgt_all = syn["ptr"].numpy(), np.asarray(syn["perm"]), syn["tgt"].numpy()
+    pre = syn["pre"].numpy()
     post_k = st.kidx_of_node[net._comp_node[post_comp]]
-    deg = []
-    for t in range(syn["n_tgt"]):
-        j, mine = 0, []
-        while t < ptr[j + 1] - ptr[j]:
-            mine.append(perm[ptr[j] + t])
-            j += 1
-        deg.append(j)
-        assert mine == [e for e in range(len(post_k)) if post_k[e] == tgt[t]]
-    assert deg == sorted(deg, reverse=True) and sum(deg) == len(post_k) and len(set(deg)) > 1
+    pre_k = st.kidx_of_node[net._comp_node[pre_comp]]
+    seen = []
+    for q in range(parts):
+        n_p, nb, n_tgt, tgb, spb, synb, n_syn = (int(x) for x in st.topo_row[q, [0, 2, 7, 8, 9, 10, 11]])
+        ptr = ptr_all[spb:]
+        deg = []
+        for t in range(n_tgt):
+            j, mine = 0, []
+            while t < ptr[j + 1] - ptr[j]:
+                mine.append(synb + ptr[j] + t)
+                j += 1
+            deg.append(j)
+            node = nb + tgt_all[tgb + t]
+            assert 0 <= tgt_all[tgb + t] < n_p
+            assert [perm[m] for m in mine] == [e for e in range(len(post_k)) if post_k[e] == node]
+            assert all(pre[m] == pre_k[perm[m]] for m in mine)
+            seen += mine
+        assert deg == sorted(deg, reverse=True) and sum(deg) == n_syn
+    assert sorted(seen) == list(range(len(post_k)))
     assert sorted(perm.tolist()) == list(range(len(post_k)))
 
 

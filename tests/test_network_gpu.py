@@ -198,3 +198,39 @@ def test_two_cell_sigmoid_rate_closed_form(device, dtype, tol):
     err = float(np.abs(v[0].double().cpu().numpy() - ref).max())
     print(dtype, "sigmoid-rate closed form max err", err)
     assert err < tol
+
+
+def _parts_of(net):
+    return max(st.parts for st in net.__dict__.get("_fused_cache", {}).values())
+
+
+@pytest.mark.parametrize("n_cells, dtype, tol", [(1024, torch.float64, 1e-6), (4096, torch.float32, 1e-2)])
+def test_network_split_full_size(device, n_cells, dtype, tol):
+    """Networks larger than one workgroup's LDS: the 1024-cell network (10,240 solve
+    nodes) in float64 and a 4096-cell one (40,960 nodes) in float32 ("mixed"), 32
+    KineticSynapses per cell, split into parts that advance in lockstep; current pulses
+    that make the somata spike; against the float64 oracle (float32: on the same float32
+    inputs), spike counts identical."""
+    k, T, B = 32, 400, 2
+    net, om, info = build_c4(n_cells, k, seed=12)
+    soma = info["soma_comps"]
+    rec = soma[::64]
+    for c in rec:
+        net.cell(int(c) // 9).branch(0).comp(0).record()
+    cur = c4_stimulus(B, n_cells, T, seed=13, amp=1.5, p=0.1)
+    if dtype == torch.float32:
+        cur = cur.astype(np.float32).astype(np.float64)
+    ds = None
+    for i, c in enumerate(soma):
+        ds = net.cell(int(c) // 9).branch(0).comp(0).data_stimulate(
+            torch.tensor(cur[:, i, :], dtype=dtype, device=device), ds)
+    v = jx.integrate(net, data_stimuli=ds, delta_t=DT)
+    parts = _parts_of(net)
+    ref = O.simulate(om, None, DT, stim=torch.tensor(cur), stim_comps=list(soma), rec_comps=list(rec))
+    err = float((v.double().cpu() - ref).abs().max())
+    sc = ((ref[..., 1:] >= 0) & (ref[..., :-1] < 0)).sum(-1)
+    sv = ((v.double().cpu()[..., 1:] >= 0) & (v.double().cpu()[..., :-1] < 0)).sum(-1)
+    print(f"{n_cells} cells, {dtype}: {parts} parts, max err {err:.3e} mV, spikes {int(sc.sum())}")
+    assert parts > 1
+    assert int(sc.sum()) > 20 and torch.equal(sc, sv)
+    assert err < tol
