@@ -134,10 +134,12 @@ def _roots(parent: np.ndarray) -> np.ndarray:
 
 
 def _split(sizes: np.ndarray, parts: int) -> np.ndarray:
-    """Part of every tree: consecutive trees, the parts' node counts balanced."""
+    """Part of every tree: consecutive trees, the parts' node counts balanced (at most
+    `parts` parts, none empty: fewer when trees are uneven)."""
     cum = np.cumsum(sizes) - sizes                    # nodes before each tree
     total = float(sizes.sum())
-    return np.minimum((cum * parts / total).astype(np.int64), parts - 1)
+    part = np.minimum((cum * parts / total).astype(np.int64), parts - 1)
+    return np.unique(part, return_inverse=True)[1].reshape(-1)
 
 
 def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, device, parts: int = 1) -> Structure:
@@ -153,6 +155,7 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
     roots, tree_of = np.unique(root, return_inverse=True)
     parts = max(1, min(int(parts), len(roots)))
     part_of_tree = _split(np.bincount(tree_of), parts) if parts > 1 else np.zeros(len(roots), np.int64)
+    parts = int(part_of_tree.max()) + 1
     part_of_solve = part_of_tree[tree_of]
     korder_l, sgs = [], []
     for q in range(parts):

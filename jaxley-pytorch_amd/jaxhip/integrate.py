@@ -557,9 +557,8 @@ def _integrate_fused(module, inp, dt, checkpoint_lengths, return_states, device,
         cfg = fused_config(st, B, dtype, torch.tensor([], dtype=user_dtype).element_size(), needs_grad,
                            measure=measure)
     except NotImplementedError:
-        if not needs_grad:
-            raise
-        # A reverse pass whose system does not fit the LDS: the per-step path.
+        # A cell whose system (or its reverse pass's) does not fit one workgroup's LDS:
+        # the per-step path (its tree solve sweeps the levels in HBM).
         from .stepwise import integrate_stepwise
 
         return integrate_stepwise(module, inp, dt, checkpoint_lengths, device, user_dtype, return_states,

@@ -354,3 +354,16 @@ def test_host_tables_network(cpu_model, synapse):
     v = jx.integrate(net, data_stimuli=ds, delta_t=DT)
     ref = O.simulate(om, None, DT, stim=torch.tensor(cur)[None], stim_comps=list(soma), rec_comps=list(soma))
     assert np.abs(v.numpy() - ref[0].numpy()).max() < 1e-9
+
+
+def test_split_parts_are_never_empty():
+    """Uneven cells (six one-compartment cells and one of 20 compartments) asked for seven
+    parts: consecutive cells, no empty part, every node in exactly one."""
+    from jaxhip.fused import build_structure
+
+    morph = jx.Cell(jx.Branch(jx.Compartment(), 4), parents=[-1, 0, 0, 1, 1])
+    net = jx.Network([jx.Cell()] * 6 + [morph])
+    net.insert(jx.HH())
+    st = build_structure(net, np.array([0]), np.zeros(0, np.int64), torch.device("cpu"), 7)
+    assert 1 < st.parts <= 7 and (st.part_n > 0).all() and st.part_n.sum() == st.n
+    assert np.array_equal(st.part_nb, np.concatenate([[0], np.cumsum(st.part_n)[:-1]]))

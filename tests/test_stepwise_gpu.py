@@ -132,3 +132,26 @@ def test_gradient_wrt_initial_gates(device):
     ref[0, -1].backward()
     g = m0.grad[torch.as_tensor(cell._comp_node, device=device)]
     _grad_check(g, m0o.grad, "initial HH_m")
+
+
+def test_cell_larger_than_lds(device):
+    """A single cell whose system exceeds one workgroup's LDS in float64 (3000 branches x
+    4 compartments, 12,000 compartments; cells are never split between programs): the
+    forward takes the per-step path (its tree solve sweeps the levels in HBM); 200 steps
+    against the oracle (1e-6 mV)."""
+    from models import build_tree, c1_stimulus
+
+    T = 200
+    cell, om = build_tree(3000, seed=5)
+    cell.branch(0).comp(0).record()
+    cell.branch(1500).comp(2).record()
+    cur = torch.tensor(c1_stimulus(T) * 5.0, dtype=torch.float64, device=device)
+    ds = cell.branch(0).comp(0).data_stimulate(cur)
+    v = jx.integrate(cell, data_stimuli=ds, delta_t=DT)
+    rc = int(cell._cumsum_ncomp[1500] + 2)
+    ref = O.simulate(om, None, DT, stim=cur.cpu()[None], stim_comps=[0], rec_comps=[0, rc]).reshape(2, T + 1)
+    assert v.shape == (2, T + 1)
+    err = float((v.cpu() - ref).abs().max())
+    print(f"12,000-compartment cell: max err {err:.2e} mV, root range {float(ref[0].min()):.1f}.."
+          f"{float(ref[0].max()):.1f} mV")
+    assert err < 1e-6
