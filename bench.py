@@ -413,6 +413,9 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--tsteps", type=int, default=2000)
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated legs to run beside the headline: f32, plain, fwd_adj, "
+                         "fwd_adj_sweep, network, swc, network_fwd_adj, served, cpu (default: all)")
     ap.add_argument("--cpu-sims", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -435,6 +438,10 @@ def main():
         sys.exit(2)
     device = torch.device("cuda:0")
     B, T = args.batch, args.tsteps
+    only = None if args.only is None else set(args.only.split(","))
+
+    def want(leg):
+        return not args.no_extras and (only is None or leg in only)
     from jaxhip.integrate import set_float32_mode
 
     _integ = sys.modules["jaxhip.integrate"]
@@ -454,7 +461,7 @@ def main():
     ktime = kernel_time_ms(run64, device)
     from jaxhip.autotune import _CACHE
 
-    cfg = [v for k, v in _CACHE.items() if k[0] == "fused" and k[5] == B and "float64" in k[6]]
+    cfg = [v for k, v in _CACHE.items() if k[0] == "fused" and k[6] == B and "float64" in k[7]]
     bpc = BYTES_PER_COMP_STEP["f64"]
     traffic = MEASURED_TRAFFIC.get(("float64", B, T, str(cfg[0])[13:-1] if cfg else ""))
     achieved = bpc * comp_steps / (ktime * 1e-3)
@@ -487,7 +494,7 @@ def main():
                      "bytes_per_comp_step": bpc},
     }
     rec64 = run64()
-    if not args.no_extras:
+    if want("f32"):
         # ---- float32 modes vs float64 on the GPU (all simulations) ----
         cell32, ps32, ds32, _ = setup(B, T, torch.float32, device)
         # float64 on the float32 inputs (geometry, conductances, stimulus rounded): the
@@ -533,6 +540,7 @@ def main():
             log(f"[bench] f32 {mode}: {comp_steps / s32:.3e} comp-steps/s, max err {float(err.max()):.3e} mV, "
                 f"{float((err <= 1e-2).double().mean()):.4f} within 1e-2")
         set_float32_mode(saved_mode)
+    if want("plain"):
         # ---- plain: one simulation per launch ----
         n_plain = 4
         cp, _, dsp, _ = setup(1, T, torch.float64, device)
@@ -550,6 +558,7 @@ def main():
         out["plain_comp_steps_per_s"] = plain
         out["batched_over_plain"] = value / plain
         log(f"[bench] plain {plain:.3e} comp-steps/s ({value / plain:.0f}x)")
+    if want("fwd_adj"):
         # ---- forward + adjoint: per-branch trainable conductances, B = 256 ----
         Bg = 256
         cg, _, dsg, _ = setup(1, T, torch.float32, device)
@@ -579,15 +588,20 @@ def main():
                           "bytes_per_comp_step": 112,
                           "frac_of_hbm_roofline": 112 * n_comp * T * Bg / sg / HBM_PEAK}
         log(f"[bench] forward+adjoint B={Bg}: {sg*1e3:.1f} ms, {n_comp * T * Bg / sg:.3e} comp-steps/s")
+    if want("fwd_adj_sweep"):
         # ---- forward + adjoint of the batched sweep itself: B = 4096, float64 and float32 ----
         out["fwd_adj_sweep"] = bench_fwd_adj_sweep(device, B, T)
         out["fwd_adj_sweep_f32"] = bench_fwd_adj_sweep(device, B, T, dtype=torch.float32, mode="fast")
         out["fwd_adj_sweep_f32_mixed"] = bench_fwd_adj_sweep(device, B, T, dtype=torch.float32, mode="mixed")
+    if want("network"):
         # ---- network: 1024 ball-and-stick cells, 32768 KineticSynapses, B = 64 ----
         out["network"] = bench_network(device)
+    if want("swc"):
         # ---- a traced morphology (SWC): the CA1 cell, B = 2048, float64 ----
         out["swc"] = bench_swc(device)
+    if want("network_fwd_adj"):
         out["network_fwd_adj"] = bench_network_fwd_adj(device)
+    if want("served"):
         # ---- served: continuous batching of uneven requests ----
         if args.serve_requests > 0:
             for key, rate in (("served", args.serve_rate), ("served_burst", None)):
@@ -598,6 +612,9 @@ def main():
                 log(f"[bench] {key} {sv['requests']} requests at {sv['arrival_rate_per_s']}/s: p50 "
                     f"{sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms, "
                     f"{sv['comp_steps_per_s']:.3e} comp-steps/s")
+    if only is not None and "cpu" not in only:
+        print(json.dumps(out), flush=True)
+        return
     # ---- CPU baseline on bounded samples (rank 0, N = 1) ----
     threads = physical_cores()
     log(f"[bench] cpu baseline: the oracle on {threads} threads (physical cores)")

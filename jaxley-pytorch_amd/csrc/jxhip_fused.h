@@ -1349,8 +1349,16 @@ __device__ __forceinline__ T d_gate_dv(T x, T dt, T al, T be, T dal, T dbe, T& e
 // the system of the transposed solve and, per node, its first recording. Per K-step
 // segment (last first): recompute the segment from its checkpoint into the work arrays,
 // then run the adjoint of each step backwards.
+// Launch bound of the reverse pass: 512 threads with the adjoints in VGPRs; with them in
+// HBM (IPT 0) as many threads as the registers allow without spills, 768 with float64
+// gates (170 VGPRs), 1024 with float32 (128).
+template <typename G>
+constexpr int adjoint_max_threads(int ipt) {
+  return ipt > 0 ? 512 : (sizeof(G) == 8 ? 768 : 1024);
+}
+
 template <typename G, typename L, int IPT, bool kGC, bool kSyn>
-__global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
+__global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
@@ -1499,14 +1507,32 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         }
         __syncthreads();
       }
+      // The item's state loaded one item ahead (its loads in flight during the build).
+      double vc = 0.0;
+      G mc = G(0), hc = G(0), nc = G(0);
+      if (tid < nS) {
+        vc = wv[j * rowS + tid];
+        mc = wm[j * rowS + tid];
+        hc = wh[j * rowS + tid];
+        nc = wn[j * rowS + tid];
+      }
       for (int i = tid; i < nS; i += nthr) {
         const int k = i >> lS, s = i & (S - 1);
-        G m = wm[j * rowS + i], h = wh[j * rowS + i], n = wn[j * rowS + i];
+        const int inx = i + nthr < nS ? i + nthr : i;
+        __builtin_amdgcn_sched_barrier(0);
+        const double vx = wv[j * rowS + inx];
+        const G mx = wm[j * rowS + inx], hx = wh[j * rowS + inx], nx = wn[j * rowS + inx];
+        __builtin_amdgcn_sched_barrier(0);
+        G m = mc, h = hc, n = nc;
         if (s < pg.nsim) {
-          build_item<kCS>(a, pg, tp, Ld, wv + j * rowS, k, s, wv[j * rowS + i], pg.t0 + ts + j, m, h, n);
+          build_item<kCS>(a, pg, tp, Ld, wv + j * rowS, k, s, vc, pg.t0 + ts + j, m, h, n);
         } else {
           idle_item(a, tp, Ld, k, i);
         }
+        vc = vx;
+        mc = mx;
+        hc = hx;
+        nc = nx;
         wm[(j + 1) * rowS + i] = m;
         wh[(j + 1) * rowS + i] = h;
         wn[(j + 1) * rowS + i] = n;
@@ -1521,7 +1547,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
     // right-hand side the voltage adjoint at the step's end (recordings included). The
     // segment's last step takes it from the recomputed trajectory; every other step's
     // system is written by the previous (later) step's item pass.
-    auto system = [&](int i, int64_t t1, const G* mg, const G* hg, const G* ng, L lv) {
+    auto system = [&](int i, int64_t t1, G mm, G hh, G nn, L lv) {
       const int k = i >> lS, s = i & (S - 1);
       const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
       const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
@@ -1529,7 +1555,6 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       G lin = G(0);
       if ((flags & kComp) && s < pg.nsim) {
         if (flags & kHH) {
-          const G mm = mg[i], hh = hg[i], nn = ng[i];
           const G n2 = nn * nn;
           lin += G(1000) * (prm<0>(a, pg, tp, k, s) * (mm * mm * mm) * hh +
                             prm<1>(a, pg, tp, k, s) * (n2 * n2) + prm<2>(a, pg, tp, k, s));
@@ -1553,7 +1578,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       for (int jj = 0; jj < kJ; ++jj) {
         const int i = tid + jj * nthr;
         if (i >= nS) break;
-        system(i, te, wm + jl * rowS, wh + jl * rowS, wn + jl * rowS, lv_get(jj, i));
+        system(i, te, wm[jl * rowS + i], wh[jl * rowS + i], wn[jl * rowS + i], lv_get(jj, i));
       }
       __syncthreads();
     }
@@ -1567,10 +1592,33 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
       const G* m1 = wm + (j + 1) * rowS;
       const G* h1 = wh + (j + 1) * rowS;
       const G* n1 = wn + (j + 1) * rowS;
+      // The trajectory of an item around the step, loaded one item ahead (the loads of
+      // the next item are in flight while the current one computes).
+      struct RevItem {
+        double v0, v1;
+        G m0, h0, n0, m1, h1, n1;
+      };
+      auto load_rev = [&](int i) {
+        RevItem r;
+        r.v0 = v0[i];
+        r.v1 = v1[i];
+        r.m0 = m0[i];
+        r.h0 = h0[i];
+        r.n0 = n0[i];
+        r.m1 = m1[i];
+        r.h1 = h1[i];
+        r.n1 = n1[i];
+        return r;
+      };
+      RevItem cur = load_rev(tid < nS ? tid : 0);
 #pragma unroll 1
       for (int jj = 0; jj < kJ; ++jj) {
         const int i = tid + jj * nthr;
         if (i >= nS) break;
+        const int inx = i + nthr < nS ? i + nthr : i;
+        __builtin_amdgcn_sched_barrier(0);
+        const RevItem nxt = load_rev(inx);
+        __builtin_amdgcn_sched_barrier(0);
         const int k = i >> lS, s = i & (S - 1);
         const int32_t* nik = a.node_i + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS;
         const L* nfk = a.node_f + static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_FCOLS;
@@ -1579,7 +1627,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
         if ((flags & kComp) && s < pg.nsim) {
           const G inv_cm = static_cast<G>(nfk[kNfInvCm]);
           const G mu = static_cast<G>(Ld.xs[i]);
-          const G gl = -mu * static_cast<G>(v1[i]) * dt * inv_cm;  // dL/dlin
+          const G gl = -mu * static_cast<G>(cur.v1) * dt * inv_cm;  // dL/dlin
           const G gc = mu * dt * inv_cm;                           // dL/dcst
           lvn = mu;
           if (flags & kHH) {
@@ -1588,7 +1636,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
             const G eNa = prm<3>(a, pg, tp, k, s);
             const G eK = prm<4>(a, pg, tp, k, s);
             const G eL = prm<5>(a, pg, tp, k, s);
-            const G mm = m1[i], hh = h1[i], nn = n1[i];
+            const G mm = cur.m1, hh = cur.h1, nn = cur.n1;
             const G m3 = mm * mm * mm, n3 = nn * nn * nn, n4 = n3 * nn;
             const G wNa = G(1000) * (gl + gc * eNa);
             const G wK = G(1000) * (gl + gc * eK);
@@ -1614,11 +1662,11 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
             const G lam_h1 = lam_h0 + wNa * gNa * m3;
             const G lam_n1 = lam_n0 + wK * gK * G(4) * n3;
             G am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn;
-            hh_rates_dv_step(static_cast<G>(v0[i]), am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn);
+            hh_rates_dv_step(static_cast<G>(cur.v0), am, bm, ah, bh, an, bn, dam, dbm, dah, dbh, dan, dbn);
             G em, eh, en;
-            const G dmv = d_gate_dv(m0[i], dt, am, bm, dam, dbm, em);
-            const G dhv = d_gate_dv(h0[i], dt, ah, bh, dah, dbh, eh);
-            const G dnv = d_gate_dv(n0[i], dt, an, bn, dan, dbn, en);
+            const G dmv = d_gate_dv(cur.m0, dt, am, bm, dam, dbm, em);
+            const G dhv = d_gate_dv(cur.h0, dt, ah, bh, dah, dbh, eh);
+            const G dnv = d_gate_dv(cur.n0, dt, an, bn, dan, dbn, en);
             lvn += lam_m1 * dmv + lam_h1 * dhv + lam_n1 * dnv;
             if constexpr (IPT > 0) {
               lmr[jj] = lam_m1 * em;
@@ -1650,10 +1698,11 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
           // The item's own slots of the earlier step's system (the solve above read every
           // slot before its closing barrier); at the segment's first step the adjoint
           // waits for the next segment in VGPRs / HBM.
-          system(i, ts + j, m0, h0, n0, static_cast<L>(lvn));
+          system(i, ts + j, cur.m0, cur.h0, cur.n0, static_cast<L>(lvn));
         } else {
           lv_set(jj, i, static_cast<L>(lvn));
         }
+        cur = nxt;
       }
       __syncthreads();
       if constexpr (kSyn) {
@@ -1734,7 +1783,7 @@ __global__ void __launch_bounds__(512) fused_adjoint_kernel(Fused<G, L> a, Adj<G
             lv = static_cast<L>(static_cast<G>(lv) + add);
           }
           if (j > 0) {
-            system(i, ts + j, m0, h0, n0, lv);
+            system(i, ts + j, m0[i], h0[i], n0[i], lv);
           } else {
             lv_set(jj, i, lv);
           }
@@ -1902,7 +1951,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (syn && a.syn_model == 0 && (adj->work_s == nullptr || adj->lam_s == nullptr || a.ckpt_s == nullptr))
     return hipErrorInvalidValue;
   if (a.node_p2 != nullptr) return hipErrorInvalidValue;  // HH / Leak cells only
-  if (a.threads > 512) return hipErrorInvalidValue;  // fused_adjoint_kernel's launch bound
+  if (a.threads > adjoint_max_threads<G>(a.items_per_thread)) return hipErrorInvalidValue;
   // The reverse pass differentiates the backward-Euler step only.
   if ((a.dt_v > 0 && a.dt_v != a.dt) || (a.rhs_scale > 0 && a.rhs_scale != 1.0) || a.explicit_step)
     return hipErrorInvalidValue;

@@ -17,7 +17,8 @@ from typing import Dict, List, Tuple
 import torch
 
 from . import _lib
-from .fused import (ALL_PARAMS, BATCHABLE, ForwardLaunch, adjoint_config, adjoint_lds_bytes, from_prog,
+from .fused import (ALL_PARAMS, BATCHABLE, ForwardLaunch, adjoint_config, adjoint_lds_bytes, adjoint_threads,
+                    from_prog,
                     to_prog)
 
 
@@ -106,7 +107,8 @@ class _FusedIntegrate(torch.autograd.Function):
         acfg = adjoint_config(nmax, run.B, run.node_f.element_size(), st.sched, S=S, n_tgt=n_tgt)
         saved = (args.lds_bytes, args.threads, args.items_per_thread, args.coef_global)
         args.lds_bytes = adjoint_lds_bytes(nmax, S, run.node_f.element_size(), st.sched, acfg.gcoef, n_tgt)
-        args.threads, args.items_per_thread, args.coef_global = acfg.threads, acfg.ipt, int(acfg.gcoef)
+        args.threads = adjoint_threads(acfg, torch.tensor([], dtype=dtype).element_size())
+        args.items_per_thread, args.coef_global = acfg.ipt, int(acfg.gcoef)
         f = run.kernel_name("jxhip_fused_adjoint")
         try:
             _lib.check(f(args, adj, _lib.stream_ptr(dev)), "fused_adjoint")

@@ -502,6 +502,13 @@ def default_config(n: int, n_tgt: int, B: int, esize: int, gsize: int = 0,
 
 
 ADJOINT_THREADS_MAX = 512
+
+
+def adjoint_threads(cfg: "AdjointConfig", gsize: int) -> int:
+    """Threads of the reverse pass: the config's, or with the adjoints in HBM (ipt 0) the
+    kernel's larger instance (more waves in flight): 768 with float64 gates, 1024 with
+    float32."""
+    return cfg.threads if cfg.ipt else (768 if gsize == 8 else 1024)
 # Voltage adjoints per thread the reverse pass keeps in VGPRs (its kernels' instances);
 # 0: in HBM, for programs with more items than that.
 ADJ_IPT = (1, 2, 4, 8, 12, 16)
