@@ -415,7 +415,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--only", default=None,
                     help="comma-separated legs to run beside the headline: f32, plain, fwd_adj, "
-                         "fwd_adj_sweep, network, swc, network_fwd_adj, served, cpu (default: all)")
+                         "fwd_adj_sweep (all three dtypes), fwd_adj_sweep_f32, fwd_adj_sweep_f32_mixed, "
+                         "network, swc, network_fwd_adj, served, cpu (default: all)")
     ap.add_argument("--cpu-sims", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -588,10 +589,12 @@ def main():
                           "bytes_per_comp_step": 112,
                           "frac_of_hbm_roofline": 112 * n_comp * T * Bg / sg / HBM_PEAK}
         log(f"[bench] forward+adjoint B={Bg}: {sg*1e3:.1f} ms, {n_comp * T * Bg / sg:.3e} comp-steps/s")
+    # ---- forward + adjoint of the batched sweep itself: B = 4096, float64 and float32 ----
     if want("fwd_adj_sweep"):
-        # ---- forward + adjoint of the batched sweep itself: B = 4096, float64 and float32 ----
         out["fwd_adj_sweep"] = bench_fwd_adj_sweep(device, B, T)
+    if want("fwd_adj_sweep") or want("fwd_adj_sweep_f32"):
         out["fwd_adj_sweep_f32"] = bench_fwd_adj_sweep(device, B, T, dtype=torch.float32, mode="fast")
+    if want("fwd_adj_sweep") or want("fwd_adj_sweep_f32_mixed"):
         out["fwd_adj_sweep_f32_mixed"] = bench_fwd_adj_sweep(device, B, T, dtype=torch.float32, mode="mixed")
     if want("network"):
         # ---- network: 1024 ball-and-stick cells, 32768 KineticSynapses, B = 64 ----
