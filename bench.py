@@ -353,56 +353,84 @@ def bench_network_fwd_adj(device, n_cells=1024, k=32, B=64, T=4000, K=100, dtype
     return res
 
 
-def bench_served(device, n_req, rate, dtype, chunk, **server_kw):
-    """Poisson arrivals of `n_req` requests at `rate` per second into a SimServer."""
-    from jaxhip.serve import SimServer
-
-    specs = c2_specs(n_req, seed=7)
-    cells = []
-    for s in specs:                       # morphologies are built before the clock starts
+def _served_requests(n_req, dtype, seed=7):
+    """The clients' request objects (morphology, parameters, stimulus): built before any
+    clock starts; the server's work begins when a request is submitted."""
+    specs = c2_specs(n_req, seed=seed)
+    cells, reqs = [], []
+    cpu = torch.device("cpu")          # request tensors live on the host until admission
+    for s in specs:
         cell = tree_cell(s["nbranch"], s["seed"])[0]
         cell.branch(0).comp(0).record()
         cells.append(cell)
+        reqs.append(c2_request(s, cpu, dtype, cell))
+    return cells, reqs
+
+
+def _run_served(srv, cells, reqs, rate):
+    """Poisson arrivals of the requests at `rate` per second (None: all at time 0) into
+    an open server; latency counts from the arrival time."""
+    n_req = len(reqs)
     rng = np.random.default_rng(8)
-    # rate None: every request arrives at time 0 (the server's capacity).
     arrivals = (np.zeros(n_req) if rate is None
                 else np.cumsum(rng.exponential(1.0 / rate, size=n_req)))
-    srv = SimServer(device, dtype=dtype, delta_t=DT, hbm_budget=4 << 30, chunk_steps=chunk,
-                    max_steps=4001, max_stim_entries=1024, **server_kw)
-    # Warm the launch path (kernel load, first shapes) and every preparation worker.
-    cpu = torch.device("cpu")          # request tensors live on the host until admission
-    for _ in range(max(1, server_kw.get("prep_processes", 0))):
-        srv.submit(c2_request(specs[0], cpu, dtype, cells[0]))
-    srv.run_until_idle()
     srv.handles.clear()
+    launches0 = srv.launches
     handles = []
     t0 = time.perf_counter()
     i = 0
     while i < n_req or srv.outstanding():
         now = time.perf_counter() - t0
         while i < n_req and arrivals[i] <= now:
-            h = srv.submit(c2_request(specs[i], cpu, dtype, cells[i]))
-            h.t_submit = t0 + arrivals[i]     # latency counts from the arrival time
+            h = srv.submit(reqs[i])
+            h.t_submit = t0 + arrivals[i]
             handles.append(h)
             i += 1
+        if i == n_req:
+            srv.run_until_idle()              # every request has arrived
+            break
         srv.poll(block=False)
-        if len(srv.inflight) >= 2:
+        if len(srv.inflight) >= srv.max_inflight:
             srv.poll(block=True)              # waits on the GPU (releases the interpreter)
         else:
-            # Yield to the preparation thread until the next arrival or event.
-            nxt = arrivals[i] - (time.perf_counter() - t0) if i < n_req else 5e-4
+            # Yield to the preparation workers until the next arrival or event.
+            nxt = arrivals[i] - (time.perf_counter() - t0)
             time.sleep(max(5e-5, min(5e-4, nxt)))
     makespan = time.perf_counter() - t0
-    srv.close()
     lat = np.array([h.latency for h in handles])
     comp_steps = sum(len(c._comp_node) * h.T for c, h in zip(cells, handles))
     return {"requests": n_req, "arrival_rate_per_s": rate if rate is not None else "all at t=0",
-            "chunk_steps": chunk,
+            "chunk_steps": srv.chunk_steps,
             "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
             "makespan_s": makespan, "comp_steps_per_s": comp_steps / makespan,
-            "launches": srv.launches, "dtype": str(dtype).replace("torch.", ""),
-            "prep_processes": server_kw.get("prep_processes", 0)}
+            "launches": srv.launches - launches0, "prep_order": srv.prep_order}
+
+
+def bench_served(device, n_req, legs, chunk=200, prep_processes=14):
+    """The served legs `(name, rate, dtype, prep_order)`: one long-running server per
+    dtype (its preparation workers warmed by an untimed burst of other requests), the
+    legs one after the other on it."""
+    from jaxhip.serve import SimServer
+
+    out = {}
+    for dtype in dict.fromkeys(d for _, _, d, _ in legs):
+        cells, reqs = _served_requests(n_req, dtype)
+        wcells, wreqs = _served_requests(128, dtype, seed=99)
+        srv = SimServer(device, dtype=dtype, delta_t=DT, hbm_budget=4 << 30, chunk_steps=chunk,
+                        max_steps=4001, max_stim_entries=1024, prep_processes=prep_processes)
+        try:
+            _run_served(srv, wcells, wreqs, None)
+            for name, rate, d, order in legs:
+                if d != dtype:
+                    continue
+                srv.prep_order = order
+                res = _run_served(srv, cells, reqs, rate)
+                res.update(dtype=str(dtype).replace("torch.", ""), prep_processes=prep_processes)
+                out[name] = res
+        finally:
+            srv.close()
+    return out
 
 
 def main():
@@ -420,7 +448,9 @@ def main():
     ap.add_argument("--cpu-sims", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--prep-processes", type=int, default=12,
+    ap.add_argument("--serve-rate-high", type=float, default=2000.0,
+                    help="arrival rate of the saturated served leg (requests per second)")
+    ap.add_argument("--prep-processes", type=int, default=14,
                     help="request preparation workers of the served runs (0: one thread)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (calibrates HBM byte counters)")
@@ -606,14 +636,25 @@ def main():
         out["network_fwd_adj"] = bench_network_fwd_adj(device)
     if want("served"):
         # ---- served: continuous batching of uneven requests ----
+        # Poisson arrivals at the given rate and at a rate above the server's capacity (the
+        # server, not the arrivals, bounds latency there), then the same requests all at
+        # time 0 (capacity; preparation of the longest simulations first), in float64 and
+        # float32.
         if args.serve_requests > 0:
-            for key, rate in (("served", args.serve_rate), ("served_burst", None)):
-                ncpu_s = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-                sv = bench_served(device, args.serve_requests, rate, torch.float64, 200,
-                                  prep_processes=max(0, min(args.prep_processes, ncpu_s - 2)))
+            ncpu_s = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+            procs = max(0, min(args.prep_processes, ncpu_s - 2))
+            legs = (("served", args.serve_rate, torch.float64, "fifo"),
+                    ("served_saturated", args.serve_rate_high, torch.float64, "fifo"),
+                    ("served_burst", None, torch.float64, "longest_first"),
+                    ("served_burst_f32", None, torch.float32, "longest_first"))
+            res = bench_served(device, args.serve_requests, legs, 200, procs)
+            for key, _, _, _ in legs:
+                sv = res[key]
+                if "plain_comp_steps_per_s" in out:
+                    sv["over_plain"] = sv["comp_steps_per_s"] / out["plain_comp_steps_per_s"]
                 out[key] = sv
-                log(f"[bench] {key} {sv['requests']} requests at {sv['arrival_rate_per_s']}/s: p50 "
-                    f"{sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms, "
+                log(f"[bench] {key} {sv['requests']} requests at {sv['arrival_rate_per_s']}/s ({sv['dtype']}): "
+                    f"p50 {sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms, "
                     f"{sv['comp_steps_per_s']:.3e} comp-steps/s")
     if only is not None and "cpu" not in only:
         print(json.dumps(out), flush=True)

@@ -13,6 +13,7 @@ from typing import Optional
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libjxhip.so"
+HOST_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libjxhost.so"
 
 JXHIP_PROG_COLS = 8
 JXHIP_TOPO_COLS = 16
@@ -155,3 +156,34 @@ def require_gpu(t: torch.Tensor, what: str) -> None:
         raise RuntimeError(
             f"{what}: the jaxhip kernels run on the GPU only (got a tensor on {t.device})"
         )
+
+
+# Host library (include/jxhip_host.h): schedule construction, no GPU runtime.
+_HOST_SIGS = {
+    "jxhost_bfs_edge_hops": [_i64, _i64, _p, _p, _i64, _i64, _p, _p, _p, _p],
+    "jxhost_segments": [_i64, _p, _p, _p, _p, _p, _p],
+}
+_host: Optional[C.CDLL] = None
+
+
+def host() -> C.CDLL:
+    """Load libjxhost.so (raises if it has not been built)."""
+    global _host
+    if _host is not None:
+        return _host
+    if not HOST_LIB_PATH.exists():
+        raise RuntimeError(f"{HOST_LIB_PATH} is missing: build the library first (jaxhip.build.build())")
+    lib = C.CDLL(str(HOST_LIB_PATH))
+    for name, args in _HOST_SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _host = lib
+    return lib
+
+
+def host_check(code: int, what: str) -> None:
+    if code != 0:
+        reason = {1: "invalid argument", 2: "the graph is not connected",
+                  3: "not a forest with parents before children"}.get(code, str(code))
+        raise ValueError(f"{what}: {reason}")

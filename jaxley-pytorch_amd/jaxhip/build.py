@@ -1,4 +1,5 @@
-"""Build `libjxhip.so` (all HIP kernels of the package) for gfx950 with hipcc.
+"""Build `libjxhip.so` (all HIP kernels of the package) for gfx950 with hipcc, and the
+host library `libjxhost.so` (schedule construction, `csrc/host/*.cpp`) with g++.
 
 Compiles every `csrc/*.hip` to an object in parallel and links one shared library into
 `jaxhip/_lib/`. Called by `__graft_entry__.build()`; a user can also run
@@ -18,6 +19,7 @@ ROOT = PKG_DIR.parent                      # jaxley-pytorch_amd/
 CSRC = ROOT / "csrc"
 LIB_DIR = PKG_DIR / "_lib"
 LIB_PATH = LIB_DIR / "libjxhip.so"
+HOST_LIB_PATH = LIB_DIR / "libjxhost.so"
 INCLUDE = ROOT.parent / "include"
 ARCH = os.environ.get("JAXHIP_ARCH", "gfx950")
 
@@ -74,7 +76,28 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB_PATH)
+    build_host(verbose)
     return LIB_PATH
+
+
+def build_host(verbose: bool = False) -> Path:
+    """Compile the host library `libjxhost.so` (schedule construction; plain C++, no
+    HIP, so that request-preparation processes can load it without the GPU runtime)."""
+    sources = sorted((CSRC / "host").glob("*.cpp"))
+    LIB_DIR.mkdir(exist_ok=True)
+    tmp = HOST_LIB_PATH.with_suffix(".so.tmp")
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("g++ not found; the jaxhip host library needs a C++ compiler")
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", *map(str, sources),
+           "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host library build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, HOST_LIB_PATH)
+    return HOST_LIB_PATH
 
 
 if __name__ == "__main__":

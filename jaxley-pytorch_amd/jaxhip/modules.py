@@ -226,21 +226,7 @@ class Module(View):
         self._cell_specs = [(np.asarray(p, dtype=np.int64), np.asarray(nc, dtype=np.int64)) for p, nc in cells]
         self._vectorize = vectorize_cells
         self._solver_device = "gpu"
-        tabs = [sch.cell_tables(p, nc) for p, nc in self._cell_specs]
-        node_off = np.concatenate([[0], np.cumsum([t["n_nodes"] for t in tabs])]).astype(np.int64)
-        self._cell_node_off = node_off
-        self._comp_node = np.concatenate([t["internal"] + node_off[i] for i, t in enumerate(tabs)])
-        self._n_comp = len(self._comp_node)
-        self._n_nodes = int(node_off[-1])
-        self._ncomp_per_branch = np.concatenate([nc for _, nc in self._cell_specs])
-        self._cumsum_ncomp = np.concatenate([[0], np.cumsum(self._ncomp_per_branch)]).astype(np.int64)
-        nbranch = np.array([len(p) for p, _ in self._cell_specs])
-        self._cell_branch_off = np.concatenate([[0], np.cumsum(nbranch)]).astype(np.int64)
-        self._comp_branch = np.repeat(np.arange(len(self._ncomp_per_branch)), self._ncomp_per_branch)
-        self._comp_cell = np.repeat(np.arange(len(self._cell_specs)), [int(nc.sum()) for _, nc in self._cell_specs])
-        self._cell_tabs = tabs
-        self._tables = None
-        self._indexer = None
+        self._init_layout()
         self._allowed_nodes_per_level = 32
         self._node_data: Dict[str, np.ndarray] = {
             k: np.full(self._n_comp, v, dtype=float) for k, v in {**COMP_PARAMS, **COMP_STATES}.items()
@@ -259,8 +245,59 @@ class Module(View):
         self.indices_set_by_trainables: List[np.ndarray] = []
         self.num_trainable_params = 0
         self._version = 0
-        comps = np.arange(self._n_comp)
-        super().__init__(self, comps, "module", np.zeros(self._n_comp, dtype=np.int64))
+
+    # Attributes derived from the cell specs (rebuilt by `_init_layout`) and caches: not
+    # part of a pickled module.
+    _DERIVED = ("_cell_tabs", "_cell_node_off", "_comp_node", "_n_comp", "_n_nodes", "_ncomp_per_branch",
+                "_cumsum_ncomp", "_cell_branch_off", "_comp_branch", "_comp_cell", "_tables", "_indexer",
+                "_fused_cache", "base", "_comps", "_groups")
+
+    def __getstate__(self):
+        """A compact pickled form (request preparation in worker processes pickles every
+        served module): the cell specs and the user-set data, the per-compartment
+        columns packed into one array; the derived layout is rebuilt on unpickling."""
+        d = {k: v for k, v in self.__dict__.items() if k not in self._DERIVED}
+        cols = d["_node_data"]
+        keys = list(cols)
+        if keys and all(isinstance(cols[k], np.ndarray) and cols[k].dtype == np.float64
+                        and cols[k].shape == (self._n_comp,) for k in keys):
+            d["_node_data"] = ("__packed__", keys, np.stack([cols[k] for k in keys]))
+        d["__scope_groups__"] = None if self._groups is None else (self._groups if self._groups.any() else None)
+        return d
+
+    def __setstate__(self, d):
+        cols = d.pop("_node_data")
+        groups = d.pop("__scope_groups__", None)
+        self.__dict__.update(d)
+        self._init_layout(reset_view=False)
+        if isinstance(cols, tuple) and len(cols) == 3 and cols[0] == "__packed__":
+            _, keys, arr = cols
+            cols = {k: arr[i].copy() for i, k in enumerate(keys)}
+        self._node_data = cols
+        self.base = self
+        self._comps = np.arange(self._n_comp)
+        self._groups = np.zeros(self._n_comp, dtype=np.int64) if groups is None else groups
+
+    def _init_layout(self, reset_view: bool = True):
+        """Compartment / node layout of the cell specs (tables, offsets, index maps)."""
+        tabs = [sch.cell_tables(p, nc) for p, nc in self._cell_specs]
+        node_off = np.concatenate([[0], np.cumsum([t["n_nodes"] for t in tabs])]).astype(np.int64)
+        self._cell_node_off = node_off
+        self._comp_node = np.concatenate([t["internal"] + node_off[i] for i, t in enumerate(tabs)])
+        self._n_comp = len(self._comp_node)
+        self._n_nodes = int(node_off[-1])
+        self._ncomp_per_branch = np.concatenate([nc for _, nc in self._cell_specs])
+        self._cumsum_ncomp = np.concatenate([[0], np.cumsum(self._ncomp_per_branch)]).astype(np.int64)
+        nbranch = np.array([len(p) for p, _ in self._cell_specs])
+        self._cell_branch_off = np.concatenate([[0], np.cumsum(nbranch)]).astype(np.int64)
+        self._comp_branch = np.repeat(np.arange(len(self._ncomp_per_branch)), self._ncomp_per_branch)
+        self._comp_cell = np.repeat(np.arange(len(self._cell_specs)), [int(nc.sum()) for _, nc in self._cell_specs])
+        self._cell_tabs = tabs
+        self._tables = None
+        self._indexer = None
+        if reset_view:
+            comps = np.arange(self._n_comp)
+            View.__init__(self, self, comps, "module", np.zeros(self._n_comp, dtype=np.int64))
 
     # -- structure ----------------------------------------------------------------------
     @property

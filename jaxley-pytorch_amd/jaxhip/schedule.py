@@ -12,12 +12,16 @@ reference bit for bit:
   `dhs_permutation_indices` (solver_utils.py:80-132) and the level grouping
   (solver_utils.py:226-251); networks concatenate cells (network.py:192-293).
 
-The kernel schedules (`forest_schedule`, `FusedTopology`) are this package's own:
-heights, height-sorted node order and child lists (CSR) of the forest in solve order.
+The kernel schedules (`forest_schedule`, `FusedTopology`, `segments`) are this
+package's own: heights, height-sorted node order and child lists (CSR) of the forest in
+solve order, and the segment order of the fused kernels.
+
+The breadth-first walk with its level hops and the segment order run in the native host
+library (`libjxhost.so`, include/jxhip_host.h): they are the per-node loops of a
+request's preparation.
 """
 from __future__ import annotations
 
-from collections import deque
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -88,60 +92,35 @@ def cell_tables(parents: Sequence[int], ncomp_per_branch: Sequence[int]) -> Dict
     }
 
 
-def _bfs_order(tables: Dict, root: int = 0):
-    """BFS tree edges (parent, child) and distances over the undirected compartment
-    graph, visiting neighbours in networkx's `to_undirected` adjacency order."""
+def _bfs_edge_hops(tables: Dict, allowed_nodes_per_level: int, root: int = 0):
+    """BFS tree edges (parent, child) over the undirected compartment graph, visiting
+    neighbours in networkx's `to_undirected` adjacency order, with their levels (native:
+    `jxhost_bfs_edge_hops`)."""
+    from . import _lib
+
     n = tables["n_nodes"]
     typ, s, t = tables["type"], tables["source"], tables["sink"]
     keep = np.isin(typ, [2, 3]) | ((typ == 0) & (s < t))
-    ks, kt = s[keep], t[keep]
-    # Successors per node in insertion (row) order.
-    succ: List[List[int]] = [[] for _ in range(n)]
-    for u, v in zip(ks.tolist(), kt.tolist()):
-        if v not in succ[u]:
-            succ[u].append(v)
-    adj: List[Dict[int, None]] = [dict() for _ in range(n)]
-    for u in range(n):
-        au = adj[u]
-        for v in succ[u]:
-            au[v] = None
-            adj[v][u] = None
-    dist = np.full(n, -1, dtype=np.int64)
-    dist[root] = 0
-    edges = []
-    q = deque([root])
-    while q:
-        u = q.popleft()
-        for v in adj[u]:
-            if dist[v] < 0:
-                dist[v] = dist[u] + 1
-                edges.append((u, v))
-                q.append(v)
-    if len(edges) != n - 1:
-        raise ValueError("the compartment graph is not connected")
-    return edges, dist
+    ks = np.ascontiguousarray(s[keep], dtype=np.int64)
+    kt = np.ascontiguousarray(t[keep], dtype=np.int64)
+    par = np.empty(max(n - 1, 1), dtype=np.int64)
+    chi = np.empty_like(par)
+    lev = np.empty_like(par)
+    dist = np.empty(n, dtype=np.int64)
+    code = _lib.host().jxhost_bfs_edge_hops(n, len(ks), ks.ctypes.data, kt.ctypes.data, root,
+                                            int(allowed_nodes_per_level), par.ctypes.data, chi.ctypes.data,
+                                            dist.ctypes.data, lev.ctypes.data)
+    _lib.host_check(code, "the DHS solve order")
+    return par[:n - 1], chi[:n - 1], lev[:n - 1]
 
 
 def dhs_indexer(tables: Dict, allowed_nodes_per_level: int = 32, root: int = 0) -> Dict:
     """DHS solve order of one cell (see module docstring)."""
     n = tables["n_nodes"]
-    edges, dist = _bfs_order(tables, root)
-    levels = np.zeros(len(edges), dtype=np.int64)
-    depth, since, last = 0, 0, 0
-    for i, (_, v) in enumerate(edges):
-        if dist[v] > last:
-            depth += 1
-            since = 0
-            last = dist[v]
-        elif since >= allowed_nodes_per_level:
-            depth += 1
-            since = 0
-        since += 1
-        levels[i] = depth
+    e_par, e_chi, levels = _bfs_edge_hops(tables, allowed_nodes_per_level, root)
     inv = np.zeros(n, dtype=np.int64)        # node -> solve index
     inv[root] = 0
-    for i, (_, v) in enumerate(edges):
-        inv[v] = i + 1
+    inv[e_chi] = np.arange(1, n, dtype=np.int64)
     mapping = np.argsort(inv)                # solve index -> node
     # Lower / upper values: edge k carries value k. In solve order, an edge whose
     # source precedes its sink is a "lower" of the sink (child); otherwise an "upper" of
@@ -153,16 +132,13 @@ def dhs_indexer(tables: Dict, allowed_nodes_per_level: int = 32, root: int = 0) 
     low_vals = k[is_low][np.argsort(st[is_low], kind="stable")]
     up_vals = k[~is_low][np.argsort(ss[~is_low], kind="stable")]
     lu = np.concatenate([low_vals, up_vals])
-    node_order = np.zeros((len(edges), 3), dtype=np.int64)
-    if len(edges):
-        e = np.asarray(edges, dtype=np.int64)
-        node_order[:, 0] = inv[e[:, 1]]
-        node_order[:, 1] = inv[e[:, 0]]
-        node_order[:, 2] = levels
+    node_order = np.zeros((n - 1, 3), dtype=np.int64)
+    node_order[:, 0] = inv[e_chi]
+    node_order[:, 1] = inv[e_par]
+    node_order[:, 2] = levels
     grouped = group_levels(node_order)
     parent_lookup = -np.ones(n + 1, dtype=np.int64)
-    if len(edges):
-        parent_lookup[node_order[:, 0]] = node_order[:, 1]
+    parent_lookup[node_order[:, 0]] = node_order[:, 1]
     return {
         "map_to_solve_order": mapping,
         "inv_map_to_solve_order": inv,
@@ -344,60 +320,31 @@ def segments(parent: np.ndarray):
     of single-child nodes (top first); a segment's level is 0 if its bottom node has no
     children, else 1 + the largest level of its child segments. Returns the kernel order
     (kernel index -> input index), the segments `(beg, end)` in kernel indices grouped by
-    level (level pointer; longest first within a level), and the child-segment tops of
-    every segment."""
-    parent = np.asarray(parent, dtype=np.int64)
+    level (level pointer; longest first within a level, so that the threads of a
+    wavefront walk chains of similar length), and the child-segment tops of every
+    segment (native: `jxhost_segments`)."""
+    from . import _lib
+
+    parent = np.ascontiguousarray(parent, dtype=np.int64)
     n = len(parent)
-    nch = np.bincount(parent[parent >= 0], minlength=n)
-    kids: List[List[int]] = [[] for _ in range(n)]
-    for c in np.nonzero(parent >= 0)[0]:
-        kids[parent[c]].append(int(c))
-    is_top = (parent < 0) | (nch[np.maximum(parent, 0)] != 1)
-    tops = np.nonzero(is_top)[0]
-    seg_nodes = []
-    seg_of = -np.ones(n, dtype=np.int64)
-    for t in tops:
-        chain = [int(t)]
-        while nch[chain[-1]] == 1:
-            chain.append(kids[chain[-1]][0])
-        seg_of[chain] = len(seg_nodes)
-        seg_nodes.append(chain)
-    nseg = len(seg_nodes)
-    child_segs = [[int(seg_of[c]) for c in kids[ch[-1]]] if nch[ch[-1]] >= 2 else [] for ch in seg_nodes]
-    level = np.zeros(nseg, dtype=np.int64)
-    # Children have larger input indices than their parents: process segments by
-    # decreasing top index so child levels are final before their parent's.
-    for q in np.argsort([ch[0] for ch in seg_nodes])[::-1]:
-        if child_segs[q]:
-            level[q] = 1 + max(level[c] for c in child_segs[q])
-    # Within a level, longer segments first: the threads of a wavefront then walk chains
-    # of similar length.
-    seg_len = np.array([len(ch) for ch in seg_nodes], dtype=np.int64)
-    seg_order = np.lexsort((np.array([ch[0] for ch in seg_nodes]), -seg_len, level))
-    korder = np.concatenate([seg_nodes[q] for q in seg_order]).astype(np.int64)
+    korder = np.empty(max(n, 1), dtype=np.int64)
+    seg = np.empty((max(n, 1), 4), dtype=np.int32)
+    lptr = np.empty(n + 2, dtype=np.int32)
+    seg_child = np.empty(max(n, 1), dtype=np.int32)
+    sizes = np.zeros(3, dtype=np.int64)
+    code = _lib.host().jxhost_segments(n, parent.ctypes.data, korder.ctypes.data, seg.ctypes.data,
+                                       lptr.ctypes.data, seg_child.ctypes.data, sizes.ctypes.data)
+    _lib.host_check(code, "the segment order")
+    nseg, L, nsc = (int(x) for x in sizes)
+    korder = korder[:n]
     kidx = np.empty(n, dtype=np.int64)
     kidx[korder] = np.arange(n)
-    seg = np.zeros((nseg, 4), dtype=np.int32)
-    seg_child = []
-    pos = 0
-    for r, q in enumerate(seg_order):
-        ch = seg_nodes[q]
-        seg[r, 0] = kidx[ch[0]]
-        seg[r, 1] = kidx[ch[0]] + len(ch)
-        tops_k = [int(kidx[seg_nodes[c][0]]) for c in child_segs[q]]
-        seg[r, 2] = pos
-        seg[r, 3] = pos + len(tops_k)
-        seg_child += tops_k
-        pos += len(tops_k)
-    L = int(level.max()) if nseg else 0
-    counts = np.bincount(level, minlength=L + 1)
-    lptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     return {
         "korder": korder,
         "kidx": kidx,
-        "seg": seg,
-        "seg_level_ptr": lptr,
-        "seg_child": np.asarray(seg_child if seg_child else [0], dtype=np.int32),
+        "seg": seg[:nseg].copy(),
+        "seg_level_ptr": lptr[:L + 2].copy(),
+        "seg_child": seg_child[:nsc].copy() if nsc else np.zeros(1, np.int32),
         "L": L,
         "nseg": nseg,
     }

@@ -29,8 +29,10 @@ tests check alone vs in a mix, and against `integrate`).
 from __future__ import annotations
 
 import concurrent.futures as cf
+import heapq
 import importlib
 import multiprocessing
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import Any, Dict, List, Optional
@@ -82,6 +84,45 @@ class _Arena:
         self.free_list = merged
 
 
+class _Staging:
+    """Pinned host buffers and their device copies, one pair per element type, reused for
+    every upload (grown when a batch needs more). `put` waits until the previous upload
+    has left the host buffers, packs the arrays, starts one host-to-device copy per type
+    on `stream` and returns device views of the arrays in the same structure."""
+
+    def __init__(self, device, dtypes: Dict[str, torch.dtype]):
+        self.device, self.dtypes = device, dtypes
+        self.host: Dict[str, torch.Tensor] = {}
+        self.dev: Dict[str, torch.Tensor] = {}
+        self.done: Optional[torch.cuda.Event] = None
+
+    def put(self, groups: Dict[str, List[np.ndarray]], stream) -> Dict[str, List[torch.Tensor]]:
+        if self.done is not None:
+            self.done.synchronize()
+        out: Dict[str, List[torch.Tensor]] = {}
+        with torch.cuda.stream(stream):
+            for kind, arrays in groups.items():
+                total = sum(int(x.size) for x in arrays)
+                cap = self.host[kind].numel() if kind in self.host else 0
+                if total > cap:
+                    # Earlier scatters on `stream` may still read the old device buffer.
+                    stream.synchronize()
+                    cap = max(total, 2 * cap, 1 << 16)
+                    self.host[kind] = torch.empty(cap, dtype=self.dtypes[kind], pin_memory=True)
+                    self.dev[kind] = torch.empty(cap, dtype=self.dtypes[kind], device=self.device)
+                hbuf, views, off = self.host[kind].numpy(), [], 0
+                for x in arrays:
+                    hbuf[off:off + x.size] = x.reshape(-1)
+                    views.append((off, x.size))
+                    off += x.size
+                d = self.dev[kind]
+                d[:total].copy_(self.host[kind][:total], non_blocking=True)
+                out[kind] = [d[o:o + sz] for o, sz in views]
+            self.done = torch.cuda.Event()
+            self.done.record(stream)
+        return out
+
+
 @dataclass
 class SimRequest:
     """One simulation (batch size 1), with `integrate`'s arguments."""
@@ -102,6 +143,7 @@ class SimHandle:
         self.id = rid
         self.request = request
         self.t_submit = time.perf_counter()
+        self.t_prepared: Optional[float] = None
         self.t_admit: Optional[float] = None
         self.t_done: Optional[float] = None
         self.result: Optional[torch.Tensor] = None
@@ -119,7 +161,42 @@ class SimHandle:
 
 class _Prepared:
     """A request prepared on the host: schedule, node tables (channel parameters with the
-    request's values merged in), initial state and stimulus, as CPU tensors."""
+    request's values merged in), initial state and stimulus. Every integer array lies in
+    one int32 array (`ints`) and every floating array in one float64 array (`reals`),
+    back to back, so that a prepared request crosses a process boundary as two buffers;
+    the arrays are read as named views (`layout`: name -> (kind, offset, shape))."""
+
+    _INTS = ("node_i", "seg", "seg_level_ptr", "seg_child", "child_idx", "rec_node", "stim_col")
+    _REALS = ("node_f", "node_p", "v0", "gate_m", "gate_h", "gate_n", "stim_scale", "stim")
+
+    def pack(self, arrays: Dict[str, Any]) -> None:
+        layout, parts, off = {}, {"i": [], "r": []}, {"i": 0, "r": 0}
+        for name in self._INTS + self._REALS:
+            x = arrays.get(name)
+            if x is None:
+                continue
+            kind = "i" if name in self._INTS else "r"
+            x = _np(x, np.int32 if kind == "i" else np.float64)
+            layout[name] = (kind, off[kind], x.shape)
+            parts[kind].append(x.reshape(-1))
+            off[kind] += x.size
+        self.layout = layout
+        self.ints = np.concatenate(parts["i"]) if parts["i"] else np.zeros(0, np.int32)
+        self.reals = np.concatenate(parts["r"]) if parts["r"] else np.zeros(0, np.float64)
+
+    def __getattr__(self, name):
+        lay = self.__dict__.get("layout")
+        if lay is None or name.startswith("__"):
+            raise AttributeError(name)
+        if name == "gates0":
+            return [getattr(self, k) for k in ("gate_m", "gate_h", "gate_n")]
+        if name not in lay:
+            if name in self._INTS + self._REALS:
+                return None
+            raise AttributeError(name)
+        kind, off, shape = lay[name]
+        buf = self.ints if kind == "i" else self.reals
+        return buf[off:off + int(np.prod(shape))].reshape(shape)
 
 
 def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_states, dt: float,
@@ -144,9 +221,6 @@ def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_state
     st = P.st
     q = _Prepared()
     q.n, q.L, q.T, q.n_rec, q.sched = st.n, st.L, P.T, st.n_rec, st.sched
-    q.node_i, q.seg, q.seg_level_ptr, q.seg_child = st.node_i, st.seg, st.seg_level_ptr, st.seg_child
-    q.child_idx, q.rec_node, q.stim_col = st.child_idx, st.rec_node, st.stim_col
-    q.node_f = P.node_f
     node_p = P.node_p.clone()
     if P.bmode:
         bp = P.bparam_for(1)                       # (1, 4) or (1, n, 1, 4)
@@ -155,12 +229,40 @@ def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_state
             val = bp[0, c] if P.bmode == 1 else bp.reshape(st.n, 4)[:, c]
             mask = (flags & (2 if j < 6 else 4)) != 0
             node_p[:, j] = torch.where(mask, val.to(node_p.dtype) * torch.ones_like(node_p[:, j]), node_p[:, j])
-    q.node_p = node_p
-    q.v0 = P.v0[0].double()
-    q.gates0 = [g[0] for g in P.gates0]
-    q.stim = None if P.stim is None else P.stim[:, :, 0].contiguous()
-    q.stim_scale = P.stim_scale
+    nseg = int(st.seg.shape[0]) if st.n else 0
+    arrays = {"node_i": st.node_i, "seg": st.seg, "seg_level_ptr": st.seg_level_ptr[:st.L + 2],
+              "seg_child": st.seg_child, "child_idx": st.child_idx, "rec_node": st.rec_node,
+              "node_f": P.node_f, "node_p": node_p, "v0": P.v0[0],
+              "gate_m": P.gates0[0][0], "gate_h": P.gates0[1][0], "gate_n": P.gates0[2][0]}
+    if nseg == 0:
+        arrays["seg"] = np.zeros((0, 4), np.int32)
+    if P.stim is not None:
+        stim = P.stim[:, :, 0]
+        if stim.shape[1] > len(P.stim_scale):
+            raise ValueError("more stimulus columns than stimulus entries")
+        arrays.update(stim=stim, stim_scale=P.stim_scale, stim_col=st.stim_col[:len(P.stim_scale)])
+    q.pack(arrays)
     return q
+
+
+def _request_cost(r: SimRequest, dt: float) -> float:
+    """Compartment-steps of a request (its duration from `t_max`, else its stimulus)."""
+    if r.t_max is not None:
+        T = int(r.t_max // dt + 1)
+    elif r.data_stimuli is not None and len(r.data_stimuli[1]):
+        T = int(np.asarray(r.data_stimuli[1][0].shape)[-1])
+    else:
+        T = 1
+    return float(r.module._n_comp) * T
+
+
+def _np(x, dtype=None):
+    """A contiguous numpy copy that owns its memory (CPU tensors and arrays alike: a
+    view of a tensor's storage would dangle once torch moves that storage, as pickling a
+    module with cached tensors into a worker process does)."""
+    if isinstance(x, torch.Tensor):
+        x = x.detach().cpu().numpy()
+    return np.array(x, dtype=dtype, order="C", copy=True)
 
 
 def _to_numpy(x):
@@ -188,9 +290,7 @@ def _to_torch(x):
 def _prepare_in_worker(args):
     """Worker-process side of `_prepare_payload`: numpy in, numpy out."""
     torch.set_num_threads(1)
-    q = _prepare_payload(*_to_torch(args))
-    q.__dict__ = _to_numpy(q.__dict__)
-    return q
+    return _prepare_payload(*_to_torch(args))      # plain numpy arrays already
 
 
 class _Live:
@@ -199,10 +299,11 @@ class _Live:
     def __init__(self, handle: SimHandle, P: _Prepared):
         self.handle, self.P = handle, P
         self.n, self.T, self.n_rec = P.n, P.T, P.n_rec
-        self.nseg = int(P.seg.shape[0])
-        self.ncs = int(P.seg_child.shape[0]) if self.nseg else 0
-        self.nchild = int(P.child_idx.shape[0])
-        self.n_stim = 0 if P.stim is None else int(P.stim_scale.shape[0])
+        lay = P.layout
+        self.nseg = int(lay["seg"][2][0])
+        self.ncs = int(lay["seg_child"][2][0]) if self.nseg else 0
+        self.nchild = int(lay["child_idx"][2][0])
+        self.n_stim = int(lay["stim_scale"][2][0]) if "stim" in lay else 0
         self.aux_size = 4 * self.nseg + (P.L + 2) + self.ncs + self.nchild + max(self.n_rec, 1)
 
 
@@ -215,13 +316,18 @@ class SimServer:
     are started with "spawn", so a script that creates the server does its work under
     `if __name__ == "__main__":`); otherwise `prep_threads` > 0 prepares requests in that
     many background threads (0: inside `submit`); with more than one thread, requests
-    must not share a module object (its schedule cache is built on first use). `close()`
-    ends the workers."""
+    must not share a module object (its schedule cache is built on first use).
+    `prep_order` ("fifo" or "longest_first") is the order in which submitted requests
+    are handed to the preparation workers. With worker processes the interpreter's switch
+    interval is set to `switch_interval` (None: left alone) while the server is open, so
+    that the pool's hand-off threads keep the workers busy. `close()` ends the workers
+    and restores the switch interval."""
 
     def __init__(self, device=None, dtype=torch.float32, delta_t: float = 0.025,
                  hbm_budget: int = 2 << 30, max_active: int = 1024, chunk_steps: int = 200,
                  max_steps: int = 4001, max_stim_entries: int = 4096, autotune: bool = False,
-                 thread_order=(256, 512, 1024), prep_threads: int = 1, prep_processes: int = 0):
+                 thread_order=(256, 512, 1024), prep_threads: int = 1, prep_processes: int = 0,
+                 prep_order: str = "fifo", switch_interval: Optional[float] = 1e-3):
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         _require_gpu(device)
         self.device, self.user_dtype, self.dt = device, dtype, float(delta_t)
@@ -258,9 +364,20 @@ class SimServer:
         self.nodes_a, self.aux_a = _Arena(self.node_cap), _Arena(self.aux.numel())
         self.rec_a, self.stim_a = _Arena(self.rec_cap), _Arena(max_stim_entries)
         self.slots = list(range(max_active))[::-1]
+        self._s_run = np.zeros(max_active, dtype=bool)
+        self._s_t0, self._s_T, self._s_node_off, self._s_rec_off, self._s_nrec1, self._s_n, self._s_sched = (
+            np.zeros(max_active, dtype=np.int64) for _ in range(7))
+        self._s_live: List[Optional[_Live]] = [None] * max_active
+        self.max_inflight = 3
         self.queue: List[SimHandle] = []
         self.preparing: List = []     # (handle, future) in submission order
         self._procs = prep_processes > 0
+        self._switch = None
+        if self._procs and switch_interval is not None:
+            # The pool's hand-off threads need the interpreter often: with the default 5 ms
+            # switch interval a busy serving loop starves them and the workers sit idle.
+            self._switch = sys.getswitchinterval()
+            sys.setswitchinterval(switch_interval)
         if self._procs:
             # Worker processes started fresh ("spawn": nothing of this process's GPU
             # context is inherited); they import jaxhip and never touch the GPU.
@@ -268,6 +385,13 @@ class SimServer:
                                                 mp_context=multiprocessing.get_context("spawn"))
         else:
             self._pool = cf.ThreadPoolExecutor(max_workers=prep_threads) if prep_threads > 0 else None
+        if prep_order not in ("fifo", "longest_first"):
+            raise ValueError(f"prep_order must be 'fifo' or 'longest_first', not {prep_order!r}")
+        self.prep_order = prep_order
+        # Submitted requests not yet handed to a preparation worker: at most a few per
+        # worker are queued in the pool, the rest wait here, so that `prep_order` decides.
+        self.waiting: List = []       # heap of (priority, id, handle, args)
+        self._prep_depth = 2 * max(prep_processes, prep_threads, 1)
         self.live: Dict[int, _Live] = {}
         self.inflight: List = []      # (event, [live finishing], pinned progs)
         self.handles: List[SimHandle] = []
@@ -275,6 +399,8 @@ class SimServer:
         self._cfg_cache: Dict[int, LaunchConfig] = {}
         self.stream = torch.cuda.current_stream(dev)
         self.up_stream = torch.cuda.Stream(dev)
+        self._stage = _Staging(dev, {"i64": torch.int64, "i32": torch.int32, "l": self.ldtype,
+                                     "g": self.gdtype, "d": torch.float64})
         self.launches = 0
         self.steps_run = 0
 
@@ -292,15 +418,25 @@ class SimServer:
         if self._pool is None:
             self._accept(h, _prepare_payload(*args))
             self.queue.append(h)
-        elif self._procs:
-            self.preparing.append((h, self._pool.submit(_prepare_in_worker, _to_numpy(args))))
         else:
-            self.preparing.append((h, self._pool.submit(_prepare_payload, *args)))
+            key = -_request_cost(r, self.dt) if self.prep_order == "longest_first" else 0.0
+            heapq.heappush(self.waiting, (key, h.id, h, args))
+            self._dispatch()
         return h
 
+    def _dispatch(self) -> None:
+        """Hands waiting requests to the preparation pool while it holds fewer than a few
+        per worker: in submission order ("fifo") or the most compartment-steps first
+        ("longest_first": a burst's long simulations start first, which shortens the
+        time to finish all of them)."""
+        while self.waiting and len(self.preparing) < self._prep_depth:
+            _, _, h, args = heapq.heappop(self.waiting)
+            if self._procs:
+                self.preparing.append((h, self._pool.submit(_prepare_in_worker, _to_numpy(args))))
+            else:
+                self.preparing.append((h, self._pool.submit(_prepare_payload, *args)))
+
     def _accept(self, h: SimHandle, P: _Prepared) -> None:
-        if self._procs:
-            P.__dict__ = _to_torch(P.__dict__)
         L = _Live(h, P)
         if (L.n > self.node_cap or L.aux_size + 4 > self.aux.numel()
                 or (L.T + 1) * max(L.n_rec, 1) > self.rec_cap or L.n_stim > self.stim_col.numel()):
@@ -309,36 +445,48 @@ class SimServer:
             raise ValueError(f"the request's tree ({L.n} nodes) does not fit a workgroup's LDS")
         h.T = P.T
         h._P = P
+        h.t_prepared = time.perf_counter()
 
-    def _collect_prepared(self, wait: bool = False) -> None:
-        """Moves prepared requests (in submission order) to the admission queue; a failed
-        preparation raises here. `wait`: block for the oldest one if none is ready."""
-        while self.preparing:
+    def _collect_prepared(self, wait: bool = False, limit: Optional[int] = None) -> None:
+        """Moves prepared requests (in the order they were handed to the pool, at most
+        `limit`) to the admission queue and refills the pool; a failed preparation raises
+        here. `wait`: block for the oldest one if none is ready."""
+        done = 0
+        while self.preparing and (limit is None or done < limit):
+            done += 1
             h, fut = self.preparing[0]
             if not fut.done():
                 if not wait:
-                    return
+                    break
                 fut.result()
             self._accept(h, fut.result())     # re-raises a preparation error
             self.preparing.pop(0)
             self.queue.append(h)
             wait = False
+        self._dispatch()
 
     def close(self) -> None:
-        """Ends the preparation workers (threads or processes)."""
+        """Ends the preparation workers (threads or processes) and restores the
+        interpreter's switch interval."""
         if self._pool is not None:
             self._pool.shutdown(wait=True)
             self._pool = None
+        if self._switch is not None:
+            sys.setswitchinterval(self._switch)
+            self._switch = None
 
     def outstanding(self) -> int:
-        """Requests not finished: being prepared, queued for admission or running."""
-        return len(self.preparing) + len(self.queue) + len(self.live)
+        """Requests not finished: waiting for or in preparation, queued for admission or
+        running."""
+        return len(self.waiting) + len(self.preparing) + len(self.queue) + len(self.live)
 
-    def _try_admit(self, h: SimHandle) -> bool:
+    def _try_admit(self, h: SimHandle) -> Optional[_Live]:
+        """Places an admitted request in the arenas (its data is uploaded with the other
+        requests admitted in the same poll, `_upload_batch`)."""
         P = h._P
         L = _Live(h, P)
         if not self.slots:
-            return False
+            return None
         allocs = []
         for arena, size, align in ((self.nodes_a, L.n, 1), (self.aux_a, L.aux_size, 4),
                                    (self.rec_a, (L.T + 1) * max(L.n_rec, 1), 1),
@@ -347,67 +495,109 @@ class SimServer:
             if off is None:
                 for a, o, sz in allocs:
                     a.free(o, sz)
-                return False
+                return None
             allocs.append((arena, off, size))
         L.allocs = allocs
         L.node_off, L.aux_off, L.rec_off, L.stim_off = (o for _, o, _ in allocs)
-        L.slot = self.slots.pop()
-        self._upload(L)
+        L.slot = sl = self.slots.pop()
         h.t_admit = time.perf_counter()
         self.live[h.id] = L
-        return True
+        # Per-slot launch fields (a chunk's program table is built from them vectorized).
+        self._s_run[sl] = L.T > 0
+        self._s_t0[sl] = 0
+        self._s_T[sl] = L.T
+        self._s_node_off[sl] = L.node_off
+        self._s_rec_off[sl] = L.rec_off
+        self._s_nrec1[sl] = max(L.n_rec, 1)
+        self._s_n[sl] = L.n
+        self._s_sched[sl] = P.sched
+        self._s_live[sl] = L
+        return L
 
-    def _upload(self, L: _Live) -> None:
-        """Copies the request's tables and initial state into its arena places: pinned
-        host buffers, asynchronous copies on the upload stream (the compute stream waits
-        for it before the next chunk)."""
-        P = L.P
-        n, no = L.n, L.node_off
-        pins = []
-
-        def put(dst, src):
-            src = src.contiguous()
-            if src.device.type == "cpu":
-                src = src.pin_memory()
-                pins.append(src)
-            dst.copy_(src, non_blocking=True)
-
-        ni = P.node_i.clone()
-        ni[:, 4:6] += L.stim_off
-        node_p = P.node_p
-        a = L.aux_off
-        sb = a // 4
-        parts = [P.seg.reshape(-1), P.seg_level_ptr[:P.L + 2], P.seg_child[:L.ncs], P.child_idx,
-                 P.rec_node[:max(L.n_rec, 1)]]
-        offs = np.cumsum([a] + [int(x.numel()) for x in parts])
-        lb, scb, cb, rb = (int(x) for x in offs[1:5])
-        row = torch.tensor([n, P.L, no, lb, cb, L.n_rec, rb, 0, 0, 0, 0, 0, sb, scb, 0, 0],
-                           dtype=torch.int32)
-        with torch.cuda.stream(self.up_stream):
-            put(self.node_i[no:no + n], ni)
-            put(self.node_f[no:no + n], P.node_f.to(self.ldtype))
-            put(self.node_p[no:no + n], node_p.to(self.gdtype))
-            put(self.v[no:no + n], P.v0)
-            for buf, g in zip((self.gm, self.gh, self.gn), P.gates0):
-                put(buf[no:no + n], g.to(self.gdtype))
-            put(self.aux[a:int(offs[-1])], torch.cat([x.to(torch.int32) for x in parts]))
-            put(self.topos[L.slot], row)
+    def _upload_batch(self, lives: List[_Live]) -> None:
+        """Copies the tables and initial states of the requests admitted in one poll into
+        their arena places: every array is packed on the host into one pinned staging
+        buffer per element type (reused from poll to poll), copied to the device in one
+        transfer per type and scattered to the arena places by index (a handful of
+        operations per poll, however many requests); all on the upload stream, which the
+        compute stream waits for before the next chunk."""
+        if not lives:
+            return
+        node_idx, node_i, node_f, node_p, v0, gates = [], [], [], [], [], ([], [], [])
+        aux_idx, aux_val, topo_rows = [], [], []
+        ent_idx, ent_col, ent_scale, stim_idx, stim_val = [], [], [], [], []
+        ncols = self.stim.shape[1]
+        for L in lives:
+            P, n, no = L.P, L.n, L.node_off
+            lay = P.layout
+            node_idx.append(np.arange(no, no + n, dtype=np.int64))
+            ni = P.node_i.copy()
+            ni[:, 4:6] += L.stim_off
+            node_i.append(ni)
+            node_f.append(P.node_f)
+            node_p.append(P.node_p)
+            v0.append(P.v0)
+            for lst, k in zip(gates, ("gate_m", "gate_h", "gate_n")):
+                lst.append(getattr(P, k))
+            # Schedule tables, child lists and recorded nodes lie back to back in `ints`.
+            a = L.aux_off
+            s0 = lay["seg"][1]
+            aux_idx.append(np.arange(a, a + L.aux_size, dtype=np.int64))
+            aux_val.append(P.ints[s0:s0 + L.aux_size])
+            lb = a + 4 * L.nseg
+            scb = lb + P.L + 2
+            cb = scb + L.ncs
+            rb = cb + L.nchild
+            topo_rows.append([n, P.L, no, lb, cb, L.n_rec, rb, 0, 0, 0, 0, 0, a // 4, scb, 0, 0])
             if L.n_stim:
                 e0 = L.stim_off
-                put(self.stim_col[e0:e0 + L.n_stim], P.stim_col[:L.n_stim] + e0)
-                put(self.stim_scale[e0:e0 + L.n_stim], P.stim_scale.to(self.gdtype))
-                put(self.stim[:P.T, e0:e0 + P.stim.shape[1], 0], P.stim.to(self.gdtype))
-        L.pins = pins
+                ent_idx.append(np.arange(e0, e0 + L.n_stim, dtype=np.int64))
+                ent_col.append(P.stim_col + e0)
+                ent_scale.append(P.stim_scale)
+                stim = P.stim
+                T, c = stim.shape
+                stim_idx.append((np.arange(T, dtype=np.int64)[:, None] * ncols
+                                 + np.arange(e0, e0 + c, dtype=np.int64)[None, :]).reshape(-1))
+                stim_val.append(stim.reshape(-1))
+        slots = np.array([L.slot for L in lives], dtype=np.int64)
+        cat = np.concatenate
+        nidx, aidx = cat(node_idx), cat(aux_idx)
+        i64 = [nidx, aidx, slots] + ([cat(ent_idx), cat(stim_idx)] if ent_idx else [])
+        i32 = [cat(node_i).reshape(-1), cat(aux_val), np.asarray(topo_rows, dtype=np.int32).reshape(-1)]
+        if ent_idx:
+            i32.append(cat(ent_col).astype(np.int32))
+        gvals = [cat(node_p).reshape(-1)] + [cat(g) for g in gates]
+        if ent_idx:
+            gvals += [cat(ent_scale), cat(stim_val)]
+        dev = self._stage.put({"i64": i64, "i32": i32, "l": [cat(node_f).reshape(-1)], "g": gvals,
+                               "d": [cat(v0)]}, self.up_stream)
+        with torch.cuda.stream(self.up_stream):
+            nix, aix, six = dev["i64"][:3]
+            self.node_i.index_copy_(0, nix, dev["i32"][0].view(-1, self.node_i.shape[1]))
+            self.node_f.index_copy_(0, nix, dev["l"][0].view(-1, self.node_f.shape[1]))
+            self.node_p.index_copy_(0, nix, dev["g"][0].view(-1, self.node_p.shape[1]))
+            self.v.index_copy_(0, nix, dev["d"][0])
+            for buf, x in zip((self.gm, self.gh, self.gn), dev["g"][1:4]):
+                buf.index_copy_(0, nix, x)
+            self.aux.index_copy_(0, aix, dev["i32"][1])
+            self.topos.index_copy_(0, six, dev["i32"][2].view(-1, self.topos.shape[1]))
+            if ent_idx:
+                eix, stx = dev["i64"][3:5]
+                self.stim_col.index_copy_(0, eix, dev["i32"][3])
+                self.stim_scale.index_copy_(0, eix, dev["g"][4])
+                self.stim.view(-1).index_copy_(0, stx, dev["g"][5])
 
     def _retire(self, L: _Live) -> None:
         h = L.handle
         r = self.rec[L.rec_off:L.rec_off + (L.T + 1) * max(L.n_rec, 1)]
         h.result = r.reshape(L.T + 1, max(L.n_rec, 1))[:, :L.n_rec].t().to(self.user_dtype).clone()
         h.t_done = time.perf_counter()
-        L.pins = None
         for arena, off, size in L.allocs:
             arena.free(off, size)
+        h.steps_done = L.T
         self.slots.append(L.slot)
+        self._s_run[L.slot] = False
+        self._s_live[L.slot] = None
         del self.live[h.id]
         h._P = None
 
@@ -468,17 +658,24 @@ class SimServer:
         return _lib.fn("jxhip_fused_forward", self.ldtype)
 
     def _launch_chunk(self) -> None:
-        lives = list(self.live.values())
-        progs = np.zeros((len(lives), _lib.JXHIP_PROG_COLS), dtype=np.int64)
-        finishing = []
-        for r, L in enumerate(lives):
-            t0 = L.handle.steps_done
-            t1 = min(t0 + self.chunk_steps, L.T)
-            progs[r] = [L.slot, 1, t0, t1, L.node_off, L.slot, L.rec_off + t0 * max(L.n_rec, 1), 0]
-            if t1 >= L.T:
-                finishing.append(L)
-        n_max = max(L.n for L in lives)
-        sched_max = max(L.P.sched for L in lives)
+        """One launch advancing every admitted simulation with steps left by up to
+        `chunk_steps` steps (program table built from the per-slot fields)."""
+        act = np.flatnonzero(self._s_run)
+        t0 = self._s_t0[act]
+        T = self._s_T[act]
+        t1 = np.minimum(t0 + self.chunk_steps, T)
+        progs = np.zeros((len(act), _lib.JXHIP_PROG_COLS), dtype=np.int64)
+        progs[:, 0] = act
+        progs[:, 1] = 1
+        progs[:, 2] = t0
+        progs[:, 3] = t1
+        progs[:, 4] = self._s_node_off[act]
+        progs[:, 5] = act
+        progs[:, 6] = self._s_rec_off[act] + t0 * self._s_nrec1[act]
+        fin = act[t1 >= T]
+        finishing = [self._s_live[sl] for sl in fin]
+        n_max = int(self._s_n[act].max())
+        sched_max = int(self._s_sched[act].max())
         # Shapes are cached per size class; a cached shape is used only if it still fits
         # (a larger tree of the same class may need the streaming kernel).
         key = n_max.bit_length()
@@ -489,18 +686,17 @@ class SimServer:
         self.stream.wait_stream(self.up_stream)
         host = torch.from_numpy(progs).pin_memory()
         dev = host.to(self.device, non_blocking=True)
-        a = self._args(dev, len(lives), cfg, n_max, sched_max)
+        a = self._args(dev, len(act), cfg, n_max, sched_max)
         _lib.check(self._kernel()(a, _lib.stream_ptr(self.device)), "fused_forward (served chunk)")
-        # Only a launched chunk advances its simulations.
-        for r, L in enumerate(lives):
-            L.handle.steps_done = int(progs[r, 3])
+        # Only a launched chunk advances its simulations; finishing ones leave the live
+        # set when their chunk has completed and are not launched again.
+        self._s_t0[act] = t1
+        self._s_run[fin] = False
         ev = torch.cuda.Event()
         ev.record(self.stream)
         self.inflight.append((ev, finishing, host, dev))
         self.launches += 1
-        self.steps_run += int((progs[:, 3] - progs[:, 2]).sum())
-        # Finishing simulations leave the live set only when their chunk has completed;
-        # they are not launched again (steps_done == T).
+        self.steps_run += int((t1 - t0).sum())
 
     def _tune(self, n_max, sched_max, progs) -> LaunchConfig:
         """Measure the candidate shapes on copies of the live state (a few steps)."""
@@ -527,31 +723,37 @@ class SimServer:
         return best
 
     # ---------------------------------------------------------------- driving
-    def poll(self, block: bool = False) -> int:
-        """Retire completed chunks, admit queued requests, launch the next chunk.
-        Returns the number of simulations still queued or running."""
+    def poll(self, block: bool = False, max_collect: int = 64) -> int:
+        """Retire completed chunks, keep up to `max_inflight` chunks queued on the GPU,
+        collect prepared requests (at most `max_collect` per call, so that launches stay
+        regular), admit and upload them, launch again. Returns the number of simulations
+        still waiting, in preparation, queued or running."""
         while self.inflight and (block or self.inflight[0][0].query()):
             ev, finishing, _, _ = self.inflight.pop(0)
             ev.synchronize()
             for L in finishing:
                 self._retire(L)
             block = False
+        # The GPU first: the simulations already admitted get their next chunk before the
+        # host work of new admissions.
+        self._maybe_launch()
         # With nothing running or queued, a blocking poll waits for a preparation.
-        self._collect_prepared(wait=block and not self.queue and not self.live)
-        pending = []
+        self._collect_prepared(wait=block and not self.queue and not self.live, limit=max_collect)
+        pending, admitted = [], []
         for h in self.queue:
-            if not self._try_admit(h):
+            L = self._try_admit(h)
+            if L is None:
                 pending.append(h)
+            else:
+                admitted.append(L)
         self.queue = pending
-        runnable = [L for L in self.live.values() if L.handle.steps_done < L.T]
-        if runnable and len(self.inflight) < 2:
-            saved = self.live
-            self.live = {k: L for k, L in saved.items() if L.handle.steps_done < L.T}
-            try:
-                self._launch_chunk()
-            finally:
-                self.live = saved
+        self._upload_batch(admitted)
+        self._maybe_launch()
         return self.outstanding()
+
+    def _maybe_launch(self) -> None:
+        if len(self.inflight) < self.max_inflight and self._s_run.any():
+            self._launch_chunk()
 
     def run_until_idle(self) -> None:
         while self.poll(block=bool(self.inflight) or bool(self.preparing)):

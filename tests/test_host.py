@@ -73,6 +73,93 @@ def test_library_exports_every_declared_symbol():
     assert _lib.fn("jxhip_version")() == 100
 
 
+def test_host_library_exports_every_declared_symbol():
+    """libjxhost.so (g++ only, no GPU runtime) exports what include/jxhip_host.h declares."""
+    header = (ROOT / "include" / "jxhip_host.h").read_text()
+    declared = set(re.findall(r"^int\s+(jxhost_\w+)\(", header, re.M))
+    assert declared == set(_lib._HOST_SIGS)
+    lib = ctypes.CDLL(str(_lib.HOST_LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    # No GPU runtime among its dependencies (preparation workers load it).
+    assert b"libamdhip64" not in _lib.HOST_LIB_PATH.read_bytes()
+
+
+def _segments_reference(parent):
+    """The segment order written out directly (test side): chains of single-child nodes
+    from every top, levels from the leaves, sorted by (level, -length, top)."""
+    n = len(parent)
+    kids = [[] for _ in range(n)]
+    for c in range(n):
+        if parent[c] >= 0:
+            kids[parent[c]].append(c)
+    tops = [t for t in range(n) if parent[t] < 0 or len(kids[parent[t]]) != 1]
+    chains = []
+    for t in tops:
+        ch = [t]
+        while len(kids[ch[-1]]) == 1:
+            ch.append(kids[ch[-1]][0])
+        chains.append(ch)
+    of = {c: q for q, ch in enumerate(chains) for c in ch}
+
+    def level(q):
+        b = chains[q][-1]
+        return 0 if len(kids[b]) < 2 else 1 + max(level(of[c]) for c in kids[b])
+    lv = [level(q) for q in range(len(chains))]
+    order = sorted(range(len(chains)), key=lambda q: (lv[q], -len(chains[q]), chains[q][0]))
+    korder = [c for q in order for c in chains[q]]
+    kidx = {c: i for i, c in enumerate(korder)}
+    seg, child = [], []
+    for q in order:
+        b = chains[q][-1]
+        tk = [kidx[c] for c in kids[b]] if len(kids[b]) >= 2 else []
+        seg.append((kidx[chains[q][0]], kidx[chains[q][0]] + len(chains[q]), len(child), len(child) + len(tk)))
+        child += tk
+    L = max(lv) if lv else 0
+    lptr = np.concatenate([[0], np.cumsum(np.bincount(np.array(lv, dtype=np.int64), minlength=L + 1))])
+    return np.array(korder), np.array(seg).reshape(-1, 4), lptr, np.array(child if child else [0]), L
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_native_segments_match_reference(seed):
+    """`jxhost_segments` (native) against the order written out in Python, on random
+    forests with chains, branchings and several roots."""
+    rng = np.random.default_rng(300 + seed)
+    n = int(rng.integers(1, 400))
+    parent = np.full(n, -1, dtype=np.int64)
+    for i in range(1, n):
+        if rng.random() > 0.03:
+            # Mostly chains (the parent is the previous node) with random branchings.
+            parent[i] = i - 1 if rng.random() < 0.7 else int(rng.integers(0, i))
+    sg = S.segments(parent)
+    korder, seg, lptr, child, L = _segments_reference(parent)
+    assert np.array_equal(sg["korder"], korder)
+    assert np.array_equal(sg["seg"], seg)
+    assert np.array_equal(sg["seg_level_ptr"], lptr)
+    assert np.array_equal(sg["seg_child"], child)
+    assert sg["L"] == L and sg["nseg"] == len(seg)
+    # Invariants: a permutation; every segment a parent chain; child tops belong to
+    # segments of a lower level.
+    assert sorted(sg["korder"].tolist()) == list(range(n))
+    for b, e, cb, ce in sg["seg"]:
+        nodes = sg["korder"][b:e]
+        assert all(parent[nodes[i + 1]] == nodes[i] for i in range(len(nodes) - 1))
+        for t in sg["seg_child"][cb:ce]:
+            assert parent[sg["korder"][t]] == nodes[-1]
+
+
+def test_native_schedule_rejects_bad_input():
+    with pytest.raises(ValueError, match="forest"):
+        S.segments(np.array([-1, 2, 0]))
+    tab = S.cell_tables([-1, 0], [2, 2])
+    bad = dict(tab)
+    keep = ~((tab["type"] == 0) & (tab["source"] == 0))
+    bad = {**tab, "source": tab["source"][keep], "sink": tab["sink"][keep], "type": tab["type"][keep],
+           "ordered": tab["ordered"][keep]}
+    with pytest.raises(ValueError, match="connected"):
+        S.dhs_indexer(bad)
+
+
 def test_struct_layouts_match_header():
     """ctypes mirrors of the argument blocks have the C field order of jxhip.h."""
     header = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "jxhip.h").read_text(), flags=re.S)
@@ -175,7 +262,7 @@ def test_serve_preparation_in_worker_processes():
     import concurrent.futures as cf
     import multiprocessing
 
-    from jaxhip.serve import _prepare_in_worker, _prepare_payload, _to_numpy, _to_torch
+    from jaxhip.serve import _prepare_in_worker, _prepare_payload, _to_numpy
     from models import c2_request, c2_specs
 
     cpu = torch.device("cpu")
@@ -186,13 +273,13 @@ def test_serve_preparation_in_worker_processes():
     with cf.ProcessPoolExecutor(max_workers=2, mp_context=multiprocessing.get_context("spawn")) as pool:
         remote = list(pool.map(_prepare_in_worker, [_to_numpy(a) for a in args]))
     for p, q in zip(local, remote):
-        q.__dict__ = _to_torch(q.__dict__)
         assert (p.n, p.L, p.T, p.n_rec, p.sched) == (q.n, q.L, q.T, q.n_rec, q.sched)
         for name in ("node_i", "seg", "seg_level_ptr", "seg_child", "child_idx", "rec_node", "stim_col",
                      "node_f", "node_p", "v0", "stim", "stim_scale"):
-            assert torch.equal(getattr(p, name), getattr(q, name)), name
+            x, y = getattr(p, name), getattr(q, name)
+            assert isinstance(x, np.ndarray) and x.dtype == y.dtype and np.array_equal(x, y), name
         for g, h in zip(p.gates0, q.gates0):
-            assert torch.equal(g, h)
+            assert np.array_equal(g, h)
 
 
 def test_tree_solve_operator_fake_rule():
