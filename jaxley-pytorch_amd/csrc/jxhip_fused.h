@@ -97,6 +97,8 @@ struct Fused {
   int64_t group_nodes;
   double* vx;
   int64_t vx_stride;
+  double* traj_v;      // voltage trajectory of the checkpointed forward, or null
+  int64_t traj_steps;
 };
 
 template <typename G, typename L>
@@ -158,6 +160,8 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.group_nodes = a.group_nodes;
   f.vx = static_cast<double*>(a.vx);
   f.vx_stride = a.vx_stride;
+  f.traj_v = static_cast<double*>(a.traj_v);
+  f.traj_steps = a.traj_steps;
   return f;
 }
 
@@ -1089,6 +1093,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     grid_barrier();
   }
   const int64_t nsteps = pg.t1 - pg.t0;
+  const int64_t tbase = pg.soff * a.traj_steps;  // the program's voltage trajectory
   for (int64_t step = 0; step < nsteps; ++step) {
     const bool pend = step > 0;  // the last solve's increments wait in LDS
     const bool ck = kCkpt && (step % a.ckpt_every == 0);
@@ -1121,6 +1126,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
               a.ckpt_h[coff + i] = cur.h;
               a.ckpt_n[coff + i] = cur.n;
             }
+            if (!kSyn && a.traj_v != nullptr) a.traj_v[tbase + step * nS + i] = cur.v;
           }
           vg[i] = cur.v;
           axg[i] = cur.ax;
@@ -1152,6 +1158,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
         if (last || (c.r0.w & kExport)) {
           vg[i] += dv;
           if (kSyn && a.coop) a.vx[((step + 1) & 1) * a.vx_stride + pg.soff + i] = vg[i];
+          if (kCkpt && !kSyn && last && a.traj_v != nullptr) a.traj_v[tbase + nsteps * nS + i] = vg[i];
         }
         if (!last) Ld.ds[i] = static_cast<L>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
         c = cn;
@@ -1470,13 +1477,46 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
   G* wm = g.wm + wbase;
   G* wh = g.wh + wbase;
   G* wn = g.wn + wbase;
+  // With the forward's voltage trajectory (cells only) the segment's voltages are read
+  // from it and only the gates are recomputed: an item-local recurrence, no tree solve.
+  const bool traj = !kSyn && a.traj_v != nullptr;
+  const double* tv = traj ? a.traj_v + pg.soff * a.traj_steps : nullptr;
   for (int64_t c = C - 1; c >= 0; --c) {
     const int64_t ts = c * K;
     const int64_t te = ts + K < nsteps ? ts + K : nsteps;
+    const int64_t coff = pg.soff + c * a.ckpt_stride;
+    if (traj) {
+      // Gates of every step of the segment from the checkpoint's, with the forward's
+      // arithmetic (rates at the step's starting voltage, exponential Euler).
+#pragma unroll 1
+      for (int i = tid; i < nS; i += nthr) {
+        const int k = i >> lS, s = i & (S - 1);
+        G m = a.ckpt_m[coff + i], h = a.ckpt_h[coff + i], n = a.ckpt_n[coff + i];
+        wm[i] = m;
+        wh[i] = h;
+        wn[i] = n;
+        const int32_t flags = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiFlags];
+        if (!((flags & kComp) && (flags & kHH)) || s >= pg.nsim) continue;
+        double vn = tv[ts * nS + i];
+#pragma unroll 1
+        for (int64_t j = 0; j < te - ts; ++j) {
+          const G vv = static_cast<G>(vn);
+          vn = tv[(ts + j + 1) * nS + i];  // the next step's voltage in flight
+          G am, bm, ah, bh, an, bn;
+          hh_rates_step(vv, am, bm, ah, bh, an, bn);
+          m = gate_step(m, dt, am, bm);
+          h = gate_step(h, dt, ah, bh);
+          n = gate_step(n, dt, an, bn);
+          wm[(j + 1) * rowS + i] = m;
+          wh[(j + 1) * rowS + i] = h;
+          wn[(j + 1) * rowS + i] = n;
+        }
+      }
+      __syncthreads();
+    }
     // Recompute the segment from its checkpoint into work[0 .. te - ts]; the voltage
     // copy of step j is work_v[j] (program layout, as the forward's HBM copy).
-    const int64_t coff = pg.soff + c * a.ckpt_stride;
-    for (int i = tid; i < nS; i += nthr) {
+    for (int i = tid; i < (traj ? 0 : nS); i += nthr) {
       wv[i] = a.ckpt_v[coff + i];
       wm[i] = a.ckpt_m[coff + i];
       wh[i] = a.ckpt_h[coff + i];
@@ -1492,7 +1532,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     // Split network: every part's checkpoint loaded before any part reads presynaptic
     // voltages from it (the same after every recomputed step).
     if (kSyn && a.coop) grid_barrier();
-    for (int64_t j = 0; j < te - ts; ++j) {
+    for (int64_t j = 0; j < (traj ? 0 : te - ts); ++j) {
       if constexpr (kSyn) {
         // The synapse step of the recomputed forward (states into work_s[j + 1]).
         for (int i2 = tid; i2 < ntS; i2 += nthr) {
@@ -1584,8 +1624,8 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     }
     for (int64_t j = te - ts - 1; j >= 0; --j) {
       tree_solve<G, L, true, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
-      const double* v0 = wv + j * rowS;
-      const double* v1 = wv + (j + 1) * rowS;
+      const double* v0 = traj ? tv + (ts + j) * nS : wv + j * rowS;
+      const double* v1 = traj ? tv + (ts + j + 1) * nS : wv + (j + 1) * rowS;
       const G* m0 = wm + j * rowS;
       const G* h0 = wh + j * rowS;
       const G* n0 = wn + j * rowS;
@@ -1929,6 +1969,9 @@ int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
   const Fused<G, L> f = unpack<G, L>(a);
   const bool syn = a.syn_model >= 0, ckpt = a.ckpt_v != nullptr;
   if (ckpt && a.ckpt_every <= 0) return hipErrorInvalidValue;
+  // A voltage trajectory is kept by the checkpointed streaming kernel of cells only.
+  if (a.traj_v != nullptr && (!ckpt || syn || a.items_per_thread != 0 || a.traj_steps < 1))
+    return hipErrorInvalidValue;
   if (syn && ckpt) return launch_forward_feat<G, L, true, true>(a, f, stream);
   if (syn) return launch_forward_feat<G, L, true, false>(a, f, stream);
   if (ckpt) return launch_forward_feat<G, L, false, true>(a, f, stream);
@@ -1959,6 +2002,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (a.items_per_thread > 0 && a.max_items > static_cast<int64_t>(a.items_per_thread) * a.threads)
     return hipErrorInvalidValue;
   if (a.items_per_thread == 0 && adj->lam_v == nullptr) return hipErrorInvalidValue;
+  if (a.traj_v != nullptr && (syn || a.traj_steps < 1)) return hipErrorInvalidValue;
   const Fused<G, L> f = unpack<G, L>(a);
   Adj<G> g;
   g.g_rec = static_cast<const G*>(adj->g_rec);

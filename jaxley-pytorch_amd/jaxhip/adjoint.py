@@ -1,6 +1,10 @@
 """Gradients of the fused simulation: forward with checkpoints every K steps and the
 hand-written reverse pass (`jxhip_fused_adjoint_*`), wrapped in an autograd Function.
 
+With a voltage trajectory (`ForwardLaunch.traj`: the streaming forward of a cell keeps
+every step's voltages when they fit the memory budget) the reverse pass recomputes only
+the gates between checkpoints and reads the voltages from it.
+
 Semantics: the gradient of `integrate`'s recordings with respect to the HH / Leak
 parameters and the initial voltages, as autograd through the reference's step would
 give (jaxley_pytorch/integrate.py:624-662 with `checkpoint_lengths=[T / K, K]`,
@@ -48,7 +52,8 @@ class _FusedIntegrate(torch.autograd.Function):
         gr = run.recordings_to_prog(g_rec.to(dtype))
         # Recomputed trajectories per group of simulations (a split network's parts side
         # by side).
-        work = [torch.empty((P, K + 1, n, S), dtype=torch.float64, device=dev)]
+        # (The voltages come from the forward's trajectory when it kept one.)
+        work = [torch.empty((P, K + 1, n, S) if run.traj is None else (1,), dtype=torch.float64, device=dev)]
         work += [torch.empty((P, K + 1, n, S), dtype=dtype, device=dev) for _ in range(3)]
         lam = [torch.zeros((P, n, S), dtype=dtype, device=dev) for _ in range(3)]
         g_v0 = torch.zeros((P, n, S), dtype=dtype, device=dev)
@@ -128,6 +133,7 @@ class _FusedIntegrate(torch.autograd.Function):
         gv = from_prog(g_v0, run.B) if ctx.needs_input_grad[1] else None
         # gS is shared by the simulations: its gradient is the sum over them.
         g_syn = from_prog(g_gs, run.B).sum(0) if g_gs is not None else None
+        run.traj = None
         holder.pop("run", None)
         holder.pop("syn_bufs", None)
         return None, gv, g_node_p, g_bparam, g_syn

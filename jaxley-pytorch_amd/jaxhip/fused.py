@@ -13,6 +13,7 @@ so a wavefront reading one level of nodes reads contiguous memory.
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Tuple
 
@@ -394,6 +395,9 @@ def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor
     (already / cm, per compartment edge) and the node capacitances."""
     dev = g_axial.device
     n = st.n
+    if dev.type == "cpu" and g_axial.ndim == 1 and not g_axial.requires_grad and not cm_node.requires_grad:
+        return torch.from_numpy(_node_float_table_np(st, g_axial.double().numpy(), cm_node.double().numpy(),
+                                                     dt)).to(dtype)
     gdt = (dt * g_axial.double())
     if len(gdt):
         lo = -gdt[torch.as_tensor(st.lo_edge, device=dev)]
@@ -415,6 +419,28 @@ def node_float_table(st: Structure, g_axial: torch.Tensor, cm_node: torch.Tensor
     cm = cm_node.double()[torch.as_tensor(st.mp, device=dev)]
     inv_cm = torch.where(torch.as_tensor(st.comp_of_k >= 0, device=dev), 1.0 / cm, torch.zeros_like(cm))
     return torch.stack([lo, up, dbase, inv_cm], 1).to(dtype).contiguous()
+
+
+def _node_float_table_np(st: Structure, g: np.ndarray, cm_node: np.ndarray, dt: float) -> np.ndarray:
+    """`node_float_table` on the host (numpy; the same operations in the same order)."""
+    n = st.n
+    gdt = dt * g
+    has_par = st.parent_k >= 0
+    if len(gdt):
+        lo = np.where(has_par, -gdt[st.lo_edge], 0.0)
+        up = np.where(has_par, -gdt[st.up_edge], 0.0)
+    else:
+        lo = up = np.zeros(n)
+    dbase = (st.comp_of_k >= 0).astype(np.float64)
+    if len(gdt):
+        inc = st.in_edges
+        for j in range(inc.shape[1]):
+            col = inc[:, j]
+            dbase = dbase + np.where(col >= 0, gdt[np.maximum(col, 0)], 0.0)
+    cm = cm_node[st.mp]
+    is_comp = st.comp_of_k >= 0
+    inv_cm = np.where(is_comp, 1.0 / np.where(is_comp, cm, 1.0), 0.0)
+    return np.ascontiguousarray(np.stack([lo, up, dbase, inv_cm], 1))
 
 
 @dataclass
@@ -560,6 +586,19 @@ def grad_config(n: int, B: int, esize: int, sched: int = 0, n_tgt: int = 0) -> L
     S = adjoint_config(n, B, esize, sched, n_tgt=n_tgt).S
     fits = lds_bytes(n, n_tgt, S, esize, sched) <= 160 * 1024
     return LaunchConfig(S, 1024, stream=True, gcoef=not fits)
+
+
+# Share of the free device memory the voltage trajectory of a checkpointed forward may
+# take (JAXHIP_TRAJ_FRACTION; 0 disables it: the reverse pass then recomputes the
+# voltages between checkpoints with tree solves).
+TRAJ_FRACTION = float(os.environ.get("JAXHIP_TRAJ_FRACTION", "0.5"))
+
+
+def keep_trajectory(nbytes: int, device) -> bool:
+    if TRAJ_FRACTION <= 0 or torch.device(device).type != "cuda":
+        return False
+    free, _ = torch.cuda.mem_get_info(device)
+    return nbytes <= TRAJ_FRACTION * free
 
 
 class ForwardLaunch:
@@ -708,6 +747,14 @@ class ForwardLaunch:
             a.ckpt_every = ckpt_every
             a.ckpt_stride = self.v.numel()
             a.ckpt_syn_stride = self.syn_s.numel() if self.syn_s is not None else 0
+        # The voltage of every step (for the reverse pass: no recomputed tree solves) when
+        # the streaming kernel runs a cell and the trajectory fits the memory budget.
+        self.traj = None
+        if ckpt_every > 0 and a.items_per_thread == 0 and st.syn is None and keep_trajectory(
+                (T + 1) * self.v.numel() * 8, dev):
+            self.traj = torch.empty((T + 1) * self.v.numel(), dtype=torch.float64, device=dev)
+            a.traj_v = self.traj.data_ptr()
+            a.traj_steps = T + 1
         self.args = a
 
     def kernel_name(self, base: str):

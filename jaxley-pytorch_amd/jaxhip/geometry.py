@@ -10,7 +10,8 @@ half-length loads of modules/compartment.py:70-78) and `compute_axial_conductanc
       sink's in-load for `ordered` edges, else its out-load;
   types 3, 4 (comp -> branchpoint): g = 1 / (r_l(src) R_a(src)) (not / area, / cm) with
       the load of the side the mirrored type 1 / 2 edge uses.
-Leading batch dimensions broadcast.
+Leading batch dimensions broadcast. Unbatched CPU inputs without autograd take a numpy
+form of the same arithmetic (request preparation on the host).
 """
 from __future__ import annotations
 
@@ -37,8 +38,41 @@ def comp_geometry(vals: Dict[str, torch.Tensor]) -> Dict[str, torch.Tensor]:
     return {"area": vals["area"], "rl_in": vals["resistive_load_in"], "rl_out": vals["resistive_load_out"]}
 
 
+def _host_values(*ts) -> bool:
+    """Unbatched CPU tensors without autograd (a served request's preparation): the
+    tables are computed with numpy, the same operations in the same order."""
+    return all(t.device.type == "cpu" and t.ndim == 1 and not t.requires_grad for t in ts)
+
+
+def _axial_conductances_np(tables, comp_node, n_nodes, rl_in, rl_out, area, Ra, cm) -> np.ndarray:
+    typ, src, snk, ordd = tables["type"], tables["source"], tables["sink"], tables["ordered"]
+    comp_of_node = -np.ones(n_nodes, dtype=np.int64)
+    comp_of_node[comp_node] = np.arange(len(comp_node))
+    out = np.zeros(len(typ), dtype=area.dtype)
+    rl = np.stack([rl_in, rl_out])
+    side = lambda o, c: rl[o, c] * Ra[c]
+    e0 = np.nonzero(typ == 0)[0]
+    if len(e0):
+        cs, ct = comp_of_node[src[e0]], comp_of_node[snk[e0]]
+        r = side(ordd[e0], cs) + side(1 - ordd[e0], ct)
+        out[e0] = 1.0 / r / area[ct] / cm[ct] * 1e7
+    e12 = np.nonzero((typ == 1) | (typ == 2))[0]
+    if len(e12):
+        ct = comp_of_node[snk[e12]]
+        out[e12] = 1.0 / side(1 - ordd[e12], ct) / area[ct] / cm[ct] * 1e7
+    e34 = np.nonzero((typ == 3) | (typ == 4))[0]
+    if len(e34):
+        cs = comp_of_node[src[e34]]
+        out[e34] = 1.0 / rl[1 - ordd[e12], cs] * (1.0 / Ra[cs])
+    return out
+
+
 def axial_conductances(tables: Dict, comp_node: np.ndarray, n_nodes: int, geo: Dict[str, torch.Tensor],
                        axial_resistivity: torch.Tensor, capacitance: torch.Tensor) -> torch.Tensor:
+    if _host_values(geo["rl_in"], geo["rl_out"], geo["area"], axial_resistivity, capacitance):
+        return torch.from_numpy(_axial_conductances_np(
+            tables, comp_node, n_nodes, geo["rl_in"].numpy(), geo["rl_out"].numpy(), geo["area"].numpy(),
+            axial_resistivity.numpy(), capacitance.numpy()))
     typ, src, snk, ordd = tables["type"], tables["source"], tables["sink"], tables["ordered"]
     dev = axial_resistivity.device
     comp_of_node = -np.ones(n_nodes, dtype=np.int64)

@@ -136,18 +136,27 @@ def dhs_indexer(tables: Dict, allowed_nodes_per_level: int = 32, root: int = 0) 
     node_order[:, 0] = inv[e_chi]
     node_order[:, 1] = inv[e_par]
     node_order[:, 2] = levels
-    grouped = group_levels(node_order)
     parent_lookup = -np.ones(n + 1, dtype=np.int64)
     parent_lookup[node_order[:, 0]] = node_order[:, 1]
-    return {
+    return _Indexer({
         "map_to_solve_order": mapping,
         "inv_map_to_solve_order": inv,
         "map_to_solve_order_lower": np.concatenate([[0], lu[: n - 1]]).astype(np.int64),
         "map_to_solve_order_upper": np.concatenate([[0], lu[n - 1:]]).astype(np.int64),
         "node_order": node_order,
-        "node_order_grouped": grouped,
         "parent_lookup": parent_lookup,
-    }
+    })
+
+
+class _Indexer(dict):
+    """A DHS indexer; its level grouping (`node_order_grouped`, the reference's solve
+    levels) is built when first read."""
+
+    def __missing__(self, key):
+        if key == "node_order_grouped":
+            self[key] = group_levels(self["node_order"])
+            return self[key]
+        raise KeyError(key)
 
 
 def group_levels(node_order: np.ndarray) -> List[np.ndarray]:
@@ -202,15 +211,14 @@ def network_indexer(cell_tabs: List[Dict], cell_inds: List[Dict], vectorize: boo
         "internal": np.concatenate(internal),
     }
     no_all = np.concatenate(node_order, axis=0)
-    indexer = {
+    indexer = _Indexer({
         "inv_map_to_solve_order": np.concatenate(inv),
         "map_to_solve_order": np.concatenate(mp),
         "map_to_solve_order_lower": np.minimum(np.concatenate(low), max_ind),
         "map_to_solve_order_upper": np.minimum(np.concatenate(up), max_ind),
         "node_order": no_all,
-        "node_order_grouped": group_levels(no_all),
         "parent_lookup": np.concatenate(plook + [np.asarray([-1])]),
-    }
+    })
     return tables, indexer
 
 

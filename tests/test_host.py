@@ -454,3 +454,32 @@ def test_split_parts_are_never_empty():
     st = build_structure(net, np.array([0]), np.zeros(0, np.int64), torch.device("cpu"), 7)
     assert 1 < st.parts <= 7 and (st.part_n > 0).all() and st.part_n.sum() == st.n
     assert np.array_equal(st.part_nb, np.concatenate([[0], np.cumsum(st.part_n)[:-1]]))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_host_tables_numpy_equal_torch(dtype):
+    """The numpy forms of the axial conductances and the node table (unbatched CPU inputs,
+    a served request's preparation) equal the tensor forms bit for bit."""
+    from jaxhip import fused as F
+    from jaxhip.geometry import axial_conductances
+    from jaxhip.workloads import c2_specs, tree_cell
+
+    sp = c2_specs(3, seed=5)[2]
+    cell = tree_cell(sp["nbranch"], sp["seed"])[0]
+    cell.branch(0).comp(0).record()
+    rng = np.random.default_rng(0)
+    nc = cell._n_comp
+    geo = {k: torch.tensor(rng.uniform(0.5, 2.0, nc), dtype=dtype) for k in ("area", "rl_in", "rl_out")}
+    ra = torch.tensor(rng.uniform(50, 5000, nc), dtype=dtype)
+    cm = torch.tensor(rng.uniform(0.5, 2.0, nc), dtype=dtype)
+    tables = cell._comp_edges
+    g_np = axial_conductances(tables, cell._comp_node, cell._n_nodes, geo, ra, cm)
+    g_t = axial_conductances(tables, cell._comp_node, cell._n_nodes, {k: v[None] for k, v in geo.items()},
+                             ra[None], cm[None])[0]
+    assert g_np.dtype == g_t.dtype and torch.equal(g_np, g_t)
+    st = F.build_structure(cell, np.array([0]), np.array([0]), torch.device("cpu"))
+    cm_node = torch.full((cell._n_nodes,), -1.0, dtype=dtype)
+    cm_node[torch.as_tensor(cell._comp_node)] = cm
+    a = F.node_float_table(st, g_np, cm_node, 0.025, dtype)
+    b = F.node_float_table(st, g_np[None].expand(1, -1)[0].clone().requires_grad_(True), cm_node, 0.025, dtype)
+    assert torch.equal(a, b.detach())
