@@ -35,7 +35,7 @@ import multiprocessing
 import sys
 import time
 from dataclasses import dataclass, field
-from typing import Any, Dict, List, Optional
+from typing import Any, Dict, List, Optional, Tuple
 
 import numpy as np
 import torch
@@ -245,15 +245,16 @@ def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_state
     return q
 
 
-def _request_cost(r: SimRequest, dt: float) -> float:
-    """Compartment-steps of a request (its duration from `t_max`, else its stimulus)."""
+def _request_length(r: SimRequest, dt: float) -> Tuple[int, int]:
+    """(time steps, compartments) of a request (its duration from `t_max`, else from its
+    stimulus)."""
     if r.t_max is not None:
         T = int(r.t_max // dt + 1)
     elif r.data_stimuli is not None and len(r.data_stimuli[1]):
         T = int(np.asarray(r.data_stimuli[1][0].shape)[-1])
     else:
         T = 1
-    return float(r.module._n_comp) * T
+    return T, int(r.module._n_comp)
 
 
 def _np(x, dtype=None):
@@ -291,6 +292,35 @@ def _prepare_in_worker(args):
     """Worker-process side of `_prepare_payload`: numpy in, numpy out."""
     torch.set_num_threads(1)
     return _prepare_payload(*_to_torch(args))      # plain numpy arrays already
+
+
+def _prepare_batch(fn, batch):
+    """Several requests in one pool task: per request (True, payload) or (False, the
+    exception its preparation raised)."""
+    out = []
+    for args in batch:
+        try:
+            out.append((True, fn(*args)))
+        except Exception as e:  # noqa: BLE001 - re-raised by the server for that request
+            out.append((False, e))
+    return out
+
+
+def _prepare_batch_in_worker(batch):
+    return _prepare_batch(_prepare_in_worker, [(a,) for a in batch])
+
+
+class _Done:
+    """A finished pool task's remaining results (a future that is already done)."""
+
+    def __init__(self, results):
+        self._results = results
+
+    def done(self) -> bool:
+        return True
+
+    def result(self):
+        return self._results
 
 
 class _Live:
@@ -370,7 +400,7 @@ class SimServer:
         self._s_live: List[Optional[_Live]] = [None] * max_active
         self.max_inflight = 3
         self.queue: List[SimHandle] = []
-        self.preparing: List = []     # (handle, future) in submission order
+        self.preparing: List = []     # (handles, future of their payloads) in dispatch order
         self._procs = prep_processes > 0
         self._switch = None
         if self._procs and switch_interval is not None:
@@ -419,22 +449,37 @@ class SimServer:
             self._accept(h, _prepare_payload(*args))
             self.queue.append(h)
         else:
-            key = -_request_cost(r, self.dt) if self.prep_order == "longest_first" else 0.0
+            if self.prep_order == "longest_first":
+                T, nc = _request_length(r, self.dt)
+                key = (-T, -nc)
+            else:
+                key = (0, 0)
             heapq.heappush(self.waiting, (key, h.id, h, args))
             self._dispatch()
         return h
 
     def _dispatch(self) -> None:
         """Hands waiting requests to the preparation pool while it holds fewer than a few
-        per worker: in submission order ("fifo") or the most compartment-steps first
-        ("longest_first": a burst's long simulations start first, which shortens the
+        per worker: in submission order ("fifo") or the most time steps first, larger
+        trees first among equal durations ("longest_first": a simulation's steps run
+        chunk after chunk, so a burst's long simulations start first, which shortens the
         time to finish all of them)."""
         while self.waiting and len(self.preparing) < self._prep_depth:
-            _, _, h, args = heapq.heappop(self.waiting)
+            # With many requests waiting (a burst) small ones go to a worker together, up
+            # to 8 and about 2000 compartments per task (fewer hand-offs through the pool
+            # per request); a large tree goes alone, so that it starts early.
+            nb = max(1, min(8, len(self.waiting) // self._prep_depth))
+            items = [heapq.heappop(self.waiting)]
+            ncomp = items[0][2].request.module._n_comp
+            while len(items) < nb and self.waiting and ncomp + self.waiting[0][2].request.module._n_comp <= 2048:
+                items.append(heapq.heappop(self.waiting))
+                ncomp += items[-1][2].request.module._n_comp
+            hs = [h for _, _, h, _ in items]
             if self._procs:
-                self.preparing.append((h, self._pool.submit(_prepare_in_worker, _to_numpy(args))))
+                fut = self._pool.submit(_prepare_batch_in_worker, [_to_numpy(args) for _, _, _, args in items])
             else:
-                self.preparing.append((h, self._pool.submit(_prepare_payload, *args)))
+                fut = self._pool.submit(_prepare_batch, _prepare_payload, [args for _, _, _, args in items])
+            self.preparing.append((hs, fut))
 
     def _accept(self, h: SimHandle, P: _Prepared) -> None:
         L = _Live(h, P)
@@ -453,15 +498,26 @@ class SimServer:
         here. `wait`: block for the oldest one if none is ready."""
         done = 0
         while self.preparing and (limit is None or done < limit):
-            done += 1
-            h, fut = self.preparing[0]
+            hs, fut = self.preparing[0]
             if not fut.done():
                 if not wait:
                     break
                 fut.result()
-            self._accept(h, fut.result())     # re-raises a preparation error
             self.preparing.pop(0)
-            self.queue.append(h)
+            results = fut.result()
+            for j, (h, (ok, P)) in enumerate(zip(hs, results)):
+                try:
+                    if not ok:
+                        raise P
+                    self._accept(h, P)
+                except Exception:
+                    # The failed request is dropped; the rest of the task's requests go
+                    # back to the head of the line.
+                    if j + 1 < len(hs):
+                        self.preparing.insert(0, (hs[j + 1:], _Done(results[j + 1:])))
+                    raise
+                self.queue.append(h)
+                done += 1
             wait = False
         self._dispatch()
 
@@ -478,7 +534,8 @@ class SimServer:
     def outstanding(self) -> int:
         """Requests not finished: waiting for or in preparation, queued for admission or
         running."""
-        return len(self.waiting) + len(self.preparing) + len(self.queue) + len(self.live)
+        return (len(self.waiting) + sum(len(hs) for hs, _ in self.preparing) + len(self.queue)
+                + len(self.live))
 
     def _try_admit(self, h: SimHandle) -> Optional[_Live]:
         """Places an admitted request in the arenas (its data is uploaded with the other

@@ -483,3 +483,17 @@ def test_host_tables_numpy_equal_torch(dtype):
     a = F.node_float_table(st, g_np, cm_node, 0.025, dtype)
     b = F.node_float_table(st, g_np[None].expand(1, -1)[0].clone().requires_grad_(True), cm_node, 0.025, dtype)
     assert torch.equal(a, b.detach())
+
+
+def test_serve_batched_preparation_results():
+    """A pool task of several requests returns each one's payload or its own error."""
+    from jaxhip.serve import _prepare_batch
+
+    def fn(x):
+        if x == 2:
+            raise ValueError("bad request")
+        return x * 10
+
+    out = _prepare_batch(fn, [(1,), (2,), (3,)])
+    assert [ok for ok, _ in out] == [True, False, True]
+    assert out[0][1] == 10 and out[2][1] == 30 and isinstance(out[1][1], ValueError)
