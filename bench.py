@@ -413,8 +413,14 @@ def bench_served(device, n_req, legs, chunk=200, prep_processes=14):
     legs one after the other on it."""
     from jaxhip.serve import SimServer
 
+    import gc
+
     out = {}
     for dtype in dict.fromkeys(d for _, _, d, _ in legs):
+        # Memory of the earlier legs returned (host objects, cached device blocks) before
+        # the server starts.
+        gc.collect()
+        torch.cuda.empty_cache()
         cells, reqs = _served_requests(n_req, dtype)
         wcells, wreqs = _served_requests(128, dtype, seed=99)
         srv = SimServer(device, dtype=dtype, delta_t=DT, hbm_budget=4 << 30, chunk_steps=chunk,
