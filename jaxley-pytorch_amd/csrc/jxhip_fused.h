@@ -1093,7 +1093,11 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     grid_barrier();
   }
   const int64_t nsteps = pg.t1 - pg.t0;
-  const int64_t tbase = pg.soff * a.traj_steps;  // the program's voltage trajectory
+  // The voltage trajectory: per group of simulations [T + 1][group nodes][S], the
+  // program's nodes at its node base.
+  const int64_t trow = static_cast<int64_t>(a.group_nodes > 0 ? a.group_nodes : tp.n) << lS;
+  const int64_t tbase = (pg.soff - (static_cast<int64_t>(tp.nb) << lS)) * a.traj_steps +
+                        (static_cast<int64_t>(tp.nb) << lS);
   for (int64_t step = 0; step < nsteps; ++step) {
     const bool pend = step > 0;  // the last solve's increments wait in LDS
     const bool ck = kCkpt && (step % a.ckpt_every == 0);
@@ -1126,7 +1130,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
               a.ckpt_h[coff + i] = cur.h;
               a.ckpt_n[coff + i] = cur.n;
             }
-            if (!kSyn && a.traj_v != nullptr) a.traj_v[tbase + step * nS + i] = cur.v;
+            if (a.traj_v != nullptr) a.traj_v[tbase + step * trow + i] = cur.v;
           }
           vg[i] = cur.v;
           axg[i] = cur.ax;
@@ -1158,7 +1162,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
         if (last || (c.r0.w & kExport)) {
           vg[i] += dv;
           if (kSyn && a.coop) a.vx[((step + 1) & 1) * a.vx_stride + pg.soff + i] = vg[i];
-          if (kCkpt && !kSyn && last && a.traj_v != nullptr) a.traj_v[tbase + nsteps * nS + i] = vg[i];
+          if (kCkpt && last && a.traj_v != nullptr) a.traj_v[tbase + nsteps * trow + i] = vg[i];
         }
         if (!last) Ld.ds[i] = static_cast<L>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
         c = cn;
@@ -1479,10 +1483,13 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
   G* wm = g.wm + wbase;
   G* wh = g.wh + wbase;
   G* wn = g.wn + wbase;
-  // With the forward's voltage trajectory (cells only) the segment's voltages are read
-  // from it and only the gates are recomputed: an item-local recurrence, no tree solve.
-  const bool traj = !kSyn && a.traj_v != nullptr;
-  const double* tv = traj ? a.traj_v + pg.soff * a.traj_steps : nullptr;
+  // With the forward's voltage trajectory the segment's voltages are read from it and
+  // only the gates (and synapse states) are recomputed: recurrences local to an item (a
+  // synapse), no tree solve and no grid-wide barrier. Its layout is the work arrays':
+  // per group of simulations [T + 1][group nodes][S], the program's nodes at its base.
+  const bool traj = a.traj_v != nullptr;
+  const double* tvg = traj ? a.traj_v + (pg.soff - (static_cast<int64_t>(tp.nb) << lS)) * a.traj_steps : nullptr;
+  const double* tv = traj ? tvg + (static_cast<int64_t>(tp.nb) << lS) : nullptr;
   for (int64_t c = C - 1; c >= 0; --c) {
     const int64_t ts = c * K;
     const int64_t te = ts + K < nsteps ? ts + K : nsteps;
@@ -1499,11 +1506,11 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         wn[i] = n;
         const int32_t flags = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiFlags];
         if (!((flags & kComp) && (flags & kHH)) || s >= pg.nsim) continue;
-        double vn = tv[ts * nS + i];
+        double vn = tv[ts * rowS + i];
 #pragma unroll 1
         for (int64_t j = 0; j < te - ts; ++j) {
           const G vv = static_cast<G>(vn);
-          vn = tv[(ts + j + 1) * nS + i];  // the next step's voltage in flight
+          vn = tv[(ts + j + 1) * rowS + i];  // the next step's voltage in flight
           G am, bm, ah, bh, an, bn;
           hh_rates_step(vv, am, bm, ah, bh, an, bn);
           m = gate_step(m, dt, am, bm);
@@ -1532,8 +1539,28 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     }
     __syncthreads();
     // Split network: every part's checkpoint loaded before any part reads presynaptic
-    // voltages from it (the same after every recomputed step).
+    // voltages from it (the same after every recomputed step); with the trajectory, the
+    // previous segment's presynaptic shares summed by every part before this segment's
+    // reverse steps write the step-parity buffers again.
     if (kSyn && a.coop) grid_barrier();
+    if constexpr (kSyn) {
+      if (traj) {
+        // Synapse states and synaptic slopes of every step of the segment, per (target,
+        // simulation) over the steps, the voltages from the trajectory.
+        for (int i2 = tid; i2 < ntS; i2 += nthr) {
+          const int tgt = i2 >> lS, s = i2 & (S - 1);
+#pragma unroll 1
+          for (int64_t j = 0; j < te - ts; ++j) {
+            G lin = G(0), cur = G(0);
+            if (s < pg.nsim)
+              synapse_target(a, pg, tp, tgt, s, lin, cur, tvg + (ts + j) * rowS, tv + (ts + j) * rowS,
+                             ws + j * nsynS, ws + (j + 1) * nsynS);
+            wl[j * ntS + i2] = lin;
+          }
+        }
+        __syncthreads();
+      }
+    }
     for (int64_t j = 0; j < (traj ? 0 : te - ts); ++j) {
       if constexpr (kSyn) {
         // The synapse step of the recomputed forward (states into work_s[j + 1]).
@@ -1626,8 +1653,8 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     }
     for (int64_t j = te - ts - 1; j >= 0; --j) {
       tree_solve<G, L, true, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
-      const double* v0 = traj ? tv + (ts + j) * nS : wv + j * rowS;
-      const double* v1 = traj ? tv + (ts + j + 1) * nS : wv + (j + 1) * rowS;
+      const double* v0 = traj ? tv + (ts + j) * rowS : wv + j * rowS;
+      const double* v1 = traj ? tv + (ts + j + 1) * rowS : wv + (j + 1) * rowS;
       const G* m0 = wm + j * rowS;
       const G* h0 = wh + j * rowS;
       const G* n0 = wn + j * rowS;
@@ -1778,8 +1805,8 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         // postsynaptic voltage's adjoint (the slope multiplies v_post in the right-hand
         // side) into tl.
         const int32_t* jdp = a.syn_ptr + tp.spb;
-        const double* v0 = wv + j * rowS;
-        const double* vg0 = wvg + j * rowS;
+        const double* v0 = traj ? tv + (ts + j) * rowS : wv + j * rowS;
+        const double* vg0 = traj ? tvg + (ts + j) * rowS : wvg + j * rowS;
         const G* s0 = ws + j * nsynS;
         G* wqp = wq + (j & 1) * g.wq_stride;
         for (int i2 = tid; i2 < ntS; i2 += nthr) {
@@ -1994,8 +2021,8 @@ int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
   const Fused<G, L> f = unpack<G, L>(a);
   const bool syn = a.syn_model >= 0, ckpt = a.ckpt_v != nullptr;
   if (ckpt && a.ckpt_every <= 0) return hipErrorInvalidValue;
-  // A voltage trajectory is kept by the checkpointed streaming kernel of cells only.
-  if (a.traj_v != nullptr && (!ckpt || syn || a.items_per_thread != 0 || a.traj_steps < 1))
+  // A voltage trajectory is kept by the checkpointed streaming kernel only.
+  if (a.traj_v != nullptr && (!ckpt || a.items_per_thread != 0 || a.traj_steps < 1))
     return hipErrorInvalidValue;
   if (syn && ckpt) return launch_forward_feat<G, L, true, true>(a, f, stream);
   if (syn) return launch_forward_feat<G, L, true, false>(a, f, stream);
@@ -2027,7 +2054,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   if (a.items_per_thread > 0 && a.max_items > static_cast<int64_t>(a.items_per_thread) * a.threads)
     return hipErrorInvalidValue;
   if (a.items_per_thread == 0 && adj->lam_v == nullptr) return hipErrorInvalidValue;
-  if (a.traj_v != nullptr && (syn || a.traj_steps < 1)) return hipErrorInvalidValue;
+  if (a.traj_v != nullptr && a.traj_steps < 1) return hipErrorInvalidValue;
   const Fused<G, L> f = unpack<G, L>(a);
   Adj<G> g;
   g.g_rec = static_cast<const G*>(adj->g_rec);

@@ -93,3 +93,45 @@ def test_trajectory_reverse_pass_float32(device, streaming_only, monkeypatch):
     rel = np.abs(g_t - g_r).max() / np.abs(g_r).max()
     print(f"float32: trajectory vs recompute, max gradient difference {rel:.3e}")
     assert rel < 1e-3
+
+
+def _net_grads(device, synapse, B, T=200, K=40, n_cells=16, k=4, seed=8):
+    from models import build_c4, c4_stimulus
+
+    net, om, info = build_c4(n_cells, k, seed, synapse)
+    soma = info["soma_comps"]
+    for c in soma:
+        net.cell(int(c) // 9).branch(0).comp(0).record()
+    cur = c4_stimulus(B, n_cells, T, seed + 1) * 4.0
+    ds = None
+    for i, c in enumerate(soma):
+        ds = net.cell(int(c) // 9).branch(0).comp(0).data_stimulate(
+            torch.tensor(cur[:, i, :], dtype=torch.float64, device=device), ds)
+    gs = torch.tensor(np.asarray(om.synapses[0]["params"]["gS"]) * 20.0, dtype=torch.float64, device=device,
+                      requires_grad=True)
+    gna = torch.full((B, 1), 0.12, dtype=torch.float64, device=device, requires_grad=True)
+    ps = net.data_set(f"{synapse}_gS", gs, None)
+    ps = net.data_set("HH_gNa", gna, ps)
+    w = torch.tensor(np.random.default_rng(seed + 2).normal(size=(B, n_cells, T + 1)), dtype=torch.float64)
+    v = jx.integrate(net, param_state=ps, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[T // K, K])
+    (v.cpu() * w).sum().backward()
+    return gs.grad.cpu().numpy(), gna.grad.cpu().numpy()
+
+
+@pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse"])
+@pytest.mark.parametrize("B,K", [(2, 40), (2, 25), (300, 40)])
+def test_network_trajectory_reverse_pass(device, streaming_only, monkeypatch, synapse, B, K):
+    """Networks: the reverse pass from the trajectory (synapse states and gates
+    recomputed per synapse / item, no tree solves) against the recomputing one, split
+    into parts that run in lockstep (B = 2; K = 25: segments of odd length) and not
+    split (B = 300)."""
+    import jaxhip.fused as F
+
+    a1, a2 = _net_grads(device, synapse, B, K=K)
+    assert streaming_only and all(streaming_only), "the trajectory was not kept"
+    monkeypatch.setattr(F, "TRAJ_FRACTION", 0.0)
+    b1, b2 = _net_grads(device, synapse, B, K=K)
+    e1 = np.abs(a1 - b1).max() / np.abs(b1).max()
+    e2 = np.abs(a2 - b2).max() / np.abs(b2).max()
+    print(f"{synapse} B={B} K={K}: trajectory vs recompute, gS {e1:.2e}, gNa {e2:.2e}")
+    assert e1 < 1e-9 and e2 < 1e-9
