@@ -328,11 +328,13 @@ typedef struct jxhip_fused_args {
   void* vx;
   int64_t vx_stride;
   /* Voltage trajectory (float64), optional: the checkpointed streaming forward (ckpt_v
-     set, items_per_thread 0, no synapses) also stores every item's voltage at every step
-     t = 0 .. traj_steps - 1 (traj_steps = t_end - t_begin + 1) at
-     traj_v + state_off * traj_steps + t * n * S + k * S + s; the reverse pass given the
-     same trajectory then recomputes only the gates between checkpoints (no tree solve)
-     and reads the voltages from it. NULL = checkpoints only. */
+     set, items_per_thread 0) also stores every item's voltage at every step t = 0 ..
+     traj_steps - 1 (traj_steps = t_end - t_begin + 1), per group of simulations
+     [traj_steps][group nodes][S] at traj_v + (state_off - node_base * S) * traj_steps
+     (the program's nodes at node_base * S in each row; group nodes = group_nodes, or n
+     without parts); the reverse pass given the same trajectory then recomputes only the
+     gates and synapse states between checkpoints (no tree solve) and reads the voltages
+     from it. NULL = checkpoints only. */
   void* traj_v;
   int64_t traj_steps;
 } jxhip_fused_args;
