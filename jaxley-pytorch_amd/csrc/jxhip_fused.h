@@ -28,6 +28,8 @@
 #pragma once
 #include <hip/hip_cooperative_groups.h>
 
+#include <type_traits>
+
 #include "jxhip_common.h"
 #include "../../include/jxhip.h"
 
@@ -97,7 +99,7 @@ struct Fused {
   int64_t group_nodes;
   double* vx;
   int64_t vx_stride;
-  double* traj_v;      // voltage trajectory of the checkpointed forward, or null
+  G* traj_v;           // voltage trajectory of the checkpointed forward (gate type), or null
   int64_t traj_steps;
 };
 
@@ -160,7 +162,7 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.group_nodes = a.group_nodes;
   f.vx = static_cast<double*>(a.vx);
   f.vx_stride = a.vx_stride;
-  f.traj_v = static_cast<double*>(a.traj_v);
+  f.traj_v = static_cast<G*>(a.traj_v);
   f.traj_steps = a.traj_steps;
   return f;
 }
@@ -241,10 +243,10 @@ __device__ __forceinline__ G prm(const Fused<G, L>& a, const Prog& pg, const Top
 // i / area * 1e5 at the unperturbed voltages.
 // `v`: the program's voltages (program layout); `s_in` / `s_out`: the program's synapse
 // states before / after the step ([syn][S]; the same array in the forward).
-template <typename G, typename L>
+template <typename G, typename L, typename V>
 __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog& pg,
                                                const Topo& tp, int tgt, int s, G& lin, G& cur,
-                                               const double* vpre_g, const double* v,
+                                               const V* vpre_g, const V* v,
                                                const G* s_in, G* s_out) {
   G sl = G(0), sc = G(0);
   const G dt = a.dt;
@@ -1130,7 +1132,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
               a.ckpt_h[coff + i] = cur.h;
               a.ckpt_n[coff + i] = cur.n;
             }
-            if (a.traj_v != nullptr) a.traj_v[tbase + step * trow + i] = cur.v;
+            if (a.traj_v != nullptr) a.traj_v[tbase + step * trow + i] = static_cast<G>(cur.v);
           }
           vg[i] = cur.v;
           axg[i] = cur.ax;
@@ -1162,7 +1164,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
         if (last || (c.r0.w & kExport)) {
           vg[i] += dv;
           if (kSyn && a.coop) a.vx[((step + 1) & 1) * a.vx_stride + pg.soff + i] = vg[i];
-          if (kCkpt && last && a.traj_v != nullptr) a.traj_v[tbase + nsteps * trow + i] = vg[i];
+          if (kCkpt && last && a.traj_v != nullptr) a.traj_v[tbase + nsteps * trow + i] = static_cast<G>(vg[i]);
         }
         if (!last) Ld.ds[i] = static_cast<L>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
         c = cn;
@@ -1370,7 +1372,7 @@ constexpr int adjoint_max_threads(int ipt) {
   return ipt > 0 ? 512 : (sizeof(G) == 8 ? 768 : 1024);
 }
 
-template <typename G, typename L, int IPT, bool kGC, bool kSyn>
+template <typename G, typename L, int IPT, bool kGC, bool kSyn, bool kTraj>
 __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_kernel(Fused<G, L> a, Adj<G> g) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
@@ -1487,9 +1489,13 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
   // only the gates (and synapse states) are recomputed: recurrences local to an item (a
   // synapse), no tree solve and no grid-wide barrier. Its layout is the work arrays':
   // per group of simulations [T + 1][group nodes][S], the program's nodes at its base.
-  const bool traj = a.traj_v != nullptr;
-  const double* tvg = traj ? a.traj_v + (pg.soff - (static_cast<int64_t>(tp.nb) << lS)) * a.traj_steps : nullptr;
-  const double* tv = traj ? tvg + (static_cast<int64_t>(tp.nb) << lS) : nullptr;
+  // kTraj: the trajectory is in the gate type (the voltages the gates, synapses and
+  // linearization computed with in the forward).
+  constexpr bool traj = kTraj;
+  const G* tvg = kTraj ? a.traj_v + (pg.soff - (static_cast<int64_t>(tp.nb) << lS)) * a.traj_steps : nullptr;
+  const G* tv = kTraj ? tvg + (static_cast<int64_t>(tp.nb) << lS) : nullptr;
+  // The voltages the reverse steps read: the trajectory or the recomputed float64 copy.
+  using TV = std::conditional_t<kTraj, G, double>;
   for (int64_t c = C - 1; c >= 0; --c) {
     const int64_t ts = c * K;
     const int64_t te = ts + K < nsteps ? ts + K : nsteps;
@@ -1506,10 +1512,10 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         wn[i] = n;
         const int32_t flags = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiFlags];
         if (!((flags & kComp) && (flags & kHH)) || s >= pg.nsim) continue;
-        double vn = tv[ts * rowS + i];
+        G vn = tv[ts * rowS + i];
 #pragma unroll 1
         for (int64_t j = 0; j < te - ts; ++j) {
-          const G vv = static_cast<G>(vn);
+          const G vv = vn;
           vn = tv[(ts + j + 1) * rowS + i];  // the next step's voltage in flight
           G am, bm, ah, bh, an, bn;
           hh_rates_step(vv, am, bm, ah, bh, an, bn);
@@ -1653,8 +1659,15 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     }
     for (int64_t j = te - ts - 1; j >= 0; --j) {
       tree_solve<G, L, true, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);  // xs = mu = A^-T lambda_v(t+1)
-      const double* v0 = traj ? tv + (ts + j) * rowS : wv + j * rowS;
-      const double* v1 = traj ? tv + (ts + j + 1) * rowS : wv + (j + 1) * rowS;
+      const TV* v0;
+      const TV* v1;
+      if constexpr (kTraj) {
+        v0 = tv + (ts + j) * rowS;
+        v1 = tv + (ts + j + 1) * rowS;
+      } else {
+        v0 = wv + j * rowS;
+        v1 = wv + (j + 1) * rowS;
+      }
       const G* m0 = wm + j * rowS;
       const G* h0 = wh + j * rowS;
       const G* n0 = wn + j * rowS;
@@ -1667,7 +1680,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
       // step; fewer loads, measured faster); float32: loaded (measured faster).
       constexpr bool kLoadG1 = sizeof(G) == 4;
       struct RevItem {
-        double v0, v1;
+        TV v0, v1;
         G m0, h0, n0, m1, h1, n1;
       };
       auto load_rev = [&](int i) {
@@ -1805,8 +1818,15 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         // postsynaptic voltage's adjoint (the slope multiplies v_post in the right-hand
         // side) into tl.
         const int32_t* jdp = a.syn_ptr + tp.spb;
-        const double* v0 = traj ? tv + (ts + j) * rowS : wv + j * rowS;
-        const double* vg0 = traj ? tvg + (ts + j) * rowS : wvg + j * rowS;
+        const TV* v0;
+        const TV* vg0;
+        if constexpr (kTraj) {
+          v0 = tv + (ts + j) * rowS;
+          vg0 = tvg + (ts + j) * rowS;
+        } else {
+          v0 = wv + j * rowS;
+          vg0 = wvg + j * rowS;
+        }
         const G* s0 = ws + j * nsynS;
         G* wqp = wq + (j & 1) * g.wq_stride;
         for (int i2 = tid; i2 < ntS; i2 += nthr) {
@@ -2081,13 +2101,18 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
   g.wl = static_cast<G*>(adj->work_lin);
   g.work_lin_stride = adj->work_lin_stride;
   g.wq_stride = adj->work_q_stride;
+#define JXHIP_ADJ_T(IPT, TR)                                                                        \
+  if (syn)                                                                                          \
+    return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true, true, TR>, a, stream, f, g) \
+                         : launch_kernel(fused_adjoint_kernel<G, L, IPT, false, true, TR>, a, stream, f, g); \
+  return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true, false, TR>, a, stream, f, g) \
+                       : launch_kernel(fused_adjoint_kernel<G, L, IPT, false, false, TR>, a, stream, f, g);
 #define JXHIP_ADJ(IPT)                                                                              \
   case IPT:                                                                                         \
-    if (syn)                                                                                        \
-      return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true, true>, a, stream, f, g) \
-                           : launch_kernel(fused_adjoint_kernel<G, L, IPT, false, true>, a, stream, f, g); \
-    return a.coef_global ? launch_kernel(fused_adjoint_kernel<G, L, IPT, true, false>, a, stream, f, g) \
-                         : launch_kernel(fused_adjoint_kernel<G, L, IPT, false, false>, a, stream, f, g);
+    if (a.traj_v != nullptr) {                                                                      \
+      JXHIP_ADJ_T(IPT, true)                                                                        \
+    }                                                                                               \
+    JXHIP_ADJ_T(IPT, false)
   switch (a.items_per_thread) {
     JXHIP_ADJ(0)
     JXHIP_ADJ(1)
@@ -2100,6 +2125,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
       return hipErrorInvalidValue;
   }
 #undef JXHIP_ADJ
+#undef JXHIP_ADJ_T
 }
 
 }  // namespace jxhip

@@ -751,8 +751,9 @@ class ForwardLaunch:
         # the streaming kernel runs and the trajectory fits the memory budget.
         self.traj = None
         if ckpt_every > 0 and a.items_per_thread == 0 and keep_trajectory(
-                (T + 1) * self.v.numel() * 8, dev):
-            self.traj = torch.empty((T + 1) * self.v.numel(), dtype=torch.float64, device=dev)
+                (T + 1) * self.v.numel() * self.gm.element_size(), dev):
+            # In the gate type: the voltages the gates, synapses and linearization use.
+            self.traj = torch.empty((T + 1) * self.v.numel(), dtype=self.gm.dtype, device=dev)
             a.traj_v = self.traj.data_ptr()
             a.traj_steps = T + 1
         self.args = a
