@@ -284,7 +284,12 @@ def bench_network(device, n_cells=1024, k=32, B=64, T=4000, dtype=torch.float32)
     res = {"comp_steps_per_s": n_comp * T * B / sec, "ms": sec * 1e3, "batch": B, "cells": n_cells,
            "synapses": n_cells * k, "compartments": n_comp, "time_steps": T,
            "dtype": str(dtype).replace("torch.", ""),
-           "mean_spikes_per_soma": float(spikes(v.double()).float().mean())}
+           "mean_spikes_per_soma": float(spikes(v.double()).float().mean()),
+           # Algorithmic bytes per compartment-step of the float32 network forward: per
+           # cell and step 8 B x (9 voltages + 3 gates + 32 synapse states) over its 9
+           # compartments (presynaptic voltages come from L2, not counted).
+           "bytes_per_comp_step": 39.1,
+           "frac_of_hbm_roofline": 39.1 * n_comp * T * B / sec / HBM_PEAK}
     log(f"[bench] network {n_cells} cells x {k} synapses, B={B}, T={T}: {sec*1e3:.1f} ms, "
         f"{res['comp_steps_per_s']:.3e} comp-steps/s")
     return res
