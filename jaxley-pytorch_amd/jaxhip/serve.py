@@ -28,6 +28,7 @@ tests check alone vs in a mix, and against `integrate`).
 """
 from __future__ import annotations
 
+import bisect
 import concurrent.futures as cf
 import heapq
 import importlib
@@ -73,15 +74,17 @@ class _Arena:
     def free(self, off: int, size: int) -> None:
         size = max(int(size), 1)
         self.used -= size
-        self.free_list.append((off, size))
-        self.free_list.sort()
-        merged = []
-        for o, ln in self.free_list:
-            if merged and merged[-1][0] + merged[-1][1] == o:
-                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
-            else:
-                merged.append((o, ln))
-        self.free_list = merged
+        fl = self.free_list
+        j = bisect.bisect_left(fl, (off, size))
+        # Merge with the free ranges just before and after it.
+        if j > 0 and fl[j - 1][0] + fl[j - 1][1] == off:
+            j -= 1
+            off, size = fl[j][0], fl[j][1] + size
+            del fl[j]
+        if j < len(fl) and off + size == fl[j][0]:
+            size += fl[j][1]
+            del fl[j]
+        fl.insert(j, (off, size))
 
 
 class _Staging:
@@ -177,7 +180,7 @@ class _Prepared:
                 continue
             kind = "i" if name in self._INTS else "r"
             x = _np(x, np.int32 if kind == "i" else np.float64)
-            layout[name] = (kind, off[kind], x.shape)
+            layout[name] = (kind, off[kind], x.shape, x.size)
             parts[kind].append(x.reshape(-1))
             off[kind] += x.size
         self.layout = layout
@@ -194,9 +197,9 @@ class _Prepared:
             if name in self._INTS + self._REALS:
                 return None
             raise AttributeError(name)
-        kind, off, shape = lay[name]
+        kind, off, shape, size = lay[name]
         buf = self.ints if kind == "i" else self.reals
-        return buf[off:off + int(np.prod(shape))].reshape(shape)
+        return buf[off:off + size].reshape(shape)
 
 
 def _prepare_payload(module, params, param_state, data_stimuli, t_max, all_states, dt: float,
@@ -270,7 +273,9 @@ def _to_numpy(x):
     """Tensors in nested lists / tuples / dicts as numpy arrays (they cross a process
     boundary as plain bytes; torch tensors would go through shared-memory handles)."""
     if isinstance(x, torch.Tensor):
-        return ("__tensor__", x.detach().cpu().numpy())
+        if x.device.type != "cpu" or x.requires_grad:
+            x = x.detach().cpu()
+        return ("__tensor__", x.numpy())
     if isinstance(x, dict):
         return {k: _to_numpy(v) for k, v in x.items()}
     if isinstance(x, (list, tuple)):
@@ -580,54 +585,68 @@ class SimServer:
         compute stream waits for before the next chunk."""
         if not lives:
             return
-        node_idx, node_i, node_f, node_p, v0, gates = [], [], [], [], [], ([], [], [])
-        aux_idx, aux_val, topo_rows = [], [], []
-        ent_idx, ent_col, ent_scale, stim_idx, stim_val = [], [], [], [], []
-        ncols = self.stim.shape[1]
-        for L in lives:
-            P, n, no = L.P, L.n, L.node_off
-            lay = P.layout
-            node_idx.append(np.arange(no, no + n, dtype=np.int64))
-            ni = P.node_i.copy()
-            ni[:, 4:6] += L.stim_off
-            node_i.append(ni)
-            node_f.append(P.node_f)
-            node_p.append(P.node_p)
-            v0.append(P.v0)
-            for lst, k in zip(gates, ("gate_m", "gate_h", "gate_n")):
-                lst.append(getattr(P, k))
-            # Schedule tables, child lists and recorded nodes lie back to back in `ints`.
-            a = L.aux_off
-            s0 = lay["seg"][1]
-            aux_idx.append(np.arange(a, a + L.aux_size, dtype=np.int64))
-            aux_val.append(P.ints[s0:s0 + L.aux_size])
-            lb = a + 4 * L.nseg
-            scb = lb + P.L + 2
-            cb = scb + L.ncs
-            rb = cb + L.nchild
-            topo_rows.append([n, P.L, no, lb, cb, L.n_rec, rb, 0, 0, 0, 0, 0, a // 4, scb, 0, 0])
-            if L.n_stim:
-                e0 = L.stim_off
-                ent_idx.append(np.arange(e0, e0 + L.n_stim, dtype=np.int64))
-                ent_col.append(P.stim_col + e0)
-                ent_scale.append(P.stim_scale)
-                stim = P.stim
-                T, c = stim.shape
-                stim_idx.append((np.arange(T, dtype=np.int64)[:, None] * ncols
-                                 + np.arange(e0, e0 + c, dtype=np.int64)[None, :]).reshape(-1))
-                stim_val.append(stim.reshape(-1))
+        m = len(lives)
+        # Per request: arena places, sizes and where each array lies in its payload.
+        f = np.zeros((13, m), dtype=np.int64)
+        for r, L in enumerate(lives):
+            lay, P = L.P.layout, L.P
+            st = lay.get("stim")
+            f[:, r] = (L.node_off, L.n, L.aux_off, L.aux_size, L.stim_off, L.n_stim,
+                       st[2][1] if st else 0, st[2][0] if st else 0, lay["seg"][1],
+                       lay["stim_col"][1] if st else 0, lay["node_f"][1], lay["stim_scale"][1] if st else 0,
+                       lay["stim"][1] if st else 0)
+        node_off, n, aux_off, aux_size, stim_off, n_stim, ncol, Tst, seg0, scol0, nf0, ss0, stv0 = f
+        ilen = np.array([L.P.ints.size for L in lives], dtype=np.int64)
+        rlen = np.array([L.P.reals.size for L in lives], dtype=np.int64)
+        ib = np.concatenate([[0], np.cumsum(ilen)[:-1]])
+        rb = np.concatenate([[0], np.cumsum(rlen)[:-1]])
+        I = np.concatenate([L.P.ints for L in lives])
+        R = np.concatenate([L.P.reals for L in lives])
+
+        def ranges(starts, lens):
+            """Concatenated aranges [starts[r], starts[r] + lens[r])."""
+            tot = int(lens.sum())
+            return np.arange(tot, dtype=np.int64) - np.repeat(np.cumsum(lens) - lens - starts, lens)
+
+        # Node rows: node_i (its stimulus range moved to the request's entries), node_f,
+        # node_p, v0 and the three gates lie in this order in each payload (float arrays
+        # one after another from node_f's offset).
+        ni = I[ranges(ib, 8 * n)].reshape(-1, _lib.JXHIP_NODE_ICOLS)
+        ni[:, 4:6] += np.repeat(stim_off, n)[:, None].astype(np.int32)
+        nf = R[ranges(rb + nf0, 4 * n)]
+        npar = R[ranges(rb + nf0 + 4 * n, 8 * n)]
+        v0 = R[ranges(rb + nf0 + 12 * n, n)]
+        gm, gh, gn = (R[ranges(rb + nf0 + (13 + q) * n, n)] for q in range(3))
+        nidx = ranges(node_off, n)
+        # Schedule tables, child lists and recorded nodes: one block per request.
+        aidx = ranges(aux_off, aux_size)
+        aval = I[ranges(ib + seg0, aux_size)]
+        lb = aux_off + 4 * np.array([L.nseg for L in lives], dtype=np.int64)
+        Ls = np.array([L.P.L for L in lives], dtype=np.int64)
+        scb = lb + Ls + 2
+        cb = scb + np.array([L.ncs for L in lives], dtype=np.int64)
+        rbase = cb + np.array([L.nchild for L in lives], dtype=np.int64)
+        topo = np.zeros((m, _lib.JXHIP_TOPO_COLS), dtype=np.int64)
+        topo[:, 0], topo[:, 1], topo[:, 2], topo[:, 3], topo[:, 4] = n, Ls, node_off, lb, cb
+        topo[:, 5] = [L.n_rec for L in lives]
+        topo[:, 6], topo[:, 12], topo[:, 13] = rbase, aux_off // 4, scb
         slots = np.array([L.slot for L in lives], dtype=np.int64)
-        cat = np.concatenate
-        nidx, aidx = cat(node_idx), cat(aux_idx)
-        i64 = [nidx, aidx, slots] + ([cat(ent_idx), cat(stim_idx)] if ent_idx else [])
-        i32 = [cat(node_i).reshape(-1), cat(aux_val), np.asarray(topo_rows, dtype=np.int32).reshape(-1)]
-        if ent_idx:
-            i32.append(cat(ent_col).astype(np.int32))
-        gvals = [cat(node_p).reshape(-1)] + [cat(g) for g in gates]
-        if ent_idx:
-            gvals += [cat(ent_scale), cat(stim_val)]
-        dev = self._stage.put({"i64": i64, "i32": i32, "l": [cat(node_f).reshape(-1)], "g": gvals,
-                               "d": [cat(v0)]}, self.up_stream)
+        i64 = [nidx, aidx, slots]
+        i32 = [ni.reshape(-1), aval, topo.astype(np.int32).reshape(-1)]
+        gvals = [npar, gm, gh, gn]
+        has_stim = bool(n_stim.any())
+        if has_stim:
+            # Stimulus entries and the (T, columns) stimulus of each request at its
+            # entries' columns of the shared table.
+            i64.append(ranges(stim_off, n_stim))
+            i32.append((I[ranges(ib + scol0, n_stim)] + np.repeat(stim_off, n_stim)).astype(np.int32))
+            gvals.append(R[ranges(rb + ss0, n_stim)])
+            cnt = Tst * ncol
+            k = ranges(np.zeros(m, dtype=np.int64), cnt)
+            c_rep = np.repeat(np.maximum(ncol, 1), cnt)
+            i64.append(k // c_rep * self.stim.shape[1] + np.repeat(stim_off, cnt) + k % c_rep)
+            gvals.append(R[ranges(rb + stv0, cnt)])
+        dev = self._stage.put({"i64": i64, "i32": i32, "l": [nf], "g": gvals, "d": [v0]}, self.up_stream)
         with torch.cuda.stream(self.up_stream):
             nix, aix, six = dev["i64"][:3]
             self.node_i.index_copy_(0, nix, dev["i32"][0].view(-1, self.node_i.shape[1]))
@@ -638,16 +657,30 @@ class SimServer:
                 buf.index_copy_(0, nix, x)
             self.aux.index_copy_(0, aix, dev["i32"][1])
             self.topos.index_copy_(0, six, dev["i32"][2].view(-1, self.topos.shape[1]))
-            if ent_idx:
+            if has_stim:
                 eix, stx = dev["i64"][3:5]
                 self.stim_col.index_copy_(0, eix, dev["i32"][3])
                 self.stim_scale.index_copy_(0, eix, dev["g"][4])
                 self.stim.view(-1).index_copy_(0, stx, dev["g"][5])
 
+    def _retire_all(self, lives: List[_Live]) -> None:
+        """Results of simulations whose last chunk has completed: their recordings are
+        gathered from the arena in one copy (views of it per request), then their places
+        are freed."""
+        if not lives:
+            return
+        lens = np.array([(L.T + 1) * max(L.n_rec, 1) for L in lives], dtype=np.int64)
+        offs = np.array([L.rec_off for L in lives], dtype=np.int64)
+        idx = np.arange(int(lens.sum()), dtype=np.int64) - np.repeat(np.cumsum(lens) - lens - offs, lens)
+        flat = self.rec[torch.from_numpy(idx).to(self.device, non_blocking=True)].to(self.user_dtype)
+        pos = 0
+        for L, ln in zip(lives, lens.tolist()):
+            L.handle.result = flat[pos:pos + ln].view(L.T + 1, max(L.n_rec, 1))[:, :L.n_rec].t()
+            pos += ln
+            self._retire(L)
+
     def _retire(self, L: _Live) -> None:
         h = L.handle
-        r = self.rec[L.rec_off:L.rec_off + (L.T + 1) * max(L.n_rec, 1)]
-        h.result = r.reshape(L.T + 1, max(L.n_rec, 1))[:, :L.n_rec].t().to(self.user_dtype).clone()
         h.t_done = time.perf_counter()
         for arena, off, size in L.allocs:
             arena.free(off, size)
@@ -788,8 +821,7 @@ class SimServer:
         while self.inflight and (block or self.inflight[0][0].query()):
             ev, finishing, _, _ = self.inflight.pop(0)
             ev.synchronize()
-            for L in finishing:
-                self._retire(L)
+            self._retire_all(finishing)
             block = False
         # The GPU first: the simulations already admitted get their next chunk before the
         # host work of new admissions.
