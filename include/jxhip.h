@@ -329,8 +329,8 @@ typedef struct jxhip_fused_args {
   int64_t vx_stride;
   /* Voltage trajectory, optional, in the element type of the gates (the float64
      voltages rounded to it: the values the gates, synapses and linearization compute
-     with): the checkpointed streaming forward (ckpt_v set, items_per_thread 0) also
-     stores every item's voltage at every step t = 0 ..
+     with): the checkpointed forward (ckpt_v set) also stores every item's voltage at
+     every step t = 0 ..
      traj_steps - 1 (traj_steps = t_end - t_begin + 1), per group of simulations
      [traj_steps][group nodes][S] at traj_v + (state_off - node_base * S) * traj_steps
      (the program's nodes at node_base * S in each row; group nodes = group_nodes, or n

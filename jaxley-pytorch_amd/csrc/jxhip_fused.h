@@ -1225,6 +1225,10 @@ __global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
   }
   record_step(a, pg, tp, vg, 0);
   const int64_t nsteps = pg.t1 - pg.t0;
+  // The voltage trajectory (layout as in the streaming kernel).
+  const int64_t trow = static_cast<int64_t>(a.group_nodes > 0 ? a.group_nodes : tp.n) << lS;
+  const int64_t tbase = (pg.soff - (static_cast<int64_t>(tp.nb) << lS)) * a.traj_steps +
+                        (static_cast<int64_t>(tp.nb) << lS);
   for (int64_t step = 0; step < nsteps; ++step) {
     if constexpr (kCkpt) {
       if (step % a.ckpt_every == 0) {
@@ -1241,6 +1245,13 @@ __global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
           }
         }
         if constexpr (kSyn) checkpoint_syn(a, pg, tp, c);
+      }
+      if (a.traj_v != nullptr) {
+#pragma unroll 1
+        for (int j = 0; j < IPT; ++j) {
+          const int i = tid + j * nthr;
+          if (i < nS) a.traj_v[tbase + step * trow + i] = static_cast<G>(vr[j]);
+        }
       }
     }
     if constexpr (kSyn) synapse_phase(a, pg, tp, Ld, a.v + pg.soff - (static_cast<int64_t>(tp.nb) << lS));
@@ -1287,6 +1298,7 @@ __global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
       gm[i] = mr[j];
       gh[i] = hr[j];
       gn[i] = nr[j];
+      if (kCkpt && a.traj_v != nullptr) a.traj_v[tbase + nsteps * trow + i] = static_cast<G>(vr[j]);
     }
   }
 }
@@ -2041,9 +2053,8 @@ int launch_forward(const jxhip_fused_args* args, hipStream_t stream) {
   const Fused<G, L> f = unpack<G, L>(a);
   const bool syn = a.syn_model >= 0, ckpt = a.ckpt_v != nullptr;
   if (ckpt && a.ckpt_every <= 0) return hipErrorInvalidValue;
-  // A voltage trajectory is kept by the checkpointed streaming kernel only.
-  if (a.traj_v != nullptr && (!ckpt || a.items_per_thread != 0 || a.traj_steps < 1))
-    return hipErrorInvalidValue;
+  // A voltage trajectory is kept by the checkpointed kernels only.
+  if (a.traj_v != nullptr && (!ckpt || a.traj_steps < 1)) return hipErrorInvalidValue;
   if (syn && ckpt) return launch_forward_feat<G, L, true, true>(a, f, stream);
   if (syn) return launch_forward_feat<G, L, true, false>(a, f, stream);
   if (ckpt) return launch_forward_feat<G, L, false, true>(a, f, stream);

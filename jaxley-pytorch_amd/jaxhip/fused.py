@@ -748,9 +748,9 @@ class ForwardLaunch:
             a.ckpt_stride = self.v.numel()
             a.ckpt_syn_stride = self.syn_s.numel() if self.syn_s is not None else 0
         # The voltage of every step (for the reverse pass: no recomputed tree solves) when
-        # the streaming kernel runs and the trajectory fits the memory budget.
+        # the trajectory fits the memory budget.
         self.traj = None
-        if ckpt_every > 0 and a.items_per_thread == 0 and keep_trajectory(
+        if ckpt_every > 0 and keep_trajectory(
                 (T + 1) * self.v.numel() * self.gm.element_size(), dev):
             # In the gate type: the voltages the gates, synapses and linearization use.
             self.traj = torch.empty((T + 1) * self.v.numel(), dtype=self.gm.dtype, device=dev)

@@ -135,3 +135,26 @@ def test_network_trajectory_reverse_pass(device, streaming_only, monkeypatch, sy
     e2 = np.abs(a2 - b2).max() / np.abs(b2).max()
     print(f"{synapse} B={B} K={K}: trajectory vs recompute, gS {e1:.2e}, gNa {e2:.2e}")
     assert e1 < 1e-9 and e2 < 1e-9
+
+
+def test_trajectory_register_resident_forward(device, monkeypatch):
+    """The register-resident forward keeps the trajectory too (the launch shapes the
+    autotuner picks for small batches): against the recomputing reverse pass."""
+    import jaxhip.autotune as A
+    import jaxhip.fused as F
+
+    monkeypatch.setattr(A, "_CACHE", {})
+    shapes = []
+    orig = F.ForwardLaunch.__init__
+
+    def spy(self, *args, **kw):
+        orig(self, *args, **kw)
+        shapes.append((self.args.items_per_thread, self.traj is not None))
+    monkeypatch.setattr(F.ForwardLaunch, "__init__", spy)
+    _, g_t, _ = _grads(device, torch.float64, 2, 200, 50)
+    assert any(ipt > 0 and kept for ipt, kept in shapes), shapes
+    monkeypatch.setattr(F, "TRAJ_FRACTION", 0.0)
+    _, g_r, _ = _grads(device, torch.float64, 2, 200, 50)
+    rel = np.abs(g_t - g_r).max() / np.abs(g_r).max()
+    print(f"register-resident forward: trajectory vs recompute {rel:.3e}; shapes {shapes}")
+    assert rel < 1e-9
