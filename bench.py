@@ -63,7 +63,8 @@ BYTES_PER_COMP_STEP = {"f64": 64, "f32": 32}
 # not collected inside this run (a counter pass is a separate rocprofv3 run).
 TRAFFIC_SOURCE = "profiles/fused_stream_f64_c1_b4096_t2000.md"
 MEASURED_TRAFFIC = {
-    ("float64", 4096, 2000, "S=8, threads=1024, stream=True, gcoef=True"): 376.4e9 + 372.3e9,
+    ("float64", 4096, 2000, "S=8, threads=1024, stream=True, gcoef=False, sym=True"): 374.2e9 + 371.7e9,
+    ("float64", 4096, 2000, "S=8, threads=1024, stream=True, gcoef=True, sym=False"): 376.4e9 + 372.3e9,
 }
 
 

@@ -279,7 +279,13 @@ typedef struct jxhip_fused_args {
                                threads (else the call fails with an invalid argument) */
   int32_t coef_global;      /* forward, streaming only: 1 = the tree's coupling
                                coefficients are read from node_f (HBM / L2) and lds_bytes
-                               excludes their LDS copy; 0 = copied to LDS */
+                               excludes their LDS copy; 0 = copied to LDS; 2 = the solve
+                               runs on the row-scaled symmetric system of node_sym, whose
+                               one coefficient per node is copied to LDS (lds_bytes
+                               includes n * sizeof(solve type) rounded up to 16 in place
+                               of the pairs), and the residual updates read node_f from
+                               HBM / L2 (not with explicit_step; the reverse pass
+                               rejects it) */
   void* work_ax;            /* forward, streaming only (required there): scratch of the
                                axial part of every item's residual, program layout
                                [node][S] at state_off, gate type; computed from v at the
@@ -339,6 +345,13 @@ typedef struct jxhip_fused_args {
      from it. NULL = checkpoints only. */
   void* traj_v;
   int64_t traj_steps;
+  /* coef_global = 2 only (else ignored): per kernel node, rows as node_f, the pair
+     (s_k, w_k) in the solve type. w_k > 0 scales row k of the system (the tree's root
+     rows 1, then w_child = w_parent * upper / lower of the child's edge), and s_k =
+     w_k * lower_k = w_parent * upper_k is the coupling of node k and its parent in the
+     scaled, symmetric system (0 at roots). The solution is that of the unscaled system;
+     the rounding differs from it. */
+  const void* node_sym;
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

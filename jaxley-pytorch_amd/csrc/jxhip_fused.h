@@ -101,6 +101,7 @@ struct Fused {
   int64_t vx_stride;
   G* traj_v;           // voltage trajectory of the checkpointed forward (gate type), or null
   int64_t traj_steps;
+  const L* node_sym;   // (symmetric coupling coefficient, row scale) per node, or null
 };
 
 template <typename G, typename L>
@@ -164,6 +165,7 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.vx_stride = a.vx_stride;
   f.traj_v = static_cast<G*>(a.traj_v);
   f.traj_steps = a.traj_steps;
+  f.node_sym = static_cast<const L*>(a.node_sym);
   return f;
 }
 
@@ -370,6 +372,7 @@ struct Lds {
   L* tl;   // synapse slope per (target, simulation)
   L* tc;   // synapse current per (target, simulation)
   L* cl;   // lower / upper coefficient per node
+  L* cs;   // symmetric coupling coefficient per node (solve of the row-scaled system)
   Sched sc;
 };
 
@@ -414,13 +417,18 @@ __device__ __forceinline__ Sched stage_schedule(const Fused<G, L>& a, const Topo
 
 // Lower / upper coefficient pair of node k in one load; with kT (solve with A^T) the two
 // swap roles. kCS = 0: packed pairs (LDS copy); kCS = 1: the node table rows (lo, up,
-// dbase, inv_cm) in HBM / L2.
+// dbase, inv_cm) in HBM / L2; kCS = 2: one coefficient per node, lo = up (the system
+// with every row scaled so that it is symmetric).
 template <typename L, bool kT, int kCS = 0>
 __device__ __forceinline__ void coef(const L* cl, int k, L& lo, L& up) {
-  using V = typename Vec2<L>::type;
-  const V c = reinterpret_cast<const V*>(cl)[k << kCS];
-  lo = kT ? c.y : c.x;
-  up = kT ? c.x : c.y;
+  if constexpr (kCS == 2) {
+    lo = up = cl[k];
+  } else {
+    using V = typename Vec2<L>::type;
+    const V c = reinterpret_cast<const V*>(cl)[k << kCS];
+    lo = kT ? c.y : c.x;
+    up = kT ? c.x : c.y;
+  }
 }
 
 // Eliminates segment q of simulation s, its child segments being eliminated: merges the
@@ -804,6 +812,7 @@ template <typename G, typename L>
 struct ItemConst {
   int32_t flags, sbeg, send, tgt;
   L dbase, inv_cm;
+  L w;  // row scale of the symmetric system (kSym), else 1
   G p[JXHIP_NPARAM];
 };
 
@@ -819,7 +828,7 @@ __device__ __forceinline__ ItemState<G> load_state(const Fused<G, L>& a, const P
   return st;
 }
 
-template <typename G, typename L>
+template <bool kSym = false, typename G, typename L>
 __device__ __forceinline__ ItemConst<G, L> load_const(const Fused<G, L>& a, const Prog& pg,
                                                       const Topo& tp, int i) {
   using VG = typename Vec2<G>::type;
@@ -836,6 +845,7 @@ __device__ __forceinline__ ItemConst<G, L> load_const(const Fused<G, L>& a, cons
   const VL f = reinterpret_cast<const VL*>(a.node_f + row * JXHIP_NODE_FCOLS)[1];
   c.dbase = f.x;
   c.inv_cm = f.y;
+  c.w = kSym ? a.node_sym[row * 2 + 1] : L(1);
   const G* prow = a.node_p + row * JXHIP_NPARAM;
   const VG* np = reinterpret_cast<const VG*>(prow);
 #pragma unroll
@@ -860,15 +870,15 @@ __device__ __forceinline__ ItemConst<G, L> load_const(const Fused<G, L>& a, cons
 }
 
 // build_core from loaded inputs: diagonal and residual r = b - A v of item i into LDS,
-// the updated gates into `it`.
-template <typename G, typename L>
+// the updated gates into `it`. kSym: the row scaled by c.w (the symmetric system).
+template <bool kSym = false, typename G, typename L>
 __device__ __forceinline__ void build_loaded(const Fused<G, L>& a, const Prog& pg,
                                              const Topo& tp, const Lds<L>& Ld, ItemState<G>& it,
                                              const ItemConst<G, L>& c, int i, int s,
                                              int64_t t) {
   if (!(c.flags & kComp)) {
-    Ld.ds[i] = c.dbase;
-    Ld.xs[i] = static_cast<L>(a.rscale * it.ax);
+    Ld.ds[i] = kSym ? c.w * c.dbase : c.dbase;
+    Ld.xs[i] = kSym ? c.w * static_cast<L>(a.rscale * it.ax) : static_cast<L>(a.rscale * it.ax);
     return;
   }
   const int lS = a.log2S;
@@ -912,8 +922,10 @@ __device__ __forceinline__ void build_loaded(const Fused<G, L>& a, const Prog& p
     const int64_t bidx = a.stim_b > 1 ? (pg.simoff + s) : 0;
     r += a.stim[(t * a.stim_cols + col) * a.stim_b + bidx] * a.stim_scale[e];
   }
-  Ld.ds[i] = c.dbase + static_cast<L>(a.dtv * inv_cm * lin);
-  Ld.xs[i] = static_cast<L>(a.rscale * r);
+  const L d = c.dbase + static_cast<L>(a.dtv * inv_cm * lin);
+  const L x = static_cast<L>(a.rscale * r);
+  Ld.ds[i] = kSym ? c.w * d : d;
+  Ld.xs[i] = kSym ? c.w * x : x;
 }
 
 // Items of simulations beyond `nsim` (padding): a well-conditioned row, zero rhs.
@@ -948,20 +960,31 @@ __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& 
 }
 
 // LDS of a forward program: the staged schedule, the lower / upper coefficient per node
-// (unless kGC: read from the node table in HBM / L2), rhs / increment and diagonal per
-// (node, simulation), synapse terms per (target, simulation). Stages the schedule and
-// the coefficients (the caller synchronizes).
-template <typename G, typename L, bool kGC = false>
+// (kCM = 0; kCM = 1: read from the node table in HBM / L2; kCM = 2: one symmetric
+// coefficient per node for the solve, the pairs of the residual updates from the node
+// table), rhs / increment and diagonal per (node, simulation), synapse terms per
+// (target, simulation). Stages the schedule and the coefficients (the caller
+// synchronizes).
+template <typename G, typename L, int kCM = 0>
 __device__ __forceinline__ Lds<L> forward_lds(const Fused<G, L>& a, unsigned char* smem, int nS,
                                               const Topo& tp, int lS) {
   Lds<L> Ld;
   const SchedSize z = schedule_size(a, tp);
   Ld.sc = stage_schedule(a, tp, z, smem);
-  if constexpr (kGC) {
+  if constexpr (kCM != 0) {
     Ld.cl = const_cast<L*>(a.node_f + static_cast<int64_t>(tp.nb) * JXHIP_NODE_FCOLS);
+    Ld.cs = Ld.cl;
     Ld.xs = reinterpret_cast<L*>(smem + z.bytes);
+    if constexpr (kCM == 2) {
+      // Rounded up to 16 bytes, as the pairs.
+      Ld.cs = reinterpret_cast<L*>(smem + z.bytes);
+      Ld.xs = Ld.cs + ((tp.n * sizeof(L) + 15) & ~15) / sizeof(L);
+      const L* sym = a.node_sym + static_cast<int64_t>(tp.nb) * 2;
+      for (int k = threadIdx.x; k < tp.n; k += blockDim.x) Ld.cs[k] = sym[2 * k];
+    }
   } else {
     Ld.cl = reinterpret_cast<L*>(smem + z.bytes);
+    Ld.cs = Ld.cl;
     Ld.xs = Ld.cl + 2 * tp.n;
     load_coefficients(a, tp, Ld.cl);
   }
@@ -1057,17 +1080,22 @@ __device__ __forceinline__ const T& kernarg() {
 // the item's own LDS slots when it loads the item's state (and takes the checkpoints).
 // The last step applies every increment in that pass.
 // kSyn: the topologies carry synapses; kCkpt: checkpoints are written for the adjoint;
-// kGC: the tree's coupling coefficients are read from the node table (HBM / L2) instead
-// of an LDS copy, which leaves the LDS to the system of more simulations.
-template <typename G, typename L, bool kSyn, bool kCkpt, bool kGC>
+// kCM, where the tree's coupling coefficients are: 0, a copy of the (lower, upper) pairs
+// in LDS; 1, the node table (HBM / L2), which leaves the LDS to the system of more
+// simulations; 2, the node table for the residual updates, and for the solve one
+// coefficient per node in LDS, half the pairs' bytes: every row of the system is scaled
+// by its node's w (w_child / w_parent = upper / lower of the edge), which makes the
+// matrix symmetric without changing the solution.
+template <typename G, typename L, bool kSyn, bool kCkpt, int kCM>
 __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   extern __shared__ unsigned char smem_raw[];
   const Prog pg = load_prog(a.progs, blockIdx.x);
   const Topo tp = load_topo(a.topos, pg.topo);
   const int lS = a.log2S, S = a.S;
   const int nS = tp.n << lS;
-  const Lds<L> Ld = forward_lds<G, L, kGC>(a, smem_raw, nS, tp, lS);
-  constexpr int kCS = kGC ? 1 : 0;
+  const Lds<L> Ld = forward_lds<G, L, kCM>(a, smem_raw, nS, tp, lS);
+  constexpr int kCS = kCM ? 1 : 0;
+  constexpr bool kSym = kCM == 2;
   __syncthreads();
   const int tid = threadIdx.x, nthr = blockDim.x;
   double* vg = a.v + pg.soff;
@@ -1116,7 +1144,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
       for (int i = tid; i < nS; i += nthr) {
         const Fused<G, L>& a = kernarg<Fused<G, L>>();  // re-read per item (SGPR pressure)
         const int inx = i + nthr < nS ? i + nthr : i;
-        const ItemConst<G, L> c = load_const(a, pg, tp, i);
+        const ItemConst<G, L> c = load_const<kSym>(a, pg, tp, i);
         __builtin_amdgcn_sched_barrier(0);
         const ItemState<G> nxt = load_state(a, pg, inx);
         __builtin_amdgcn_sched_barrier(0);
@@ -1136,12 +1164,13 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
           }
           vg[i] = cur.v;
           axg[i] = cur.ax;
-          build_loaded(a, pg, tp, Ld, cur, c, i, s_own, t);
+          build_loaded<kSym>(a, pg, tp, Ld, cur, c, i, s_own, t);
           gm[i] = cur.m;
           gh[i] = cur.h;
           gn[i] = cur.n;
         } else {
-          Ld.ds[i] = c.dbase + ((c.flags & kComp) ? L(1) : L(0));
+          const L d = c.dbase + ((c.flags & kComp) ? L(1) : L(0));
+          Ld.ds[i] = kSym ? c.w * d : d;
           Ld.xs[i] = L(0);
         }
         cur = nxt;
@@ -1151,7 +1180,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     if (a.expl) {
       branchpoint_pass<kCS>(a, tp, ni, Ld);
     } else {
-      tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
+      tree_solve<G, L, false, kSym ? 2 : kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cs);
     }
     const bool last = step + 1 == nsteps;
     if (s_own < pg.nsim && tid < nS) {
@@ -1960,6 +1989,7 @@ inline int check_args(const jxhip_fused_args& a) {
   if (a.dt_v < 0 || a.rhs_scale < 0 || a.explicit_step < 0 || a.explicit_step > 1) return hipErrorInvalidValue;
   if (a.node_p2 != nullptr && (a.gate_ext == nullptr || a.gate_ext_stride <= 0)) return hipErrorInvalidValue;
   if (a.parts < 0 || a.coop < 0 || a.coop > 1 || a.group_nodes < 0) return hipErrorInvalidValue;
+  if (a.coef_global < 0 || a.coef_global > 2) return hipErrorInvalidValue;
   // Lockstep programs: whole groups of parts.
   if (a.coop && (a.parts < 2 || a.n_progs % a.parts)) return hipErrorInvalidValue;
   return hipSuccess;
@@ -2004,8 +2034,14 @@ int launch_forward_feat(const jxhip_fused_args& a, const Fused<G, L>& f, hipStre
   if (a.items_per_thread == 0) {
     if (a.work_ax == nullptr) return hipErrorInvalidValue;
     if (kSyn && a.coop && (a.vx == nullptr || a.vx_stride <= 0)) return hipErrorInvalidValue;
-    if (a.coef_global) return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, true>, a, st, f);
-    return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, false>, a, st, f);
+    if (a.coef_global == 2) {
+      // The symmetric system: its coefficients and row scales; not with forward Euler
+      // (its branchpoint pass reads the unscaled rows).
+      if (a.node_sym == nullptr || a.explicit_step) return hipErrorInvalidValue;
+      return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, 2>, a, st, f);
+    }
+    if (a.coef_global) return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, 1>, a, st, f);
+    return launch_kernel(fused_stream_kernel<G, L, kSyn, kCkpt, 0>, a, st, f);
   }
   if (a.coef_global || a.coop) return hipErrorInvalidValue;  // lockstep programs: streaming only
   if (a.max_items > static_cast<int64_t>(a.items_per_thread) * a.threads) return hipErrorInvalidValue;
@@ -2086,6 +2122,7 @@ int launch_adjoint(const jxhip_fused_args* args, const jxhip_adjoint_args* adj,
     return hipErrorInvalidValue;
   if (a.items_per_thread == 0 && adj->lam_v == nullptr) return hipErrorInvalidValue;
   if (a.traj_v != nullptr && a.traj_steps < 1) return hipErrorInvalidValue;
+  if (a.coef_global > 1) return hipErrorInvalidValue;  // the symmetric system: forward only
   const Fused<G, L> f = unpack<G, L>(a);
   Adj<G> g;
   g.g_rec = static_cast<const G*>(adj->g_rec);

@@ -75,13 +75,14 @@ def _key(st, B, dtype, lsize, grad):
 
 
 def parse_config(text: str):
-    """"S,threads[,stream|gcoef]" (the JAXHIP_FUSED_CONFIG format)."""
+    """"S,threads[,stream|gcoef|sym]" (the JAXHIP_FUSED_CONFIG format)."""
     from .fused import LaunchConfig
 
     parts = [x.strip() for x in text.split(",")]
     S, thr = int(parts[0]), int(parts[1])
     flag = parts[2] if len(parts) > 2 else ""
-    return LaunchConfig(S, thr, stream=flag in ("stream", "gcoef"), gcoef=flag == "gcoef")
+    return LaunchConfig(S, thr, stream=flag in ("stream", "gcoef", "sym"), gcoef=flag == "gcoef",
+                        sym=flag == "sym")
 
 
 def fused_config(st, B: int, dtype, lsize: int = 0, grad: bool = False, measure=None):

@@ -449,6 +449,8 @@ class LaunchConfig:
     threads: int
     stream: bool = False       # streaming kernel even if the items fit the registers
     gcoef: bool = False        # streaming kernel reading the coupling coefficients from HBM / L2
+    sym: bool = False          # streaming kernel solving the row-scaled symmetric system (one
+                               # coefficient per node in LDS, the residual's pairs from L2)
 
 
 # Register-resident items per thread with a forward kernel, by the element sizes of
@@ -466,11 +468,13 @@ def sched_bytes(nseg: int, L: int, ncs: int) -> int:
     return (16 * nseg + 4 * (L + 2 + nseg + ncs) + 15) & ~15
 
 
-def lds_bytes(n: int, n_tgt: int, S: int, esize: int, sched: int, gcoef: bool = False) -> int:
+def lds_bytes(n: int, n_tgt: int, S: int, esize: int, sched: int, gcoef: bool = False,
+              sym: bool = False) -> int:
     """Forward: the staged schedule, lower / upper coefficient per node (unless read from
-    HBM / L2), diagonal and rhs per (node, simulation), synapse terms per (target,
-    simulation)."""
-    return sched + (0 if gcoef else 2 * n * esize) + (2 * n + 2 * n_tgt) * S * esize
+    HBM / L2; with `sym` one symmetric coefficient per node, rounded up to 16 bytes),
+    diagonal and rhs per (node, simulation), synapse terms per (target, simulation)."""
+    coefs = (n * esize + 15) // 16 * 16 if sym else 0 if gcoef else 2 * n * esize
+    return sched + coefs + (2 * n + 2 * n_tgt) * S * esize
 
 
 def items_per_thread(n: int, S: int, threads: int, gsize: int, lsize: int) -> int:
@@ -499,6 +503,7 @@ def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
         if S > 1 and S * 256 > B:
             break
         fits = lds_bytes(n, n_tgt, S, lsize, sched) <= 160 * 1024
+        fits_s = lds_bytes(n, n_tgt, S, lsize, sched, sym=True) <= 160 * 1024
         fits_g = lds_bytes(n, n_tgt, S, lsize, sched, gcoef=True) <= 160 * 1024
         if not fits_g:
             break
@@ -508,6 +513,10 @@ def config_candidates(n: int, n_tgt: int, B: int, gsize: int, lsize: int,
                     out.append(LaunchConfig(S, thr))
         thr = int(min(1024, max(64, 64 * -(-n * S // (64 * 16)))))
         for t in sorted({thr, 1024}):
+            if not fits and fits_s:
+                # Only the symmetric system's coefficients fit beside the system (a
+                # structure that cannot be symmetrized takes the L2 pairs instead).
+                out.append(LaunchConfig(S, t, stream=True, sym=True))
             out.append(LaunchConfig(S, t, stream=True, gcoef=not fits))
         S *= 2
     if not out:
@@ -585,7 +594,8 @@ def grad_config(n: int, B: int, esize: int, sched: int = 0, n_tgt: int = 0) -> L
     (coefficients from L2 when the LDS copy does not fit)."""
     S = adjoint_config(n, B, esize, sched, n_tgt=n_tgt).S
     fits = lds_bytes(n, n_tgt, S, esize, sched) <= 160 * 1024
-    return LaunchConfig(S, 1024, stream=True, gcoef=not fits)
+    sym = not fits and lds_bytes(n, n_tgt, S, esize, sched, sym=True) <= 160 * 1024
+    return LaunchConfig(S, 1024, stream=True, gcoef=not fits and not sym, sym=sym)
 
 
 # Share of the free device memory the voltage trajectory of a checkpointed forward may
@@ -599,6 +609,33 @@ def keep_trajectory(nbytes: int, device) -> bool:
         return False
     free, _ = torch.cuda.mem_get_info(device)
     return nbytes <= TRAJ_FRACTION * free
+
+
+def symmetric_rows(st: Structure, node_f: torch.Tensor) -> Optional[torch.Tensor]:
+    """(n, 2) per kernel node (s, w) of the row-scaled symmetric system, in node_f's
+    dtype and device, or None when the coupling cannot be symmetrized (an edge with
+    coefficients of opposite signs, or one zero and one not). Row k of the system is
+    scaled by w_k > 0: w = 1 at the roots and w_child = w_parent * upper / lower of the
+    child's edge (the lower coefficient couples the child's row to its parent, the upper
+    the parent's row to the child), so that s_child = w_child * lower = w_parent * upper
+    couples both rows. The products run root to leaf (depth by depth), every w is one
+    rounding from its parent's."""
+    f = node_f.detach().to("cpu", torch.float64).numpy()
+    lo, up = f[:, 0], f[:, 1]
+    par = st.parent_k
+    has = par >= 0
+    if np.any(has & ((lo == 0) != (up == 0))) or np.any(has & (lo * up < 0)):
+        return None
+    ratio = np.where(has & (lo != 0), up / np.where(lo != 0, lo, 1.0), 1.0)
+    # Root to leaf: every pass sets the nodes whose parent is set.
+    w = np.ones(len(par))
+    known = ~has
+    while not known.all():
+        ready = ~known & known[np.maximum(par, 0)]
+        w[ready] = w[par[ready]] * ratio[ready]
+        known |= ready
+    sym = np.stack([np.where(has, w * lo, 0.0), w], 1)
+    return torch.from_numpy(sym).to(device=node_f.device, dtype=node_f.dtype)
 
 
 class ForwardLaunch:
@@ -734,9 +771,22 @@ class ForwardLaunch:
         stream = cfg.stream or coop
         a.items_per_thread = 0 if stream else items_per_thread(
             nmax, S, cfg.threads, self.gm.element_size(), node_f.element_size())
-        gcoef = cfg.gcoef or (stream and lds_bytes(nmax, n_tgt, S, node_f.element_size(), st.sched) > 160 * 1024)
-        a.coef_global = int(stream and gcoef)
-        a.lds_bytes = lds_bytes(nmax, n_tgt, S, node_f.element_size(), st.sched, bool(a.coef_global))
+        esz = node_f.element_size()
+        fits = lds_bytes(nmax, n_tgt, S, esz, st.sched) <= 160 * 1024
+        # Where the solve finds the tree's coupling coefficients: the pairs in LDS, the
+        # symmetric system's coefficients in LDS (when asked for, or when only they fit
+        # beside the system), or the pairs in L2.
+        self.node_sym = None
+        if stream and not explicit_step and (cfg.sym or (not fits and not cfg.gcoef)) and \
+                lds_bytes(nmax, n_tgt, S, esz, st.sched, sym=True) <= 160 * 1024:
+            self.node_sym = symmetric_rows(st, node_f)
+        if self.node_sym is not None:
+            a.coef_global = 2
+            a.node_sym = self.node_sym.data_ptr()
+            a.lds_bytes = lds_bytes(nmax, n_tgt, S, esz, st.sched, sym=True)
+        else:
+            a.coef_global = int(stream and (cfg.gcoef or cfg.sym or not fits))
+            a.lds_bytes = lds_bytes(nmax, n_tgt, S, esz, st.sched, bool(a.coef_global))
         # Scratch of the streaming kernel: the axial part of every item's residual.
         self.work_ax = torch.empty_like(self.gm) if a.items_per_thread == 0 else None
         a.work_ax = self.work_ax.data_ptr() if self.work_ax is not None else 0

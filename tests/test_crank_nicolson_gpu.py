@@ -17,7 +17,7 @@ from oracle import jx_oracle as O
 pytestmark = pytest.mark.gpu
 
 CN = "crank_nicolson"
-SHAPES = ["8,1024,gcoef", "4,512,stream", "2,256", "1,1024"]
+SHAPES = ["8,1024,gcoef", "8,1024,sym", "4,512,stream", "2,256", "1,1024"]
 
 
 def spike_counts(v: np.ndarray) -> np.ndarray:

@@ -124,13 +124,31 @@ def test_c1_sweep_float32_conditioned(device, f32_mode):
     assert err[good].max() < (1e-2 if f32_mode == "mixed" else 0.15)
 
 
-def test_c1_tree_float64_vs_oracle(device):
+@pytest.mark.parametrize("shape", [None, "8,1024,sym", "8,1024,gcoef"])
+def test_c1_tree_float64_vs_oracle(device, shape, monkeypatch):
     """The real sweep tree at float64: four simulations of 2000 steps within 1e-6 mV of the
-    oracle, identical spike counts."""
+    oracle, identical spike counts; with the library's launch shape and with the sweep's
+    (8 simulations per program: the symmetric system's coefficients in LDS, or the pairs
+    from L2; 4 padding simulations)."""
+    from jaxhip import autotune
+    import jaxhip.fused as F
+
     B, T = 4, 2000
     prm = c1_params(B, SEED)
     cell, om = _c1_cell(False)
+    modes = []
+    if shape is not None:
+        monkeypatch.setenv("JAXHIP_FUSED_CONFIG", shape)
+        monkeypatch.setattr(autotune, "_CACHE", {})
+        orig = F.ForwardLaunch.__init__
+
+        def spy(self, *args, **kw):
+            orig(self, *args, **kw)
+            modes.append(self.args.coef_global)
+        monkeypatch.setattr(F.ForwardLaunch, "__init__", spy)
     v = _run(cell, prm, torch.float64, device, T)
+    if shape is not None:
+        assert modes == [2 if shape.endswith("sym") else 1], modes
     ref = O.simulate(om, None, DT, stim=torch.tensor(c1_stimulus(T))[None], stim_comps=[0], rec_comps=[0],
                      overrides=oracle_overrides(om, prm))[:, 0]
     err = float((v - ref).abs().max())

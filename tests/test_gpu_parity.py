@@ -69,14 +69,14 @@ def test_batched_random_tree(device, dtype, tol):
     assert spike_counts(ref.numpy()).sum() > 0
 
 
-SHAPES = ["8,1024,gcoef", "4,512,gcoef", "4,512,stream", "2,256", "1,1024", "8,512"]
+SHAPES = ["8,1024,gcoef", "8,1024,sym", "4,512,gcoef", "4,512,sym", "4,512,stream", "2,256", "1,1024", "8,512"]
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-6), (torch.float32, 1e-2)])
 def test_launch_shapes_match_oracle(device, dtype, tol, monkeypatch):
     """Kernels and launch shapes of the fused forward (streaming with the coupling
-    coefficients in LDS or read from L2, register-resident; B = 5 leaves padding
-    simulations in the last program) against the oracle."""
+    coefficients in LDS, read from L2 or the symmetric system's in LDS, register-resident;
+    B = 5 leaves padding simulations in the last program) against the oracle."""
     from jaxhip import autotune
 
     B, T, nb = 5, 300, 30
@@ -230,7 +230,7 @@ def test_adjoint_vs_central_differences(device):
     assert rel < 1e-2
 
 
-@pytest.mark.parametrize("shape", ["8,1024,gcoef", "2,256"])
+@pytest.mark.parametrize("shape", ["8,1024,gcoef", "8,1024,sym", "2,256"])
 def test_repeated_launches_bitwise_identical(device, shape, monkeypatch):
     """Deterministic like the reference's kernels (no atomics): the same simulation run
     three times gives bitwise identical traces, for the streaming and the register

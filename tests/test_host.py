@@ -497,3 +497,62 @@ def test_serve_batched_preparation_results():
     out = _prepare_batch(fn, [(1,), (2,), (3,)])
     assert [ok for ok, _ in out] == [True, False, True]
     assert out[0][1] == 10 and out[2][1] == 30 and isinstance(out[1][1], ValueError)
+
+
+@pytest.mark.parametrize("nbranch,seed", [(250, 11), (40, 3)])
+def test_symmetric_rows_scale_the_system_symmetric(nbranch, seed):
+    """The row scales of the symmetric solve: W A is symmetric (its couplings are the
+    table's s) and the scaled system has the solution of A (dense solves), on the
+    1000-compartment random tree and a smaller one, branchpoints included."""
+    from jaxhip import fused as F
+    from jaxhip.geometry import axial_conductances
+    from jaxhip.workloads import tree_cell
+
+    cell = tree_cell(nbranch, seed)[0]
+    cell.branch(0).comp(0).record()
+    rng = np.random.default_rng(seed)
+    nc = cell._n_comp
+    geo = {k: torch.tensor(rng.uniform(0.5, 2.0, nc), dtype=torch.float64) for k in ("area", "rl_in", "rl_out")}
+    ra = torch.tensor(rng.uniform(50, 5000, nc), dtype=torch.float64)
+    cm = torch.tensor(rng.uniform(0.5, 2.0, nc), dtype=torch.float64)
+    g = axial_conductances(cell._comp_edges, cell._comp_node, cell._n_nodes, geo, ra, cm)
+    st = F.build_structure(cell, np.array([0]), np.array([0]), torch.device("cpu"))
+    cm_node = torch.full((cell._n_nodes,), -1.0, dtype=torch.float64)
+    cm_node[torch.as_tensor(cell._comp_node)] = cm
+    nf = F.node_float_table(st, g, cm_node, 0.025, torch.float64).numpy()
+    sym = F.symmetric_rows(st, torch.from_numpy(nf)).numpy()
+    n, par = st.n, st.parent_k
+    A = np.diag(nf[:, 2] + rng.uniform(0.1, 1.0, n) * nf[:, 3])
+    ch = np.nonzero(par >= 0)[0]
+    A[ch, par[ch]] = nf[ch, 0]
+    A[par[ch], ch] = nf[ch, 1]
+    w, s = sym[:, 1], sym[:, 0]
+    assert (w > 0).all() and w[par < 0].tolist() == [1.0] * int((par < 0).sum())
+    WA = w[:, None] * A
+    off = WA[par[ch], ch]
+    np.testing.assert_allclose(WA[ch, par[ch]], s[ch], rtol=1e-15)
+    np.testing.assert_allclose(off, s[ch], rtol=4e-16)
+    b = rng.normal(size=n)
+    x = np.linalg.solve(A, b)
+    Ws = np.diag(WA.diagonal())
+    Ws[ch, par[ch]] = s[ch]
+    Ws[par[ch], ch] = s[ch]
+    # Both solves round differently; the system's condition (1e4 here) bounds the gap.
+    np.testing.assert_allclose(np.linalg.solve(Ws, w * b), x, rtol=1e-10)
+
+
+def test_symmetric_rows_refuse_couplings_of_opposite_sign():
+    from jaxhip import fused as F
+    from jaxhip.workloads import tree_cell
+
+    cell = tree_cell(6, 0)[0]
+    cell.branch(0).comp(0).record()
+    st = F.build_structure(cell, np.array([0]), np.array([0]), torch.device("cpu"))
+    nf = torch.zeros((st.n, 4), dtype=torch.float64)
+    nf[:, 0] = nf[:, 1] = -0.5
+    assert F.symmetric_rows(st, nf) is not None
+    k = int(np.nonzero(st.parent_k >= 0)[0][3])
+    nf[k, 1] = 0.5
+    assert F.symmetric_rows(st, nf) is None
+    nf[k, 1] = 0.0
+    assert F.symmetric_rows(st, nf) is None

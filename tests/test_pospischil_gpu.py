@@ -13,7 +13,7 @@ from oracle import jx_oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = ["8,1024,gcoef", "4,512,stream", "2,256", "1,1024"]
+SHAPES = ["8,1024,gcoef", "8,1024,sym", "4,512,stream", "2,256", "1,1024"]
 
 
 def _tree():

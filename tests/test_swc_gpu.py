@@ -90,7 +90,7 @@ def test_swc_launch_shapes_match_oracle(device, monkeypatch):
     ds = cell.soma.branch(0).loc(1.0).data_stimulate(cur)
     ref = O.simulate(oracle_of_swc([cell]), None, 0.025, stim=cur.cpu()[None], stim_comps=[comp],
                      rec_comps=rec, overrides={"HH_gNa": torch.tensor(gna)}).numpy()
-    for shape in ["8,1024,gcoef", "4,512,stream", "2,256", "1,1024"]:
+    for shape in ["8,1024,gcoef", "8,1024,sym", "4,512,stream", "2,256", "1,1024"]:
         monkeypatch.setenv("JAXHIP_FUSED_CONFIG", shape)
         autotune._CACHE.clear()
         v = jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=0.025).cpu().numpy()
