@@ -26,7 +26,12 @@ Pinning (what checks each part; tests/test_oracle.py, tests/test_host.py):
     one; the sigmoid's symmetry and midpoint; the finite-difference slope against the
     analytic derivative; whole two-cell networks through `simulate` against the
     closed-form backward-Euler recurrence of the postsynaptic membrane; the golden
-    network path through IonotropicSynapse exercises the same linearization.
+    network path through IonotropicSynapse exercises the same linearization;
+  * vectors written by the reference itself (tests/golden/reference_vectors.npz,
+    tests/test_reference_vectors.py): the DHS indexers of a 60-compartment cell, a
+    1000-compartment random tree and a 6-cell network (bit-exact), and 30 ms traces of a
+    ball-and-stick cell and of 6-cell networks with KineticSynapse / SigmoidRateSynapse
+    (equal bit for bit).
 """
 from __future__ import annotations
 

@@ -1,0 +1,162 @@
+"""Vectors written by jaxley-pytorch itself (tests/golden/make_reference_vectors.py,
+float64 on the CPU): the DHS solve indexers of the binary-tree cell, the 250-branch
+random tree and a 6-cell network, bit for bit; 30 ms traces of one ball-and-stick cell
+and of 6-cell ball-and-stick networks with KineticSynapse / SigmoidRateSynapse (spiking
+somata) through the oracle (CPU) and through the fused kernel and the per-step path
+(GPU)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import jaxhip as jx
+from jaxhip.workloads import binary_tree_parents, random_tree
+from oracle import jx_oracle as O
+
+REF = np.load(Path(__file__).resolve().parent / "golden" / "reference_vectors.npz")
+DT, T_MAX = float(REF["dt"]), float(REF["t_max"])
+N_CELLS = 6
+LENGTH = np.array([20.0] + [25.0] * 8)
+RADIUS = np.array([10.0] + [1.0] * 8)
+SYNAPSES = ["KineticSynapse", "SigmoidRateSynapse"]
+KEYS = ("map_to_solve_order", "inv_map_to_solve_order", "map_to_solve_order_lower",
+        "map_to_solve_order_upper", "node_order", "parent_lookup")
+
+
+def _ball_and_stick():
+    cell = jx.Cell([jx.Branch(jx.Compartment(), 1), jx.Branch(jx.Compartment(), 8)], [-1, 0])
+    cell.set("length", LENGTH)
+    cell.set("radius", RADIUS)
+    cell.branch(0).insert(jx.HH())
+    cell.branch(1).insert(jx.Leak())
+    return cell
+
+
+def _spec(synapse):
+    key = synapse.lower()
+    return {k: REF[f"{key}_{k}"] for k in ("pre", "tgt", "gS", "e_syn", "amp", "delay", "v")}
+
+
+def _assert_indexer(module, prefix):
+    ind = module._dhs_solve_indexer
+    for k in KEYS:
+        assert np.array_equal(np.asarray(ind[k]), REF[f"{prefix}_{k}"]), (prefix, k)
+    grouped = ind["node_order_grouped"]
+    assert [len(g) for g in grouped] == REF[f"{prefix}_grouped_sizes"].tolist()
+    assert np.array_equal(np.concatenate(grouped, 0), REF[f"{prefix}_grouped"])
+
+
+def test_indexer_binary_tree_cell():
+    cell = jx.Cell([jx.Branch(jx.Compartment(), 4)] * 15, binary_tree_parents(15))
+    _assert_indexer(cell, "c0")
+
+
+def test_indexer_random_tree():
+    parents, _, _ = random_tree(250, 11)
+    assert np.array_equal(parents, REF["c1_parents"])
+    cell = jx.Cell([jx.Branch(jx.Compartment(), 4)] * 250, parents)
+    _assert_indexer(cell, "c1")
+
+
+def test_indexer_network():
+    net = jx.Network([_ball_and_stick() for _ in range(N_CELLS)])
+    _assert_indexer(net, "net")
+
+
+def _network(synapse, sp, device=None, dtype=torch.float64):
+    net = jx.Network([_ball_and_stick() for _ in range(N_CELLS)])
+    cls = getattr(jx, synapse)
+    for post in range(N_CELLS):
+        for j in range(sp["pre"].shape[1]):
+            jx.connect(net.cell(int(sp["pre"][post, j])).branch(0).comp(0),
+                       net.cell(post).branch(1).comp(int(sp["tgt"][post])), cls())
+    getattr(net, synapse).set(f"{synapse}_gS", sp["gS"].reshape(-1))
+    if synapse == "KineticSynapse":
+        getattr(net, synapse).set(f"{synapse}_e_syn", sp["e_syn"].reshape(-1))
+    return net
+
+
+def _currents(sp):
+    # step_current builds the current in the default dtype (as the reference's, which
+    # wrote these vectors with float64 as the default).
+    prev = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    try:
+        return [jx.step_current(float(sp["delay"][c]), 15.0, float(sp["amp"][c]), DT, T_MAX)
+                for c in range(N_CELLS)]
+    finally:
+        torch.set_default_dtype(prev)
+
+
+@pytest.mark.parametrize("synapse", SYNAPSES)
+def test_oracle_network_traces(synapse):
+    """The oracle against the reference's traces (every recorded soma spikes twice)."""
+    sp = _spec(synapse)
+    net = _network(synapse, sp)
+    cls = getattr(jx, synapse)()
+    om = O.OracleModel([([-1, 0], [1, 8])] * N_CELLS)
+    om.length = np.tile(LENGTH, N_CELLS)
+    om.radius = np.tile(RADIUS, N_CELLS)
+    om.hh_comps = np.arange(N_CELLS) * 9
+    om.leak_comps = np.setdiff1d(np.arange(9 * N_CELLS), om.hh_comps)
+    edge = net._edges[0]
+    params = {name: np.asarray(edge["params"][f"{synapse}_{name}"], dtype=np.float64).copy()
+              for name in cls.param_names}
+    pre_comp = (sp["pre"] * 9).reshape(-1)
+    post_comp = np.repeat(np.arange(N_CELLS) * 9 + 1 + sp["tgt"], sp["pre"].shape[1])
+    s0 = np.zeros(len(pre_comp)) if synapse == "KineticSynapse" else None
+    om.synapses = [{"type": synapse, "pre": om.internal[pre_comp], "post": om.internal[post_comp],
+                    "params": params, "s0": s0}]
+    stim = torch.stack(_currents(sp))[None]
+    rec = list(np.arange(N_CELLS) * 9) + list(np.arange(N_CELLS) * 9 + 1 + sp["tgt"])
+    ref = O.simulate(om, T_MAX, DT, stim=stim, stim_comps=list(np.arange(N_CELLS) * 9), rec_comps=rec)[0]
+    err = float(np.abs(ref.numpy()[:, ::4] - sp["v"]).max())
+    print(synapse, "oracle vs reference vectors, max", err)
+    assert err < 1e-10   # measured 0: the oracle repeats the reference's operations
+
+
+def test_oracle_single_cell_trace():
+    """One ball-and-stick cell (HH soma, Leak dendrite), a step current into the soma."""
+    om = O.OracleModel([([-1, 0], [1, 8])])
+    om.length, om.radius = LENGTH.copy(), RADIUS.copy()
+    om.hh_comps, om.leak_comps = np.array([0]), np.arange(1, 9)
+    prev = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    try:
+        stim = jx.step_current(1.31, 15.0, 0.1536, DT, T_MAX)[None, None]
+    finally:
+        torch.set_default_dtype(prev)
+    ref = O.simulate(om, T_MAX, DT, stim=stim, stim_comps=[0], rec_comps=[0, 4])[0].numpy()[:, ::4]
+    err = float(np.abs(ref - REF["cell_v"]).max())
+    print("single cell, oracle vs reference vectors, max", err)
+    assert err < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", ["fused", "stepwise"])
+@pytest.mark.parametrize("synapse", SYNAPSES)
+def test_network_traces_gpu(device, synapse, path, monkeypatch):
+    """`integrate` on the GPU against the reference's traces: the fused kernel (synapse
+    step fused into the voltage and gate steps) and the per-step path (synapse and
+    tree-solve operators), float64, 1e-6 mV; identical spike counts."""
+    import sys
+
+    integ = sys.modules["jaxhip.integrate"]   # the module (the package exports the function)
+    sp = _spec(synapse)
+    net = _network(synapse, sp)
+    for c, cur in enumerate(_currents(sp)):
+        net.cell(c).branch(0).comp(0).stimulate(cur.to(device))
+    for c in range(N_CELLS):
+        net.cell(c).branch(0).comp(0).record()
+    for c in range(N_CELLS):
+        net.cell(c).branch(1).comp(int(sp["tgt"][c])).record()
+    if path == "stepwise":
+        monkeypatch.setattr(integ, "_per_step_path", lambda *a, **k: True)
+    v = jx.integrate(net, delta_t=DT, t_max=T_MAX).double().cpu().numpy()
+    v = v.reshape(-1, v.shape[-1])[:, ::4]
+    err = float(np.abs(v - sp["v"]).max())
+    print(synapse, path, "vs reference vectors, max", err)
+    assert err < 1e-6
+    up = lambda x: ((x[:, 1:] >= 0) & (x[:, :-1] < 0)).sum(1)
+    assert np.array_equal(up(v[:N_CELLS]), up(sp["v"][:N_CELLS])) and up(sp["v"][:N_CELLS]).min() > 0
