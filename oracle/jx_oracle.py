@@ -31,7 +31,8 @@ Pinning (what checks each part; tests/test_oracle.py, tests/test_host.py):
     tests/test_reference_vectors.py): the DHS indexers of a 60-compartment cell, a
     1000-compartment random tree and a 6-cell network (bit-exact), and 30 ms traces of a
     ball-and-stick cell and of 6-cell networks with KineticSynapse / SigmoidRateSynapse
-    (equal bit for bit).
+    (equal bit for bit), and the reference's autograd gradients of a tree's per-branch
+    HH conductances and of the Kinetic network's gS and gNa (3e-13).
 """
 from __future__ import annotations
 

@@ -11,7 +11,11 @@ reference's own tests holds a golden value for:
   SigmoidRateSynapse (3 synapses onto one dendritic compartment of every cell, ordered
   by postsynaptic cell: the reference's regular layout), with step currents into the
   somata that make the cells spike; 30 ms at dt 0.025 ms, stored every 4th step;
-* the same for one ball-and-stick cell without synapses (0.1536 nA from 1.31 ms).
+* the same for one ball-and-stick cell without synapses (0.1536 nA from 1.31 ms);
+* gradients of the reference's autograd: per-branch gNa / gK / gLeak of a 12-branch
+  random tree (2 parameter sets, MSE against a target trace, checkpoints every 50 steps)
+  and every gS and the somata's gNa of the KineticSynapse network (2 simulations,
+  a weighted sum of the somatic recordings, checkpoints every 100 steps).
 
 The inputs (structures, synapse ends, parameters, stimuli) are stored beside the
 outputs, so that the tests rebuild the same models. Run in the build container only:
@@ -98,6 +102,71 @@ def simulate_network(synapse: str, spec: dict):
     return net, v.numpy()
 
 
+def tree_gradients(out: dict):
+    """Per-branch trainable gNa / gK / gLeak of a 12-branch random tree (B = 2 parameter
+    sets), MSE of the root voltage against a target trace over 200 steps, checkpoints
+    every 50 steps: the loss and the gradients of the reference's autograd."""
+    from jaxhip.workloads import tree_cell as hip_tree_cell
+
+    nb, B, T, K = 12, 2, 200, 50
+    _, parents, comp_len, comp_rad = hip_tree_cell(nb, 7)
+    cell = rjx.Cell([rjx.Branch(rjx.Compartment(), 4)] * nb, np.asarray(parents))
+    cell.set("length", comp_len)
+    cell.set("radius", comp_rad)
+    cell.insert(HH())
+    cell.branch(0).comp(0).record(verbose=False)
+    keys = ("HH_gNa", "HH_gK", "HH_gLeak")
+    for k in keys:
+        cell.branch("all").make_trainable(k, verbose=False)
+    rng = np.random.default_rng(1)
+    vals = np.array([0.12, 0.036, 0.0003])[None, :, None] * rng.uniform(0.7, 1.3, size=(B, 3, nb))
+    params = [{k: torch.tensor(vals[:, j], requires_grad=True)} for j, k in enumerate(keys)]
+    cur = np.zeros(T)
+    cur[int(1.0 / DT):int(21.0 / DT)] = 0.2
+    ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur))
+    v = rjx.integrate(cell, params=params, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[T // K, K])
+    target = torch.linspace(-70, -60, T + 1)
+    loss = ((v[:, 0] - target) ** 2).mean()
+    loss.backward()
+    out["grad_tree_parents"] = np.asarray(parents, dtype=np.int64)
+    out["grad_tree_length"] = np.asarray(comp_len, dtype=np.float64)
+    out["grad_tree_radius"] = np.asarray(comp_rad, dtype=np.float64)
+    out["grad_tree_vals"] = vals
+    out["grad_tree_stim"] = cur
+    out["grad_tree_loss"] = np.array(loss.item())
+    out["grad_tree_grads"] = np.stack([p[k].grad.numpy() for p, k in zip(params, keys)], 1)
+
+
+def network_gradients(spec: dict, out: dict):
+    """The KineticSynapse network (B = 2): gradients of a weighted sum of the recordings
+    with respect to every synapse's gS (shared by the batch) and the somata's gNa (one
+    per simulation), checkpoints every 100 steps."""
+    net = rjx.Network([ball_and_stick() for _ in range(N_CELLS)])
+    for post in range(N_CELLS):
+        for j in range(K_IN):
+            connect(net.cell(int(spec["pre"][post, j])).branch(0).comp(0),
+                    net.cell(post).branch(1).comp(int(spec["tgt"][post])), KineticSynapse())
+    net.KineticSynapse.set("KineticSynapse_e_syn", spec["e_syn"].reshape(-1))
+    for c in range(N_CELLS):
+        cur = rjx.step_current(float(spec["delay"][c]), 15.0, float(spec["amp"][c]), DT, T_MAX)
+        net.cell(c).branch(0).comp(0).stimulate(cur, verbose=False)
+    for c in range(N_CELLS):
+        net.cell(c).branch(0).comp(0).record(verbose=False)
+    gs = torch.tensor(spec["gS"].reshape(-1), requires_grad=True)
+    gna = torch.tensor([[0.12], [0.1]], requires_grad=True)
+    ps = net.data_set("KineticSynapse_gS", gs, None)
+    ps = net.data_set("HH_gNa", gna, ps)
+    v = rjx.integrate(net, param_state=ps, delta_t=DT, t_max=T_MAX, checkpoint_lengths=[12, 100])
+    # The weights: numpy's seeded generator (the tests draw the same ones).
+    w = np.random.default_rng(5).normal(size=tuple(v.shape))
+    (v * torch.tensor(w)).sum().backward()
+    out["grad_net_w_seed_shape"] = np.array([5, *v.shape], dtype=np.int64)
+    out["grad_net_gna"] = gna.detach().numpy()
+    out["grad_net_grad_gS"] = gs.grad.numpy()
+    out["grad_net_grad_gNa"] = gna.grad.numpy()
+    out["grad_net_v"] = v.detach().numpy()[:, :, ::4]
+
+
 def indexer_arrays(module, prefix: str, out: dict):
     ind = module._dhs_solve_indexer
     for k in INDEXER_KEYS:
@@ -136,6 +205,8 @@ def main():
               [int(((v[i, 1:] >= 0) & (v[i, :-1] < 0)).sum()) for i in range(N_CELLS)])
         if synapse == "KineticSynapse":
             indexer_arrays(net, "net", out)
+            network_gradients(spec, out)
+    tree_gradients(out)
     out["dt"] = np.array(DT)
     out["t_max"] = np.array(T_MAX)
     np.savez_compressed(HERE / "reference_vectors.npz", **out)

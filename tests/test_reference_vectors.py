@@ -89,10 +89,8 @@ def _currents(sp):
         torch.set_default_dtype(prev)
 
 
-@pytest.mark.parametrize("synapse", SYNAPSES)
-def test_oracle_network_traces(synapse):
-    """The oracle against the reference's traces (every recorded soma spikes twice)."""
-    sp = _spec(synapse)
+def _oracle_network(synapse, sp):
+    """The oracle's model of the network (synapse parameters of the built network)."""
     net = _network(synapse, sp)
     cls = getattr(jx, synapse)()
     om = O.OracleModel([([-1, 0], [1, 8])] * N_CELLS)
@@ -108,6 +106,14 @@ def test_oracle_network_traces(synapse):
     s0 = np.zeros(len(pre_comp)) if synapse == "KineticSynapse" else None
     om.synapses = [{"type": synapse, "pre": om.internal[pre_comp], "post": om.internal[post_comp],
                     "params": params, "s0": s0}]
+    return om
+
+
+@pytest.mark.parametrize("synapse", SYNAPSES)
+def test_oracle_network_traces(synapse):
+    """The oracle against the reference's traces (every recorded soma spikes twice)."""
+    sp = _spec(synapse)
+    om = _oracle_network(synapse, sp)
     stim = torch.stack(_currents(sp))[None]
     rec = list(np.arange(N_CELLS) * 9) + list(np.arange(N_CELLS) * 9 + 1 + sp["tgt"])
     ref = O.simulate(om, T_MAX, DT, stim=stim, stim_comps=list(np.arange(N_CELLS) * 9), rec_comps=rec)[0]
@@ -160,3 +166,138 @@ def test_network_traces_gpu(device, synapse, path, monkeypatch):
     assert err < 1e-6
     up = lambda x: ((x[:, 1:] >= 0) & (x[:, :-1] < 0)).sum(1)
     assert np.array_equal(up(v[:N_CELLS]), up(sp["v"][:N_CELLS])) and up(sp["v"][:N_CELLS]).min() > 0
+
+
+# ---- gradients of the reference's autograd ----
+
+GT_KEYS = ("HH_gNa", "HH_gK", "HH_gLeak")
+
+
+def _grad_tree_cell():
+    nb = len(REF["grad_tree_parents"])
+    cell = jx.Cell([jx.Branch(jx.Compartment(), 4)] * nb, REF["grad_tree_parents"])
+    cell.set("length", REF["grad_tree_length"])
+    cell.set("radius", REF["grad_tree_radius"])
+    cell.insert(jx.HH())
+    cell.branch(0).comp(0).record()
+    for k in GT_KEYS:
+        cell.branch("all").make_trainable(k, verbose=False)
+    return cell
+
+
+def test_oracle_tree_gradients():
+    """Autograd through the oracle against the reference's autograd: per-branch HH
+    conductances of two parameter sets, MSE of the root voltage."""
+    vals = REF["grad_tree_vals"]
+    parents, nb = REF["grad_tree_parents"], vals.shape[-1]
+    om = O.OracleModel([(parents, [4] * nb)])
+    om.length, om.radius = REF["grad_tree_length"].copy(), REF["grad_tree_radius"].copy()
+    om.hh_comps = np.arange(4 * nb)
+    ovals = torch.tensor(vals, requires_grad=True)
+    comp = torch.repeat_interleave(ovals, 4, dim=-1)
+    ov = {k: comp[:, j] for j, k in enumerate(GT_KEYS)}
+    stim = torch.tensor(REF["grad_tree_stim"])[None]
+    T = stim.shape[-1]
+    v = O.simulate(om, None, DT, stim=stim, stim_comps=[0], rec_comps=[0], overrides=ov)
+    loss = ((v[:, 0] - torch.linspace(-70, -60, T + 1, dtype=torch.float64)) ** 2).mean()
+    loss.backward()
+    np.testing.assert_allclose(loss.item(), float(REF["grad_tree_loss"]), rtol=1e-12)
+    rel = np.abs(ovals.grad.numpy() - REF["grad_tree_grads"]).max() / np.abs(REF["grad_tree_grads"]).max()
+    print("oracle tree gradients vs the reference's autograd, max relative to the largest", rel)
+    assert rel < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,rtol", [(torch.float64, 1e-6), (torch.float32, 1e-2)])
+def test_tree_gradients_gpu(device, dtype, rtol):
+    """The hand-written reverse pass (fused kernel, checkpoints every 50 steps) against
+    the reference's autograd; float32 runs the library's default ("mixed") mode."""
+    cell = _grad_tree_cell()
+    vals = REF["grad_tree_vals"]
+    params = [{k: torch.tensor(vals[:, j], dtype=dtype, device=device, requires_grad=True)}
+              for j, k in enumerate(GT_KEYS)]
+    stim = torch.tensor(REF["grad_tree_stim"], dtype=dtype, device=device)
+    T = stim.shape[-1]
+    ds = cell.branch(0).comp(0).data_stimulate(stim)
+    v = jx.integrate(cell, params, data_stimuli=ds, delta_t=DT, checkpoint_lengths=[T // 50, 50])
+    target = torch.linspace(-70, -60, T + 1, dtype=torch.float64, device=device)
+    loss = ((v[:, 0].double() - target) ** 2).mean()
+    loss.backward()
+    g = np.stack([list(p.values())[0].grad.double().cpu().numpy() for p in params], 1)
+    ref = REF["grad_tree_grads"]
+    rel = np.abs(g - ref).max() / np.abs(ref).max()
+    print(dtype, "tree gradients vs the reference's autograd, max relative to the largest", rel)
+    np.testing.assert_allclose(loss.item(), float(REF["grad_tree_loss"]), rtol=1e-9 if dtype == torch.float64 else 1e-3)
+    assert rel < rtol
+    if dtype == torch.float64:
+        np.testing.assert_allclose(g, ref, rtol=1e-6, atol=1e-9 * np.abs(ref).max())
+
+
+def _grad_net_inputs(device):
+    sp = _spec("KineticSynapse")
+    net = jx.Network([_ball_and_stick() for _ in range(N_CELLS)])
+    for post in range(N_CELLS):
+        for j in range(sp["pre"].shape[1]):
+            jx.connect(net.cell(int(sp["pre"][post, j])).branch(0).comp(0),
+                       net.cell(post).branch(1).comp(int(sp["tgt"][post])), jx.KineticSynapse())
+    net.KineticSynapse.set("KineticSynapse_e_syn", sp["e_syn"].reshape(-1))
+    for c, cur in enumerate(_currents(sp)):
+        net.cell(c).branch(0).comp(0).stimulate(cur.to(device) if device is not None else cur)
+    for c in range(N_CELLS):
+        net.cell(c).branch(0).comp(0).record()
+    seed, *shape = REF["grad_net_w_seed_shape"].tolist()
+    w = np.random.default_rng(seed).normal(size=tuple(shape))
+    return net, sp, w
+
+
+def test_oracle_network_gradients():
+    """Autograd through the oracle against the reference's: every gS and the somata's
+    gNa of the KineticSynapse network, two simulations."""
+    sp = _spec("KineticSynapse")
+    om = _oracle_network("KineticSynapse", sp)
+    gs = torch.tensor(sp["gS"].reshape(-1), requires_grad=True)
+    om.synapses[0]["params"]["gS"] = gs
+    gna = torch.tensor(REF["grad_net_gna"], requires_grad=True)
+    soma = list(np.arange(N_CELLS) * 9)
+    stim = torch.stack(_currents(sp))[None]
+    v = O.simulate(om, T_MAX, DT, stim=stim, stim_comps=soma, rec_comps=soma,
+                   overrides={"HH_gNa": gna.expand(2, len(om.internal))})
+    seed, *shape = REF["grad_net_w_seed_shape"].tolist()
+    w = np.random.default_rng(seed).normal(size=tuple(shape))
+    (v * torch.tensor(w)).sum().backward()
+    err_v = float(np.abs(v.detach().numpy()[:, :, ::4] - REF["grad_net_v"]).max())
+    e1 = np.abs(gs.grad.numpy() - REF["grad_net_grad_gS"]).max() / np.abs(REF["grad_net_grad_gS"]).max()
+    e2 = np.abs(gna.grad.numpy() - REF["grad_net_grad_gNa"]).max() / np.abs(REF["grad_net_grad_gNa"]).max()
+    print(f"oracle network vs the reference: v {err_v:.2e} mV, gS {e1:.2e}, gNa {e2:.2e}")
+    assert err_v < 1e-10 and e1 < 1e-10 and e2 < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", ["fused", "stepwise"])
+def test_network_gradients_gpu(device, path, monkeypatch):
+    """Every gS and the somata's gNa of the KineticSynapse network against the
+    reference's autograd: the fused reverse pass (synapse adjoint in the kernel) and the
+    per-step path."""
+    import sys
+
+    integ = sys.modules["jaxhip.integrate"]
+    net, sp, w = _grad_net_inputs(device)
+    gs = torch.tensor(sp["gS"].reshape(-1), dtype=torch.float64, device=device, requires_grad=True)
+    gna = torch.tensor(REF["grad_net_gna"], dtype=torch.float64, device=device, requires_grad=True)
+    ps = net.data_set("KineticSynapse_gS", gs, None)
+    ps = net.data_set("HH_gNa", gna, ps)
+    if path == "stepwise":
+        monkeypatch.setattr(integ, "_per_step_path", lambda *a, **k: True)
+    else:
+        import jaxhip.stepwise as stepwise
+
+        def refuse(*a, **k):
+            raise AssertionError("network gradients took the per-step path")
+        monkeypatch.setattr(stepwise, "integrate_stepwise", refuse)
+    v = jx.integrate(net, param_state=ps, delta_t=DT, t_max=T_MAX, checkpoint_lengths=[12, 100])
+    (v.cpu() * torch.tensor(w)).sum().backward()
+    err_v = float(np.abs(v.detach().cpu().numpy()[:, :, ::4] - REF["grad_net_v"]).max())
+    e1 = np.abs(gs.grad.cpu().numpy() - REF["grad_net_grad_gS"]).max() / np.abs(REF["grad_net_grad_gS"]).max()
+    e2 = np.abs(gna.grad.cpu().numpy() - REF["grad_net_grad_gNa"]).max() / np.abs(REF["grad_net_grad_gNa"]).max()
+    print(path, f"network vs the reference: v {err_v:.2e} mV, gS {e1:.2e}, gNa {e2:.2e}")
+    assert err_v < 1e-6 and e1 < 1e-6 and e2 < 1e-6
