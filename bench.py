@@ -537,6 +537,15 @@ def main():
                      "bytes_per_comp_step": bpc},
     }
     rec64 = run64()
+    # The first simulations of the sweep against vectors the reference itself computed
+    # for the same inputs (its first four parameter sets, float64 on the CPU).
+    vec = Path(__file__).resolve().parent / "tests" / "golden" / "reference_vectors.npz"
+    if vec.exists() and B >= 4 and T == 2000:
+        rv = np.load(vec)["c1_v"]
+        out["max_err_f64_vs_reference_vectors_mV"] = float(
+            np.abs(rec64[:4].cpu().numpy()[:, :, ::4] - rv).max())
+        log(f"[bench] float64 GPU vs the reference's own vectors (4 simulations): "
+            f"{out['max_err_f64_vs_reference_vectors_mV']:.3e} mV")
     if want("f32"):
         # ---- float32 modes vs float64 on the GPU (all simulations) ----
         cell32, ps32, ds32, _ = setup(B, T, torch.float32, device)

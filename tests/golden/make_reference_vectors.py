@@ -12,6 +12,9 @@ reference's own tests holds a golden value for:
   by postsynaptic cell: the reference's regular layout), with step currents into the
   somata that make the cells spike; 30 ms at dt 0.025 ms, stored every 4th step;
 * the same for one ball-and-stick cell without synapses (0.1536 nA from 1.31 ms);
+* float64 traces of the 60-compartment binary-tree cell (10 ms, all compartments) and
+  of four simulations of the 250-branch random tree (2000 steps, the root), every 4th
+  step;
 * gradients of the reference's autograd: per-branch gNa / gK / gLeak of a 12-branch
   random tree (2 parameter sets, MSE against a target trace, checkpoints every 50 steps)
   and every gS and the somata's gNa of the KineticSynapse network (2 simulations,
@@ -167,6 +170,40 @@ def network_gradients(spec: dict, out: dict):
     out["grad_net_v"] = v.detach().numpy()[:, :, ::4]
 
 
+def config_traces(out: dict):
+    """The 60-compartment binary-tree cell (HH, 2.5 um compartments, 0.1 nA into
+    compartment 0 from 1 to 2 ms, 10 ms, every compartment recorded) and four
+    simulations of the 250-branch random tree of the batched sweep (per-simulation gNa,
+    gK, gLeak; 0.2 nA into the root from 1 to 21 ms; 2000 steps; root recorded)."""
+    from jaxhip.workloads import c1_params, tree_cell as hip_tree_cell
+
+    c0 = rjx.Cell([rjx.Branch(rjx.Compartment(), 4)] * 15, np.asarray(binary_tree_parents(15)))
+    c0.set("length", 2.5)
+    c0.insert(HH())
+    c0.record(verbose=False)
+    c0.branch(0).comp(0).stimulate(rjx.step_current(1.0, 1.0, 0.1, DT, 10.0), verbose=False)
+    with torch.no_grad():
+        out["c0_v"] = rjx.integrate(c0, delta_t=DT, t_max=10.0).numpy()[:, ::4]
+    _, parents, comp_len, comp_rad = hip_tree_cell(250, 11)
+    c1 = rjx.Cell([rjx.Branch(rjx.Compartment(), 4)] * 250, np.asarray(parents))
+    c1.set("length", comp_len)
+    c1.set("radius", comp_rad)
+    c1.insert(HH())
+    c1.branch(0).comp(0).record(verbose=False)
+    prm = c1_params(4, 11)
+    ps = None
+    for k, v in prm.items():
+        ps = c1.data_set(k, torch.tensor(v), ps)
+    cur = np.zeros(2000)
+    cur[int(1.0 / DT):int(21.0 / DT)] = 0.2
+    ds = c1.branch(0).comp(0).data_stimulate(torch.tensor(cur))
+    with torch.no_grad():
+        v = rjx.integrate(c1, param_state=ps, data_stimuli=ds, delta_t=DT)
+    out["c1_v"] = v.numpy()[:, :, ::4]
+    out["c1_params"] = np.concatenate([prm[k] for k in ("HH_gNa", "HH_gK", "HH_gLeak")], 1)
+    print("c1 spikes per simulation", [int(((x[0, 1:] >= 0) & (x[0, :-1] < 0)).sum()) for x in v.numpy()])
+
+
 def indexer_arrays(module, prefix: str, out: dict):
     ind = module._dhs_solve_indexer
     for k in INDEXER_KEYS:
@@ -207,6 +244,7 @@ def main():
             indexer_arrays(net, "net", out)
             network_gradients(spec, out)
     tree_gradients(out)
+    config_traces(out)
     out["dt"] = np.array(DT)
     out["t_max"] = np.array(T_MAX)
     np.savez_compressed(HERE / "reference_vectors.npz", **out)

@@ -301,3 +301,84 @@ def test_network_gradients_gpu(device, path, monkeypatch):
     e2 = np.abs(gna.grad.cpu().numpy() - REF["grad_net_grad_gNa"]).max() / np.abs(REF["grad_net_grad_gNa"]).max()
     print(path, f"network vs the reference: v {err_v:.2e} mV, gS {e1:.2e}, gNa {e2:.2e}")
     assert err_v < 1e-6 and e1 < 1e-6 and e2 < 1e-6
+
+
+# ---- the configurations' cells: 60-compartment binary tree, 1000-compartment sweep tree ----
+
+def _c0_cell():
+    cell = jx.Cell([jx.Branch(jx.Compartment(), 4)] * 15, binary_tree_parents(15))
+    cell.set("length", 2.5)
+    cell.insert(jx.HH())
+    cell.record()
+    return cell
+
+
+def _c0_current():
+    prev = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    try:
+        return jx.step_current(1.0, 1.0, 0.1, DT, 10.0)
+    finally:
+        torch.set_default_dtype(prev)
+
+
+def _c1_cell():
+    from jaxhip.workloads import tree_cell
+
+    cell = tree_cell(250, 11)[0]
+    cell.branch(0).comp(0).record()
+    return cell
+
+
+def _c1_stim():
+    cur = np.zeros(2000)
+    cur[int(1.0 / DT):int(21.0 / DT)] = 0.2
+    return cur
+
+
+def test_oracle_config_traces():
+    """The oracle against the reference on the 60-compartment cell (all compartments)
+    and on four simulations of the sweep tree (2000 steps)."""
+    from models import build_c0, build_tree, oracle_overrides
+
+    _, om = build_c0()
+    ref = O.simulate(om, 10.0, DT, stim=_c0_current()[None, None], stim_comps=[0], rec_comps=list(range(60)))
+    e0 = float(np.abs(ref.numpy()[0][:, ::4] - REF["c0_v"]).max())
+    _, om1 = build_tree(250, 11)
+    p = REF["c1_params"]
+    prm = {"HH_gNa": p[:, 0:1], "HH_gK": p[:, 1:2], "HH_gLeak": p[:, 2:3]}
+    ref1 = O.simulate(om1, None, DT, stim=torch.tensor(_c1_stim())[None], stim_comps=[0], rec_comps=[0],
+                      overrides=oracle_overrides(om1, prm))
+    e1 = float(np.abs(ref1.numpy()[:, :, ::4] - REF["c1_v"]).max())
+    print("oracle vs reference: 60-compartment cell", e0, "sweep tree", e1)
+    assert e0 < 1e-10 and e1 < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [None, "8,1024,sym", "1,1024"])
+def test_config_traces_gpu(device, shape, monkeypatch):
+    """The fused forward against the reference: the 60-compartment cell and four
+    simulations of the sweep tree (2000 steps; the library's launch shape, the sweep's
+    8 simulations per program with the symmetric system, one per program), float64,
+    1e-6 mV, identical spike counts."""
+    from jaxhip import autotune
+    from jaxhip.workloads import data_set_params
+
+    if shape is not None:
+        monkeypatch.setenv("JAXHIP_FUSED_CONFIG", shape)
+    monkeypatch.setattr(autotune, "_CACHE", {})
+    c0 = _c0_cell()
+    c0.branch(0).comp(0).stimulate(_c0_current().to(device))
+    v0 = jx.integrate(c0, delta_t=DT, t_max=10.0).cpu().numpy()[:, ::4]
+    c1 = _c1_cell()
+    p = REF["c1_params"]
+    prm = {"HH_gNa": p[:, 0:1], "HH_gK": p[:, 1:2], "HH_gLeak": p[:, 2:3]}
+    ps = data_set_params(c1, prm, device, torch.float64)
+    ds = c1.branch(0).comp(0).data_stimulate(torch.tensor(_c1_stim(), dtype=torch.float64, device=device))
+    v1 = jx.integrate(c1, param_state=ps, data_stimuli=ds, delta_t=DT).cpu().numpy()[:, :, ::4]
+    e0 = float(np.abs(v0 - REF["c0_v"]).max())
+    e1 = float(np.abs(v1 - REF["c1_v"]).max())
+    print(shape, "vs reference: 60-compartment cell", e0, "sweep tree", e1)
+    assert e0 < 1e-6 and e1 < 1e-6
+    up = lambda x: ((x[..., 1:] >= 0) & (x[..., :-1] < 0)).sum(-1)
+    assert np.array_equal(up(v1), up(REF["c1_v"])) and up(REF["c1_v"]).sum() > 0
