@@ -293,7 +293,10 @@ def set_float32_mode(mode: str) -> None:
     compartments, 2000 steps) against float64 on the same float32 inputs: mixed max
     0.038 mV (3 simulations above 1e-2 mV), fast max 0.21 mV (129 above 1e-2 mV). Those
     simulations sit near a spike threshold: rounding the inputs themselves to float32
-    moves the float64 trajectories by up to 0.023 mV."""
+    moves the float64 trajectories by up to 0.023 mV. jaxley-pytorch's own float32
+    simulation of 138 of them (those above, and 8 others) ends up to 0.86 mV (median
+    0.094 mV) from its float64 one; on those the mixed mode stays within 0.022 mV
+    (median 0.0002) and the fast mode within 0.19 mV (median 0.009)."""
     global _F32_MODE
     if mode not in ("mixed", "fast"):
         raise ValueError(f"unknown float32 mode {mode!r}")

@@ -382,3 +382,63 @@ def test_config_traces_gpu(device, shape, monkeypatch):
     assert e0 < 1e-6 and e1 < 1e-6
     up = lambda x: ((x[..., 1:] >= 0) & (x[..., :-1] < 0)).sum(-1)
     assert np.array_equal(up(v1), up(REF["c1_v"])) and up(REF["c1_v"]).sum() > 0
+
+
+# ---- float32: the reference's own float32 simulation of the sweep ----
+
+def test_reference_float32_misses_1e2_itself():
+    """The reference's own float32 simulation of 138 sweep simulations (the ones whose
+    float32 GPU run missed 1e-2 mV in one measurement, and the first eight) against its
+    float64 simulation of the same float32 inputs: the tolerance is missed by the
+    reference itself on most of them (controls included), while the spike counts agree."""
+    e = REF["f32_sweep_err"]
+    print("reference float32 vs float64: max", e.max(), "median", np.median(e), "over 1e-2:", (e > 1e-2).sum(),
+          "of", len(e), "controls", np.round(e[:8], 4).tolist())
+    assert (e > 1e-2).sum() > len(e) // 2 and e[:8].max() > 1e-2
+    assert np.array_equal(REF["f32_sweep_spikes32"], REF["f32_sweep_spikes64"])
+
+
+@pytest.mark.gpu
+def test_float32_modes_vs_reference_float32(device):
+    """The library's float32 modes on those simulations against its float64 run of the
+    same float32 inputs, beside the reference's own float32 error on each: the largest
+    and the median error of both modes stay below the reference's float32 ones; spike
+    counts equal float64's."""
+    from jaxhip.integrate import set_float32_mode
+    from jaxhip.workloads import c1_params
+
+    idx = REF["f32_sweep_sims"]
+    parents, blen, rad = random_tree(250, 11)
+    L32 = np.repeat(blen / 4, 4).astype(np.float32).astype(np.float64)
+    R32 = np.repeat(rad, 4).astype(np.float32).astype(np.float64)
+    prm = {k: v[idx].astype(np.float32) for k, v in c1_params(4096, 11).items()}
+    cur = np.zeros(2000, dtype=np.float32)
+    cur[int(1.0 / DT):int(21.0 / DT)] = np.float32(0.2)
+
+    def run(dtype):
+        cell = jx.Cell([jx.Branch(jx.Compartment(), 4)] * 250, parents)
+        cell.set("length", L32)
+        cell.set("radius", R32)
+        cell.insert(jx.HH())
+        cell.branch(0).comp(0).record()
+        ps = None
+        for k, v in prm.items():
+            ps = cell.data_set(k, torch.tensor(v, dtype=dtype, device=device), ps)
+        ds = cell.branch(0).comp(0).data_stimulate(torch.tensor(cur, dtype=dtype, device=device))
+        with torch.no_grad():
+            return jx.integrate(cell, param_state=ps, data_stimuli=ds, delta_t=DT).double().cpu().numpy()
+
+    v64 = run(torch.float64)
+    ref_e = REF["f32_sweep_err"]
+    up = lambda v: ((v[..., 1:] >= 0) & (v[..., :-1] < 0)).sum(-1)[:, 0]
+    try:
+        for mode in ("mixed", "fast"):
+            set_float32_mode(mode)
+            e = np.abs(run(torch.float32) - v64).max(axis=(1, 2))
+            print(f"{mode}: max {e.max():.4f} median {np.median(e):.5f} over 1e-2 {(e > 1e-2).sum()} of {len(e)}; "
+                  f"reference float32: max {ref_e.max():.4f} median {np.median(ref_e):.5f}; "
+                  f"below the reference's on {(e <= ref_e).sum()}")
+            assert e.max() < ref_e.max() and np.median(e) < np.median(ref_e)
+            assert np.array_equal(up(run(torch.float32)), up(v64))
+    finally:
+        set_float32_mode("mixed")
