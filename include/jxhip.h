@@ -241,12 +241,14 @@ typedef struct jxhip_fused_args {
   void* rec;
   /* synapses in jagged-diagonal order: targets numbered by non-increasing in-degree; the
      j-th synapse of target t is syn_ptr[j] + t while t < syn_ptr[j+1] - syn_ptr[j] (the
-     pointer list ends with a repeated entry) */
+     pointer list ends with 8 entries equal to the topology's synapse count: the forward
+     walks a target's synapses with up to 8 threads, each reading up to 7 entries past
+     the target's last one) */
   int32_t syn_model;        /* -1 none, 0 kinetic, 1 sigmoid rate, 2 ionotropic, 3 test
                                synapse, JXHIP_SYN_MIXED: per synapse in syn_kind */
   const int32_t* syn_pre;   /* presynaptic node (of the group's nodes: the topology's own
                                unless the network is split), at syn_base */
-  const int32_t* syn_ptr;   /* max in-degree + 2 per topology at syn_ptr_base */
+  const int32_t* syn_ptr;   /* max in-degree + 9 per topology at syn_ptr_base */
   const int32_t* tgt_node;  /* topology-relative target node, at tgt_base */
   const void* syn_param[JXHIP_NSYNPARAM]; /* per synapse at syn_base */
   const void* syn_scale;    /* 1e5 / area(post) per synapse at syn_base */

@@ -244,17 +244,20 @@ __device__ __forceinline__ G prm(const Fused<G, L>& a, const Prog& pg, const Top
 // reference's finite difference with both voltages perturbed by `diff`), `cur` = summed
 // i / area * 1e5 at the unperturbed voltages.
 // `v`: the program's voltages (program layout); `s_in` / `s_out`: the program's synapse
-// states before / after the step ([syn][S]; the same array in the forward).
+// states before / after the step ([syn][S]; the same array in the forward). The target's
+// synapses j0, j0 + jstep, ... (jstep <= 8: the pointer list's 8 repeated end entries
+// keep every read of it in range).
 template <typename G, typename L, typename V>
 __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog& pg,
                                                const Topo& tp, int tgt, int s, G& lin, G& cur,
                                                const V* vpre_g, const V* v,
-                                               const G* s_in, G* s_out) {
+                                               const G* s_in, G* s_out, int j0 = 0,
+                                               int jstep = 1) {
   G sl = G(0), sc = G(0);
   const G dt = a.dt;
   const int32_t* jdp = a.syn_ptr + tp.spb;
   const int S = a.S;
-  for (int j = 0;; ++j) {
+  for (int j = j0;; j += jstep) {
     // The target's j-th synapse (jagged-diagonal order: consecutive targets, consecutive
     // synapses).
     const int beg = jdp[j];
@@ -941,20 +944,47 @@ __device__ __forceinline__ void idle_item(const Fused<G, L>& a, const Topo& tp, 
 // Synapse step of every (target, simulation) of a program; the presynaptic voltages are
 // read from the group block vpre_g (the program's own block of a.v, or the exported
 // voltages of all parts of a split network).
+// log2 of the threads per (target, simulation) pair of a synapse pass over `pairs` pairs:
+// up to 8 while the pairs still take one pass of the workgroup.
+__device__ __forceinline__ int syn_lanes_log2(int pairs) {
+  int l = 0;
+  while (l < 3 && (pairs << (l + 1)) <= static_cast<int>(blockDim.x)) ++l;
+  return l;
+}
+
+// Sum of x over the 2^l adjacent lanes of a pair, in a fixed order (every lane of the
+// group ends with the same bits: each addition pairs the same two operands).
+template <typename T>
+__device__ __forceinline__ T pair_sum(T x, int l) {
+  for (int o = 1; o < (1 << l); o <<= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
 template <typename G, typename L>
 __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
                                               const Lds<L>& Ld, const double* vpre_g) {
   if (a.syn_model < 0) return;
   const int lS = a.log2S, S = a.S;
-  for (int i = threadIdx.x; i < (tp.n_tgt << lS); i += blockDim.x) {
-    const int tgt = i >> lS, s = i & (S - 1);
+  const int nts = tp.n_tgt << lS;
+  // Threads per (target, simulation): thread `sub` of a pair walks the target's synapses
+  // sub, sub + nsub, ... (a target's synapses are walked one after the other, each
+  // waiting for its loads).
+  const int lsub = syn_lanes_log2(nts);
+  const int nsub = 1 << lsub;
+  for (int i = threadIdx.x; i < (nts << lsub); i += blockDim.x) {
+    const int pr = i >> lsub, sub = i & (nsub - 1);
+    const int tgt = pr >> lS, s = pr & (S - 1);
     G lin = G(0), cur = G(0);
     if (s < pg.nsim) {
       G* st = a.syn_s + pg.synoff;
-      synapse_target(a, pg, tp, tgt, s, lin, cur, vpre_g, a.v + pg.soff, st, st);
+      synapse_target(a, pg, tp, tgt, s, lin, cur, vpre_g, a.v + pg.soff, st, st, sub, nsub);
     }
-    Ld.tl[i] = static_cast<L>(lin);
-    Ld.tc[i] = static_cast<L>(cur);
+    lin = pair_sum(lin, lsub);
+    cur = pair_sum(cur, lsub);
+    if (sub == 0) {
+      Ld.tl[pr] = static_cast<L>(lin);
+      Ld.tc[pr] = static_cast<L>(cur);
+    }
   }
   __syncthreads();
 }
@@ -1593,16 +1623,20 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     if constexpr (kSyn) {
       if (traj) {
         // Synapse states and synaptic slopes of every step of the segment, per (target,
-        // simulation) over the steps, the voltages from the trajectory.
-        for (int i2 = tid; i2 < ntS; i2 += nthr) {
+        // simulation) over the steps, the voltages from the trajectory; the synapses of a
+        // pair split over its lanes as in the forward.
+        const int lsub = syn_lanes_log2(ntS);
+        for (int ii = tid; ii < (ntS << lsub); ii += nthr) {
+          const int i2 = ii >> lsub, sub = ii & ((1 << lsub) - 1);
           const int tgt = i2 >> lS, s = i2 & (S - 1);
 #pragma unroll 1
           for (int64_t j = 0; j < te - ts; ++j) {
             G lin = G(0), cur = G(0);
             if (s < pg.nsim)
               synapse_target(a, pg, tp, tgt, s, lin, cur, tvg + (ts + j) * rowS, tv + (ts + j) * rowS,
-                             ws + j * nsynS, ws + (j + 1) * nsynS);
-            wl[j * ntS + i2] = lin;
+                             ws + j * nsynS, ws + (j + 1) * nsynS, sub, 1 << lsub);
+            lin = pair_sum(lin, lsub);
+            if (sub == 0) wl[j * ntS + i2] = lin;
           }
         }
         __syncthreads();
@@ -1611,12 +1645,17 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     for (int64_t j = 0; j < (traj ? 0 : te - ts); ++j) {
       if constexpr (kSyn) {
         // The synapse step of the recomputed forward (states into work_s[j + 1]).
-        for (int i2 = tid; i2 < ntS; i2 += nthr) {
+        const int lsub = syn_lanes_log2(ntS);
+        for (int ii = tid; ii < (ntS << lsub); ii += nthr) {
+          const int i2 = ii >> lsub, sub = ii & ((1 << lsub) - 1);
           const int tgt = i2 >> lS, s = i2 & (S - 1);
           G lin = G(0), cur = G(0);
           if (s < pg.nsim)
             synapse_target(a, pg, tp, tgt, s, lin, cur, wvg + j * rowS, wv + j * rowS, ws + j * nsynS,
-                           ws + (j + 1) * nsynS);
+                           ws + (j + 1) * nsynS, sub, 1 << lsub);
+          lin = pair_sum(lin, lsub);
+          cur = pair_sum(cur, lsub);
+          if (sub != 0) continue;
           Ld.tl[i2] = static_cast<L>(lin);
           Ld.tc[i2] = static_cast<L>(cur);
           wl[j * ntS + i2] = lin;
@@ -1870,13 +1909,17 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         }
         const G* s0 = ws + j * nsynS;
         G* wqp = wq + (j & 1) * g.wq_stride;
-        for (int i2 = tid; i2 < ntS; i2 += nthr) {
+        // The synapses of a pair split over its lanes (each synapse's results are its own;
+        // the postsynaptic shares are summed over the lanes).
+        const int lsub = syn_lanes_log2(ntS);
+        for (int ii = tid; ii < (ntS << lsub); ii += nthr) {
+          const int i2 = ii >> lsub, sub = ii & ((1 << lsub) - 1);
           const int tgt = i2 >> lS, s = i2 & (S - 1);
           G post = G(0);
           if (s < pg.nsim) {
             const G gl = static_cast<G>(Ld.tl[i2]), gc = static_cast<G>(Ld.tc[i2]);
             const int qn = a.tgt_node[tp.tgb + tgt];
-            for (int q = 0;; ++q) {
+            for (int q = sub;; q += 1 << lsub) {
               const int beg = jdp[q];
               if (tgt >= jdp[q + 1] - beg) break;
               const int syn = beg + tgt;
@@ -1913,7 +1956,9 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
               }
             }
           }
-          Ld.tl[i2] = static_cast<L>(post);
+          post = pair_sum(post, lsub);
+          // Every lane of the pair read tl / tc above; they are replaced after the sum.
+          if (sub == 0) Ld.tl[i2] = static_cast<L>(post);
         }
         __syncthreads();
         // Split network: the presynaptic shares of every part written before any part

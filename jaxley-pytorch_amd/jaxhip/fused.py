@@ -294,6 +294,12 @@ def build_structure(module, rec_comps: np.ndarray, stim_nodes: np.ndarray, devic
     )
 
 
+# Repeated end entries of a part's jagged-diagonal pointer list (the forward's synapse
+# phase walks a target's synapses with up to 8 threads, each reading up to 7 entries
+# past the target's last synapse).
+SYN_PTR_END = 8
+
+
 def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray,
                     part_of_k: np.ndarray, nb_k: np.ndarray, parts: int) -> Optional[Dict]:
     """All synapses of the network, grouped by the part of their target, each part's in
@@ -301,7 +307,8 @@ def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray
     non-increasing in-degree (ties by node), and the list holding every target's first
     synapse, then every target's second one, and so on, so that the j-th synapse of
     target t is `ptr[j] + t` (for t below `ptr[j + 1] - ptr[j]`; a part's `ptr` ends with
-    a repeated entry). Threads of consecutive targets then read consecutive synapses. A
+    SYN_PTR_END entries equal to its synapse count, so that the kernels' threads that
+    walk every 2nd to 8th synapse of a target read within it). Threads of consecutive targets then read consecutive synapses. A
     target's synapses keep the order of the types in the network and of the edges within
     a type (the order the currents are summed in). Several types give the per-synapse
     model ids of JXHIP_SYN_MIXED. Presynaptic nodes index the group's nodes, targets are
@@ -329,8 +336,8 @@ def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray
         mine = np.nonzero(part_of_k[post] == q)[0]
         part_nsyn[q] = len(mine)
         if not len(mine):
-            ptr_l.append(np.zeros(2, np.int64))
-            part_nptr[q] = 2
+            ptr_l.append(np.zeros(1 + SYN_PTR_END, np.int64))
+            part_nptr[q] = 1 + SYN_PTR_END
             continue
         order = mine[np.argsort(post[mine], kind="stable")]   # grouped by target
         targets, counts = np.unique(post[order], return_counts=True)
@@ -338,7 +345,7 @@ def _build_synapses(module, kidx_of_node: np.ndarray, n: int, node_i: np.ndarray
         rank = np.argsort(-counts, kind="stable")              # non-increasing in-degree
         targets, counts, first = targets[rank], counts[rank], first[rank]
         per_j = np.array([int((counts > j).sum()) for j in range(int(counts.max()))], dtype=np.int64)
-        ptr = np.concatenate([[0], np.cumsum(per_j), [len(mine)]]).astype(np.int64)
+        ptr = np.concatenate([[0], np.cumsum(per_j), [len(mine)] * SYN_PTR_END]).astype(np.int64)
         perm = np.empty(len(mine), dtype=np.int64)
         for j, c in enumerate(per_j):
             perm[ptr[j]:ptr[j] + c] = order[first[:c] + j]

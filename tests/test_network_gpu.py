@@ -31,8 +31,16 @@ def _run(device, synapse, dtype, n_cells=12, k=4, B=2, T=400, seed=4):
     return v, ref
 
 
+@pytest.mark.parametrize("shape", [None, "1,1024,sym"])
 @pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse", "IonotropicSynapse"])
-def test_network_float64(device, synapse):
+def test_network_float64(device, synapse, shape, monkeypatch):
+    """Forward against the oracle; with the library's launch shape and with the
+    symmetric system (one coupling coefficient per node in LDS)."""
+    if shape is not None:
+        from jaxhip import autotune
+
+        monkeypatch.setenv("JAXHIP_FUSED_CONFIG", shape)
+        monkeypatch.setattr(autotune, "_CACHE", {})
     v, ref = _run(device, synapse, torch.float64)
     assert v.shape == ref.shape
     err = float((v.cpu() - ref).abs().max())
@@ -104,17 +112,19 @@ def test_network_gradients_vs_oracle_autograd(device, synapse, path, monkeypatch
     assert e1 < 1e-6 and e2 < 1e-6
 
 
+@pytest.mark.parametrize("k", [4, 20])
 @pytest.mark.parametrize("synapse", ["KineticSynapse", "SigmoidRateSynapse"])
-def test_network_fused_adjoint_batched(device, synapse, monkeypatch):
+def test_network_fused_adjoint_batched(device, synapse, k, monkeypatch):
     """The fused reverse pass on two simulations with per-simulation gNa (gS shared, its
     gradient summed over the simulations) and a loss on every soma: against autograd
-    through the oracle, relative 1e-6 of the largest gradient."""
+    through the oracle, relative 1e-6 of the largest gradient. k = 20 synapses per target
+    walk several synapses per lane of a target's lane group (up to 8 lanes per target)."""
     import jaxhip.stepwise as stepwise
 
     def refuse(*a, **k):
         raise AssertionError("network gradients took the per-step path")
     monkeypatch.setattr(stepwise, "integrate_stepwise", refuse)
-    n_cells, k, T, seed, B = 16, 4, 200, 8, 2
+    n_cells, T, seed, B = 16, 200, 8, 2
     net, om, info = build_c4(n_cells, k, seed, synapse)
     soma = info["soma_comps"]
     for c in soma:
