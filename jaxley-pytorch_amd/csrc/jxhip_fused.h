@@ -1583,11 +1583,20 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         wn[i] = n;
         const int32_t flags = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiFlags];
         if (!((flags & kComp) && (flags & kHH)) || s >= pg.nsim) continue;
-        G vn = tv[ts * rowS + i];
+        // The voltages of the next four steps in flight (the trajectory is read-only; the
+        // row index stops at the segment's end, te <= the trajectory's last row).
+        auto row = [&](int64_t t) { return (t < te ? t : te) * rowS + i; };
+        G b0 = tv[row(ts)];
+        G b1 = tv[row(ts + 1)];
+        G b2 = tv[row(ts + 2)];
+        G b3 = tv[row(ts + 3)];
 #pragma unroll 1
         for (int64_t j = 0; j < te - ts; ++j) {
-          const G vv = vn;
-          vn = tv[(ts + j + 1) * rowS + i];  // the next step's voltage in flight
+          const G vv = b0;
+          b0 = b1;
+          b1 = b2;
+          b2 = b3;
+          b3 = tv[row(ts + j + 4)];
           G am, bm, ah, bh, an, bn;
           hh_rates_step(vv, am, bm, ah, bh, an, bn);
           m = gate_step(m, dt, am, bm);
@@ -1762,6 +1771,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
       struct RevItem {
         TV v0, v1;
         G m0, h0, n0, m1, h1, n1;
+        G lm, lh, ln;  // gate adjoints (IPT == 0: in HBM, loaded with the item)
       };
       auto load_rev = [&](int i) {
         RevItem r;
@@ -1776,6 +1786,13 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
           r.n1 = n1[i];
         } else {
           r.m1 = r.h1 = r.n1 = G(0);
+        }
+        if constexpr (IPT == 0) {
+          r.lm = lm[i];
+          r.lh = lh[i];
+          r.ln = ln[i];
+        } else {
+          r.lm = r.lh = r.ln = G(0);
         }
         return r;
       };
@@ -1844,9 +1861,9 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
               lam_h0 = lhr[jj];
               lam_n0 = lnr[jj];
             } else {
-              lam_m0 = lm[i];
-              lam_h0 = lh[i];
-              lam_n0 = ln[i];
+              lam_m0 = cur.lm;
+              lam_h0 = cur.lh;
+              lam_n0 = cur.ln;
             }
             const G lam_m1 = lam_m0 + wNa * gNa * G(3) * mm * mm * hh;
             const G lam_h1 = lam_h0 + wNa * gNa * m3;
