@@ -354,6 +354,13 @@ typedef struct jxhip_fused_args {
      scaled, symmetric system (0 at roots). The solution is that of the unscaled system;
      the rounding differs from it. */
   const void* node_sym;
+  /* coop = 1 only (else ignored): n_progs / parts + 1 zeroed 32-bit words. The programs
+     of one group (the parts of a split network for one group of simulations) then
+     synchronize through their group's counter instead of a grid-wide barrier; the last
+     word is set to 1 if a barrier gave up waiting (after about 2^26 polls; the results
+     are then invalid). Zero the words before every launch (the forward and the reverse
+     pass count their barriers from 0). NULL: grid-wide barriers. */
+  unsigned int* group_sync;
 } jxhip_fused_args;
 
 int jxhip_fused_forward_f32(const jxhip_fused_args* args, void* stream);

@@ -102,6 +102,8 @@ struct Fused {
   G* traj_v;           // voltage trajectory of the checkpointed forward (gate type), or null
   int64_t traj_steps;
   const L* node_sym;   // (symmetric coupling coefficient, row scale) per node, or null
+  unsigned int* gsync;      // lockstep programs: one counter per group (this launch's), or null
+  unsigned int* gsync_err;  // set when a group barrier gave up waiting
 };
 
 template <typename G, typename L>
@@ -166,6 +168,8 @@ Fused<G, L> unpack(const jxhip_fused_args& a) {
   f.traj_v = static_cast<G*>(a.traj_v);
   f.traj_steps = a.traj_steps;
   f.node_sym = static_cast<const L*>(a.node_sym);
+  f.gsync = a.group_sync;
+  f.gsync_err = a.group_sync != nullptr ? a.group_sync + a.n_progs / f.parts : nullptr;
   return f;
 }
 
@@ -177,6 +181,38 @@ struct Prog {
 // Barrier of every workgroup of a cooperative launch (the programs of split networks):
 // the writes of all workgroups before it are visible to all after it.
 __device__ __forceinline__ void grid_barrier() { cooperative_groups::this_grid().sync(); }
+
+// Barrier of the lockstep programs of one group (the parts of a split network run for
+// one group of simulations): the writes of all of them before it are visible to all
+// after it. With a.gsync the group's programs count their arrivals on their own counter
+// (no wait on the other groups, no contention with them); without it the whole grid
+// synchronizes. `nbar` counts the barriers this program has passed (uniform). The wait
+// gives up after about 2^26 polls and sets the error word (the host reports it).
+template <typename F>
+__device__ __forceinline__ void group_barrier(const F& a, unsigned int& nbar) {
+  if (a.gsync == nullptr) {
+    grid_barrier();
+    return;
+  }
+  ++nbar;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int* ctr = a.gsync + blockIdx.x / a.parts;
+    const unsigned int target = nbar * static_cast<unsigned int>(a.parts);
+    __threadfence();
+    atomicAdd(ctr, 1u);
+    unsigned int polls = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++polls == (1u << 26)) {
+        atomicExch(a.gsync_err, 1u);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
 struct Topo {
   int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb, ppb;
 };
@@ -1146,11 +1182,12 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   record_step(a, pg, tp, vg, 0);
   // The group's block of the state layout (the programs of a split network share it).
   const int64_t gbase = pg.soff - (static_cast<int64_t>(tp.nb) << lS);
+  unsigned int nbar = 0;  // group barriers passed
   if (kSyn && a.coop) {
     // Split network: every part's voltages at the start into the exported copy of step 0.
     for (int i = tid; i < nS; i += nthr) a.vx[pg.soff + i] = vg[i];
     __syncthreads();
-    grid_barrier();
+    group_barrier(a, nbar);
   }
   const int64_t nsteps = pg.t1 - pg.t0;
   // The voltage trajectory: per group of simulations [T + 1][group nodes][S], the
@@ -1232,7 +1269,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     __syncthreads();
     // Split network: every part's exported voltages of the step written before any part
     // starts the next step's synapses.
-    if (kSyn && a.coop) grid_barrier();
+    if (kSyn && a.coop) group_barrier(a, nbar);
     record_step(a, pg, tp, vg, step + 1);
   }
 }
@@ -1488,6 +1525,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
   const G dt = a.dt;
   const int64_t nsteps = pg.t1 - pg.t0;
   const int64_t K = a.ckpt_every;
+  unsigned int nbar = 0;  // group barriers passed
   const int64_t C = (nsteps + K - 1) / K;
   // Recomputed trajectory: per group of simulations [K + 1][group nodes][S] (the parts of
   // a split network side by side, so presynaptic voltages of other parts are in reach);
@@ -1628,7 +1666,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
     // voltages from it (the same after every recomputed step); with the trajectory, the
     // previous segment's presynaptic shares summed by every part before this segment's
     // reverse steps write the step-parity buffers again.
-    if (kSyn && a.coop) grid_barrier();
+    if (kSyn && a.coop) group_barrier(a, nbar);
     if constexpr (kSyn) {
       if (traj) {
         // Synapse states and synaptic slopes of every step of the segment, per (target,
@@ -1705,7 +1743,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
       tree_solve<G, L, false, kCS>(a, tp, Ld.sc, Ld.xs, Ld.ds, Ld.cl);
       for (int i = tid; i < nS; i += nthr) wv[(j + 1) * rowS + i] = wv[j * rowS + i] + static_cast<double>(Ld.xs[i]);
       __syncthreads();
-      if (kSyn && a.coop) grid_barrier();
+      if (kSyn && a.coop) group_barrier(a, nbar);
     }
     // The transposed system of a step: its diagonal at the gates after the step, its
     // right-hand side the voltage adjoint at the step's end (recordings included). The
@@ -1980,7 +2018,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         __syncthreads();
         // Split network: the presynaptic shares of every part written before any part
         // sums its nodes' slots.
-        if (a.coop) grid_barrier();
+        if (a.coop) group_barrier(a, nbar);
         // The voltage adjoints completed by the synapse shares: the target's own, and the
         // presynaptic shares of every synapse leaving the node (by presynaptic node).
         const int32_t* pptr = g.pre_ptr + tp.ppb;
@@ -2080,6 +2118,7 @@ int launch_kernel(K kern, const jxhip_fused_args& a, hipStream_t stream, F f, Re
     const int64_t cnt = a.n_progs - p0 < cap ? a.n_progs - p0 : cap;
     F fc = f;
     fc.progs = f.progs + p0 * JXHIP_PROG_COLS;
+    if (f.gsync != nullptr) fc.gsync = f.gsync + p0 / a.parts;
     void* kargs[] = {&fc, &rest...};
     if (hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern),
                                                   dim3(static_cast<unsigned>(cnt)), block, kargs,

@@ -51,7 +51,7 @@ class FusedArgs(C.Structure):
         ("dt_v", C.c_double), ("rhs_scale", C.c_double), ("explicit_step", _i32),
         ("syn_kind", _p), ("node_p2", _p), ("gate_ext", _p), ("gate_ext_stride", C.c_int64),
         ("parts", _i32), ("coop", _i32), ("group_nodes", _i64), ("vx", _p), ("vx_stride", _i64),
-        ("traj_v", _p), ("traj_steps", _i64), ("node_sym", _p),
+        ("traj_v", _p), ("traj_steps", _i64), ("node_sym", _p), ("group_sync", _p),
     ]
 
 

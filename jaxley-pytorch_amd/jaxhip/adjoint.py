@@ -115,10 +115,13 @@ class _FusedIntegrate(torch.autograd.Function):
         args.threads = adjoint_threads(acfg, torch.tensor([], dtype=dtype).element_size())
         args.items_per_thread, args.coef_global = acfg.ipt, int(acfg.gcoef)
         f = run.kernel_name("jxhip_fused_adjoint")
+        if run.gsync is not None:
+            run.gsync.zero_()
         try:
             _lib.check(f(args, adj, _lib.stream_ptr(dev)), "fused_adjoint")
         finally:
             args.lds_bytes, args.threads, args.items_per_thread, args.coef_global = saved
+        run.check_group_sync()
         g_node_p = g_bparam = None
         zero = torch.zeros((P, n, S), dtype=dtype, device=dev)
         gpar = [zero if g is None else g for g in gpar]

@@ -711,11 +711,16 @@ class ForwardLaunch:
         a.group_nodes = n
         a.coop = int(coop)
         self.vx = None
+        self.gsync = None
         if coop:
             # Exported voltages of every part by step parity (presynaptic reads).
             self.vx = torch.empty((2,) + self.v.shape, dtype=torch.float64, device=dev)
             a.vx = self.vx.data_ptr()
             a.vx_stride = self.v.numel()
+            # One barrier counter per group of programs (the parts of a group wait for
+            # each other only) and an error word.
+            self.gsync = torch.zeros(P + 1, dtype=torch.int32, device=dev)
+            a.group_sync = self.gsync.data_ptr()
         a.topos = self.topos.data_ptr()
         a.seg = st.seg.data_ptr()
         a.seg_level_ptr = st.seg_level_ptr.data_ptr()
@@ -822,7 +827,15 @@ class ForwardLaunch:
 
     def launch(self):
         f = self.kernel_name("jxhip_fused_forward")
+        if self.gsync is not None:
+            self.gsync.zero_()
         _lib.check(f(self.args, _lib.stream_ptr(self.v.device)), "fused_forward")
+        self.check_group_sync()
+
+    def check_group_sync(self):
+        """Raises if a group barrier of the last launch gave up waiting (waits for it)."""
+        if self.gsync is not None and int(self.gsync[-1]) != 0:
+            raise RuntimeError("a group barrier of the lockstep programs gave up waiting")
 
     def recordings(self) -> torch.Tensor:
         """(B, n_rec, T + 1), the recordings in their order."""
