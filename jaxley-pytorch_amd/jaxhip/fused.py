@@ -833,8 +833,11 @@ class ForwardLaunch:
         self.check_group_sync()
 
     def check_group_sync(self):
-        """Raises if a group barrier of the last launch gave up waiting (waits for it)."""
-        if self.gsync is not None and int(self.gsync[-1]) != 0:
+        """Raises if a group barrier of the last launch gave up waiting (waits for it; not
+        while a graph is being captured, where the word is left for the caller)."""
+        if self.gsync is None or torch.cuda.is_current_stream_capturing():
+            return
+        if int(self.gsync[-1]) != 0:
             raise RuntimeError("a group barrier of the lockstep programs gave up waiting")
 
     def recordings(self) -> torch.Tensor:
