@@ -188,18 +188,24 @@ __device__ __forceinline__ void grid_barrier() { cooperative_groups::this_grid()
 // (no wait on the other groups, no contention with them); without it the whole grid
 // synchronizes. `nbar` counts the barriers this program has passed (uniform). The wait
 // gives up after about 2^26 polls and sets the error word (the host reports it).
-template <typename F>
+// kCoherentOnly: the only data crossing the barrier was written with store_coherent and
+// is read with load_coherent (device-scope accesses that each L2 passes through to the
+// device's coherence point); the barrier then waits for every thread's own stores to
+// complete instead of the device-scope fences, which write back and invalidate the whole
+// L2 of the XCD and made the forward's step of a split network 15-16 us slower.
+template <bool kCoherentOnly = false, typename F>
 __device__ __forceinline__ void group_barrier(const F& a, unsigned int& nbar) {
   if (a.gsync == nullptr) {
     grid_barrier();
     return;
   }
   ++nbar;
+  if (kCoherentOnly) __builtin_amdgcn_s_waitcnt(0);  // this thread's stores acknowledged
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned int* ctr = a.gsync + blockIdx.x / a.parts;
     const unsigned int target = nbar * static_cast<unsigned int>(a.parts);
-    __threadfence();
+    if (!kCoherentOnly) __threadfence();
     atomicAdd(ctr, 1u);
     unsigned int polls = 0;
     while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -209,9 +215,18 @@ __device__ __forceinline__ void group_barrier(const F& a, unsigned int& nbar) {
         break;
       }
     }
-    __threadfence();
+    if (!kCoherentOnly) __threadfence();
   }
   __syncthreads();
+}
+
+// Device-scope relaxed store and load of one exported value (see group_barrier).
+__device__ __forceinline__ void store_coherent(double* p, double x) {
+  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename V>
+__device__ __forceinline__ V load_coherent(const V* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 struct Topo {
   int32_t n, L, nb, lb, cb, n_rec, rb, n_tgt, tgb, spb, synb, n_syn, sb, scb, ppb;
@@ -283,7 +298,8 @@ __device__ __forceinline__ G prm(const Fused<G, L>& a, const Prog& pg, const Top
 // states before / after the step ([syn][S]; the same array in the forward). The target's
 // synapses j0, j0 + jstep, ... (jstep <= 8: the pointer list's 8 repeated end entries
 // keep every read of it in range).
-template <typename G, typename L, typename V>
+// kCoh: vpre_g is a split network's exported copy, read with load_coherent.
+template <typename G, typename L, typename V, bool kCoh = false>
 __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog& pg,
                                                const Topo& tp, int tgt, int s, G& lin, G& cur,
                                                const V* vpre_g, const V* v,
@@ -301,7 +317,8 @@ __device__ __forceinline__ void synapse_target(const Fused<G, L>& a, const Prog&
     const int syn = beg + tgt;
     const int64_t gsyn = tp.synb + syn;
     const int pre = a.syn_pre[gsyn];
-    const G vpre = static_cast<G>(vpre_g[static_cast<int64_t>(pre) * S + s]);
+    const int64_t ip = static_cast<int64_t>(pre) * S + s;
+    const G vpre = static_cast<G>(kCoh ? load_coherent(vpre_g + ip) : vpre_g[ip]);
     const G scale = a.syn_scale[gsyn];
     const G gs = a.syn_param[0][gsyn];
     const int model = a.syn_kind != nullptr ? a.syn_kind[gsyn] : a.syn_model;
@@ -996,7 +1013,7 @@ __device__ __forceinline__ T pair_sum(T x, int l) {
   return x;
 }
 
-template <typename G, typename L>
+template <bool kCoh, typename G, typename L>
 __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& pg, const Topo& tp,
                                               const Lds<L>& Ld, const double* vpre_g) {
   if (a.syn_model < 0) return;
@@ -1013,7 +1030,8 @@ __device__ __forceinline__ void synapse_phase(const Fused<G, L>& a, const Prog& 
     G lin = G(0), cur = G(0);
     if (s < pg.nsim) {
       G* st = a.syn_s + pg.synoff;
-      synapse_target(a, pg, tp, tgt, s, lin, cur, vpre_g, a.v + pg.soff, st, st, sub, nsub);
+      synapse_target<G, L, double, kCoh>(a, pg, tp, tgt, s, lin, cur, vpre_g, a.v + pg.soff, st, st, sub,
+                                        nsub);
     }
     lin = pair_sum(lin, lsub);
     cur = pair_sum(cur, lsub);
@@ -1185,9 +1203,9 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
   unsigned int nbar = 0;  // group barriers passed
   if (kSyn && a.coop) {
     // Split network: every part's voltages at the start into the exported copy of step 0.
-    for (int i = tid; i < nS; i += nthr) a.vx[pg.soff + i] = vg[i];
+    for (int i = tid; i < nS; i += nthr) store_coherent(a.vx + pg.soff + i, vg[i]);
     __syncthreads();
-    group_barrier(a, nbar);
+    group_barrier<true>(a, nbar);
   }
   const int64_t nsteps = pg.t1 - pg.t0;
   // The voltage trajectory: per group of simulations [T + 1][group nodes][S], the
@@ -1203,7 +1221,10 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
       if (ck) checkpoint_syn(a, pg, tp, step / a.ckpt_every);
     }
     if constexpr (kSyn) {
-      synapse_phase(a, pg, tp, Ld, a.coop ? a.vx + (step & 1) * a.vx_stride + gbase : a.v + gbase);
+      if (a.coop)
+        synapse_phase<true>(a, pg, tp, Ld, a.vx + (step & 1) * a.vx_stride + gbase);
+      else
+        synapse_phase<false>(a, pg, tp, Ld, a.v + gbase);
     }
     const int64_t t = pg.t0 + step;
     if (tid < nS) {
@@ -1259,7 +1280,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
         const double dv = static_cast<double>(Ld.xs[i]);
         if (last || (c.r0.w & kExport)) {
           vg[i] += dv;
-          if (kSyn && a.coop) a.vx[((step + 1) & 1) * a.vx_stride + pg.soff + i] = vg[i];
+          if (kSyn && a.coop) store_coherent(a.vx + ((step + 1) & 1) * a.vx_stride + pg.soff + i, vg[i]);
           if (kCkpt && last && a.traj_v != nullptr) a.traj_v[tbase + nsteps * trow + i] = static_cast<G>(vg[i]);
         }
         if (!last) Ld.ds[i] = static_cast<L>(apply_update<kCS>(a, tp, c, k, Ld.cl, Ld.xs, s_own, dv));
@@ -1269,7 +1290,7 @@ __global__ void __launch_bounds__(1024) fused_stream_kernel(Fused<G, L> a) {
     __syncthreads();
     // Split network: every part's exported voltages of the step written before any part
     // starts the next step's synapses.
-    if (kSyn && a.coop) group_barrier(a, nbar);
+    if (kSyn && a.coop) group_barrier<true>(a, nbar);
     record_step(a, pg, tp, vg, step + 1);
   }
 }
@@ -1350,7 +1371,7 @@ __global__ void __launch_bounds__(MAXT) fused_regs_kernel(Fused<G, L> a) {
         }
       }
     }
-    if constexpr (kSyn) synapse_phase(a, pg, tp, Ld, a.v + pg.soff - (static_cast<int64_t>(tp.nb) << lS));
+    if constexpr (kSyn) synapse_phase<false>(a, pg, tp, Ld, a.v + pg.soff - (static_cast<int64_t>(tp.nb) << lS));
     const int64_t t = pg.t0 + step;
 #pragma unroll 1
     for (int j = 0; j < IPT; ++j) {
