@@ -221,7 +221,8 @@ __device__ __forceinline__ void group_barrier(const F& a, unsigned int& nbar) {
 }
 
 // Device-scope relaxed store and load of one exported value (see group_barrier).
-__device__ __forceinline__ void store_coherent(double* p, double x) {
+template <typename V>
+__device__ __forceinline__ void store_coherent(V* p, V x) {
   __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename V>
@@ -2012,7 +2013,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
                 const G lsp = lam_s[si] + dg * gs * scale;     // dL/ds'
                 if (g.g_gs) g.g_gs[pg.synoff + si] += dg * snew * scale;
                 lam_s[si] = lsp * E;
-                wqp[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = lsp * dsdv;
+                store_coherent(wqp + static_cast<int64_t>(g.pre_pos[gsyn]) * S + s, lsp * dsdv);
               } else {
                 const G xo = a.syn_param[1][gsyn];
                 const G slope = a.syn_param[2][gsyn];
@@ -2027,7 +2028,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
                 post += gc * vt;
                 const G dvt_dv = -gs * scale / a.diff * slope * (sb * (G(1) - sb) - sa * (G(1) - sa));
                 const G dd0_dv = -gs * scale * slope * sa * (G(1) - sa);
-                wqp[static_cast<int64_t>(g.pre_pos[gsyn]) * S + s] = dvt * dvt_dv + dd0 * dd0_dv;
+                store_coherent(wqp + static_cast<int64_t>(g.pre_pos[gsyn]) * S + s, dvt * dvt_dv + dd0 * dd0_dv);
                 if (g.g_gs) g.g_gs[pg.synoff + si] += dvt * (-dsig * scale / a.diff) + dd0 * (-sa * scale);
               }
             }
@@ -2038,8 +2039,9 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
         }
         __syncthreads();
         // Split network: the presynaptic shares of every part written before any part
-        // sums its nodes' slots.
-        if (a.coop) group_barrier(a, nbar);
+        // sums its nodes' slots (the shares are the only data crossing this barrier; the
+        // recomputed voltages crossed the barriers of the recomputation).
+        if (a.coop) group_barrier<true>(a, nbar);
         // The voltage adjoints completed by the synapse shares: the target's own, and the
         // presynaptic shares of every synapse leaving the node (by presynaptic node).
         const int32_t* pptr = g.pre_ptr + tp.ppb;
@@ -2053,7 +2055,7 @@ __global__ void __launch_bounds__(adjoint_max_threads<G>(IPT)) fused_adjoint_ker
             G add = G(0);
             const int tg = a.node_i[static_cast<int64_t>(tp.nb + k) * JXHIP_NODE_ICOLS + kNiTgt];
             if (tg >= 0) add += static_cast<G>(Ld.tl[(tg << lS) + s]);
-            for (int e = pptr[k]; e < pptr[k + 1]; ++e) add += wqp[static_cast<int64_t>(e) * S + s];
+            for (int e = pptr[k]; e < pptr[k + 1]; ++e) add += load_coherent(wqp + static_cast<int64_t>(e) * S + s);
             lv = static_cast<L>(static_cast<G>(lv) + add);
           }
           if (j > 0) {
